@@ -1,0 +1,155 @@
+/*
+ * vmp.h — C ABI of the MI355X registration path for voxel_plus
+ * (VoxelMapPlus + FAST-LIO2 IESKF).  One context per process/GPU, one session
+ * per LiDAR sequence.  A session owns, in HBM: the voxel hash map with its LRU
+ * order, the per-point residual slots, and the filter state (x, P).
+ *
+ * Conventions
+ *   - every 3x3 / 12x12 / 23x23 matrix is column-major (Eigen's default), so
+ *     an Eigen object's .data() can be passed directly;
+ *   - status codes: 0 ok, < 0 error (see VMP_E*); vmp_last_error() explains;
+ *   - host pointers are only read/written during the call;
+ *   - calls on one session must not overlap; distinct sessions may be driven
+ *     from distinct host threads.
+ *
+ * Reference entry points each function replaces (paths relative to the
+ * reference root, voxel_plus/src/map_builder/):
+ *   vmp_scan_set_points   residual-slot prep, lio_builder.cpp:225-229
+ *   vmp_observe           kf::measure_func = LIOBuilder::sharedUpdateFunc,
+ *                         ieskf.h:74, lio_builder.cpp:250-311
+ *   vmp_map_update        VoxelMap::update / build, voxel_map.h:114-116,
+ *                         voxel_map.cpp:200-256, with the insertion points and
+ *                         covariances of lio_builder.cpp:155-163,185-211,231-246
+ *   vmp_ieskf_update      kf::IESKF::update, ieskf.cpp:125-156, with
+ *                         vmp_observe as its measurement function
+ *   vmp_register_scan     lio_builder.cpp:224-246 (LIO_MAPPING branch after
+ *                         undistortion): set points, update, map update
+ */
+#ifndef VMP_H
+#define VMP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  VMP_OK = 0,
+  VMP_EINVAL = -1,     /* bad argument / size beyond the session's limits */
+  VMP_ENOMEM = -2,     /* device allocation failed */
+  VMP_EDEVICE = -3,    /* HIP runtime error */
+  VMP_ERANGE = -4,     /* voxel key outside +-2^20 voxels per axis */
+  VMP_ECAPACITY = -5,  /* one scan touches more voxels than the LRU can evict
+                          in scan order (map_capacity too small for the scan) */
+  VMP_ESTATE = -6      /* call sequence error (e.g. observe before points) */
+};
+
+/* LIOConfig subset on the path (lio_builder.h:17-42) plus device limits. */
+typedef struct {
+  double voxel_size;                /* 0.5 */
+  double plane_thresh;              /* 0.01 */
+  double merge_thresh_for_angle;    /* 0.2 (mapping.yaml:16) */
+  double merge_thresh_for_distance; /* 0.04 */
+  double ranging_cov;               /* 0.04 */
+  double angle_cov;                 /* 0.1 (degrees) */
+  int32_t max_point_thresh;         /* 100 */
+  int32_t update_size_thresh;       /* 10 */
+  int32_t map_capacity;             /* LRU capacity in voxels */
+  int32_t max_scan_points;          /* residual slots = largest scan */
+  int32_t max_iter;                 /* IESKF iterations, 5 */
+  int32_t estimate_ext;             /* 0 */
+} vmp_map_cfg;
+
+/* kf::State (ieskf.h:31-62); rot and rot_ext column-major. */
+typedef struct {
+  double pos[3];
+  double rot[9];
+  double rot_ext[9];
+  double pos_ext[3];
+  double vel[3];
+  double bg[3];
+  double ba[3];
+  double g[3];
+} vmp_state;
+
+typedef struct {
+  int32_t points;      /* N */
+  int32_t hits;        /* lookups that found a voxel */
+  int32_t plane_hits;  /* ... whose voxel is a plane */
+  int32_t effective;   /* valid slots entering H (incl. stale ones) */
+  int32_t key_fallback;/* points re-keyed in FP64 near a voxel face */
+  int32_t iterations;  /* IESKF iterations run (vmp_ieskf_update) */
+} vmp_obs_stats;
+
+typedef struct {
+  int32_t points;
+  int32_t touched;     /* distinct voxels touched */
+  int32_t created;
+  int32_t evicted;
+  int32_t refits;
+  int32_t merge_calls;
+  int32_t merge_evals; /* merge() calls actually evaluated on device */
+  int32_t merges;
+  int32_t dropped;
+  int32_t map_size;
+} vmp_map_stats;
+
+typedef struct vmp_ctx vmp_ctx;
+typedef struct vmp_session vmp_session;
+
+const char* vmp_version(void);
+
+int vmp_ctx_create(int device, vmp_ctx** out);
+int vmp_ctx_destroy(vmp_ctx* ctx);
+
+int vmp_session_create(vmp_ctx* ctx, const vmp_map_cfg* cfg, vmp_session** out);
+int vmp_session_destroy(vmp_session* s);
+/* bytes of HBM the session holds (for memory-based admission) */
+int64_t vmp_session_bytes(const vmp_map_cfg* cfg);
+const char* vmp_last_error(const vmp_session* s);
+
+/* Filter state resident on the device (x and P, P column-major 23x23). */
+int vmp_state_set(vmp_session* s, const vmp_state* x, const double* P);
+int vmp_state_get(vmp_session* s, vmp_state* x, double* P);
+/* Motion prior between scans (stand-in for IMU propagation, which is outside
+ * this path): pos += rot*dp; rot = rot*dR; P += diag(qdiag[23]). */
+int vmp_state_apply_motion(vmp_session* s, const double* dR, const double* dp, const double* qdiag);
+
+/* Scan points in the LiDAR frame, SoA fp32, scan (time) order.  Host
+ * pointers are copied; _device takes pointers that already live in HBM. */
+int vmp_scan_set_points(vmp_session* s, const float* x, const float* y, const float* z, int32_t n);
+int vmp_scan_set_points_device(vmp_session* s, const float* dx, const float* dy, const float* dz, int32_t n);
+
+/* One measurement pass at state x with the session's P as prior:
+ * H (12x12) and b (12) exactly as SharedState (ieskf.h:22-29). */
+int vmp_observe(vmp_session* s, const vmp_state* x, double* H, double* b, vmp_obs_stats* st);
+
+/* Insert the current scan with the session's (posterior) state.
+ * is_build != 0 runs VoxelMap::build semantics (MAP_INIT, empty map only). */
+int vmp_map_update(vmp_session* s, int is_build, vmp_map_stats* st);
+
+/* IESKF update on the device with vmp_observe as measurement function. */
+int vmp_ieskf_update(vmp_session* s, vmp_obs_stats* st);
+
+/* set points (device) + IESKF update + map update, no host round trip */
+int vmp_register_scan(vmp_session* s, vmp_obs_stats* ost, vmp_map_stats* mst);
+
+/* Map export, LRU order (most recent first).  Any output may be NULL.
+ * keys[3i]; ints[5i] = {n, flags(1 init,2 plane,4 update_enable,8 merged),
+ * newly_add_point, points held, 0}; group[i]; dbl[8i] = {mean xyz, norm xyz,
+ * trace of normal-cov block, trace of centre-cov block}.  Returns count. */
+int64_t vmp_map_export(vmp_session* s, int64_t cap, int64_t* keys, int32_t* ints, uint64_t* group, double* dbl);
+int64_t vmp_map_size(vmp_session* s);
+/* Residual slots: valid flag, residual, normal (3 per point). */
+int vmp_slots_export(vmp_session* s, int32_t n, uint8_t* valid, float* resid, float* norm3);
+
+/* Batched forms: sessions advance together, one launch per stage. */
+int vmp_register_batch(vmp_session** s, int32_t n, vmp_obs_stats* ost, vmp_map_stats* mst);
+
+/* Device events around the last call's kernels (for roofline timing). */
+int vmp_session_timing(vmp_session* s, double* observe_ms, double* map_ms, int32_t* observe_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

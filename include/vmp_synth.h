@@ -1,0 +1,58 @@
+/*
+ * vmp_synth.h — seeded synthetic LiDAR sequences for the voxel_plus
+ * registration path (C ABI, host-only, deterministic on every x86-64 box).
+ *
+ * The reference has no test data of its own (its demo rosbag is external), so
+ * every workload here is generated:
+ *   scene 0 "room":   30 x 30 x 6 m room (walls, floor, ceiling) + 40 seeded
+ *                     yawed boxes with sides U[0.5, 3] m.
+ *   scene 1 "street": 200 m street along x with a sloped ground plane,
+ *                     building facades on both sides, 0.3 m-radius poles and a
+ *                     15 % share of foliage returns scattered uniformly along
+ *                     the ray.
+ * Rays follow a non-repeating rosette-like pattern over 360 deg azimuth and
+ * -7..52 deg elevation (Livox MID-360 field of view), range noise N(0, 0.02 m),
+ * kept ranges 0.1..40 m.  Every scan has exactly `pts_per_scan` points, in
+ * ascending, unique per-point time order (the order lio_builder.cpp:75-76 sorts
+ * by; utils.cpp:20 time convention).  Points are emitted already motion-
+ * compensated: each is expressed in the LiDAR frame at the scan end pose, which
+ * is what undistortCloud (lio_builder.cpp:65-153) hands to the registration
+ * path.  Trajectory: 1 m/s, |yaw rate| <= 0.5 rad/s, small roll/pitch sway;
+ * scans at 10 Hz.
+ *
+ * Matrices are 3x3 column-major (Eigen's default layout).
+ */
+#ifndef VMP_SYNTH_H
+#define VMP_SYNTH_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vmp_synth vmp_synth;
+
+enum { VMP_SCENE_ROOM = 0, VMP_SCENE_STREET = 1 };
+
+/* origin_offset: added to every world coordinate (precision study: 10 km). */
+int vmp_synth_create(int scene, uint64_t seed, int32_t pts_per_scan,
+                     const double origin_offset[3], vmp_synth** out);
+void vmp_synth_destroy(vmp_synth* s);
+
+/* Ground-truth LiDAR pose (world <- lidar) at the end of scan k (10 Hz). */
+int vmp_synth_pose(const vmp_synth* s, int32_t k, double R[9], double t[3]);
+
+/* Scan k: xyz are LiDAR-frame fp32 points (n = pts_per_scan each), t_ms the
+ * per-point time offset in ms (strictly ascending).  Any output may be NULL. */
+int vmp_synth_scan(const vmp_synth* s, int32_t k, float* x, float* y, float* z,
+                   float* t_ms);
+
+/* Noisy odometry increment between scans k-1 and k, expressed in the LiDAR
+ * frame of scan k-1: pose_k ~= pose_{k-1} * [dR | dp].  Seeded noise of
+ * 2 cm / 0.2 deg per axis stands in for IMU propagation error. */
+int vmp_synth_motion(const vmp_synth* s, int32_t k, double dR[9], double dp[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

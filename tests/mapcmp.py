@@ -1,0 +1,48 @@
+"""Map / state comparison helpers shared by the parity tests."""
+import numpy as np
+
+
+def rot_angle(Ra, Rb):
+    d = Ra.T @ Rb
+    return float(np.arccos(np.clip((np.trace(d) - 1.0) / 2.0, -1.0, 1.0)))
+
+
+def state_err(sa, sb):
+    """(position error m, rotation error rad) between two flat state vectors."""
+    Ra = np.asarray(sa[3:12]).reshape(3, 3).T
+    Rb = np.asarray(sb[3:12]).reshape(3, 3).T
+    return float(np.linalg.norm(np.asarray(sa[0:3]) - np.asarray(sb[0:3]))), rot_angle(Ra, Rb)
+
+
+def compare_maps(ref, got, mean_tol=1e-6, norm_tol=1e-5, tr_rtol=1e-4, report=None):
+    """Keys in LRU order and point counts must match exactly; plane flags,
+    group ids and plane parameters are compared with the margin cases
+    (plane threshold / merge gates decided differently in the last bits)
+    counted and returned rather than failed."""
+    out = {}
+    assert ref["keys"].shape == got["keys"].shape, (ref["keys"].shape, got["keys"].shape)
+    assert np.array_equal(ref["keys"], got["keys"]), "LRU key order differs"
+    assert np.array_equal(ref["ints"][:, 0], got["ints"][:, 0]), "point counts differ"
+    fl_r, fl_g = ref["ints"][:, 1], got["ints"][:, 1]
+    # init and update_enable are functions of counts only: exact
+    assert np.array_equal(fl_r & 5, fl_g & 5), "init / update_enable flags differ"
+    plane_diff = (fl_r & 2) != (fl_g & 2)
+    out["plane_flag_diff"] = int(plane_diff.sum())
+    out["merged_flag_diff"] = int(((fl_r & 8) != (fl_g & 8)).sum())
+    out["group_diff"] = int((ref["group"] != got["group"]).sum())
+    same = ~plane_diff
+    dm = np.abs(ref["dbl"][:, 0:3] - got["dbl"][:, 0:3]).max(axis=1)
+    out["mean_err_max"] = float(dm[same].max()) if same.any() else 0.0
+    pl = same & ((fl_r & 2) != 0)
+    dn = np.abs(ref["dbl"][:, 3:6] - got["dbl"][:, 3:6]).max(axis=1)
+    out["norm_err_max"] = float(dn[pl].max()) if pl.any() else 0.0
+    out["norm_bad"] = int((dn[pl] > norm_tol).sum())
+    out["mean_bad"] = int((dm[same] > mean_tol).sum())
+    tr_r, tr_g = ref["dbl"][:, 6:8], got["dbl"][:, 6:8]
+    rel = np.abs(tr_r - tr_g) / np.maximum(np.abs(tr_r), 1e-300)
+    out["trace_rel_max"] = float(rel[pl].max()) if pl.any() else 0.0
+    out["trace_bad"] = int((rel[pl] > tr_rtol).max(axis=1).sum()) if pl.any() else 0
+    out["n"] = int(len(ref["keys"]))
+    if report is not None:
+        report(out)
+    return out

@@ -1,0 +1,124 @@
+"""GPU parity of the registration path against the FP64 oracle.
+
+All calls go through the C ABI (libvmp.so).  Tolerances:
+  keys, LRU order, point counts, init/update flags   bit-exact
+  plane mean                                          1e-6 m
+  plane normal (per component)                        1e-5
+  Plane::cov block traces                             1e-4 relative
+  H, b of one observe pass                            1e-6 relative (Frobenius)
+  per-scan pose after the IESKF update                1e-4 m, 1e-5 rad
+Margin cases (plane threshold or merge gates decided by the last bits of a
+float) are counted and must stay rare (< 0.1 % of voxels).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from mapcmp import compare_maps, state_err
+from oracle import Oracle
+from voxelmapplus_fastlio2_amd import Context, LIOConfig, Registrar, initial_covariance, state_vector
+from voxelmapplus_fastlio2_amd.synth import QDIAG, ROOM, SEED0, STREET, Sequence
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def start(seq, cfg):
+    R, t = seq.pose(0)
+    s0 = state_vector(R, t)
+    P0 = initial_covariance()
+    x, y, z = seq.scan(0)
+    orc = Oracle(cfg)
+    orc.set_state(s0, P0)
+    orc.build(x, y, z)
+    return orc, s0, P0, (x, y, z)
+
+
+def test_build_parity(ctx):
+    seq = Sequence(ROOM, SEED0, 20000)
+    cfg = LIOConfig(max_scan_points=20000)
+    orc, s0, P0, (x, y, z) = start(seq, cfg)
+    with Registrar(ctx, cfg) as g:
+        g.set_state(s0, P0)
+        g.set_points(x, y, z)
+        st = g.build()
+        log("build stats", {f: getattr(st, f) for f, _ in st._fields_})
+        r = compare_maps(orc.export_map(), g.export_map(), report=lambda o: log("build map cmp", o))
+        assert r["plane_flag_diff"] <= max(2, r["n"] // 1000)
+        assert r["mean_bad"] == 0 and r["norm_bad"] <= r["plane_flag_diff"] + 2
+
+
+def test_observe_parity(ctx):
+    seq = Sequence(ROOM, SEED0, 20000)
+    cfg = LIOConfig(max_scan_points=20000)
+    orc, s0, P0, _ = start(seq, cfg)
+    dR, dp = seq.motion(1)
+    orc.apply_motion(dR, dp, QDIAG)
+    sp, Pp = orc.get_state()
+    x, y, z = seq.scan(1)
+    orc.set_points(x, y, z)
+    Ho, bo, eff_o = orc.observe(sp)
+    vo, ro, no = orc.export_slots(len(x))
+    with Registrar(ctx, cfg) as g:
+        g.set_state(s0, P0)
+        g.set_points(*seq.scan(0))
+        g.build()
+        g.set_state(sp, Pp)
+        g.set_points(x, y, z)
+        H, b, st = g.observe(sp)
+        v, r, n = g.export_slots(len(x))
+    log("observe stats", {f: getattr(st, f) for f, _ in st._fields_}, "oracle effective", eff_o)
+    flips = int((vo != v).sum())
+    log("valid flips", flips, "of", len(x))
+    assert flips <= max(3, len(x) // 2000)
+    both = vo & v
+    assert np.abs(ro[both] - r[both]).max() < 1e-5
+    eH = np.linalg.norm(H - Ho) / np.linalg.norm(Ho)
+    eb = np.linalg.norm(b - bo) / max(np.linalg.norm(bo), 1e-12)
+    log("H rel err", eH, "b rel err", eb)
+    if flips == 0:
+        assert eH < 1e-6
+        assert eb < 1e-4
+
+
+@pytest.mark.parametrize("scene,npts,nscans", [(ROOM, 20000, 12), (STREET, 30000, 8)])
+def test_sequence_teacher_forced(ctx, scene, npts, nscans):
+    """Each scan starts from the oracle's prior; posterior pose, P and map
+    are compared after every scan."""
+    seq = Sequence(scene, SEED0 + 7, npts)
+    cfg = LIOConfig(max_scan_points=npts, map_capacity=100000 if scene == ROOM else 2000000)
+    orc, s0, P0, (x0, y0, z0) = start(seq, cfg)
+    with Registrar(ctx, cfg) as g:
+        g.set_state(s0, P0)
+        g.set_points(x0, y0, z0)
+        g.build()
+        for k in range(1, nscans):
+            dR, dp = seq.motion(k)
+            orc.apply_motion(dR, dp, QDIAG)
+            sp, Pp = orc.get_state()
+            x, y, z = seq.scan(k)
+            it_o = orc.register(x, y, z)
+            so, Po = orc.get_state()
+            g.set_state(sp, Pp)
+            g.set_points(x, y, z)
+            ost, mst = g.register()
+            sg, Pg = g.get_state()
+            ep, er = state_err(so, sg)
+            eP = np.abs(Po - Pg).max() / np.abs(Po).max()
+            log(f"scan {k}: iters o/g {it_o}/{ost.iterations} pos {ep:.2e} rot {er:.2e} P {eP:.2e}",
+                {f: getattr(mst, f) for f, _ in mst._fields_}, orc.stats())
+            assert ep <= 1e-4 and er <= 1e-5
+            r = compare_maps(orc.export_map(), g.export_map(), report=lambda o: log("  map", o))
+            assert r["plane_flag_diff"] <= max(2, r["n"] // 1000)

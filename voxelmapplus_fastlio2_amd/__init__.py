@@ -1,0 +1,18 @@
+"""MI355X registration path of voxel_plus (VoxelMapPlus + FAST-LIO2 IESKF).
+
+The product is the HIP library behind include/vmp.h; this package holds its
+kernels (csrc/), the in-tree build (build.py), ctypes bindings and a Python
+mirror of the reference interface (registration.py), and the seeded synthetic
+sequence generator (synth.py).
+"""
+from .registration import Context, LIOConfig, Registrar, VmpError, initial_covariance, register_batch, state_vector
+
+__all__ = [
+    "Context",
+    "LIOConfig",
+    "Registrar",
+    "VmpError",
+    "initial_covariance",
+    "register_batch",
+    "state_vector",
+]

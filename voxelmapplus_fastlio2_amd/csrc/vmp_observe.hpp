@@ -1,0 +1,318 @@
+// Observe pass: the measurement function of the IESKF
+// (LIOBuilder::sharedUpdateFunc, lio_builder.cpp:250-311) for one or many
+// sessions per launch.
+//
+// Per point (one lane, SoA point stream, grid-stride over PPT points):
+//   u   = r_wl p + (p_wl - anchor)   compensated fp32 (hi/lo split of r_wl
+//                                    and t, TwoProduct/TwoSum) ~ 1e-12 m
+//   key = floor(u / vs) + anchor     fp32 unless within 1e-4 voxel of a face,
+//                                    then the reference's fp64 expression
+//   hash probe -> voxel id -> 32 B plane record (mean relative to the voxel
+//   corner, normal, flags); residual + 3-sigma gate (voxel_map.cpp:258-276);
+//   the slot (normal, residual | NaN = invalid) persists across passes and
+//   scans exactly like residual_info (stale on a miss).
+//   Valid slots contribute w J^T J and w J r (lio_builder.cpp:283-307).
+// Per-point math is fp32; per-lane sums, the wave/block reduction and the
+// fixed-order cross-block reduction are fp64.  The block decomposition of a
+// session depends only on its point count, so a session's H, b are bitwise
+// identical alone or inside a batch.
+#pragma once
+#include "vmp_common.hpp"
+
+namespace vmp {
+
+constexpr int kObsThreads = 256;
+
+struct ObsJob {
+  const float* x;
+  const float* y;
+  const float* z;
+  float4* slots;
+  const HashEntry* table;
+  const VoxHot* hot;
+  const ObsParams* prm;
+  double* partials;       // [nblocks][nacc]
+  int32_t* counters;      // hits, plane_hits, effective, fallback (+ error)
+  unsigned int* ticket;
+  double* out;            // H (144, column-major) then b (12)
+  uint32_t hmask;
+  int32_t n;
+  int32_t nblocks;
+  int32_t block0;
+  int32_t ppt;            // points per thread
+  int32_t pad;
+};
+
+__device__ inline void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// Wave-level fp64 sum (64 lanes)
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline int wave_sum_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ inline int find_job(const ObsJob* jobs, int njobs, int blk) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].block0 <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ inline int32_t hash_find(const HashEntry* table, uint32_t mask, uint64_t key, int32_t* err) {
+  uint32_t h = hash_slot(key, mask);
+  for (uint32_t probe = 0; probe <= mask; ++probe) {
+    const uint4 e = *reinterpret_cast<const uint4*>(table + h);
+    const uint64_t k = uint64_t(e.x) | (uint64_t(e.y) << 32);
+    if (k == key) return int32_t(e.z);
+    if (k == kKeyEmpty) return -1;
+    h = (h + 1) & mask;
+  }
+  if (err) atomicOr(err, 1);
+  return -1;
+}
+
+template <int NJ>
+__global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __restrict__ jobs, int njobs) {
+  constexpr int NH = NJ * (NJ + 1) / 2;
+  constexpr int NACC = NH + NJ;
+  __shared__ double red[kObsThreads / 64][NACC];
+  __shared__ int redi[kObsThreads / 64][4];
+  __shared__ int is_last;
+
+  const int ji = find_job(jobs, njobs, blockIdx.x);
+  const ObsJob& J = jobs[ji];
+  const ObsParams& P = *J.prm;
+  if (!P.active) return;  // IESKF converged: reference leaves the loop
+  const int lb = blockIdx.x - J.block0;
+  const int tid = threadIdx.x;
+
+  double acc[NACC];
+#pragma unroll
+  for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
+  int c_hit = 0, c_plane = 0, c_eff = 0, c_fb = 0;
+
+  for (int k = 0; k < J.ppt; ++k) {
+    const int i = (lb * J.ppt + k) * kObsThreads + tid;
+    if (i >= J.n) break;
+    const float px = J.x[i], py = J.y[i];
+    float pz = J.z[i];
+    const bool zfix = (pz == 0.0f);  // calcBodyCov mutation, commons.cpp:20-23
+    if (zfix) pz = 0.001f;
+    const float p[3] = {px, py, pz};
+    // ---- compensated transform into the anchor frame
+    float uh[3], ul[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      float s = P.th[r], c = P.tl[r];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const float hi = P.Rh[r * 3 + q];
+        const float prod = hi * p[q];
+        const float perr = __builtin_fmaf(hi, p[q], -prod);
+        float s2, e2;
+        two_sum(s, prod, s2, e2);
+        s = s2;
+        c += perr + e2 + P.Rl[r * 3 + q] * p[q];
+      }
+      float h2, l2;
+      two_sum(s, c, h2, l2);
+      uh[r] = h2;
+      ul[r] = l2;
+    }
+    // ---- key (bit-exact with the reference's fp64 floor(p_w / vs))
+    int64_t key[3];
+    float kl[3];
+    bool near_face = false;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const float f = (uh[r] + ul[r]) * P.inv_vs;
+      kl[r] = floorf(f);
+      const float fr = f - kl[r];
+      near_face |= (fr < 1e-4f) || (fr > 1.0f - 1e-4f);
+    }
+    if (near_face) {
+      ++c_fb;
+      const double pd[3] = {double(px), double(py), zfix ? 0.001 : double(pz)};
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        double s = P.r_wl[r * 3] * pd[0];
+        s += P.r_wl[r * 3 + 1] * pd[1];
+        s += P.r_wl[r * 3 + 2] * pd[2];
+        const double pw = s + P.p_wl[r];
+        key[r] = int64_t(floor(pw / P.vs64));
+        kl[r] = float(key[r] - P.anchor[r]);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) key[r] = int64_t(kl[r]) + P.anchor[r];
+    }
+    float4 slot = J.slots[i];
+    bool in_range = key_ok(key[0]) && key_ok(key[1]) && key_ok(key[2]);
+    const int32_t vid = in_range ? hash_find(J.table, J.hmask, pack_key(key[0], key[1], key[2]), J.counters + 4) : -1;
+
+    const float r2 = px * px + py * py + pz * pz;
+    const float ca = r2 * P.s2;
+    const float cb = (P.rv - ca) / r2;
+    const float w0 = P.rwl32[0] * px + P.rwl32[1] * py + P.rwl32[2] * pz;
+    const float w1 = P.rwl32[3] * px + P.rwl32[4] * py + P.rwl32[5] * pz;
+    const float w2 = P.rwl32[6] * px + P.rwl32[7] * py + P.rwl32[8] * pz;
+    if (vid >= 0) {
+      ++c_hit;
+      const float4 h0 = *reinterpret_cast<const float4*>(J.hot + vid);
+      const float4 h1 = *(reinterpret_cast<const float4*>(J.hot + vid) + 1);
+      const uint32_t fl = __float_as_uint(h0.w);
+      if (fl & kPlane) {
+        ++c_plane;
+        const float qx = (uh[0] - kl[0] * P.vs) + ul[0];
+        const float qy = (uh[1] - kl[1] * P.vs) + ul[1];
+        const float qz = (uh[2] - kl[2] * P.vs) + ul[2];
+        const float nx = h1.x, ny = h1.y, nz = h1.z;
+        const float res = nx * (qx - h0.x) + ny * (qy - h0.y) + nz * (qz - h0.z);
+        // sigma^2 = n^T (R C_l R^T + [p]x Prr [p]x^T + Ppp) n   (plane_cov term is 0)
+        const float nn = nx * nx + ny * ny + nz * nz;
+        const float nw = nx * w0 + ny * w1 + nz * w2;
+        const float cx = ny * pz - nz * py, cy = nz * px - nx * pz, cz = nx * py - ny * px;  // n x p
+        const float* A = P.Prr;
+        const float* B = P.Ppp;
+        const float qa = cx * (A[0] * cx + A[1] * cy + A[2] * cz) + cy * (A[1] * cx + A[3] * cy + A[4] * cz) +
+                         cz * (A[2] * cx + A[4] * cy + A[5] * cz);
+        const float qb = nx * (B[0] * nx + B[1] * ny + B[2] * nz) + ny * (B[1] * nx + B[3] * ny + B[4] * nz) +
+                         nz * (B[2] * nx + B[4] * ny + B[5] * nz);
+        const float sig2 = ca * nn + cb * nw * nw + qa + qb;
+        const bool valid = fabsf(res) < 3.0f * sqrtf(sig2);
+        slot = make_float4(nx, ny, nz, valid ? res : __int_as_float(0x7fc00000));
+      } else {
+        slot.w = __int_as_float(0x7fc00000);
+      }
+      J.slots[i] = slot;
+    }
+    if (!isnan(slot.w)) {
+      ++c_eff;
+      const float nx = slot.x, ny = slot.y, nz = slot.z, res = slot.w;
+      const float nw = nx * w0 + ny * w1 + nz * w2;
+      const float rc = ca * (nx * nx + ny * ny + nz * nz) + cb * nw * nw;
+      const float wi = rc < 0.0002f ? 5000.0f : 1.0f / rc;
+      // a' = rot^T n ; v = rot_ext p + pos_ext ; J_rot = v x a'
+      const float* Rt = P.rot;
+      const float ax = Rt[0] * nx + Rt[3] * ny + Rt[6] * nz;
+      const float ay = Rt[1] * nx + Rt[4] * ny + Rt[7] * nz;
+      const float az = Rt[2] * nx + Rt[5] * ny + Rt[8] * nz;
+      const float* E = P.ext_R;
+      const float vx = E[0] * px + E[1] * py + E[2] * pz + P.ext_t[0];
+      const float vy = E[3] * px + E[4] * py + E[5] * pz + P.ext_t[1];
+      const float vz = E[6] * px + E[7] * py + E[8] * pz + P.ext_t[2];
+      float Jv[NJ];
+      Jv[0] = nx;
+      Jv[1] = ny;
+      Jv[2] = nz;
+      Jv[3] = vy * az - vz * ay;
+      Jv[4] = vz * ax - vx * az;
+      Jv[5] = vx * ay - vy * ax;
+      if constexpr (NJ == 12) {
+        const float* W = P.rwl32;
+        const float bx = W[0] * nx + W[3] * ny + W[6] * nz;
+        const float by = W[1] * nx + W[4] * ny + W[7] * nz;
+        const float bz = W[2] * nx + W[5] * ny + W[8] * nz;
+        Jv[6] = py * bz - pz * by;
+        Jv[7] = pz * bx - px * bz;
+        Jv[8] = px * by - py * bx;
+        Jv[9] = ax;
+        Jv[10] = ay;
+        Jv[11] = az;
+      }
+      int o = 0;
+#pragma unroll
+      for (int a = 0; a < NJ; ++a) {
+        const double wa = double(wi) * double(Jv[a]);
+#pragma unroll
+        for (int b2 = a; b2 < NJ; ++b2) acc[o++] += wa * double(Jv[b2]);
+      }
+#pragma unroll
+      for (int a = 0; a < NJ; ++a) acc[NH + a] += double(wi) * double(Jv[a]) * double(res);
+    }
+  }
+
+  // ---- block reduction (fp64, fixed order)
+  const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int k = 0; k < NACC; ++k) {
+    const double s = wave_sum(acc[k]);
+    if (lane == 0) red[wv][k] = s;
+  }
+  c_hit = wave_sum_i(c_hit);
+  c_plane = wave_sum_i(c_plane);
+  c_eff = wave_sum_i(c_eff);
+  c_fb = wave_sum_i(c_fb);
+  if (lane == 0) {
+    redi[wv][0] = c_hit;
+    redi[wv][1] = c_plane;
+    redi[wv][2] = c_eff;
+    redi[wv][3] = c_fb;
+  }
+  __syncthreads();
+  if (tid < NACC) {
+    double s = red[0][tid];
+    for (int w = 1; w < kObsThreads / 64; ++w) s += red[w][tid];
+    J.partials[size_t(lb) * NACC + tid] = s;
+  }
+  if (tid < 4) {
+    int s = 0;
+    for (int w = 0; w < kObsThreads / 64; ++w) s += redi[w][tid];
+    atomicAdd(J.counters + tid, s);
+  }
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) is_last = (atomicAdd(J.ticket, 1u) == unsigned(J.nblocks - 1));
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+
+  // ---- last block: reduce all block partials of this job in a fixed order
+  double fin[NACC];
+#pragma unroll
+  for (int k = 0; k < NACC; ++k) fin[k] = 0.0;
+  for (int b = tid; b < J.nblocks; b += kObsThreads) {
+    const double* src = J.partials + size_t(b) * NACC;
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) fin[k] += __builtin_nontemporal_load(src + k);
+  }
+#pragma unroll
+  for (int k = 0; k < NACC; ++k) {
+    const double s = wave_sum(fin[k]);
+    if (lane == 0) red[wv][k] = s;
+  }
+  __syncthreads();
+  if (tid < NACC) {
+    double s = red[0][tid];
+    for (int w = 1; w < kObsThreads / 64; ++w) s += red[w][tid];
+    red[0][tid] = s;
+  }
+  __syncthreads();
+  if (tid < 144) {  // H, 12x12 column-major, symmetric from the upper triangle
+    const int r = tid % 12, c = tid / 12;
+    double v = 0.0;
+    if (r < NJ && c < NJ) {
+      const int a = r < c ? r : c, b2 = r < c ? c : r;
+      const int o = a * NJ - a * (a - 1) / 2 + (b2 - a);
+      v = red[0][o];
+    }
+    J.out[tid] = v;
+  }
+  if (tid < 12) J.out[144 + tid] = tid < NJ ? red[0][NH + tid] : 0.0;
+  if (tid == 0) *J.ticket = 0u;
+}
+
+}  // namespace vmp

@@ -1,0 +1,285 @@
+"""Python mirror of the reference's registration interfaces over the C ABI.
+
+Names follow voxel_plus (paths relative to the reference root,
+voxel_plus/src/map_builder/):
+
+  LIOConfig             lio_builder.h:17-42 (subset on the registration path)
+  Registrar.set_points  residual-slot prep, lio_builder.cpp:225-229
+  Registrar.observe     LIOBuilder::sharedUpdateFunc, lio_builder.cpp:250-311
+  Registrar.update      kf::IESKF::update, ieskf.cpp:125-156
+  Registrar.build       VoxelMap::build via MAP_INIT, lio_builder.cpp:185-211
+  Registrar.map_update  VoxelMap::update via lio_builder.cpp:231-246
+  Registrar.register    the LIO_MAPPING branch after undistortion
+
+State vectors use one flat layout of 36 float64 values:
+pos[3], rot[9], rot_ext[9], pos_ext[3], vel[3], bg[3], ba[3], g[3], with the
+3x3 blocks column-major (Eigen's memory order); P is 23x23.
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+
+class VmpError(RuntimeError):
+    pass
+
+
+@dataclass
+class LIOConfig:
+    """Effective reference parameters (lio_builder.h:17-42, mapping.yaml:11-17).
+
+    update_size_thresh is 10 because the yaml key is misspelled
+    (``update_size_threshes``) and the node falls back to its default;
+    merge_thresh_for_angle is the yaml's 0.2.
+    """
+
+    voxel_size: float = 0.5
+    plane_thresh: float = 0.01
+    merge_thresh_for_angle: float = 0.2
+    merge_thresh_for_distance: float = 0.04
+    ranging_cov: float = 0.04
+    angle_cov: float = 0.1
+    max_point_thresh: int = 100
+    update_size_thresh: int = 10
+    map_capacity: int = 100000
+    max_scan_points: int = 20000
+    max_iter: int = 5
+    estimate_ext: int = 0
+
+    def to_c(self):
+        return _lib.MapCfg(
+            self.voxel_size,
+            self.plane_thresh,
+            self.merge_thresh_for_angle,
+            self.merge_thresh_for_distance,
+            self.ranging_cov,
+            self.angle_cov,
+            self.max_point_thresh,
+            self.update_size_thresh,
+            self.map_capacity,
+            self.max_scan_points,
+            self.max_iter,
+            self.estimate_ext,
+        )
+
+
+def state_vector(R=None, t=None):
+    """Flat 36-value state with the given LiDAR pose, gravity (0,0,-9.81)."""
+    s = np.zeros(36)
+    s[3:12] = (np.eye(3) if R is None else np.asarray(R)).T.reshape(-1)  # column-major
+    s[12:21] = np.eye(3).reshape(-1)
+    if t is not None:
+        s[0:3] = t
+    s[33:36] = [0.0, 0.0, -9.81]
+    return s
+
+
+def state_rot(s):
+    return np.asarray(s[3:12]).reshape(3, 3).T
+
+
+def initial_covariance():
+    """P after IMU initialisation (lio_builder.cpp:55-60)."""
+    P = np.eye(23)
+    for i in range(6, 12):
+        P[i, i] = 1e-5
+    for i in range(15, 21):
+        P[i, i] = 1e-4
+    P[21, 21] = P[22, 22] = 1e-5
+    return P
+
+
+def _state_c(s):
+    st = _lib.State()
+    C.memmove(C.addressof(st), np.ascontiguousarray(s, dtype=np.float64).ctypes.data, 36 * 8)
+    return st
+
+
+def _state_np(st):
+    out = np.empty(36)
+    C.memmove(out.ctypes.data, C.addressof(st), 36 * 8)
+    return out
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Context:
+    """One device (vmp_ctx)."""
+
+    def __init__(self, device=0):
+        self.lib = _lib.vmp()
+        h = C.c_void_p()
+        rc = self.lib.vmp_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise VmpError(f"vmp_ctx_create failed ({rc})")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.vmp_ctx_destroy(self.h)
+            self.h = None
+
+
+class Registrar:
+    """One LiDAR sequence: voxel map, residual slots and filter state in HBM."""
+
+    def __init__(self, ctx, cfg=None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.cfg = cfg or LIOConfig()
+        h = C.c_void_p()
+        c = self.cfg.to_c()
+        rc = self.lib.vmp_session_create(ctx.h, C.byref(c), C.byref(h))
+        if rc != 0:
+            raise VmpError(f"vmp_session_create failed ({rc})")
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if self.h:
+            self.lib.vmp_session_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.vmp_last_error(self.h)
+            raise VmpError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    # ---- filter state
+    def set_state(self, s, P=None):
+        st = _state_c(s)
+        Pc = None if P is None else np.asfortranarray(P, dtype=np.float64)
+        self._check(
+            self.lib.vmp_state_set(self.h, C.byref(st), None if Pc is None else Pc.ctypes.data_as(C.POINTER(C.c_double))),
+            "vmp_state_set",
+        )
+
+    def get_state(self):
+        st = _lib.State()
+        P = np.zeros((23, 23), order="F")
+        self._check(self.lib.vmp_state_get(self.h, C.byref(st), P.ctypes.data_as(C.POINTER(C.c_double))), "vmp_state_get")
+        return _state_np(st), np.ascontiguousarray(P)
+
+    def apply_motion(self, dR, dp, qdiag):
+        dRc = np.asfortranarray(dR, dtype=np.float64)
+        dpc = np.ascontiguousarray(dp, dtype=np.float64)
+        q = np.ascontiguousarray(qdiag, dtype=np.float64)
+        self._check(
+            self.lib.vmp_state_apply_motion(self.h, dRc.ctypes.data_as(C.POINTER(C.c_double)), _dptr(dpc), _dptr(q)),
+            "vmp_state_apply_motion",
+        )
+
+    # ---- scan
+    def set_points(self, x, y, z):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        z = np.ascontiguousarray(z, dtype=np.float32)
+        self._check(self.lib.vmp_scan_set_points(self.h, _fptr(x), _fptr(y), _fptr(z), len(x)), "vmp_scan_set_points")
+        self.n = len(x)
+
+    def set_points_device(self, px, py, pz, n):
+        """Device pointers (e.g. torch tensors' data_ptr()) already in HBM."""
+        self._check(
+            self.lib.vmp_scan_set_points_device(self.h, C.c_void_p(px), C.c_void_p(py), C.c_void_p(pz), int(n)),
+            "vmp_scan_set_points_device",
+        )
+        self.n = int(n)
+
+    # ---- path entry points
+    def observe(self, s):
+        st = _state_c(s)
+        H = np.zeros((12, 12), order="F")
+        b = np.zeros(12)
+        o = _lib.ObsStats()
+        self._check(
+            self.lib.vmp_observe(self.h, C.byref(st), H.ctypes.data_as(C.POINTER(C.c_double)), _dptr(b), C.byref(o)),
+            "vmp_observe",
+        )
+        return np.ascontiguousarray(H), b, o
+
+    def update(self):
+        o = _lib.ObsStats()
+        self._check(self.lib.vmp_ieskf_update(self.h, C.byref(o)), "vmp_ieskf_update")
+        return o
+
+    def build(self):
+        m = _lib.MapStats()
+        self._check(self.lib.vmp_map_update(self.h, 1, C.byref(m)), "vmp_map_update(build)")
+        return m
+
+    def map_update(self):
+        m = _lib.MapStats()
+        self._check(self.lib.vmp_map_update(self.h, 0, C.byref(m)), "vmp_map_update")
+        return m
+
+    def register(self):
+        o = _lib.ObsStats()
+        m = _lib.MapStats()
+        self._check(self.lib.vmp_register_scan(self.h, C.byref(o), C.byref(m)), "vmp_register_scan")
+        return o, m
+
+    def timing(self):
+        a = C.c_double()
+        b = C.c_double()
+        n = C.c_int32()
+        self._check(self.lib.vmp_session_timing(self.h, C.byref(a), C.byref(b), C.byref(n)), "vmp_session_timing")
+        return a.value, b.value, n.value
+
+    # ---- inspection
+    def map_size(self):
+        return int(self.lib.vmp_map_size(self.h))
+
+    def export_map(self):
+        n = self.map_size()
+        keys = np.zeros((max(n, 1), 3), np.int64)
+        ints = np.zeros((max(n, 1), 5), np.int32)
+        group = np.zeros(max(n, 1), np.uint64)
+        dbl = np.zeros((max(n, 1), 8))
+        cnt = self.lib.vmp_map_export(
+            self.h,
+            n,
+            keys.ctypes.data_as(C.POINTER(C.c_int64)),
+            ints.ctypes.data_as(C.POINTER(C.c_int32)),
+            group.ctypes.data_as(C.POINTER(C.c_uint64)),
+            _dptr(dbl),
+        )
+        if cnt < 0:
+            raise VmpError("vmp_map_export failed")
+        return {"keys": keys[:cnt], "ints": ints[:cnt], "group": group[:cnt], "dbl": dbl[:cnt]}
+
+    def export_slots(self, n):
+        valid = np.zeros(n, np.uint8)
+        res = np.zeros(n, np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        self._check(
+            self.lib.vmp_slots_export(self.h, n, valid.ctypes.data_as(C.POINTER(C.c_uint8)), _fptr(res), _fptr(nrm)),
+            "vmp_slots_export",
+        )
+        return valid.astype(bool), res, nrm
+
+
+def register_batch(registrars):
+    lib = registrars[0].lib
+    n = len(registrars)
+    arr = (C.c_void_p * n)(*[r.h.value for r in registrars])
+    o = (_lib.ObsStats * n)()
+    m = (_lib.MapStats * n)()
+    rc = lib.vmp_register_batch(arr, n, o, m)
+    if rc != 0:
+        raise VmpError(f"vmp_register_batch failed ({rc})")
+    return o, m

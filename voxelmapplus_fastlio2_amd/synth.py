@@ -1,0 +1,74 @@
+"""Seeded synthetic sequences (room / street) — see include/vmp_synth.h."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+ROOM, STREET = 0, 1
+SEED0 = 0x5EED0000
+
+# Motion-prior process noise added to P between scans (stand-in for IMU
+# propagation, which is outside the registration path): 2 cm and 0.2 deg
+# per axis match the generator's odometry noise.
+QDIAG = np.zeros(23)
+QDIAG[0:3] = 4e-4
+QDIAG[3:6] = 1.2e-5
+QDIAG[6:12] = 1e-10
+QDIAG[12:15] = 1e-4
+QDIAG[15:23] = 1e-8
+
+
+class Sequence:
+    def __init__(self, scene=ROOM, seed=SEED0, n=20000, origin_offset=None):
+        self.lib = _lib.synth()
+        self.n = int(n)
+        self.scene = scene
+        off = None
+        if origin_offset is not None:
+            self._off = np.ascontiguousarray(origin_offset, dtype=np.float64)
+            off = self._off.ctypes.data_as(C.POINTER(C.c_double))
+        h = C.c_void_p()
+        rc = self.lib.vmp_synth_create(scene, seed, self.n, off, C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"vmp_synth_create failed ({rc})")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.vmp_synth_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def scan(self, k):
+        x = np.empty(self.n, np.float32)
+        y = np.empty(self.n, np.float32)
+        z = np.empty(self.n, np.float32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        rc = self.lib.vmp_synth_scan(self.h, k, fp(x), fp(y), fp(z), None)
+        if rc != 0:
+            raise RuntimeError(f"vmp_synth_scan failed ({rc})")
+        return x, y, z
+
+    def pose(self, k):
+        R = np.zeros(9)
+        t = np.zeros(3)
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        rc = self.lib.vmp_synth_pose(self.h, k, dp(R), dp(t))
+        if rc != 0:
+            raise RuntimeError("vmp_synth_pose failed")
+        return R.reshape(3, 3).T.copy(), t
+
+    def motion(self, k):
+        dR = np.zeros(9)
+        dpv = np.zeros(3)
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        rc = self.lib.vmp_synth_motion(self.h, k, dp(dR), dp(dpv))
+        if rc != 0:
+            raise RuntimeError("vmp_synth_motion failed")
+        return dR.reshape(3, 3).T.copy(), dpv
