@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Registered points/s of the voxel_plus registration path on one MI355X.
+
+A step is one registered scan: the IESKF update (<= 5 observe passes of
+sharedUpdateFunc + the FP64 filter algebra) followed by the map update
+(VoxelMap::update), on a seeded synthetic street sequence of 120,000 points
+per scan (0.5 m voxels, 2,000,000-voxel LRU).  Scans are generated on the
+host and copied to HBM before timing; the timed region is the device path
+with one host synchronisation per scan (the plain, single-scan form), so the
+per-scan wall time is also the latency reported as p50 / p99.
+
+The same JSON line carries the live roofline figure of the observe kernel
+(HIP events around every launch on its stream) and the FP64 oracle timed on
+the host over a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "registered points/s (batched vs plain, same GPU); p50/p99 scan latency"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes of one observe pass (see the observe kernel's header):
+# every point reads xyz (12 B), its residual slot (16 B) and one hash slot
+# (16 B); a hit also reads the 32 B plane record and writes the slot (16 B).
+OBS_BYTES_POINT = 44
+OBS_BYTES_HIT = 48
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scans, motions, cfg, s0, P0, n_warm, n_timed):
+    """FP64 oracle (one host thread) on the first scans of the same workload."""
+    from oracle import Oracle
+    from voxelmapplus_fastlio2_amd.synth import QDIAG
+
+    orc = Oracle(cfg)
+    orc.set_state(s0, P0)
+    orc.build(*scans[0])
+    t_reg = 0.0
+    for k in range(1, 1 + n_warm + n_timed):
+        orc.apply_motion(*motions[k], QDIAG)
+        t0 = time.perf_counter()
+        orc.register(*scans[k])
+        dt = time.perf_counter() - t0
+        if k > n_warm:
+            t_reg += dt
+    n = len(scans[0][0])
+    return {
+        "value": n * n_timed / t_reg,
+        "unit": "points/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"street {n} pts/scan, scans {n_warm + 1}-{n_warm + n_timed} after MAP_INIT "
+        f"({n_timed} registered scans, {t_reg:.1f} s), FP64 oracle, 1 thread",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--points", type=int, default=120000)
+    ap.add_argument("--capacity", type=int, default=2000000)
+    ap.add_argument("--cpu-warm", type=int, default=3)
+    ap.add_argument("--cpu-timed", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            "bench.py measures one GPU: every box in this build has a single MI355X and the path "
+            "shards by session with no exchange; run with --gpus 1.",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+
+    import torch
+
+    from voxelmapplus_fastlio2_amd import Context, LIOConfig, Registrar, initial_covariance, state_vector
+    from voxelmapplus_fastlio2_amd.synth import QDIAG, SEED0, STREET, Sequence
+
+    N, W, K = args.points, args.warmup, args.steps
+    total = W + K + 1
+    seq = Sequence(STREET, SEED0, N)
+    log(f"generating {total} scans x {N} points")
+    t0 = time.perf_counter()
+    host = [seq.scan(k) for k in range(total)]
+    motions = [None] + [seq.motion(k) for k in range(1, total)]
+    log(f"generated in {time.perf_counter() - t0:.1f} s")
+    dev = torch.empty((total, 3, N), dtype=torch.float32, device="cuda:0")
+    for k, (x, y, z) in enumerate(host):
+        dev[k, 0].copy_(torch.from_numpy(x))
+        dev[k, 1].copy_(torch.from_numpy(y))
+        dev[k, 2].copy_(torch.from_numpy(z))
+    torch.cuda.synchronize()
+
+    cfg = LIOConfig(max_scan_points=N, map_capacity=args.capacity)
+    R, t = seq.pose(0)
+    s0, P0 = state_vector(R, t), initial_covariance()
+    ctx = Context(0)
+    g = Registrar(ctx, cfg)
+
+    def bind(k):
+        g.set_points_device(dev[k, 0].data_ptr(), dev[k, 1].data_ptr(), dev[k, 2].data_ptr(), N)
+
+    g.set_state(s0, P0)
+    bind(0)
+    g.build()
+    for k in range(1, W + 1):
+        g.apply_motion(*motions[k], QDIAG)
+        bind(k)
+        g.register()
+    torch.cuda.synchronize()
+    log(f"warmup done: map {g.map_size()} voxels")
+
+    g.set_timing(True)
+    lat = []
+    iters = 0
+    mstat = {"created": 0, "evicted": 0, "merges": 0, "merge_evals": 0, "refits": 0}
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(W + 1, W + K + 1):
+        ts = time.perf_counter()
+        g.apply_motion(*motions[k], QDIAG)
+        bind(k)
+        o, m = g.register()
+        lat.append(time.perf_counter() - ts)
+        iters += o.iterations
+        for f in mstat:
+            mstat[f] += getattr(m, f)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    tt = g.timing_totals()
+    map_size = g.map_size()
+    lat_ms = np.array(lat) * 1e3
+
+    obs_launches = max(tt["obs_launches"], 1)
+    obs_avg_ms = tt["obs_ms"] / obs_launches
+    obs_bytes = (tt["obs_points"] * OBS_BYTES_POINT + tt["obs_hits"] * OBS_BYTES_HIT) / obs_launches
+    achieved = obs_bytes / (obs_avg_ms * 1e-3) / 1e9
+    value = N * K / elapsed
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": elapsed / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 per point, f64 reductions and filter",
+        "data": "synthetic",
+        "config": {
+            "workload": f"street-{N // 1000}k-plain",
+            "form": "plain (one session, one scan per step, host sync per scan)",
+            "points_per_scan": N,
+            "voxel_size": cfg.voxel_size,
+            "map_capacity": cfg.map_capacity,
+            "map_voxels_end": map_size,
+            "ieskf_iterations_mean": iters / K,
+            "per_scan_mean": {f: v / K for f, v in mstat.items()},
+        },
+        "latency_ms": {
+            "p50": float(np.percentile(lat_ms, 50)),
+            "p99": float(np.percentile(lat_ms, 99)),
+            "max": float(lat_ms.max()),
+        },
+        "stage_ms_per_scan": {
+            "observe_kernels": tt["obs_ms"] / K,
+            "map_update": tt["map_ms"] / max(tt["map_calls"], 1),
+        },
+        "roofline": {
+            "kernel": "observe_kernel<6>",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": obs_bytes,
+            "avg_launch_ms": obs_avg_ms,
+            "launches": tt["obs_launches"],
+        },
+    }
+    g.close()
+    ctx.close()
+    if not args.no_cpu:
+        log("timing the FP64 oracle on the host")
+        line["cpu_baseline"] = cpu_baseline(host, motions, cfg, s0, P0, args.cpu_warm, args.cpu_timed)
+        line["vs_cpu"] = value / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

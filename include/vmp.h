@@ -148,6 +148,12 @@ int vmp_register_batch(vmp_session** s, int32_t n, vmp_obs_stats* ost, vmp_map_s
 
 /* Device events around the last call's kernels (for roofline timing). */
 int vmp_session_timing(vmp_session* s, double* observe_ms, double* map_ms, int32_t* observe_launches);
+/* Per-launch HIP-event timing of the observe kernel and of the map-update
+ * pipeline, accumulated until the next vmp_session_set_timing call.
+ * No reference counterpart (instrumentation). */
+int vmp_session_set_timing(vmp_session* s, int on);
+int vmp_session_timing_totals(vmp_session* s, double* obs_ms, int64_t* obs_launches, double* map_ms,
+                              int64_t* map_calls, int64_t* obs_points, int64_t* obs_hits);
 
 #ifdef __cplusplus
 }

@@ -122,3 +122,57 @@ def test_sequence_teacher_forced(ctx, scene, npts, nscans):
             assert ep <= 1e-4 and er <= 1e-5
             r = compare_maps(orc.export_map(), g.export_map(), report=lambda o: log("  map", o))
             assert r["plane_flag_diff"] <= max(2, r["n"] // 1000)
+            assert r["group_diff"] <= r["plane_flag_diff"] * 8
+            assert r["mean_bad"] <= r["plane_flag_diff"] * 8 and r["norm_bad"] <= r["plane_flag_diff"] * 8
+
+
+@pytest.mark.parametrize("capacity", [30000, 18000])
+def test_lru_eviction_parity(ctx, capacity):
+    """Map capacity below the growth of the street map: every scan evicts the
+    least recently touched voxels (voxel_map.cpp:244-248); keys, LRU order and
+    group ids must stay bit-exact."""
+    npts, nscans = 30000, 7
+    seq = Sequence(STREET, SEED0 + 11, npts)
+    cfg = LIOConfig(max_scan_points=npts, map_capacity=capacity)
+    orc, s0, P0, (x0, y0, z0) = start(seq, cfg)
+    with Registrar(ctx, cfg) as g:
+        g.set_state(s0, P0)
+        g.set_points(x0, y0, z0)
+        g.build()
+        evicted = 0
+        for k in range(1, nscans):
+            dR, dp = seq.motion(k)
+            orc.apply_motion(dR, dp, QDIAG)
+            sp, Pp = orc.get_state()
+            x, y, z = seq.scan(k)
+            orc.register(x, y, z)
+            g.set_state(sp, Pp)
+            g.set_points(x, y, z)
+            ost, mst = g.register()
+            evicted += mst.evicted
+            so, _ = orc.get_state()
+            sg, _ = g.get_state()
+            ep, er = state_err(so, sg)
+            log(f"cap {capacity} scan {k}: evicted {mst.evicted} created {mst.created} size {mst.map_size} "
+                f"oracle {orc.stats()} pos {ep:.2e} rot {er:.2e}")
+            assert ep <= 1e-4 and er <= 1e-5
+            r = compare_maps(orc.export_map(), g.export_map(), report=lambda o: log("  map", o))
+            assert r["plane_flag_diff"] <= max(2, r["n"] // 1000)
+        assert evicted > 0
+        assert orc.stats()["evicted"] == evicted
+
+
+def test_capacity_below_scan_is_an_error(ctx):
+    """A scan touching more voxels than the LRU can evict in point order is
+    reported (VMP_ECAPACITY) instead of silently diverging."""
+    from voxelmapplus_fastlio2_amd import VmpError
+
+    npts = 30000
+    seq = Sequence(STREET, SEED0 + 3, npts)
+    cfg = LIOConfig(max_scan_points=npts, map_capacity=4000)
+    R, t = seq.pose(0)
+    with Registrar(ctx, cfg) as g:
+        g.set_state(state_vector(R, t), initial_covariance())
+        g.set_points(*seq.scan(0))
+        with pytest.raises(VmpError, match="-5"):
+            g.build()

@@ -94,6 +94,7 @@ struct __align__(16) VoxState {
   int32_t seg;        // its segment in this scan's key-sorted points
   int32_t merge_pos;  // position in the segment where merge() calls start
   int32_t dirty_t;    // latest point time at which a merge() call could act
+  int32_t taint_t;    // first point time a merge changed its neighbourhood
 };
 
 // Plane record in FP64 (what merge() reads and writes, voxel_map.cpp:138-186):
@@ -206,6 +207,12 @@ __host__ __device__ inline void sym6_from_P(const double* P, int o, float out[6]
   out[5] = float(at(2, 2));
 }
 
+// sin^2 of the angular resolution (commons.cpp:26-28)
+__host__ __device__ inline float angle_s2(const vmp_map_cfg& cfg) {
+  const double sa = sin(cfg.angle_cov * 3.14159265358979323846 / 180.0);
+  return float(sa * sa);
+}
+
 __host__ __device__ inline void make_obs_params(const vmp_state& x, const double* P, const vmp_map_cfg& cfg,
                                                 ObsParams& o) {
   D33 r;
@@ -231,8 +238,7 @@ __host__ __device__ inline void make_obs_params(const vmp_state& x, const double
   o.vs = float(cfg.voxel_size);
   o.inv_vs = float(1.0 / cfg.voxel_size);
   o.vs64 = cfg.voxel_size;
-  double sa = sin(cfg.angle_cov * 3.14159265358979323846 / 180.0);
-  o.s2 = float(sa * sa);
+  o.s2 = angle_s2(cfg);
   o.rv = float(cfg.ranging_cov * cfg.ranging_cov);
   o.estimate_ext = cfg.estimate_ext;
   o.active = 1;
@@ -250,8 +256,7 @@ __host__ __device__ inline void make_upd_params(const vmp_state& x, const double
   for (int i = 0; i < 3; ++i) u.t[i] = float(p.v[i]);
   sym6_from_P(P, 3, u.Prr);
   sym6_from_P(P, 0, u.Ppp);
-  double sa = sin(cfg.angle_cov * 3.14159265358979323846 / 180.0);
-  u.s2 = float(sa * sa);
+  u.s2 = angle_s2(cfg);
   u.rv = float(cfg.ranging_cov * cfg.ranging_cov);
   u.vs64 = cfg.voxel_size;
 }

@@ -158,35 +158,24 @@ __device__ inline void getMx_d(const double g[3], const double res[2], double Mx
     for (int c = 0; c < 2; ++c) Mx[r * 2 + c] = -(A.m[r * 3] * B[c] + A.m[r * 3 + 1] * B[2 + c] + A.m[r * 3 + 2] * B[4 + c]);
 }
 
-// x - xp (ieskf.cpp:23-55)
-__device__ inline void state_minus(const vmp_state& x, const vmp_state& o, double d[23]) {
-  for (int i = 0; i < 3; ++i) d[i] = x.pos[i] - o.pos[i];
-  const D33 R = d33_mul(d33_t(d33_from_cm(o.rot)), d33_from_cm(x.rot));
-  const D33 Re = d33_mul(d33_t(d33_from_cm(o.rot_ext)), d33_from_cm(x.rot_ext));
-  so3_log_d(R, d + 3);
-  so3_log_d(Re, d + 6);
-  for (int i = 0; i < 3; ++i) {
-    d[9 + i] = x.pos_ext[i] - o.pos_ext[i];
-    d[12 + i] = x.vel[i] - o.vel[i];
-    d[15 + i] = x.bg[i] - o.bg[i];
-    d[18 + i] = x.ba[i] - o.ba[i];
-  }
-  const D33 hg = hat_d(x.g);
+// gravity part of x - xp on S2 (ieskf.cpp:34-52)
+__device__ inline void s2_minus_d(const double* g, const double* og, double* out) {
+  const D33 hg = hat_d(g);
   double hv[3];
-  for (int i = 0; i < 3; ++i) hv[i] = hg.m[i * 3] * o.g[0] + hg.m[i * 3 + 1] * o.g[1] + hg.m[i * 3 + 2] * o.g[2];
+  for (int i = 0; i < 3; ++i) hv[i] = hg.m[i * 3] * og[0] + hg.m[i * 3 + 1] * og[1] + hg.m[i * 3 + 2] * og[2];
   const double v_sin = sqrt(hv[0] * hv[0] + hv[1] * hv[1] + hv[2] * hv[2]);
-  const double v_cos = x.g[0] * o.g[0] + x.g[1] * o.g[1] + x.g[2] * o.g[2];
+  const double v_cos = g[0] * og[0] + g[1] * og[1] + g[2] * og[2];
   const double theta = atan2(v_sin, v_cos);
   if (v_sin < 1e-11) {
-    d[21] = fabs(theta) > 1e-11 ? 3.1415926 : 0.0;
-    d[22] = 0.0;
+    out[0] = fabs(theta) > 1e-11 ? 3.1415926 : 0.0;
+    out[1] = 0.0;
   } else {
     double B[6];
-    getBx_d(o.g, B);
-    const D33 ho = hat_d(o.g);
+    getBx_d(og, B);
+    const D33 ho = hat_d(og);
     double hx[3];
-    for (int i = 0; i < 3; ++i) hx[i] = ho.m[i * 3] * x.g[0] + ho.m[i * 3 + 1] * x.g[1] + ho.m[i * 3 + 2] * x.g[2];
-    for (int r = 0; r < 2; ++r) d[21 + r] = theta / v_sin * (B[r] * hx[0] + B[2 + r] * hx[1] + B[4 + r] * hx[2]);
+    for (int i = 0; i < 3; ++i) hx[i] = ho.m[i * 3] * g[0] + ho.m[i * 3 + 1] * g[1] + ho.m[i * 3 + 2] * g[2];
+    for (int r = 0; r < 2; ++r) out[r] = theta / v_sin * (B[r] * hx[0] + B[2 + r] * hx[1] + B[4 + r] * hx[2]);
   }
 }
 
@@ -233,6 +222,125 @@ __device__ inline double jac_at(const JacBlocks& J, int r, int c) {
   if (r >= 6 && r < 9 && c >= 6 && c < 9) return J.b[(r - 6) * 3 + (c - 6)];
   if (r >= 21 && c >= 21) return J.c[(r - 21) * 2 + (c - 21)];
   return r == c ? 1.0 : 0.0;
+}
+
+// ---- Cholesky on a symmetric positive definite 23x23 (row-major, LDS) ----
+// Right-looking factorisation in place (lower triangle = L), one wave.
+__device__ inline void chol_factor(double* A, int lane) {
+  constexpr int n = 23;
+  for (int k = 0; k < n; ++k) {
+    if (lane == 0) A[k * n + k] = sqrt(A[k * n + k]);
+    __syncthreads();
+    const double d = A[k * n + k];
+    const int i = k + 1 + lane;
+    if (i < n) A[i * n + k] = A[i * n + k] / d;
+    __syncthreads();
+    if (i < n) {
+      const double lik = A[i * n + k];
+      for (int j = k + 1; j <= i; ++j) A[i * n + j] -= lik * A[j * n + k];
+    }
+    __syncthreads();
+  }
+}
+// x = A^-1 b with A = L L^T factored by chol_factor; lane 0 substitutes.
+__device__ inline void chol_solve_vec(const double* L, const double* b, double* x, int lane) {
+  constexpr int n = 23;
+  if (lane == 0) {
+    double y[n];
+    for (int i = 0; i < n; ++i) {
+      double s = b[i];
+      for (int j = 0; j < i; ++j) s -= L[i * n + j] * y[j];
+      y[i] = s / L[i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double s = y[i];
+      for (int j = i + 1; j < n; ++j) s -= L[j * n + i] * x[j];
+      x[i] = s / L[i * n + i];
+    }
+  }
+  __syncthreads();
+}
+// Inv = A^-1 (row-major), one column per lane.
+__device__ inline void chol_inverse(const double* L, double* Inv, int lane) {
+  constexpr int n = 23;
+  if (lane < n) {
+    const int c = lane;
+    double y[n];
+    for (int i = 0; i < n; ++i) {
+      double s = (i == c) ? 1.0 : 0.0;
+      for (int j = 0; j < i; ++j) s -= L[i * n + j] * y[j];
+      y[i] = s / L[i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double s = y[i];
+      for (int j = i + 1; j < n; ++j) s -= L[j * n + i] * y[j];
+      y[i] = s / L[i * n + i];
+    }
+    for (int i = 0; i < n; ++i) Inv[i * n + c] = y[i];
+  }
+  __syncthreads();
+}
+
+// ---- lane-parallel pieces of the state algebra (inputs in LDS) ----------
+// x - xp split over three lanes: rotation, extrinsic rotation, gravity chart
+__device__ inline void state_minus_par(const vmp_state& x, const vmp_state& o, double* d, int lane) {
+  if (lane == 0) {
+    so3_log_d(d33_mul(d33_t(d33_from_cm(o.rot)), d33_from_cm(x.rot)), d + 3);
+  } else if (lane == 1) {
+    so3_log_d(d33_mul(d33_t(d33_from_cm(o.rot_ext)), d33_from_cm(x.rot_ext)), d + 6);
+  } else if (lane == 2) {
+    s2_minus_d(x.g, o.g, d + 21);
+  } else if (lane == 3) {
+    for (int i = 0; i < 3; ++i) {
+      d[i] = x.pos[i] - o.pos[i];
+      d[9 + i] = x.pos_ext[i] - o.pos_ext[i];
+      d[12 + i] = x.vel[i] - o.vel[i];
+      d[15 + i] = x.bg[i] - o.bg[i];
+      d[18 + i] = x.ba[i] - o.ba[i];
+    }
+  }
+  __syncthreads();
+}
+__device__ inline void jac_blocks_par(const vmp_state& x, const vmp_state& xp, const double* d, JacBlocks& J,
+                                      int lane) {
+  if (lane == 0) {
+    const D33 A = d33_t(left_jac_d(d + 3));
+    for (int i = 0; i < 9; ++i) J.a[i] = A.m[i];
+  } else if (lane == 1) {
+    const D33 B = d33_t(left_jac_d(d + 6));
+    for (int i = 0; i < 9; ++i) J.b[i] = B.m[i];
+  } else if (lane == 2) {
+    double N[6], M[6];
+    getNx_d(x.g, N);
+    getMx_d(xp.g, d + 21, M);
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 2; ++c) J.c[r * 2 + c] = N[r * 3] * M[c] + N[r * 3 + 1] * M[2 + c] + N[r * 3 + 2] * M[4 + c];
+  }
+  __syncthreads();
+}
+__device__ inline void state_plus_par(vmp_state& x, const double* d, int lane) {
+  if (lane == 0) {
+    d33_to_cm(d33_mul(d33_from_cm(x.rot), so3_exp_d(d + 3)), x.rot);
+  } else if (lane == 1) {
+    d33_to_cm(d33_mul(d33_from_cm(x.rot_ext), so3_exp_d(d + 6)), x.rot_ext);
+  } else if (lane == 2) {
+    double B[6];
+    getBx_d(x.g, B);
+    const double w[3] = {B[0] * d[21] + B[1] * d[22], B[2] * d[21] + B[3] * d[22], B[4] * d[21] + B[5] * d[22]};
+    const D33 E = so3_exp_d(w);
+    double g2[3];
+    for (int i = 0; i < 3; ++i) g2[i] = E.m[i * 3] * x.g[0] + E.m[i * 3 + 1] * x.g[1] + E.m[i * 3 + 2] * x.g[2];
+    for (int i = 0; i < 3; ++i) x.g[i] = g2[i];
+  } else if (lane == 3) {
+    for (int i = 0; i < 3; ++i) {
+      x.pos[i] += d[i];
+      x.pos_ext[i] += d[9 + i];
+      x.vel[i] += d[12 + i];
+      x.bg[i] += d[15 + i];
+      x.ba[i] += d[18 + i];
+    }
+  }
+  __syncthreads();
 }
 
 // Gauss-Jordan inverse with partial pivoting of the 23x23 row-major M (LDS),

@@ -1,23 +1,24 @@
 // Map update: VoxelMap::update / build (voxel_map.cpp:200-256) with
 // VoxelGrid::pushPoint / updatePlane / merge (voxel_map.cpp:42-186), executed
-// as a pipeline of data-parallel stages plus two short ordered replays.
+// for a batch of independent sessions as a pipeline of data-parallel stages
+// plus two short ordered replays (one wave per session).
+//
+// Points of all sessions in a batch share one global index space, session
+// after session (g = off[s] + i); times inside a session are its point
+// indices i, which is the order the reference loops in.
 //
 //   K1 ins_keys      per point: fp32 insertion transform (PCL order), key,
-//                    hash find-or-insert -> (hash slot, point index)
-//   K2 sort          radix sort by hash slot (stable: point order kept)
-//   K3 seg_heads     segment = one key's points in scan order
-//   K4 seg_info      existing voxel of each segment; new keys flagged at their
-//                    first point
+//                    hash find-or-insert in the session's table
+//   K2 sort          per session, stable radix sort by hash slot: each key's
+//                    points become a contiguous segment in point order
+//   K3/K4 segments   segment heads, the existing voxel of each segment, new
+//                    keys flagged at their first point
 //   K5 create        creation events in point order: parallel when the LRU
-//                    cannot overflow, else one wave replays creations and
-//                    evictions against the LRU queue (oldest stamp first,
-//                    skipping voxels already touched earlier in the scan)
+//                    cannot overflow, else one wave per session replays
+//                    creations and evictions against the LRU queue
 //   K6 replay        one lane per voxel walks its points: the pushPoint state
-//                    machine; refits from centre-local moments (no point pool)
-//   K7 candidates    merge() calls that can change state: the first call after
-//                    each event that could make a previously idle call act
-//   K8 merge replay  one wave evaluates those calls in global point order,
-//                    adding the follow-up calls of every voxel a merge touched
+//                    machine, refits from centre-local moments (no point pool)
+//   K7/K8 merge      see vmp_merge.hpp
 //   K9 finalize      dirty bits, LRU queue append, hash tombstones, free list
 // Every ordering decision (LRU order, victims, group ids, merge order) is
 // therefore the reference's point order.
@@ -32,23 +33,24 @@ constexpr int kHeapCap = 4096;
 constexpr int32_t kNever = 0x7fffffff;
 
 struct MapCounters {
-  int64_t size;           // live voxels
-  uint64_t group_counter; // VoxelGrid::count
-  int64_t point_base;     // global point index of this scan's point 0
+  int64_t size;            // live voxels
+  uint64_t group_counter;  // VoxelGrid::count
+  int64_t point_base;      // global point index of this scan's point 0 (LRU stamps)
   int64_t ring_head, ring_tail;
   int32_t free_top, bump;
-  int32_t used_entries;   // non-empty hash slots, tombstones included
+  int32_t used_entries;    // non-empty hash slots, tombstones included
   int32_t tombs;
-  int32_t tag;            // scan tag
-  int32_t error;          // bit 1 hash probe overflow, 2 range, 4 capacity, 8 pool, 16 heap
+  int32_t tag;             // scan tag
+  int32_t error;           // bit 1 hash probe overflow, 2 range, 4 capacity, 8 pool, 16 replay queue
   // per-scan outputs
   int32_t nseg, ncreate_ev, ncre, nevict, ncand;
   int32_t refits, merge_calls, merge_evals, merges, dropped;
-  int32_t fast;  // 1: this scan's creations took the parallel (no-eviction) path
-  int32_t nlru;  // voxels appended to the LRU queue by this scan
+  int32_t fast;            // 1: this scan's creations took the parallel (no-eviction) path
+  int32_t nlru;            // voxels appended to the LRU queue by this scan
   int32_t pad[2];
 };
 
+// one session's map (device pointers)
 struct MapDev {
   HashEntry* table;
   uint32_t hmask;
@@ -64,37 +66,66 @@ struct MapDev {
   MapCounters* ctr;
   int32_t capacity;
   int32_t max_pt, upd_pt;
+  int32_t hbits;      // log2 of the table size
   double plane_thresh, merge_a, merge_d, vs64;
 };
 
-struct ScanBufs {
+// one session of a batch
+struct SessInfo {
   const float* x;
   const float* y;
   const float* z;
-  int32_t n;
-  int32_t is_build;
   const UpdParams* prm;
+  int32_t n, off;     // points, global offset of point 0
+  int32_t is_build;
+  int32_t seg_lo, seg_hi;  // its segments in the batch segment list
+  int32_t cre_lo, cre_hi;  // its creation events in the batch creation list
+  int32_t lru_lo, lru_hi;  // its LRU appends in the batch append list
+  int32_t pad;
+};
+
+// batch workspace, all arrays sized for the batch's total points
+struct BatchBufs {
+  int32_t ntot;
+  int32_t nsess;
   uint32_t* skey_in;   // hash slot per point
-  int32_t* sval_in;    // point index
+  int32_t* sval_in;    // global point index
   uint32_t* skey_out;
   int32_t* sval_out;
-  int32_t* flags;      // scratch flag / value array [n]
-  int32_t* seg_start;  // [n]
-  int32_t* seg_vid;    // [n]
-  int32_t* seg_entry;  // [n]
-  int32_t* cre_t;      // creation events: point index
-  int32_t* cre_seg;    //                   segment
-  int32_t* co_vid;     // resolved creations: vid, seg, t, prev, group lo
+  int32_t* flags;      // scratch flag / value array
+  int32_t* sel;        // compaction output (ascending global indices)
+  int32_t* cnt;        // counts: [0] segments, [1] creation events, [2] LRU appends
+  int32_t* seg_start;  // segment -> first sorted position
+  int32_t* seg_end;
+  int32_t* seg_sess;
+  int32_t* seg_vid;
+  int32_t* seg_entry;
+  int32_t* cre_t;      // creation events: local point index, segment
+  int32_t* cre_seg;
+  int32_t* co_vid;     // resolved creations, per session at its offset
   int32_t* co_seg;
   int32_t* co_t;
   int32_t* co_prev;
   uint64_t* co_group;
-  int32_t* ev_vid;     // evictions in order
+  int32_t* ev_vid;     // evictions in order, per session at its offset
   int32_t* ev_t;
-  uint64_t* cand_in;   // merge candidates (t << 32 | vid)
+  uint64_t* cand_in;   // merge candidates (t << 32 | vid), per session at its offset
   uint64_t* cand_out;
-  int32_t* lru_sel;    // compaction output for the LRU append
+  int32_t* cand_nbr;   // per sorted candidate: 6 eligible neighbour ids at its time
+  int32_t* cand_spec;  // per sorted candidate: 1 if it merges on the pre-replay state
+  int32_t* sort_lo;    // per-session segment bounds for segmented sorts
+  int32_t* sort_hi;
 };
+
+__device__ inline int sess_of(const SessInfo* si, int nsess, int g) {
+  int lo = 0, hi = nsess - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (si[mid].off <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 // ------------------------------------------------------------ helpers -----
 __device__ inline int32_t hash_find_or_insert(HashEntry* table, uint32_t mask, uint64_t key, MapCounters* ctr) {
@@ -200,113 +231,165 @@ __device__ inline void write_hot(const MapDev& m, int32_t vid, uint32_t flags) {
   m.hot[vid] = h;
 }
 
-__global__ void k_begin_scan(MapDev m) {
-  MapCounters* c = m.ctr;
+#define VMP_GRID_LOOP(i, n) for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+
+__global__ void k_begin_scan(const MapDev* maps, int nsess) {
+  if (threadIdx.x != 0 || int(blockIdx.x) >= nsess) return;
+  MapCounters* c = maps[blockIdx.x].ctr;
   c->tag += 1;
   c->fast = 0;
-  c->nseg = c->ncreate_ev = c->ncre = c->nevict = c->ncand = 0;
+  c->nseg = c->ncreate_ev = c->ncre = c->nevict = c->ncand = c->nlru = 0;
   c->refits = c->merge_calls = c->merge_evals = c->merges = c->dropped = 0;
 }
 __global__ void k_fill_i32(int32_t* p, int32_t v, int n) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+  VMP_GRID_LOOP(i, n) p[i] = v;
 }
 __global__ void k_fill_u64(uint64_t* p, uint64_t v, int n) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+  VMP_GRID_LOOP(i, n) p[i] = v;
+}
+
+// per-session [lo, hi) of an ascending list of global indices
+// which: 0 segments, 1 creation events, 2 LRU appends
+__global__ void k_ranges(const MapDev* maps, SessInfo* si, int nsess, const int32_t* list, const int32_t* count,
+                         int which) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsess) return;
+  const int n = *count;
+  auto lb = [&](int v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (list[mid] < v) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  const int lo = lb(si[s].off), hi = lb(si[s].off + si[s].n);
+  MapCounters* c = maps[s].ctr;
+  if (which == 0) {
+    si[s].seg_lo = lo;
+    si[s].seg_hi = hi;
+    c->nseg = hi - lo;
+  } else if (which == 1) {
+    si[s].cre_lo = lo;
+    si[s].cre_hi = hi;
+    c->ncreate_ev = hi - lo;
+  } else {
+    si[s].lru_lo = lo;
+    si[s].lru_hi = hi;
+    c->nlru = hi - lo;
+  }
 }
 
 // ------------------------------------------------------------------ K1 ----
-__global__ void k_ins_keys(MapDev m, ScanBufs b) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= b.n) return;
-  const UpdParams& P = *b.prm;
-  float pw[3];
-  ins_transform(P.m, P.t, b.x[i], b.y[i], b.z[i], pw);
-  int64_t k[3];
-  bool ok = true;
-  for (int r = 0; r < 3; ++r) {
-    k[r] = int64_t(floor(double(pw[r]) / m.vs64));
-    ok &= key_ok(k[r]);
+__global__ void k_ins_keys(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
+  VMP_GRID_LOOP(g, b.ntot) {
+    const int s = sess_of(si, nsess, g);
+    const SessInfo& S = si[s];
+    const MapDev& m = maps[s];
+    const int i = g - S.off;
+    const UpdParams& P = *S.prm;
+    float pw[3];
+    ins_transform(P.m, P.t, S.x[i], S.y[i], S.z[i], pw);
+    int64_t k[3];
+    bool ok = true;
+    for (int r = 0; r < 3; ++r) {
+      k[r] = int64_t(floor(double(pw[r]) / m.vs64));
+      ok &= key_ok(k[r]);
+    }
+    int32_t e = -1;
+    if (ok) e = hash_find_or_insert(m.table, m.hmask, pack_key(k[0], k[1], k[2]), m.ctr);
+    else atomicOr(&m.ctr->error, 2);
+    b.skey_in[g] = uint32_t(e < 0 ? 0 : e);
+    b.sval_in[g] = g;
   }
-  int32_t e = -1;
-  if (ok) e = hash_find_or_insert(m.table, m.hmask, pack_key(k[0], k[1], k[2]), m.ctr);
-  else atomicOr(&m.ctr->error, 2);
-  b.skey_in[i] = uint32_t(e < 0 ? 0 : e);
-  b.sval_in[i] = i;
 }
 
 // ------------------------------------------------------------------ K3 ----
-__global__ void k_seg_heads(ScanBufs b) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= b.n) return;
-  b.flags[j] = (j == 0 || b.skey_out[j] != b.skey_out[j - 1]) ? 1 : 0;
+__global__ void k_seg_heads(const SessInfo* si, int nsess, BatchBufs b) {
+  VMP_GRID_LOOP(j, b.ntot) {
+    const int s = sess_of(si, nsess, j);
+    b.flags[j] = (j == si[s].off || b.skey_out[j] != b.skey_out[j - 1]) ? 1 : 0;
+  }
 }
 
 // ------------------------------------------------------------------ K4 ----
-__global__ void k_seg_info(MapDev m, ScanBufs b) {
-  const int nseg = m.ctr->nseg;
-  const int tag = m.ctr->tag;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += gridDim.x * blockDim.x) {
-    const int start = b.seg_start[s];
+__global__ void k_seg_info(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
+  const int nseg = b.cnt[0];
+  VMP_GRID_LOOP(q, nseg) {
+    const int start = b.sel[q];
+    const int s = sess_of(si, nsess, start);
+    const SessInfo& S = si[s];
+    const int lim = S.off + S.n;
+    const int end = (q + 1 < nseg && b.sel[q + 1] < lim) ? b.sel[q + 1] : lim;
+    const MapDev& m = maps[s];
     const uint32_t e = b.skey_out[start];
-    const int first = b.sval_out[start];
-    b.seg_entry[s] = int32_t(e);
+    const int first = b.sval_out[start] - S.off;
+    b.seg_start[q] = start;
+    b.seg_end[q] = end;
+    b.seg_sess[q] = s;
+    b.seg_entry[q] = int32_t(e);
     const int32_t vid = m.table[e].vid;
-    b.seg_vid[s] = vid;
+    b.seg_vid[q] = vid;
     if (vid >= 0) {
       VoxState& st = m.vs[vid];
-      st.tag = tag;
+      st.tag = m.ctr->tag;
       st.create_t = -1;
       st.evict_t = kNever;
       st.freeze_t = -1;
       st.prev_vid = -1;
       st.first_t = first;
-      st.seg = s;
+      st.seg = q;
       st.merge_pos = kNever;
       st.dirty_t = -2;
+      st.taint_t = kNever;
+    } else {
+      b.flags[S.off + first] = q + 1;  // creation event at its first point (array cleared before)
     }
   }
 }
-// marks the first point of each new key with its segment (+1); the flag array
-// was cleared beforehand
-__global__ void k_mark_new(MapDev m, ScanBufs b) {
-  const int nseg = m.ctr->nseg;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += gridDim.x * blockDim.x) {
-    if (b.seg_vid[s] < 0) b.flags[b.sval_out[b.seg_start[s]]] = s + 1;
-  }
-}
-__global__ void k_cre_split(MapDev m, ScanBufs b) {
-  // lru_sel holds the compacted point indices of creation events
-  const int nc = m.ctr->ncreate_ev;
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nc; r += gridDim.x * blockDim.x) {
-    const int t = b.lru_sel[r];
-    b.cre_t[r] = t;
-    b.cre_seg[r] = b.flags[t] - 1;
+
+__global__ void k_cre_split(const SessInfo* si, int nsess, BatchBufs b) {
+  const int nc = b.cnt[1];
+  VMP_GRID_LOOP(r, nc) {
+    const int g = b.sel[r];
+    const int s = sess_of(si, nsess, g);
+    b.cre_t[r] = g - si[s].off;
+    b.cre_seg[r] = b.flags[g] - 1;
   }
 }
 
 // ------------------------------------------------------------------ K5 ----
-// Parallel creation when no eviction can happen in this scan.
-__global__ void k_create_fast(MapDev m, ScanBufs b) {
-  const int nc = m.ctr->ncreate_ev;
-  if (m.ctr->size + nc > m.capacity) return;
-  const int ftop = m.ctr->free_top, bump = m.ctr->bump;
-  const uint64_t gc = m.ctr->group_counter;
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nc; r += gridDim.x * blockDim.x) {
-    const int32_t vid = r < ftop ? m.freelist[ftop - 1 - r] : bump + (r - ftop);
-    b.co_vid[r] = vid;
-    b.co_seg[r] = b.cre_seg[r];
-    b.co_t[r] = b.cre_t[r];
-    b.co_prev[r] = -1;
-    b.co_group[r] = gc + uint64_t(r);
+// Parallel creation for sessions whose LRU cannot overflow in this scan.
+__global__ void k_create_fast(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
+  const int nc = b.cnt[1];
+  VMP_GRID_LOOP(r, nc) {
+    const int s = sess_of(si, nsess, b.sel[r]);
+    const MapDev& m = maps[s];
+    const MapCounters* c = m.ctr;
+    const int ncs = si[s].cre_hi - si[s].cre_lo;
+    if (c->size + ncs > m.capacity) continue;
+    const int rl = r - si[s].cre_lo;
+    const int ftop = c->free_top, bump = c->bump;
+    const int32_t vid = rl < ftop ? m.freelist[ftop - 1 - rl] : bump + (rl - ftop);
+    const int o = si[s].off + rl;
+    b.co_vid[o] = vid;
+    b.co_seg[o] = b.cre_seg[r];
+    b.co_t[o] = b.cre_t[r];
+    b.co_prev[o] = -1;
+    b.co_group[o] = c->group_counter + uint64_t(rl);
   }
 }
-__global__ void k_create_fast_ctr(MapDev m) {
+__global__ void k_create_fast_ctr(const MapDev* maps, int nsess) {
+  if (threadIdx.x != 0 || int(blockIdx.x) >= nsess) return;
+  const MapDev& m = maps[blockIdx.x];
   MapCounters* c = m.ctr;
   const int nc = c->ncreate_ev;
   if (c->size + nc > m.capacity) return;
   const int ftop = c->free_top;
-  if (nc <= ftop) c->free_top = ftop - nc;
-  else {
+  if (nc <= ftop) {
+    c->free_top = ftop - nc;
+  } else {
     c->bump += nc - ftop;
     c->free_top = 0;
   }
@@ -318,13 +401,20 @@ __global__ void k_create_fast_ctr(MapDev m) {
   c->fast = 1;
 }
 
-// Ordered creation + eviction (one wave).  LRU order is the descending order
-// of last-touch stamps; the ring holds (vid, stamp) in ascending stamp order
-// with stale entries skipped lazily.
-__global__ void __launch_bounds__(64) k_create_evict(MapDev m, ScanBufs b) {
+// Ordered creation + eviction, one wave per session that may overflow.  LRU
+// order is the descending order of last-touch stamps; the ring holds
+// (vid, stamp) in ascending stamp order with stale entries skipped lazily.
+// The victim of each creation is the oldest live voxel not yet touched by an
+// earlier point of this scan; a victim touched later comes back as a new
+// voxel at its first touch (a creation event pushed on a min-heap).
+__global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
+  const int s = blockIdx.x;
+  if (s >= nsess) return;
+  const MapDev& m = maps[s];
   MapCounters* c = m.ctr;
-  const int ncre_ev = c->ncreate_ev;
   if (c->fast) return;
+  const int clo = si[s].cre_lo, ncre_ev = si[s].cre_hi - clo;
+  const int off = si[s].off;
   __shared__ uint64_t heap[kHeapCap];
   const int lane = threadIdx.x;
   const int tag = c->tag;
@@ -334,20 +424,20 @@ __global__ void __launch_bounds__(64) k_create_evict(MapDev m, ScanBufs b) {
   int ftop = c->free_top, bump = c->bump;
   int64_t ptr = c->ring_head;
   const int64_t tail = c->ring_tail;
-  int64_t base = -1;       // first ring index held by the lanes
+  int64_t base = -1;  // first ring index held by the lanes
   int32_t l_vid = -1;
   int32_t l_first = kNever;  // first touch this scan (kNever if untouched)
   bool l_live = false;
   int ci = 0, nco = 0, nev = 0;
   int err = 0;
   while (true) {
-    const int t_s = ci < ncre_ev ? b.cre_t[ci] : kNever;
+    const int t_s = ci < ncre_ev ? b.cre_t[clo + ci] : kNever;
     const int t_h = hn > 0 ? int(heap[0] >> 32) : kNever;
     if (t_s == kNever && t_h == kNever) break;
     int t, seg, prev;
     if (t_s < t_h) {
       t = t_s;
-      seg = b.cre_seg[ci];
+      seg = b.cre_seg[clo + ci];
       prev = -1;
       ++ci;
     } else {
@@ -356,20 +446,21 @@ __global__ void __launch_bounds__(64) k_create_evict(MapDev m, ScanBufs b) {
       prev = int32_t(top & 0xffffffffu);  // evicted incarnation of this key
       seg = m.vs[prev].seg;
       __syncthreads();
-      if (lane == 0) {  // pop
-        const uint64_t last = heap[hn - 1];
-        int i = 0;
+      if (lane == 0) {
+        const uint64_t lastv = heap[hn - 1];
         const int nn = hn - 1;
+        int i = 0;
         while (true) {
-          int l = 2 * i + 1, r = l + 1, s = i;
-          uint64_t sv = last;
-          if (l < nn && heap[l] < sv) { s = l; sv = heap[l]; }
-          if (r < nn && heap[r] < sv) { s = r; sv = heap[r]; }
-          if (s == i) break;
-          heap[i] = heap[s];
-          i = s;
+          const int l = 2 * i + 1, r = l + 1;
+          int sm = i;
+          uint64_t sv = lastv;
+          if (l < nn && heap[l] < sv) { sm = l; sv = heap[l]; }
+          if (r < nn && heap[r] < sv) { sm = r; sv = heap[r]; }
+          if (sm == i) break;
+          heap[i] = heap[sm];
+          i = sm;
         }
-        if (nn > 0) heap[i] = last;
+        if (nn > 0) heap[i] = lastv;
       }
       __syncthreads();
       --hn;
@@ -377,13 +468,16 @@ __global__ void __launch_bounds__(64) k_create_evict(MapDev m, ScanBufs b) {
     int32_t vid;
     if (ftop > 0) vid = m.freelist[--ftop];
     else vid = bump++;
-    if (vid >= m.pool) { err |= 8; break; }
+    if (vid >= m.pool) {
+      err |= 8;
+      break;
+    }
     if (lane == 0) {
-      b.co_vid[nco] = vid;
-      b.co_seg[nco] = seg;
-      b.co_t[nco] = t;
-      b.co_prev[nco] = prev;
-      b.co_group[nco] = gc;
+      b.co_vid[off + nco] = vid;
+      b.co_seg[off + nco] = seg;
+      b.co_t[off + nco] = t;
+      b.co_prev[off + nco] = prev;
+      b.co_group[off + nco] = gc;
     }
     ++gc;
     ++nco;
@@ -391,7 +485,10 @@ __global__ void __launch_bounds__(64) k_create_evict(MapDev m, ScanBufs b) {
     if (size > m.capacity) {
       int32_t victim = -1;
       while (victim < 0) {
-        if (ptr >= tail) { err |= 4; break; }
+        if (ptr >= tail) {
+          err |= 4;
+          break;
+        }
         if (base < 0 || ptr >= base + 64) {
           base = ptr;
           const int64_t idx = base + lane;
@@ -433,21 +530,25 @@ __global__ void __launch_bounds__(64) k_create_evict(MapDev m, ScanBufs b) {
           vst.seg = -1;
           vst.merge_pos = kNever;
           vst.dirty_t = -2;
+          vst.taint_t = kNever;
         }
         vst.evict_t = t;
-        b.ev_vid[nev] = victim;
-        b.ev_t[nev] = t;
+        b.ev_vid[off + nev] = victim;
+        b.ev_t[off + nev] = t;
       }
       ++nev;
       --size;
       if (vfirst != kNever) {  // the key comes back later in this scan
-        if (hn >= kHeapCap) { err |= 16; break; }
+        if (hn >= kHeapCap) {
+          err |= 16;
+          break;
+        }
         __syncthreads();
         if (lane == 0) {
           int i = hn;
           const uint64_t v = (uint64_t(uint32_t(vfirst)) << 32) | uint32_t(victim);
           while (i > 0) {
-            int p = (i - 1) >> 1;
+            const int p = (i - 1) >> 1;
             if (heap[p] <= v) break;
             heap[i] = heap[p];
             i = p;
@@ -473,30 +574,33 @@ __global__ void __launch_bounds__(64) k_create_evict(MapDev m, ScanBufs b) {
 }
 
 // Initialise created voxels and point their hash slots at them.
-__global__ void k_create_init(MapDev m, ScanBufs b) {
-  const int nco = m.ctr->ncre;
-  const int tag = m.ctr->tag;
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nco; r += gridDim.x * blockDim.x) {
-    const int32_t vid = b.co_vid[r];
-    const int s = b.co_seg[r];
-    if (vid < 0 || vid >= m.pool || s < 0) continue;  // error already flagged
-    const int32_t e = b.seg_entry[s];
+__global__ void k_create_init(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
+  VMP_GRID_LOOP(p, b.ntot) {
+    const int s = sess_of(si, nsess, p);
+    const MapDev& m = maps[s];
+    const int r = p - si[s].off;
+    if (r >= m.ctr->ncre) continue;
+    const int32_t vid = b.co_vid[p];
+    const int q = b.co_seg[p];
+    if (vid < 0 || vid >= m.pool || q < 0) continue;  // error already flagged
+    const int32_t e = b.seg_entry[q];
     VoxState st;
     st.key = m.table[e].key;
-    st.group_id = b.co_group[r];
+    st.group_id = b.co_group[p];
     st.stamp = 0;
     st.n = 0;
     st.newly_add = 0;
     st.flags = kEnable | kAlive;
-    st.tag = tag;
-    st.create_t = b.co_t[r];
+    st.tag = m.ctr->tag;
+    st.create_t = b.co_t[p];
     st.evict_t = kNever;
     st.freeze_t = -1;
-    st.prev_vid = b.co_prev[r];
-    st.first_t = b.co_t[r];
-    st.seg = s;
+    st.prev_vid = b.co_prev[p];
+    st.first_t = b.co_t[p];
+    st.seg = q;
     st.merge_pos = kNever;
     st.dirty_t = -2;
+    st.taint_t = kNever;
     m.vs[vid] = st;
     double* pl = m.plane + size_t(vid) * kPlaneD;
     for (int k = 0; k < kPlaneD; ++k) pl[k] = 0.0;
@@ -506,7 +610,7 @@ __global__ void k_create_init(MapDev m, ScanBufs b) {
     h.flags = kEnable | kAlive;
     m.hot[vid] = h;
     m.table[e].vid = vid;
-    b.seg_vid[s] = vid;
+    b.seg_vid[q] = vid;
   }
 }
 
@@ -617,15 +721,17 @@ __device__ inline void refit(const MapDev& m, Moments& M, int n, uint32_t& flags
   for (int a = 0; a < 3; ++a) pl[3 + a] = sg * nn[a];
 }
 
-__global__ void __launch_bounds__(kMapThreads) k_replay(MapDev m, ScanBufs b) {
-  MapCounters* c = m.ctr;
-  const int nseg = c->nseg;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kMapThreads) k_replay(const MapDev* maps, const SessInfo* si, BatchBufs b) {
+  const int nseg = b.cnt[0];
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
   int refits = 0, calls = 0, dropped = 0;
-  const int32_t vid = s < nseg ? b.seg_vid[s] : -1;
+  const int32_t vid = q < nseg ? b.seg_vid[q] : -1;
+  const int s = q < nseg ? b.seg_sess[q] : 0;
+  const MapDev& m = maps[s];
   if (vid >= 0) {
-    const int start = b.seg_start[s];
-    const int end = s + 1 < nseg ? b.seg_start[s + 1] : b.n;
+    const SessInfo& S = si[s];
+    const int start = b.seg_start[q];
+    const int end = b.seg_end[q];
     VoxState st = m.vs[vid];
     double* pl = m.plane + size_t(vid) * kPlaneD;
     double plv[kPlaneD];
@@ -635,13 +741,13 @@ __global__ void __launch_bounds__(kMapThreads) k_replay(MapDev m, ScanBufs b) {
     int64_t kx, ky, kz;
     unpack_key(st.key, kx, ky, kz);
     const double center[3] = {(double(kx) + 0.5) * m.vs64, (double(ky) + 0.5) * m.vs64, (double(kz) + 0.5) * m.vs64};
-    const UpdParams& P = *b.prm;
+    const UpdParams& P = *S.prm;
     uint32_t flags = st.flags;
     int n = st.n, newly = st.newly_add;
     int merge_pos = kNever, freeze_t = -1;
     bool moved = false;
     for (int pos = start; pos < end; ++pos) {
-      if ((flags & kInit) && !(flags & kEnable) && !b.is_build) {
+      if ((flags & kInit) && !(flags & kEnable) && !S.is_build) {
         if (flags & kPlane) {
           merge_pos = pos - start;
           calls += end - pos;
@@ -650,19 +756,19 @@ __global__ void __launch_bounds__(kMapThreads) k_replay(MapDev m, ScanBufs b) {
         }
         break;
       }
-      const int i = b.sval_out[pos];
-      const float px = b.x[i], py = b.y[i], pz0 = b.z[i];
+      const int i = b.sval_out[pos] - S.off;
+      const float px = S.x[i], py = S.y[i], pz0 = S.z[i];
       float pw[3];
       ins_transform(P.m, P.t, px, py, pz0, pw);
-      const float q[3] = {float(double(pw[0]) - center[0]), float(double(pw[1]) - center[1]),
-                          float(double(pw[2]) - center[2])};
+      const float qv[3] = {float(double(pw[0]) - center[0]), float(double(pw[1]) - center[1]),
+                           float(double(pw[2]) - center[2])};
       const float pz = pz0 == 0.0f ? 0.001f : pz0;
       float C[6];
       point_world_cov(P.rwl32, P.Prr, P.Ppp, P.s2, P.rv, px, py, pz, C);
-      mom_add(M, q, C);
+      mom_add(M, qv, C);
       ++n;
       moved = true;
-      if (b.is_build) continue;
+      if (S.is_build) continue;
       if (!(flags & kInit)) {
         refit(m, M, n, flags, plv, center, refits);
         continue;
@@ -677,7 +783,7 @@ __global__ void __launch_bounds__(kMapThreads) k_replay(MapDev m, ScanBufs b) {
         freeze_t = i;
       }
     }
-    if (b.is_build) refit(m, M, n, flags, plv, center, refits);
+    if (S.is_build) refit(m, M, n, flags, plv, center, refits);
     if (moved) {
       const double invn = 1.0 / double(n);
       for (int a = 0; a < 3; ++a) plv[a] = center[a] + M.s1[a] * invn;
@@ -692,324 +798,46 @@ __global__ void __launch_bounds__(kMapThreads) k_replay(MapDev m, ScanBufs b) {
     m.vs[vid] = st;
     write_hot(m, vid, flags);
   }
-  // block-level counter reduction
-  refits = wave_sum_i(refits);
-  calls = wave_sum_i(calls);
-  dropped = wave_sum_i(dropped);
-  if ((threadIdx.x & 63) == 0) {
-    if (refits) atomicAdd(&c->refits, refits);
-    if (calls) atomicAdd(&c->merge_calls, calls);
-    if (dropped) atomicAdd(&c->dropped, dropped);
-  }
+  if (refits) atomicAdd(&m.ctr->refits, refits);
+  if (calls) atomicAdd(&m.ctr->merge_calls, calls);
+  if (dropped) atomicAdd(&m.ctr->dropped, dropped);
 }
 
-// ------------------------------------------------------------------ K7 ----
-// incarnation of a key at point time t (-1: none)
-__device__ inline int32_t vid_at(const MapDev& m, int32_t vid, int tag, int t) {
-  if (vid < 0) return -1;
-  const VoxState& st = m.vs[vid];
-  if (st.tag == tag && st.create_t > t) vid = st.prev_vid;
-  if (vid < 0) return -1;
-  const VoxState& s2 = m.vs[vid];
-  if (s2.tag == tag && s2.evict_t <= t) return -1;
-  return vid;
-}
+}  // namespace vmp
 
-// first merge() call of segment s strictly after point t (kNever if none)
-__device__ inline int first_call_after(const ScanBufs& b, int nseg, int s, int merge_pos, int t) {
-  const int start = b.seg_start[s];
-  const int end = s + 1 < nseg ? b.seg_start[s + 1] : b.n;
-  int lo = start + merge_pos, hi = end;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (b.sval_out[mid] > t) hi = mid;
-    else lo = mid + 1;
-  }
-  return lo < end ? b.sval_out[lo] : kNever;
-}
+#include "vmp_merge.hpp"
 
-__device__ inline int32_t lookup_vid(const MapDev& m, uint64_t key) {
-  const int32_t e = hash_find_slot(m.table, m.hmask, key);
-  return e < 0 ? -1 : m.table[e].vid;
-}
-
-__global__ void k_candidates(MapDev m, ScanBufs b) {
-  MapCounters* c = m.ctr;
-  const int nseg = c->nseg, tag = c->tag;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += gridDim.x * blockDim.x) {
-    const int32_t vid = b.seg_vid[s];
-    if (vid < 0) continue;
-    VoxState& st = m.vs[vid];
-    int dirty = (st.flags & kDirty) ? -1 : -2;
-    const bool own_freeze = st.freeze_t >= 0 && (st.flags & kPlane);
-    if (own_freeze && st.freeze_t > dirty) dirty = st.freeze_t;
-    const bool has_calls = st.merge_pos != kNever;
-    int emitted[7];
-    int ne = 0;
-    auto emit = [&](int after) {
-      const int t = first_call_after(b, nseg, s, st.merge_pos, after);
-      if (t == kNever) return;
-      for (int q = 0; q < ne; ++q)
-        if (emitted[q] == t) return;  // one candidate per distinct call
-      emitted[ne++] = t;
-      const int slot = atomicAdd(&c->ncand, 1);
-      if (slot < b.n) b.cand_in[slot] = (uint64_t(uint32_t(t)) << 32) | uint32_t(vid);
-      else atomicOr(&c->error, 16);
-    };
-    if (has_calls && ((st.flags & kDirty) || own_freeze)) emit(-1);
-    for (int j = 0; j < 6; ++j) {
-      if (!key_neighbour_ok(st.key, j)) continue;
-      const int32_t nv = lookup_vid(m, key_neighbour(st.key, j));
-      if (nv < 0) continue;
-      const VoxState& ns = m.vs[nv];
-      if (ns.tag != tag || ns.freeze_t < 0 || !(ns.flags & kPlane)) continue;
-      if (ns.freeze_t > dirty) dirty = ns.freeze_t;
-      if (has_calls) emit(ns.freeze_t);
-    }
-    st.dirty_t = dirty;
-  }
-}
-
-// voxels not touched in this scan whose neighbour froze as a plane
-__global__ void k_dirty_untouched(MapDev m, ScanBufs b) {
-  MapCounters* c = m.ctr;
-  const int nseg = c->nseg, tag = c->tag;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += gridDim.x * blockDim.x) {
-    if (b.seg_vid[s] < 0) continue;
-    const VoxState& st = m.vs[b.seg_vid[s]];
-    if (st.freeze_t < 0 || !(st.flags & kPlane)) continue;
-    for (int j = 0; j < 6; ++j) {
-      if (!key_neighbour_ok(st.key, j)) continue;
-      const int32_t nv = lookup_vid(m, key_neighbour(st.key, j));
-      if (nv < 0) continue;
-      VoxState& ns = m.vs[nv];
-      if (ns.tag != tag && (ns.flags & kAlive)) atomicOr(&ns.flags, kDirty);
-    }
-  }
-}
-
-// ------------------------------------------------------------------ K8 ----
-struct NbrRec {
-  int32_t vid;
-  int32_t ok;  // eligible: exists, frozen and plane at time t
-  uint64_t group;
-};
-
-__global__ void __launch_bounds__(64) k_merge_replay(MapDev m, ScanBufs b) {
-  MapCounters* c = m.ctr;
-  const int ncand = c->ncand, nseg = c->nseg, tag = c->tag;
-  __shared__ uint64_t heap[kHeapCap];
-  __shared__ NbrRec nb[6];
-  __shared__ int32_t clos[64];
-  __shared__ int nclos;
-  __shared__ int merged_any;
-  const int lane = threadIdx.x;
-  int hn = 0, ci = 0, evals = 0, merges = 0;
-  uint64_t last = ~0ull;
-  int err = 0;
-  while (true) {
-    const uint64_t vs_ = ci < ncand ? b.cand_out[ci] : ~0ull;
-    const uint64_t vh = hn > 0 ? heap[0] : ~0ull;
-    if (vs_ == ~0ull && vh == ~0ull) break;
-    uint64_t cur;
-    if (vs_ <= vh) {
-      cur = vs_;
-      ++ci;
-    } else {
-      cur = vh;
-      __syncthreads();
-      if (lane == 0) {
-        const uint64_t lastv = heap[hn - 1];
-        const int nn = hn - 1;
-        int i = 0;
-        while (true) {
-          int l = 2 * i + 1, r = l + 1, s = i;
-          uint64_t sv = lastv;
-          if (l < nn && heap[l] < sv) { s = l; sv = heap[l]; }
-          if (r < nn && heap[r] < sv) { s = r; sv = heap[r]; }
-          if (s == i) break;
-          heap[i] = heap[s];
-          i = s;
-        }
-        if (nn > 0) heap[i] = lastv;
-      }
-      __syncthreads();
-      --hn;
-    }
-    if (cur == last) continue;
-    last = cur;
-    const int t = int(cur >> 32);
-    const int32_t V = int32_t(cur & 0xffffffffu);
-    ++evals;
-    const uint64_t vkey = m.vs[V].key;
-    // ---- neighbours at time t (lanes 0..5)
-    if (lane < 6) {
-      NbrRec r{-1, 0, 0};
-      if (key_neighbour_ok(vkey, lane)) {
-        const int32_t nv = vid_at(m, lookup_vid(m, key_neighbour(vkey, lane)), tag, t);
-        if (nv >= 0) {
-          const VoxState& ns = m.vs[nv];
-          const bool frozen = !(ns.flags & kEnable) && !(ns.tag == tag && ns.freeze_t > t);
-          r.vid = nv;
-          r.ok = frozen && (ns.flags & kPlane) && (ns.flags & kInit);
-          r.group = ns.group_id;
-        }
-      }
-      nb[lane] = r;
-    }
-    if (lane == 0) {
-      merged_any = 0;
-      nclos = 0;
-    }
-    __syncthreads();
-    // ---- merge(): sequential over the six neighbours (lane 0)
-    if (lane == 0) {
-      double* pv = m.plane + size_t(V) * kPlaneD;
-      const uint64_t gV = m.vs[V].group_id;
-      for (int j = 0; j < 6; ++j) {
-        const NbrRec r = nb[j];
-        if (r.vid < 0 || !r.ok) continue;
-        if (m.vs[r.vid].group_id == gV) continue;
-        double* pn = m.plane + size_t(r.vid) * kPlaneD;
-        double dn = pn[3] * pv[3];
-        dn += pn[4] * pv[4];
-        dn += pn[5] * pv[5];
-        const double norm_distance = 1.0 - dn;
-        double a1 = pn[3] * pn[0];
-        a1 += pn[4] * pn[1];
-        a1 += pn[5] * pn[2];
-        double a0 = pv[3] * pv[0];
-        a0 += pv[4] * pv[1];
-        a0 += pv[5] * pv[2];
-        const double axis_distance = fabs(a1 - a0);
-        if (norm_distance > m.merge_a || axis_distance > m.merge_d) continue;
-        const double tn0 = pv[6], tm0 = pv[7], tn1 = pn[6], tm1 = pn[7];
-        const double tc0 = tn0 + tm0, tc1 = tn1 + tm1;
-        double nm[3], nrm[3];
-        for (int k = 0; k < 3; ++k) {
-          nm[k] = tm0 * pn[k] + (tm1 * pv[k]) / (tm0 + tm1);        // precedence as in the reference
-          nrm[k] = tn0 * pn[3 + k] + (tn1 * pv[3 + k]) / (tn0 + tn1);
-        }
-        const double den = (tc0 + tc1) * (tc0 + tc1);
-        const double ntn = (tc0 * tc0 * tn1 + tc1 * tc1 * tn0) / den;
-        const double ntm = (tc0 * tc0 * tm1 + tc1 * tc1 * tm0) / den;
-        m.vs[r.vid].group_id = gV;
-        m.vs[r.vid].flags |= kMerged;
-        m.vs[V].flags |= kMerged;
-        double dd = nm[0] * nrm[0];
-        dd += nm[1] * nrm[1];
-        dd += nm[2] * nrm[2];
-        if (-dd < 0.0)
-          for (int k = 0; k < 3; ++k) nrm[k] = -nrm[k];
-        for (int k = 0; k < 3; ++k) {
-          pv[k] = nm[k];
-          pv[3 + k] = nrm[k];
-          pn[k] = nm[k];
-          pn[3 + k] = nrm[k];
-        }
-        pv[6] = ntn;
-        pv[7] = ntm;
-        pn[6] = ntn;
-        pn[7] = ntm;
-        write_hot(m, r.vid, m.vs[r.vid].flags);
-        ++merges;
-        clos[nclos++] = r.vid;
-        merged_any = 1;
-      }
-      if (merged_any) {
-        write_hot(m, V, m.vs[V].flags);
-        clos[nclos++] = V;
-      }
-    }
-    __syncthreads();
-    if (!merged_any) continue;
-    // ---- follow-up calls: V, merged neighbours and their six neighbours
-    const int nc0 = nclos;
-    __syncthreads();
-    for (int q0 = 0; q0 < nc0 * 7; q0 += 64) {
-      const int q = q0 + lane;
-      int32_t X = -1;
-      if (q < nc0 * 7) {
-        const int32_t base_v = clos[q / 7];
-        const int j = q % 7;
-        X = base_v;
-        if (j < 6) {
-          const uint64_t bk = m.vs[base_v].key;
-          X = key_neighbour_ok(bk, j) ? vid_at(m, lookup_vid(m, key_neighbour(bk, j)), tag, t) : -1;
-        }
-      }
-      int tn = kNever;
-      if (X >= 0) {
-        VoxState& xs = m.vs[X];
-        if (xs.tag == tag && xs.seg >= 0 && xs.evict_t == kNever) {
-          if (xs.dirty_t < t) xs.dirty_t = t;  // benign race: all writers store t
-          if (xs.merge_pos != kNever) tn = first_call_after(b, nseg, xs.seg, xs.merge_pos, t);
-        } else if (xs.flags & kAlive) {
-          atomicOr(&xs.flags, kDirty);
-        }
-      }
-      // serialise heap pushes through lane 0
-      for (int w = 0; w < 64; ++w) {
-        const int tw = __shfl(tn, w, 64);
-        const int32_t xw = __shfl(X, w, 64);
-        if (tw == kNever) continue;
-        if (lane == 0) {
-          if (hn >= kHeapCap) {
-            err |= 16;
-          } else {
-            const uint64_t v = (uint64_t(uint32_t(tw)) << 32) | uint32_t(xw);
-            int i = hn;
-            while (i > 0) {
-              int p = (i - 1) >> 1;
-              if (heap[p] <= v) break;
-              heap[i] = heap[p];
-              i = p;
-            }
-            heap[i] = v;
-          }
-        }
-        hn = __shfl(hn + ((lane == 0 && hn < kHeapCap) ? 1 : 0), 0, 64);
-      }
-    }
-    __syncthreads();
-    if (err) break;
-  }
-  if (lane == 0) {
-    c->merge_evals += evals;
-    c->merges += merges;
-    if (err) atomicOr(&c->error, err);
-  }
-}
+namespace vmp {
 
 // ------------------------------------------------------------------ K9 ----
-__global__ void k_finalize_segs(MapDev m, ScanBufs b) {
-  MapCounters* c = m.ctr;
-  const int nseg = c->nseg;
-  const int64_t base = c->point_base;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += gridDim.x * blockDim.x) {
-    const int start = b.seg_start[s];
-    const int end = s + 1 < nseg ? b.seg_start[s + 1] : b.n;
-    const int32_t vid = b.seg_vid[s];
+__global__ void k_finalize_segs(const MapDev* maps, const SessInfo* si, BatchBufs b) {
+  const int nseg = b.cnt[0];
+  VMP_GRID_LOOP(q, nseg) {
+    const int32_t vid = b.seg_vid[q];
     if (vid < 0) continue;
+    const int s = b.seg_sess[q];
+    const MapDev& m = maps[s];
     VoxState& st = m.vs[vid];
-    const int last_t = b.sval_out[end - 1];
+    const int last_t = b.sval_out[b.seg_end[q] - 1] - si[s].off;
     const bool calls = st.merge_pos != kNever;
     const bool frozen_plane = (st.flags & kPlane) && !(st.flags & kEnable) && (st.flags & kInit);
     // a dirtying event is resolved by a later (evaluated) merge() call
     const bool dirty = frozen_plane && st.dirty_t > -2 && !(calls && last_t > st.dirty_t);
     if (dirty) st.flags |= kDirty;
     else st.flags &= ~kDirty;
-    st.stamp = uint64_t(base + last_t);
-    b.flags[last_t] = vid + 1;  // LRU append marker (array cleared before)
+    st.stamp = uint64_t(m.ctr->point_base + last_t);
+    b.flags[si[s].off + last_t] = vid + 1;  // LRU append marker (array cleared before)
   }
 }
 
-__global__ void k_evict_finalize(MapDev m, ScanBufs b) {
-  MapCounters* c = m.ctr;
-  const int nev = c->nevict;
-  const int ftop = c->free_top;
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nev; r += gridDim.x * blockDim.x) {
-    const int32_t v = b.ev_vid[r];
+__global__ void k_evict_finalize(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
+  VMP_GRID_LOOP(p, b.ntot) {
+    const int s = sess_of(si, nsess, p);
+    const MapDev& m = maps[s];
+    const MapCounters* c = m.ctr;
+    const int r = p - si[s].off;
+    if (r >= c->nevict) continue;
+    const int32_t v = b.ev_vid[p];
     VoxState& st = m.vs[v];
     const int32_t e = hash_find_slot(m.table, m.hmask, st.key);
     if (e >= 0 && m.table[e].vid == v) {
@@ -1018,39 +846,51 @@ __global__ void k_evict_finalize(MapDev m, ScanBufs b) {
     }
     st.flags = 0;
     m.hot[v].flags = 0;
-    m.freelist[ftop + r] = v;
+    m.freelist[c->free_top + r] = v;
   }
 }
 
-__global__ void k_evict_finalize_ctr(MapDev m) {
-  MapCounters* c = m.ctr;
+__global__ void k_evict_finalize_ctr(const MapDev* maps, int nsess) {
+  if (threadIdx.x != 0 || int(blockIdx.x) >= nsess) return;
+  MapCounters* c = maps[blockIdx.x].ctr;
   c->free_top += c->nevict;
   c->tombs += c->nevict;
 }
 
-__global__ void k_lru_append(MapDev m, ScanBufs b) {
-  MapCounters* c = m.ctr;
-  const int nsel = c->nlru;  // every touched voxel appears exactly once
-  const int64_t tail = c->ring_tail;
-  const int64_t base = c->point_base;
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nsel; r += gridDim.x * blockDim.x) {
-    const int t = b.lru_sel[r];
-    const int32_t vid = b.flags[t] - 1;
-    const int64_t ph = (tail + r) % m.qcap;
+__global__ void k_lru_append(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
+  const int nsel = b.cnt[2];
+  VMP_GRID_LOOP(r, nsel) {
+    const int g = b.sel[r];
+    const int s = sess_of(si, nsess, g);
+    const MapDev& m = maps[s];
+    const int t = g - si[s].off;
+    const int32_t vid = b.flags[g] - 1;
+    const int64_t ph = (m.ctr->ring_tail + (r - si[s].lru_lo)) % m.qcap;
     m.ring_vid[ph] = vid;
-    m.ring_stamp[ph] = uint64_t(base + t);
+    m.ring_stamp[ph] = uint64_t(m.ctr->point_base + t);
   }
 }
 
-__global__ void k_end_scan(MapDev m, ScanBufs b) {
-  MapCounters* c = m.ctr;
+__global__ void k_end_scan(const MapDev* maps, const SessInfo* si, int nsess) {
+  if (threadIdx.x != 0 || int(blockIdx.x) >= nsess) return;
+  MapCounters* c = maps[blockIdx.x].ctr;
   c->ring_tail += c->nlru;
-  c->point_base += b.n;
+  c->point_base += si[blockIdx.x].n;
 }
 
+// per-session bounds for the segmented sorts: [off, off + n) for points,
+// [off, off + ncand) for merge candidates
+__global__ void k_sort_bounds(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b, int cand) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsess) return;
+  b.sort_lo[s] = si[s].off;
+  b.sort_hi[s] = si[s].off + (cand ? maps[s].ctr->ncand : si[s].n);
+}
+
+// ---------------------------------------------------- housekeeping -----
 // hash rebuild: every live voxel is re-inserted into a cleared table
 __global__ void k_rehash_collect(MapDev m, int32_t* live, int32_t* nlive) {
-  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.pool; v += gridDim.x * blockDim.x) {
+  VMP_GRID_LOOP(v, m.pool) {
     if (m.vs[v].flags & kAlive) live[atomicAdd(nlive, 1)] = v;
   }
 }
@@ -1062,19 +902,10 @@ __global__ void k_rehash_clear(MapDev m) {
 }
 __global__ void k_rehash_insert(MapDev m, const int32_t* live, const int32_t* nlive) {
   const int n = *nlive;
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+  VMP_GRID_LOOP(r, n) {
     const int32_t v = live[r];
     const int32_t e = hash_find_or_insert(m.table, m.hmask, m.vs[v].key, m.ctr);
     if (e >= 0) m.table[e].vid = v;
-  }
-}
-// ring compaction: keep live entries in order (src -> dst), one pass
-__global__ void k_ring_flags(MapDev m, int32_t* keep, int64_t head, int64_t count) {
-  for (int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; r < count; r += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t ph = (head + r) % m.qcap;
-    const int32_t v = m.ring_vid[ph];
-    const VoxState& st = m.vs[v];
-    keep[r] = ((st.flags & kAlive) && st.stamp == m.ring_stamp[ph]) ? 1 : 0;
   }
 }
 

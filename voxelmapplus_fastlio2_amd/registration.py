@@ -240,6 +240,27 @@ class Registrar:
         self._check(self.lib.vmp_session_timing(self.h, C.byref(a), C.byref(b), C.byref(n)), "vmp_session_timing")
         return a.value, b.value, n.value
 
+    def set_timing(self, on=True):
+        self._check(self.lib.vmp_session_set_timing(self.h, 1 if on else 0), "vmp_session_set_timing")
+
+    def timing_totals(self):
+        om, mm = C.c_double(), C.c_double()
+        on, mn, op, oh = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(
+            self.lib.vmp_session_timing_totals(
+                self.h, C.byref(om), C.byref(on), C.byref(mm), C.byref(mn), C.byref(op), C.byref(oh)
+            ),
+            "vmp_session_timing_totals",
+        )
+        return {
+            "obs_ms": om.value,
+            "obs_launches": on.value,
+            "map_ms": mm.value,
+            "map_calls": mn.value,
+            "obs_points": op.value,
+            "obs_hits": oh.value,
+        }
+
     # ---- inspection
     def map_size(self):
         return int(self.lib.vmp_map_size(self.h))
