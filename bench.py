@@ -64,6 +64,83 @@ def cpu_baseline(scans, motions, cfg, s0, P0, n_warm, n_timed):
     }
 
 
+def run_batched(ctx, B, W, K, capacity, nmax=120000):
+    """B independent sequences (room / street alternating), per-scan sizes
+    uniform in [2k, nmax], advanced together by vmp_register_batch."""
+    import torch
+
+    from voxelmapplus_fastlio2_amd import LIOConfig, Registrar, initial_covariance, register_batch, state_vector
+    from voxelmapplus_fastlio2_amd.synth import QDIAG, ROOM, SEED0, STREET, Sequence
+
+    rng = np.random.default_rng(0xBA7C4)
+    total = W + K + 1
+    sizes = rng.integers(2000, nmax + 1, size=(total, B))
+    seqs = [Sequence(ROOM if i % 2 == 0 else STREET, SEED0 + 1000 + i, nmax) for i in range(B)]
+    t0 = time.perf_counter()
+    steps = []
+    for k in range(total):
+        offs = np.concatenate([[0], np.cumsum(sizes[k])])
+        buf = torch.empty((3, int(offs[-1])), dtype=torch.float32)
+        for i in range(B):
+            x, y, z = seqs[i].scan(k, sizes[k, i])
+            buf[0, offs[i] : offs[i + 1]] = torch.from_numpy(x)
+            buf[1, offs[i] : offs[i + 1]] = torch.from_numpy(y)
+            buf[2, offs[i] : offs[i + 1]] = torch.from_numpy(z)
+        steps.append((buf.to("cuda:0"), offs))
+    motions = [[None] * B] + [[seqs[i].motion(k) for i in range(B)] for k in range(1, total)]
+    log(f"batched: generated {total} steps x {B} sessions ({int(sizes.sum())} points) in {time.perf_counter() - t0:.1f} s")
+    cfg = LIOConfig(max_scan_points=nmax, map_capacity=capacity)
+    regs = [Registrar(ctx, cfg) for _ in range(B)]
+
+    def bind(k):
+        dev, offs = steps[k]
+        base = dev.data_ptr()
+        stride = dev.stride(0) * 4
+        for i, g in enumerate(regs):
+            o = int(offs[i]) * 4
+            g.set_points_device(base + o, base + stride + o, base + 2 * stride + o, int(sizes[k, i]))
+
+    bind(0)
+    for i, g in enumerate(regs):
+        R, t = seqs[i].pose(0)
+        g.set_state(state_vector(R, t), initial_covariance())
+        g.build()
+    for k in range(1, W + 1):
+        for i, g in enumerate(regs):
+            g.apply_motion(*motions[k][i], QDIAG)
+        bind(k)
+        register_batch(regs)
+    torch.cuda.synchronize()
+    lat = []
+    pts = 0
+    t_start = time.perf_counter()
+    for k in range(W + 1, W + K + 1):
+        ts = time.perf_counter()
+        for i, g in enumerate(regs):
+            g.apply_motion(*motions[k][i], QDIAG)
+        bind(k)
+        register_batch(regs)
+        lat.append(time.perf_counter() - ts)
+        pts += int(sizes[k].sum())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    for g in regs:
+        g.close()
+    lat_ms = np.array(lat) * 1e3
+    return {
+        "value": pts / elapsed,
+        "unit": "points/s",
+        "sessions": B,
+        "steps": K,
+        "warmup": W,
+        "points_per_step_mean": pts / K,
+        "ms_per_step": elapsed / K * 1e3,
+        "step_latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "map_capacity_per_session": capacity,
+        "workload": f"{B} sequences room/street alternating, per-scan points U[2000,{nmax}]",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +151,10 @@ def main():
     ap.add_argument("--cpu-warm", type=int, default=3)
     ap.add_argument("--cpu-timed", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-sessions", type=int, default=128)
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--batch-warmup", type=int, default=3)
+    ap.add_argument("--batch-capacity", type=int, default=200000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -195,6 +276,11 @@ def main():
         },
     }
     g.close()
+    del dev
+    if args.batch_sessions > 0:
+        bt = run_batched(ctx, args.batch_sessions, args.batch_warmup, args.batch_steps, args.batch_capacity)
+        bt["speedup_vs_plain"] = bt["value"] / value
+        line["batched"] = bt
     ctx.close()
     if not args.no_cpu:
         log("timing the FP64 oracle on the host")

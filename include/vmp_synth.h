@@ -46,6 +46,10 @@ int vmp_synth_pose(const vmp_synth* s, int32_t k, double R[9], double t[3]);
  * per-point time offset in ms (strictly ascending).  Any output may be NULL. */
 int vmp_synth_scan(const vmp_synth* s, int32_t k, float* x, float* y, float* z,
                    float* t_ms);
+/* Same scan with n points instead of pts_per_scan: the first n returns of
+ * the same ray sequence (a partial sweep). */
+int vmp_synth_scan_n(const vmp_synth* s, int32_t k, int32_t n, float* x, float* y,
+                     float* z, float* t_ms);
 
 /* Noisy odometry increment between scans k-1 and k, expressed in the LiDAR
  * frame of scan k-1: pose_k ~= pose_{k-1} * [dR | dp].  Seeded noise of

@@ -142,6 +142,7 @@ def synth():
     _proto(lib, "vmp_synth_destroy", None, [P])
     _proto(lib, "vmp_synth_pose", C.c_int, [P, C.c_int32, dp, dp])
     _proto(lib, "vmp_synth_scan", C.c_int, [P, C.c_int32, fp, fp, fp, fp])
+    _proto(lib, "vmp_synth_scan_n", C.c_int, [P, C.c_int32, C.c_int32, fp, fp, fp, fp])
     _proto(lib, "vmp_synth_motion", C.c_int, [P, C.c_int32, dp, dp])
     _synth = lib
     return lib

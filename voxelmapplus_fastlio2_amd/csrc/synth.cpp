@@ -301,12 +301,12 @@ extern "C" int vmp_synth_pose(const vmp_synth* s, int32_t k, double R[9], double
   return 0;
 }
 
-extern "C" int vmp_synth_scan(const vmp_synth* s, int32_t k, float* x, float* y, float* z, float* t_ms) {
-  if (!s || k < 0 || size_t(k) + 1 >= s->xy_table.size()) return -1;
+extern "C" int vmp_synth_scan_n(const vmp_synth* s, int32_t k, int32_t n, float* x, float* y, float* z,
+                                float* t_ms) {
+  if (!s || k < 0 || n <= 0 || size_t(k) + 1 >= s->xy_table.size()) return -1;
   M3 R;
   V3 o;
   s->pose_local(k, R, o);
-  const int n = s->n;
   const double a1 = 0.7548776662466927, a2 = 0.5698402909980532;
   const double b1 = 0.1370 * k, b2 = 0.4210 * k;
   const double el_lo = -7.0 * kPi / 180.0, el_span = 59.0 * kPi / 180.0;
@@ -354,6 +354,11 @@ extern "C" int vmp_synth_scan(const vmp_synth* s, int32_t k, float* x, float* y,
     if (ray0 > 64LL * n + 100000) return -2;  // scene produced too few returns
   }
   return 0;
+}
+
+extern "C" int vmp_synth_scan(const vmp_synth* s, int32_t k, float* x, float* y, float* z, float* t_ms) {
+  if (!s) return -1;
+  return vmp_synth_scan_n(s, k, s->n, x, y, z, t_ms);
 }
 
 extern "C" int vmp_synth_motion(const vmp_synth* s, int32_t k, double dR[9], double dp[3]) {

@@ -212,8 +212,15 @@ static int ws_reserve(vmp_ctx* c, int64_t points, int nsess) {
   ok &= dalloc(A, &b.cand_out, n) == hipSuccess;
   ok &= dalloc(A, &b.cand_nbr, n * 6) == hipSuccess;
   ok &= dalloc(A, &b.cand_spec, n) == hipSuccess;
-  ok &= dalloc(A, &b.sort_lo, size_t(ns)) == hipSuccess;
-  ok &= dalloc(A, &b.sort_hi, size_t(ns)) == hipSuccess;
+  ok &= dalloc(A, &b.cand_start, size_t(ns)) == hipSuccess;
+  ok &= dalloc(A, &b.spl, n) == hipSuccess;
+  ok &= dalloc(A, &b.seg_ct, n * 7) == hipSuccess;
+  ok &= dalloc(A, &b.seg_nct, n) == hipSuccess;
+  ok &= dalloc(A, &b.seg_nbr, n * 6) == hipSuccess;
+  ok &= dalloc(A, &b.len_in, n) == hipSuccess;
+  ok &= dalloc(A, &b.len_out, n) == hipSuccess;
+  ok &= dalloc(A, &b.ord_in, n) == hipSuccess;
+  ok &= dalloc(A, &b.ord, n) == hipSuccess;
   ok &= dalloc(A, &w.d_maps, size_t(ns)) == hipSuccess;
   ok &= dalloc(A, &w.d_si, size_t(ns)) == hipSuccess;
   ok &= dalloc(A, &w.d_jobs, size_t(ns)) == hipSuccess;
@@ -240,15 +247,12 @@ static int ws_reserve(vmp_ctx* c, int64_t points, int nsess) {
     return VMP_ENOMEM;
   }
   // rocprim scratch: the largest of the sorts and compactions at this size
-  size_t t[6] = {0, 0, 0, 0, 0, 0};
-  rocprim::radix_sort_pairs(nullptr, t[0], b.skey_in, b.skey_out, b.sval_in, b.sval_out, n, 0, 32, c->stream);
-  rocprim::segmented_radix_sort_pairs(nullptr, t[1], b.skey_in, b.skey_out, b.sval_in, b.sval_out, n, (unsigned)ns,
-                                      b.sort_lo, b.sort_hi, 0, 32, c->stream);
-  rocprim::radix_sort_keys(nullptr, t[2], b.cand_in, b.cand_out, n, 0, 64, c->stream);
-  rocprim::segmented_radix_sort_keys(nullptr, t[3], b.cand_in, b.cand_out, n, (unsigned)ns, b.sort_lo, b.sort_hi, 0,
-                                     64, c->stream);
-  rocprim::select(nullptr, t[4], rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt, n, c->stream);
-  w.temp_bytes = *std::max_element(t, t + 6) + 256;
+  size_t t[4] = {0, 0, 0, 0};
+  rocprim::radix_sort_pairs(nullptr, t[0], b.skey_in, b.skey_out, b.sval_in, b.sval_out, n, 0, 64, c->stream);
+  rocprim::radix_sort_keys(nullptr, t[1], b.cand_in, b.cand_out, n, 0, 64, c->stream);
+  rocprim::select(nullptr, t[2], rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt, n, c->stream);
+  rocprim::radix_sort_pairs(nullptr, t[3], b.len_in, b.len_out, b.ord_in, b.ord, n, 0, 32, c->stream);
+  w.temp_bytes = *std::max_element(t, t + 4) + 256;
   if (dalloc(A, reinterpret_cast<uint8_t**>(&w.temp), w.temp_bytes) != hipSuccess) {
     w.release();
     return VMP_ENOMEM;
@@ -265,7 +269,7 @@ static Sizes sizes_for(const vmp_map_cfg& c) {
   Sizes z;
   z.maxn = c.max_scan_points;
   z.pool = int64_t(c.map_capacity) + c.max_scan_points + 64;
-  z.hcap = next_pow2(uint64_t(4) * (uint64_t(c.map_capacity) + uint64_t(c.max_scan_points)));
+  z.hcap = next_pow2(uint64_t(2) * (uint64_t(c.map_capacity) + uint64_t(c.max_scan_points)));
   z.qcap = 2 * (int64_t(c.map_capacity) + 2 * int64_t(c.max_scan_points)) + 1024;
   z.max_blocks = (c.max_scan_points + kObsThreads - 1) / kObsThreads + 1;
   return z;
@@ -580,21 +584,18 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
     const int gn = grid_for(ntot, T);
     int hbits = 1;
     for (int i = 0; i < n; ++i) hbits = std::max(hbits, int(ss[i]->map.hbits));
+    int sbits = 0;
+    while ((1 << sbits) < n) ++sbits;
+    b.hbits = hbits;
     hipLaunchKernelGGL(k_begin_scan, dim3(n), dim3(1), 0, st, M, n);
     if (ntot > 0) {
       size_t tb = w.temp_bytes;
       hipLaunchKernelGGL(k_ins_keys, dim3(gn), dim3(T), 0, st, M, S, n, b);
-      if (n == 1) {
-        if (rocprim::radix_sort_pairs(w.temp, tb, b.skey_in, b.skey_out, b.sval_in, b.sval_out, size_t(ntot), 0,
-                                      hbits, st) != hipSuccess)
-          return VMP_EDEVICE;
-      } else {
-        hipLaunchKernelGGL(k_sort_bounds, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b, 0);
-        if (rocprim::segmented_radix_sort_pairs(w.temp, tb, b.skey_in, b.skey_out, b.sval_in, b.sval_out,
-                                                size_t(ntot), unsigned(n), b.sort_lo, b.sort_hi, 0, hbits,
-                                                st) != hipSuccess)
-          return VMP_EDEVICE;
-      }
+      // one stable radix sort over (session, hash slot): every key's points
+      // become a contiguous segment in point order, sessions stay in order
+      if (rocprim::radix_sort_pairs(w.temp, tb, b.skey_in, b.skey_out, b.sval_in, b.sval_out, size_t(ntot), 0,
+                                    hbits + sbits, st) != hipSuccess)
+        return VMP_EDEVICE;
       hipLaunchKernelGGL(k_seg_heads, dim3(gn), dim3(T), 0, st, S, n, b);
       tb = w.temp_bytes;
       if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt + 0, size_t(ntot),
@@ -613,22 +614,29 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
       hipLaunchKernelGGL(k_create_fast_ctr, dim3(n), dim3(1), 0, st, M, n);
       hipLaunchKernelGGL(k_create_evict, dim3(n), dim3(64), 0, st, M, S, n, b);
       hipLaunchKernelGGL(k_create_init, dim3(gn), dim3(T), 0, st, M, S, n, b);
+      hipLaunchKernelGGL(k_seg_lenkeys, dim3(gn), dim3(T), 0, st, b);
+      tb = w.temp_bytes;
+      if (rocprim::radix_sort_pairs(w.temp, tb, b.len_in, b.len_out, b.ord_in, b.ord, size_t(ntot), 0, 32, st) !=
+          hipSuccess)
+        return VMP_EDEVICE;
       hipLaunchKernelGGL(k_replay, dim3(gn), dim3(T), 0, st, M, S, b);
       if (!(mode & kStepBuild)) {
         hipLaunchKernelGGL(k_fill_u64, dim3(gn), dim3(T), 0, st, b.cand_in, ~0ull, ntot);
         hipLaunchKernelGGL(k_candidates, dim3(gn), dim3(T), 0, st, M, S, b);
         hipLaunchKernelGGL(k_dirty_untouched, dim3(gn), dim3(T), 0, st, M, b);
         tb = w.temp_bytes;
-        if (n == 1) {
-          if (rocprim::radix_sort_keys(w.temp, tb, b.cand_in, b.cand_out, size_t(ntot), 0, 64, st) != hipSuccess)
-            return VMP_EDEVICE;
-        } else {
-          hipLaunchKernelGGL(k_sort_bounds, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b, 1);
-          if (rocprim::segmented_radix_sort_keys(w.temp, tb, b.cand_in, b.cand_out, size_t(ntot), unsigned(n),
-                                                 b.sort_lo, b.sort_hi, 0, 64, st) != hipSuccess)
-            return VMP_EDEVICE;
-        }
-        hipLaunchKernelGGL(k_spec_eval, dim3(gn), dim3(T), 0, st, M, S, n, b);
+        // unused slots hold ~0 and sort last
+        if (rocprim::radix_sort_keys(w.temp, tb, b.cand_in, b.cand_out, size_t(ntot), 0, 49 + sbits, st) !=
+            hipSuccess)
+          return VMP_EDEVICE;
+        hipLaunchKernelGGL(k_cand_starts, dim3(1), dim3(1), 0, st, M, n, b);
+        hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.cand_spec, 0, ntot);
+        hipLaunchKernelGGL(k_spec_eval, dim3(gn), dim3(T), 0, st, M, b);
+        tb = w.temp_bytes;
+        if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.cand_spec, b.spl, b.cnt + 4,
+                            size_t(ntot), st) != hipSuccess)
+          return VMP_EDEVICE;
+        hipLaunchKernelGGL(k_spl_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b);
         hipLaunchKernelGGL(k_merge_replay, dim3(n), dim3(64), 0, st, M, S, n, b);
       }
       hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.flags, 0, ntot);

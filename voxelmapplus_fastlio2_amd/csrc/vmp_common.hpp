@@ -41,13 +41,18 @@ __host__ __device__ inline bool key_neighbour_ok(uint64_t k, int j) {
   const uint64_t c = (k >> sh) & kKeyMask;
   return j < 3 ? c > 0 : c < kKeyMask;
 }
+// Home slot of a key: the 2x2x2 brick of the voxel is hashed and its eight
+// voxels take eight consecutive slots (one 128-byte line of 16-byte entries),
+// so spatially coherent lookups of a scan share cache lines.
 __host__ __device__ inline uint32_t hash_slot(uint64_t k, uint32_t mask) {
-  k ^= k >> 33;
-  k *= 0xff51afd7ed558ccdull;
-  k ^= k >> 33;
-  k *= 0xc4ceb9fe1a85ec53ull;
-  k ^= k >> 33;
-  return uint32_t(k) & mask;
+  const uint64_t sub = (k & 1ull) | ((k >> 20) & 2ull) | ((k >> 40) & 4ull);
+  uint64_t h = (k >> 1) & ~((1ull << 20) | (1ull << 41));  // drop the low bit of every axis
+  h ^= h >> 33;
+  h *= 0xff51afd7ed558ccdull;
+  h ^= h >> 33;
+  h *= 0xc4ceb9fe1a85ec53ull;
+  h ^= h >> 33;
+  return uint32_t((h << 3) | sub) & mask;
 }
 
 struct __align__(16) HashEntry {
@@ -101,8 +106,9 @@ struct __align__(16) VoxState {
 // mean xyz, normal xyz, trace of the normal block and of the centre block of
 // Plane::cov (only these traces are ever consumed, voxel_map.cpp:161-164).
 constexpr int kPlaneD = 8;
-// Refit moments, centre-local: S1q(3) S2q(6) S0C(6) S1C(18) S2C(36).
-constexpr int kMomD = 69;
+// Refit moments, centre-local: S1q(3) S2q(6) S0C(6) S1C(18) S2C(36), padded
+// to 70 doubles so every record is 16-byte aligned.
+constexpr int kMomD = 70;
 
 // ------------------------------------------------------------- params -----
 // Per-pass constants derived from the filter state in FP64.

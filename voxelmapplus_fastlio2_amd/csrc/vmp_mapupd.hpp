@@ -81,6 +81,7 @@ struct SessInfo {
   int32_t seg_lo, seg_hi;  // its segments in the batch segment list
   int32_t cre_lo, cre_hi;  // its creation events in the batch creation list
   int32_t lru_lo, lru_hi;  // its LRU appends in the batch append list
+  int32_t spl_lo, spl_hi;  // its acting merge candidates in the compacted list
   int32_t pad;
 };
 
@@ -88,9 +89,11 @@ struct SessInfo {
 struct BatchBufs {
   int32_t ntot;
   int32_t nsess;
-  uint32_t* skey_in;   // hash slot per point
+  int32_t hbits;       // bits of the largest hash-slot index in the batch
+  int32_t pad0;
+  uint64_t* skey_in;   // (session << hbits) | hash slot, per point
   int32_t* sval_in;    // global point index
-  uint32_t* skey_out;
+  uint64_t* skey_out;
   int32_t* sval_out;
   int32_t* flags;      // scratch flag / value array
   int32_t* sel;        // compaction output (ascending global indices)
@@ -109,13 +112,28 @@ struct BatchBufs {
   uint64_t* co_group;
   int32_t* ev_vid;     // evictions in order, per session at its offset
   int32_t* ev_t;
-  uint64_t* cand_in;   // merge candidates (t << 32 | vid), per session at its offset
-  uint64_t* cand_out;
+  uint64_t* cand_in;   // merge candidates, see cand_key(); per session at its offset
+  uint64_t* cand_out;  // sorted: session-major, then point time, then voxel
   int32_t* cand_nbr;   // per sorted candidate: 6 eligible neighbour ids at its time
   int32_t* cand_spec;  // per sorted candidate: 1 if it merges on the pre-replay state
-  int32_t* sort_lo;    // per-session segment bounds for segmented sorts
-  int32_t* sort_hi;
+  int32_t* cand_start; // per session: first sorted candidate
+  int32_t* spl;        // positions of candidates with cand_spec = 1, ascending
+  int32_t* seg_ct;     // per segment: its candidate times (up to 7, ascending)
+  int32_t* seg_nct;
+  int32_t* seg_nbr;    // per segment: the six neighbour voxels (current incarnation, -1 none)
+  uint32_t* len_in;    // replay order: segments by descending length
+  uint32_t* len_out;
+  int32_t* ord_in;
+  int32_t* ord;
 };
+
+// merge-candidate key: session (15 bits) | point time (24 bits) | voxel (25 bits)
+__host__ __device__ inline uint64_t cand_key(int s, int t, int32_t vid) {
+  return (uint64_t(uint32_t(s)) << 49) | (uint64_t(uint32_t(t)) << 25) | uint64_t(uint32_t(vid));
+}
+__host__ __device__ inline int cand_t(uint64_t k) { return int((k >> 25) & 0xFFFFFFull); }
+__host__ __device__ inline int32_t cand_vid(uint64_t k) { return int32_t(k & 0x1FFFFFFull); }
+constexpr uint64_t kCandLocal = (uint64_t(1) << 49) - 1;  // key without the session bits
 
 __device__ inline int sess_of(const SessInfo* si, int nsess, int g) {
   int lo = 0, hi = nsess - 1;
@@ -300,7 +318,7 @@ __global__ void k_ins_keys(const MapDev* maps, const SessInfo* si, int nsess, Ba
     int32_t e = -1;
     if (ok) e = hash_find_or_insert(m.table, m.hmask, pack_key(k[0], k[1], k[2]), m.ctr);
     else atomicOr(&m.ctr->error, 2);
-    b.skey_in[g] = uint32_t(e < 0 ? 0 : e);
+    b.skey_in[g] = (uint64_t(s) << b.hbits) | uint64_t(uint32_t(e < 0 ? 0 : e));
     b.sval_in[g] = g;
   }
 }
@@ -323,7 +341,7 @@ __global__ void k_seg_info(const MapDev* maps, const SessInfo* si, int nsess, Ba
     const int lim = S.off + S.n;
     const int end = (q + 1 < nseg && b.sel[q + 1] < lim) ? b.sel[q + 1] : lim;
     const MapDev& m = maps[s];
-    const uint32_t e = b.skey_out[start];
+    const uint32_t e = uint32_t(b.skey_out[start] & ((uint64_t(1) << b.hbits) - 1));
     const int first = b.sval_out[start] - S.off;
     b.seg_start[q] = start;
     b.seg_end[q] = end;
@@ -677,21 +695,28 @@ __device__ inline void refit(const MapDev& m, Moments& M, int n, uint32_t& flags
     return;
   }
   flags |= kPlane;
-  // T[ab][lm] = sum_i d_a d_b C_lm
-  double T[6][6];
-  for (int a = 0; a < 3; ++a)
-    for (int bb = a; bb < 3; ++bb)
-      for (int l = 0; l < 6; ++l)
-        T[s6(a, bb)][l] = M.c2[s6(a, bb)][l] - mu[a] * M.c1[bb][l] - mu[bb] * M.c1[a][l] + mu[a] * mu[bb] * M.c0[l];
+  // With d_i = q_i - mu and T[ab][lm] = sum_i d_a d_b C_lm
+  //   = S2C[ab][lm] - mu_a S1C[b][lm] - mu_b S1C[a][lm] + mu_a mu_b S0C[lm],
+  // G(w,x)[ab] = sum_lm T[ab][lm] w_l x_m is formed from the moments
+  // contracted with the symmetric weights of w x^T (no T is materialised).
   const double nn[3] = {V[0], V[3], V[6]};
-  // G(w,x)[ab] = sum_lm T[ab][lm] w_l x_m ; Q = u^T G v
   auto G = [&](const double* w, const double* x, double g[6]) {
-    for (int ab = 0; ab < 6; ++ab) {
-      double s = 0.0;
-      for (int l = 0; l < 3; ++l)
-        for (int mm = 0; mm < 3; ++mm) s += T[ab][s6(l, mm)] * w[l] * x[mm];
-      g[ab] = s;
+    double W6[6];
+    for (int l = 0; l < 3; ++l)
+      for (int mm = l; mm < 3; ++mm) W6[s6(l, mm)] = l == mm ? w[l] * x[l] : w[l] * x[mm] + w[mm] * x[l];
+    double c1w[3], c0w = 0.0;
+    for (int l = 0; l < 6; ++l) c0w += M.c0[l] * W6[l];
+    for (int a = 0; a < 3; ++a) {
+      double sacc = 0.0;
+      for (int l = 0; l < 6; ++l) sacc += M.c1[a][l] * W6[l];
+      c1w[a] = sacc;
     }
+    for (int a = 0; a < 3; ++a)
+      for (int bb = a; bb < 3; ++bb) {
+        double sacc = 0.0;
+        for (int l = 0; l < 6; ++l) sacc += M.c2[s6(a, bb)][l] * W6[l];
+        g[s6(a, bb)] = sacc - mu[a] * c1w[bb] - mu[bb] * c1w[a] + mu[a] * mu[bb] * c0w;
+      }
   };
   auto Q = [&](const double* g, const double* u, const double* v) {
     double s = 0.0;
@@ -721,14 +746,36 @@ __device__ inline void refit(const MapDev& m, Moments& M, int n, uint32_t& flags
   for (int a = 0; a < 3; ++a) pl[3 + a] = sg * nn[a];
 }
 
+// sort keys that put long segments first (waves of the replay then walk
+// segments of similar length); slots past the segment count sort last
+__global__ void k_seg_lenkeys(BatchBufs b) {
+  const int nseg = b.cnt[0];
+  VMP_GRID_LOOP(q, b.ntot) {
+    b.len_in[q] = q < nseg ? 0xFFFFFFFFu - uint32_t(b.seg_end[q] - b.seg_start[q]) : 0xFFFFFFFFu;
+    b.ord_in[q] = q;
+  }
+}
+
 __global__ void __launch_bounds__(kMapThreads) k_replay(const MapDev* maps, const SessInfo* si, BatchBufs b) {
   const int nseg = b.cnt[0];
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = qi < nseg ? b.ord[qi] : nseg;
   int refits = 0, calls = 0, dropped = 0;
   const int32_t vid = q < nseg ? b.seg_vid[q] : -1;
   const int s = q < nseg ? b.seg_sess[q] : 0;
   const MapDev& m = maps[s];
-  if (vid >= 0) {
+  const bool frozen0 = vid >= 0 && !si[s].is_build && (m.vs[vid].flags & kInit) && !(m.vs[vid].flags & kEnable);
+  if (frozen0) {
+    // frozen before this scan: every point is a merge() call (plane) or dropped
+    const int len = b.seg_end[q] - b.seg_start[q];
+    VoxState& st = m.vs[vid];
+    if (st.flags & kPlane) {
+      st.merge_pos = 0;
+      calls += len;
+    } else {
+      dropped += len;
+    }
+  } else if (vid >= 0) {
     const SessInfo& S = si[s];
     const int start = b.seg_start[q];
     const int end = b.seg_end[q];
@@ -878,13 +925,16 @@ __global__ void k_end_scan(const MapDev* maps, const SessInfo* si, int nsess) {
   c->point_base += si[blockIdx.x].n;
 }
 
-// per-session bounds for the segmented sorts: [off, off + n) for points,
-// [off, off + ncand) for merge candidates
-__global__ void k_sort_bounds(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b, int cand) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsess) return;
-  b.sort_lo[s] = si[s].off;
-  b.sort_hi[s] = si[s].off + (cand ? maps[s].ctr->ncand : si[s].n);
+// start of each session's candidates in the sorted candidate list, and
+// their total (cnt[3])
+__global__ void k_cand_starts(const MapDev* maps, int nsess, BatchBufs b) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int acc = 0;
+  for (int s = 0; s < nsess; ++s) {
+    b.cand_start[s] = acc;
+    acc += maps[s].ctr->ncand;
+  }
+  b.cnt[3] = acc;
 }
 
 // ---------------------------------------------------- housekeeping -----

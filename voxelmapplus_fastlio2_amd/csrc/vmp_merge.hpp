@@ -9,18 +9,20 @@
 // per-voxel replay) and (b) merges themselves.  Evictions only remove
 // neighbours and new incarnations start unfrozen, so neither can turn an idle
 // call into an acting one.
-//   k_candidates   first call after each (a)-event of the voxel or its
-//                  neighbours, plus the first call of voxels still dirty from
-//                  earlier scans
+//   k_candidates   candidates = the first call after each (a)-event of the
+//                  voxel or its neighbours, plus the first call of voxels
+//                  still dirty from earlier scans; also caches each touched
+//                  voxel's six neighbours and its candidate times
 //   k_spec_eval    every candidate evaluated in parallel on the state before
-//                  any merge of this scan: the eligible neighbours at its time
-//                  (existence, incarnation, frozen-ness and plane-ness do not
-//                  depend on merges) and whether it would merge
-//   k_merge_replay one wave per session walks candidates in point order; a
-//                  candidate is evaluated for real only if it would merge or
-//                  a merge earlier in the scan touched its closed
-//                  neighbourhood ("tainted"); each real merge adds the next
-//                  call of every voxel whose neighbourhood it changed.
+//                  any merge of this scan: its eligible neighbours at its
+//                  time (existence, incarnation, frozen-ness and plane-ness
+//                  do not depend on merges) and whether it would merge
+//   k_merge_replay one wave per session walks, in point order, the
+//                  candidates that would merge plus the calls a real merge
+//                  makes necessary: for every voxel whose closed
+//                  neighbourhood a merge changed ("tainted"), its next call
+//                  and all its later candidates, whose speculative result
+//                  assumed the unchanged neighbourhood.
 #pragma once
 
 namespace vmp {
@@ -55,13 +57,16 @@ __device__ inline int32_t lookup_vid(const MapDev& m, uint64_t key) {
 
 // neighbour j of a voxel if merge() may consider it at time t: it exists,
 // is frozen (update_enable false), initialised and a plane; else -1
-__device__ inline int32_t nbr_at(const MapDev& m, uint64_t vkey, int j, int tag, int t) {
-  if (!key_neighbour_ok(vkey, j)) return -1;
-  const int32_t nv = vid_at(m, lookup_vid(m, key_neighbour(vkey, j)), tag, t);
+__device__ inline int32_t eligible_at(const MapDev& m, int32_t cur, int tag, int t) {
+  const int32_t nv = vid_at(m, cur, tag, t);
   if (nv < 0) return -1;
   const VoxState& ns = m.vs[nv];
   const bool frozen = !(ns.flags & kEnable) && !(ns.tag == tag && ns.freeze_t > t);
   return (frozen && (ns.flags & kPlane) && (ns.flags & kInit)) ? nv : -1;
+}
+__device__ inline int32_t nbr_at(const MapDev& m, uint64_t vkey, int j, int tag, int t) {
+  if (!key_neighbour_ok(vkey, j)) return -1;
+  return eligible_at(m, lookup_vid(m, key_neighbour(vkey, j)), tag, t);
 }
 
 // angle and distance gates (voxel_map.cpp:147-155); planes as kPlaneD records
@@ -114,7 +119,11 @@ __global__ void k_candidates(const MapDev* maps, const SessInfo* si, BatchBufs b
   const int nseg = b.cnt[0];
   VMP_GRID_LOOP(q, nseg) {
     const int32_t vid = b.seg_vid[q];
-    if (vid < 0) continue;
+    b.seg_nct[q] = 0;
+    if (vid < 0) {
+      for (int j = 0; j < 6; ++j) b.seg_nbr[q * 6 + j] = -1;
+      continue;
+    }
     const int s = b.seg_sess[q];
     const MapDev& m = maps[s];
     MapCounters* c = m.ctr;
@@ -133,13 +142,14 @@ __global__ void k_candidates(const MapDev* maps, const SessInfo* si, BatchBufs b
         if (emitted[k] == t) return;  // one candidate per distinct call
       emitted[ne++] = t;
       const int slot = atomicAdd(&c->ncand, 1);
-      if (slot < ns_pts) b.cand_in[off + slot] = (uint64_t(uint32_t(t)) << 32) | uint32_t(vid);
+      if (slot < ns_pts) b.cand_in[off + slot] = cand_key(s, t, vid);
       else atomicOr(&c->error, 16);
     };
     if (has_calls && ((st.flags & kDirty) || own_freeze)) emit(-1);
     for (int j = 0; j < 6; ++j) {
-      if (!key_neighbour_ok(st.key, j)) continue;
-      const int32_t nv = lookup_vid(m, key_neighbour(st.key, j));
+      int32_t nv = -1;
+      if (key_neighbour_ok(st.key, j)) nv = lookup_vid(m, key_neighbour(st.key, j));
+      b.seg_nbr[q * 6 + j] = nv;
       if (nv < 0) continue;
       const VoxState& nbs = m.vs[nv];
       if (nbs.tag != tag || nbs.freeze_t < 0 || !(nbs.flags & kPlane)) continue;
@@ -147,6 +157,18 @@ __global__ void k_candidates(const MapDev* maps, const SessInfo* si, BatchBufs b
       if (has_calls) emit(nbs.freeze_t);
     }
     st.dirty_t = dirty;
+    // candidate times, ascending (insertion sort of <= 7)
+    for (int i = 1; i < ne; ++i) {
+      const int v = emitted[i];
+      int k = i - 1;
+      while (k >= 0 && emitted[k] > v) {
+        emitted[k + 1] = emitted[k];
+        --k;
+      }
+      emitted[k + 1] = v;
+    }
+    for (int k = 0; k < ne; ++k) b.seg_ct[q * 7 + k] = emitted[k];
+    b.seg_nct[q] = ne;
   }
 }
 
@@ -160,8 +182,7 @@ __global__ void k_dirty_untouched(const MapDev* maps, BatchBufs b) {
     const VoxState& st = m.vs[b.seg_vid[q]];
     if (st.freeze_t < 0 || !(st.flags & kPlane)) continue;
     for (int j = 0; j < 6; ++j) {
-      if (!key_neighbour_ok(st.key, j)) continue;
-      const int32_t nv = lookup_vid(m, key_neighbour(st.key, j));
+      const int32_t nv = b.seg_nbr[q * 6 + j];
       if (nv < 0) continue;
       VoxState& nbs = m.vs[nv];
       if (nbs.tag != tag && (nbs.flags & kAlive)) atomicOr(&nbs.flags, kDirty);
@@ -170,22 +191,23 @@ __global__ void k_dirty_untouched(const MapDev* maps, BatchBufs b) {
 }
 
 // ------------------------------------------------------------------ K8 ----
-__global__ void k_spec_eval(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
-  VMP_GRID_LOOP(p, b.ntot) {
-    const int s = sess_of(si, nsess, p);
-    const MapDev& m = maps[s];
-    if (p - si[s].off >= m.ctr->ncand) continue;
-    const int tag = m.ctr->tag;
+__global__ void k_spec_eval(const MapDev* maps, BatchBufs b) {
+  const int ntot = b.cnt[3];
+  VMP_GRID_LOOP(p, ntot) {
     const uint64_t cur = b.cand_out[p];
-    const int t = int(cur >> 32);
-    const int32_t V = int32_t(cur & 0xffffffffu);
-    const uint64_t vkey = m.vs[V].key;
-    const uint64_t gV = m.vs[V].group_id;
+    const int s = int(cur >> 49);
+    const MapDev& m = maps[s];
+    const int tag = m.ctr->tag;
+    const int t = cand_t(cur);
+    const int32_t V = cand_vid(cur);
+    const VoxState& vst = m.vs[V];
+    const uint64_t gV = vst.group_id;
     double pv[kPlaneD];
     for (int k = 0; k < kPlaneD; ++k) pv[k] = m.plane[size_t(V) * kPlaneD + k];
+    const bool cached = vst.tag == tag && vst.seg >= 0;
     int spec = 0;
     for (int j = 0; j < 6; ++j) {
-      const int32_t nv = nbr_at(m, vkey, j, tag, t);
+      const int32_t nv = cached ? eligible_at(m, b.seg_nbr[vst.seg * 6 + j], tag, t) : nbr_at(m, vst.key, j, tag, t);
       b.cand_nbr[size_t(p) * 6 + j] = nv;
       if (nv >= 0 && !spec && m.vs[nv].group_id != gV &&
           merge_gate(pv, m.plane + size_t(nv) * kPlaneD, m.merge_a, m.merge_d))
@@ -193,6 +215,25 @@ __global__ void k_spec_eval(const MapDev* maps, const SessInfo* si, int nsess, B
     }
     b.cand_spec[p] = spec;
   }
+}
+
+// per-session range of the compacted acting candidates (cnt[4] of them)
+__global__ void k_spl_ranges(const MapDev* maps, SessInfo* si, int nsess, BatchBufs b) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsess) return;
+  const int n = b.cnt[4];
+  auto lb = [&](int v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (b.spl[mid] < v) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  const int c0 = b.cand_start[s];
+  si[s].spl_lo = lb(c0);
+  si[s].spl_hi = lb(c0 + maps[s].ctr->ncand);
 }
 
 __device__ inline void heap_push(uint64_t* heap, int hn, uint64_t v) {
@@ -228,52 +269,33 @@ __device__ inline void heap_pop(uint64_t* heap, int hn) {
   if (nn > 0) heap[i] = lastv;
 }
 
+constexpr int kStageCap = 512;
+
 __global__ void __launch_bounds__(64) k_merge_replay(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
   const int s = blockIdx.x;
   if (s >= nsess) return;
   const MapDev& m = maps[s];
   MapCounters* c = m.ctr;
-  const int ncand = c->ncand, tag = c->tag, off = si[s].off;
-  const uint64_t* cand = b.cand_out + off;
-  const int32_t* cspec = b.cand_spec + off;
-  const int32_t* cnbr = b.cand_nbr + size_t(off) * 6;
+  const int tag = c->tag, off = si[s].off;
+  const int spl_hi = si[s].spl_hi;
   __shared__ uint64_t heap[kHeapCap];
+  __shared__ uint64_t stage[kStageCap];
+  __shared__ int nstage;
   __shared__ double pl[7][kPlaneD];
   __shared__ uint64_t grp[7];
   __shared__ int32_t nbv[7];
   __shared__ int32_t clos[8];
+  __shared__ uint64_t ckey[8];
   __shared__ int nclos;
   const int lane = threadIdx.x;
-  int hn = 0, ci = 0, evals = 0, merges = 0, err = 0;
-  int64_t cbase = -1;  // static candidates [cbase, cbase + 64) held by the lanes
-  uint64_t l_key = ~0ull;
-  bool l_need = false;
+  int hn = 0, sp = si[s].spl_lo, evals = 0, merges = 0, err = 0;
   uint64_t last = ~0ull;
   while (true) {
-    // ---- next static candidate that needs a real evaluation
+    int pos = -1;
     uint64_t st_key = ~0ull;
-    int st_idx = -1;
-    while (ci < ncand) {
-      if (cbase < 0 || ci >= cbase + 64) {
-        cbase = ci;
-        const int idx = int(cbase) + lane;
-        l_key = ~0ull;
-        l_need = false;
-        if (idx < ncand) {
-          l_key = cand[idx];
-          const int t = int(l_key >> 32);
-          const VoxState& vst = m.vs[int32_t(l_key & 0xffffffffu)];
-          l_need = cspec[idx] || vst.taint_t < t;
-        }
-      }
-      const unsigned long long bal = __ballot(l_need && (cbase + lane >= ci));
-      if (bal) {
-        const int w = __ffsll(bal) - 1;
-        st_key = __shfl(l_key, w, 64);
-        st_idx = int(cbase) + w;
-        break;
-      }
-      ci = int(cbase) + 64;
+    if (sp < spl_hi) {
+      pos = b.spl[sp];
+      st_key = b.cand_out[pos] & kCandLocal;
     }
     const uint64_t h_key = hn > 0 ? heap[0] : ~0ull;
     if (st_key == ~0ull && h_key == ~0ull) break;
@@ -282,7 +304,7 @@ __global__ void __launch_bounds__(64) k_merge_replay(const MapDev* maps, const S
     if (st_key <= h_key) {
       cur = st_key;
       from_static = true;
-      ci = st_idx + 1;
+      ++sp;
     } else {
       cur = h_key;
       from_static = false;
@@ -293,12 +315,19 @@ __global__ void __launch_bounds__(64) k_merge_replay(const MapDev* maps, const S
     }
     if (cur == last) continue;
     last = cur;
-    const int t = int(cur >> 32);
-    const int32_t V = int32_t(cur & 0xffffffffu);
+    const int t = cand_t(cur);
+    const int32_t V = cand_vid(cur);
     ++evals;
     // ---- stage the six neighbours and the voxel itself in LDS
     if (lane < 6) {
-      const int32_t nv = from_static ? cnbr[size_t(st_idx) * 6 + lane] : nbr_at(m, m.vs[V].key, lane, tag, t);
+      int32_t nv;
+      if (from_static) {
+        nv = b.cand_nbr[size_t(pos) * 6 + lane];
+      } else {
+        const VoxState& vst = m.vs[V];
+        nv = (vst.tag == tag && vst.seg >= 0) ? eligible_at(m, b.seg_nbr[vst.seg * 6 + lane], tag, t)
+                                              : nbr_at(m, vst.key, lane, tag, t);
+      }
       nbv[lane] = nv;
       if (nv >= 0) {
         grp[lane] = m.vs[nv].group_id;
@@ -309,7 +338,10 @@ __global__ void __launch_bounds__(64) k_merge_replay(const MapDev* maps, const S
       grp[6] = m.vs[V].group_id;
       for (int k = 0; k < kPlaneD; ++k) pl[6][k] = m.plane[size_t(V) * kPlaneD + k];
     }
-    if (lane == 0) nclos = 0;
+    if (lane == 0) {
+      nclos = 0;
+      nstage = 0;
+    }
     __syncthreads();
     // ---- merge(): the six neighbours in order against the (updated) own plane
     if (lane == 0) {
@@ -336,46 +368,90 @@ __global__ void __launch_bounds__(64) k_merge_replay(const MapDev* maps, const S
       write_hot(m, X, fl);
     }
     merges += nc0 - 1;
+    if (lane < nc0) ckey[lane] = m.vs[nbv[clos[lane]]].key;
     __syncthreads();
-    // ---- taint the closed neighbourhoods of every changed voxel and queue
-    //      the next call of each touched one
+    // ---- taint the closed neighbourhoods of every changed voxel: each
+    //      touched one re-evaluates its next call and its later candidates.
+    //      A neighbour that did not change can only start merging with a
+    //      changed voxel (its other neighbours were already idle against its
+    //      unchanged plane), so it is re-evaluated only if it passes the
+    //      group test and the gates against one of the new records.
     for (int q0 = 0; q0 < nc0 * 7; q0 += 64) {
       const int qq = q0 + lane;
-      int32_t X = -1;
       if (qq < nc0 * 7) {
         const int32_t base_v = nbv[clos[qq / 7]];
         const int j = qq % 7;
-        X = base_v;
+        int32_t X = base_v;
         if (j < 6) {
-          const uint64_t bk = m.vs[base_v].key;
-          X = key_neighbour_ok(bk, j) ? vid_at(m, lookup_vid(m, key_neighbour(bk, j)), tag, t) : -1;
+          const VoxState& bs = m.vs[base_v];
+          int32_t cur_id;
+          if (bs.tag == tag && bs.seg >= 0) cur_id = b.seg_nbr[bs.seg * 6 + j];
+          else cur_id = key_neighbour_ok(bs.key, j) ? lookup_vid(m, key_neighbour(bs.key, j)) : -1;
+          X = vid_at(m, cur_id, tag, t);
         }
-      }
-      int tn = kNever;
-      if (X >= 0) {
-        VoxState& xs = m.vs[X];
-        if (xs.tag == tag && xs.seg >= 0 && xs.evict_t == kNever) {
-          if (xs.dirty_t < t) xs.dirty_t = t;
-          if (xs.taint_t > t) xs.taint_t = t;
-          if (xs.merge_pos != kNever) tn = first_call_after(b, xs.seg, xs.merge_pos, t, off);
-        } else if (xs.flags & kAlive) {
-          atomicOr(&xs.flags, kDirty);
+        bool need = false;
+        if (X >= 0) {
+          for (int k = 0; k < nc0; ++k) need |= (nbv[clos[k]] == X);
+          if (!need) {
+            const VoxState& xv = m.vs[X];
+            if ((xv.flags & kPlane) && (xv.flags & kInit)) {
+              double px[kPlaneD];
+              for (int k = 0; k < kPlaneD; ++k) px[k] = m.plane[size_t(X) * kPlaneD + k];
+              int64_t ax, ay, az;
+              unpack_key(xv.key, ax, ay, az);
+              for (int k = 0; k < nc0; ++k) {
+                int64_t bx, by, bz;
+                unpack_key(ckey[k], bx, by, bz);
+                const int64_t dist = (ax > bx ? ax - bx : bx - ax) + (ay > by ? ay - by : by - ay) +
+                                     (az > bz ? az - bz : bz - az);
+                const int c = clos[k];
+                if (dist == 1 && grp[c] != xv.group_id && merge_gate(px, pl[c], m.merge_a, m.merge_d)) need = true;
+              }
+            }
+          }
         }
-      }
-      for (int w = 0; w < 64; ++w) {
-        const int tw = __shfl(tn, w, 64);
-        const int32_t xw = __shfl(X, w, 64);
-        if (tw == kNever) continue;
-        if (hn >= kHeapCap) {
-          err |= 16;
-          continue;
+        if (X >= 0 && need) {
+          VoxState& xs = m.vs[X];
+          if (xs.tag == tag && xs.seg >= 0 && xs.evict_t == kNever) {
+            if (xs.dirty_t < t) xs.dirty_t = t;
+            if (xs.taint_t > t) xs.taint_t = t;
+            uint64_t loc[8];
+            int nl = 0;
+            if (xs.merge_pos != kNever) {
+              const int tn = first_call_after(b, xs.seg, xs.merge_pos, t, off);
+              if (tn != kNever) loc[nl++] = cand_key(0, tn, X);
+            }
+            const int nct = b.seg_nct[xs.seg];
+            for (int k = 0; k < nct; ++k) {
+              const int tc = b.seg_ct[xs.seg * 7 + k];
+              if (tc > t) loc[nl++] = cand_key(0, tc, X);
+            }
+            if (nl > 0) {
+              const int at = atomicAdd(&nstage, nl);
+              for (int k = 0; k < nl; ++k)
+                if (at + k < kStageCap) stage[at + k] = loc[k];
+            }
+          } else if (xs.flags & kAlive) {
+            atomicOr(&xs.flags, kDirty);
+          }
         }
-        if (lane == 0) heap_push(heap, hn, (uint64_t(uint32_t(tw)) << 32) | uint32_t(xw));
-        ++hn;
       }
     }
     __syncthreads();
-    cbase = -1;  // taints changed: re-read the static window
+    if (lane == 0) {
+      const int ns = nstage < kStageCap ? nstage : kStageCap;
+      if (nstage > kStageCap || hn + ns > kHeapCap) {
+        err |= 16;
+      } else {
+        for (int k = 0; k < ns; ++k) heap_push(heap, hn + k, stage[k]);
+      }
+    }
+    __syncthreads();
+    {
+      const int ns = nstage < kStageCap ? nstage : kStageCap;
+      if (hn + ns <= kHeapCap) hn += ns;
+    }
+    err = __shfl(err, 0, 64);
     if (err) break;
   }
   if (lane == 0) {

@@ -45,12 +45,13 @@ class Sequence:
         except Exception:
             pass
 
-    def scan(self, k):
-        x = np.empty(self.n, np.float32)
-        y = np.empty(self.n, np.float32)
-        z = np.empty(self.n, np.float32)
+    def scan(self, k, n=None):
+        n = self.n if n is None else int(n)
+        x = np.empty(n, np.float32)
+        y = np.empty(n, np.float32)
+        z = np.empty(n, np.float32)
         fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
-        rc = self.lib.vmp_synth_scan(self.h, k, fp(x), fp(y), fp(z), None)
+        rc = self.lib.vmp_synth_scan_n(self.h, k, n, fp(x), fp(y), fp(z), None)
         if rc != 0:
             raise RuntimeError(f"vmp_synth_scan failed ({rc})")
         return x, y, z
