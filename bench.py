@@ -205,6 +205,7 @@ def main():
 
     g.set_timing(True)
     lat = []
+    maint = []
     iters = 0
     mstat = {"created": 0, "evicted": 0, "merges": 0, "merge_evals": 0, "refits": 0}
     torch.cuda.synchronize()
@@ -215,6 +216,7 @@ def main():
         bind(k)
         o, m = g.register()
         lat.append(time.perf_counter() - ts)
+        maint.append(m.maintenance)
         iters += o.iterations
         for f in mstat:
             mstat[f] += getattr(m, f)
@@ -223,6 +225,8 @@ def main():
     tt = g.timing_totals()
     map_size = g.map_size()
     lat_ms = np.array(lat) * 1e3
+    for j in np.argsort(lat_ms)[::-1][:3]:
+        log(f"slow step {j}: {lat_ms[j]:.2f} ms, maintenance flags {maint[j]}")
 
     obs_launches = max(tt["obs_launches"], 1)
     obs_avg_ms = tt["obs_ms"] / obs_launches
@@ -252,6 +256,7 @@ def main():
             "map_voxels_end": map_size,
             "ieskf_iterations_mean": iters / K,
             "per_scan_mean": {f: v / K for f, v in mstat.items()},
+            "maintenance_events": {"hash_rebuilds": sum(1 for f in maint if f & 1), "ring_compactions": sum(1 for f in maint if f & 2)},
         },
         "latency_ms": {
             "p50": float(np.percentile(lat_ms, 50)),

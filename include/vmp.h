@@ -92,6 +92,8 @@ typedef struct {
   int32_t merges;
   int32_t dropped;
   int32_t map_size;
+  int32_t maintenance; /* housekeeping after this scan: 1 = hash table
+                          rebuilt, 2 = LRU ring compacted (bit set) */
 } vmp_map_stats;
 
 typedef struct vmp_ctx vmp_ctx;

@@ -180,12 +180,14 @@ V3 so3_log(const M3& m) {
 }
 
 // Left Jacobian of SO3 (Sophus::SO3d::leftJacobian; ieskf.cpp:5-8 takes its
-// transpose as the right Jacobian).
+// transpose as the right Jacobian).  First-order branch for theta^2 < 1e-10
+// (Sophus' Constants<double>::epsilon()), where 1 - cos(theta) cancels.
 M3 left_jacobian(const V3& w) {
-  double th = std::sqrt(dot(w, w));
+  double tsq = dot(w, w);
   M3 O = hat(w);
   M3 I = M3::ident();
-  if (th < 1e-10) return I + 0.5 * O;
+  if (tsq < 1e-10) return I + 0.5 * O;
+  double th = std::sqrt(tsq);
   M3 O2 = O * O;
   return (I + ((1.0 - std::cos(th)) / (th * th)) * O) + ((th - std::sin(th)) / (th * th * th)) * O2;
 }

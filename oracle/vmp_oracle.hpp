@@ -19,7 +19,8 @@
 // here because Eigen/Sophus/PCL/ROS are absent): summation order of small
 // products is left-to-right over the inner index; PCL's fp32 transform is
 // x' = m0*x + (m1*y + (m2*z + t)); DEG2RAD uses pi/180; SO3 exp/log follow
-// Rodrigues with Taylor branches below 1e-10 rad; 23x23 inverse is LU with
+// Rodrigues with Taylor branches below 1e-10 rad (exp, log) and a first-order
+// left Jacobian below theta^2 = 1e-10 (Sophus' epsilon); 23x23 inverse is LU with
 // partial pivoting; 3x3 eigenvectors come from cyclic Jacobi.
 #pragma once
 #include <cstdint>

@@ -162,6 +162,43 @@ def test_lru_eviction_parity(ctx, capacity):
         assert orc.stats()["evicted"] == evicted
 
 
+def test_lru_ring_compaction_parity(ctx):
+    """Long run on a small map: the device LRU ring (one entry per voxel
+    touch) wraps and is compacted several times; LRU order, keys and group
+    ids must stay bit-exact with the oracle's std::list through it."""
+    npts, cap, nscans = 8000, 12000, 40
+    qcap = 2 * (cap + 2 * npts) + 1024  # ring entries of a session with these limits
+    seq = Sequence(STREET, SEED0 + 21, npts)
+    cfg = LIOConfig(max_scan_points=npts, map_capacity=cap)
+    orc, s0, P0, (x0, y0, z0) = start(seq, cfg)
+    touched = 0
+    with Registrar(ctx, cfg) as g:
+        g.set_state(s0, P0)
+        g.set_points(x0, y0, z0)
+        touched += g.build().touched
+        for k in range(1, nscans):
+            dR, dp = seq.motion(k)
+            orc.apply_motion(dR, dp, QDIAG)
+            sp, Pp = orc.get_state()
+            x, y, z = seq.scan(k)
+            orc.register(x, y, z)
+            g.set_state(sp, Pp)
+            g.set_points(x, y, z)
+            _, mst = g.register()
+            touched += mst.touched
+            so, _ = orc.get_state()
+            sg, _ = g.get_state()
+            ep, er = state_err(so, sg)
+            assert ep <= 1e-4 and er <= 1e-5, (k, ep, er)
+            if k % 5 == 0 or k == nscans - 1:
+                a, b = orc.export_map(), g.export_map()
+                log(f"scan {k}: ring entries written {touched} (ring {qcap}), map {len(b['keys'])}, pos {ep:.1e}")
+                assert np.array_equal(a["keys"], b["keys"]), f"LRU order differs after scan {k}"
+                r = compare_maps(a, b)
+                assert r["plane_flag_diff"] <= max(2, r["n"] // 1000)
+    assert touched > 2 * qcap, "the ring must have been compacted at least twice"
+
+
 def test_capacity_below_scan_is_an_error(ctx):
     """A scan touching more voxels than the LRU can evict in point order is
     reported (VMP_ECAPACITY) instead of silently diverging."""

@@ -64,6 +64,7 @@ class MapStats(C.Structure):
             "merges",
             "dropped",
             "map_size",
+            "maintenance",
         )
     ]
 

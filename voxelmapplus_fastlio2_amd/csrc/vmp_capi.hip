@@ -96,11 +96,27 @@ struct BatchWs {
   }
 };
 
+// Scratch of the device-side LRU ring compaction (grown on demand).
+struct RingTmp {
+  int64_t cap = 0;
+  std::vector<void*> allocs;
+  int32_t *tv = nullptr, *keep = nullptr, *sel = nullptr, *cnt = nullptr;
+  uint64_t* ts = nullptr;
+  void* temp = nullptr;
+  size_t temp_bytes = 0;
+  void release() {
+    for (void* p : allocs) hipFree(p);
+    allocs.clear();
+    cap = 0;
+  }
+};
+
 struct vmp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::mutex mu;
   BatchWs ws;
+  RingTmp rt;
   hipEvent_t ev[4]{};
   // optional per-launch timing of the observe kernel / map pipeline
   bool timing = false;
@@ -166,6 +182,7 @@ extern "C" int vmp_ctx_destroy(vmp_ctx* ctx) {
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   ctx->ws.release();
+  ctx->rt.release();
   for (hipEvent_t e : ctx->ev)
     if (e) hipEventDestroy(e);
   for (hipEvent_t e : ctx->tev)
@@ -660,7 +677,8 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
 }
 
 // housekeeping between scans: hash rebuild and LRU ring compaction (rare)
-static int maintain(vmp_session* s) {
+static int maintain(vmp_session* s, int32_t* did) {
+  *did = 0;
   MapDev& m = s->map;
   const MapCounters& c = s->h_ctr;
   hipStream_t st = s->ctx->stream;
@@ -677,38 +695,46 @@ static int maintain(vmp_session* s) {
     hipLaunchKernelGGL(k_rehash_insert, dim3(grid_for(m.pool, T)), dim3(T), 0, st, m, s->live, s->nlive);
     VMP_HIP(hipMemcpyAsync(&s->h_ctr, m.ctr, sizeof(MapCounters), hipMemcpyDeviceToHost, st));
     VMP_HIP(hipStreamSynchronize(st));
+    *did |= 1;
   }
   const int64_t used = s->h_ctr.ring_tail - s->h_ctr.ring_head;
   if (used + int64_t(s->cfg.max_scan_points) > m.qcap) {
-    // keep live entries (ascending stamps preserved) at the front
-    const int64_t head = s->h_ctr.ring_head;
-    std::vector<int32_t> rv(static_cast<size_t>(m.qcap));
-    std::vector<uint64_t> rs(static_cast<size_t>(m.qcap));
-    std::vector<VoxState> vs(static_cast<size_t>(m.pool));
-    VMP_HIP(hipMemcpy(rv.data(), m.ring_vid, sizeof(int32_t) * size_t(m.qcap), hipMemcpyDeviceToHost));
-    VMP_HIP(hipMemcpy(rs.data(), m.ring_stamp, sizeof(uint64_t) * size_t(m.qcap), hipMemcpyDeviceToHost));
-    VMP_HIP(hipMemcpy(vs.data(), m.vs, sizeof(VoxState) * size_t(m.pool), hipMemcpyDeviceToHost));
-    std::vector<int32_t> hv;
-    std::vector<uint64_t> hs;
-    hv.reserve(static_cast<size_t>(used));
-    hs.reserve(static_cast<size_t>(used));
-    for (int64_t r = 0; r < used; ++r) {
-      const size_t ph = size_t((head + r) % m.qcap);
-      const int32_t v = rv[ph];
-      if ((vs[size_t(v)].flags & kAlive) && vs[size_t(v)].stamp == rs[ph]) {
-        hv.push_back(v);
-        hs.push_back(rs[ph]);
+    RingTmp& t = s->ctx->rt;
+    if (t.cap < used) {
+      VMP_HIP(hipStreamSynchronize(st));
+      t.release();
+      const size_t n = size_t(used);
+      bool ok = dalloc(t.allocs, &t.tv, n) == hipSuccess && dalloc(t.allocs, &t.ts, n) == hipSuccess &&
+                dalloc(t.allocs, &t.keep, n) == hipSuccess && dalloc(t.allocs, &t.sel, n) == hipSuccess &&
+                dalloc(t.allocs, &t.cnt, 1) == hipSuccess;
+      t.temp_bytes = 0;
+      if (ok)
+        ok = rocprim::select(nullptr, t.temp_bytes, rocprim::counting_iterator<int32_t>(0), t.keep, t.sel, t.cnt, n,
+                             st) == hipSuccess;
+      if (ok) ok = dalloc(t.allocs, reinterpret_cast<uint8_t**>(&t.temp), t.temp_bytes + 256) == hipSuccess;
+      if (!ok) {
+        t.release();
+        s->err = "LRU ring compaction: device allocation failed";
+        return VMP_ENOMEM;
       }
+      t.temp_bytes += 256;
+      t.cap = used;
     }
-    if (!hv.empty()) {
-      VMP_HIP(hipMemcpy(m.ring_vid, hv.data(), sizeof(int32_t) * hv.size(), hipMemcpyHostToDevice));
-      VMP_HIP(hipMemcpy(m.ring_stamp, hs.data(), sizeof(uint64_t) * hs.size(), hipMemcpyHostToDevice));
-    }
+    hipLaunchKernelGGL(k_ring_mark, dim3(grid_for(used, T)), dim3(T), 0, st, m, s->h_ctr.ring_head, used, t.tv, t.ts,
+                       t.keep);
+    size_t tb = t.temp_bytes;
+    VMP_HIP(rocprim::select(t.temp, tb, rocprim::counting_iterator<int32_t>(0), t.keep, t.sel, t.cnt, size_t(used), st));
+    hipLaunchKernelGGL(k_ring_scatter, dim3(grid_for(used, T)), dim3(T), 0, st, m, t.tv, t.ts, t.sel, t.cnt);
+    int32_t kept = 0;
+    VMP_HIP(hipMemcpyAsync(&kept, t.cnt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    VMP_HIP(hipStreamSynchronize(st));
     MapCounters cc = s->h_ctr;
     cc.ring_head = 0;
-    cc.ring_tail = int64_t(hv.size());
-    VMP_HIP(hipMemcpy(m.ctr, &cc, sizeof(cc), hipMemcpyHostToDevice));
+    cc.ring_tail = int64_t(kept);
+    VMP_HIP(hipMemcpyAsync(m.ctr, &cc, sizeof(cc), hipMemcpyHostToDevice, st));
+    VMP_HIP(hipStreamSynchronize(st));
     s->h_ctr = cc;
+    *did |= 2;
   }
   return VMP_OK;
 }
@@ -828,7 +854,9 @@ static int run_step(vmp_session** ss, int n, int mode, vmp_obs_stats* ost, vmp_m
         mst[i].dropped = m.dropped;
         mst[i].map_size = int32_t(m.size);
       }
-      rc = maintain(si);
+      int32_t did = 0;
+      rc = maintain(si, &did);
+      if (mst) mst[i].maintenance = did;
       if (rc != VMP_OK && first_err == VMP_OK) first_err = rc;
     }
   }

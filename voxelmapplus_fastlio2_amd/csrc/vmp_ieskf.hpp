@@ -112,12 +112,14 @@ __device__ inline D33 hat_d(const double v[3]) {
   return h;
 }
 
-// left Jacobian I + (1-cos)/th^2 W + (th-sin)/th^3 W^2
+// left Jacobian I + (1-cos)/th^2 W + (th-sin)/th^3 W^2; first-order branch
+// for th^2 < 1e-10, where 1 - cos(th) cancels (Sophus' threshold).
 __device__ inline D33 left_jac_d(const double w[3]) {
-  const double th = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  const double tsq = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  const double th = sqrt(tsq);
   const D33 W = hat_d(w);
   D33 J;
-  if (th < 1e-10) {
+  if (tsq < 1e-10) {
     for (int i = 0; i < 9; ++i) J.m[i] = (i % 4 == 0 ? 1.0 : 0.0) + 0.5 * W.m[i];
     return J;
   }

@@ -959,4 +959,28 @@ __global__ void k_rehash_insert(MapDev m, const int32_t* live, const int32_t* nl
   }
 }
 
+// LRU ring compaction.  An entry stays when its voxel is alive and still
+// carries the entry's stamp (older entries of a re-touched voxel are stale).
+// Pass 1 linearises the circular range [head, head + used) into tv/ts and
+// flags the survivors; a stable select gives their positions; pass 2 writes
+// them back to the front of the ring, so ascending stamp order is kept.
+__global__ void k_ring_mark(MapDev m, int64_t head, int64_t used, int32_t* tv, uint64_t* ts, int32_t* keep) {
+  for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < used; r += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t ph = (head + r) % m.qcap;
+    const int32_t v = m.ring_vid[ph];
+    const uint64_t s = m.ring_stamp[ph];
+    tv[r] = v;
+    ts[r] = s;
+    keep[r] = (v >= 0 && v < m.pool && (m.vs[v].flags & kAlive) && m.vs[v].stamp == s) ? 1 : 0;
+  }
+}
+__global__ void k_ring_scatter(MapDev m, const int32_t* tv, const uint64_t* ts, const int32_t* sel, const int32_t* cnt) {
+  const int n = *cnt;
+  VMP_GRID_LOOP(j, n) {
+    const int32_t r = sel[j];
+    m.ring_vid[j] = tv[r];
+    m.ring_stamp[j] = ts[r];
+  }
+}
+
 }  // namespace vmp
