@@ -182,6 +182,11 @@ def main():
         dev[k, 0].copy_(torch.from_numpy(x))
         dev[k, 1].copy_(torch.from_numpy(y))
         dev[k, 2].copy_(torch.from_numpy(z))
+    # one elementwise pass of known size (256 MiB read, 256 MiB written): the
+    # reference launch that scales the FETCH_SIZE / WRITE_SIZE counters
+    cal = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+    torch.mul(cal, 2.0, out=torch.empty_like(cal))
+    del cal
     torch.cuda.synchronize()
 
     cfg = LIOConfig(max_scan_points=N, map_capacity=args.capacity)
