@@ -155,6 +155,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--batch-warmup", type=int, default=3)
     ap.add_argument("--batch-capacity", type=int, default=200000)
+    ap.add_argument("--obs-ppt", type=int, default=0, help="observe points per thread (0 = by scan size)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -193,6 +194,8 @@ def main():
     R, t = seq.pose(0)
     s0, P0 = state_vector(R, t), initial_covariance()
     ctx = Context(0)
+    if args.obs_ppt:
+        ctx.set_param("obs_ppt", args.obs_ppt)
     g = Registrar(ctx, cfg)
 
     def bind(k):

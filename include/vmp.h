@@ -148,6 +148,11 @@ int vmp_slots_export(vmp_session* s, int32_t n, uint8_t* valid, float* resid, fl
 /* Batched forms: sessions advance together, one launch per stage. */
 int vmp_register_batch(vmp_session** s, int32_t n, vmp_obs_stats* ost, vmp_map_stats* mst);
 
+/* Launch-shape parameters of a context (no reference counterpart):
+ *   "obs_ppt"  points per thread of the observe kernel, 1..64; 0 = chosen by
+ *              scan size (the default).  Applies to later calls. */
+int vmp_ctx_set_param(vmp_ctx* ctx, const char* name, int64_t value);
+
 /* Device events around the last call's kernels (for roofline timing). */
 int vmp_session_timing(vmp_session* s, double* observe_ms, double* map_ms, int32_t* observe_launches);
 /* Per-launch HIP-event timing of the observe kernel and of the map-update

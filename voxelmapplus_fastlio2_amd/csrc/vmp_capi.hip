@@ -126,7 +126,17 @@ struct vmp_ctx {
   int64_t acc_obs_n = 0, acc_map_n = 0;
   int64_t acc_obs_points = 0, acc_obs_hits = 0;
   int obs_launches = 0;
+  // launch shape of the observe kernel: points per thread, 0 = by scan size
+  int obs_ppt = 0;
 };
+
+// Points per thread of the observe kernel for a scan of n points.  It depends
+// only on n (never on the batch), so a session's block decomposition, and
+// with it the summation order of H and b, is the same alone and in a batch.
+static int obs_ppt_for(const vmp_ctx* c, int n) {
+  (void)n;
+  return c->obs_ppt > 0 ? c->obs_ppt : 2;
+}
 
 struct vmp_session {
   vmp_ctx* ctx = nullptr;
@@ -143,8 +153,6 @@ struct vmp_session {
   double* partials = nullptr;
   int32_t* obs_ctr = nullptr;
   int32_t* obs_stat = nullptr;
-  unsigned int* ticket = nullptr;
-  int ppt = 2;
   int32_t* live = nullptr;
   int32_t* nlive = nullptr;
   bool has_points = false;
@@ -300,7 +308,7 @@ extern "C" int64_t vmp_session_bytes(const vmp_map_cfg* cfg) {
   b += z.pool * int64_t(sizeof(VoxHot) + sizeof(VoxState) + 8 * kPlaneD + 8 * kMomD + 4 + 4);
   b += z.qcap * 12;
   b += z.maxn * (3 * 4 + 16);
-  b += z.max_blocks * 90 * 8;
+  b += z.max_blocks * kObsRowMax * 8;
   b += int64_t(sizeof(FilterDev));
   return b;
 }
@@ -354,10 +362,9 @@ extern "C" int vmp_session_create(vmp_ctx* ctx, const vmp_map_cfg* cfg, vmp_sess
   ok &= dalloc(A, &s->own_z, maxn) == hipSuccess;
   ok &= dalloc(A, &s->slots, maxn) == hipSuccess;
   ok &= dalloc(A, &s->f, 1) == hipSuccess;
-  ok &= dalloc(A, &s->partials, size_t(z.max_blocks) * 90) == hipSuccess;
+  ok &= dalloc(A, &s->partials, size_t(z.max_blocks) * kObsRowMax) == hipSuccess;
   ok &= dalloc(A, &s->obs_ctr, 8) == hipSuccess;
   ok &= dalloc(A, &s->obs_stat, 8) == hipSuccess;
-  ok &= dalloc(A, &s->ticket, 1) == hipSuccess;
   ok &= dalloc(A, &s->live, size_t(z.pool)) == hipSuccess;
   ok &= dalloc(A, &s->nlive, 1) == hipSuccess;
   if (!ok) return fail(VMP_ENOMEM);
@@ -407,7 +414,6 @@ extern "C" int vmp_session_create(vmp_ctx* ctx, const vmp_map_cfg* cfg, vmp_sess
   f0.x.g[2] = -kGravity;
   for (int i = 0; i < 23; ++i) f0.P[i * 23 + i] = 1.0;
   if (hipMemcpy(s->f, &f0, sizeof(f0), hipMemcpyHostToDevice) != hipSuccess) return fail(VMP_EDEVICE);
-  if (hipMemset(s->ticket, 0, sizeof(unsigned int)) != hipSuccess) return fail(VMP_EDEVICE);
   if (hipMemset(s->obs_ctr, 0, 8 * sizeof(int32_t)) != hipSuccess) return fail(VMP_EDEVICE);
   if (hipMemset(s->obs_stat, 0, 8 * sizeof(int32_t)) != hipSuccess) return fail(VMP_EDEVICE);
   if (hipDeviceSynchronize() != hipSuccess) return fail(VMP_EDEVICE);
@@ -500,12 +506,11 @@ static void fill_job(vmp_session* s, ObsJob& j, int block0) {
   j.prm = &s->f->obs_prm;
   j.partials = s->partials;
   j.counters = s->obs_ctr;
-  j.ticket = s->ticket;
   j.out = s->f->obs;
   j.hmask = s->map.hmask;
   j.n = s->n;
-  j.ppt = s->ppt;
-  j.nblocks = std::max(1, (s->n + kObsThreads * s->ppt - 1) / (kObsThreads * s->ppt));
+  j.ppt = obs_ppt_for(s->ctx, s->n);
+  j.nblocks = std::max(1, (s->n + kObsThreads * j.ppt - 1) / (kObsThreads * j.ppt));
   j.block0 = block0;
   j.pad = 0;
 }
@@ -570,6 +575,10 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
       hipEventRecord(c->tev[c->tev_used + 1], st);
       c->tev_used += 2;
     }
+    if (ext)
+      hipLaunchKernelGGL(k_obs_reduce<12>, dim3(n), dim3(kObsThreads), 0, st, w.d_jobs);
+    else
+      hipLaunchKernelGGL(k_obs_reduce<6>, dim3(n), dim3(kObsThreads), 0, st, w.d_jobs);
     c->obs_launches++;
   };
   if (mode & kStepObserveOnce) {
@@ -936,6 +945,17 @@ extern "C" int vmp_session_timing(vmp_session* s, double* observe_ms, double* ma
   }
   if (observe_launches) *observe_launches = c->obs_launches;
   return VMP_OK;
+}
+
+extern "C" int vmp_ctx_set_param(vmp_ctx* c, const char* name, int64_t value) {
+  if (!c || !name) return VMP_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (std::strcmp(name, "obs_ppt") == 0) {
+    if (value < 0 || value > 64) return VMP_EINVAL;
+    c->obs_ppt = int(value);
+    return VMP_OK;
+  }
+  return VMP_EINVAL;
 }
 
 extern "C" int vmp_session_set_timing(vmp_session* s, int on) {

@@ -23,6 +23,13 @@ namespace vmp {
 
 constexpr int kObsThreads = 256;
 
+// length of one block's partial row: the H upper triangle, b, four counters
+template <int NJ>
+__host__ __device__ constexpr int obs_row() {
+  return NJ * (NJ + 1) / 2 + NJ + 4;
+}
+constexpr int kObsRowMax = 96;  // >= obs_row<12>() = 94
+
 struct ObsJob {
   const float* x;
   const float* y;
@@ -31,9 +38,8 @@ struct ObsJob {
   const HashEntry* table;
   const VoxHot* hot;
   const ObsParams* prm;
-  double* partials;       // [nblocks][nacc]
+  double* partials;       // [nblocks][obs_row]
   int32_t* counters;      // hits, plane_hits, effective, fallback (+ error)
-  unsigned int* ticket;
   double* out;            // H (144, column-major) then b (12)
   uint32_t hmask;
   int32_t n;
@@ -90,7 +96,6 @@ __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __re
   constexpr int NACC = NH + NJ;
   __shared__ double red[kObsThreads / 64][NACC];
   __shared__ int redi[kObsThreads / 64][4];
-  __shared__ int is_last;
 
   const int ji = find_job(jobs, njobs, blockIdx.x);
   const ObsJob& J = jobs[ji];
@@ -245,7 +250,11 @@ __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __re
     }
   }
 
-  // ---- block reduction (fp64, fixed order)
+  // ---- block reduction (fp64, fixed order) into this block's partial row:
+  //      NACC sums, then hits, plane hits, effective, fallbacks.  No fence or
+  //      ticket: k_obs_reduce runs after this launch and the kernel boundary
+  //      makes the rows visible (an agent-scope fence per block would write
+  //      back and invalidate the whole L2 of its XCD).
   const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
   for (int k = 0; k < NACC; ++k) {
@@ -263,39 +272,46 @@ __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __re
     redi[wv][3] = c_fb;
   }
   __syncthreads();
+  double* row = J.partials + size_t(lb) * obs_row<NJ>();
   if (tid < NACC) {
     double s = red[0][tid];
     for (int w = 1; w < kObsThreads / 64; ++w) s += red[w][tid];
-    J.partials[size_t(lb) * NACC + tid] = s;
+    row[tid] = s;
   }
   if (tid < 4) {
     int s = 0;
     for (int w = 0; w < kObsThreads / 64; ++w) s += redi[w][tid];
-    atomicAdd(J.counters + tid, s);
+    row[NACC + tid] = double(s);
   }
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) is_last = (atomicAdd(J.ticket, 1u) == unsigned(J.nblocks - 1));
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
+}
 
-  // ---- last block: reduce all block partials of this job in a fixed order
-  double fin[NACC];
+// Reduce the partial rows of every job (one workgroup per session) in a fixed
+// order: row b is added by thread b mod 256, threads are combined wave by
+// wave.  Writes H (12x12 column-major, symmetric), b and the counters.
+template <int NJ>
+__global__ void __launch_bounds__(kObsThreads) k_obs_reduce(const ObsJob* __restrict__ jobs) {
+  constexpr int NH = NJ * (NJ + 1) / 2;
+  constexpr int NACC = NH + NJ;
+  constexpr int ROW = obs_row<NJ>();
+  __shared__ double red[kObsThreads / 64][ROW];
+  const ObsJob& J = jobs[blockIdx.x];
+  if (!J.prm->active) return;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double fin[ROW];
 #pragma unroll
-  for (int k = 0; k < NACC; ++k) fin[k] = 0.0;
+  for (int k = 0; k < ROW; ++k) fin[k] = 0.0;
   for (int b = tid; b < J.nblocks; b += kObsThreads) {
-    const double* src = J.partials + size_t(b) * NACC;
+    const double* src = J.partials + size_t(b) * ROW;
 #pragma unroll
-    for (int k = 0; k < NACC; ++k) fin[k] += __builtin_nontemporal_load(src + k);
+    for (int k = 0; k < ROW; ++k) fin[k] += src[k];
   }
 #pragma unroll
-  for (int k = 0; k < NACC; ++k) {
+  for (int k = 0; k < ROW; ++k) {
     const double s = wave_sum(fin[k]);
     if (lane == 0) red[wv][k] = s;
   }
   __syncthreads();
-  if (tid < NACC) {
+  if (tid < ROW) {
     double s = red[0][tid];
     for (int w = 1; w < kObsThreads / 64; ++w) s += red[w][tid];
     red[0][tid] = s;
@@ -312,7 +328,7 @@ __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __re
     J.out[tid] = v;
   }
   if (tid < 12) J.out[144 + tid] = tid < NJ ? red[0][NH + tid] : 0.0;
-  if (tid == 0) *J.ticket = 0u;
+  if (tid < 4) J.counters[tid] = int32_t(red[0][NACC + tid]);
 }
 
 }  // namespace vmp

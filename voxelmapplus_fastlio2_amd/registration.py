@@ -128,6 +128,11 @@ class Context:
             self.lib.vmp_ctx_destroy(self.h)
             self.h = None
 
+    def set_param(self, name, value):
+        rc = self.lib.vmp_ctx_set_param(self.h, name.encode(), int(value))
+        if rc != 0:
+            raise VmpError(f"vmp_ctx_set_param({name}, {value}) failed ({rc})")
+
 
 class Registrar:
     """One LiDAR sequence: voxel map, residual slots and filter state in HBM."""
