@@ -69,7 +69,14 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000):
     uniform in [2k, nmax], advanced together by vmp_register_batch."""
     import torch
 
-    from voxelmapplus_fastlio2_amd import LIOConfig, Registrar, initial_covariance, register_batch, state_vector
+    from voxelmapplus_fastlio2_amd import (
+        LIOConfig,
+        Registrar,
+        apply_motion_batch,
+        initial_covariance,
+        register_batch,
+        state_vector,
+    )
     from voxelmapplus_fastlio2_amd.synth import QDIAG, ROOM, SEED0, STREET, Sequence
 
     rng = np.random.default_rng(0xBA7C4)
@@ -105,9 +112,11 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000):
         R, t = seqs[i].pose(0)
         g.set_state(state_vector(R, t), initial_covariance())
         g.build()
+    def motion(k):
+        apply_motion_batch(regs, [m[0] for m in motions[k]], [m[1] for m in motions[k]], QDIAG)
+
     for k in range(1, W + 1):
-        for i, g in enumerate(regs):
-            g.apply_motion(*motions[k][i], QDIAG)
+        motion(k)
         bind(k)
         register_batch(regs)
     torch.cuda.synchronize()
@@ -116,8 +125,7 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000):
     t_start = time.perf_counter()
     for k in range(W + 1, W + K + 1):
         ts = time.perf_counter()
-        for i, g in enumerate(regs):
-            g.apply_motion(*motions[k][i], QDIAG)
+        motion(k)
         bind(k)
         register_batch(regs)
         lat.append(time.perf_counter() - ts)

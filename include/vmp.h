@@ -116,6 +116,9 @@ int vmp_state_get(vmp_session* s, vmp_state* x, double* P);
 /* Motion prior between scans (stand-in for IMU propagation, which is outside
  * this path): pos += rot*dp; rot = rot*dR; P += diag(qdiag[23]). */
 int vmp_state_apply_motion(vmp_session* s, const double* dR, const double* dp, const double* qdiag);
+/* The same for n sessions of one context in one launch: dR 9n (column-major
+ * per session), dp 3n, qdiag 23n. */
+int vmp_state_apply_motion_batch(vmp_session** s, int32_t n, const double* dR, const double* dp, const double* qdiag);
 
 /* Scan points in the LiDAR frame, SoA fp32, scan (time) order.  Host
  * pointers are copied; _device takes pointers that already live in HBM. */

@@ -10,7 +10,7 @@ import pytest
 from voxelmapplus_fastlio2_amd import LIOConfig, _lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = {"vmp.h": _lib.LIBVMP, "vmp_synth.h": _lib.LIBSYNTH}
+HEADERS = {"vmp.h": _lib.LIBVMP, "vmp_serve.h": _lib.LIBVMP, "vmp_synth.h": _lib.LIBSYNTH}
 
 
 def declared(header):

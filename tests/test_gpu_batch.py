@@ -1,5 +1,6 @@
-"""Batched registration (vmp_register_batch): sessions of different scenes and
-scan sizes advance together, one launch per stage.  Every session's pose, P
+"""Batched registration (vmp_state_apply_motion_batch + vmp_register_batch):
+sessions of different scenes and scan sizes advance together, one launch per
+stage.  Every session's pose, P
 and map must be bitwise identical to the same session registered alone
 (each session's reductions use a block decomposition that depends only on
 its own point count), and the alone run is itself checked against the oracle
@@ -7,7 +8,15 @@ in test_gpu_parity.py."""
 import numpy as np
 import pytest
 
-from voxelmapplus_fastlio2_amd import Context, LIOConfig, Registrar, initial_covariance, register_batch, state_vector
+from voxelmapplus_fastlio2_amd import (
+    Context,
+    LIOConfig,
+    Registrar,
+    apply_motion_batch,
+    initial_covariance,
+    register_batch,
+    state_vector,
+)
 from voxelmapplus_fastlio2_amd.synth import QDIAG, ROOM, SEED0, STREET, Sequence
 
 pytestmark = pytest.mark.gpu
@@ -37,8 +46,9 @@ def test_batch_matches_alone():
             g.apply_motion(*seq.motion(k), QDIAG)
             g.set_points(*seq.scan(k))
             g.register()
+        mot = [seq.motion(k) for seq, _ in mixed]
+        apply_motion_batch([g for _, g in mixed], [a for a, _ in mot], [b for _, b in mot], QDIAG)
         for (seq, g) in mixed:
-            g.apply_motion(*seq.motion(k), QDIAG)
             g.set_points(*seq.scan(k))
         o, m = register_batch([g for _, g in mixed])
         for i, ((_, ga), (_, gb)) in enumerate(zip(alone, mixed)):

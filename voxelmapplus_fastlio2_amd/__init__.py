@@ -5,13 +5,23 @@ kernels (csrc/), the in-tree build (build.py), ctypes bindings and a Python
 mirror of the reference interface (registration.py), and the seeded synthetic
 sequence generator (synth.py).
 """
-from .registration import Context, LIOConfig, Registrar, VmpError, initial_covariance, register_batch, state_vector
+from .registration import (
+    Context,
+    LIOConfig,
+    Registrar,
+    VmpError,
+    apply_motion_batch,
+    initial_covariance,
+    register_batch,
+    state_vector,
+)
 
 __all__ = [
     "Context",
     "LIOConfig",
     "Registrar",
     "VmpError",
+    "apply_motion_batch",
     "initial_covariance",
     "register_batch",
     "state_vector",

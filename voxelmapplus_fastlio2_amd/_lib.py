@@ -105,6 +105,7 @@ def vmp():
     _proto(lib, "vmp_state_set", C.c_int, [P, C.POINTER(State), dp])
     _proto(lib, "vmp_state_get", C.c_int, [P, C.POINTER(State), dp])
     _proto(lib, "vmp_state_apply_motion", C.c_int, [P, dp, dp, dp])
+    _proto(lib, "vmp_state_apply_motion_batch", C.c_int, [pp, C.c_int32, dp, dp, dp])
     _proto(lib, "vmp_scan_set_points", C.c_int, [P, fp, fp, fp, C.c_int32])
     _proto(lib, "vmp_scan_set_points_device", C.c_int, [P, P, P, P, C.c_int32])
     _proto(lib, "vmp_observe", C.c_int, [P, C.POINTER(State), dp, dp, C.POINTER(ObsStats)])

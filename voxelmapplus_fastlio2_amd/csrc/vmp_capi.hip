@@ -24,9 +24,24 @@
 
 using namespace vmp;
 
-// gather the per-session counters of a batch into one array (one copy back)
+// session initialisation fills
+__global__ void k_init_slots(float4* slots, int n) {
+  VMP_GRID_LOOP(i, n) slots[i] = make_float4(0.f, 0.f, 0.f, __builtin_nanf(""));
+}
+__global__ void k_init_vs(VoxState* vs, int n) {
+  VMP_GRID_LOOP(i, n) {
+    VoxState v;
+    std::memset(&v, 0, sizeof(v));
+    v.tag = -1;
+    vs[i] = v;
+  }
+}
+
+// gather the per-session counters and posterior states of a batch into one
+// array (one copy back)
 struct BatchOut {
   MapCounters ctr;
+  vmp_state x;
   int32_t obs[8];
   int32_t iters;
   int32_t pad[3];
@@ -37,6 +52,7 @@ __global__ void k_gather(const MapDev* maps, FilterDev** fs, int32_t** obs_stat,
   out[s].ctr = *maps[s].ctr;
   for (int k = 0; k < 8; ++k) out[s].obs[k] = obs_stat[s][k];
   out[s].iters = fs[s]->iters;
+  out[s].x = fs[s]->x;
 }
 
 // ------------------------------------------------------------- host -------
@@ -84,6 +100,10 @@ struct BatchWs {
   int32_t** h_obsctr = nullptr;
   int32_t** h_obsstat = nullptr;
   BatchOut* h_out = nullptr;
+  MotionRec* d_motion = nullptr;
+  MotionRec* h_motion = nullptr;
+  hipEvent_t motion_done = nullptr;  // the last motion table upload finished
+  bool motion_pending = false;
   std::vector<void*> host_allocs;
 
   void release() {
@@ -190,6 +210,7 @@ extern "C" int vmp_ctx_destroy(vmp_ctx* ctx) {
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   ctx->ws.release();
+  if (ctx->ws.motion_done) hipEventDestroy(ctx->ws.motion_done);
   ctx->rt.release();
   for (hipEvent_t e : ctx->ev)
     if (e) hipEventDestroy(e);
@@ -253,6 +274,7 @@ static int ws_reserve(vmp_ctx* c, int64_t points, int nsess) {
   ok &= dalloc(A, &w.d_obsctr, size_t(ns)) == hipSuccess;
   ok &= dalloc(A, &w.d_obsstat, size_t(ns)) == hipSuccess;
   ok &= dalloc(A, &w.d_out, size_t(ns)) == hipSuccess;
+  ok &= dalloc(A, &w.d_motion, size_t(ns)) == hipSuccess;
   auto halloc = [&](auto** p, size_t bytes) {
     void* q = nullptr;
     if (hipHostMalloc(&q, std::max<size_t>(bytes, 16)) != hipSuccess) return false;
@@ -267,6 +289,9 @@ static int ws_reserve(vmp_ctx* c, int64_t points, int nsess) {
   ok &= halloc(&w.h_obsctr, sizeof(int32_t*) * ns);
   ok &= halloc(&w.h_obsstat, sizeof(int32_t*) * ns);
   ok &= halloc(&w.h_out, sizeof(BatchOut) * ns);
+  ok &= halloc(&w.h_motion, sizeof(MotionRec) * ns);
+  if (ok && !w.motion_done) ok = hipEventCreateWithFlags(&w.motion_done, hipEventDisableTiming) == hipSuccess;
+  w.motion_pending = false;
   if (!ok) {
     w.release();
     return VMP_ENOMEM;
@@ -384,26 +409,11 @@ extern "C" int vmp_session_create(vmp_ctx* ctx, const vmp_map_cfg* cfg, vmp_sess
   hipLaunchKernelGGL(k_rehash_clear, dim3(grid_for(std::min<int64_t>(z.hcap, 1 << 20), 256)), dim3(256), 0, st, m);
   MapCounters c0{};
   if (hipMemcpyAsync(m.ctr, &c0, sizeof(c0), hipMemcpyHostToDevice, st) != hipSuccess) return fail(VMP_EDEVICE);
-  {
-    // residual slots start invalid (NaN residual)
-    std::vector<float4> init(size_t(maxn), make_float4(0.f, 0.f, 0.f, __builtin_nanf("")));
-    if (hipMemcpyAsync(s->slots, init.data(), sizeof(float4) * size_t(maxn), hipMemcpyHostToDevice, st) != hipSuccess)
-      return fail(VMP_EDEVICE);
-    if (hipStreamSynchronize(st) != hipSuccess) return fail(VMP_EDEVICE);
-  }
-  {
-    std::vector<VoxState> vs0(size_t(std::min<int64_t>(z.pool, 1 << 16)));
-    for (auto& v : vs0) {
-      std::memset(&v, 0, sizeof(v));
-      v.tag = -1;
-    }
-    for (int64_t o = 0; o < z.pool; o += int64_t(vs0.size())) {
-      const size_t cnt = size_t(std::min<int64_t>(int64_t(vs0.size()), z.pool - o));
-      if (hipMemcpy(m.vs + o, vs0.data(), cnt * sizeof(VoxState), hipMemcpyHostToDevice) != hipSuccess)
-        return fail(VMP_EDEVICE);
-    }
-  }
-  if (hipMemset(m.hot, 0, sizeof(VoxHot) * size_t(z.pool)) != hipSuccess) return fail(VMP_EDEVICE);
+  // residual slots start invalid (NaN residual), voxel records empty; all
+  // stream-ordered so creating a session does not wait for other streams
+  hipLaunchKernelGGL(k_init_slots, dim3(grid_for(maxn, 256)), dim3(256), 0, st, s->slots, maxn);
+  hipLaunchKernelGGL(k_init_vs, dim3(grid_for(z.pool, 256)), dim3(256), 0, st, m.vs, int32_t(z.pool));
+  if (hipMemsetAsync(m.hot, 0, sizeof(VoxHot) * size_t(z.pool), st) != hipSuccess) return fail(VMP_EDEVICE);
   FilterDev f0;
   std::memset(&f0, 0, sizeof(f0));
   f0.cfg = *cfg;
@@ -413,10 +423,10 @@ extern "C" int vmp_session_create(vmp_ctx* ctx, const vmp_map_cfg* cfg, vmp_sess
   }
   f0.x.g[2] = -kGravity;
   for (int i = 0; i < 23; ++i) f0.P[i * 23 + i] = 1.0;
-  if (hipMemcpy(s->f, &f0, sizeof(f0), hipMemcpyHostToDevice) != hipSuccess) return fail(VMP_EDEVICE);
-  if (hipMemset(s->obs_ctr, 0, 8 * sizeof(int32_t)) != hipSuccess) return fail(VMP_EDEVICE);
-  if (hipMemset(s->obs_stat, 0, 8 * sizeof(int32_t)) != hipSuccess) return fail(VMP_EDEVICE);
-  if (hipDeviceSynchronize() != hipSuccess) return fail(VMP_EDEVICE);
+  if (hipMemcpyAsync(s->f, &f0, sizeof(f0), hipMemcpyHostToDevice, st) != hipSuccess) return fail(VMP_EDEVICE);
+  if (hipMemsetAsync(s->obs_ctr, 0, 8 * sizeof(int32_t), st) != hipSuccess) return fail(VMP_EDEVICE);
+  if (hipMemsetAsync(s->obs_stat, 0, 8 * sizeof(int32_t), st) != hipSuccess) return fail(VMP_EDEVICE);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return fail(VMP_EDEVICE);
   *out = s;
   return VMP_OK;
 }
@@ -457,6 +467,43 @@ extern "C" int vmp_state_apply_motion(vmp_session* s, const double* dR, const do
   hipLaunchKernelGGL(k_apply_motion, dim3(1), dim3(64), 0, s->ctx->stream, s->f, R, p, d);
   VMP_HIP(hipGetLastError());
   return VMP_OK;
+}
+
+// motion prior of n sessions on context c's stream (c->mu held by the caller);
+// qdiag holds 23 values per session, `qstride` apart (0: one shared vector)
+static int enqueue_motion(vmp_ctx* c, vmp_session** ss, int n, const double* dR, const double* dp,
+                          const double* qdiag, int qstride) {
+  vmp_session* s = ss[0];  // carries the error message
+  VMP_HIP(hipSetDevice(c->device));
+  if (ws_reserve(c, 1, n) != VMP_OK) return VMP_ENOMEM;
+  BatchWs& w = c->ws;
+  // the pinned table is reused: wait until the previous upload has been read
+  if (w.motion_pending) VMP_HIP(hipEventSynchronize(w.motion_done));
+  for (int i = 0; i < n; ++i) {
+    MotionRec& r = w.h_motion[i];
+    r.f = ss[i]->f;
+    std::memcpy(r.dR, dR + 9 * size_t(i), sizeof(r.dR));
+    std::memcpy(r.dp, dp + 3 * size_t(i), sizeof(r.dp));
+    std::memcpy(r.q, qdiag + size_t(qstride) * size_t(i), sizeof(r.q));
+  }
+  VMP_HIP(hipMemcpyAsync(w.d_motion, w.h_motion, sizeof(MotionRec) * n, hipMemcpyHostToDevice, c->stream));
+  VMP_HIP(hipEventRecord(w.motion_done, c->stream));
+  w.motion_pending = true;
+  hipLaunchKernelGGL(k_apply_motion_batch, dim3((n + 63) / 64), dim3(64), 0, c->stream, w.d_motion, n);
+  VMP_HIP(hipGetLastError());
+  return VMP_OK;
+}
+
+extern "C" int vmp_state_apply_motion_batch(vmp_session** ss, int32_t n, const double* dR, const double* dp,
+                                            const double* qdiag) {
+  if (!ss || n <= 0 || !dR || !dp || !qdiag) return VMP_EINVAL;
+  vmp_session* s = ss[0];  // carries the error message
+  vmp_ctx* c = s ? s->ctx : nullptr;
+  if (!c) return VMP_EINVAL;
+  for (int i = 0; i < n; ++i)
+    if (!ss[i] || ss[i]->ctx != c) return VMP_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  return enqueue_motion(c, ss, n, dR, dp, qdiag, 23);
 }
 
 extern "C" int vmp_scan_set_points(vmp_session* s, const float* x, const float* y, const float* z, int32_t n) {
@@ -686,11 +733,11 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
 }
 
 // housekeeping between scans: hash rebuild and LRU ring compaction (rare)
-static int maintain(vmp_session* s, int32_t* did) {
+static int maintain(vmp_ctx* ex, vmp_session* s, int32_t* did) {
   *did = 0;
   MapDev& m = s->map;
   const MapCounters& c = s->h_ctr;
-  hipStream_t st = s->ctx->stream;
+  hipStream_t st = ex->stream;
   const int64_t hcap = int64_t(m.hmask) + 1;
   const int T = 256;
   if (int64_t(c.used_entries) + 2 * int64_t(s->cfg.max_scan_points) > hcap / 2) {
@@ -708,7 +755,7 @@ static int maintain(vmp_session* s, int32_t* did) {
   }
   const int64_t used = s->h_ctr.ring_tail - s->h_ctr.ring_head;
   if (used + int64_t(s->cfg.max_scan_points) > m.qcap) {
-    RingTmp& t = s->ctx->rt;
+    RingTmp& t = ex->rt;
     if (t.cap < used) {
       VMP_HIP(hipStreamSynchronize(st));
       t.release();
@@ -759,12 +806,13 @@ static void collect_timing(vmp_ctx* c, int active_max) {
   c->tev_used = 0;
 }
 
-// run one step for a batch and distribute the results
-static int run_step(vmp_session** ss, int n, int mode, vmp_obs_stats* ost, vmp_map_stats* mst) {
-  vmp_ctx* c = ss[0]->ctx;
+// run one step for a batch on the stream and workspace of context `c` (the
+// sessions may belong to other contexts of the same device; the caller makes
+// sure no session is in two batches at once) and distribute the results
+static int run_step_on(vmp_ctx* c, vmp_session** ss, int n, int mode, vmp_obs_stats* ost, vmp_map_stats* mst) {
   vmp_session* s = ss[0];
   for (int i = 0; i < n; ++i) {
-    if (!ss[i] || ss[i]->ctx != c) return VMP_EINVAL;
+    if (!ss[i] || ss[i]->ctx->device != c->device) return VMP_EINVAL;
     if (!ss[i]->has_points) {
       ss[i]->err = "registration call before vmp_scan_set_points";
       return VMP_ESTATE;
@@ -864,12 +912,20 @@ static int run_step(vmp_session** ss, int n, int mode, vmp_obs_stats* ost, vmp_m
         mst[i].map_size = int32_t(m.size);
       }
       int32_t did = 0;
-      rc = maintain(si, &did);
+      rc = maintain(c, si, &did);
       if (mst) mst[i].maintenance = did;
       if (rc != VMP_OK && first_err == VMP_OK) first_err = rc;
     }
   }
   return first_err;
+}
+
+static int run_step(vmp_session** ss, int n, int mode, vmp_obs_stats* ost, vmp_map_stats* mst) {
+  if (!ss || n <= 0 || !ss[0]) return VMP_EINVAL;
+  vmp_ctx* c = ss[0]->ctx;
+  for (int i = 0; i < n; ++i)
+    if (!ss[i] || ss[i]->ctx != c) return VMP_EINVAL;
+  return run_step_on(c, ss, n, mode, ost, mst);
 }
 
 extern "C" int vmp_observe(vmp_session* s, const vmp_state* x, double* H, double* b, vmp_obs_stats* st) {
@@ -1053,3 +1109,6 @@ extern "C" int vmp_slots_export(vmp_session* s, int32_t n, uint8_t* valid, float
   }
   return VMP_OK;
 }
+
+// ------------------------------------------------------------ serving -----
+#include "vmp_serve.hpp"

@@ -35,6 +35,26 @@ __global__ void k_apply_motion(FilterDev* f, D33 dR, D3 dp, Diag23 q) {
   for (int i = 0; i < 23; ++i) f->P[i * 23 + i] += q.q[i];
 }
 
+// the same for many sessions: one thread per session
+struct MotionRec {
+  FilterDev* f;
+  double dR[9];  // column-major
+  double dp[3];
+  double q[23];
+};
+__global__ void k_apply_motion_batch(const MotionRec* recs, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const MotionRec& r = recs[i];
+  FilterDev* f = r.f;
+  const D33 rot = d33_from_cm(f->x.rot);
+  const D3 dp{{r.dp[0], r.dp[1], r.dp[2]}};
+  const D3 a = d33_mulv(rot, dp);
+  for (int k = 0; k < 3; ++k) f->x.pos[k] = f->x.pos[k] + a.v[k];
+  d33_to_cm(d33_mul(rot, d33_from_cm(r.dR)), f->x.rot);
+  for (int k = 0; k < 23; ++k) f->P[k * 23 + k] += r.q[k];
+}
+
 // zero the per-session observe counters before a pass
 __global__ void k_zero_lists(int32_t** lists, int n) {
   const int s = blockIdx.x;

@@ -309,3 +309,17 @@ def register_batch(registrars):
     if rc != 0:
         raise VmpError(f"vmp_register_batch failed ({rc})")
     return o, m
+
+
+def apply_motion_batch(registrars, dRs, dps, qdiag):
+    """Motion prior for many sessions of one context in one launch."""
+    lib = registrars[0].lib
+    n = len(registrars)
+    arr = (C.c_void_p * n)(*[r.h.value for r in registrars])
+    dR = np.ascontiguousarray(np.stack([np.asarray(R, np.float64).T.reshape(-1) for R in dRs]))  # column-major
+    dp = np.ascontiguousarray(np.stack([np.asarray(p, np.float64) for p in dps]))
+    q = np.asarray(qdiag, np.float64)
+    q = np.ascontiguousarray(np.broadcast_to(q, (n, 23)) if q.ndim == 1 else q)
+    rc = lib.vmp_state_apply_motion_batch(arr, n, _dptr(dR), _dptr(dp), _dptr(q))
+    if rc != 0:
+        raise VmpError(f"vmp_state_apply_motion_batch failed ({rc})")
