@@ -149,6 +149,104 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000):
     }
 
 
+def run_serving(sessions, rate, duration, nmax=120000, nmin=2000, capacity=200000, budget_gb=200, executors=2,
+                max_batch_sessions=128, seed=0x5E27E):
+    """Continuous batching: `sessions` sequences (room / street alternating),
+    Poisson scan arrivals at `rate` scans/s in aggregate for `duration` s,
+    scan sizes log-uniform in [nmin, nmax], maps and states resident in HBM
+    under `budget_gb`.  Every session's MAP_INIT scan is served first (all
+    arriving at once, not timed); the timed trace holds the registered scans.
+    A second replay of the same scans with every request arriving at t = 0
+    measures the serving loop's saturation throughput."""
+    import torch
+
+    from voxelmapplus_fastlio2_amd import LIOConfig, initial_covariance, state_vector
+    from voxelmapplus_fastlio2_amd.serving import Server, make_requests
+    from voxelmapplus_fastlio2_amd.synth import QDIAG, ROOM, SEED0, STREET, Sequence
+
+    rng = np.random.default_rng(seed)
+    t_arr = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * duration * 1.2) + 16))
+    t_arr = t_arr[t_arr < duration]
+    who = rng.integers(0, sessions, size=len(t_arr))
+    counts = np.bincount(who, minlength=sessions)
+    lo, hi = np.log(nmin), np.log(nmax)
+    seqs = [Sequence(ROOM if i % 2 == 0 else STREET, SEED0 + 5000 + i, nmax) for i in range(sessions)]
+    # per session: scan 0 (MAP_INIT) then one scan per arrival
+    sizes = [np.exp(rng.uniform(lo, hi, size=counts[i] + 1)).astype(np.int64) for i in range(sessions)]
+    total = int(sum(int(z.sum()) for z in sizes))
+    t0 = time.perf_counter()
+    host = np.empty((3, total), np.float32)
+    where = {}
+    off = 0
+    for i in range(sessions):
+        for k, n in enumerate(sizes[i]):
+            x, y, z = seqs[i].scan(k, int(n))
+            host[0, off : off + n], host[1, off : off + n], host[2, off : off + n] = x, y, z
+            where[(i, k)] = (off, int(n))
+            off += int(n)
+    dev = torch.from_numpy(host).to("cuda:0")
+    del host
+    base, stride = dev.data_ptr(), dev.stride(0) * 4
+    log(f"serving: generated {total} points ({len(t_arr)} timed scans, {sessions} sessions) in {time.perf_counter() - t0:.1f} s")
+
+    def item(i, k, arrival):
+        o, n = where[(i, k)]
+        it = {"session": i, "n": n, "x": base + 4 * o, "y": base + stride + 4 * o, "z": base + 2 * stride + 4 * o,
+              "arrival": float(arrival)}
+        if k > 0:
+            it["dR"], it["dp"] = seqs[i].motion(k)
+        return it
+
+    k_next = np.ones(sessions, np.int64)
+    timed = []
+    for t, i in zip(t_arr, who):
+        timed.append(item(int(i), int(k_next[i]), t))
+        k_next[i] += 1
+    cfg = LIOConfig(max_scan_points=nmax, map_capacity=capacity)
+    out = {}
+    with Server(cfg, sessions, executors=executors, max_batch_sessions=max_batch_sessions,
+                hbm_budget=int(budget_gb * (1 << 30))) as sv:
+        for mode in ("poisson", "saturation"):
+            for i in range(sessions):
+                R, t = seqs[i].pose(0)
+                sv.session_init(i, state_vector(R, t), initial_covariance())
+            init = make_requests([item(i, 0, 0.0) for i in range(sessions)])
+            sv.run(init, QDIAG)
+            reqs = make_requests(timed if mode == "poisson" else [dict(it, arrival=0.0) for it in timed])
+            torch.cuda.synchronize()
+            sv.run(reqs, QDIAG)
+            st = sv.stats()
+            arr = np.array([r.arrival for r in reqs])
+            done = np.array([r.done for r in reqs])
+            npts = np.array([r.n for r in reqs])
+            lat = (done - arr) * 1e3
+            span = done.max() - arr.min()
+            out[mode] = {
+                "value": float(npts.sum() / span),
+                "scans": len(reqs),
+                "scans_per_s": float(len(reqs) / span),
+                "latency_ms": {q: float(np.percentile(lat, p)) for q, p in (("p50", 50), ("p99", 99), ("p999", 99.9))},
+                "mean_batch_sessions": float(st["requests"] / max(st["batches"], 1)),
+                "batches": st["batches"],
+                "evictions": st["evictions"],
+                "resident_gb": st["resident_bytes"] / (1 << 30),
+            }
+            log(f"serving {mode}: {out[mode]}")
+    del dev
+    offered = {
+        "scans_per_s": rate,
+        "points_per_s": float(sum(r["n"] for r in timed) / duration),
+    }
+    return {
+        "unit": "points/s",
+        "workload": f"{sessions} sessions room/street alternating, Poisson {rate:g} scans/s for {duration:g} s, "
+        f"scan points log-U[{nmin},{nmax}], {budget_gb} GB HBM budget, {executors} executor streams",
+        "offered": offered,
+        "poisson": out["poisson"],
+        "saturation": out["saturation"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +262,10 @@ def main():
     ap.add_argument("--batch-warmup", type=int, default=3)
     ap.add_argument("--batch-capacity", type=int, default=200000)
     ap.add_argument("--obs-ppt", type=int, default=0, help="observe points per thread (0 = by scan size)")
+    ap.add_argument("--serve-sessions", type=int, default=512)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-executors", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -303,6 +405,10 @@ def main():
         bt["speedup_vs_plain"] = bt["value"] / value
         line["batched"] = bt
     ctx.close()
+    if args.serve_sessions > 0:
+        sv = run_serving(args.serve_sessions, args.serve_rate, args.serve_seconds, executors=args.serve_executors)
+        sv["saturation_vs_plain"] = sv["saturation"]["value"] / value
+        line["serving"] = sv
     if not args.no_cpu:
         log("timing the FP64 oracle on the host")
         line["cpu_baseline"] = cpu_baseline(host, motions, cfg, s0, P0, args.cpu_warm, args.cpu_timed)

@@ -299,7 +299,10 @@ extern "C" int vmp_server_session_init(vmp_server* sv, int32_t id, const vmp_sta
   s.has_state = true;
   s.built = false;
   s.seen = false;
-  return VMP_OK;
+  // resident at once while the budget has room (no eviction here)
+  int rc = VMP_OK;
+  if (sv->resident_bytes + sv->sess_bytes <= sv->cfg.hbm_budget && !serve_admit(sv, id, &rc)) return rc;
+  return rc;
 }
 
 extern "C" int vmp_server_run(vmp_server* sv, vmp_request* reqs, int32_t n, const double* qdiag) {
