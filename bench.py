@@ -127,8 +127,15 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000):
         ts = time.perf_counter()
         motion(k)
         bind(k)
-        register_batch(regs)
+        _, mst = register_batch(regs)
         lat.append(time.perf_counter() - ts)
+        if k == W + K:
+            ev = np.array([m.merge_evals for m in mst])
+            mg = np.array([m.merges for m in mst])
+            log(f"batched step {k}: merge evals room max {ev[0::2].max()} mean {ev[0::2].mean():.0f}, "
+                f"street max {ev[1::2].max()} mean {ev[1::2].mean():.0f}; merges room mean {mg[0::2].mean():.0f} "
+                f"street mean {mg[1::2].mean():.1f}; session with most evals: {int(ev.argmax())} "
+                f"({int(sizes[k, ev.argmax()])} points, {mg[ev.argmax()]} merges)")
         pts += int(sizes[k].sum())
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
@@ -266,6 +273,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-executors", type=int, default=2)
+    ap.add_argument("--serve-max-batch", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -406,7 +414,8 @@ def main():
         line["batched"] = bt
     ctx.close()
     if args.serve_sessions > 0:
-        sv = run_serving(args.serve_sessions, args.serve_rate, args.serve_seconds, executors=args.serve_executors)
+        sv = run_serving(args.serve_sessions, args.serve_rate, args.serve_seconds, executors=args.serve_executors,
+                         max_batch_sessions=args.serve_max_batch)
         sv["saturation_vs_plain"] = sv["saturation"]["value"] / value
         line["serving"] = sv
     if not args.no_cpu:

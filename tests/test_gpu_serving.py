@@ -35,7 +35,8 @@ def build_requests(seqs, bufs, nscans, rate, seed):
     t = 0.0
     for k in range(nscans):
         for i in rng.permutation(len(seqs)):
-            t += rng.exponential(1.0 / rate)
+            if k > 0:  # every session's MAP_INIT scan arrives at once, then Poisson
+                t += rng.exponential(1.0 / rate)
             dev, n = bufs[(int(i), k)]
             base, st = dev.data_ptr(), dev.stride(0) * 4
             it = {"session": int(i), "n": n, "x": base, "y": base + st, "z": base + 2 * st, "arrival": t}

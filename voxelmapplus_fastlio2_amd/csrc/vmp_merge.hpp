@@ -270,7 +270,17 @@ __device__ inline void heap_pop(uint64_t* heap, int hn) {
 }
 
 constexpr int kStageCap = 512;
+constexpr int kWin = 9;  // candidates evaluated together (9 x 7 records <= 64 lanes)
 
+// One wave per session replays, in point order, the candidates that act.
+// Windowed speculation with in-order commit: the next kWin candidates (the
+// static list merged with the heap of re-evaluations) have their seven
+// records (voxel + six eligible neighbours) loaded and their merge() run in
+// parallel, each on its own copy of the current state.  They are then
+// committed in point order; the first one that read a record an earlier
+// commit of the window changed, or that a re-evaluation created by such a
+// commit must precede, and every later one go back to the heap.  A committed
+// candidate therefore saw exactly the state the serial order gives it.
 __global__ void __launch_bounds__(64) k_merge_replay(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
   const int s = blockIdx.x;
   if (s >= nsess) return;
@@ -281,176 +291,244 @@ __global__ void __launch_bounds__(64) k_merge_replay(const MapDev* maps, const S
   __shared__ uint64_t heap[kHeapCap];
   __shared__ uint64_t stage[kStageCap];
   __shared__ int nstage;
-  __shared__ double pl[7][kPlaneD];
-  __shared__ uint64_t grp[7];
-  __shared__ int32_t nbv[7];
-  __shared__ int32_t clos[8];
+  __shared__ uint64_t skey[kWin];
+  __shared__ int spos[kWin];
+  __shared__ uint64_t wkey[kWin];
+  __shared__ int wpos[kWin];
+  __shared__ double pl[kWin][7][kPlaneD];
+  __shared__ uint64_t grp[kWin][7];
+  __shared__ int32_t nbv[kWin][7];
+  __shared__ int clos[kWin][8];
+  __shared__ int nclos[kWin];
+  __shared__ int32_t chg[kWin * 7];
   __shared__ uint64_t ckey[8];
-  __shared__ int nclos;
+  __shared__ uint64_t newmin;
+  __shared__ int sh_n, sh_sp, sh_hn;
+  __shared__ uint64_t sh_last, sh_last_before;
   const int lane = threadIdx.x;
-  int hn = 0, sp = si[s].spl_lo, evals = 0, merges = 0, err = 0;
+  int hn = 0, sp = si[s].spl_lo, evals = 0, merges = 0, err = 0, nchg = 0;
   uint64_t last = ~0ull;
   while (true) {
-    int pos = -1;
-    uint64_t st_key = ~0ull;
-    if (sp < spl_hi) {
-      pos = b.spl[sp];
-      st_key = b.cand_out[pos] & kCandLocal;
+    // ---- next static candidates (key and position in the candidate list)
+    if (lane < kWin) {
+      const int q = sp + lane;
+      if (q < spl_hi) {
+        const int pos = b.spl[q];
+        spos[lane] = pos;
+        skey[lane] = b.cand_out[pos] & kCandLocal;
+      } else {
+        skey[lane] = ~0ull;
+      }
     }
-    const uint64_t h_key = hn > 0 ? heap[0] : ~0ull;
-    if (st_key == ~0ull && h_key == ~0ull) break;
-    uint64_t cur;
-    bool from_static;
-    if (st_key <= h_key) {
-      cur = st_key;
-      from_static = true;
-      ++sp;
-    } else {
-      cur = h_key;
-      from_static = false;
-      __syncthreads();
-      if (lane == 0) heap_pop(heap, hn);
-      __syncthreads();
-      --hn;
+    __syncthreads();
+    // ---- the window: the smallest keys of the static stream and the heap
+    if (lane == 0) {
+      sh_last_before = last;
+      int used = 0, n = 0;
+      while (n < kWin) {
+        if (used == kWin && sp + used < spl_hi) break;  // next static key not loaded
+        const uint64_t st_key = used < kWin ? skey[used] : ~0ull;
+        const uint64_t h_key = hn > 0 ? heap[0] : ~0ull;
+        if (st_key == ~0ull && h_key == ~0ull) break;
+        uint64_t cur;
+        int pos;
+        if (st_key <= h_key) {
+          cur = st_key;
+          pos = spos[used];
+          ++used;
+        } else {
+          cur = h_key;
+          pos = -1;
+          heap_pop(heap, hn);
+          --hn;
+        }
+        if (cur == last) continue;
+        last = cur;
+        wkey[n] = cur;
+        wpos[n] = pos;
+        ++n;
+      }
+      sh_n = n;
+      sh_sp = sp + used;
+      sh_hn = hn;
+      newmin = ~0ull;
     }
-    if (cur == last) continue;
-    last = cur;
-    const int t = cand_t(cur);
-    const int32_t V = cand_vid(cur);
-    ++evals;
-    // ---- stage the six neighbours and the voxel itself in LDS
-    if (lane < 6) {
+    __syncthreads();
+    const int n = sh_n;
+    sp = sh_sp;
+    hn = sh_hn;
+    if (n == 0) break;
+    // ---- records of every window entry: lane w*7+j, j = 6 is the voxel
+    if (lane < n * 7) {
+      const int w = lane / 7, j = lane % 7;
+      const uint64_t cur = wkey[w];
+      const int t = cand_t(cur);
+      const int32_t V = cand_vid(cur);
       int32_t nv;
-      if (from_static) {
-        nv = b.cand_nbr[size_t(pos) * 6 + lane];
+      if (j == 6) {
+        nv = V;
+      } else if (wpos[w] >= 0) {
+        nv = b.cand_nbr[size_t(wpos[w]) * 6 + j];
       } else {
         const VoxState& vst = m.vs[V];
-        nv = (vst.tag == tag && vst.seg >= 0) ? eligible_at(m, b.seg_nbr[vst.seg * 6 + lane], tag, t)
-                                              : nbr_at(m, vst.key, lane, tag, t);
+        nv = (vst.tag == tag && vst.seg >= 0) ? eligible_at(m, b.seg_nbr[vst.seg * 6 + j], tag, t)
+                                              : nbr_at(m, vst.key, j, tag, t);
       }
-      nbv[lane] = nv;
+      nbv[w][j] = nv;
       if (nv >= 0) {
-        grp[lane] = m.vs[nv].group_id;
-        for (int k = 0; k < kPlaneD; ++k) pl[lane][k] = m.plane[size_t(nv) * kPlaneD + k];
+        grp[w][j] = m.vs[nv].group_id;
+        for (int k = 0; k < kPlaneD; ++k) pl[w][j][k] = m.plane[size_t(nv) * kPlaneD + k];
       }
-    } else if (lane == 6) {
-      nbv[6] = V;
-      grp[6] = m.vs[V].group_id;
-      for (int k = 0; k < kPlaneD; ++k) pl[6][k] = m.plane[size_t(V) * kPlaneD + k];
-    }
-    if (lane == 0) {
-      nclos = 0;
-      nstage = 0;
     }
     __syncthreads();
-    // ---- merge(): the six neighbours in order against the (updated) own plane
-    if (lane == 0) {
+    // ---- merge(): each entry's six neighbours in order against its own plane
+    if (lane < n) {
+      const int w = lane;
+      int nc = 0;
       for (int j = 0; j < 6; ++j) {
-        if (nbv[j] < 0 || grp[j] == grp[6]) continue;
-        if (!merge_gate(pl[6], pl[j], m.merge_a, m.merge_d)) continue;
-        merge_fuse(pl[6], pl[j]);
-        grp[j] = grp[6];
-        clos[nclos++] = j;
+        if (nbv[w][j] < 0 || grp[w][j] == grp[w][6]) continue;
+        if (!merge_gate(pl[w][6], pl[w][j], m.merge_a, m.merge_d)) continue;
+        merge_fuse(pl[w][6], pl[w][j]);
+        grp[w][j] = grp[w][6];
+        clos[w][nc++] = j;
       }
-      if (nclos > 0) clos[nclos++] = 6;
+      if (nc > 0) clos[w][nc++] = 6;
+      nclos[w] = nc;
     }
     __syncthreads();
-    const int nc0 = nclos;
-    if (nc0 == 0) continue;
-    // ---- write back the merged records
-    if (lane < nc0) {
-      const int j = clos[lane];
-      const int32_t X = nbv[j];
-      for (int k = 0; k < kPlaneD; ++k) m.plane[size_t(X) * kPlaneD + k] = pl[j][k];
-      m.vs[X].group_id = grp[j];
-      const uint32_t fl = m.vs[X].flags | kMerged;
-      m.vs[X].flags = fl;
-      write_hot(m, X, fl);
-    }
-    merges += nc0 - 1;
-    if (lane < nc0) ckey[lane] = m.vs[nbv[clos[lane]]].key;
-    __syncthreads();
-    // ---- taint the closed neighbourhoods of every changed voxel: each
-    //      touched one re-evaluates its next call and its later candidates.
-    //      A neighbour that did not change can only start merging with a
-    //      changed voxel (its other neighbours were already idle against its
-    //      unchanged plane), so it is re-evaluated only if it passes the
-    //      group test and the gates against one of the new records.
-    for (int q0 = 0; q0 < nc0 * 7; q0 += 64) {
-      const int qq = q0 + lane;
-      if (qq < nc0 * 7) {
-        const int32_t base_v = nbv[clos[qq / 7]];
-        const int j = qq % 7;
-        int32_t X = base_v;
-        if (j < 6) {
-          const VoxState& bs = m.vs[base_v];
-          int32_t cur_id;
-          if (bs.tag == tag && bs.seg >= 0) cur_id = b.seg_nbr[bs.seg * 6 + j];
-          else cur_id = key_neighbour_ok(bs.key, j) ? lookup_vid(m, key_neighbour(bs.key, j)) : -1;
-          X = vid_at(m, cur_id, tag, t);
-        }
-        bool need = false;
-        if (X >= 0) {
-          for (int k = 0; k < nc0; ++k) need |= (nbv[clos[k]] == X);
-          if (!need) {
-            const VoxState& xv = m.vs[X];
-            if ((xv.flags & kPlane) && (xv.flags & kInit)) {
-              double px[kPlaneD];
-              for (int k = 0; k < kPlaneD; ++k) px[k] = m.plane[size_t(X) * kPlaneD + k];
-              int64_t ax, ay, az;
-              unpack_key(xv.key, ax, ay, az);
-              for (int k = 0; k < nc0; ++k) {
-                int64_t bx, by, bz;
-                unpack_key(ckey[k], bx, by, bz);
-                const int64_t dist = (ax > bx ? ax - bx : bx - ax) + (ay > by ? ay - by : by - ay) +
-                                     (az > bz ? az - bz : bz - az);
-                const int c = clos[k];
-                if (dist == 1 && grp[c] != xv.group_id && merge_gate(px, pl[c], m.merge_a, m.merge_d)) need = true;
+    // ---- commit in point order
+    nchg = 0;
+    int w = 0;
+    for (; w < n; ++w) {
+      bool bad = false;
+      if (lane < 7) {
+        const int32_t v = nbv[w][lane];
+        if (v >= 0)
+          for (int k = 0; k < nchg; ++k) bad |= (chg[k] == v);
+      }
+      // a re-evaluation created in this window at or before this key runs first
+      if (__any(bad) || newmin <= wkey[w]) break;
+      ++evals;
+      const int nc0 = nclos[w];
+      if (nc0 == 0) continue;
+      const int t = cand_t(wkey[w]);
+      if (lane < nc0) {
+        const int j = clos[w][lane];
+        const int32_t X = nbv[w][j];
+        for (int k = 0; k < kPlaneD; ++k) m.plane[size_t(X) * kPlaneD + k] = pl[w][j][k];
+        m.vs[X].group_id = grp[w][j];
+        const uint32_t fl = m.vs[X].flags | kMerged;
+        m.vs[X].flags = fl;
+        write_hot(m, X, fl);
+        chg[nchg + lane] = X;
+        ckey[lane] = m.vs[X].key;
+      }
+      if (lane == 0) nstage = 0;
+      merges += nc0 - 1;
+      nchg += nc0;
+      __syncthreads();
+      // ---- taint the closed neighbourhoods of every changed voxel: each
+      //      touched one re-evaluates its next call and its later candidates.
+      //      A neighbour that did not change can only start merging with a
+      //      changed voxel (its other neighbours were already idle against its
+      //      unchanged plane), so it is re-evaluated only if it passes the
+      //      group test and the gates against one of the new records.
+      for (int q0 = 0; q0 < nc0 * 7; q0 += 64) {
+        const int qq = q0 + lane;
+        if (qq < nc0 * 7) {
+          const int32_t base_v = nbv[w][clos[w][qq / 7]];
+          const int j = qq % 7;
+          int32_t X = base_v;
+          if (j < 6) {
+            const VoxState& bs = m.vs[base_v];
+            int32_t cur_id;
+            if (bs.tag == tag && bs.seg >= 0) cur_id = b.seg_nbr[bs.seg * 6 + j];
+            else cur_id = key_neighbour_ok(bs.key, j) ? lookup_vid(m, key_neighbour(bs.key, j)) : -1;
+            X = vid_at(m, cur_id, tag, t);
+          }
+          bool need = false;
+          if (X >= 0) {
+            for (int k = 0; k < nc0; ++k) need |= (nbv[w][clos[w][k]] == X);
+            if (!need) {
+              const VoxState& xv = m.vs[X];
+              if ((xv.flags & kPlane) && (xv.flags & kInit)) {
+                double px[kPlaneD];
+                for (int k = 0; k < kPlaneD; ++k) px[k] = m.plane[size_t(X) * kPlaneD + k];
+                int64_t ax, ay, az;
+                unpack_key(xv.key, ax, ay, az);
+                for (int k = 0; k < nc0; ++k) {
+                  int64_t bx, by, bz;
+                  unpack_key(ckey[k], bx, by, bz);
+                  const int64_t dist = (ax > bx ? ax - bx : bx - ax) + (ay > by ? ay - by : by - ay) +
+                                       (az > bz ? az - bz : bz - az);
+                  const int cc = clos[w][k];
+                  if (dist == 1 && grp[w][cc] != xv.group_id && merge_gate(px, pl[w][cc], m.merge_a, m.merge_d))
+                    need = true;
+                }
               }
             }
           }
-        }
-        if (X >= 0 && need) {
-          VoxState& xs = m.vs[X];
-          if (xs.tag == tag && xs.seg >= 0 && xs.evict_t == kNever) {
-            if (xs.dirty_t < t) xs.dirty_t = t;
-            if (xs.taint_t > t) xs.taint_t = t;
-            uint64_t loc[8];
-            int nl = 0;
-            if (xs.merge_pos != kNever) {
-              const int tn = first_call_after(b, xs.seg, xs.merge_pos, t, off);
-              if (tn != kNever) loc[nl++] = cand_key(0, tn, X);
+          if (X >= 0 && need) {
+            VoxState& xs = m.vs[X];
+            if (xs.tag == tag && xs.seg >= 0 && xs.evict_t == kNever) {
+              if (xs.dirty_t < t) xs.dirty_t = t;
+              if (xs.taint_t > t) xs.taint_t = t;
+              uint64_t loc[8];
+              int nl = 0;
+              if (xs.merge_pos != kNever) {
+                const int tn = first_call_after(b, xs.seg, xs.merge_pos, t, off);
+                if (tn != kNever) loc[nl++] = cand_key(0, tn, X);
+              }
+              const int nct = b.seg_nct[xs.seg];
+              for (int k = 0; k < nct; ++k) {
+                const int tc = b.seg_ct[xs.seg * 7 + k];
+                if (tc > t) loc[nl++] = cand_key(0, tc, X);
+              }
+              if (nl > 0) {
+                const int at = atomicAdd(&nstage, nl);
+                for (int k = 0; k < nl; ++k)
+                  if (at + k < kStageCap) stage[at + k] = loc[k];
+              }
+            } else if (xs.flags & kAlive) {
+              atomicOr(&xs.flags, kDirty);
             }
-            const int nct = b.seg_nct[xs.seg];
-            for (int k = 0; k < nct; ++k) {
-              const int tc = b.seg_ct[xs.seg * 7 + k];
-              if (tc > t) loc[nl++] = cand_key(0, tc, X);
-            }
-            if (nl > 0) {
-              const int at = atomicAdd(&nstage, nl);
-              for (int k = 0; k < nl; ++k)
-                if (at + k < kStageCap) stage[at + k] = loc[k];
-            }
-          } else if (xs.flags & kAlive) {
-            atomicOr(&xs.flags, kDirty);
           }
         }
       }
+      __syncthreads();
+      if (lane == 0) {
+        const int ns = nstage < kStageCap ? nstage : kStageCap;
+        if (nstage > kStageCap || hn + ns > kHeapCap) {
+          err |= 16;
+        } else {
+          for (int k = 0; k < ns; ++k) {
+            heap_push(heap, hn + k, stage[k]);
+            if (stage[k] < newmin) newmin = stage[k];
+          }
+        }
+        sh_hn = err ? hn : hn + ns;
+      }
+      __syncthreads();
+      hn = sh_hn;
+      err = __shfl(err, 0, 64);
+      if (err) break;
     }
-    __syncthreads();
+    if (err) break;
+    // ---- entries not committed go back to the heap for re-evaluation
     if (lane == 0) {
-      const int ns = nstage < kStageCap ? nstage : kStageCap;
-      if (nstage > kStageCap || hn + ns > kHeapCap) {
+      if (hn + (n - w) > kHeapCap) {
         err |= 16;
       } else {
-        for (int k = 0; k < ns; ++k) heap_push(heap, hn + k, stage[k]);
+        for (int k = w; k < n; ++k) heap_push(heap, hn++, wkey[k]);
       }
+      // duplicates of a pushed-back key still in the sources must not be
+      // skipped against it: the last processed key is the last committed one
+      sh_last = w > 0 ? wkey[w - 1] : sh_last_before;
+      sh_hn = hn;
     }
     __syncthreads();
-    {
-      const int ns = nstage < kStageCap ? nstage : kStageCap;
-      if (hn + ns <= kHeapCap) hn += ns;
-    }
+    hn = sh_hn;
+    last = sh_last;
     err = __shfl(err, 0, 64);
     if (err) break;
   }
