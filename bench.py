@@ -64,9 +64,14 @@ def cpu_baseline(scans, motions, cfg, s0, P0, n_warm, n_timed):
     }
 
 
-def run_batched(ctx, B, W, K, capacity, nmax=120000):
+def run_batched(ctx, B, W, K, capacity, nmax=120000, groups=1):
     """B independent sequences (room / street alternating), per-scan sizes
-    uniform in [2k, nmax], advanced together by vmp_register_batch."""
+    uniform in [2k, nmax], advanced together by vmp_register_batch.  With
+    groups > 1 the sessions are split round-robin into that many groups, each
+    on its own context (HIP stream) and host thread, so one group's
+    latency-bound stages overlap another group's bandwidth-bound ones."""
+    import threading
+
     import torch
 
     from voxelmapplus_fastlio2_amd import (
@@ -97,55 +102,77 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000):
     motions = [[None] * B] + [[seqs[i].motion(k) for i in range(B)] for k in range(1, total)]
     log(f"batched: generated {total} steps x {B} sessions ({int(sizes.sum())} points) in {time.perf_counter() - t0:.1f} s")
     cfg = LIOConfig(max_scan_points=nmax, map_capacity=capacity)
-    regs = [Registrar(ctx, cfg) for _ in range(B)]
+    from voxelmapplus_fastlio2_amd import Context
 
-    def bind(k):
+    ctxs = [ctx] + [Context(0) for _ in range(groups - 1)]
+    regs = [Registrar(ctxs[i % groups], cfg) for i in range(B)]
+    members = [list(range(gi, B, groups)) for gi in range(groups)]
+
+    def bind(k, idx):
         dev, offs = steps[k]
         base = dev.data_ptr()
         stride = dev.stride(0) * 4
-        for i, g in enumerate(regs):
+        for i in idx:
             o = int(offs[i]) * 4
-            g.set_points_device(base + o, base + stride + o, base + 2 * stride + o, int(sizes[k, i]))
+            regs[i].set_points_device(base + o, base + stride + o, base + 2 * stride + o, int(sizes[k, i]))
 
-    bind(0)
+    bind(0, range(B))
     for i, g in enumerate(regs):
         R, t = seqs[i].pose(0)
         g.set_state(state_vector(R, t), initial_covariance())
         g.build()
-    def motion(k):
-        apply_motion_batch(regs, [m[0] for m in motions[k]], [m[1] for m in motions[k]], QDIAG)
+
+    def step(k, idx):
+        rg = [regs[i] for i in idx]
+        apply_motion_batch(rg, [motions[k][i][0] for i in idx], [motions[k][i][1] for i in idx], QDIAG)
+        bind(k, idx)
+        return register_batch(rg)
 
     for k in range(1, W + 1):
-        motion(k)
-        bind(k)
-        register_batch(regs)
+        for idx in members:
+            step(k, idx)
     torch.cuda.synchronize()
-    lat = []
-    pts = 0
+    lat = [[] for _ in members]
+    last_mst = [None] * groups
+
+    def run_group(gi):
+        for k in range(W + 1, W + K + 1):
+            ts = time.perf_counter()
+            _, mst = step(k, members[gi])
+            lat[gi].append(time.perf_counter() - ts)
+            last_mst[gi] = mst
+
     t_start = time.perf_counter()
-    for k in range(W + 1, W + K + 1):
-        ts = time.perf_counter()
-        motion(k)
-        bind(k)
-        _, mst = register_batch(regs)
-        lat.append(time.perf_counter() - ts)
-        if k == W + K:
-            ev = np.array([m.merge_evals for m in mst])
-            mg = np.array([m.merges for m in mst])
-            log(f"batched step {k}: merge evals room max {ev[0::2].max()} mean {ev[0::2].mean():.0f}, "
-                f"street max {ev[1::2].max()} mean {ev[1::2].mean():.0f}; merges room mean {mg[0::2].mean():.0f} "
-                f"street mean {mg[1::2].mean():.1f}; session with most evals: {int(ev.argmax())} "
-                f"({int(sizes[k, ev.argmax()])} points, {mg[ev.argmax()]} merges)")
-        pts += int(sizes[k].sum())
+    if groups == 1:
+        run_group(0)
+    else:
+        th = [threading.Thread(target=run_group, args=(gi,)) for gi in range(groups)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
+    pts = int(sizes[W + 1 : W + K + 1].sum())
+    ev = np.zeros(B, np.int64)
+    mg = np.zeros(B, np.int64)
+    for gi, idx in enumerate(members):
+        for j, i in enumerate(idx):
+            ev[i], mg[i] = last_mst[gi][j].merge_evals, last_mst[gi][j].merges
+    log(f"batched last step: merge evals room max {ev[0::2].max()} mean {ev[0::2].mean():.0f}, "
+        f"street max {ev[1::2].max()} mean {ev[1::2].mean():.0f}; merges room mean {mg[0::2].mean():.0f} "
+        f"street mean {mg[1::2].mean():.1f}")
+    lat = [x for grp in lat for x in grp]
     for g in regs:
         g.close()
+    for c in ctxs[1:]:
+        c.close()
     lat_ms = np.array(lat) * 1e3
     return {
         "value": pts / elapsed,
         "unit": "points/s",
         "sessions": B,
+        "groups": groups,
         "steps": K,
         "warmup": W,
         "points_per_step_mean": pts / K,
@@ -268,6 +295,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--batch-warmup", type=int, default=3)
     ap.add_argument("--batch-capacity", type=int, default=200000)
+    ap.add_argument("--batch-groups", type=int, default=1, help="session groups on separate streams")
     ap.add_argument("--obs-ppt", type=int, default=0, help="observe points per thread (0 = by scan size)")
     ap.add_argument("--serve-sessions", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
@@ -409,7 +437,8 @@ def main():
     g.close()
     del dev
     if args.batch_sessions > 0:
-        bt = run_batched(ctx, args.batch_sessions, args.batch_warmup, args.batch_steps, args.batch_capacity)
+        bt = run_batched(ctx, args.batch_sessions, args.batch_warmup, args.batch_steps, args.batch_capacity,
+                         groups=args.batch_groups)
         bt["speedup_vs_plain"] = bt["value"] / value
         line["batched"] = bt
     ctx.close()

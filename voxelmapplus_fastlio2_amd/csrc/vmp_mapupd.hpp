@@ -793,42 +793,74 @@ __global__ void __launch_bounds__(kMapThreads) k_replay(const MapDev* maps, cons
     int n = st.n, newly = st.newly_add;
     int merge_pos = kNever, freeze_t = -1;
     bool moved = false;
-    for (int pos = start; pos < end; ++pos) {
-      if ((flags & kInit) && !(flags & kEnable) && !S.is_build) {
-        if (flags & kPlane) {
-          merge_pos = pos - start;
-          calls += end - pos;
-        } else {
-          dropped += end - pos;
+    // Points are replayed in rounds: each lane advances to its next refit
+    // (or the end of its segment), then the lanes that reached one refit
+    // together, so the wave runs the FP64 refit once per round instead of
+    // whenever any lane needs it.  Per lane the order of operations is the
+    // reference's: add the point, refit, then the freeze check.
+    int pos = start;
+    bool active = true;
+    // prefetch of the next point (its index, then its coordinates)
+    int i_nx = pos < end ? b.sval_out[pos] - S.off : 0;
+    float x_nx = 0.f, y_nx = 0.f, z_nx = 0.f;
+    if (pos < end) {
+      x_nx = S.x[i_nx];
+      y_nx = S.y[i_nx];
+      z_nx = S.z[i_nx];
+    }
+    while (__any(active)) {
+      bool need = false;
+      while (active && !need) {
+        if (pos >= end) {
+          active = false;
+          break;
         }
-        break;
+        if ((flags & kInit) && !(flags & kEnable) && !S.is_build) {
+          if (flags & kPlane) {
+            merge_pos = pos - start;
+            calls += end - pos;
+          } else {
+            dropped += end - pos;
+          }
+          active = false;
+          break;
+        }
+        const int i = i_nx;
+        const float px = x_nx, py = y_nx, pz0 = z_nx;
+        if (pos + 1 < end) {
+          i_nx = b.sval_out[pos + 1] - S.off;
+          x_nx = S.x[i_nx];
+          y_nx = S.y[i_nx];
+          z_nx = S.z[i_nx];
+        }
+        ++pos;
+        float pw[3];
+        ins_transform(P.m, P.t, px, py, pz0, pw);
+        const float qv[3] = {float(double(pw[0]) - center[0]), float(double(pw[1]) - center[1]),
+                             float(double(pw[2]) - center[2])};
+        const float pz = pz0 == 0.0f ? 0.001f : pz0;
+        float C[6];
+        point_world_cov(P.rwl32, P.Prr, P.Ppp, P.s2, P.rv, px, py, pz, C);
+        mom_add(M, qv, C);
+        ++n;
+        moved = true;
+        if (S.is_build) continue;
+        if (!(flags & kInit)) {
+          // updatePlane after every point until the voxel initialises
+          need = n >= m.upd_pt;
+          continue;
+        }
+        ++newly;
+        if (newly >= m.upd_pt) {
+          need = true;
+          newly = 0;
+        }
+        if (n >= m.max_pt) {
+          flags &= ~kEnable;
+          freeze_t = i;
+        }
       }
-      const int i = b.sval_out[pos] - S.off;
-      const float px = S.x[i], py = S.y[i], pz0 = S.z[i];
-      float pw[3];
-      ins_transform(P.m, P.t, px, py, pz0, pw);
-      const float qv[3] = {float(double(pw[0]) - center[0]), float(double(pw[1]) - center[1]),
-                           float(double(pw[2]) - center[2])};
-      const float pz = pz0 == 0.0f ? 0.001f : pz0;
-      float C[6];
-      point_world_cov(P.rwl32, P.Prr, P.Ppp, P.s2, P.rv, px, py, pz, C);
-      mom_add(M, qv, C);
-      ++n;
-      moved = true;
-      if (S.is_build) continue;
-      if (!(flags & kInit)) {
-        refit(m, M, n, flags, plv, center, refits);
-        continue;
-      }
-      ++newly;
-      if (newly >= m.upd_pt) {
-        refit(m, M, n, flags, plv, center, refits);
-        newly = 0;
-      }
-      if (n >= m.max_pt) {
-        flags &= ~kEnable;
-        freeze_t = i;
-      }
+      if (need) refit(m, M, n, flags, plv, center, refits);
     }
     if (S.is_build) refit(m, M, n, flags, plv, center, refits);
     if (moved) {
