@@ -687,9 +687,12 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
       hipLaunchKernelGGL(k_create_fast_ctr, dim3(n), dim3(1), 0, st, M, n);
       hipLaunchKernelGGL(k_create_evict, dim3(n), dim3(64), 0, st, M, S, n, b);
       hipLaunchKernelGGL(k_create_init, dim3(gn), dim3(T), 0, st, M, S, n, b);
-      hipLaunchKernelGGL(k_seg_lenkeys, dim3(gn), dim3(T), 0, st, b);
+      // segment lengths fit in lbits bits (a segment holds at most ntot points)
+      int lbits = 1;
+      while ((int64_t(1) << lbits) <= int64_t(ntot)) ++lbits;
+      hipLaunchKernelGGL(k_seg_lenkeys, dim3(gn), dim3(T), 0, st, b, lbits);
       tb = w.temp_bytes;
-      if (rocprim::radix_sort_pairs(w.temp, tb, b.len_in, b.len_out, b.ord_in, b.ord, size_t(ntot), 0, 32, st) !=
+      if (rocprim::radix_sort_pairs(w.temp, tb, b.len_in, b.len_out, b.ord_in, b.ord, size_t(ntot), 0, lbits, st) !=
           hipSuccess)
         return VMP_EDEVICE;
       hipLaunchKernelGGL(k_replay, dim3(gn), dim3(T), 0, st, M, S, b);
@@ -698,8 +701,10 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         hipLaunchKernelGGL(k_candidates, dim3(gn), dim3(T), 0, st, M, S, b);
         hipLaunchKernelGGL(k_dirty_untouched, dim3(gn), dim3(T), 0, st, M, b);
         tb = w.temp_bytes;
+        // sorted on (session, point time) only: the call at point time t is
+        // made by the voxel that received point t, so t fixes the voxel bits;
         // unused slots hold ~0 and sort last
-        if (rocprim::radix_sort_keys(w.temp, tb, b.cand_in, b.cand_out, size_t(ntot), 0, 49 + sbits, st) !=
+        if (rocprim::radix_sort_keys(w.temp, tb, b.cand_in, b.cand_out, size_t(ntot), 25, 49 + sbits, st) !=
             hipSuccess)
           return VMP_EDEVICE;
         hipLaunchKernelGGL(k_cand_starts, dim3(1), dim3(1), 0, st, M, n, b);

@@ -747,11 +747,13 @@ __device__ inline void refit(const MapDev& m, Moments& M, int n, uint32_t& flags
 }
 
 // sort keys that put long segments first (waves of the replay then walk
-// segments of similar length); slots past the segment count sort last
-__global__ void k_seg_lenkeys(BatchBufs b) {
+// segments of similar length); slots past the segment count sort last.
+// Lengths are below 2^lbits, so the keys use the low lbits bits only.
+__global__ void k_seg_lenkeys(BatchBufs b, int lbits) {
   const int nseg = b.cnt[0];
+  const uint32_t top = uint32_t((uint64_t(1) << lbits) - 1);
   VMP_GRID_LOOP(q, b.ntot) {
-    b.len_in[q] = q < nseg ? 0xFFFFFFFFu - uint32_t(b.seg_end[q] - b.seg_start[q]) : 0xFFFFFFFFu;
+    b.len_in[q] = q < nseg ? top - uint32_t(b.seg_end[q] - b.seg_start[q]) : top;
     b.ord_in[q] = q;
   }
 }

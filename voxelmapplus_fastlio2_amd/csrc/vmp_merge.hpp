@@ -38,16 +38,34 @@ __device__ inline int32_t vid_at(const MapDev& m, int32_t vid, int tag, int t) {
   return vid;
 }
 
-// first merge() call of segment q strictly after local point t (kNever if none)
+// first merge() call of segment q strictly after local point t (kNever if none).
+// The segment's point times ascend; the search narrows the range by sixteen
+// independent samples per step (one memory round trip each) instead of a
+// binary search's chain of dependent loads.
 __device__ inline int first_call_after(const BatchBufs& b, int q, int merge_pos, int t, int off) {
   const int end = b.seg_end[q];
-  int lo = b.seg_start[q] + merge_pos, hi = end;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (b.sval_out[mid] - off > t) hi = mid;
-    else lo = mid + 1;
+  int lo = b.seg_start[q] + merge_pos, hi = end;  // answer in [lo, hi], hi = none
+  const int tg = t + off;                          // compare global indices
+  while (hi - lo > 16) {
+    const int step = (hi - lo + 15) >> 4;
+    int v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = b.sval_out[min(lo + (k + 1) * step - 1, hi - 1)];
+    int k = 0;
+    while (k < 16 && v[k] <= tg) ++k;
+    if (k == 16) return kNever;  // the last sample is hi - 1
+    const int nhi = min(lo + (k + 1) * step - 1, hi - 1);
+    lo = k == 0 ? lo : lo + k * step;
+    hi = nhi;  // sval_out[nhi] > tg: the answer is at most nhi
+    if (lo >= hi) return b.sval_out[hi] - off;
+    ++hi;      // search [lo, nhi] inclusive as the half-open [lo, nhi + 1)
   }
-  return lo < end ? b.sval_out[lo] - off : kNever;
+  int v[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = lo + k < hi ? b.sval_out[lo + k] : 0x7fffffff;
+  for (int k = 0; k < 16; ++k)
+    if (lo + k < hi && v[k] > tg) return v[k] - off;
+  return kNever;
 }
 
 __device__ inline int32_t lookup_vid(const MapDev& m, uint64_t key) {
