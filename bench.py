@@ -156,12 +156,17 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000, groups=1):
     pts = int(sizes[W + 1 : W + K + 1].sum())
     ev = np.zeros(B, np.int64)
     mg = np.zeros(B, np.int64)
+    wn = np.zeros(B, np.int64)
+    rq = np.zeros(B, np.int64)
     for gi, idx in enumerate(members):
         for j, i in enumerate(idx):
-            ev[i], mg[i] = last_mst[gi][j].merge_evals, last_mst[gi][j].merges
+            m = last_mst[gi][j]
+            ev[i], mg[i], wn[i], rq[i] = m.merge_evals, m.merges, m.merge_windows, m.merge_requeued
+    h = int(ev.argmax())
     log(f"batched last step: merge evals room max {ev[0::2].max()} mean {ev[0::2].mean():.0f}, "
         f"street max {ev[1::2].max()} mean {ev[1::2].mean():.0f}; merges room mean {mg[0::2].mean():.0f} "
-        f"street mean {mg[1::2].mean():.1f}")
+        f"street mean {mg[1::2].mean():.1f}; heaviest session: {ev[h]} evals, {mg[h]} merges, {wn[h]} windows, "
+        f"{rq[h]} requeued")
     lat = [x for grp in lat for x in grp]
     for g in regs:
         g.close()

@@ -94,6 +94,8 @@ typedef struct {
   int32_t map_size;
   int32_t maintenance; /* housekeeping after this scan: 1 = hash table
                           rebuilt, 2 = LRU ring compacted (bit set) */
+  int32_t merge_windows;  /* evaluation windows of the merge replay */
+  int32_t merge_requeued; /* window entries re-evaluated after a conflict */
 } vmp_map_stats;
 
 typedef struct vmp_ctx vmp_ctx;

@@ -65,6 +65,8 @@ class MapStats(C.Structure):
             "dropped",
             "map_size",
             "maintenance",
+            "merge_windows",
+            "merge_requeued",
         )
     ]
 

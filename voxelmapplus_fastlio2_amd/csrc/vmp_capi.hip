@@ -715,7 +715,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
                             size_t(ntot), st) != hipSuccess)
           return VMP_EDEVICE;
         hipLaunchKernelGGL(k_spl_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b);
-        hipLaunchKernelGGL(k_merge_replay, dim3(n), dim3(64), 0, st, M, S, n, b);
+        hipLaunchKernelGGL(k_merge_replay, dim3(n), dim3(kMrThreads), 0, st, M, S, n, b);
       }
       hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.flags, 0, ntot);
       hipLaunchKernelGGL(k_finalize_segs, dim3(gn), dim3(T), 0, st, M, S, b);
@@ -912,6 +912,8 @@ static int run_step_on(vmp_ctx* c, vmp_session** ss, int n, int mode, vmp_obs_st
         mst[i].refits = m.refits;
         mst[i].merge_calls = m.merge_calls;
         mst[i].merge_evals = m.merge_evals;
+        mst[i].merge_windows = m.merge_windows;
+        mst[i].merge_requeued = m.merge_requeued;
         mst[i].merges = m.merges;
         mst[i].dropped = m.dropped;
         mst[i].map_size = int32_t(m.size);

@@ -254,63 +254,60 @@ __global__ void k_spl_ranges(const MapDev* maps, SessInfo* si, int nsess, BatchB
   si[s].spl_hi = lb(c0 + maps[s].ctr->ncand);
 }
 
-__device__ inline void heap_push(uint64_t* heap, int hn, uint64_t v) {
-  int i = hn;
-  while (i > 0) {
-    const int p = (i - 1) >> 1;
-    if (heap[p] <= v) break;
-    heap[i] = heap[p];
-    i = p;
+constexpr int kMrThreads = 256;  // threads of one session's merge replay
+constexpr int kWin = 36;         // window entries (36 x 7 records <= 256 threads)
+constexpr int kSPre = 2 * kWin;  // static candidates loaded per window
+constexpr int kPoolCap = 2048;   // pending re-evaluations, kept sorted
+constexpr int kNewCap = 2048;    // keys created by one window's taint phase
+constexpr int kFxCap = 2048;     // deferred taint side effects of one window
+constexpr int kTaintPer = 49;    // up to 7 changed voxels x their closed neighbourhood
+
+__device__ inline int rank_lt(const uint64_t* a, int n, uint64_t v) {  // elements < v
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
   }
-  heap[i] = v;
+  return lo;
 }
-__device__ inline void heap_pop(uint64_t* heap, int hn) {
-  const uint64_t lastv = heap[hn - 1];
-  const int nn = hn - 1;
-  int i = 0;
-  while (true) {
-    const int l = 2 * i + 1, r = l + 1;
-    int sm = i;
-    uint64_t sv = lastv;
-    if (l < nn && heap[l] < sv) {
-      sm = l;
-      sv = heap[l];
-    }
-    if (r < nn && heap[r] < sv) {
-      sm = r;
-      sv = heap[r];
-    }
-    if (sm == i) break;
-    heap[i] = heap[sm];
-    i = sm;
+__device__ inline int rank_le(const uint64_t* a, int n, uint64_t v) {  // elements <= v
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] <= v) lo = mid + 1;
+    else hi = mid;
   }
-  if (nn > 0) heap[i] = lastv;
+  return lo;
 }
 
-constexpr int kStageCap = 512;
-constexpr int kWin = 9;  // candidates evaluated together (9 x 7 records <= 64 lanes)
-
-// One wave per session replays, in point order, the candidates that act.
-// Windowed speculation with in-order commit: the next kWin candidates (the
-// static list merged with the heap of re-evaluations) have their seven
-// records (voxel + six eligible neighbours) loaded and their merge() run in
-// parallel, each on its own copy of the current state.  They are then
-// committed in point order; the first one that read a record an earlier
-// commit of the window changed, or that a re-evaluation created by such a
-// commit must precede, and every later one go back to the heap.  A committed
-// candidate therefore saw exactly the state the serial order gives it.
-__global__ void __launch_bounds__(64) k_merge_replay(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
+// One workgroup per session replays, in point order, the merge() calls that
+// act.  Windowed speculation with in-order commit:
+//   fill     the next kWin keys of the static list (acting candidates) merged
+//            with the sorted pool of re-evaluations, duplicates dropped;
+//   records  each entry's voxel and six eligible neighbours (group, plane);
+//   evaluate each entry's merge() on its own copy of the current state;
+//   taint    for every acting entry, in parallel, the re-evaluations its
+//            merges make necessary (new keys and the voxels' dirty/taint
+//            marks, held back until the entry commits);
+//   commit   in point order.  An entry whose records, or (if it acts) the
+//            records its taint phase read, were changed by an earlier commit
+//            of the window, or that a new key must precede, stops the commit:
+//            it and every later entry return to the pool.
+// A committed entry therefore saw exactly the state the serial order gives
+// it, and its taint phase read the same records a serial one would.
+__global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps, const SessInfo* si, int nsess,
+                                                             BatchBufs b) {
   const int s = blockIdx.x;
   if (s >= nsess) return;
   const MapDev& m = maps[s];
   MapCounters* c = m.ctr;
   const int tag = c->tag, off = si[s].off;
   const int spl_hi = si[s].spl_hi;
-  __shared__ uint64_t heap[kHeapCap];
-  __shared__ uint64_t stage[kStageCap];
-  __shared__ int nstage;
-  __shared__ uint64_t skey[kWin];
-  __shared__ int spos[kWin];
+  const int tid = threadIdx.x;
+  __shared__ uint64_t pool[2][kPoolCap];
+  __shared__ uint64_t skey[kSPre];
+  __shared__ int spos[kSPre];
   __shared__ uint64_t wkey[kWin];
   __shared__ int wpos[kWin];
   __shared__ double pl[kWin][7][kPlaneD];
@@ -318,67 +315,72 @@ __global__ void __launch_bounds__(64) k_merge_replay(const MapDev* maps, const S
   __shared__ int32_t nbv[kWin][7];
   __shared__ int clos[kWin][8];
   __shared__ int nclos[kWin];
+  __shared__ int act[kWin];
+  __shared__ int32_t r2[kWin][kTaintPer];
   __shared__ int32_t chg[kWin * 7];
-  __shared__ uint64_t ckey[8];
-  __shared__ uint64_t newmin;
-  __shared__ int sh_n, sh_sp, sh_hn;
-  __shared__ uint64_t sh_last, sh_last_before;
-  const int lane = threadIdx.x;
-  int hn = 0, sp = si[s].spl_lo, evals = 0, merges = 0, err = 0, nchg = 0;
+  __shared__ uint64_t nkey[kNewCap];
+  __shared__ int nkown[kNewCap];
+  __shared__ int32_t fx_v[kFxCap];
+  __shared__ int fx_wk[kFxCap];  // owner entry * 2 + kind (0 in-scan taint, 1 dirty flag)
+  __shared__ uint64_t gbuf[kNewCap + kWin];
+  __shared__ uint64_t sbuf[kNewCap + kWin];
+  __shared__ int sh_i[8];
+  __shared__ uint64_t sh_u[2];
+  int sp = si[s].spl_lo, pb = 0, ph = 0, pn = 0;
+  int evals = 0, merges = 0, windows = 0, requeued = 0, err = 0;
   uint64_t last = ~0ull;
   while (true) {
-    // ---- next static candidates (key and position in the candidate list)
-    if (lane < kWin) {
-      const int q = sp + lane;
+    // ---- fill the window
+    if (tid < kSPre) {
+      const int q = sp + tid;
       if (q < spl_hi) {
         const int pos = b.spl[q];
-        spos[lane] = pos;
-        skey[lane] = b.cand_out[pos] & kCandLocal;
+        spos[tid] = pos;
+        skey[tid] = b.cand_out[pos] & kCandLocal;
       } else {
-        skey[lane] = ~0ull;
+        skey[tid] = ~0ull;
       }
     }
     __syncthreads();
-    // ---- the window: the smallest keys of the static stream and the heap
-    if (lane == 0) {
-      sh_last_before = last;
-      int used = 0, n = 0;
-      while (n < kWin) {
-        if (used == kWin && sp + used < spl_hi) break;  // next static key not loaded
-        const uint64_t st_key = used < kWin ? skey[used] : ~0ull;
-        const uint64_t h_key = hn > 0 ? heap[0] : ~0ull;
-        if (st_key == ~0ull && h_key == ~0ull) break;
+    if (tid == 0) {
+      const bool more_static = sp + kSPre < spl_hi;
+      int used = 0, taken = 0, pi = ph;
+      uint64_t lst = last;
+      while (taken < kWin) {
+        if (used == kSPre && more_static) break;  // the next static key is not loaded
+        const uint64_t st_key = used < kSPre ? skey[used] : ~0ull;
+        const uint64_t p_key = pi < pn ? pool[pb][pi] : ~0ull;
+        if (st_key == ~0ull && p_key == ~0ull) break;
         uint64_t cur;
         int pos;
-        if (st_key <= h_key) {
+        if (st_key <= p_key) {
           cur = st_key;
           pos = spos[used];
           ++used;
         } else {
-          cur = h_key;
+          cur = p_key;
           pos = -1;
-          heap_pop(heap, hn);
-          --hn;
+          ++pi;
         }
-        if (cur == last) continue;
-        last = cur;
-        wkey[n] = cur;
-        wpos[n] = pos;
-        ++n;
+        if (cur == lst) continue;
+        lst = cur;
+        wkey[taken] = cur;
+        wpos[taken] = pos;
+        ++taken;
       }
-      sh_n = n;
-      sh_sp = sp + used;
-      sh_hn = hn;
-      newmin = ~0ull;
+      sh_i[0] = taken;
+      sh_i[1] = used;
+      sh_i[2] = pi;
     }
     __syncthreads();
-    const int n = sh_n;
-    sp = sh_sp;
-    hn = sh_hn;
+    const int n = sh_i[0];
+    sp += sh_i[1];
+    ph = sh_i[2];
     if (n == 0) break;
-    // ---- records of every window entry: lane w*7+j, j = 6 is the voxel
-    if (lane < n * 7) {
-      const int w = lane / 7, j = lane % 7;
+    ++windows;
+    // ---- records: thread w*7+j, j = 6 is the voxel itself
+    if (tid < n * 7) {
+      const int w = tid / 7, j = tid % 7;
       const uint64_t cur = wkey[w];
       const int t = cand_t(cur);
       const int32_t V = cand_vid(cur);
@@ -400,8 +402,8 @@ __global__ void __launch_bounds__(64) k_merge_replay(const MapDev* maps, const S
     }
     __syncthreads();
     // ---- merge(): each entry's six neighbours in order against its own plane
-    if (lane < n) {
-      const int w = lane;
+    if (tid < n) {
+      const int w = tid;
       int nc = 0;
       for (int j = 0; j < 6; ++j) {
         if (nbv[w][j] < 0 || grp[w][j] == grp[w][6]) continue;
@@ -414,145 +416,198 @@ __global__ void __launch_bounds__(64) k_merge_replay(const MapDev* maps, const S
       nclos[w] = nc;
     }
     __syncthreads();
+    if (tid == 0) {
+      int na = 0;
+      for (int w = 0; w < n; ++w)
+        if (nclos[w] > 0) act[na++] = w;
+      sh_i[3] = na;
+      sh_i[4] = 0;  // new keys
+      sh_i[5] = 0;  // side effects
+      sh_u[0] = ~0ull;
+    }
+    __syncthreads();
+    const int na = sh_i[3];
+    // ---- taint phase of every acting entry: for each changed voxel and each
+    //      voxel of its closed neighbourhood X: X re-evaluates its next call
+    //      and its later candidates if X changed itself, or if X (a plane not
+    //      in the group) passes the gates against a new record.  A voxel of
+    //      the scan records its dirty/taint times, one outside is marked dirty.
+    for (int T = tid; T < na * kTaintPer; T += kMrThreads) {
+      const int a = T / kTaintPer, r = T % kTaintPer, ci = r / 7, j = r % 7;
+      const int w = act[a];
+      const int nc0 = nclos[w];
+      int32_t X = -1;
+      if (ci < nc0) {
+        const int t = cand_t(wkey[w]);
+        const int32_t base_v = nbv[w][clos[w][ci]];
+        X = base_v;
+        if (j < 6) {
+          const VoxState& bs = m.vs[base_v];
+          int32_t cur_id;
+          if (bs.tag == tag && bs.seg >= 0) cur_id = b.seg_nbr[bs.seg * 6 + j];
+          else cur_id = key_neighbour_ok(bs.key, j) ? lookup_vid(m, key_neighbour(bs.key, j)) : -1;
+          X = vid_at(m, cur_id, tag, t);
+        }
+        bool need = false;
+        if (X >= 0) {
+          for (int k = 0; k < nc0; ++k) need |= (nbv[w][clos[w][k]] == X);
+          if (!need) {
+            const VoxState& xv = m.vs[X];
+            if ((xv.flags & kPlane) && (xv.flags & kInit)) {
+              double px[kPlaneD];
+              for (int k = 0; k < kPlaneD; ++k) px[k] = m.plane[size_t(X) * kPlaneD + k];
+              int64_t ax, ay, az;
+              unpack_key(xv.key, ax, ay, az);
+              for (int k = 0; k < nc0; ++k) {
+                const int cc = clos[w][k];
+                int64_t bx, by, bz;
+                unpack_key(m.vs[nbv[w][cc]].key, bx, by, bz);
+                const int64_t dist = (ax > bx ? ax - bx : bx - ax) + (ay > by ? ay - by : by - ay) +
+                                     (az > bz ? az - bz : bz - az);
+                if (dist == 1 && grp[w][cc] != xv.group_id && merge_gate(px, pl[w][cc], m.merge_a, m.merge_d))
+                  need = true;
+              }
+            }
+          }
+        }
+        if (X >= 0 && need) {
+          const VoxState& xs = m.vs[X];
+          if (xs.tag == tag && xs.seg >= 0 && xs.evict_t == kNever) {
+            uint64_t loc[8];
+            int nl = 0;
+            if (xs.merge_pos != kNever) {
+              const int tn = first_call_after(b, xs.seg, xs.merge_pos, t, off);
+              if (tn != kNever) loc[nl++] = cand_key(0, tn, X);
+            }
+            const int nct = b.seg_nct[xs.seg];
+            for (int k = 0; k < nct; ++k) {
+              const int tc = b.seg_ct[xs.seg * 7 + k];
+              if (tc > t) loc[nl++] = cand_key(0, tc, X);
+            }
+            if (nl > 0) {
+              const int at = atomicAdd(&sh_i[4], nl);
+              for (int k = 0; k < nl; ++k)
+                if (at + k < kNewCap) {
+                  nkey[at + k] = loc[k];
+                  nkown[at + k] = w;
+                }
+            }
+            const int f = atomicAdd(&sh_i[5], 1);
+            if (f < kFxCap) {
+              fx_v[f] = X;
+              fx_wk[f] = w * 2;
+            }
+          } else if (xs.flags & kAlive) {
+            const int f = atomicAdd(&sh_i[5], 1);
+            if (f < kFxCap) {
+              fx_v[f] = X;
+              fx_wk[f] = w * 2 + 1;
+            }
+          }
+        }
+      }
+      r2[w][r] = X;
+    }
+    __syncthreads();
+    const int nnew = sh_i[4], nfx = sh_i[5];
+    if (nnew > kNewCap || nfx > kFxCap) {
+      err |= 16;
+      break;
+    }
     // ---- commit in point order
-    nchg = 0;
-    int w = 0;
+    int nchg = 0, w = 0;
+    uint64_t newmin = ~0ull;
     for (; w < n; ++w) {
+      const int nc0 = nclos[w];
       bool bad = false;
-      if (lane < 7) {
-        const int32_t v = nbv[w][lane];
+      if (tid < 7) {
+        const int32_t v = nbv[w][tid];
+        if (v >= 0)
+          for (int k = 0; k < nchg; ++k) bad |= (chg[k] == v);
+      } else if (nc0 > 0 && tid >= 64 && tid < 64 + kTaintPer) {
+        const int32_t v = r2[w][tid - 64];
         if (v >= 0)
           for (int k = 0; k < nchg; ++k) bad |= (chg[k] == v);
       }
       // a re-evaluation created in this window at or before this key runs first
-      if (__any(bad) || newmin <= wkey[w]) break;
+      if (__syncthreads_or(bad) || newmin <= wkey[w]) break;
       ++evals;
-      const int nc0 = nclos[w];
       if (nc0 == 0) continue;
       const int t = cand_t(wkey[w]);
-      if (lane < nc0) {
-        const int j = clos[w][lane];
+      if (tid < nc0) {
+        const int j = clos[w][tid];
         const int32_t X = nbv[w][j];
         for (int k = 0; k < kPlaneD; ++k) m.plane[size_t(X) * kPlaneD + k] = pl[w][j][k];
         m.vs[X].group_id = grp[w][j];
         const uint32_t fl = m.vs[X].flags | kMerged;
         m.vs[X].flags = fl;
         write_hot(m, X, fl);
-        chg[nchg + lane] = X;
-        ckey[lane] = m.vs[X].key;
+        chg[nchg + tid] = X;
       }
-      if (lane == 0) nstage = 0;
+      for (int f = tid; f < nfx; f += kMrThreads) {
+        if ((fx_wk[f] >> 1) != w) continue;
+        VoxState& xs = m.vs[fx_v[f]];
+        if (fx_wk[f] & 1) {
+          atomicOr(&xs.flags, kDirty);
+        } else {
+          atomicMax(&xs.dirty_t, t);
+          atomicMin(&xs.taint_t, t);
+        }
+      }
+      for (int k = tid; k < nnew; k += kMrThreads)
+        if (nkown[k] == w) atomicMin(reinterpret_cast<unsigned long long*>(&sh_u[0]), (unsigned long long)nkey[k]);
       merges += nc0 - 1;
       nchg += nc0;
       __syncthreads();
-      // ---- taint the closed neighbourhoods of every changed voxel: each
-      //      touched one re-evaluates its next call and its later candidates.
-      //      A neighbour that did not change can only start merging with a
-      //      changed voxel (its other neighbours were already idle against its
-      //      unchanged plane), so it is re-evaluated only if it passes the
-      //      group test and the gates against one of the new records.
-      for (int q0 = 0; q0 < nc0 * 7; q0 += 64) {
-        const int qq = q0 + lane;
-        if (qq < nc0 * 7) {
-          const int32_t base_v = nbv[w][clos[w][qq / 7]];
-          const int j = qq % 7;
-          int32_t X = base_v;
-          if (j < 6) {
-            const VoxState& bs = m.vs[base_v];
-            int32_t cur_id;
-            if (bs.tag == tag && bs.seg >= 0) cur_id = b.seg_nbr[bs.seg * 6 + j];
-            else cur_id = key_neighbour_ok(bs.key, j) ? lookup_vid(m, key_neighbour(bs.key, j)) : -1;
-            X = vid_at(m, cur_id, tag, t);
-          }
-          bool need = false;
-          if (X >= 0) {
-            for (int k = 0; k < nc0; ++k) need |= (nbv[w][clos[w][k]] == X);
-            if (!need) {
-              const VoxState& xv = m.vs[X];
-              if ((xv.flags & kPlane) && (xv.flags & kInit)) {
-                double px[kPlaneD];
-                for (int k = 0; k < kPlaneD; ++k) px[k] = m.plane[size_t(X) * kPlaneD + k];
-                int64_t ax, ay, az;
-                unpack_key(xv.key, ax, ay, az);
-                for (int k = 0; k < nc0; ++k) {
-                  int64_t bx, by, bz;
-                  unpack_key(ckey[k], bx, by, bz);
-                  const int64_t dist = (ax > bx ? ax - bx : bx - ax) + (ay > by ? ay - by : by - ay) +
-                                       (az > bz ? az - bz : bz - az);
-                  const int cc = clos[w][k];
-                  if (dist == 1 && grp[w][cc] != xv.group_id && merge_gate(px, pl[w][cc], m.merge_a, m.merge_d))
-                    need = true;
-                }
-              }
-            }
-          }
-          if (X >= 0 && need) {
-            VoxState& xs = m.vs[X];
-            if (xs.tag == tag && xs.seg >= 0 && xs.evict_t == kNever) {
-              if (xs.dirty_t < t) xs.dirty_t = t;
-              if (xs.taint_t > t) xs.taint_t = t;
-              uint64_t loc[8];
-              int nl = 0;
-              if (xs.merge_pos != kNever) {
-                const int tn = first_call_after(b, xs.seg, xs.merge_pos, t, off);
-                if (tn != kNever) loc[nl++] = cand_key(0, tn, X);
-              }
-              const int nct = b.seg_nct[xs.seg];
-              for (int k = 0; k < nct; ++k) {
-                const int tc = b.seg_ct[xs.seg * 7 + k];
-                if (tc > t) loc[nl++] = cand_key(0, tc, X);
-              }
-              if (nl > 0) {
-                const int at = atomicAdd(&nstage, nl);
-                for (int k = 0; k < nl; ++k)
-                  if (at + k < kStageCap) stage[at + k] = loc[k];
-              }
-            } else if (xs.flags & kAlive) {
-              atomicOr(&xs.flags, kDirty);
-            }
-          }
-        }
-      }
-      __syncthreads();
-      if (lane == 0) {
-        const int ns = nstage < kStageCap ? nstage : kStageCap;
-        if (nstage > kStageCap || hn + ns > kHeapCap) {
-          err |= 16;
-        } else {
-          for (int k = 0; k < ns; ++k) {
-            heap_push(heap, hn + k, stage[k]);
-            if (stage[k] < newmin) newmin = stage[k];
-          }
-        }
-        sh_hn = err ? hn : hn + ns;
-      }
-      __syncthreads();
-      hn = sh_hn;
-      err = __shfl(err, 0, 64);
-      if (err) break;
+      newmin = sh_u[0];
     }
-    if (err) break;
-    // ---- entries not committed go back to the heap for re-evaluation
-    if (lane == 0) {
-      if (hn + (n - w) > kHeapCap) {
-        err |= 16;
+    requeued += n - w;
+    // ---- pool := pool[ph, pn) merged with the committed entries' new keys and
+    //      the entries sent back (duplicates are dropped at the next fill)
+    if (tid == 0) sh_i[6] = 0;
+    __syncthreads();
+    for (int k = tid; k < nnew + n; k += kMrThreads) {
+      uint64_t v;
+      bool keep;
+      if (k < nnew) {
+        v = nkey[k];
+        keep = nkown[k] < w;
       } else {
-        for (int k = w; k < n; ++k) heap_push(heap, hn++, wkey[k]);
+        v = wkey[k - nnew];
+        keep = k - nnew >= w;
       }
-      // duplicates of a pushed-back key still in the sources must not be
-      // skipped against it: the last processed key is the last committed one
-      sh_last = w > 0 ? wkey[w - 1] : sh_last_before;
-      sh_hn = hn;
+      if (keep) gbuf[atomicAdd(&sh_i[6], 1)] = v;
     }
     __syncthreads();
-    hn = sh_hn;
-    last = sh_last;
-    err = __shfl(err, 0, 64);
-    if (err) break;
+    const int g = sh_i[6], rest = pn - ph;
+    if (g + rest > kPoolCap) {
+      err |= 16;
+      break;
+    }
+    for (int k = tid; k < g; k += kMrThreads) {  // rank sort (ties by index)
+      const uint64_t v = gbuf[k];
+      int rk = 0;
+      for (int q = 0; q < g; ++q) rk += (gbuf[q] < v) || (gbuf[q] == v && q < k);
+      sbuf[rk] = v;
+    }
+    __syncthreads();
+    const uint64_t* prest = &pool[pb][ph];
+    uint64_t* dst = pool[1 - pb];
+    for (int k = tid; k < g; k += kMrThreads) dst[k + rank_lt(prest, rest, sbuf[k])] = sbuf[k];
+    for (int k = tid; k < rest; k += kMrThreads) dst[k + rank_le(sbuf, g, prest[k])] = prest[k];
+    // duplicates of a key sent back must not be dropped against it: the last
+    // processed key is the last committed one
+    if (w > 0) last = wkey[w - 1];
+    pb = 1 - pb;
+    ph = 0;
+    pn = g + rest;
+    __syncthreads();
   }
-  if (lane == 0) {
+  if (tid == 0) {
     c->merge_evals += evals;
     c->merges += merges;
+    c->merge_windows += windows;
+    c->merge_requeued += requeued;
     if (err) atomicOr(&c->error, err);
   }
 }
