@@ -317,7 +317,6 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
   __shared__ int nclos[kWin];
   __shared__ int act[kWin];
   __shared__ int32_t r2[kWin][kTaintPer];
-  __shared__ int32_t chg[kWin * 7];
   __shared__ uint64_t nkey[kNewCap];
   __shared__ int nkown[kNewCap];
   __shared__ int32_t fx_v[kFxCap];
@@ -325,7 +324,11 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
   __shared__ uint64_t gbuf[kNewCap + kWin];
   __shared__ uint64_t sbuf[kNewCap + kWin];
   __shared__ int sh_i[8];
-  __shared__ uint64_t sh_u[2];
+  __shared__ int sh_off[kWin];
+  __shared__ uint64_t sh_minkey[kWin];
+  __shared__ int sh_conf[kWin];
+  __shared__ uint64_t wr_in[kWin * 7];  // (voxel << 8) | entry of every write
+  __shared__ uint64_t wr[kWin * 7];
   int sp = si[s].spl_lo, pb = 0, ph = 0, pn = 0;
   int evals = 0, merges = 0, windows = 0, requeued = 0, err = 0;
   uint64_t last = ~0ull;
@@ -423,7 +426,6 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
       sh_i[3] = na;
       sh_i[4] = 0;  // new keys
       sh_i[5] = 0;  // side effects
-      sh_u[0] = ~0ull;
     }
     __syncthreads();
     const int na = sh_i[3];
@@ -514,53 +516,95 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
       err |= 16;
       break;
     }
-    // ---- commit in point order
-    int nchg = 0, w = 0;
-    uint64_t newmin = ~0ull;
-    for (; w < n; ++w) {
-      const int nc0 = nclos[w];
-      bool bad = false;
-      if (tid < 7) {
-        const int32_t v = nbv[w][tid];
-        if (v >= 0)
-          for (int k = 0; k < nchg; ++k) bad |= (chg[k] == v);
-      } else if (nc0 > 0 && tid >= 64 && tid < 64 + kTaintPer) {
-        const int32_t v = r2[w][tid - 64];
-        if (v >= 0)
-          for (int k = 0; k < nchg; ++k) bad |= (chg[k] == v);
+    // ---- commit in point order, all at once: the first entry that read a
+    //      voxel an earlier acting entry of the window writes (its seven
+    //      records, and if it acts the records of its taint phase), or that
+    //      an earlier entry's new key must precede, stops the commit.  The
+    //      entries before it commit together: their write sets are disjoint
+    //      (an entry that writes a voxel also read it).
+    if (tid == 0) {
+      int o = 0;
+      for (int a = 0; a < na; ++a) {
+        sh_off[a] = o;
+        o += nclos[act[a]];
       }
-      // a re-evaluation created in this window at or before this key runs first
-      if (__syncthreads_or(bad) || newmin <= wkey[w]) break;
-      ++evals;
-      if (nc0 == 0) continue;
-      const int t = cand_t(wkey[w]);
-      if (tid < nc0) {
-        const int j = clos[w][tid];
-        const int32_t X = nbv[w][j];
-        for (int k = 0; k < kPlaneD; ++k) m.plane[size_t(X) * kPlaneD + k] = pl[w][j][k];
-        m.vs[X].group_id = grp[w][j];
-        const uint32_t fl = m.vs[X].flags | kMerged;
-        m.vs[X].flags = fl;
-        write_hot(m, X, fl);
-        chg[nchg + tid] = X;
-      }
-      for (int f = tid; f < nfx; f += kMrThreads) {
-        if ((fx_wk[f] >> 1) != w) continue;
-        VoxState& xs = m.vs[fx_v[f]];
-        if (fx_wk[f] & 1) {
-          atomicOr(&xs.flags, kDirty);
-        } else {
-          atomicMax(&xs.dirty_t, t);
-          atomicMin(&xs.taint_t, t);
-        }
-      }
-      for (int k = tid; k < nnew; k += kMrThreads)
-        if (nkown[k] == w) atomicMin(reinterpret_cast<unsigned long long*>(&sh_u[0]), (unsigned long long)nkey[k]);
-      merges += nc0 - 1;
-      nchg += nc0;
-      __syncthreads();
-      newmin = sh_u[0];
+      sh_i[7] = o;  // writer records
     }
+    if (tid < kWin) {
+      sh_minkey[tid] = ~0ull;
+      sh_conf[tid] = 0;
+    }
+    __syncthreads();
+    const int nwr = sh_i[7];
+    if (tid < na * 8) {
+      const int a = tid >> 3, ci = tid & 7;
+      const int k = act[a];
+      if (ci < nclos[k]) wr_in[sh_off[a] + ci] = (uint64_t(uint32_t(nbv[k][clos[k][ci]])) << 8) | uint64_t(k);
+    }
+    for (int q = tid; q < nnew; q += kMrThreads)
+      atomicMin(reinterpret_cast<unsigned long long*>(&sh_minkey[nkown[q]]), (unsigned long long)nkey[q]);
+    __syncthreads();
+    for (int q = tid; q < nwr; q += kMrThreads) {  // sort (voxel, entry) pairs
+      const uint64_t v = wr_in[q];
+      int rk = 0;
+      for (int u = 0; u < nwr; ++u) rk += wr_in[u] < v;
+      wr[rk] = v;
+    }
+    __syncthreads();
+    // reads: (entry, voxel) for the seven records of every entry and the
+    // taint reads of acting ones; conflict if the voxel's first writer of
+    // the window is an earlier entry
+    for (int q = tid; q < n * (7 + kTaintPer); q += kMrThreads) {
+      const int w = q / (7 + kTaintPer), r = q % (7 + kTaintPer);
+      int32_t v;
+      if (r < 7) v = nbv[w][r];
+      else v = nclos[w] > 0 ? r2[w][r - 7] : -1;
+      if (v < 0) continue;
+      const uint64_t lo = uint64_t(uint32_t(v)) << 8;
+      const int at = rank_lt(wr, nwr, lo);
+      if (at < nwr && (wr[at] >> 8) == uint64_t(uint32_t(v)) && int(wr[at] & 0xFF) < w) sh_conf[w] = 1;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      uint64_t pm = ~0ull;
+      int stop = n;
+      for (int w2 = 0; w2 < n; ++w2) {
+        if (sh_conf[w2] || pm <= wkey[w2]) {
+          stop = w2;
+          break;
+        }
+        if (nclos[w2] > 0 && sh_minkey[w2] < pm) pm = sh_minkey[w2];
+      }
+      sh_i[7] = stop;
+    }
+    __syncthreads();
+    const int w = sh_i[7];
+    evals += w;
+    for (int q = tid; q < w * 8; q += kMrThreads) {
+      const int e = q >> 3, ci = q & 7;
+      if (ci >= nclos[e]) continue;
+      const int j = clos[e][ci];
+      const int32_t X = nbv[e][j];
+      for (int k = 0; k < kPlaneD; ++k) m.plane[size_t(X) * kPlaneD + k] = pl[e][j][k];
+      m.vs[X].group_id = grp[e][j];
+      const uint32_t fl = m.vs[X].flags | kMerged;
+      m.vs[X].flags = fl;
+      write_hot(m, X, fl);
+    }
+    for (int f = tid; f < nfx; f += kMrThreads) {
+      const int e = fx_wk[f] >> 1;
+      if (e >= w) continue;
+      VoxState& xs = m.vs[fx_v[f]];
+      if (fx_wk[f] & 1) {
+        atomicOr(&xs.flags, kDirty);
+      } else {
+        const int t = cand_t(wkey[e]);
+        atomicMax(&xs.dirty_t, t);
+        atomicMin(&xs.taint_t, t);
+      }
+    }
+    for (int e = 0; e < w; ++e)
+      if (nclos[e] > 0) merges += nclos[e] - 1;
     requeued += n - w;
     // ---- pool := pool[ph, pn) merged with the committed entries' new keys and
     //      the entries sent back (duplicates are dropped at the next fill)
