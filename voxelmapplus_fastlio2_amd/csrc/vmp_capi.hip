@@ -715,6 +715,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
                             size_t(ntot), st) != hipSuccess)
           return VMP_EDEVICE;
         hipLaunchKernelGGL(k_spl_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b);
+        hipLaunchKernelGGL(k_spl_keys, dim3(gn), dim3(T), 0, st, b);
         hipLaunchKernelGGL(k_merge_replay, dim3(n), dim3(kMrThreads), 0, st, M, S, n, b);
       }
       hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.flags, 0, ntot);

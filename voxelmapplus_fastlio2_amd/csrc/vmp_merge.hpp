@@ -236,6 +236,13 @@ __global__ void k_spec_eval(const MapDev* maps, BatchBufs b) {
 }
 
 // per-session range of the compacted acting candidates (cnt[4] of them)
+// keys of the compacted acting candidates, contiguous (into cand_in, free
+// after the candidate sort), so the replay loads a window's keys in one step
+__global__ void k_spl_keys(BatchBufs b) {
+  const int n = b.cnt[4];
+  VMP_GRID_LOOP(k, n) b.cand_in[k] = b.cand_out[b.spl[k]] & kCandLocal;
+}
+
 __global__ void k_spl_ranges(const MapDev* maps, SessInfo* si, int nsess, BatchBufs b) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsess) return;
@@ -324,6 +331,13 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
   __shared__ uint64_t gbuf[kNewCap + kWin];
   __shared__ uint64_t sbuf[kNewCap + kWin];
   __shared__ int sh_i[8];
+  __shared__ uint64_t mk[2 * kSPre];  // fill: statics and pool entries merged
+  __shared__ int mpos[2 * kSPre];
+  __shared__ int mval[2 * kSPre];
+  __shared__ int mpre[2 * kSPre];
+  __shared__ int32_t rseg[kWin][7];   // record voxels: segment of this scan (-1 none), key
+  __shared__ uint64_t rkey[kWin][7];
+  __shared__ int pass[kWin];
   __shared__ int sh_off[kWin];
   __shared__ uint64_t sh_minkey[kWin];
   __shared__ int sh_conf[kWin];
@@ -333,53 +347,73 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
   int evals = 0, merges = 0, windows = 0, requeued = 0, err = 0;
   uint64_t last = ~0ull;
   while (true) {
-    // ---- fill the window
-    if (tid < kSPre) {
-      const int q = sp + tid;
-      if (q < spl_hi) {
-        const int pos = b.spl[q];
-        spos[tid] = pos;
-        skey[tid] = b.cand_out[pos] & kCandLocal;
-      } else {
-        skey[tid] = ~0ull;
+    // ---- fill the window: the loaded statics and pool entries merged in
+    //      parallel (merge path; statics first on equal keys), duplicates and
+    //      keys beyond what is loaded dropped, the first kWin taken
+    const int ns = min(kSPre, spl_hi - sp), np = min(kSPre, pn - ph);
+    if (tid < ns) {
+      skey[tid] = b.cand_in[sp + tid];
+      spos[tid] = b.spl[sp + tid];
+    }
+    __syncthreads();
+    const uint64_t* pseg = &pool[pb][ph];
+    uint64_t lim = ~0ull;  // keys above an unloaded static or pool entry wait
+    if (spl_hi - sp > kSPre) lim = skey[ns - 1];
+    if (pn - ph > kSPre && pseg[np - 1] < lim) lim = pseg[np - 1];
+    const int M = ns + np;
+    if (tid < ns) {
+      const int p = tid + rank_lt(pseg, np, skey[tid]);
+      mk[p] = skey[tid];
+      mpos[p] = spos[tid];
+    } else if (tid >= 128 && tid - 128 < np) {
+      const int j = tid - 128;
+      const int p = j + rank_le(skey, ns, pseg[j]);
+      mk[p] = pseg[j];
+      mpos[p] = -1;
+    }
+    __syncthreads();
+    if (tid < M) mval[tid] = mk[tid] <= lim && mk[tid] != (tid == 0 ? last : mk[tid - 1]);
+    __syncthreads();
+    if (tid < M) {
+      int pre = 0;
+      for (int q = 0; q < tid; ++q) pre += mval[q];
+      mpre[tid] = pre;
+      if (mval[tid] && pre < kWin) {
+        wkey[pre] = mk[tid];
+        wpos[pre] = mpos[tid];
       }
     }
     __syncthreads();
     if (tid == 0) {
-      const bool more_static = sp + kSPre < spl_hi;
-      int used = 0, taken = 0, pi = ph;
-      uint64_t lst = last;
-      while (taken < kWin) {
-        if (used == kSPre && more_static) break;  // the next static key is not loaded
-        const uint64_t st_key = used < kSPre ? skey[used] : ~0ull;
-        const uint64_t p_key = pi < pn ? pool[pb][pi] : ~0ull;
-        if (st_key == ~0ull && p_key == ~0ull) break;
-        uint64_t cur;
-        int pos;
-        if (st_key <= p_key) {
-          cur = st_key;
-          pos = spos[used];
-          ++used;
-        } else {
-          cur = p_key;
-          pos = -1;
-          ++pi;
+      // consumed: everything before the kWin-th taken key, or before the
+      // first key over the limit
+      int cut = M, taken = 0;
+      for (int p = 0; p < M; ++p) {
+        if (mk[p] > lim) {
+          cut = p;
+          break;
         }
-        if (cur == lst) continue;
-        lst = cur;
-        wkey[taken] = cur;
-        wpos[taken] = pos;
-        ++taken;
+        if (mval[p] && ++taken == kWin) {
+          cut = p + 1;
+          break;
+        }
       }
-      sh_i[0] = taken;
+      if (cut == M) taken = M > 0 ? mpre[M - 1] + mval[M - 1] : 0;
+      int used = 0;
+      for (int p = 0; p < cut; ++p) used += mpos[p] >= 0;
+      sh_i[0] = taken < kWin ? taken : kWin;
       sh_i[1] = used;
-      sh_i[2] = pi;
+      sh_i[2] = ph + (cut - used);
     }
     __syncthreads();
     const int n = sh_i[0];
+    const int consumed = sh_i[1] + (sh_i[2] - ph);
     sp += sh_i[1];
     ph = sh_i[2];
-    if (n == 0) break;
+    if (n == 0) {
+      if (consumed == 0) break;  // nothing left
+      continue;                  // only duplicates of the last key were loaded
+    }
     ++windows;
     // ---- records: thread w*7+j, j = 6 is the voxel itself
     if (tid < n * 7) {
@@ -399,13 +433,27 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
       }
       nbv[w][j] = nv;
       if (nv >= 0) {
-        grp[w][j] = m.vs[nv].group_id;
+        const VoxState& ns_ = m.vs[nv];
+        grp[w][j] = ns_.group_id;
+        rseg[w][j] = (ns_.tag == tag && ns_.seg >= 0) ? ns_.seg : -1;
+        rkey[w][j] = ns_.key;
         for (int k = 0; k < kPlaneD; ++k) pl[w][j][k] = m.plane[size_t(nv) * kPlaneD + k];
       }
     }
     __syncthreads();
+    // ---- gates of every neighbour against the entry's own (unchanged) plane:
+    //      an entry none passes does not act, the others run merge() below
+    if (tid < n) pass[tid] = 0;
+    __syncthreads();
+    if (tid < n * 7) {
+      const int w = tid / 7, j = tid % 7;
+      if (j < 6 && nbv[w][j] >= 0 && grp[w][j] != grp[w][6] && merge_gate(pl[w][6], pl[w][j], m.merge_a, m.merge_d))
+        pass[w] = 1;
+    }
+    __syncthreads();
     // ---- merge(): each entry's six neighbours in order against its own plane
-    if (tid < n) {
+    if (tid < n && !pass[tid]) nclos[tid] = 0;
+    if (tid < n && pass[tid]) {
       const int w = tid;
       int nc = 0;
       for (int j = 0; j < 6; ++j) {
@@ -441,13 +489,15 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
       int32_t X = -1;
       if (ci < nc0) {
         const int t = cand_t(wkey[w]);
-        const int32_t base_v = nbv[w][clos[w][ci]];
+        const int cj = clos[w][ci];
+        const int32_t base_v = nbv[w][cj];
         X = base_v;
         if (j < 6) {
-          const VoxState& bs = m.vs[base_v];
+          const int bseg = rseg[w][cj];
+          const uint64_t bkey = rkey[w][cj];
           int32_t cur_id;
-          if (bs.tag == tag && bs.seg >= 0) cur_id = b.seg_nbr[bs.seg * 6 + j];
-          else cur_id = key_neighbour_ok(bs.key, j) ? lookup_vid(m, key_neighbour(bs.key, j)) : -1;
+          if (bseg >= 0) cur_id = b.seg_nbr[bseg * 6 + j];
+          else cur_id = key_neighbour_ok(bkey, j) ? lookup_vid(m, key_neighbour(bkey, j)) : -1;
           X = vid_at(m, cur_id, tag, t);
         }
         bool need = false;
@@ -463,7 +513,7 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
               for (int k = 0; k < nc0; ++k) {
                 const int cc = clos[w][k];
                 int64_t bx, by, bz;
-                unpack_key(m.vs[nbv[w][cc]].key, bx, by, bz);
+                unpack_key(rkey[w][cc], bx, by, bz);
                 const int64_t dist = (ax > bx ? ax - bx : bx - ax) + (ay > by ? ay - by : by - ay) +
                                      (az > bz ? az - bz : bz - az);
                 if (dist == 1 && grp[w][cc] != xv.group_id && merge_gate(px, pl[w][cc], m.merge_a, m.merge_d))
