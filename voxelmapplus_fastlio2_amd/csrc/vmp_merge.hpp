@@ -339,6 +339,7 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
   __shared__ uint64_t rkey[kWin][7];
   __shared__ int pass[kWin];
   __shared__ int sh_off[kWin];
+  __shared__ int sh_mg;
   __shared__ uint64_t sh_minkey[kWin];
   __shared__ int sh_conf[kWin];
   __shared__ uint64_t wr_in[kWin * 7];  // (voxel << 8) | entry of every write
@@ -372,40 +373,59 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
       mpos[p] = -1;
     }
     __syncthreads();
-    if (tid < M) mval[tid] = mk[tid] <= lim && mk[tid] != (tid == 0 ? last : mk[tid - 1]);
-    __syncthreads();
-    if (tid < M) {
-      int pre = 0;
-      for (int q = 0; q < tid; ++q) pre += mval[q];
-      mpre[tid] = pre;
-      if (mval[tid] && pre < kWin) {
+    // valid: not over the limit, not a duplicate of the previous key; prefix
+    // counts from wave ballots (M <= 2 * kSPre = 144 < 3 waves)
+    {
+      const int lane = tid & 63, wv = tid >> 6;
+      const bool in = tid < M;
+      const bool over = in && mk[tid] > lim;
+      const bool ok = in && !over && mk[tid] != (tid == 0 ? last : mk[tid - 1]);
+      const bool stat = in && !over && mpos[tid] >= 0;
+      const uint64_t bok = __ballot(ok), bst = __ballot(stat);
+      if (lane == 0 && wv < 3) {
+        mpre[wv] = __popcll(bok);
+        mval[wv] = __popcll(bst);
+      }
+      if (tid == 0) {
+        sh_i[0] = 0x7fffffff;  // first position over the limit
+      }
+      __syncthreads();
+      int base = 0, sbase = 0;
+      for (int q = 0; q < wv && q < 3; ++q) {
+        base += mpre[q];
+        sbase += mval[q];
+      }
+      const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+      const int pre = base + __popcll(bok & below);
+      const int spre = sbase + __popcll(bst & below);
+      if (ok && pre < kWin) {
         wkey[pre] = mk[tid];
         wpos[pre] = mpos[tid];
       }
-    }
-    __syncthreads();
-    if (tid == 0) {
-      // consumed: everything before the kWin-th taken key, or before the
-      // first key over the limit
-      int cut = M, taken = 0;
-      for (int p = 0; p < M; ++p) {
-        if (mk[p] > lim) {
-          cut = p;
-          break;
-        }
-        if (mval[p] && ++taken == kWin) {
-          cut = p + 1;
-          break;
-        }
+      if (over && (tid == 0 || mk[tid - 1] <= lim)) atomicMin(&sh_i[0], tid);
+      // the kWin-th taken key ends the consumed range
+      if (ok && pre == kWin - 1) {
+        sh_i[3] = tid + 1;
+        sh_i[4] = spre + (stat ? 1 : 0);
       }
-      if (cut == M) taken = M > 0 ? mpre[M - 1] + mval[M - 1] : 0;
-      int used = 0;
-      for (int p = 0; p < cut; ++p) used += mpos[p] >= 0;
-      sh_i[0] = taken < kWin ? taken : kWin;
-      sh_i[1] = used;
-      sh_i[2] = ph + (cut - used);
+      __syncthreads();
+      if (tid == 0) {
+        const int total = mpre[0] + mpre[1] + mpre[2];
+        const int statics = mval[0] + mval[1] + mval[2];
+        int cut, used;
+        if (total >= kWin) {
+          cut = sh_i[3];
+          used = sh_i[4];
+        } else {
+          cut = sh_i[0] < M ? sh_i[0] : M;  // statics counted only below the limit
+          used = statics;
+        }
+        sh_i[0] = total < kWin ? total : kWin;
+        sh_i[1] = used;
+        sh_i[2] = ph + (cut - used);
+      }
+      __syncthreads();
     }
-    __syncthreads();
     const int n = sh_i[0];
     const int consumed = sh_i[1] + (sh_i[2] - ph);
     sp += sh_i[1];
@@ -615,20 +635,27 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
       if (at < nwr && (wr[at] >> 8) == uint64_t(uint32_t(v)) && int(wr[at] & 0xFF) < w) sh_conf[w] = 1;
     }
     __syncthreads();
-    if (tid == 0) {
-      uint64_t pm = ~0ull;
-      int stop = n;
-      for (int w2 = 0; w2 < n; ++w2) {
-        if (sh_conf[w2] || pm <= wkey[w2]) {
-          stop = w2;
-          break;
-        }
-        if (nclos[w2] > 0 && sh_minkey[w2] < pm) pm = sh_minkey[w2];
+    if (tid < 64) {  // one wave (n <= kWin < 64): exclusive prefix minimum of the new keys
+      uint64_t pm = tid < n ? sh_minkey[tid] : ~0ull;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t u = __shfl_up(pm, o, 64);
+        if (tid >= o && u < pm) pm = u;
       }
-      sh_i[7] = stop;
+      uint64_t before = __shfl_up(pm, 1, 64);
+      if (tid == 0) before = ~0ull;
+      const bool stop_here = tid < n && (sh_conf[tid] || before <= wkey[tid]);
+      const uint64_t bm = __ballot(stop_here);
+      const int stop = bm ? __ffsll((unsigned long long)bm) - 1 : n;
+      int mg = (tid < stop && nclos[tid] > 0) ? nclos[tid] - 1 : 0;
+      for (int o = 32; o > 0; o >>= 1) mg += __shfl_xor(mg, o, 64);
+      if (tid == 0) {
+        sh_i[7] = stop;
+        sh_mg = mg;
+      }
     }
     __syncthreads();
     const int w = sh_i[7];
+    merges += sh_mg;
     evals += w;
     for (int q = tid; q < w * 8; q += kMrThreads) {
       const int e = q >> 3, ci = q & 7;
@@ -653,8 +680,6 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
         atomicMin(&xs.taint_t, t);
       }
     }
-    for (int e = 0; e < w; ++e)
-      if (nclos[e] > 0) merges += nclos[e] - 1;
     requeued += n - w;
     // ---- pool := pool[ph, pn) merged with the committed entries' new keys and
     //      the entries sent back (duplicates are dropped at the next fill)
