@@ -126,6 +126,9 @@ int vmp_state_apply_motion_batch(vmp_session** s, int32_t n, const double* dR, c
  * pointers are copied; _device takes pointers that already live in HBM. */
 int vmp_scan_set_points(vmp_session* s, const float* x, const float* y, const float* z, int32_t n);
 int vmp_scan_set_points_device(vmp_session* s, const float* dx, const float* dy, const float* dz, int32_t n);
+/* The same for n sessions in one call: device addresses and point counts. */
+int vmp_scan_set_points_device_batch(vmp_session** s, int32_t n, const uint64_t* dx, const uint64_t* dy,
+                                     const uint64_t* dz, const int32_t* counts);
 
 /* One measurement pass at state x with the session's P as prior:
  * H (12x12) and b (12) exactly as SharedState (ieskf.h:22-29). */
@@ -152,6 +155,14 @@ int vmp_slots_export(vmp_session* s, int32_t n, uint8_t* valid, float* resid, fl
 
 /* Batched forms: sessions advance together, one launch per stage. */
 int vmp_register_batch(vmp_session** s, int32_t n, vmp_obs_stats* ost, vmp_map_stats* mst);
+/* The same without waiting: the batch is launched and the call returns; the
+ * next submit (or any call that needs the context) first completes it, so the
+ * host can prepare the next scans while the device works.  Returns the status
+ * of the batch it completed (or of this launch).  vmp_register_batch_wait
+ * completes the batch in flight and copies the statistics of the last
+ * completed batch (n entries). */
+int vmp_register_batch_submit(vmp_session** s, int32_t n);
+int vmp_register_batch_wait(vmp_ctx* ctx, int32_t n, vmp_obs_stats* ost, vmp_map_stats* mst);
 
 /* Launch-shape parameters of a context (no reference counterpart):
  *   "obs_ppt"  points per thread of the observe kernel, 1..64; 0 = chosen by

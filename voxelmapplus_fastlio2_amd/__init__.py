@@ -13,6 +13,9 @@ from .registration import (
     apply_motion_batch,
     initial_covariance,
     register_batch,
+    register_batch_submit,
+    register_batch_wait,
+    set_points_device_batch,
     state_vector,
 )
 
@@ -24,5 +27,8 @@ __all__ = [
     "apply_motion_batch",
     "initial_covariance",
     "register_batch",
+    "register_batch_submit",
+    "register_batch_wait",
+    "set_points_device_batch",
     "state_vector",
 ]

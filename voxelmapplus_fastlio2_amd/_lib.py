@@ -110,11 +110,15 @@ def vmp():
     _proto(lib, "vmp_state_apply_motion_batch", C.c_int, [pp, C.c_int32, dp, dp, dp])
     _proto(lib, "vmp_scan_set_points", C.c_int, [P, fp, fp, fp, C.c_int32])
     _proto(lib, "vmp_scan_set_points_device", C.c_int, [P, P, P, P, C.c_int32])
+    u64p = C.POINTER(C.c_uint64)
+    _proto(lib, "vmp_scan_set_points_device_batch", C.c_int, [pp, C.c_int32, u64p, u64p, u64p, C.POINTER(C.c_int32)])
     _proto(lib, "vmp_observe", C.c_int, [P, C.POINTER(State), dp, dp, C.POINTER(ObsStats)])
     _proto(lib, "vmp_map_update", C.c_int, [P, C.c_int, C.POINTER(MapStats)])
     _proto(lib, "vmp_ieskf_update", C.c_int, [P, C.POINTER(ObsStats)])
     _proto(lib, "vmp_register_scan", C.c_int, [P, C.POINTER(ObsStats), C.POINTER(MapStats)])
     _proto(lib, "vmp_register_batch", C.c_int, [pp, C.c_int32, C.POINTER(ObsStats), C.POINTER(MapStats)])
+    _proto(lib, "vmp_register_batch_submit", C.c_int, [pp, C.c_int32])
+    _proto(lib, "vmp_register_batch_wait", C.c_int, [P, C.c_int32, C.POINTER(ObsStats), C.POINTER(MapStats)])
     _proto(
         lib,
         "vmp_map_export",

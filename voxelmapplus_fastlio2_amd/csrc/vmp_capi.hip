@@ -148,6 +148,14 @@ struct vmp_ctx {
   int obs_launches = 0;
   // launch shape of the observe kernel: points per thread, 0 = by scan size
   int obs_ppt = 0;
+  // a batch submitted without waiting (vmp_register_batch_submit)
+  bool fl = false;
+  int fl_mode = 0;
+  std::vector<vmp_session*> fl_ss;
+  std::vector<int32_t> fl_n;
+  std::vector<vmp_obs_stats> res_ost;  // results of the last finished batch
+  std::vector<vmp_map_stats> res_mst;
+  int res_rc = 0;
 };
 
 // Points per thread of the observe kernel for a scan of n points.  It depends
@@ -205,8 +213,14 @@ extern "C" int vmp_ctx_create(int device, vmp_ctx** out) {
   return VMP_OK;
 }
 
+static int settle(vmp_ctx* c);
+
 extern "C" int vmp_ctx_destroy(vmp_ctx* ctx) {
   if (!ctx) return VMP_EINVAL;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    settle(ctx);
+  }
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   ctx->ws.release();
@@ -221,10 +235,13 @@ extern "C" int vmp_ctx_destroy(vmp_ctx* ctx) {
   return VMP_OK;
 }
 
+static int settle(vmp_ctx* c);
+
 // grow the batch workspace to hold `points` points and `nsess` sessions
 static int ws_reserve(vmp_ctx* c, int64_t points, int nsess) {
   BatchWs& w = c->ws;
   if (points <= w.cap_points && nsess <= w.cap_sess) return VMP_OK;
+  settle(c);  // a batch in flight still reads the old workspace
   hipStreamSynchronize(c->stream);
   const int64_t np = std::max<int64_t>(points, w.cap_points);
   const int ns = std::max(nsess, w.cap_sess);
@@ -350,6 +367,14 @@ static int check_cfg(const vmp_map_cfg* c) {
 
 extern "C" int vmp_session_destroy(vmp_session* s) {
   if (!s) return VMP_EINVAL;
+  if (s->ctx) {
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    for (vmp_session* q : s->ctx->fl_ss)
+      if (q == s) {
+        settle(s->ctx);
+        break;
+      }
+  }
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
   for (void* p : s->allocs) hipFree(p);
@@ -539,6 +564,18 @@ extern "C" int vmp_scan_set_points_device(vmp_session* s, const float* dx, const
   s->cz = dz;
   s->n = n;
   s->has_points = true;
+  return VMP_OK;
+}
+
+extern "C" int vmp_scan_set_points_device_batch(vmp_session** ss, int32_t n, const uint64_t* dx, const uint64_t* dy,
+                                                const uint64_t* dz, const int32_t* counts) {
+  if (!ss || n <= 0 || !dx || !dy || !dz || !counts) return VMP_EINVAL;
+  for (int i = 0; i < n; ++i) {
+    const int rc = vmp_scan_set_points_device(ss[i], reinterpret_cast<const float*>(dx[i]),
+                                              reinterpret_cast<const float*>(dy[i]),
+                                              reinterpret_cast<const float*>(dz[i]), counts[i]);
+    if (rc != VMP_OK) return rc;
+  }
   return VMP_OK;
 }
 
@@ -815,7 +852,12 @@ static void collect_timing(vmp_ctx* c, int active_max) {
 // run one step for a batch on the stream and workspace of context `c` (the
 // sessions may belong to other contexts of the same device; the caller makes
 // sure no session is in two batches at once) and distribute the results
-static int run_step_on(vmp_ctx* c, vmp_session** ss, int n, int mode, vmp_obs_stats* ost, vmp_map_stats* mst) {
+static int step_finish(vmp_ctx* c, vmp_obs_stats* ost, vmp_map_stats* mst);
+
+// validate and launch a batch on the stream and workspace of context `c` (the
+// sessions may belong to other contexts of the same device; the caller makes
+// sure no session is in two batches at once); c->mu held, no batch in flight
+static int step_begin(vmp_ctx* c, vmp_session** ss, int n, int mode) {
   vmp_session* s = ss[0];
   for (int i = 0; i < n; ++i) {
     if (!ss[i] || ss[i]->ctx->device != c->device) return VMP_EINVAL;
@@ -828,7 +870,6 @@ static int run_step_on(vmp_ctx* c, vmp_session** ss, int n, int mode, vmp_obs_st
       return VMP_EINVAL;
     }
   }
-  std::lock_guard<std::mutex> lk(c->mu);
   VMP_HIP(hipSetDevice(c->device));
   int64_t ntot = 0;
   for (int i = 0; i < n; ++i) ntot += ss[i]->n;
@@ -855,13 +896,31 @@ static int run_step_on(vmp_ctx* c, vmp_session** ss, int n, int mode, vmp_obs_st
     s->err = "launch failed";
     return rc;
   }
+  c->fl = true;
+  c->fl_mode = mode;
+  c->fl_ss.assign(ss, ss + n);
+  c->fl_n.resize(size_t(n));
+  for (int i = 0; i < n; ++i) c->fl_n[size_t(i)] = ss[i]->n;
+  return VMP_OK;
+}
+
+// wait for the batch in flight on c, distribute its results (ost / mst may be
+// null) and run the sessions' housekeeping; c->mu held
+static int step_finish(vmp_ctx* c, vmp_obs_stats* ost, vmp_map_stats* mst) {
+  if (!c->fl) return VMP_OK;
+  c->fl = false;
+  vmp_session** ss = c->fl_ss.data();
+  const int n = int(c->fl_ss.size());
+  const int mode = c->fl_mode;
+  vmp_session* s = ss[0];
+  int rc = VMP_OK;
   VMP_HIP(hipStreamSynchronize(c->stream));
   int max_iters = 0;
   int64_t pts_iters = 0, hits = 0;
   for (int i = 0; i < n; ++i) {
     const BatchOut& o = c->ws.h_out[i];
     max_iters = std::max(max_iters, o.iters);
-    pts_iters += int64_t(ss[i]->n) * o.iters;
+    pts_iters += int64_t(c->fl_n[size_t(i)]) * o.iters;
     hits += o.obs[4];
   }
   if (c->timing) {
@@ -886,7 +945,7 @@ static int run_step_on(vmp_ctx* c, vmp_session** ss, int n, int mode, vmp_obs_st
       if (first_err == VMP_OK) first_err = VMP_EDEVICE;
     }
     if (ost) {
-      ost[i].points = si->n;
+      ost[i].points = c->fl_n[size_t(i)];
       ost[i].hits = o.obs[0];
       ost[i].plane_hits = o.obs[1];
       ost[i].effective = o.obs[2];
@@ -906,7 +965,7 @@ static int run_step_on(vmp_ctx* c, vmp_session** ss, int n, int mode, vmp_obs_st
         continue;
       }
       if (mst) {
-        mst[i].points = si->n;
+        mst[i].points = c->fl_n[size_t(i)];
         mst[i].touched = m.nseg;
         mst[i].created = m.ncre;
         mst[i].evicted = m.nevict;
@@ -928,6 +987,27 @@ static int run_step_on(vmp_ctx* c, vmp_session** ss, int n, int mode, vmp_obs_st
   return first_err;
 }
 
+// complete a submitted batch, keeping its results for vmp_register_batch_wait
+// (c->mu held)
+static int settle(vmp_ctx* c) {
+  if (!c->fl) return VMP_OK;
+  const size_t n = c->fl_ss.size();
+  c->res_ost.assign(n, vmp_obs_stats{});
+  c->res_mst.assign(n, vmp_map_stats{});
+  c->res_rc = step_finish(c, c->res_ost.data(), c->res_mst.data());
+  return c->res_rc;
+}
+
+// run one batch step on context c and distribute the results
+static int run_step_on(vmp_ctx* c, vmp_session** ss, int n, int mode, vmp_obs_stats* ost, vmp_map_stats* mst) {
+  std::lock_guard<std::mutex> lk(c->mu);
+  int rc = settle(c);  // a submitted batch completes first
+  if (rc != VMP_OK) return rc;
+  rc = step_begin(c, ss, n, mode);
+  if (rc != VMP_OK) return rc;
+  return step_finish(c, ost, mst);
+}
+
 static int run_step(vmp_session** ss, int n, int mode, vmp_obs_stats* ost, vmp_map_stats* mst) {
   if (!ss || n <= 0 || !ss[0]) return VMP_EINVAL;
   vmp_ctx* c = ss[0]->ctx;
@@ -945,6 +1025,7 @@ extern "C" int vmp_observe(vmp_session* s, const vmp_state* x, double* H, double
   vmp_ctx* c = s->ctx;
   std::lock_guard<std::mutex> lk(c->mu);
   VMP_HIP(hipSetDevice(c->device));
+  settle(c);
   if (ws_reserve(c, std::max(s->n, 1), 1) != VMP_OK) return VMP_ENOMEM;
   VMP_HIP(hipMemcpyAsync(&s->f->x, x, sizeof(vmp_state), hipMemcpyHostToDevice, c->stream));
   vmp_session* one[1] = {s};
@@ -987,6 +1068,29 @@ extern "C" int vmp_register_scan(vmp_session* s, vmp_obs_stats* ost, vmp_map_sta
   if (!s) return VMP_EINVAL;
   vmp_session* one[1] = {s};
   return run_step(one, 1, kStepIeskf | kStepMap, ost, mst);
+}
+
+extern "C" int vmp_register_batch_submit(vmp_session** ss, int32_t n) {
+  if (!ss || n <= 0 || !ss[0]) return VMP_EINVAL;
+  vmp_ctx* c = ss[0]->ctx;
+  for (int i = 0; i < n; ++i)
+    if (!ss[i] || ss[i]->ctx != c) return VMP_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  const int prev = settle(c);
+  const int rc = step_begin(c, ss, n, kStepIeskf | kStepMap);
+  return rc != VMP_OK ? rc : prev;
+}
+
+extern "C" int vmp_register_batch_wait(vmp_ctx* c, int32_t n, vmp_obs_stats* ost, vmp_map_stats* mst) {
+  if (!c) return VMP_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  settle(c);
+  const int m = std::min<int>(n, int(c->res_mst.size()));
+  for (int i = 0; i < m; ++i) {
+    if (ost) ost[i] = c->res_ost[size_t(i)];
+    if (mst) mst[i] = c->res_mst[size_t(i)];
+  }
+  return c->res_rc;
 }
 
 extern "C" int vmp_register_batch(vmp_session** ss, int32_t n, vmp_obs_stats* ost, vmp_map_stats* mst) {

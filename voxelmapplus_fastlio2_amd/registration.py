@@ -323,3 +323,43 @@ def apply_motion_batch(registrars, dRs, dps, qdiag):
     rc = lib.vmp_state_apply_motion_batch(arr, n, _dptr(dR), _dptr(dp), _dptr(q))
     if rc != 0:
         raise VmpError(f"vmp_state_apply_motion_batch failed ({rc})")
+
+
+def set_points_device_batch(registrars, xs, ys, zs, counts):
+    """Bind device point buffers (addresses) of many sessions in one call."""
+    lib = registrars[0].lib
+    n = len(registrars)
+    arr = (C.c_void_p * n)(*[r.h.value for r in registrars])
+    ax = np.ascontiguousarray(xs, dtype=np.uint64)
+    ay = np.ascontiguousarray(ys, dtype=np.uint64)
+    az = np.ascontiguousarray(zs, dtype=np.uint64)
+    cn = np.ascontiguousarray(counts, dtype=np.int32)
+    u64 = C.POINTER(C.c_uint64)
+    rc = lib.vmp_scan_set_points_device_batch(
+        arr, n, ax.ctypes.data_as(u64), ay.ctypes.data_as(u64), az.ctypes.data_as(u64),
+        cn.ctypes.data_as(C.POINTER(C.c_int32)),
+    )
+    if rc != 0:
+        raise VmpError(f"vmp_scan_set_points_device_batch failed ({rc})")
+    for r, c in zip(registrars, cn):
+        r.n = int(c)
+
+
+def register_batch_submit(registrars):
+    """Launch a batch without waiting (the next submit or a wait completes it)."""
+    lib = registrars[0].lib
+    n = len(registrars)
+    arr = (C.c_void_p * n)(*[r.h.value for r in registrars])
+    rc = lib.vmp_register_batch_submit(arr, n)
+    if rc != 0:
+        raise VmpError(f"vmp_register_batch_submit failed ({rc})")
+
+
+def register_batch_wait(ctx, n):
+    """Complete the batch in flight; statistics of the last completed batch."""
+    o = (_lib.ObsStats * n)()
+    m = (_lib.MapStats * n)()
+    rc = ctx.lib.vmp_register_batch_wait(ctx.h, n, o, m)
+    if rc != 0:
+        raise VmpError(f"vmp_register_batch_wait failed ({rc})")
+    return o, m
