@@ -166,7 +166,12 @@ int vmp_register_batch_wait(vmp_ctx* ctx, int32_t n, vmp_obs_stats* ost, vmp_map
 
 /* Launch-shape parameters of a context (no reference counterpart):
  *   "obs_ppt"  points per thread of the observe kernel, 1..64; 0 = chosen by
- *              scan size (the default).  Applies to later calls. */
+ *              scan size (the default).
+ *   "obs_arith" arithmetic of the observe pass: 0 = compensated fp32 in a
+ *              frame anchored at the scan's pose voxel (the default),
+ *              1 = plain fp32 world frame, a precision-study arm whose
+ *              fallback counter counts keys that differ from the fp64 key.
+ * Applies to later calls. */
 int vmp_ctx_set_param(vmp_ctx* ctx, const char* name, int64_t value);
 
 /* Device events around the last call's kernels (for roofline timing). */

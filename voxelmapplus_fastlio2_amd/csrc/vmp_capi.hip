@@ -148,6 +148,9 @@ struct vmp_ctx {
   int obs_launches = 0;
   // launch shape of the observe kernel: points per thread, 0 = by scan size
   int obs_ppt = 0;
+  // arithmetic of the observe kernel: 0 = compensated fp32 in the anchor
+  // frame (registration), 1 = plain fp32 world frame (precision study arm)
+  int obs_arith = 0;
   // a batch submitted without waiting (vmp_register_batch_submit)
   bool fl = false;
   int fl_mode = 0;
@@ -651,8 +654,13 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
     hipLaunchKernelGGL(k_zero_lists, dim3(n), dim3(64), 0, st, w.d_obsctr, 8);
     const bool tm = c->timing && c->tev_used + 2 <= 2 * 64;
     if (tm) hipEventRecord(c->tev[c->tev_used], st);
-    if (ext)
+    const bool w32 = c->obs_arith == 1;
+    if (ext && w32)
+      hipLaunchKernelGGL((observe_kernel<12, true>), dim3(nblocks), dim3(kObsThreads), 0, st, w.d_jobs, n);
+    else if (ext)
       hipLaunchKernelGGL(observe_kernel<12>, dim3(nblocks), dim3(kObsThreads), 0, st, w.d_jobs, n);
+    else if (w32)
+      hipLaunchKernelGGL((observe_kernel<6, true>), dim3(nblocks), dim3(kObsThreads), 0, st, w.d_jobs, n);
     else
       hipLaunchKernelGGL(observe_kernel<6>, dim3(nblocks), dim3(kObsThreads), 0, st, w.d_jobs, n);
     if (tm) {
@@ -1121,6 +1129,11 @@ extern "C" int vmp_ctx_set_param(vmp_ctx* c, const char* name, int64_t value) {
   if (std::strcmp(name, "obs_ppt") == 0) {
     if (value < 0 || value > 64) return VMP_EINVAL;
     c->obs_ppt = int(value);
+    return VMP_OK;
+  }
+  if (std::strcmp(name, "obs_arith") == 0) {
+    if (value < 0 || value > 1) return VMP_EINVAL;
+    c->obs_arith = int(value);
     return VMP_OK;
   }
   return VMP_EINVAL;

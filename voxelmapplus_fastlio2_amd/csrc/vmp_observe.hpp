@@ -90,7 +90,11 @@ __device__ inline int32_t hash_find(const HashEntry* table, uint32_t mask, uint6
   return -1;
 }
 
-template <int NJ>
+// W32 selects the plain fp32 world-frame arithmetic of the precision study
+// (p_w = fl(R) p + fl(t), key = floorf(p_w / vs), residual against the fp32
+// world mean); the fourth counter then counts keys that differ from the
+// fp64 key.  The registration path uses W32 = false.
+template <int NJ, bool W32 = false>
 __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __restrict__ jobs, int njobs) {
   constexpr int NH = NJ * (NJ + 1) / 2;
   constexpr int NACC = NH + NJ;
@@ -117,52 +121,69 @@ __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __re
     const bool zfix = (pz == 0.0f);  // calcBodyCov mutation, commons.cpp:20-23
     if (zfix) pz = 0.001f;
     const float p[3] = {px, py, pz};
-    // ---- compensated transform into the anchor frame
-    float uh[3], ul[3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      float s = P.th[r], c = P.tl[r];
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const float hi = P.Rh[r * 3 + q];
-        const float prod = hi * p[q];
-        const float perr = __builtin_fmaf(hi, p[q], -prod);
-        float s2, e2;
-        two_sum(s, prod, s2, e2);
-        s = s2;
-        c += perr + e2 + P.Rl[r * 3 + q] * p[q];
-      }
-      float h2, l2;
-      two_sum(s, c, h2, l2);
-      uh[r] = h2;
-      ul[r] = l2;
-    }
-    // ---- key (bit-exact with the reference's fp64 floor(p_w / vs))
+    float uh[3], ul[3], kl[3], pw[3];
     int64_t key[3];
-    float kl[3];
-    bool near_face = false;
+    if constexpr (W32) {
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      const float f = (uh[r] + ul[r]) * P.inv_vs;
-      kl[r] = floorf(f);
-      const float fr = f - kl[r];
-      near_face |= (fr < 1e-4f) || (fr > 1.0f - 1e-4f);
-    }
-    if (near_face) {
-      ++c_fb;
+      for (int r = 0; r < 3; ++r) {
+        const float* W = P.rwl32 + r * 3;
+        pw[r] = W[0] * px + W[1] * py + W[2] * pz + float(P.p_wl[r]);
+        key[r] = int64_t(floorf(pw[r] / P.vs));
+      }
       const double pd[3] = {double(px), double(py), zfix ? 0.001 : double(pz)};
+      bool differs = false;
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
         double s = P.r_wl[r * 3] * pd[0];
         s += P.r_wl[r * 3 + 1] * pd[1];
         s += P.r_wl[r * 3 + 2] * pd[2];
-        const double pw = s + P.p_wl[r];
-        key[r] = int64_t(floor(pw / P.vs64));
-        kl[r] = float(key[r] - P.anchor[r]);
+        differs |= int64_t(floor((s + P.p_wl[r]) / P.vs64)) != key[r];
       }
+      c_fb += differs ? 1 : 0;
     } else {
+      // ---- compensated transform into the anchor frame
 #pragma unroll
-      for (int r = 0; r < 3; ++r) key[r] = int64_t(kl[r]) + P.anchor[r];
+      for (int r = 0; r < 3; ++r) {
+        float s = P.th[r], c = P.tl[r];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const float hi = P.Rh[r * 3 + q];
+          const float prod = hi * p[q];
+          const float perr = __builtin_fmaf(hi, p[q], -prod);
+          float s2, e2;
+          two_sum(s, prod, s2, e2);
+          s = s2;
+          c += perr + e2 + P.Rl[r * 3 + q] * p[q];
+        }
+        float h2, l2;
+        two_sum(s, c, h2, l2);
+        uh[r] = h2;
+        ul[r] = l2;
+      }
+      // ---- key (bit-exact with the reference's fp64 floor(p_w / vs))
+      bool near_face = false;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const float f = (uh[r] + ul[r]) * P.inv_vs;
+        kl[r] = floorf(f);
+        const float fr = f - kl[r];
+        near_face |= (fr < 1e-4f) || (fr > 1.0f - 1e-4f);
+      }
+      if (near_face) {
+        ++c_fb;
+        const double pd[3] = {double(px), double(py), zfix ? 0.001 : double(pz)};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          double s = P.r_wl[r * 3] * pd[0];
+          s += P.r_wl[r * 3 + 1] * pd[1];
+          s += P.r_wl[r * 3 + 2] * pd[2];
+          key[r] = int64_t(floor((s + P.p_wl[r]) / P.vs64));
+          kl[r] = float(key[r] - P.anchor[r]);
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) key[r] = int64_t(kl[r]) + P.anchor[r];
+      }
     }
     float4 slot = J.slots[i];
     bool in_range = key_ok(key[0]) && key_ok(key[1]) && key_ok(key[2]);
@@ -181,11 +202,20 @@ __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __re
       const uint32_t fl = __float_as_uint(h0.w);
       if (fl & kPlane) {
         ++c_plane;
-        const float qx = (uh[0] - kl[0] * P.vs) + ul[0];
-        const float qy = (uh[1] - kl[1] * P.vs) + ul[1];
-        const float qz = (uh[2] - kl[2] * P.vs) + ul[2];
         const float nx = h1.x, ny = h1.y, nz = h1.z;
-        const float res = nx * (qx - h0.x) + ny * (qy - h0.y) + nz * (qz - h0.z);
+        float res;
+        if constexpr (W32) {
+          // the plane mean held as an fp32 world point
+          const float mx = float(double(key[0]) * P.vs64 + double(h0.x));
+          const float my = float(double(key[1]) * P.vs64 + double(h0.y));
+          const float mz = float(double(key[2]) * P.vs64 + double(h0.z));
+          res = nx * (pw[0] - mx) + ny * (pw[1] - my) + nz * (pw[2] - mz);
+        } else {
+          const float qx = (uh[0] - kl[0] * P.vs) + ul[0];
+          const float qy = (uh[1] - kl[1] * P.vs) + ul[1];
+          const float qz = (uh[2] - kl[2] * P.vs) + ul[2];
+          res = nx * (qx - h0.x) + ny * (qy - h0.y) + nz * (qz - h0.z);
+        }
         // sigma^2 = n^T (R C_l R^T + [p]x Prr [p]x^T + Ppp) n   (plane_cov term is 0)
         const float nn = nx * nx + ny * ny + nz * nz;
         const float nw = nx * w0 + ny * w1 + nz * w2;
