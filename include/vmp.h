@@ -165,14 +165,28 @@ int vmp_register_batch_submit(vmp_session** s, int32_t n);
 int vmp_register_batch_wait(vmp_ctx* ctx, int32_t n, vmp_obs_stats* ost, vmp_map_stats* mst);
 
 /* Launch-shape parameters of a context (no reference counterpart):
- *   "obs_ppt"  points per thread of the observe kernel, 1..64; 0 = chosen by
- *              scan size (the default).
+ *   "obs_ppt"  points per thread of the observe kernel, 1..64; 0 = taken
+ *              from the size-bucket table (the default)
+ *   "obs_ppt_bucket0".."obs_ppt_bucket3"  table entries for scans of < 8k,
+ *              < 32k, < 128k and more points (default 2; set by
+ *              vmp_autotune_observe)
+ *   "filter_graph" 1 = the filter stage of a step (IESKF begin, the
+ *              iterations' observe/reduce/step launches, end) replays a
+ *              captured hipGraph (the default), 0 = launched one by one
+ *              (also while per-launch timing is on).
  *   "obs_arith" arithmetic of the observe pass: 0 = compensated fp32 in a
  *              frame anchored at the scan's pose voxel (the default),
  *              1 = plain fp32 world frame, a precision-study arm whose
  *              fallback counter counts keys that differ from the fp64 key.
  * Applies to later calls. */
 int vmp_ctx_set_param(vmp_ctx* ctx, const char* name, int64_t value);
+
+/* Autotuner of the observe launch shape (no reference counterpart): times
+ * the observe pass on the session's current scan and state for 1, 2, 4 and
+ * 8 points per thread (median of reps passes), stores the fastest in the
+ * context's table for the scan's size bucket and returns it (cand_ms: the
+ * four medians, may be NULL).  The residual slots are restored afterwards. */
+int vmp_autotune_observe(vmp_session* s, int32_t reps, int32_t* best_ppt, float* cand_ms);
 
 /* Device events around the last call's kernels (for roofline timing). */
 int vmp_session_timing(vmp_session* s, double* observe_ms, double* map_ms, int32_t* observe_launches);

@@ -64,3 +64,70 @@ def test_batch_matches_alone():
         g.close()
     ctx_a.close()
     ctx_b.close()
+
+
+def test_filter_graph_matches_eager():
+    """The filter stage replayed from its captured hipGraph (the default)
+    gives bitwise the results of launching its kernels one by one, over
+    several scans with the eager map stage in between every replay, alone
+    and in a batch."""
+    nscans = 5
+    ctx_g = Context(0)
+    ctx_e = Context(0)
+    ctx_e.set_param("filter_graph", 0)
+    graph = [make(ctx_g, *sp) for sp in SPECS]
+    eager = [make(ctx_e, *sp) for sp in SPECS]
+    for k in range(1, nscans):
+        for group in (graph, eager):
+            mot = [seq.motion(k) for seq, _ in group]
+            apply_motion_batch([g for _, g in group], [a for a, _ in mot], [b for _, b in mot], QDIAG)
+            for seq, g in group:
+                g.set_points(*seq.scan(k))
+            if k % 2:
+                register_batch([g for _, g in group])
+            else:
+                for _, g in group:
+                    g.register()
+        for i, ((_, ga), (_, gb)) in enumerate(zip(graph, eager)):
+            sa, Pa = ga.get_state()
+            sb, Pb = gb.get_state()
+            assert np.array_equal(sa, sb), f"session {i} scan {k}: state differs graph vs eager"
+            assert np.array_equal(Pa, Pb), f"session {i} scan {k}: P differs graph vs eager"
+        print(f"scan {k}: filter graph replay identical to eager launches", flush=True)
+    for _, g in graph + eager:
+        g.close()
+    ctx_g.close()
+    ctx_e.close()
+
+
+def test_autotune_observe_leaves_results_unchanged():
+    """vmp_autotune_observe picks points per thread for the scan's size
+    bucket and restores the residual slots its timing passes wrote: the next
+    registration is bitwise the one of a context given the same table by
+    hand, with no tuning passes."""
+    scene, n, seed = STREET, 30000, 9
+    ctx_t = Context(0)
+    ctx_r = Context(0)
+    seq_t, g_t = make(ctx_t, scene, n, seed)
+    seq_r, g_r = make(ctx_r, scene, n, seed)
+    for k in range(1, 4):
+        for seq, g in ((seq_t, g_t), (seq_r, g_r)):
+            g.apply_motion(*seq.motion(k), QDIAG)
+            g.set_points(*seq.scan(k))
+        if k == 1:
+            best, ms = ctx_t.autotune_observe(g_t, reps=5)
+            print("autotune", best, ms, flush=True)
+            assert best in (1, 2, 4, 8) and all(t > 0 for t in ms)
+            ctx_r.set_param("obs_ppt_bucket1", best)  # 30k points: the < 32k bucket
+        g_t.register()
+        g_r.register()
+        st, Pt = g_t.get_state()
+        sr, Pr = g_r.get_state()
+        assert np.array_equal(st, sr) and np.array_equal(Pt, Pr), f"scan {k}: tuned context differs"
+        vt, rt, _ = g_t.export_slots(n)
+        vr, rr, _ = g_r.export_slots(n)
+        assert np.array_equal(vt, vr) and np.array_equal(rt[vt], rr[vr])
+    for g in (g_t, g_r):
+        g.close()
+    ctx_t.close()
+    ctx_r.close()

@@ -126,6 +126,61 @@ def test_sequence_teacher_forced(ctx, scene, npts, nscans):
             assert r["mean_bad"] <= r["plane_flag_diff"] * 8 and r["norm_bad"] <= r["plane_flag_diff"] * 8
 
 
+def test_extrinsic_parity(ctx):
+    """estimate_ext = true (the 12-column Jacobian) with a non-identity
+    LiDAR-IMU extrinsic: one observe pass (H, b) and teacher-forced scans."""
+    from scipy.spatial.transform import Rotation
+
+    npts = 20000
+    seq = Sequence(ROOM, SEED0 + 21, npts)
+    cfg = LIOConfig(max_scan_points=npts, estimate_ext=1)
+    Rl, tl = seq.pose(0)
+    Rx = Rotation.from_rotvec([0.02, -0.01, 0.03]).as_matrix()
+    tx = np.array([0.05, -0.02, 0.1])
+    R = Rl @ Rx.T
+    s0 = state_vector(R, tl - R @ tx)
+    s0[12:21] = Rx.T.reshape(-1)
+    s0[21:24] = tx
+    P0 = initial_covariance()
+    x0, y0, z0 = seq.scan(0)
+    orc = Oracle(cfg)
+    orc.set_state(s0, P0)
+    orc.build(x0, y0, z0)
+    with Registrar(ctx, cfg) as g:
+        g.set_state(s0, P0)
+        g.set_points(x0, y0, z0)
+        g.build()
+        for k in range(1, 6):
+            orc.apply_motion(*seq.motion(k), QDIAG)
+            sp, Pp = orc.get_state()
+            x, y, z = seq.scan(k)
+            orc.set_points(x, y, z)
+            Ho, bo, _ = orc.observe(sp)
+            vo, _, _ = orc.export_slots(npts)
+            g.set_state(sp, Pp)
+            g.set_points(x, y, z)
+            H, b, _ = g.observe(sp)
+            v, _, _ = g.export_slots(npts)
+            flips = int((vo != v).sum())
+            eH = np.linalg.norm(H - Ho) / np.linalg.norm(Ho)
+            eb = np.linalg.norm(b - bo) / max(np.linalg.norm(bo), 1e-12)
+            assert np.abs(Ho[6:12, 6:12]).max() > 0, "extrinsic block populated"
+            it_o = orc.register(x, y, z)
+            so, Po = orc.get_state()
+            ost, _ = g.register()
+            sg, Pg = g.get_state()
+            ep, er = state_err(so, sg)
+            ex = float(np.linalg.norm(so[21:24] - sg[21:24]))
+            log(f"ext scan {k}: flips {flips} H rel {eH:.2e} b rel {eb:.2e} iters o/g {it_o}/{ost.iterations} "
+                f"pos {ep:.2e} rot {er:.2e} pos_ext {ex:.2e}")
+            assert flips <= max(3, npts // 2000)
+            if flips == 0:
+                assert eH < 1e-6 and eb < 1e-4
+            assert ep <= 1e-4 and er <= 1e-5 and ex <= 1e-4
+            r = compare_maps(orc.export_map(), g.export_map())
+            assert r["plane_flag_diff"] <= max(2, r["n"] // 1000)
+
+
 @pytest.mark.parametrize("capacity", [30000, 18000])
 def test_lru_eviction_parity(ctx, capacity):
     """Map capacity below the growth of the street map: every scan evicts the

@@ -218,3 +218,59 @@ def test_full_update_converges_to_true_pose(setup):
     assert np.linalg.norm(s[0:3] - t0) < 5e-3  # 3.5 cm start, 3 mm noise-limited
     dR = s[3:12].reshape(3, 3).T @ R0.T
     assert np.linalg.norm(Rotation.from_matrix(dR).as_rotvec()) < 1e-3  # 8 mrad start
+
+
+def test_observe_with_extrinsic_matches_finite_differences(setup):
+    """estimate_ext = true (lio_builder.cpp:283-307): all twelve Jacobian
+    columns (position, rotation, extrinsic rotation, extrinsic translation)
+    come from central differences of r(x [+] e) = n.(R (R_ext p + t_ext) + t - m)
+    per valid slot, with a non-identity extrinsic; H = sum w J^T J and
+    b = sum w J^T r are rebuilt from them."""
+    R0, t0, pl, s0, P0 = setup
+    Rx = Rotation.from_rotvec([0.02, -0.01, 0.03]).as_matrix()
+    tx = np.array([0.05, -0.02, 0.1])
+    R = R0 @ Rx.T  # keeps the LiDAR pose R0, t0 the points were generated at
+    t = t0 - R @ tx
+    s = state_vector(R, t)
+    s[12:21] = Rx.T.reshape(-1)
+    s[21:24] = tx
+    o = oracle.Oracle(LIOConfig(max_scan_points=len(pl), estimate_ext=1))
+    o.set_state(s, P0)
+    o.build(pl[:, 0], pl[:, 1], pl[:, 2])
+    o.set_points(pl[:, 0], pl[:, 1], pl[:, 2])
+    d = np.zeros(23)
+    d[0:12] = [0.01, -0.008, 0.005, 0.003, -0.002, 0.004, 0.002, 0.001, -0.003, 0.01, 0.005, -0.004]
+    s1 = oracle.boxplus(s, d)
+    H, b, eff = o.observe(s1)
+    v, r, nrm = o.export_slots(len(pl))
+    mp = o.export_map()
+    vox = {tuple(k): i for i, k in enumerate(mp["keys"].tolist())}
+    plq = pl.astype(np.float64)
+
+    def world(st):
+        Rs, Re = st[3:12].reshape(3, 3).T, st[12:21].reshape(3, 3).T
+        return plq @ (Rs @ Re).T + (Rs @ st[21:24] + st[0:3])
+
+    pw = world(s1)
+    idx = np.flatnonzero(v)
+    keys = np.floor(pw[idx] / VS).astype(np.int64)
+    rows = np.array([vox[tuple(k)] for k in keys.tolist()])
+    m, n = mp["dbl"][rows, 0:3], mp["dbl"][rows, 3:6]
+    np.testing.assert_allclose(np.sum(n * (pw[idx] - m), 1), r[idx], atol=1e-12)
+    eps = 1e-6
+    J = np.zeros((len(idx), 12))
+    for k in range(12):
+        e = np.zeros(23)
+        e[k] = eps
+        rp = np.sum(n * (world(oracle.boxplus(s1, e))[idx] - m), 1)
+        rm = np.sum(n * (world(oracle.boxplus(s1, -e))[idx] - m), 1)
+        J[:, k] = (rp - rm) / (2 * eps)
+    Rwl = s1[3:12].reshape(3, 3).T @ s1[12:21].reshape(3, 3).T
+    rc = np.array([nn @ Rwl @ body_cov(p) @ Rwl.T @ nn for p, nn in zip(plq[idx], n)])
+    w = np.where(rc < 2e-4, 5000.0, 1.0 / rc)
+    He = (J * w[:, None]).T @ J
+    be = J.T @ (w * r[idx])
+    assert eff == len(idx) and eff > 0.5 * len(pl)
+    assert np.abs(H[6:12, 6:12]).max() > 1e-3 * np.abs(H).max(), "extrinsic block populated"
+    np.testing.assert_allclose(H, He, rtol=1e-6, atol=1e-7 * np.abs(He).max())
+    np.testing.assert_allclose(b, be, rtol=1e-6, atol=1e-7 * np.abs(be).max())

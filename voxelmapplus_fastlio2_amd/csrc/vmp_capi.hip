@@ -146,11 +146,26 @@ struct vmp_ctx {
   int64_t acc_obs_n = 0, acc_map_n = 0;
   int64_t acc_obs_points = 0, acc_obs_hits = 0;
   int obs_launches = 0;
-  // launch shape of the observe kernel: points per thread, 0 = by scan size
+  // launch shape of the observe kernel: points per thread; obs_ppt > 0
+  // overrides the per-size-bucket table that vmp_autotune_observe fills
   int obs_ppt = 0;
+  int ppt_tab[4] = {2, 2, 2, 2};
   // arithmetic of the observe kernel: 0 = compensated fp32 in the anchor
   // frame (registration), 1 = plain fp32 world frame (precision study arm)
   int obs_arith = 0;
+  // the filter stage of a step (IESKF begin, max_iter x (observe, reduce,
+  // step), end) captured as one hipGraph per (sessions, observe grid, ...)
+  bool use_graph = true;
+  struct FilterGraph {
+    hipGraphExec_t exec = nullptr;
+    int n = 0, grid = 0, iters = 0, ext = 0, arith = 0;
+    uint64_t ws_gen = 0;
+    uint64_t last_use = 0;
+  };
+  FilterGraph graphs[8];
+  uint64_t graph_clock = 0;
+  uint64_t ws_gen = 1;  // bumped when the batch workspace is reallocated
+  int64_t graph_launches = 0, graph_builds = 0;
   // a batch submitted without waiting (vmp_register_batch_submit)
   bool fl = false;
   int fl_mode = 0;
@@ -163,10 +178,11 @@ struct vmp_ctx {
 
 // Points per thread of the observe kernel for a scan of n points.  It depends
 // only on n (never on the batch), so a session's block decomposition, and
-// with it the summation order of H and b, is the same alone and in a batch.
+// with it the summation order of H and b, is the same alone and in a batch
+// (for one table: sessions compared bit by bit must share it).
+static int ppt_bucket(int n) { return n < 8192 ? 0 : (n < 32768 ? 1 : (n < 131072 ? 2 : 3)); }
 static int obs_ppt_for(const vmp_ctx* c, int n) {
-  (void)n;
-  return c->obs_ppt > 0 ? c->obs_ppt : 2;
+  return c->obs_ppt > 0 ? c->obs_ppt : c->ppt_tab[ppt_bucket(n)];
 }
 
 struct vmp_session {
@@ -226,6 +242,8 @@ extern "C" int vmp_ctx_destroy(vmp_ctx* ctx) {
   }
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
+  for (auto& g : ctx->graphs)
+    if (g.exec) hipGraphExecDestroy(g.exec);
   ctx->ws.release();
   if (ctx->ws.motion_done) hipEventDestroy(ctx->ws.motion_done);
   ctx->rt.release();
@@ -246,6 +264,7 @@ static int ws_reserve(vmp_ctx* c, int64_t points, int nsess) {
   if (points <= w.cap_points && nsess <= w.cap_sess) return VMP_OK;
   settle(c);  // a batch in flight still reads the old workspace
   hipStreamSynchronize(c->stream);
+  c->ws_gen++;  // captured filter graphs hold the old workspace pointers
   const int64_t np = std::max<int64_t>(points, w.cap_points);
   const int ns = std::max(nsess, w.cap_sess);
   w.release();
@@ -632,6 +651,55 @@ static int stage_batch(vmp_ctx* c, vmp_session** ss, int n, int64_t* ntot_out, i
   return VMP_OK;
 }
 
+// Instantiated filter-stage graph for this step's shape, captured on first
+// use.  The observe grid is rounded up (to a multiple of 1/8 of its power of
+// two) so that one graph serves steps whose block totals differ a little;
+// the surplus blocks return at once.  The job tables the kernels read are
+// uploaded before each launch, so a replay sees the step's own sessions.
+template <class Stage>
+static hipGraphExec_t filter_graph(vmp_ctx* c, int n, int nblocks, int iters, bool ext, Stage&& stage) {
+  int p2 = 1;
+  while (p2 < nblocks) p2 <<= 1;
+  const int quant = std::max(1, p2 / 8);
+  const int grid = (nblocks + quant - 1) / quant * quant;
+  vmp_ctx::FilterGraph* hit = nullptr;
+  vmp_ctx::FilterGraph* victim = &c->graphs[0];
+  for (auto& g : c->graphs) {
+    if (g.exec && g.ws_gen == c->ws_gen && g.n == n && g.grid == grid && g.iters == iters && g.ext == int(ext) &&
+        g.arith == c->obs_arith) {
+      hit = &g;
+      break;
+    }
+    if (!g.exec || g.last_use < victim->last_use) victim = &g;
+  }
+  if (!hit) {
+    hipStream_t st = c->stream;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+    const int saved = c->obs_launches;
+    stage(grid);
+    c->obs_launches = saved;
+    hipGraph_t graph = nullptr;
+    if (hipStreamEndCapture(st, &graph) != hipSuccess || !graph) return nullptr;
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess || !exec) return nullptr;
+    if (victim->exec) hipGraphExecDestroy(victim->exec);
+    *victim = vmp_ctx::FilterGraph{};
+    victim->exec = exec;
+    victim->n = n;
+    victim->grid = grid;
+    victim->iters = iters;
+    victim->ext = int(ext);
+    victim->arith = c->obs_arith;
+    victim->ws_gen = c->ws_gen;
+    hit = victim;
+    c->graph_builds++;
+  }
+  hit->last_use = ++c->graph_clock;
+  return hit->exec;
+}
+
 static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
   BatchWs& w = c->ws;
   hipStream_t st = c->stream;
@@ -650,19 +718,19 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
   if (hipMemcpyAsync(w.d_obsstat, w.h_obsstat, sizeof(int32_t*) * n, hipMemcpyHostToDevice, st) != hipSuccess)
     return VMP_EDEVICE;
   const bool ext = ss[0]->cfg.estimate_ext != 0;
-  auto observe = [&]() {
+  auto observe = [&](int grid) {
     hipLaunchKernelGGL(k_zero_lists, dim3(n), dim3(64), 0, st, w.d_obsctr, 8);
     const bool tm = c->timing && c->tev_used + 2 <= 2 * 64;
     if (tm) hipEventRecord(c->tev[c->tev_used], st);
     const bool w32 = c->obs_arith == 1;
     if (ext && w32)
-      hipLaunchKernelGGL((observe_kernel<12, true>), dim3(nblocks), dim3(kObsThreads), 0, st, w.d_jobs, n);
+      hipLaunchKernelGGL((observe_kernel<12, true>), dim3(grid), dim3(kObsThreads), 0, st, w.d_jobs, n);
     else if (ext)
-      hipLaunchKernelGGL(observe_kernel<12>, dim3(nblocks), dim3(kObsThreads), 0, st, w.d_jobs, n);
+      hipLaunchKernelGGL(observe_kernel<12>, dim3(grid), dim3(kObsThreads), 0, st, w.d_jobs, n);
     else if (w32)
-      hipLaunchKernelGGL((observe_kernel<6, true>), dim3(nblocks), dim3(kObsThreads), 0, st, w.d_jobs, n);
+      hipLaunchKernelGGL((observe_kernel<6, true>), dim3(grid), dim3(kObsThreads), 0, st, w.d_jobs, n);
     else
-      hipLaunchKernelGGL(observe_kernel<6>, dim3(nblocks), dim3(kObsThreads), 0, st, w.d_jobs, n);
+      hipLaunchKernelGGL(observe_kernel<6>, dim3(grid), dim3(kObsThreads), 0, st, w.d_jobs, n);
     if (tm) {
       hipEventRecord(c->tev[c->tev_used + 1], st);
       c->tev_used += 2;
@@ -675,19 +743,32 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
   };
   if (mode & kStepObserveOnce) {
     hipLaunchKernelGGL(k_obs_prep, dim3(n), dim3(64), 0, st, w.d_flist);
-    observe();
+    observe(nblocks);
     return VMP_OK;
   }
   hipEventRecord(c->ev[0], st);
   if (mode & kStepIeskf) {
     int max_iter = 1;
     for (int i = 0; i < n; ++i) max_iter = std::max(max_iter, ss[i]->cfg.max_iter);
-    hipLaunchKernelGGL(k_ieskf_begin, dim3(n), dim3(64), 0, st, w.d_flist);
-    for (int it = 0; it < max_iter; ++it) {
-      observe();
-      hipLaunchKernelGGL(k_ieskf_step, dim3(n), dim3(64), 0, st, w.d_flist, w.d_obsctr, w.d_obsstat);
+    auto filter_stage = [&](int grid) {
+      hipLaunchKernelGGL(k_ieskf_begin, dim3(n), dim3(64), 0, st, w.d_flist);
+      for (int it = 0; it < max_iter; ++it) {
+        observe(grid);
+        hipLaunchKernelGGL(k_ieskf_step, dim3(n), dim3(64), 0, st, w.d_flist, w.d_obsctr, w.d_obsstat);
+      }
+      hipLaunchKernelGGL(k_ieskf_end, dim3(n), dim3(64), 0, st, w.d_flist);
+    };
+    // per-launch timing (vmp_session_set_timing) records events between the
+    // launches, so it takes the launch-by-launch path
+    hipGraphExec_t exec = (c->use_graph && !c->timing) ? filter_graph(c, n, nblocks, max_iter, ext, filter_stage)
+                                                       : nullptr;
+    if (exec) {
+      if (hipGraphLaunch(exec, st) != hipSuccess) return VMP_EDEVICE;
+      c->graph_launches++;
+      c->obs_launches += max_iter;
+    } else {
+      filter_stage(nblocks);
     }
-    hipLaunchKernelGGL(k_ieskf_end, dim3(n), dim3(64), 0, st, w.d_flist);
   }
   hipEventRecord(c->ev[1], st);
   hipEventRecord(c->ev[2], st);
@@ -1060,6 +1141,70 @@ extern "C" int vmp_observe(vmp_session* s, const vmp_state* x, double* H, double
   return VMP_OK;
 }
 
+// Observe-pass autotuner: times the pass on this session's current scan and
+// state for 1, 2, 4 and 8 points per thread (median of reps timed passes
+// after one untimed one) and stores the fastest in the context's table for
+// the scan's size bucket (< 8k, < 32k, < 128k, larger).  The residual slots
+// the passes write are restored, so the session's next registration is the
+// same as without tuning.
+extern "C" int vmp_autotune_observe(vmp_session* s, int32_t reps, int32_t* best_ppt, float* cand_ms) {
+  if (!s || reps < 1 || reps > 1000) return VMP_EINVAL;
+  if (!s->has_points || s->n < 1) {
+    s->err = "vmp_autotune_observe before vmp_scan_set_points";
+    return VMP_ESTATE;
+  }
+  vmp_ctx* c = s->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  VMP_HIP(hipSetDevice(c->device));
+  settle(c);
+  if (ws_reserve(c, std::max(s->n, 1), 1) != VMP_OK) return VMP_ENOMEM;
+  hipStream_t st = c->stream;
+  float4* saved = nullptr;
+  VMP_HIP(hipMalloc(&saved, sizeof(float4) * size_t(s->n)));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  int rc = hipMemcpyAsync(saved, s->slots, sizeof(float4) * size_t(s->n), hipMemcpyDeviceToDevice, st) == hipSuccess
+               ? VMP_OK
+               : VMP_EDEVICE;
+  const int cands[4] = {1, 2, 4, 8};
+  const int keep = c->obs_ppt;
+  float med[4] = {0, 0, 0, 0};
+  vmp_session* one[1] = {s};
+  for (int ci = 0; ci < 4 && rc == VMP_OK; ++ci) {
+    c->obs_ppt = cands[ci];
+    std::vector<float> ms;
+    for (int r = 0; r <= reps && rc == VMP_OK; ++r) {
+      hipEventRecord(e0, st);
+      if (enqueue_step(c, one, 1, kStepObserveOnce) != VMP_OK) rc = VMP_EDEVICE;
+      hipEventRecord(e1, st);
+      if (hipEventSynchronize(e1) != hipSuccess) rc = VMP_EDEVICE;
+      float t = 0.f;
+      if (rc == VMP_OK && r > 0 && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms.push_back(t);
+    }
+    if (!ms.empty()) {
+      std::sort(ms.begin(), ms.end());
+      med[ci] = ms[ms.size() / 2];
+    }
+  }
+  c->obs_ppt = keep;
+  if (hipMemcpyAsync(s->slots, saved, sizeof(float4) * size_t(s->n), hipMemcpyDeviceToDevice, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    rc = VMP_EDEVICE;
+  hipFree(saved);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc != VMP_OK) return rc;
+  int best = 0;
+  for (int ci = 1; ci < 4; ++ci)
+    if (med[ci] < med[best]) best = ci;
+  c->ppt_tab[ppt_bucket(s->n)] = cands[best];
+  if (best_ppt) *best_ppt = cands[best];
+  if (cand_ms)
+    for (int ci = 0; ci < 4; ++ci) cand_ms[ci] = med[ci];
+  return VMP_OK;
+}
+
 extern "C" int vmp_ieskf_update(vmp_session* s, vmp_obs_stats* st) {
   if (!s) return VMP_EINVAL;
   vmp_session* one[1] = {s};
@@ -1129,6 +1274,16 @@ extern "C" int vmp_ctx_set_param(vmp_ctx* c, const char* name, int64_t value) {
   if (std::strcmp(name, "obs_ppt") == 0) {
     if (value < 0 || value > 64) return VMP_EINVAL;
     c->obs_ppt = int(value);
+    return VMP_OK;
+  }
+  if (std::strncmp(name, "obs_ppt_bucket", 14) == 0 && name[14] >= '0' && name[14] <= '3' && name[15] == 0) {
+    if (value < 1 || value > 64) return VMP_EINVAL;
+    c->ppt_tab[name[14] - '0'] = int(value);
+    return VMP_OK;
+  }
+  if (std::strcmp(name, "filter_graph") == 0) {
+    if (value < 0 || value > 1) return VMP_EINVAL;
+    c->use_graph = value != 0;
     return VMP_OK;
   }
   if (std::strcmp(name, "obs_arith") == 0) {

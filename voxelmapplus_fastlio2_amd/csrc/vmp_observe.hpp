@@ -106,6 +106,7 @@ __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __re
   const ObsParams& P = *J.prm;
   if (!P.active) return;  // IESKF converged: reference leaves the loop
   const int lb = blockIdx.x - J.block0;
+  if (lb >= J.nblocks) return;  // a grid rounded up past the last session's blocks
   const int tid = threadIdx.x;
 
   double acc[NACC];

@@ -133,6 +133,16 @@ class Context:
         if rc != 0:
             raise VmpError(f"vmp_ctx_set_param({name}, {value}) failed ({rc})")
 
+    def autotune_observe(self, registrar, reps=10):
+        """Time the observe pass on the registrar's current scan for 1, 2, 4
+        and 8 points per thread; the fastest becomes this context's choice
+        for scans of that size bucket.  Returns (points per thread, medians
+        in ms for the four candidates)."""
+        best = C.c_int32()
+        ms = (C.c_float * 4)()
+        registrar._check(self.lib.vmp_autotune_observe(registrar.h, int(reps), C.byref(best), ms), "vmp_autotune_observe")
+        return best.value, [float(v) for v in ms]
+
 
 class Registrar:
     """One LiDAR sequence: voxel map, residual slots and filter state in HBM."""
