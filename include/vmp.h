@@ -164,6 +164,12 @@ int vmp_register_batch(vmp_session** s, int32_t n, vmp_obs_stats* ost, vmp_map_s
 int vmp_register_batch_submit(vmp_session** s, int32_t n);
 int vmp_register_batch_wait(vmp_ctx* ctx, int32_t n, vmp_obs_stats* ost, vmp_map_stats* mst);
 
+/* Size the context's batch workspace for batches of up to `points` points
+ * in total over up to `sessions` sessions, so no later batch allocates (no
+ * reference counterpart; without it the workspace grows on demand, with a
+ * quarter of headroom). */
+int vmp_ctx_reserve(vmp_ctx* ctx, int64_t points, int32_t sessions);
+
 /* Launch-shape parameters of a context (no reference counterpart):
  *   "obs_ppt"  points per thread of the observe kernel, 1..64; 0 = taken
  *              from the size-bucket table (the default)

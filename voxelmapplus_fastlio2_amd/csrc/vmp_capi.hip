@@ -259,13 +259,16 @@ extern "C" int vmp_ctx_destroy(vmp_ctx* ctx) {
 static int settle(vmp_ctx* c);
 
 // grow the batch workspace to hold `points` points and `nsess` sessions
-static int ws_reserve(vmp_ctx* c, int64_t points, int nsess) {
+static int ws_reserve(vmp_ctx* c, int64_t points, int nsess, bool exact = false) {
   BatchWs& w = c->ws;
   if (points <= w.cap_points && nsess <= w.cap_sess) return VMP_OK;
   settle(c);  // a batch in flight still reads the old workspace
   hipStreamSynchronize(c->stream);
   c->ws_gen++;  // captured filter graphs hold the old workspace pointers
-  const int64_t np = std::max<int64_t>(points, w.cap_points);
+  // growth keeps a quarter of headroom, so scans that vary in size do not
+  // reallocate (and recapture the filter graphs) step after step
+  const int64_t grown = (points > w.cap_points && !exact) ? points + points / 4 : points;
+  const int64_t np = std::max<int64_t>(grown, w.cap_points);
   const int ns = std::max(nsess, w.cap_sess);
   w.release();
   BatchBufs& b = w.b;
@@ -306,6 +309,12 @@ static int ws_reserve(vmp_ctx* c, int64_t points, int nsess) {
   ok &= dalloc(A, &b.len_out, n) == hipSuccess;
   ok &= dalloc(A, &b.ord_in, n) == hipSuccess;
   ok &= dalloc(A, &b.ord, n) == hipSuccess;
+  // taint-phase prefetch of up to 256k acting candidates per batch (the rest
+  // take the replay's own lookups)
+  b.pre_cap = int32_t(std::min<size_t>(n, size_t(1) << 18));
+  ok &= dalloc(A, &b.pre_x, size_t(b.pre_cap) * kPreItems) == hipSuccess;
+  ok &= dalloc(A, &b.pre_seg, size_t(b.pre_cap) * kPreItems) == hipSuccess;
+  ok &= dalloc(A, &b.pre_tn, size_t(b.pre_cap) * kPreItems) == hipSuccess;
   ok &= dalloc(A, &w.d_maps, size_t(ns)) == hipSuccess;
   ok &= dalloc(A, &w.d_si, size_t(ns)) == hipSuccess;
   ok &= dalloc(A, &w.d_jobs, size_t(ns)) == hipSuccess;
@@ -833,7 +842,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         if (rocprim::radix_sort_keys(w.temp, tb, b.cand_in, b.cand_out, size_t(ntot), 25, 49 + sbits, st) !=
             hipSuccess)
           return VMP_EDEVICE;
-        hipLaunchKernelGGL(k_cand_starts, dim3(1), dim3(1), 0, st, M, n, b);
+        hipLaunchKernelGGL(k_cand_starts, dim3(1), dim3(kCsThreads), 0, st, M, n, b);
         hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.cand_spec, 0, ntot);
         hipLaunchKernelGGL(k_spec_eval, dim3(gn), dim3(T), 0, st, M, b);
         tb = w.temp_bytes;
@@ -842,6 +851,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
           return VMP_EDEVICE;
         hipLaunchKernelGGL(k_spl_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b);
         hipLaunchKernelGGL(k_spl_keys, dim3(gn), dim3(T), 0, st, b);
+        hipLaunchKernelGGL(k_spl_prefetch, dim3(std::min(gn * kPreItems, 4096)), dim3(T), 0, st, M, S, b);
         hipLaunchKernelGGL(k_merge_replay, dim3(n), dim3(kMrThreads), 0, st, M, S, n, b);
       }
       hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.flags, 0, ntot);
@@ -1266,6 +1276,13 @@ extern "C" int vmp_session_timing(vmp_session* s, double* observe_ms, double* ma
   }
   if (observe_launches) *observe_launches = c->obs_launches;
   return VMP_OK;
+}
+
+extern "C" int vmp_ctx_reserve(vmp_ctx* c, int64_t points, int32_t sessions) {
+  if (!c || points < 1 || points > (int64_t(1) << 30) || sessions < 1) return VMP_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return VMP_EDEVICE;
+  return ws_reserve(c, points, sessions, true);
 }
 
 extern "C" int vmp_ctx_set_param(vmp_ctx* c, const char* name, int64_t value) {

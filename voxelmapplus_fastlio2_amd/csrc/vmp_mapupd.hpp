@@ -126,6 +126,16 @@ struct BatchBufs {
   uint32_t* len_out;
   int32_t* ord_in;
   int32_t* ord;
+  // per acting candidate (its index in spl, the first pre_cap of them): for
+  // each record r (its six eligible neighbours, then itself) and each
+  // neighbour j of r (j = 6: r itself), the voxel X there at the candidate's
+  // time, X's segment in this scan (-1: none or evicted) and X's first
+  // merge() call after that time; none of these depends on merges
+  int32_t* pre_x;
+  int32_t* pre_seg;
+  int32_t* pre_tn;
+  int32_t pre_cap;
+  int32_t pad1;
 };
 
 // merge-candidate key: session (15 bits) | point time (24 bits) | voxel (25 bits)
@@ -963,14 +973,31 @@ __global__ void k_end_scan(const MapDev* maps, const SessInfo* si, int nsess) {
 
 // start of each session's candidates in the sorted candidate list, and
 // their total (cnt[3])
-__global__ void k_cand_starts(const MapDev* maps, int nsess, BatchBufs b) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  int acc = 0;
-  for (int s = 0; s < nsess; ++s) {
-    b.cand_start[s] = acc;
-    acc += maps[s].ctr->ncand;
+// (one workgroup: every session's count is loaded at once, then an
+// exclusive scan in LDS, kCsThreads sessions per round)
+constexpr int kCsThreads = 256;
+__global__ void __launch_bounds__(kCsThreads) k_cand_starts(const MapDev* maps, int nsess, BatchBufs b) {
+  __shared__ int sc[kCsThreads];
+  __shared__ int carry;
+  const int tid = threadIdx.x;
+  if (tid == 0) carry = 0;
+  for (int s0 = 0; s0 < nsess; s0 += kCsThreads) {
+    const int s = s0 + tid;
+    const int v = s < nsess ? maps[s].ctr->ncand : 0;
+    sc[tid] = v;
+    __syncthreads();
+    for (int o = 1; o < kCsThreads; o <<= 1) {  // inclusive Hillis-Steele scan
+      const int u = tid >= o ? sc[tid - o] : 0;
+      __syncthreads();
+      sc[tid] += u;
+      __syncthreads();
+    }
+    if (s < nsess) b.cand_start[s] = carry + sc[tid] - v;
+    __syncthreads();
+    if (tid == kCsThreads - 1) carry += sc[tid];
+    __syncthreads();
   }
-  b.cnt[3] = acc;
+  if (tid == 0) b.cnt[3] = carry;
 }
 
 // ---------------------------------------------------- housekeeping -----

@@ -261,6 +261,46 @@ __global__ void k_spl_ranges(const MapDev* maps, SessInfo* si, int nsess, BatchB
   si[s].spl_hi = lb(c0 + maps[s].ctr->ncand);
 }
 
+// The merge-independent part of the taint phase of every acting candidate,
+// in parallel ahead of the replay (see BatchBufs::pre_x).
+constexpr int kPreItems = 49;  // 7 records x (6 neighbours + itself)
+__global__ void k_spl_prefetch(const MapDev* maps, const SessInfo* si, BatchBufs b) {
+  const int npre = min(b.cnt[4], b.pre_cap);
+  VMP_GRID_LOOP(it, npre * kPreItems) {
+    const int k = it / kPreItems, r = it % kPreItems, cj = r / 7, j = r % 7;
+    const int pos = b.spl[k];
+    const uint64_t cur = b.cand_out[pos];
+    const int s = int(cur >> 49);
+    const MapDev& m = maps[s];
+    const int tag = m.ctr->tag, off = si[s].off;
+    const int t = cand_t(cur);
+    const int32_t base = cj == 6 ? cand_vid(cur) : b.cand_nbr[size_t(pos) * 6 + cj];
+    int32_t X = -1;
+    if (base >= 0) {
+      if (j == 6) {
+        X = base;
+      } else {
+        const VoxState& bs = m.vs[base];
+        int32_t cur_id;
+        if (bs.tag == tag && bs.seg >= 0) cur_id = b.seg_nbr[bs.seg * 6 + j];
+        else cur_id = key_neighbour_ok(bs.key, j) ? lookup_vid(m, key_neighbour(bs.key, j)) : -1;
+        X = vid_at(m, cur_id, tag, t);
+      }
+    }
+    int xseg = -1, tn = kNever;
+    if (X >= 0) {
+      const VoxState& xs = m.vs[X];
+      if (xs.tag == tag && xs.seg >= 0 && xs.evict_t == kNever) {
+        xseg = xs.seg;
+        if (xs.merge_pos != kNever) tn = first_call_after(b, xs.seg, xs.merge_pos, t, off);
+      }
+    }
+    b.pre_x[it] = X;
+    b.pre_seg[it] = xseg;
+    b.pre_tn[it] = tn;
+  }
+}
+
 constexpr int kMrThreads = 256;  // threads of one session's merge replay
 constexpr int kWin = 36;         // window entries (36 x 7 records <= 256 threads)
 constexpr int kSPre = 2 * kWin;  // static candidates loaded per window
@@ -317,6 +357,7 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
   __shared__ int spos[kSPre];
   __shared__ uint64_t wkey[kWin];
   __shared__ int wpos[kWin];
+  __shared__ int wspl[kWin];  // index of a static entry in spl (-1: pool entry)
   __shared__ double pl[kWin][7][kPlaneD];
   __shared__ uint64_t grp[kWin][7];
   __shared__ int32_t nbv[kWin][7];
@@ -333,10 +374,12 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
   __shared__ int sh_i[8];
   __shared__ uint64_t mk[2 * kSPre];  // fill: statics and pool entries merged
   __shared__ int mpos[2 * kSPre];
+  __shared__ int mspl[2 * kSPre];
   __shared__ int mval[2 * kSPre];
   __shared__ int mpre[2 * kSPre];
-  __shared__ int32_t rseg[kWin][7];   // record voxels: segment of this scan (-1 none), key
+  __shared__ int32_t rseg[kWin][7];   // record voxels: segment of this scan (-1 none), key, flags
   __shared__ uint64_t rkey[kWin][7];
+  __shared__ uint32_t rflg[kWin][7];
   __shared__ int pass[kWin];
   __shared__ int sh_off[kWin];
   __shared__ int sh_mg;
@@ -366,11 +409,13 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
       const int p = tid + rank_lt(pseg, np, skey[tid]);
       mk[p] = skey[tid];
       mpos[p] = spos[tid];
+      mspl[p] = sp + tid;
     } else if (tid >= 128 && tid - 128 < np) {
       const int j = tid - 128;
       const int p = j + rank_le(skey, ns, pseg[j]);
       mk[p] = pseg[j];
       mpos[p] = -1;
+      mspl[p] = -1;
     }
     __syncthreads();
     // valid: not over the limit, not a duplicate of the previous key; prefix
@@ -401,6 +446,7 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
       if (ok && pre < kWin) {
         wkey[pre] = mk[tid];
         wpos[pre] = mpos[tid];
+        wspl[pre] = mspl[tid];
       }
       if (over && (tid == 0 || mk[tid - 1] <= lim)) atomicMin(&sh_i[0], tid);
       // the kWin-th taken key ends the consumed range
@@ -457,6 +503,7 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
         grp[w][j] = ns_.group_id;
         rseg[w][j] = (ns_.tag == tag && ns_.seg >= 0) ? ns_.seg : -1;
         rkey[w][j] = ns_.key;
+        rflg[w][j] = ns_.flags;
         for (int k = 0; k < kPlaneD; ++k) pl[w][j][k] = m.plane[size_t(nv) * kPlaneD + k];
       }
     }
@@ -487,13 +534,16 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
       nclos[w] = nc;
     }
     __syncthreads();
-    if (tid == 0) {
-      int na = 0;
-      for (int w = 0; w < n; ++w)
-        if (nclos[w] > 0) act[na++] = w;
-      sh_i[3] = na;
-      sh_i[4] = 0;  // new keys
-      sh_i[5] = 0;  // side effects
+    if (tid < 64) {  // acting entries in order (n <= kWin < 64: one wave)
+      const bool a = tid < n && nclos[tid] > 0;
+      const uint64_t bm = __ballot(a);
+      const uint64_t below = tid == 0 ? 0ull : (~0ull >> (64 - tid));
+      if (a) act[__popcll(bm & below)] = tid;
+      if (tid == 0) {
+        sh_i[3] = __popcll(bm);
+        sh_i[4] = 0;  // new keys
+        sh_i[5] = 0;  // side effects
+      }
     }
     __syncthreads();
     const int na = sh_i[3];
@@ -511,8 +561,15 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
         const int t = cand_t(wkey[w]);
         const int cj = clos[w][ci];
         const int32_t base_v = nbv[w][cj];
+        // a static entry's X, its segment and next call come precomputed
+        const int pre = (wspl[w] >= 0 && wspl[w] < b.pre_cap) ? wspl[w] * kPreItems + cj * 7 + j : -1;
+        int pre_seg = -1, pre_tn = kNever;
         X = base_v;
-        if (j < 6) {
+        if (pre >= 0) {
+          X = b.pre_x[pre];
+          pre_seg = b.pre_seg[pre];
+          pre_tn = b.pre_tn[pre];
+        } else if (j < 6) {
           const int bseg = rseg[w][cj];
           const uint64_t bkey = rkey[w][cj];
           int32_t cur_id;
@@ -544,16 +601,18 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
         }
         if (X >= 0 && need) {
           const VoxState& xs = m.vs[X];
-          if (xs.tag == tag && xs.seg >= 0 && xs.evict_t == kNever) {
+          int xseg = pre_seg, tn = pre_tn;
+          if (pre < 0) {
+            xseg = (xs.tag == tag && xs.seg >= 0 && xs.evict_t == kNever) ? xs.seg : -1;
+            tn = (xseg >= 0 && xs.merge_pos != kNever) ? first_call_after(b, xs.seg, xs.merge_pos, t, off) : kNever;
+          }
+          if (xseg >= 0) {
             uint64_t loc[8];
             int nl = 0;
-            if (xs.merge_pos != kNever) {
-              const int tn = first_call_after(b, xs.seg, xs.merge_pos, t, off);
-              if (tn != kNever) loc[nl++] = cand_key(0, tn, X);
-            }
-            const int nct = b.seg_nct[xs.seg];
+            if (tn != kNever) loc[nl++] = cand_key(0, tn, X);
+            const int nct = b.seg_nct[xseg];
             for (int k = 0; k < nct; ++k) {
-              const int tc = b.seg_ct[xs.seg * 7 + k];
+              const int tc = b.seg_ct[xseg * 7 + k];
               if (tc > t) loc[nl++] = cand_key(0, tc, X);
             }
             if (nl > 0) {
@@ -592,13 +651,15 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
     //      an earlier entry's new key must precede, stops the commit.  The
     //      entries before it commit together: their write sets are disjoint
     //      (an entry that writes a voxel also read it).
-    if (tid == 0) {
-      int o = 0;
-      for (int a = 0; a < na; ++a) {
-        sh_off[a] = o;
-        o += nclos[act[a]];
+    if (tid < 64) {  // offsets of the acting entries' writer records (na <= kWin < 64)
+      const int v = tid < na ? nclos[act[tid]] : 0;
+      int incl = v;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(incl, o, 64);
+        if (tid >= o) incl += u;
       }
-      sh_i[7] = o;  // writer records
+      if (tid < na) sh_off[tid] = incl - v;
+      if (tid == 63) sh_i[7] = incl;  // writer records
     }
     if (tid < kWin) {
       sh_minkey[tid] = ~0ull;
@@ -664,9 +725,22 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
       const int32_t X = nbv[e][j];
       for (int k = 0; k < kPlaneD; ++k) m.plane[size_t(X) * kPlaneD + k] = pl[e][j][k];
       m.vs[X].group_id = grp[e][j];
-      const uint32_t fl = m.vs[X].flags | kMerged;
-      m.vs[X].flags = fl;
-      write_hot(m, X, fl);
+      // an atomic bit set: another entry of this commit may set kDirty on X
+      atomicOr(&m.vs[X].flags, kMerged);
+      // the observe record from the LDS copies (merges never change the
+      // plane bit, which is all the observe pass reads of the flags)
+      int64_t kx, ky, kz;
+      unpack_key(rkey[e][j], kx, ky, kz);
+      VoxHot h;
+      h.mx = float(pl[e][j][0] - double(kx) * m.vs64);
+      h.my = float(pl[e][j][1] - double(ky) * m.vs64);
+      h.mz = float(pl[e][j][2] - double(kz) * m.vs64);
+      h.flags = rflg[e][j] | kMerged;
+      h.nx = float(pl[e][j][3]);
+      h.ny = float(pl[e][j][4]);
+      h.nz = float(pl[e][j][5]);
+      h.pad = 0.f;
+      m.hot[X] = h;
     }
     for (int f = tid; f < nfx; f += kMrThreads) {
       const int e = fx_wk[f] >> 1;

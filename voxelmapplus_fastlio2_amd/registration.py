@@ -133,6 +133,13 @@ class Context:
         if rc != 0:
             raise VmpError(f"vmp_ctx_set_param({name}, {value}) failed ({rc})")
 
+    def reserve(self, points, sessions):
+        """Size the batch workspace for batches of up to `points` points over
+        up to `sessions` sessions (no later batch then allocates)."""
+        rc = self.lib.vmp_ctx_reserve(self.h, int(points), int(sessions))
+        if rc != 0:
+            raise VmpError(f"vmp_ctx_reserve({points}, {sessions}) failed ({rc})")
+
     def autotune_observe(self, registrar, reps=10):
         """Time the observe pass on the registrar's current scan for 1, 2, 4
         and 8 points per thread; the fastest becomes this context's choice
