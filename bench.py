@@ -191,17 +191,14 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000, groups=1):
         register_batch_submit(rg)
         return None
 
-    # observe launch shape per scan-size bucket (< 8k, < 32k, < 128k points),
-    # each timed on the session whose current scan is nearest the bucket's middle
+    # observe launch shape per scan-size bucket (< 8k, < 32k, < 128k, more
+    # points), tuned on the batch itself (scan 0 of every session)
+    from voxelmapplus_fastlio2_amd import autotune_observe_batch
+
     tuned = {}
-    for bi, (lo, hi) in enumerate([(0, 8192), (8192, 32768), (32768, 131072)]):
-        cur = sizes[0]
-        inb = [i for i in range(B) if lo <= cur[i] < hi]
-        if not inb:
-            continue
-        i = min(inb, key=lambda j: abs(cur[j] - (lo + min(hi, nmax)) / 2))
-        best, ms = ctxs[i % groups].autotune_observe(regs[i], reps=5)
-        tuned[f"bucket{bi}"] = {"points": int(cur[i]), "points_per_thread": best, "candidates_ms": ms}
+    for gi in range(groups):
+        table, ms = autotune_observe_batch([regs[i] for i in members[gi]], reps=5)
+        tuned[f"group{gi}"] = {"points_per_thread": table, "candidates_ms": ms}
     log(f"batched autotune: {tuned}")
     for k in range(1, W + 1):
         for idx in members:

@@ -188,11 +188,18 @@ int vmp_ctx_reserve(vmp_ctx* ctx, int64_t points, int32_t sessions);
 int vmp_ctx_set_param(vmp_ctx* ctx, const char* name, int64_t value);
 
 /* Autotuner of the observe launch shape (no reference counterpart): times
- * the observe pass on the session's current scan and state for 1, 2, 4 and
- * 8 points per thread (median of reps passes), stores the fastest in the
- * context's table for the scan's size bucket and returns it (cand_ms: the
- * four medians, may be NULL).  The residual slots are restored afterwards. */
+ * the observe kernel on the session's current scan and state for 1, 2, 4
+ * and 8 points per thread (HIP events around the kernel, median of reps
+ * launches), stores the fastest in the context's table for the scan's size
+ * bucket and returns it (cand_ms: the four medians, may be NULL).  The
+ * residual slots are restored afterwards. */
 int vmp_autotune_observe(vmp_session* s, int32_t reps, int32_t* best_ppt, float* cand_ms);
+/* The same for a batch of sessions of one context, tuned as the batch runs:
+ * bucket by bucket (< 8k, < 32k, < 128k, larger points), each candidate is
+ * timed as one observe launch over all the sessions.  table: the resulting
+ * four entries; cand_ms: 4 x 4 medians (bucket-major, 0 for buckets no
+ * session falls in); either may be NULL. */
+int vmp_autotune_observe_batch(vmp_session** s, int32_t n, int32_t reps, int32_t* table, float* cand_ms);
 
 /* Device events around the last call's kernels (for roofline timing). */
 int vmp_session_timing(vmp_session* s, double* observe_ms, double* map_ms, int32_t* observe_launches);

@@ -131,3 +131,35 @@ def test_autotune_observe_leaves_results_unchanged():
         g.close()
     ctx_t.close()
     ctx_r.close()
+
+
+def test_autotune_observe_batch_leaves_results_unchanged():
+    """The batch form tunes every occupied size bucket on one observe launch
+    over all sessions and restores their residual slots: the batch's next
+    registrations equal those of a context given the same table by hand."""
+    from voxelmapplus_fastlio2_amd import autotune_observe_batch
+
+    ctx_t = Context(0)
+    ctx_r = Context(0)
+    tuned = [make(ctx_t, *sp) for sp in SPECS]
+    ref = [make(ctx_r, *sp) for sp in SPECS]
+    table, ms = autotune_observe_batch([g for _, g in tuned], reps=3)
+    print("batch autotune", table, ms, flush=True)
+    assert all(v in (1, 2, 4, 8) for v in table)
+    for k, v in enumerate(table):
+        ctx_r.set_param(f"obs_ppt_bucket{k}", v)
+    for k in range(1, 4):
+        for group in (tuned, ref):
+            mot = [seq.motion(k) for seq, _ in group]
+            apply_motion_batch([g for _, g in group], [a for a, _ in mot], [b for _, b in mot], QDIAG)
+            for seq, g in group:
+                g.set_points(*seq.scan(k))
+            register_batch([g for _, g in group])
+        for i, ((_, ga), (_, gb)) in enumerate(zip(tuned, ref)):
+            sa, Pa = ga.get_state()
+            sb, Pb = gb.get_state()
+            assert np.array_equal(sa, sb) and np.array_equal(Pa, Pb), f"session {i} scan {k}: tuned batch differs"
+    for _, g in tuned + ref:
+        g.close()
+    ctx_t.close()
+    ctx_r.close()

@@ -11,6 +11,7 @@ from .registration import (
     Registrar,
     VmpError,
     apply_motion_batch,
+    autotune_observe_batch,
     initial_covariance,
     register_batch,
     register_batch_submit,
@@ -25,6 +26,7 @@ __all__ = [
     "Registrar",
     "VmpError",
     "apply_motion_batch",
+    "autotune_observe_batch",
     "initial_covariance",
     "register_batch",
     "register_batch_submit",

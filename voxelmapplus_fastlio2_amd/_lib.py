@@ -132,6 +132,8 @@ def vmp():
     _proto(lib, "vmp_ctx_set_param", C.c_int, [P, C.c_char_p, C.c_int64])
     _proto(lib, "vmp_ctx_reserve", C.c_int, [P, C.c_int64, C.c_int32])
     _proto(lib, "vmp_autotune_observe", C.c_int, [P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_float)])
+    _proto(lib, "vmp_autotune_observe_batch", C.c_int,
+           [pp, C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_float)])
     i64p = C.POINTER(C.c_int64)
     _proto(lib, "vmp_session_timing_totals", C.c_int, [P, dp, i64p, dp, i64p, i64p, i64p])
     _vmp = lib

@@ -1151,12 +1151,99 @@ extern "C" int vmp_observe(vmp_session* s, const vmp_state* x, double* H, double
   return VMP_OK;
 }
 
-// Observe-pass autotuner: times the pass on this session's current scan and
-// state for 1, 2, 4 and 8 points per thread (median of reps timed passes
-// after one untimed one) and stores the fastest in the context's table for
-// the scan's size bucket (< 8k, < 32k, < 128k, larger).  The residual slots
-// the passes write are restored, so the session's next registration is the
-// same as without tuning.
+// Observe-pass autotuner.  For each size bucket (< 8k, < 32k, < 128k, larger
+// points) that holds one of the sessions, in turn, every candidate (1, 2, 4,
+// 8 points per thread) is timed on the sessions' current scans and states
+// as one observe launch over all of them (HIP events around the observe
+// kernel only, median of reps after one untimed launch), and the fastest is
+// kept in the context's table before the next bucket is tuned.  The
+// residual slots the passes write are restored, so the sessions' next
+// registrations are the same as without tuning.  c->mu held.
+static int autotune_locked(vmp_ctx* c, vmp_session** ss, int n, int reps, int32_t* table, float* cand_ms) {
+  if (hipSetDevice(c->device) != hipSuccess) return VMP_EDEVICE;
+  settle(c);
+  int64_t ntot = 0;
+  for (int i = 0; i < n; ++i) ntot += ss[i]->n;
+  if (ws_reserve(c, std::max<int64_t>(ntot, 1), n) != VMP_OK) return VMP_ENOMEM;
+  hipStream_t st = c->stream;
+  float4* saved = nullptr;
+  if (hipMalloc(&saved, sizeof(float4) * size_t(std::max<int64_t>(ntot, 1))) != hipSuccess) return VMP_ENOMEM;
+  int rc = VMP_OK;
+  int64_t o = 0;
+  for (int i = 0; i < n && rc == VMP_OK; ++i) {
+    if (hipMemcpyAsync(saved + o, ss[i]->slots, sizeof(float4) * size_t(ss[i]->n), hipMemcpyDeviceToDevice, st) !=
+        hipSuccess)
+      rc = VMP_EDEVICE;
+    o += ss[i]->n;
+  }
+  if (rc == VMP_OK && !c->tev[0])
+    for (hipEvent_t& e : c->tev)
+      if (hipEventCreate(&e) != hipSuccess) rc = VMP_EDEVICE;
+  const bool keep_timing = c->timing;
+  const int keep_ppt = c->obs_ppt;
+  c->timing = true;
+  c->obs_ppt = 0;
+  const int cands[4] = {1, 2, 4, 8};
+  for (int bk = 0; bk < 4 && rc == VMP_OK; ++bk) {
+    bool present = false;
+    for (int i = 0; i < n; ++i) present |= ppt_bucket(ss[i]->n) == bk;
+    if (!present) continue;
+    float med[4] = {0, 0, 0, 0};
+    for (int ci = 0; ci < 4 && rc == VMP_OK; ++ci) {
+      c->ppt_tab[bk] = cands[ci];
+      std::vector<float> ms;
+      for (int r = 0; r <= reps && rc == VMP_OK; ++r) {
+        c->tev_used = 0;
+        if (enqueue_step(c, ss, n, kStepObserveOnce) != VMP_OK || hipStreamSynchronize(st) != hipSuccess) {
+          rc = VMP_EDEVICE;
+          break;
+        }
+        float t = 0.f;
+        if (r > 0 && c->tev_used >= 2 && hipEventElapsedTime(&t, c->tev[0], c->tev[1]) == hipSuccess) ms.push_back(t);
+      }
+      if (!ms.empty()) {
+        std::sort(ms.begin(), ms.end());
+        med[ci] = ms[ms.size() / 2];
+      }
+    }
+    int best = 0;
+    for (int ci = 1; ci < 4; ++ci)
+      if (med[ci] > 0.f && (med[best] <= 0.f || med[ci] < med[best])) best = ci;
+    c->ppt_tab[bk] = cands[best];
+    if (cand_ms)
+      for (int ci = 0; ci < 4; ++ci) cand_ms[bk * 4 + ci] = med[ci];
+  }
+  c->tev_used = 0;
+  c->timing = keep_timing;
+  c->obs_ppt = keep_ppt;
+  o = 0;
+  for (int i = 0; i < n; ++i) {
+    if (hipMemcpyAsync(ss[i]->slots, saved + o, sizeof(float4) * size_t(ss[i]->n), hipMemcpyDeviceToDevice, st) !=
+        hipSuccess)
+      rc = VMP_EDEVICE;
+    o += ss[i]->n;
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) rc = VMP_EDEVICE;
+  hipFree(saved);
+  if (table)
+    for (int k = 0; k < 4; ++k) table[k] = c->ppt_tab[k];
+  return rc;
+}
+
+extern "C" int vmp_autotune_observe_batch(vmp_session** ss, int32_t n, int32_t reps, int32_t* table, float* cand_ms) {
+  if (!ss || n < 1 || reps < 1 || reps > 1000 || !ss[0]) return VMP_EINVAL;
+  vmp_ctx* c = ss[0]->ctx;
+  for (int i = 0; i < n; ++i) {
+    if (!ss[i] || ss[i]->ctx != c) return VMP_EINVAL;
+    if (!ss[i]->has_points || ss[i]->n < 1) {
+      ss[i]->err = "vmp_autotune_observe before vmp_scan_set_points";
+      return VMP_ESTATE;
+    }
+  }
+  std::lock_guard<std::mutex> lk(c->mu);
+  return autotune_locked(c, ss, n, reps, table, cand_ms);
+}
+
 extern "C" int vmp_autotune_observe(vmp_session* s, int32_t reps, int32_t* best_ppt, float* cand_ms) {
   if (!s || reps < 1 || reps > 1000) return VMP_EINVAL;
   if (!s->has_points || s->n < 1) {
@@ -1165,53 +1252,15 @@ extern "C" int vmp_autotune_observe(vmp_session* s, int32_t reps, int32_t* best_
   }
   vmp_ctx* c = s->ctx;
   std::lock_guard<std::mutex> lk(c->mu);
-  VMP_HIP(hipSetDevice(c->device));
-  settle(c);
-  if (ws_reserve(c, std::max(s->n, 1), 1) != VMP_OK) return VMP_ENOMEM;
-  hipStream_t st = c->stream;
-  float4* saved = nullptr;
-  VMP_HIP(hipMalloc(&saved, sizeof(float4) * size_t(s->n)));
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
-  int rc = hipMemcpyAsync(saved, s->slots, sizeof(float4) * size_t(s->n), hipMemcpyDeviceToDevice, st) == hipSuccess
-               ? VMP_OK
-               : VMP_EDEVICE;
-  const int cands[4] = {1, 2, 4, 8};
-  const int keep = c->obs_ppt;
-  float med[4] = {0, 0, 0, 0};
   vmp_session* one[1] = {s};
-  for (int ci = 0; ci < 4 && rc == VMP_OK; ++ci) {
-    c->obs_ppt = cands[ci];
-    std::vector<float> ms;
-    for (int r = 0; r <= reps && rc == VMP_OK; ++r) {
-      hipEventRecord(e0, st);
-      if (enqueue_step(c, one, 1, kStepObserveOnce) != VMP_OK) rc = VMP_EDEVICE;
-      hipEventRecord(e1, st);
-      if (hipEventSynchronize(e1) != hipSuccess) rc = VMP_EDEVICE;
-      float t = 0.f;
-      if (rc == VMP_OK && r > 0 && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms.push_back(t);
-    }
-    if (!ms.empty()) {
-      std::sort(ms.begin(), ms.end());
-      med[ci] = ms[ms.size() / 2];
-    }
-  }
-  c->obs_ppt = keep;
-  if (hipMemcpyAsync(s->slots, saved, sizeof(float4) * size_t(s->n), hipMemcpyDeviceToDevice, st) != hipSuccess ||
-      hipStreamSynchronize(st) != hipSuccess)
-    rc = VMP_EDEVICE;
-  hipFree(saved);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  int32_t table[4];
+  float all[16];
+  const int rc = autotune_locked(c, one, 1, reps, table, all);
   if (rc != VMP_OK) return rc;
-  int best = 0;
-  for (int ci = 1; ci < 4; ++ci)
-    if (med[ci] < med[best]) best = ci;
-  c->ppt_tab[ppt_bucket(s->n)] = cands[best];
-  if (best_ppt) *best_ppt = cands[best];
+  const int bk = ppt_bucket(s->n);
+  if (best_ppt) *best_ppt = table[bk];
   if (cand_ms)
-    for (int ci = 0; ci < 4; ++ci) cand_ms[ci] = med[ci];
+    for (int ci = 0; ci < 4; ++ci) cand_ms[ci] = all[bk * 4 + ci];
   return VMP_OK;
 }
 

@@ -157,7 +157,49 @@ __device__ inline int sess_of(const SessInfo* si, int nsess, int g) {
 }
 
 // ------------------------------------------------------------ helpers -----
-__device__ inline int32_t hash_find_or_insert(HashEntry* table, uint32_t mask, uint64_t key, MapCounters* ctr) {
+// Grid-stride loop whose trip count is the same for every lane of a block,
+// so wave-level operations inside see the whole wave; `valid` marks the
+// lanes with an item.
+#define VMP_UNIFORM_LOOP(i, valid, n)                                                            \
+  for (int i##_0 = blockIdx.x * blockDim.x; i##_0 < (n); i##_0 += gridDim.x * blockDim.x)         \
+    for (int i = i##_0 + int(threadIdx.x), valid = (i < (n)), i##_once = 1; i##_once; i##_once = 0)
+
+// Fetch-add on counters, aggregated over the wave: the lanes that add to the
+// same counter do so with one atomic, and each lane gets the counter's value
+// before its own contribution (lanes in lane order).  Every lane of the wave
+// calls it; a lane with v = 0 contributes nothing.
+__device__ inline int wave_fetch_add(int* p, int v) {
+  const int lane = threadIdx.x & 63;
+  uint64_t pending = __ballot(v != 0);
+  const uint64_t pv = reinterpret_cast<uint64_t>(p);
+  int out = 0;
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t lo = uint32_t(__shfl(int(uint32_t(pv)), leader, 64));
+    const uint32_t hi = uint32_t(__shfl(int(uint32_t(pv >> 32)), leader, 64));
+    const uint64_t pl = uint64_t(lo) | (uint64_t(hi) << 32);
+    const bool in = ((pending >> lane) & 1ull) && pv == pl;
+    const uint64_t grp = __ballot(in);
+    const int x = in ? v : 0;
+    int incl = x;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
+    }
+    const int total = __shfl(incl, 63, 64);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(reinterpret_cast<int*>(pl), total);
+    base = __shfl(base, leader, 64);
+    if (in) out = base + incl - x;
+    pending &= ~grp;
+  }
+  return out;
+}
+
+// *inserted = 1 when this call claimed the empty slot (the caller counts
+// used_entries); with inserted == nullptr the count is added here
+__device__ inline int32_t hash_find_or_insert(HashEntry* table, uint32_t mask, uint64_t key, MapCounters* ctr,
+                                              int* inserted = nullptr) {
   uint32_t h = hash_slot(key, mask);
   for (uint32_t probe = 0; probe <= mask; ++probe) {
     unsigned long long* kp = reinterpret_cast<unsigned long long*>(&table[h].key);
@@ -166,7 +208,8 @@ __device__ inline int32_t hash_find_or_insert(HashEntry* table, uint32_t mask, u
     if (k == kKeyEmpty) {
       unsigned long long old = atomicCAS(kp, (unsigned long long)kKeyEmpty, (unsigned long long)key);
       if (old == kKeyEmpty) {
-        atomicAdd(&ctr->used_entries, 1);
+        if (inserted) *inserted = 1;
+        else atomicAdd(&ctr->used_entries, 1);
         return int32_t(h);
       }
       if (old == key) return int32_t(h);
@@ -313,8 +356,10 @@ __global__ void k_ranges(const MapDev* maps, SessInfo* si, int nsess, const int3
 
 // ------------------------------------------------------------------ K1 ----
 __global__ void k_ins_keys(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
-  VMP_GRID_LOOP(g, b.ntot) {
-    const int s = sess_of(si, nsess, g);
+  VMP_UNIFORM_LOOP(g, valid, b.ntot) {
+    int ins = 0;
+    const int s = valid ? sess_of(si, nsess, g) : 0;
+    if (valid) {
     const SessInfo& S = si[s];
     const MapDev& m = maps[s];
     const int i = g - S.off;
@@ -328,10 +373,12 @@ __global__ void k_ins_keys(const MapDev* maps, const SessInfo* si, int nsess, Ba
       ok &= key_ok(k[r]);
     }
     int32_t e = -1;
-    if (ok) e = hash_find_or_insert(m.table, m.hmask, pack_key(k[0], k[1], k[2]), m.ctr);
+    if (ok) e = hash_find_or_insert(m.table, m.hmask, pack_key(k[0], k[1], k[2]), m.ctr, &ins);
     else atomicOr(&m.ctr->error, 2);
     b.skey_in[g] = (uint64_t(s) << b.hbits) | uint64_t(uint32_t(e < 0 ? 0 : e));
     b.sval_in[g] = g;
+    }
+    wave_fetch_add(&maps[s].ctr->used_entries, ins);
   }
 }
 
@@ -891,9 +938,9 @@ __global__ void __launch_bounds__(kMapThreads) k_replay(const MapDev* maps, cons
     m.vs[vid] = st;
     write_hot(m, vid, flags);
   }
-  if (refits) atomicAdd(&m.ctr->refits, refits);
-  if (calls) atomicAdd(&m.ctr->merge_calls, calls);
-  if (dropped) atomicAdd(&m.ctr->dropped, dropped);
+  wave_fetch_add(&m.ctr->refits, refits);
+  wave_fetch_add(&m.ctr->merge_calls, calls);
+  wave_fetch_add(&m.ctr->dropped, dropped);
 }
 
 }  // namespace vmp

@@ -135,58 +135,66 @@ __device__ inline void merge_fuse(double* pv, double* pn) {
 // ------------------------------------------------------------------ K7 ----
 __global__ void k_candidates(const MapDev* maps, const SessInfo* si, BatchBufs b) {
   const int nseg = b.cnt[0];
-  VMP_GRID_LOOP(q, nseg) {
-    const int32_t vid = b.seg_vid[q];
-    b.seg_nct[q] = 0;
-    if (vid < 0) {
-      for (int j = 0; j < 6; ++j) b.seg_nbr[q * 6 + j] = -1;
-      continue;
-    }
-    const int s = b.seg_sess[q];
+  VMP_UNIFORM_LOOP(q, valid, nseg) {
+    const int32_t vid = valid ? b.seg_vid[q] : -1;
+    const int s = valid ? b.seg_sess[q] : 0;
     const MapDev& m = maps[s];
     MapCounters* c = m.ctr;
-    const int tag = c->tag, off = si[s].off, ns_pts = si[s].n;
-    VoxState& st = m.vs[vid];
-    int dirty = (st.flags & kDirty) ? -1 : -2;
-    const bool own_freeze = st.freeze_t >= 0 && (st.flags & kPlane);
-    if (own_freeze && st.freeze_t > dirty) dirty = st.freeze_t;
-    const bool has_calls = st.merge_pos != kNever;
     int emitted[7];
     int ne = 0;
-    auto emit = [&](int after) {
-      const int t = first_call_after(b, q, st.merge_pos, after, off);
-      if (t == kNever) return;
-      for (int k = 0; k < ne; ++k)
-        if (emitted[k] == t) return;  // one candidate per distinct call
-      emitted[ne++] = t;
-      const int slot = atomicAdd(&c->ncand, 1);
-      if (slot < ns_pts) b.cand_in[off + slot] = cand_key(s, t, vid);
-      else atomicOr(&c->error, 16);
-    };
-    if (has_calls && ((st.flags & kDirty) || own_freeze)) emit(-1);
-    for (int j = 0; j < 6; ++j) {
-      int32_t nv = -1;
-      if (key_neighbour_ok(st.key, j)) nv = lookup_vid(m, key_neighbour(st.key, j));
-      b.seg_nbr[q * 6 + j] = nv;
-      if (nv < 0) continue;
-      const VoxState& nbs = m.vs[nv];
-      if (nbs.tag != tag || nbs.freeze_t < 0 || !(nbs.flags & kPlane)) continue;
-      if (nbs.freeze_t > dirty) dirty = nbs.freeze_t;
-      if (has_calls) emit(nbs.freeze_t);
+    if (valid) {
+      b.seg_nct[q] = 0;
+      if (vid < 0)
+        for (int j = 0; j < 6; ++j) b.seg_nbr[q * 6 + j] = -1;
     }
-    st.dirty_t = dirty;
-    // candidate times, ascending (insertion sort of <= 7)
-    for (int i = 1; i < ne; ++i) {
-      const int v = emitted[i];
-      int k = i - 1;
-      while (k >= 0 && emitted[k] > v) {
-        emitted[k + 1] = emitted[k];
-        --k;
+    if (vid >= 0) {
+      const int tag = c->tag, off = si[s].off;
+      VoxState& st = m.vs[vid];
+      int dirty = (st.flags & kDirty) ? -1 : -2;
+      const bool own_freeze = st.freeze_t >= 0 && (st.flags & kPlane);
+      if (own_freeze && st.freeze_t > dirty) dirty = st.freeze_t;
+      const bool has_calls = st.merge_pos != kNever;
+      auto emit = [&](int after) {
+        const int t = first_call_after(b, q, st.merge_pos, after, off);
+        if (t == kNever) return;
+        for (int k = 0; k < ne; ++k)
+          if (emitted[k] == t) return;  // one candidate per distinct call
+        emitted[ne++] = t;
+      };
+      if (has_calls && ((st.flags & kDirty) || own_freeze)) emit(-1);
+      for (int j = 0; j < 6; ++j) {
+        int32_t nv = -1;
+        if (key_neighbour_ok(st.key, j)) nv = lookup_vid(m, key_neighbour(st.key, j));
+        b.seg_nbr[q * 6 + j] = nv;
+        if (nv < 0) continue;
+        const VoxState& nbs = m.vs[nv];
+        if (nbs.tag != tag || nbs.freeze_t < 0 || !(nbs.flags & kPlane)) continue;
+        if (nbs.freeze_t > dirty) dirty = nbs.freeze_t;
+        if (has_calls) emit(nbs.freeze_t);
       }
-      emitted[k + 1] = v;
+      st.dirty_t = dirty;
     }
-    for (int k = 0; k < ne; ++k) b.seg_ct[q * 7 + k] = emitted[k];
-    b.seg_nct[q] = ne;
+    // slots in the session's candidate list, one atomic per wave and session
+    const int slot = wave_fetch_add(&c->ncand, ne);
+    if (ne > 0) {
+      const int off = si[s].off, ns_pts = si[s].n;
+      for (int k = 0; k < ne; ++k) {
+        if (slot + k < ns_pts) b.cand_in[off + slot + k] = cand_key(s, emitted[k], vid);
+        else atomicOr(&c->error, 16);
+      }
+      // candidate times, ascending (insertion sort of <= 7)
+      for (int i = 1; i < ne; ++i) {
+        const int v = emitted[i];
+        int k = i - 1;
+        while (k >= 0 && emitted[k] > v) {
+          emitted[k + 1] = emitted[k];
+          --k;
+        }
+        emitted[k + 1] = v;
+      }
+      for (int k = 0; k < ne; ++k) b.seg_ct[q * 7 + k] = emitted[k];
+      b.seg_nct[q] = ne;
+    }
   }
 }
 

@@ -372,6 +372,22 @@ def register_batch_submit(registrars):
         raise VmpError(f"vmp_register_batch_submit failed ({rc})")
 
 
+def autotune_observe_batch(registrars, reps=5):
+    """Tune the observe launch shape of every size bucket on the batch as it
+    runs (one observe launch over all the registrars per timing, all of one
+    context).  Returns (table of points per thread for the four buckets,
+    4 x 4 medians in ms)."""
+    lib = registrars[0].lib
+    n = len(registrars)
+    arr = (C.c_void_p * n)(*[r.h.value for r in registrars])
+    table = (C.c_int32 * 4)()
+    ms = (C.c_float * 16)()
+    rc = lib.vmp_autotune_observe_batch(arr, n, int(reps), table, ms)
+    if rc != 0:
+        raise VmpError(f"vmp_autotune_observe_batch failed ({rc})")
+    return list(table), [list(ms[4 * k : 4 * k + 4]) for k in range(4)]
+
+
 def register_batch_wait(ctx, n):
     """Complete the batch in flight; statistics of the last completed batch."""
     o = (_lib.ObsStats * n)()
