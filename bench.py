@@ -375,7 +375,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--batch-warmup", type=int, default=3)
     ap.add_argument("--batch-capacity", type=int, default=200000)
-    ap.add_argument("--batch-groups", type=int, default=1, help="session groups on separate streams")
+    ap.add_argument("--batch-groups", type=int, default=2,
+                    help="session groups, each on its own context (HIP stream) and host thread")
     ap.add_argument("--obs-ppt", type=int, default=0, help="observe points per thread (0 = by scan size)")
     ap.add_argument("--serve-sessions", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
