@@ -391,6 +391,7 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
   __shared__ int pass[kWin];
   __shared__ int sh_off[kWin];
   __shared__ int sh_mg;
+  __shared__ int sh_rew;  // statics of this window that are read again next window
   __shared__ uint64_t sh_minkey[kWin];
   __shared__ int sh_conf[kWin];
   __shared__ uint64_t wr_in[kWin * 7];  // (voxel << 8) | entry of every write
@@ -482,7 +483,8 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
     }
     const int n = sh_i[0];
     const int consumed = sh_i[1] + (sh_i[2] - ph);
-    sp += sh_i[1];
+    const int used = sh_i[1];
+    sp += used;
     ph = sh_i[2];
     if (n == 0) {
       if (consumed == 0) break;  // nothing left
@@ -720,10 +722,15 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
       if (tid == 0) {
         sh_i[7] = stop;
         sh_mg = mg;
+        // the statics from the stop on are read again from the static list
+        // (with their precomputed taint data) rather than sent to the pool;
+        // consumed statics ascend, so they are those with key >= the stop key
+        sh_rew = stop < n ? used - rank_lt(skey, used, wkey[stop]) : 0;
       }
     }
     __syncthreads();
     const int w = sh_i[7];
+    sp -= sh_rew;
     merges += sh_mg;
     evals += w;
     for (int q = tid; q < w * 8; q += kMrThreads) {
@@ -775,7 +782,7 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
         keep = nkown[k] < w;
       } else {
         v = wkey[k - nnew];
-        keep = k - nnew >= w;
+        keep = k - nnew >= w && wspl[k - nnew] < 0;  // a static sent back is read again
       }
       if (keep) gbuf[atomicAdd(&sh_i[6], 1)] = v;
     }
