@@ -822,9 +822,10 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
       hipLaunchKernelGGL(k_create_fast_ctr, dim3(n), dim3(1), 0, st, M, n);
       hipLaunchKernelGGL(k_create_evict, dim3(n), dim3(64), 0, st, M, S, n, b);
       hipLaunchKernelGGL(k_create_init, dim3(gn), dim3(T), 0, st, M, S, n, b);
-      // segment lengths fit in lbits bits (a segment holds at most ntot points)
+      // replay order by segment length, capped at 10 bits (two radix passes;
+      // segments of 1022 points and more share the first key)
       int lbits = 1;
-      while ((int64_t(1) << lbits) <= int64_t(ntot)) ++lbits;
+      while ((int64_t(1) << lbits) <= int64_t(ntot) && lbits < 10) ++lbits;
       hipLaunchKernelGGL(k_seg_lenkeys, dim3(gn), dim3(T), 0, st, b, lbits);
       tb = w.temp_bytes;
       if (rocprim::radix_sort_pairs(w.temp, tb, b.len_in, b.len_out, b.ord_in, b.ord, size_t(ntot), 0, lbits, st) !=

@@ -679,10 +679,8 @@ __global__ void k_create_init(const MapDev* maps, const SessInfo* si, int nsess,
     st.dirty_t = -2;
     st.taint_t = kNever;
     m.vs[vid] = st;
-    double* pl = m.plane + size_t(vid) * kPlaneD;
-    for (int k = 0; k < kPlaneD; ++k) pl[k] = 0.0;
-    double* mo = m.mom + size_t(vid) * kMomD;
-    for (int k = 0; k < kMomD; ++k) mo[k] = 0.0;
+    // the plane record and the moments are not cleared here: the voxel has
+    // points in this scan, and the replay starts a voxel with n = 0 from zero
     VoxHot h{};
     h.flags = kEnable | kAlive;
     m.hot[vid] = h;
@@ -705,6 +703,15 @@ __device__ inline void mom_load(const double* src, Moments& M) {
     for (int l = 0; l < 6; ++l) M.c1[a][l] = src[o++];
   for (int a = 0; a < 6; ++a)
     for (int l = 0; l < 6; ++l) M.c2[a][l] = src[o++];
+}
+__device__ inline void mom_zero(Moments& M) {
+  for (int a = 0; a < 3; ++a) M.s1[a] = 0.0;
+  for (int a = 0; a < 6; ++a) M.s2[a] = 0.0;
+  for (int a = 0; a < 6; ++a) M.c0[a] = 0.0;
+  for (int a = 0; a < 3; ++a)
+    for (int l = 0; l < 6; ++l) M.c1[a][l] = 0.0;
+  for (int a = 0; a < 6; ++a)
+    for (int l = 0; l < 6; ++l) M.c2[a][l] = 0.0;
 }
 __device__ inline void mom_store(double* dst, const Moments& M) {
   int o = 0;
@@ -807,12 +814,12 @@ __device__ inline void refit(const MapDev& m, Moments& M, int n, uint32_t& flags
 
 // sort keys that put long segments first (waves of the replay then walk
 // segments of similar length); slots past the segment count sort last.
-// Lengths are below 2^lbits, so the keys use the low lbits bits only.
+// Lengths are clamped below 2^lbits - 1, so the keys use the low lbits bits.
 __global__ void k_seg_lenkeys(BatchBufs b, int lbits) {
   const int nseg = b.cnt[0];
   const uint32_t top = uint32_t((uint64_t(1) << lbits) - 1);
   VMP_GRID_LOOP(q, b.ntot) {
-    b.len_in[q] = q < nseg ? top - uint32_t(b.seg_end[q] - b.seg_start[q]) : top;
+    b.len_in[q] = q < nseg ? top - min(uint32_t(b.seg_end[q] - b.seg_start[q]), top - 1) : top;
     b.ord_in[q] = q;
   }
 }
@@ -843,9 +850,14 @@ __global__ void __launch_bounds__(kMapThreads) k_replay(const MapDev* maps, cons
     VoxState st = m.vs[vid];
     double* pl = m.plane + size_t(vid) * kPlaneD;
     double plv[kPlaneD];
-    for (int k = 0; k < kPlaneD; ++k) plv[k] = pl[k];
     Moments M;
-    mom_load(m.mom + size_t(vid) * kMomD, M);
+    if (st.n > 0) {
+      for (int k = 0; k < kPlaneD; ++k) plv[k] = pl[k];
+      mom_load(m.mom + size_t(vid) * kMomD, M);
+    } else {  // a new voxel (its records were not cleared at creation)
+      for (int k = 0; k < kPlaneD; ++k) plv[k] = 0.0;
+      mom_zero(M);
+    }
     int64_t kx, ky, kz;
     unpack_key(st.key, kx, ky, kz);
     const double center[3] = {(double(kx) + 0.5) * m.vs64, (double(ky) + 0.5) * m.vs64, (double(kz) + 0.5) * m.vs64};
