@@ -316,6 +316,9 @@ constexpr int kPoolCap = 2048;   // pending re-evaluations, kept sorted
 constexpr int kNewCap = 2048;    // keys created by one window's taint phase
 constexpr int kFxCap = 2048;     // deferred taint side effects of one window
 constexpr int kTaintPer = 49;    // up to 7 changed voxels x their closed neighbourhood
+constexpr int kFwCap = 512;      // first-writer hash: >= 2 x the kWin x 7 voxels one window writes
+
+__device__ inline uint32_t fw_hash(int32_t v) { return (uint32_t(v) * 2654435761u) >> (32 - 9); }
 
 __device__ inline int rank_lt(const uint64_t* a, int n, uint64_t v) {  // elements < v
   int lo = 0, hi = n;
@@ -389,13 +392,12 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
   __shared__ uint64_t rkey[kWin][7];
   __shared__ uint32_t rflg[kWin][7];
   __shared__ int pass[kWin];
-  __shared__ int sh_off[kWin];
   __shared__ int sh_mg;
   __shared__ int sh_rew;  // statics of this window that are read again next window
   __shared__ uint64_t sh_minkey[kWin];
   __shared__ int sh_conf[kWin];
-  __shared__ uint64_t wr_in[kWin * 7];  // (voxel << 8) | entry of every write
-  __shared__ uint64_t wr[kWin * 7];
+  __shared__ int32_t fw_key[kFwCap];  // voxels written in this window -> first writing entry
+  __shared__ int32_t fw_min[kFwCap];
   int sp = si[s].spl_lo, pb = 0, ph = 0, pn = 0;
   int evals = 0, merges = 0, windows = 0, requeued = 0, err = 0;
   uint64_t last = ~0ull;
@@ -562,6 +564,14 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
     //      and its later candidates if X changed itself, or if X (a plane not
     //      in the group) passes the gates against a new record.  A voxel of
     //      the scan records its dirty/taint times, one outside is marked dirty.
+    for (int i = tid; i < kFwCap; i += kMrThreads) {  // cleared for the commit below
+      fw_key[i] = -1;
+      fw_min[i] = kWin;
+    }
+    if (tid < kWin) {
+      sh_minkey[tid] = ~0ull;
+      sh_conf[tid] = 0;
+    }
     for (int T = tid; T < na * kTaintPer; T += kMrThreads) {
       const int a = T / kTaintPer, r = T % kTaintPer, ci = r / 7, j = r % 7;
       const int w = act[a];
@@ -661,36 +671,24 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
     //      an earlier entry's new key must precede, stops the commit.  The
     //      entries before it commit together: their write sets are disjoint
     //      (an entry that writes a voxel also read it).
-    if (tid < 64) {  // offsets of the acting entries' writer records (na <= kWin < 64)
-      const int v = tid < na ? nclos[act[tid]] : 0;
-      int incl = v;
-      for (int o = 1; o < 64; o <<= 1) {
-        const int u = __shfl_up(incl, o, 64);
-        if (tid >= o) incl += u;
-      }
-      if (tid < na) sh_off[tid] = incl - v;
-      if (tid == 63) sh_i[7] = incl;  // writer records
-    }
-    if (tid < kWin) {
-      sh_minkey[tid] = ~0ull;
-      sh_conf[tid] = 0;
-    }
-    __syncthreads();
-    const int nwr = sh_i[7];
-    if (tid < na * 8) {
+    if (tid < na * 8) {  // first writer of every voxel the acting entries change
       const int a = tid >> 3, ci = tid & 7;
       const int k = act[a];
-      if (ci < nclos[k]) wr_in[sh_off[a] + ci] = (uint64_t(uint32_t(nbv[k][clos[k][ci]])) << 8) | uint64_t(k);
+      if (ci < nclos[k]) {
+        const int32_t X = nbv[k][clos[k][ci]];
+        uint32_t h = fw_hash(X);
+        while (true) {  // at most half full: the probe ends
+          const int32_t old = atomicCAS(&fw_key[h], -1, X);
+          if (old == -1 || old == X) {
+            atomicMin(&fw_min[h], k);
+            break;
+          }
+          h = (h + 1) & (kFwCap - 1);
+        }
+      }
     }
     for (int q = tid; q < nnew; q += kMrThreads)
       atomicMin(reinterpret_cast<unsigned long long*>(&sh_minkey[nkown[q]]), (unsigned long long)nkey[q]);
-    __syncthreads();
-    for (int q = tid; q < nwr; q += kMrThreads) {  // sort (voxel, entry) pairs
-      const uint64_t v = wr_in[q];
-      int rk = 0;
-      for (int u = 0; u < nwr; ++u) rk += wr_in[u] < v;
-      wr[rk] = v;
-    }
     __syncthreads();
     // reads: (entry, voxel) for the seven records of every entry and the
     // taint reads of acting ones; conflict if the voxel's first writer of
@@ -701,9 +699,16 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
       if (r < 7) v = nbv[w][r];
       else v = nclos[w] > 0 ? r2[w][r - 7] : -1;
       if (v < 0) continue;
-      const uint64_t lo = uint64_t(uint32_t(v)) << 8;
-      const int at = rank_lt(wr, nwr, lo);
-      if (at < nwr && (wr[at] >> 8) == uint64_t(uint32_t(v)) && int(wr[at] & 0xFF) < w) sh_conf[w] = 1;
+      uint32_t h = fw_hash(v);
+      while (true) {
+        const int32_t kk = fw_key[h];
+        if (kk == -1) break;
+        if (kk == v) {
+          if (fw_min[h] < w) sh_conf[w] = 1;
+          break;
+        }
+        h = (h + 1) & (kFwCap - 1);
+      }
     }
     __syncthreads();
     if (tid < 64) {  // one wave (n <= kWin < 64): exclusive prefix minimum of the new keys
