@@ -379,6 +379,7 @@ extern "C" int64_t vmp_session_bytes(const vmp_map_cfg* cfg) {
   int64_t b = 0;
   b += z.hcap * int64_t(sizeof(HashEntry));
   b += z.pool * int64_t(sizeof(VoxHot) + sizeof(VoxState) + 8 * kPlaneD + 8 * kMomD + 4 + 4);
+  b += z.pool * int64_t(std::max(cfg->update_size_thresh - 1, 1)) * kPendF * 4;
   b += z.qcap * 12;
   b += z.maxn * (3 * 4 + 16);
   b += z.max_blocks * kObsRowMax * 8;
@@ -434,6 +435,8 @@ extern "C" int vmp_session_create(vmp_ctx* ctx, const vmp_map_cfg* cfg, vmp_sess
   ok &= dalloc(A, &m.vs, size_t(z.pool)) == hipSuccess;
   ok &= dalloc(A, &m.plane, size_t(z.pool) * kPlaneD) == hipSuccess;
   ok &= dalloc(A, &m.mom, size_t(z.pool) * kMomD) == hipSuccess;
+  m.pend_cap = std::max(cfg->update_size_thresh - 1, 0);
+  ok &= dalloc(A, &m.pend, size_t(z.pool) * size_t(std::max(m.pend_cap, 1)) * kPendF) == hipSuccess;
   ok &= dalloc(A, &m.freelist, size_t(z.pool)) == hipSuccess;
   ok &= dalloc(A, &m.ring_vid, size_t(z.qcap)) == hipSuccess;
   ok &= dalloc(A, &m.ring_stamp, size_t(z.qcap)) == hipSuccess;
@@ -826,12 +829,15 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
       // segments of 1022 points and more share the first key)
       int lbits = 1;
       while ((int64_t(1) << lbits) <= int64_t(ntot) && lbits < 10) ++lbits;
-      hipLaunchKernelGGL(k_seg_lenkeys, dim3(gn), dim3(T), 0, st, b, lbits);
+      hipLaunchKernelGGL(k_seg_lenkeys, dim3(gn), dim3(T), 0, st, M, S, b, lbits);
       tb = w.temp_bytes;
-      if (rocprim::radix_sort_pairs(w.temp, tb, b.len_in, b.len_out, b.ord_in, b.ord, size_t(ntot), 0, lbits, st) !=
+      if (rocprim::radix_sort_pairs(w.temp, tb, b.len_in, b.len_out, b.ord_in, b.ord, size_t(ntot), 0, lbits + 1, st) !=
           hipSuccess)
         return VMP_EDEVICE;
-      hipLaunchKernelGGL(k_replay, dim3(gn), dim3(T), 0, st, M, S, b);
+      // the moment path runs one wave per SIMD: a grid of four waves per CU
+      // strides over the refitting voxels; the light path covers the rest
+      hipLaunchKernelGGL(k_replay_heavy, dim3(std::min(gn, 256)), dim3(T), 0, st, M, S, b);
+      hipLaunchKernelGGL(k_replay_light, dim3(gn), dim3(T), 0, st, M, S, b);
       if (!(mode & kStepBuild)) {
         hipLaunchKernelGGL(k_fill_u64, dim3(gn), dim3(T), 0, st, b.cand_in, ~0ull, ntot);
         hipLaunchKernelGGL(k_candidates, dim3(gn), dim3(T), 0, st, M, S, b);
@@ -1058,7 +1064,7 @@ static int step_finish(vmp_ctx* c, vmp_obs_stats* ost, vmp_map_stats* mst) {
         char buf[200];
         std::snprintf(buf, sizeof(buf),
                       "map update error 0x%x (1 hash probe, 2 key range, 4 LRU exhausted in one scan, 8 voxel "
-                      "pool, 16 replay queue)",
+                      "pool, 16 replay queue, 32 replay class)",
                       unsigned(m.error));
         si->err = buf;
         if (first_err == VMP_OK) first_err = (m.error & 2) ? VMP_ERANGE : ((m.error & 4) ? VMP_ECAPACITY : VMP_EDEVICE);

@@ -106,9 +106,16 @@ struct __align__(16) VoxState {
 // mean xyz, normal xyz, trace of the normal block and of the centre block of
 // Plane::cov (only these traces are ever consumed, voxel_map.cpp:161-164).
 constexpr int kPlaneD = 8;
-// Refit moments, centre-local: S1q(3) S2q(6) S0C(6) S1C(18) S2C(36), padded
-// to 70 doubles so every record is 16-byte aligned.
-constexpr int kMomD = 70;
+// Refit moments, centre-local: S1q(3) S2q(6) S0C(6) S1C(18) S2C(36) over
+// every point except the pending ones (below); then the running S1q over all
+// points (the plane mean), the number of pending points, one pad.  Points
+// added by a scan with no refit wait in the voxel's pending list (kPendF
+// floats each: the point relative to the voxel centre, its world cov) until
+// the next refit folds them in.
+constexpr int kMomD = 74;
+constexpr int kMomRun = 69;
+constexpr int kMomPend = 72;
+constexpr int kPendF = 9;
 
 // ------------------------------------------------------------- params -----
 // Per-pass constants derived from the filter state in FP64.

@@ -16,8 +16,10 @@
 //   K5 create        creation events in point order: parallel when the LRU
 //                    cannot overflow, else one wave per session replays
 //                    creations and evictions against the LRU queue
-//   K6 replay        one lane per voxel walks its points: the pushPoint state
-//                    machine, refits from centre-local moments (no point pool)
+//   K6 replay        one lane per voxel walks its points through the pushPoint
+//                    state machine; the voxels whose walk reaches a refit
+//                    (known from counts) take the FP64 moment path, the others
+//                    a light one that queues their points (no point pool)
 //   K7/K8 merge      see vmp_merge.hpp
 //   K9 finalize      dirty bits, LRU queue append, hash tombstones, free list
 // Every ordering decision (LRU order, victims, group ids, merge order) is
@@ -41,7 +43,8 @@ struct MapCounters {
   int32_t used_entries;    // non-empty hash slots, tombstones included
   int32_t tombs;
   int32_t tag;             // scan tag
-  int32_t error;           // bit 1 hash probe overflow, 2 range, 4 capacity, 8 pool, 16 replay queue
+  int32_t error;           // bit 1 hash probe overflow, 2 range, 4 capacity, 8 pool, 16 replay queue,
+                           // 32 replay class (cannot happen)
   // per-scan outputs
   int32_t nseg, ncreate_ev, ncre, nevict, ncand;
   int32_t refits, merge_calls, merge_evals, merges, dropped;
@@ -60,6 +63,8 @@ struct MapDev {
   VoxState* vs;
   double* plane;      // [pool][kPlaneD]
   double* mom;        // [pool][kMomD]
+  float* pend;        // [pool][pend_cap][kPendF] points added since the moments' last update
+  int32_t pend_cap;   // update_size_thresh - 1
   int32_t* freelist;  // [pool]
   int32_t* ring_vid;  // [qcap]
   uint64_t* ring_stamp;
@@ -384,6 +389,7 @@ __global__ void k_ins_keys(const MapDev* maps, const SessInfo* si, int nsess, Ba
 
 // ------------------------------------------------------------------ K3 ----
 __global__ void k_seg_heads(const SessInfo* si, int nsess, BatchBufs b) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) b.cnt[5] = 0;  // refitting voxels, counted by k_seg_lenkeys
   VMP_GRID_LOOP(j, b.ntot) {
     const int s = sess_of(si, nsess, j);
     b.flags[j] = (j == si[s].off || b.skey_out[j] != b.skey_out[j - 1]) ? 1 : 0;
@@ -812,82 +818,238 @@ __device__ inline void refit(const MapDev& m, Moments& M, int n, uint32_t& flags
   for (int a = 0; a < 3; ++a) pl[3 + a] = sg * nn[a];
 }
 
-// sort keys that put long segments first (waves of the replay then walk
-// segments of similar length); slots past the segment count sort last.
-// Lengths are clamped below 2^lbits - 1, so the keys use the low lbits bits.
-__global__ void k_seg_lenkeys(BatchBufs b, int lbits) {
+// Whether this scan's walk of a voxel reaches a refit (updatePlane with
+// n >= update_size_thresh, voxel_map.cpp:42-95), from counts alone: an
+// uninitialised voxel refits when n reaches the threshold, an initialised
+// enabled one when newly_add does, before the walk stops at the freeze
+// point (n >= max_point_thresh); a build refits every voxel with enough
+// points once at the end.
+__device__ inline bool walk_refits(int n0, int newly0, uint32_t flags, int len, int upd, int mx, bool build) {
+  if (build) return n0 + len >= upd;
+  if ((flags & kInit) && !(flags & kEnable)) return false;
+  if (!(flags & kInit)) return len > upd - n0 - 1;
+  const int jr = upd - newly0 - 1;                  // walk index of the next refit
+  const int f = mx - n0 - 1 > 0 ? mx - n0 - 1 : 0;  // walk index of the freeze point
+  return jr < (len < f + 1 ? len : f + 1);
+}
+
+// Replay order: the voxels that refit first (the moment path), then the
+// rest, each part by descending segment length so waves walk segments of
+// similar length; slots past the segment count sort last.  Lengths are
+// clamped below 2^lbits - 1; the class is bit lbits.  Also counts the
+// refitting voxels (cnt[5], cleared by k_seg_heads).
+__global__ void __launch_bounds__(kMapThreads) k_seg_lenkeys(const MapDev* maps, const SessInfo* si, BatchBufs b,
+                                                             int lbits) {
+  __shared__ int bsum;
   const int nseg = b.cnt[0];
   const uint32_t top = uint32_t((uint64_t(1) << lbits) - 1);
+  if (threadIdx.x == 0) bsum = 0;
+  __syncthreads();
+  int heavy_n = 0;
   VMP_GRID_LOOP(q, b.ntot) {
-    b.len_in[q] = q < nseg ? top - min(uint32_t(b.seg_end[q] - b.seg_start[q]), top - 1) : top;
+    uint32_t key = (1u << lbits) | top;
+    if (q < nseg) {
+      const int len = b.seg_end[q] - b.seg_start[q];
+      const int32_t vid = b.seg_vid[q];
+      int heavy = 0;
+      if (vid >= 0) {
+        const int s = b.seg_sess[q];
+        const MapDev& m = maps[s];
+        const VoxState& st = m.vs[vid];
+        heavy = walk_refits(st.n, st.newly_add, st.flags, len, m.upd_pt, m.max_pt, si[s].is_build != 0) ? 1 : 0;
+      }
+      heavy_n += heavy;
+      key = (uint32_t(1 - heavy) << lbits) | (top - min(uint32_t(len), top - 1));
+    }
+    b.len_in[q] = key;
     b.ord_in[q] = q;
+  }
+  heavy_n = wave_sum_i(heavy_n);
+  if ((threadIdx.x & 63) == 0 && heavy_n) atomicAdd(&bsum, heavy_n);
+  __syncthreads();
+  if (threadIdx.x == 0 && bsum) atomicAdd(&b.cnt[5], bsum);
+}
+
+// The voxels that refit in this scan (the first cnt[5] of the replay
+// order): one lane per voxel replays its pending points and then its points
+// of this scan in order through the pushPoint state machine, with the FP64
+// moments in registers and refits from them.
+__global__ void __launch_bounds__(kMapThreads) k_replay_heavy(const MapDev* maps, const SessInfo* si, BatchBufs b) {
+  const int nh = b.cnt[5];
+  VMP_UNIFORM_LOOP(qi, valid, nh) {
+    int refits = 0, calls = 0, dropped = 0;
+    const int q = valid ? b.ord[qi] : 0;
+    const int s = valid ? b.seg_sess[q] : 0;
+    const MapDev& m = maps[s];
+    const int32_t vid = valid ? b.seg_vid[q] : -1;
+    if (vid >= 0) {
+      const SessInfo& S = si[s];
+      const int start = b.seg_start[q];
+      const int end = b.seg_end[q];
+      VoxState st = m.vs[vid];
+      double* pl = m.plane + size_t(vid) * kPlaneD;
+      double* mo = m.mom + size_t(vid) * kMomD;
+      double plv[kPlaneD];
+      Moments M;
+      const int cnt = st.n > 0 ? int(mo[kMomPend]) : 0;
+      if (st.n > 0) {
+        for (int k = 0; k < kPlaneD; ++k) plv[k] = pl[k];
+      } else {  // a new voxel (its records were not cleared at creation)
+        for (int k = 0; k < kPlaneD; ++k) plv[k] = 0.0;
+      }
+      if (st.n - cnt > 0) mom_load(mo, M);
+      else mom_zero(M);  // no point has reached the moments yet
+      if (cnt > m.pend_cap) atomicOr(&m.ctr->error, 32);
+      const float* pd = m.pend + size_t(vid) * m.pend_cap * kPendF;
+      for (int k = 0; k < cnt && k < m.pend_cap; ++k) {
+        const float qv[3] = {pd[k * kPendF], pd[k * kPendF + 1], pd[k * kPendF + 2]};
+        const float C[6] = {pd[k * kPendF + 3], pd[k * kPendF + 4], pd[k * kPendF + 5],
+                            pd[k * kPendF + 6], pd[k * kPendF + 7], pd[k * kPendF + 8]};
+        mom_add(M, qv, C);
+      }
+      int64_t kx, ky, kz;
+      unpack_key(st.key, kx, ky, kz);
+      const double center[3] = {(double(kx) + 0.5) * m.vs64, (double(ky) + 0.5) * m.vs64,
+                                (double(kz) + 0.5) * m.vs64};
+      const UpdParams& P = *S.prm;
+      uint32_t flags = st.flags;
+      int n = st.n, newly = st.newly_add;
+      int merge_pos = kNever, freeze_t = -1;
+      // Points are replayed in rounds: each lane advances to its next refit
+      // (or the end of its segment), then the lanes that reached one refit
+      // together, so the wave runs the FP64 refit once per round instead of
+      // whenever any lane needs it.  Per lane the order of operations is the
+      // reference's: add the point, refit, then the freeze check.
+      int pos = start;
+      bool active = true;
+      // prefetch of the next point (its index, then its coordinates)
+      int i_nx = pos < end ? b.sval_out[pos] - S.off : 0;
+      float x_nx = 0.f, y_nx = 0.f, z_nx = 0.f;
+      if (pos < end) {
+        x_nx = S.x[i_nx];
+        y_nx = S.y[i_nx];
+        z_nx = S.z[i_nx];
+      }
+      while (__any(active)) {
+        bool need = false;
+        while (active && !need) {
+          if (pos >= end) {
+            active = false;
+            break;
+          }
+          if ((flags & kInit) && !(flags & kEnable) && !S.is_build) {
+            if (flags & kPlane) {
+              merge_pos = pos - start;
+              calls += end - pos;
+            } else {
+              dropped += end - pos;
+            }
+            active = false;
+            break;
+          }
+          const int i = i_nx;
+          const float px = x_nx, py = y_nx, pz0 = z_nx;
+          if (pos + 1 < end) {
+            i_nx = b.sval_out[pos + 1] - S.off;
+            x_nx = S.x[i_nx];
+            y_nx = S.y[i_nx];
+            z_nx = S.z[i_nx];
+          }
+          ++pos;
+          float pw[3];
+          ins_transform(P.m, P.t, px, py, pz0, pw);
+          const float qv[3] = {float(double(pw[0]) - center[0]), float(double(pw[1]) - center[1]),
+                               float(double(pw[2]) - center[2])};
+          const float pz = pz0 == 0.0f ? 0.001f : pz0;
+          float C[6];
+          point_world_cov(P.rwl32, P.Prr, P.Ppp, P.s2, P.rv, px, py, pz, C);
+          mom_add(M, qv, C);
+          ++n;
+          if (S.is_build) continue;
+          if (!(flags & kInit)) {
+            // updatePlane after every point until the voxel initialises
+            need = n >= m.upd_pt;
+            continue;
+          }
+          ++newly;
+          if (newly >= m.upd_pt) {
+            need = true;
+            newly = 0;
+          }
+          if (n >= m.max_pt) {
+            flags &= ~kEnable;
+            freeze_t = i;
+          }
+        }
+        if (need) refit(m, M, n, flags, plv, center, refits);
+      }
+      if (S.is_build) refit(m, M, n, flags, plv, center, refits);
+      const double invn = 1.0 / double(n);
+      for (int a = 0; a < 3; ++a) plv[a] = center[a] + M.s1[a] * invn;
+      mom_store(mo, M);
+      for (int a = 0; a < 3; ++a) mo[kMomRun + a] = M.s1[a];
+      mo[kMomPend] = 0.0;
+      for (int k = 0; k < kPlaneD; ++k) pl[k] = plv[k];
+      st.flags = flags;
+      st.n = n;
+      st.newly_add = newly;
+      st.freeze_t = freeze_t;
+      st.merge_pos = merge_pos;
+      m.vs[vid] = st;
+      write_hot(m, vid, flags);
+    }
+    wave_fetch_add(&m.ctr->refits, refits);
+    wave_fetch_add(&m.ctr->merge_calls, calls);
+    wave_fetch_add(&m.ctr->dropped, dropped);
   }
 }
 
-__global__ void __launch_bounds__(kMapThreads) k_replay(const MapDev* maps, const SessInfo* si, BatchBufs b) {
-  const int nseg = b.cnt[0];
-  const int qi = blockIdx.x * blockDim.x + threadIdx.x;
-  const int q = qi < nseg ? b.ord[qi] : nseg;
-  int refits = 0, calls = 0, dropped = 0;
-  const int32_t vid = q < nseg ? b.seg_vid[q] : -1;
-  const int s = q < nseg ? b.seg_sess[q] : 0;
-  const MapDev& m = maps[s];
-  const bool frozen0 = vid >= 0 && !si[s].is_build && (m.vs[vid].flags & kInit) && !(m.vs[vid].flags & kEnable);
-  if (frozen0) {
-    // frozen before this scan: every point is a merge() call (plane) or dropped
-    const int len = b.seg_end[q] - b.seg_start[q];
-    VoxState& st = m.vs[vid];
-    if (st.flags & kPlane) {
-      st.merge_pos = 0;
-      calls += len;
-    } else {
-      dropped += len;
-    }
-  } else if (vid >= 0) {
-    const SessInfo& S = si[s];
-    const int start = b.seg_start[q];
-    const int end = b.seg_end[q];
-    VoxState st = m.vs[vid];
-    double* pl = m.plane + size_t(vid) * kPlaneD;
-    double plv[kPlaneD];
-    Moments M;
-    if (st.n > 0) {
-      for (int k = 0; k < kPlaneD; ++k) plv[k] = pl[k];
-      mom_load(m.mom + size_t(vid) * kMomD, M);
-    } else {  // a new voxel (its records were not cleared at creation)
-      for (int k = 0; k < kPlaneD; ++k) plv[k] = 0.0;
-      mom_zero(M);
-    }
-    int64_t kx, ky, kz;
-    unpack_key(st.key, kx, ky, kz);
-    const double center[3] = {(double(kx) + 0.5) * m.vs64, (double(ky) + 0.5) * m.vs64, (double(kz) + 0.5) * m.vs64};
-    const UpdParams& P = *S.prm;
-    uint32_t flags = st.flags;
-    int n = st.n, newly = st.newly_add;
-    int merge_pos = kNever, freeze_t = -1;
-    bool moved = false;
-    // Points are replayed in rounds: each lane advances to its next refit
-    // (or the end of its segment), then the lanes that reached one refit
-    // together, so the wave runs the FP64 refit once per round instead of
-    // whenever any lane needs it.  Per lane the order of operations is the
-    // reference's: add the point, refit, then the freeze check.
-    int pos = start;
-    bool active = true;
-    // prefetch of the next point (its index, then its coordinates)
-    int i_nx = pos < end ? b.sval_out[pos] - S.off : 0;
-    float x_nx = 0.f, y_nx = 0.f, z_nx = 0.f;
-    if (pos < end) {
-      x_nx = S.x[i_nx];
-      y_nx = S.y[i_nx];
-      z_nx = S.z[i_nx];
-    }
-    while (__any(active)) {
-      bool need = false;
-      while (active && !need) {
-        if (pos >= end) {
-          active = false;
-          break;
-        }
+// The other voxels (from cnt[5] on): frozen ones count their points as
+// merge() calls or drops; the rest add points without reaching a refit, so
+// only the count, the running mean and the freeze state change, and each
+// point (relative to the voxel centre, with its world cov) joins the pending
+// list that the voxel's next refit folds into the moments.
+__global__ void __launch_bounds__(kMapThreads) k_replay_light(const MapDev* maps, const SessInfo* si, BatchBufs b) {
+  const int nseg = b.cnt[0], nh = b.cnt[5];
+  VMP_UNIFORM_LOOP(qi, valid, nseg - nh) {
+    int calls = 0, dropped = 0;
+    const int q = valid ? b.ord[nh + qi] : 0;
+    const int s = valid ? b.seg_sess[q] : 0;
+    const MapDev& m = maps[s];
+    const int32_t vid = valid ? b.seg_vid[q] : -1;
+    const bool frozen0 = vid >= 0 && !si[s].is_build && (m.vs[vid].flags & kInit) && !(m.vs[vid].flags & kEnable);
+    if (frozen0) {
+      // frozen before this scan: every point is a merge() call (plane) or dropped
+      const int len = b.seg_end[q] - b.seg_start[q];
+      VoxState& st = m.vs[vid];
+      if (st.flags & kPlane) {
+        st.merge_pos = 0;
+        calls += len;
+      } else {
+        dropped += len;
+      }
+    } else if (vid >= 0) {
+      const SessInfo& S = si[s];
+      const UpdParams& P = *S.prm;
+      const int start = b.seg_start[q], end = b.seg_end[q];
+      VoxState st = m.vs[vid];
+      int64_t kx, ky, kz;
+      unpack_key(st.key, kx, ky, kz);
+      const double center[3] = {(double(kx) + 0.5) * m.vs64, (double(ky) + 0.5) * m.vs64,
+                                (double(kz) + 0.5) * m.vs64};
+      double* mo = m.mom + size_t(vid) * kMomD;
+      double s1[3] = {0.0, 0.0, 0.0};
+      int cnt = 0;
+      if (st.n > 0) {
+        for (int a = 0; a < 3; ++a) s1[a] = mo[kMomRun + a];
+        cnt = int(mo[kMomPend]);
+      }
+      float* pd = m.pend + size_t(vid) * m.pend_cap * kPendF;
+      uint32_t flags = st.flags;
+      int n = st.n, newly = st.newly_add;
+      int merge_pos = kNever, freeze_t = -1;
+      int i_nx = start < end ? b.sval_out[start] - S.off : 0;
+      for (int pos = start; pos < end; ++pos) {
         if ((flags & kInit) && !(flags & kEnable) && !S.is_build) {
           if (flags & kPlane) {
             merge_pos = pos - start;
@@ -895,18 +1057,11 @@ __global__ void __launch_bounds__(kMapThreads) k_replay(const MapDev* maps, cons
           } else {
             dropped += end - pos;
           }
-          active = false;
           break;
         }
         const int i = i_nx;
-        const float px = x_nx, py = y_nx, pz0 = z_nx;
-        if (pos + 1 < end) {
-          i_nx = b.sval_out[pos + 1] - S.off;
-          x_nx = S.x[i_nx];
-          y_nx = S.y[i_nx];
-          z_nx = S.z[i_nx];
-        }
-        ++pos;
+        if (pos + 1 < end) i_nx = b.sval_out[pos + 1] - S.off;
+        const float px = S.x[i], py = S.y[i], pz0 = S.z[i];
         float pw[3];
         ins_transform(P.m, P.t, px, py, pz0, pw);
         const float qv[3] = {float(double(pw[0]) - center[0]), float(double(pw[1]) - center[1]),
@@ -914,45 +1069,45 @@ __global__ void __launch_bounds__(kMapThreads) k_replay(const MapDev* maps, cons
         const float pz = pz0 == 0.0f ? 0.001f : pz0;
         float C[6];
         point_world_cov(P.rwl32, P.Prr, P.Ppp, P.s2, P.rv, px, py, pz, C);
-        mom_add(M, qv, C);
+        if (cnt < m.pend_cap) {
+          for (int a = 0; a < 3; ++a) pd[cnt * kPendF + a] = qv[a];
+          for (int a = 0; a < 6; ++a) pd[cnt * kPendF + 3 + a] = C[a];
+        } else {
+          atomicOr(&m.ctr->error, 32);
+        }
+        ++cnt;
+        for (int a = 0; a < 3; ++a) s1[a] += double(qv[a]);
         ++n;
-        moved = true;
         if (S.is_build) continue;
         if (!(flags & kInit)) {
-          // updatePlane after every point until the voxel initialises
-          need = n >= m.upd_pt;
+          if (n >= m.upd_pt) atomicOr(&m.ctr->error, 32);  // a refit: the other kernel's voxel
           continue;
         }
         ++newly;
-        if (newly >= m.upd_pt) {
-          need = true;
-          newly = 0;
-        }
+        if (newly >= m.upd_pt) atomicOr(&m.ctr->error, 32);
         if (n >= m.max_pt) {
           flags &= ~kEnable;
           freeze_t = i;
         }
       }
-      if (need) refit(m, M, n, flags, plv, center, refits);
-    }
-    if (S.is_build) refit(m, M, n, flags, plv, center, refits);
-    if (moved) {
+      for (int a = 0; a < 3; ++a) mo[kMomRun + a] = s1[a];
+      mo[kMomPend] = double(cnt);
+      double* pl = m.plane + size_t(vid) * kPlaneD;
       const double invn = 1.0 / double(n);
-      for (int a = 0; a < 3; ++a) plv[a] = center[a] + M.s1[a] * invn;
-      mom_store(m.mom + size_t(vid) * kMomD, M);
+      for (int a = 0; a < 3; ++a) pl[a] = center[a] + s1[a] * invn;
+      if (st.n == 0)  // a new voxel: its plane record was not cleared at creation
+        for (int k = 3; k < kPlaneD; ++k) pl[k] = 0.0;
+      st.flags = flags;
+      st.n = n;
+      st.newly_add = newly;
+      st.freeze_t = freeze_t;
+      st.merge_pos = merge_pos;
+      m.vs[vid] = st;
+      write_hot(m, vid, flags);
     }
-    for (int k = 0; k < kPlaneD; ++k) pl[k] = plv[k];
-    st.flags = flags;
-    st.n = n;
-    st.newly_add = newly;
-    st.freeze_t = freeze_t;
-    st.merge_pos = merge_pos;
-    m.vs[vid] = st;
-    write_hot(m, vid, flags);
+    wave_fetch_add(&m.ctr->merge_calls, calls);
+    wave_fetch_add(&m.ctr->dropped, dropped);
   }
-  wave_fetch_add(&m.ctr->refits, refits);
-  wave_fetch_add(&m.ctr->merge_calls, calls);
-  wave_fetch_add(&m.ctr->dropped, dropped);
 }
 
 }  // namespace vmp
