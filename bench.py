@@ -162,8 +162,13 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000, groups=1):
     from voxelmapplus_fastlio2_amd import Context
 
     ctxs = [ctx] + [Context(0) for _ in range(groups - 1)]
-    regs = [Registrar(ctxs[i % groups], cfg) for i in range(B)]
-    members = [list(range(gi, B, groups)) for gi in range(groups)]
+
+    # sessions alternate room / street; each group gets both kinds (or, with
+    # VMP_BENCH_GROUPING=stride, every groups-th session)
+    stride = os.environ.get("VMP_BENCH_GROUPING") == "stride"
+    gof = [(i % groups) if stride else ((i // 2) % groups) for i in range(B)]
+    members = [[i for i in range(B) if gof[i] == gi] for gi in range(groups)]
+    regs = [Registrar(ctxs[gof[i]], cfg) for i in range(B)]
     for gi, cx in enumerate(ctxs):  # the largest step's workspace, allocated before timing
         cx.reserve(int(sizes[:, members[gi]].sum(axis=1).max()), len(members[gi]))
 
@@ -372,10 +377,10 @@ def main():
     ap.add_argument("--cpu-timed", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-sessions", type=int, default=128)
-    ap.add_argument("--batch-steps", type=int, default=10)
-    ap.add_argument("--batch-warmup", type=int, default=3)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-warmup", type=int, default=5)
     ap.add_argument("--batch-capacity", type=int, default=200000)
-    ap.add_argument("--batch-groups", type=int, default=2,
+    ap.add_argument("--batch-groups", type=int, default=3,
                     help="session groups, each on its own context (HIP stream) and host thread")
     ap.add_argument("--obs-ppt", type=int, default=0, help="observe points per thread (0 = by scan size)")
     ap.add_argument("--serve-sessions", type=int, default=512)
