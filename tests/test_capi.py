@@ -40,8 +40,9 @@ def test_session_bytes_grows_with_capacity():
     small = L.vmp_session_bytes(C.byref(LIOConfig(map_capacity=10000, max_scan_points=20000).to_c()))
     big = L.vmp_session_bytes(C.byref(LIOConfig(map_capacity=2000000, max_scan_points=120000).to_c()))
     assert 0 < small < big
-    # a 2M-voxel street session stays well under 2 GiB of HBM
-    assert big < 2 << 30
+    # a 2M-voxel street session (FP64 moments and the pending-point lists
+    # included) stays under 3 GiB of HBM
+    assert big < 3 << 30
 
 
 def test_context_without_gpu_raises():

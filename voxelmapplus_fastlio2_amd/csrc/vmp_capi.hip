@@ -289,13 +289,8 @@ static int ws_reserve(vmp_ctx* c, int64_t points, int nsess, bool exact = false)
   ok &= dalloc(A, &b.seg_entry, n) == hipSuccess;
   ok &= dalloc(A, &b.cre_t, n) == hipSuccess;
   ok &= dalloc(A, &b.cre_seg, n) == hipSuccess;
-  ok &= dalloc(A, &b.co_vid, n) == hipSuccess;
-  ok &= dalloc(A, &b.co_seg, n) == hipSuccess;
-  ok &= dalloc(A, &b.co_t, n) == hipSuccess;
-  ok &= dalloc(A, &b.co_prev, n) == hipSuccess;
-  ok &= dalloc(A, &b.co_group, n) == hipSuccess;
+  ok &= dalloc(A, &b.co, n) == hipSuccess;
   ok &= dalloc(A, &b.ev_vid, n) == hipSuccess;
-  ok &= dalloc(A, &b.ev_t, n) == hipSuccess;
   ok &= dalloc(A, &b.cand_in, n) == hipSuccess;
   ok &= dalloc(A, &b.cand_out, n) == hipSuccess;
   ok &= dalloc(A, &b.cand_nbr, n * 6) == hipSuccess;
@@ -309,6 +304,7 @@ static int ws_reserve(vmp_ctx* c, int64_t points, int nsess, bool exact = false)
   ok &= dalloc(A, &b.len_out, n) == hipSuccess;
   ok &= dalloc(A, &b.ord_in, n) == hipSuccess;
   ok &= dalloc(A, &b.ord, n) == hipSuccess;
+  ok &= dalloc(A, &b.lv, n * 2) == hipSuccess;
   // taint-phase prefetch of up to 256k acting candidates per batch (the rest
   // take the replay's own lookups)
   b.pre_cap = int32_t(std::min<size_t>(n, size_t(1) << 18));
@@ -798,6 +794,10 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
     int sbits = 0;
     while ((1 << sbits) < n) ++sbits;
     b.hbits = hbits;
+    int tbits = 1;  // point times of the largest scan fit in tbits bits
+    for (int i = 0; i < n; ++i)
+      while ((int64_t(1) << tbits) < int64_t(ss[i]->n)) ++tbits;
+    b.cand_sshift = 25 + tbits;
     hipLaunchKernelGGL(k_begin_scan, dim3(n), dim3(1), 0, st, M, n);
     if (ntot > 0) {
       size_t tb = w.temp_bytes;
@@ -823,6 +823,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
       hipLaunchKernelGGL(k_cre_split, dim3(gn), dim3(T), 0, st, S, n, b);
       hipLaunchKernelGGL(k_create_fast, dim3(gn), dim3(T), 0, st, M, S, n, b);
       hipLaunchKernelGGL(k_create_fast_ctr, dim3(n), dim3(1), 0, st, M, n);
+      hipLaunchKernelGGL(k_ring_live, dim3(kLvBlocks, n), dim3(64 * kLvWaves), 0, st, M, S, n, b);
       hipLaunchKernelGGL(k_create_evict, dim3(n), dim3(64), 0, st, M, S, n, b);
       hipLaunchKernelGGL(k_create_init, dim3(gn), dim3(T), 0, st, M, S, n, b);
       // replay order by segment length, capped at 10 bits (two radix passes;
@@ -846,7 +847,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         // sorted on (session, point time) only: the call at point time t is
         // made by the voxel that received point t, so t fixes the voxel bits;
         // unused slots hold ~0 and sort last
-        if (rocprim::radix_sort_keys(w.temp, tb, b.cand_in, b.cand_out, size_t(ntot), 25, 49 + sbits, st) !=
+        if (rocprim::radix_sort_keys(w.temp, tb, b.cand_in, b.cand_out, size_t(ntot), 25, b.cand_sshift + sbits, st) !=
             hipSuccess)
           return VMP_EDEVICE;
         hipLaunchKernelGGL(k_cand_starts, dim3(1), dim3(kCsThreads), 0, st, M, n, b);

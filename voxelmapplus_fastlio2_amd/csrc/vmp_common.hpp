@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include <cmath>
+#include <cstddef>
 
 #include "vmp.h"
 
@@ -83,6 +84,8 @@ struct __align__(16) VoxHot {
 
 // Per-voxel bookkeeping for the map update.  Fields named *_t are point
 // indices within the current scan and are valid only when tag == scan tag.
+// The per-scan fields fill bytes 40-79 (one 8-byte and two 16-byte groups),
+// so a new scan tag is written with three vector stores.
 struct __align__(16) VoxState {
   uint64_t key;
   uint64_t group_id;
@@ -90,17 +93,38 @@ struct __align__(16) VoxState {
   int32_t n;          // Plane::n
   int32_t newly_add;  // newly_add_point
   uint32_t flags;
-  int32_t tag;        // scan tag of the per-scan fields below
+  int32_t pad;
+  int32_t merge_pos;  // position in the segment where merge() calls start
+  int32_t dirty_t;    // latest point time at which a merge() call could act
+  int32_t tag;        // scan tag of the per-scan fields
   int32_t create_t;   // created by point create_t of this scan (-1: older)
   int32_t evict_t;    // evicted at point evict_t of this scan (INT32_MAX: no)
   int32_t freeze_t;   // froze at point freeze_t of this scan (-1: no)
   int32_t prev_vid;   // incarnation of the same key evicted in this scan
   int32_t first_t;    // first point of this scan that touched it
   int32_t seg;        // its segment in this scan's key-sorted points
-  int32_t merge_pos;  // position in the segment where merge() calls start
-  int32_t dirty_t;    // latest point time at which a merge() call could act
   int32_t taint_t;    // first point time a merge changed its neighbourhood
 };
+static_assert(sizeof(VoxState) == 80, "VoxState is 80 bytes");
+static_assert(offsetof(VoxState, merge_pos) == 40 && offsetof(VoxState, tag) == 48 &&
+                  offsetof(VoxState, prev_vid) == 64,
+              "per-scan VoxState fields are grouped for vector stores");
+
+// Pointers loaded from memory compile to flat accesses, which also count
+// against the LDS wait counter; this names the global address space.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* as_global(T* p) {
+  return (__attribute__((address_space(1))) T*)(p);
+}
+typedef int32_t vmp_i2 __attribute__((ext_vector_type(2)));
+typedef int32_t vmp_i4 __attribute__((ext_vector_type(4)));
+// 8- and 16-byte stores of int32 groups (p aligned to 8 / 16 bytes)
+__device__ __forceinline__ void st_global2(int32_t* p, int32_t a, int32_t b) {
+  *as_global(reinterpret_cast<vmp_i2*>(p)) = vmp_i2{a, b};
+}
+__device__ __forceinline__ void st_global4(int32_t* p, int32_t a, int32_t b, int32_t c, int32_t d) {
+  *as_global(reinterpret_cast<vmp_i4*>(p)) = vmp_i4{a, b, c, d};
+}
 
 // Plane record in FP64 (what merge() reads and writes, voxel_map.cpp:138-186):
 // mean xyz, normal xyz, trace of the normal block and of the centre block of

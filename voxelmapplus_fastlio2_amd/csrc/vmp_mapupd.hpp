@@ -96,7 +96,7 @@ struct BatchBufs {
   int32_t ntot;
   int32_t nsess;
   int32_t hbits;       // bits of the largest hash-slot index in the batch
-  int32_t pad0;
+  int32_t cand_sshift; // position of the session bits in a batch candidate key (25 + time bits)
   uint64_t* skey_in;   // (session << hbits) | hash slot, per point
   int32_t* sval_in;    // global point index
   uint64_t* skey_out;
@@ -111,13 +111,10 @@ struct BatchBufs {
   int32_t* seg_entry;
   int32_t* cre_t;      // creation events: local point index, segment
   int32_t* cre_seg;
-  int32_t* co_vid;     // resolved creations, per session at its offset
-  int32_t* co_seg;
-  int32_t* co_t;
-  int32_t* co_prev;
-  uint64_t* co_group;
-  int32_t* ev_vid;     // evictions in order, per session at its offset
-  int32_t* ev_t;
+  int4* co;            // resolved creations in order, per session at its offset:
+                       // (vid, segment, point time, evicted incarnation or -1);
+                       // the r-th takes group id group_counter - ncre + r
+  int32_t* ev_vid;     // evicted voxels in order, per session at its offset
   uint64_t* cand_in;   // merge candidates, see cand_key(); per session at its offset
   uint64_t* cand_out;  // sorted: session-major, then point time, then voxel
   int32_t* cand_nbr;   // per sorted candidate: 6 eligible neighbour ids at its time
@@ -141,6 +138,11 @@ struct BatchBufs {
   int32_t* pre_tn;
   int32_t pre_cap;
   int32_t pad1;
+  // live LRU ring entries from the head of each evicting session, compacted
+  // within 64-entry chunks: (vid, first touch in this scan or kNever,
+  // segment, offset in the chunk), vid -1 after the chunk's last live entry;
+  // chunk k of a session at 2 * off + 64 k (two entries per point)
+  int4* lv;
 };
 
 // merge-candidate key: session (15 bits) | point time (24 bits) | voxel (25 bits)
@@ -149,7 +151,13 @@ __host__ __device__ inline uint64_t cand_key(int s, int t, int32_t vid) {
 }
 __host__ __device__ inline int cand_t(uint64_t k) { return int((k >> 25) & 0xFFFFFFull); }
 __host__ __device__ inline int32_t cand_vid(uint64_t k) { return int32_t(k & 0x1FFFFFFull); }
-constexpr uint64_t kCandLocal = (uint64_t(1) << 49) - 1;  // key without the session bits
+// In the batch's candidate list the session sits right above the point time
+// (at bit cand_sshift, 25 + the bits of the largest scan), so the sort reads
+// no constant bits; the local part (time, voxel) is cand_key(0, t, vid).
+__host__ __device__ inline uint64_t cand_key_batch(int s, int t, int32_t vid, int sshift) {
+  return (uint64_t(uint32_t(s)) << sshift) | (uint64_t(uint32_t(t)) << 25) | uint64_t(uint32_t(vid));
+}
+__host__ __device__ inline uint64_t cand_local(uint64_t k, int sshift) { return k & ((uint64_t(1) << sshift) - 1); }
 
 __device__ inline int sess_of(const SessInfo* si, int nsess, int g) {
   int lo = 0, hi = nsess - 1;
@@ -366,7 +374,7 @@ __global__ void k_ins_keys(const MapDev* maps, const SessInfo* si, int nsess, Ba
     const int s = valid ? sess_of(si, nsess, g) : 0;
     if (valid) {
     const SessInfo& S = si[s];
-    const MapDev& m = maps[s];
+    const MapDev m = maps[s];
     const int i = g - S.off;
     const UpdParams& P = *S.prm;
     float pw[3];
@@ -405,7 +413,7 @@ __global__ void k_seg_info(const MapDev* maps, const SessInfo* si, int nsess, Ba
     const SessInfo& S = si[s];
     const int lim = S.off + S.n;
     const int end = (q + 1 < nseg && b.sel[q + 1] < lim) ? b.sel[q + 1] : lim;
-    const MapDev& m = maps[s];
+    const MapDev m = maps[s];
     const uint32_t e = uint32_t(b.skey_out[start] & ((uint64_t(1) << b.hbits) - 1));
     const int first = b.sval_out[start] - S.off;
     b.seg_start[q] = start;
@@ -414,18 +422,11 @@ __global__ void k_seg_info(const MapDev* maps, const SessInfo* si, int nsess, Ba
     b.seg_entry[q] = int32_t(e);
     const int32_t vid = m.table[e].vid;
     b.seg_vid[q] = vid;
-    if (vid >= 0) {
-      VoxState& st = m.vs[vid];
-      st.tag = m.ctr->tag;
-      st.create_t = -1;
-      st.evict_t = kNever;
-      st.freeze_t = -1;
-      st.prev_vid = -1;
-      st.first_t = first;
-      st.seg = q;
-      st.merge_pos = kNever;
-      st.dirty_t = -2;
-      st.taint_t = kNever;
+    if (vid >= 0) {  // this scan's fields (three vector stores)
+      VoxState* st = m.vs + vid;
+      st_global2(&st->merge_pos, kNever, -2);
+      st_global4(&st->tag, m.ctr->tag, -1, kNever, -1);
+      st_global4(&st->prev_vid, -1, first, q, kNever);
     } else {
       b.flags[S.off + first] = q + 1;  // creation event at its first point (array cleared before)
     }
@@ -448,24 +449,19 @@ __global__ void k_create_fast(const MapDev* maps, const SessInfo* si, int nsess,
   const int nc = b.cnt[1];
   VMP_GRID_LOOP(r, nc) {
     const int s = sess_of(si, nsess, b.sel[r]);
-    const MapDev& m = maps[s];
+    const MapDev m = maps[s];
     const MapCounters* c = m.ctr;
     const int ncs = si[s].cre_hi - si[s].cre_lo;
     if (c->size + ncs > m.capacity) continue;
     const int rl = r - si[s].cre_lo;
     const int ftop = c->free_top, bump = c->bump;
     const int32_t vid = rl < ftop ? m.freelist[ftop - 1 - rl] : bump + (rl - ftop);
-    const int o = si[s].off + rl;
-    b.co_vid[o] = vid;
-    b.co_seg[o] = b.cre_seg[r];
-    b.co_t[o] = b.cre_t[r];
-    b.co_prev[o] = -1;
-    b.co_group[o] = c->group_counter + uint64_t(rl);
+    b.co[si[s].off + rl] = make_int4(vid, b.cre_seg[r], b.cre_t[r], -1);
   }
 }
 __global__ void k_create_fast_ctr(const MapDev* maps, int nsess) {
   if (threadIdx.x != 0 || int(blockIdx.x) >= nsess) return;
-  const MapDev& m = maps[blockIdx.x];
+  const MapDev m = maps[blockIdx.x];
   MapCounters* c = m.ctr;
   const int nc = c->ncreate_ev;
   if (c->size + nc > m.capacity) return;
@@ -484,140 +480,359 @@ __global__ void k_create_fast_ctr(const MapDev* maps, int nsess) {
   c->fast = 1;
 }
 
+// Chunks of the ring that k_ring_live compacts for a session: enough for the
+// creations of the scan at a few stale entries per victim, within the
+// session's two entries per point; the ordered pass reads the ring directly
+// beyond them.
+__device__ inline int64_t lv_chunks(const MapCounters* c, const SessInfo& S) {
+  const int64_t used = c->ring_tail - c->ring_head;
+  const int64_t want = int64_t(6) * (S.cre_hi - S.cre_lo) + 4096;
+  return min(min((used + 63) >> 6, (want + 63) >> 6), (int64_t(2) * S.n) >> 6);
+}
+
+constexpr int kLvWaves = 4;   // waves per block of k_ring_live
+constexpr int kLvBlocks = 32; // blocks per session
+
+// Liveness of the ring entries ahead of the head (grid: kLvBlocks x
+// sessions), one wave per 64-entry chunk.  An entry is live while its voxel
+// is alive, still carries the entry's stamp and has not been evicted in this
+// scan; nothing the ordered pass changes makes another entry live.
+__global__ void __launch_bounds__(64 * kLvWaves) k_ring_live(const MapDev* maps, const SessInfo* si, int nsess,
+                                                              BatchBufs b) {
+  const int s = blockIdx.y;
+  if (s >= nsess) return;
+  const MapDev m = maps[s];
+  const MapCounters* c = m.ctr;
+  if (c->fast) return;
+  const SessInfo S = si[s];
+  const int64_t nch = lv_chunks(c, S);
+  const int64_t head = c->ring_head, tail = c->ring_tail;
+  const int tag = c->tag;
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int4* out0 = b.lv + int64_t(2) * S.off;
+  for (int64_t k = int64_t(blockIdx.x) * kLvWaves + (threadIdx.x >> 6); k < nch;
+       k += int64_t(gridDim.x) * kLvWaves) {
+    const int64_t idx = head + k * 64 + lane;
+    int32_t v = -1, first = kNever, seg = -1;
+    bool live = false;
+    if (idx < tail) {
+      const int64_t ph = idx % m.qcap;
+      v = m.ring_vid[ph];
+      const uint64_t stp = m.ring_stamp[ph];
+      const VoxState& st = m.vs[v];
+      const bool cur = st.tag == tag;
+      live = (st.flags & kAlive) && st.stamp == stp && !(cur && st.evict_t != kNever);
+      first = (cur && st.create_t < 0) ? st.first_t : kNever;
+      seg = st.seg;
+    }
+    const uint64_t bal = __ballot(live);
+    int4* out = out0 + k * 64;
+    if (live) out[__popcll(bal & below)] = make_int4(v, first, seg, lane);
+    if (lane >= __popcll(bal)) out[lane] = make_int4(-1, kNever, -1, 64);
+  }
+}
+
 // Ordered creation + eviction, one wave per session that may overflow.  LRU
 // order is the descending order of last-touch stamps; the ring holds
 // (vid, stamp) in ascending stamp order with stale entries skipped lazily.
 // The victim of each creation is the oldest live voxel not yet touched by an
 // earlier point of this scan; a victim touched later comes back as a new
 // voxel at its first touch (a creation event pushed on a min-heap).
+//
+// The wave holds 64-entry windows of the ring (with each entry's voxel
+// state), of the creation events and of the free list in its lanes, so a
+// step reads global memory only to refill a window.  While the next
+// creations precede every pending re-creation and the next live ring entries
+// are untouched in this scan, up to 64 creations take their victims in one
+// step (such victims are eligible whatever the creation's time and never come
+// back in this scan).
 __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
   const int s = blockIdx.x;
   if (s >= nsess) return;
-  const MapDev& m = maps[s];
+  const MapDev m = maps[s];
   MapCounters* c = m.ctr;
   if (c->fast) return;
   const int clo = si[s].cre_lo, ncre_ev = si[s].cre_hi - clo;
   const int off = si[s].off;
-  __shared__ uint64_t heap[kHeapCap];
+  __shared__ uint64_t heap[kHeapCap];     // (first touch << 32) | evicted vid
+  __shared__ int32_t heap_seg[kHeapCap];  // the key's segment in this scan
   const int lane = threadIdx.x;
   const int tag = c->tag;
+  const int64_t cap = m.capacity;
   int hn = 0;
   int64_t size = c->size;
   uint64_t gc = c->group_counter;
   int ftop = c->free_top, bump = c->bump;
   int64_t ptr = c->ring_head;
   const int64_t tail = c->ring_tail;
-  int64_t base = -1;  // first ring index held by the lanes
-  int32_t l_vid = -1;
-  int32_t l_first = kNever;  // first touch this scan (kNever if untouched)
-  bool l_live = false;
+  // ring window [rbase, rbase + 64): per lane an entry's vid, liveness, first
+  // touch in this scan (kNever if untouched), segment, offset in the window,
+  // and whether the voxel state carries this tag.  The first npre chunks from
+  // the head come compacted from k_ring_live (the next one prefetched).
+  const int64_t head0 = ptr;
+  const int64_t npre = lv_chunks(c, si[s]);
+  const int4* lvs = b.lv + int64_t(2) * off;
+  int64_t rbase = -1, nk = -1;
+  int4 n_e = make_int4(-1, kNever, -1, 64);
+  int32_t l_vid = -1, l_first = kNever, l_seg = -1, l_off = 64;
+  bool l_live = false, l_cur = false;
+  // creation window [cbase, cbase + 64); free-list window [fbase, fbase + 128)
+  // in two registers (the list is only popped here, its entries do not change)
+  int cbase = -1, fbase = -1;
+  int32_t c_t = kNever, c_seg = -1, f_lo = -1, f_hi = -1;
   int ci = 0, nco = 0, nev = 0;
   int err = 0;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+
+  auto load_ring = [&](int64_t at) {
+    const int64_t k = (at - head0) >> 6;
+    if (k < npre) {
+      rbase = head0 + 64 * k;
+      const int4 e = nk == k ? n_e : lvs[k * 64 + lane];
+      if (k + 1 < npre) {
+        n_e = lvs[(k + 1) * 64 + lane];
+        nk = k + 1;
+      }
+      l_vid = e.x;
+      l_first = e.y;
+      l_seg = e.z;
+      l_off = e.w;
+      l_live = e.x >= 0;
+      l_cur = e.y != kNever;  // a live voxel carries this tag iff touched in this scan
+      return;
+    }
+    rbase = at;
+    const int64_t idx = at + lane;
+    l_vid = -1;
+    l_live = false;
+    l_cur = false;
+    l_first = kNever;
+    l_seg = -1;
+    l_off = lane;
+    if (idx < tail) {
+      const int64_t ph = idx % m.qcap;
+      const int32_t v = m.ring_vid[ph];
+      const uint64_t stp = m.ring_stamp[ph];
+      const VoxState& st = m.vs[v];
+      l_vid = v;
+      l_cur = st.tag == tag;
+      l_live = (st.flags & kAlive) && st.stamp == stp && !(l_cur && st.evict_t != kNever);
+      l_first = (l_cur && st.create_t < 0) ? st.first_t : kNever;
+      l_seg = st.seg;
+    }
+  };
+  auto load_cre = [&](int at) {
+    cbase = at;
+    c_t = kNever;
+    c_seg = -1;
+    if (at + lane < ncre_ev) {
+      c_t = b.cre_t[clo + at + lane];
+      c_seg = b.cre_seg[clo + at + lane];
+    }
+  };
+  // make the window hold the next k free-list entries (k <= 64, k <= ftop)
+  auto load_free = [&](int k) {
+    if (fbase >= 0 && ftop - k >= fbase) return;
+    fbase = max(0, ftop - 128);
+    f_lo = fbase + lane < ftop ? m.freelist[fbase + lane] : -1;
+    f_hi = fbase + 64 + lane < ftop ? m.freelist[fbase + 64 + lane] : -1;
+  };
+  auto free_at = [&](int i) {  // entry i of the list (inside the window), per lane
+    const int q = i - fbase;
+    const int32_t a = __shfl(f_lo, q & 63, 64), h = __shfl(f_hi, q & 63, 64);
+    return q < 64 ? a : h;
+  };
+  auto free_at_u = [&](int i) {  // the same for an index uniform over the wave
+    const int q = i - fbase;
+    return q < 64 ? __builtin_amdgcn_readlane(f_lo, q) : __builtin_amdgcn_readlane(f_hi, q - 64);
+  };
+  auto evict_write = [&](int32_t victim, bool cur, int t) {  // caller: one lane
+    VoxState* st = m.vs + victim;
+    if (cur) {
+      *as_global(&st->evict_t) = t;
+    } else {  // this scan's fields start here (three vector stores)
+      st_global2(&st->merge_pos, kNever, -2);
+      st_global4(&st->tag, tag, -1, t, -1);
+      st_global4(&st->prev_vid, -1, kNever, -1, kNever);
+    }
+  };
+
   while (true) {
-    const int t_s = ci < ncre_ev ? b.cre_t[clo + ci] : kNever;
+    if (ci < ncre_ev && (cbase < 0 || ci >= cbase + 64)) load_cre(ci);
+    const int t_s = ci < ncre_ev ? __builtin_amdgcn_readlane(c_t, ci - cbase) : kNever;
     const int t_h = hn > 0 ? int(heap[0] >> 32) : kNever;
     if (t_s == kNever && t_h == kNever) break;
+    if (t_s < t_h) {
+      // ---- bulk: creations that need no victim (below capacity, where no
+      //      re-created key can be pending)
+      if (size < cap && hn == 0) {
+        const int mm = int(min<int64_t>(cap - size, int64_t(ncre_ev - ci)));
+        for (int j0 = 0; j0 < mm; j0 += 64) {
+          const int j = j0 + lane;
+          if (j < mm) {
+            const int fj = ftop - 1 - j;  // the same allocation order as one by one
+            const int32_t vid = fj >= 0 ? m.freelist[fj] : bump + (j - ftop);
+            if (vid >= m.pool) err |= 8;
+            b.co[off + nco + j] = make_int4(vid, b.cre_seg[clo + ci + j], b.cre_t[clo + ci + j], -1);
+          }
+        }
+        if (__ballot(err != 0)) {
+          err |= 8;
+          break;
+        }
+        if (mm <= ftop) ftop -= mm;
+        else {
+          bump += mm - ftop;
+          ftop = 0;
+        }
+        ci += mm;
+        nco += mm;
+        gc += uint64_t(mm);
+        size += mm;
+        continue;
+      }
+      // ---- bulk: creations before the next re-creation, victims from the
+      //      untouched live ring entries ahead of the first touched one
+      if (size >= cap && ptr < tail) {
+        if (rbase < 0 || ptr >= rbase + 64) load_ring(ptr);
+        const bool ahead = l_live && rbase + l_off >= ptr;
+        const uint64_t touched = __ballot(ahead && l_first != kNever);
+        const int tpos = touched ? __ffsll((unsigned long long)touched) - 1 : 64;
+        const uint64_t usable = __ballot(ahead && l_first == kNever) & (tpos >= 64 ? ~0ull : ((1ull << tpos) - 1));
+        const int d = ci - cbase;  // creation ci sits in lane d of its window
+        const int ncr = __popcll(__ballot(lane >= d && c_t < t_h));
+        const int mm = min(__popcll(usable), ncr);
+        if (mm > 0) {
+          // the r-th usable entry is the victim of creation ci + r
+          const bool mine = ((usable >> lane) & 1ull) != 0;
+          const int r = __popcll(usable & below);
+          const bool act = mine && r < mm;
+          const int src = d + (act ? r : 0);
+          const int t = __shfl(c_t, src, 64);
+          const int sg = __shfl(c_seg, src, 64);
+          const int nf = min(mm, ftop);  // victims' slots from the free list, then the bump
+          if (nf > 0) load_free(nf);
+          const int fj = ftop - 1 - (act ? r : 0);
+          const int32_t fv = nf > 0 ? free_at(max(fj, fbase)) : -1;
+          if (act) {
+            const int32_t vid = fj >= 0 ? fv : bump + (r - ftop);
+            if (vid >= m.pool) err |= 8;
+            b.co[off + nco + r] = make_int4(vid, sg, t, -1);
+            evict_write(l_vid, l_cur, t);
+            b.ev_vid[off + nev + r] = l_vid;
+            l_live = false;
+          }
+          if (__ballot(err != 0)) {
+            err |= 8;
+            break;
+          }
+          uint64_t rem = usable;  // the ring position after the mm-th victim
+          for (int k = 1; k < mm; ++k) rem &= rem - 1;
+          ptr = rbase + __builtin_amdgcn_readlane(l_off, __ffsll((unsigned long long)rem) - 1) + 1;
+          if (mm <= ftop) ftop -= mm;
+          else {
+            bump += mm - ftop;
+            ftop = 0;
+          }
+          ci += mm;
+          nco += mm;
+          nev += mm;
+          gc += uint64_t(mm);
+          continue;
+        }
+        if (touched == 0 && usable == 0) {  // no live entry left in the window
+          ptr = min<int64_t>(rbase + 64, tail);
+          continue;
+        }
+        // a voxel touched in this scan comes first: the single step decides
+      }
+    }
+    // ---- single step: the earliest event, then its victim if over capacity
     int t, seg, prev;
     if (t_s < t_h) {
       t = t_s;
-      seg = b.cre_seg[clo + ci];
+      seg = __builtin_amdgcn_readlane(c_seg, ci - cbase);
       prev = -1;
       ++ci;
     } else {
       const uint64_t top = heap[0];
       t = int(top >> 32);
       prev = int32_t(top & 0xffffffffu);  // evicted incarnation of this key
-      seg = m.vs[prev].seg;
-      __syncthreads();
+      seg = heap_seg[0];
+      __builtin_amdgcn_wave_barrier();
       if (lane == 0) {
         const uint64_t lastv = heap[hn - 1];
+        const int32_t lasts = heap_seg[hn - 1];
         const int nn = hn - 1;
         int i = 0;
         while (true) {
           const int l = 2 * i + 1, r = l + 1;
           int sm = i;
           uint64_t sv = lastv;
-          if (l < nn && heap[l] < sv) { sm = l; sv = heap[l]; }
-          if (r < nn && heap[r] < sv) { sm = r; sv = heap[r]; }
+          if (l < nn && heap[l] < sv) {
+            sm = l;
+            sv = heap[l];
+          }
+          if (r < nn && heap[r] < sv) {
+            sm = r;
+            sv = heap[r];
+          }
           if (sm == i) break;
           heap[i] = heap[sm];
+          heap_seg[i] = heap_seg[sm];
           i = sm;
         }
-        if (nn > 0) heap[i] = lastv;
+        if (nn > 0) {
+          heap[i] = lastv;
+          heap_seg[i] = lasts;
+        }
       }
-      __syncthreads();
+      __builtin_amdgcn_wave_barrier();
       --hn;
     }
     int32_t vid;
-    if (ftop > 0) vid = m.freelist[--ftop];
-    else vid = bump++;
+    if (ftop > 0) {
+      load_free(1);
+      vid = free_at_u(ftop - 1);
+      --ftop;
+    } else {
+      vid = bump++;
+    }
     if (vid >= m.pool) {
       err |= 8;
       break;
     }
-    if (lane == 0) {
-      b.co_vid[off + nco] = vid;
-      b.co_seg[off + nco] = seg;
-      b.co_t[off + nco] = t;
-      b.co_prev[off + nco] = prev;
-      b.co_group[off + nco] = gc;
-    }
+    if (lane == 0) b.co[off + nco] = make_int4(vid, seg, t, prev);
     ++gc;
     ++nco;
     ++size;
-    if (size > m.capacity) {
-      int32_t victim = -1;
-      while (victim < 0) {
+    if (size > cap) {
+      int32_t victim = -1, vfirst = kNever, vseg = -1;
+      bool vcur = false;
+      while (true) {
         if (ptr >= tail) {
           err |= 4;
           break;
         }
-        if (base < 0 || ptr >= base + 64) {
-          base = ptr;
-          const int64_t idx = base + lane;
-          l_vid = -1;
-          l_live = false;
-          l_first = kNever;
-          if (idx < tail) {
-            const int64_t ph = idx % m.qcap;
-            const int32_t v = m.ring_vid[ph];
-            const uint64_t stp = m.ring_stamp[ph];
-            const VoxState& st = m.vs[v];
-            l_vid = v;
-            l_live = (st.flags & kAlive) && st.stamp == stp && !(st.tag == tag && st.evict_t != kNever);
-            l_first = (st.tag == tag && st.create_t < 0) ? st.first_t : kNever;
-          }
-        }
-        const bool elig = (base + lane >= ptr) && (base + lane < tail) && l_live && (l_first > t);
-        const unsigned long long bal = __ballot(elig);
+        if (rbase < 0 || ptr >= rbase + 64) load_ring(ptr);
+        const uint64_t bal = __ballot(l_live && rbase + l_off >= ptr && l_first > t);
         if (bal) {
-          const int w = __ffsll(bal) - 1;
-          victim = __shfl(l_vid, w, 64);
-          ptr = base + w + 1;
+          const int w = __ffsll((unsigned long long)bal) - 1;
+          victim = __builtin_amdgcn_readlane(l_vid, w);
+          vfirst = __builtin_amdgcn_readlane(l_first, w);
+          vseg = __builtin_amdgcn_readlane(l_seg, w);
+          vcur = __builtin_amdgcn_readlane(int(l_cur), w) != 0;
+          ptr = rbase + __builtin_amdgcn_readlane(l_off, w) + 1;
           if (lane == w) l_live = false;
-        } else {
-          ptr = base + 64;
+          break;
         }
+        ptr = min<int64_t>(rbase + 64, tail);
       }
       if (victim < 0) break;
-      VoxState& vst = m.vs[victim];
-      const int32_t vfirst = (vst.tag == tag && vst.seg >= 0) ? vst.first_t : kNever;
-      __syncthreads();
       if (lane == 0) {
-        if (vst.tag != tag) {
-          vst.tag = tag;
-          vst.create_t = -1;
-          vst.freeze_t = -1;
-          vst.prev_vid = -1;
-          vst.first_t = kNever;
-          vst.seg = -1;
-          vst.merge_pos = kNever;
-          vst.dirty_t = -2;
-          vst.taint_t = kNever;
-        }
-        vst.evict_t = t;
+        evict_write(victim, vcur, t);
         b.ev_vid[off + nev] = victim;
-        b.ev_t[off + nev] = t;
       }
       ++nev;
       --size;
@@ -626,7 +841,7 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
           err |= 16;
           break;
         }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
         if (lane == 0) {
           int i = hn;
           const uint64_t v = (uint64_t(uint32_t(vfirst)) << 32) | uint32_t(victim);
@@ -634,11 +849,13 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
             const int p = (i - 1) >> 1;
             if (heap[p] <= v) break;
             heap[i] = heap[p];
+            heap_seg[i] = heap_seg[p];
             i = p;
           }
           heap[i] = v;
+          heap_seg[i] = vseg;
         }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
         ++hn;
       }
     }
@@ -660,26 +877,29 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
 __global__ void k_create_init(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
   VMP_GRID_LOOP(p, b.ntot) {
     const int s = sess_of(si, nsess, p);
-    const MapDev& m = maps[s];
+    const MapDev m = maps[s];
     const int r = p - si[s].off;
-    if (r >= m.ctr->ncre) continue;
-    const int32_t vid = b.co_vid[p];
-    const int q = b.co_seg[p];
+    const int ncre = m.ctr->ncre;
+    if (r >= ncre) continue;
+    const int4 co = b.co[p];
+    const int32_t vid = co.x;
+    const int q = co.y;
     if (vid < 0 || vid >= m.pool || q < 0) continue;  // error already flagged
     const int32_t e = b.seg_entry[q];
     VoxState st;
     st.key = m.table[e].key;
-    st.group_id = b.co_group[p];
+    st.group_id = m.ctr->group_counter - uint64_t(ncre) + uint64_t(r);
     st.stamp = 0;
     st.n = 0;
     st.newly_add = 0;
     st.flags = kEnable | kAlive;
+    st.pad = 0;
     st.tag = m.ctr->tag;
-    st.create_t = b.co_t[p];
+    st.create_t = co.z;
     st.evict_t = kNever;
     st.freeze_t = -1;
-    st.prev_vid = b.co_prev[p];
-    st.first_t = b.co_t[p];
+    st.prev_vid = co.w;
+    st.first_t = co.z;
     st.seg = q;
     st.merge_pos = kNever;
     st.dirty_t = -2;
@@ -854,7 +1074,7 @@ __global__ void __launch_bounds__(kMapThreads) k_seg_lenkeys(const MapDev* maps,
       int heavy = 0;
       if (vid >= 0) {
         const int s = b.seg_sess[q];
-        const MapDev& m = maps[s];
+        const MapDev m = maps[s];
         const VoxState& st = m.vs[vid];
         heavy = walk_refits(st.n, st.newly_add, st.flags, len, m.upd_pt, m.max_pt, si[s].is_build != 0) ? 1 : 0;
       }
@@ -1015,7 +1235,7 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_light(const MapDev* maps
     int calls = 0, dropped = 0;
     const int q = valid ? b.ord[nh + qi] : 0;
     const int s = valid ? b.seg_sess[q] : 0;
-    const MapDev& m = maps[s];
+    const MapDev m = maps[s];
     const int32_t vid = valid ? b.seg_vid[q] : -1;
     const bool frozen0 = vid >= 0 && !si[s].is_build && (m.vs[vid].flags & kInit) && !(m.vs[vid].flags & kEnable);
     if (frozen0) {
@@ -1123,7 +1343,7 @@ __global__ void k_finalize_segs(const MapDev* maps, const SessInfo* si, BatchBuf
     const int32_t vid = b.seg_vid[q];
     if (vid < 0) continue;
     const int s = b.seg_sess[q];
-    const MapDev& m = maps[s];
+    const MapDev m = maps[s];
     VoxState& st = m.vs[vid];
     const int last_t = b.sval_out[b.seg_end[q] - 1] - si[s].off;
     const bool calls = st.merge_pos != kNever;
@@ -1140,7 +1360,7 @@ __global__ void k_finalize_segs(const MapDev* maps, const SessInfo* si, BatchBuf
 __global__ void k_evict_finalize(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
   VMP_GRID_LOOP(p, b.ntot) {
     const int s = sess_of(si, nsess, p);
-    const MapDev& m = maps[s];
+    const MapDev m = maps[s];
     const MapCounters* c = m.ctr;
     const int r = p - si[s].off;
     if (r >= c->nevict) continue;
@@ -1169,7 +1389,7 @@ __global__ void k_lru_append(const MapDev* maps, const SessInfo* si, int nsess, 
   VMP_GRID_LOOP(r, nsel) {
     const int g = b.sel[r];
     const int s = sess_of(si, nsess, g);
-    const MapDev& m = maps[s];
+    const MapDev m = maps[s];
     const int t = g - si[s].off;
     const int32_t vid = b.flags[g] - 1;
     const int64_t ph = (m.ctr->ring_tail + (r - si[s].lru_lo)) % m.qcap;

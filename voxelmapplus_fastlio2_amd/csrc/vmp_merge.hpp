@@ -179,7 +179,7 @@ __global__ void k_candidates(const MapDev* maps, const SessInfo* si, BatchBufs b
     if (ne > 0) {
       const int off = si[s].off, ns_pts = si[s].n;
       for (int k = 0; k < ne; ++k) {
-        if (slot + k < ns_pts) b.cand_in[off + slot + k] = cand_key(s, emitted[k], vid);
+        if (slot + k < ns_pts) b.cand_in[off + slot + k] = cand_key_batch(s, emitted[k], vid, b.cand_sshift);
         else atomicOr(&c->error, 16);
       }
       // candidate times, ascending (insertion sort of <= 7)
@@ -203,7 +203,7 @@ __global__ void k_dirty_untouched(const MapDev* maps, BatchBufs b) {
   const int nseg = b.cnt[0];
   VMP_GRID_LOOP(q, nseg) {
     if (b.seg_vid[q] < 0) continue;
-    const MapDev& m = maps[b.seg_sess[q]];
+    const MapDev m = maps[b.seg_sess[q]];
     const int tag = m.ctr->tag;
     const VoxState& st = m.vs[b.seg_vid[q]];
     if (st.freeze_t < 0 || !(st.flags & kPlane)) continue;
@@ -221,10 +221,10 @@ __global__ void k_spec_eval(const MapDev* maps, BatchBufs b) {
   const int ntot = b.cnt[3];
   VMP_GRID_LOOP(p, ntot) {
     const uint64_t cur = b.cand_out[p];
-    const int s = int(cur >> 49);
-    const MapDev& m = maps[s];
+    const int s = int(cur >> b.cand_sshift);
+    const MapDev m = maps[s];
     const int tag = m.ctr->tag;
-    const int t = cand_t(cur);
+    const int t = cand_t(cand_local(cur, b.cand_sshift));
     const int32_t V = cand_vid(cur);
     const VoxState& vst = m.vs[V];
     const uint64_t gV = vst.group_id;
@@ -248,7 +248,7 @@ __global__ void k_spec_eval(const MapDev* maps, BatchBufs b) {
 // after the candidate sort), so the replay loads a window's keys in one step
 __global__ void k_spl_keys(BatchBufs b) {
   const int n = b.cnt[4];
-  VMP_GRID_LOOP(k, n) b.cand_in[k] = b.cand_out[b.spl[k]] & kCandLocal;
+  VMP_GRID_LOOP(k, n) b.cand_in[k] = cand_local(b.cand_out[b.spl[k]], b.cand_sshift);
 }
 
 __global__ void k_spl_ranges(const MapDev* maps, SessInfo* si, int nsess, BatchBufs b) {
@@ -278,10 +278,10 @@ __global__ void k_spl_prefetch(const MapDev* maps, const SessInfo* si, BatchBufs
     const int k = it / kPreItems, r = it % kPreItems, cj = r / 7, j = r % 7;
     const int pos = b.spl[k];
     const uint64_t cur = b.cand_out[pos];
-    const int s = int(cur >> 49);
-    const MapDev& m = maps[s];
+    const int s = int(cur >> b.cand_sshift);
+    const MapDev m = maps[s];
     const int tag = m.ctr->tag, off = si[s].off;
-    const int t = cand_t(cur);
+    const int t = cand_t(cand_local(cur, b.cand_sshift));
     const int32_t base = cj == 6 ? cand_vid(cur) : b.cand_nbr[size_t(pos) * 6 + cj];
     int32_t X = -1;
     if (base >= 0) {
@@ -358,7 +358,7 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
                                                              BatchBufs b) {
   const int s = blockIdx.x;
   if (s >= nsess) return;
-  const MapDev& m = maps[s];
+  const MapDev m = maps[s];
   MapCounters* c = m.ctr;
   const int tag = c->tag, off = si[s].off;
   const int spl_hi = si[s].spl_hi;
