@@ -658,10 +658,32 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
     }
   };
 
+  int th = kNever;  // time of the earliest pending re-creation (heap top)
+  // a re-creation at first touch f of the evicted voxel v (segment sg)
+  auto heap_push = [&](int32_t f, int32_t v, int32_t sg) {
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+      int i = hn;
+      const uint64_t key = (uint64_t(uint32_t(f)) << 32) | uint32_t(v);
+      while (i > 0) {
+        const int p = (i - 1) >> 1;
+        if (heap[p] <= key) break;
+        heap[i] = heap[p];
+        heap_seg[i] = heap_seg[p];
+        i = p;
+      }
+      heap[i] = key;
+      heap_seg[i] = sg;
+    }
+    __builtin_amdgcn_wave_barrier();
+    ++hn;
+    th = min(th, f);
+  };
+
   while (true) {
     if (ci < ncre_ev && (cbase < 0 || ci >= cbase + 64)) load_cre(ci);
     const int t_s = ci < ncre_ev ? __builtin_amdgcn_readlane(c_t, ci - cbase) : kNever;
-    const int t_h = hn > 0 ? int(heap[0] >> 32) : kNever;
+    const int t_h = th;
     if (t_s == kNever && t_h == kNever) break;
     if (t_s < t_h) {
       // ---- bulk: creations that need no victim (below capacity, where no
@@ -692,21 +714,37 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
         size += mm;
         continue;
       }
-      // ---- bulk: creations before the next re-creation, victims from the
-      //      untouched live ring entries ahead of the first touched one
+      // ---- bulk: creations before the next re-creation take their victims
+      //      from the live ring entries ahead.  An entry first touched before
+      //      t_s is passed over by all of them (creation times only grow);
+      //      an untouched one is the victim of whichever creation reaches
+      //      it; the first entry touched after t_s ("hard") ends the step,
+      //      as the victim of the next creation if that comes before the
+      //      entry's first touch (its key then returns at that touch).
       if (size >= cap && ptr < tail) {
         if (rbase < 0 || ptr >= rbase + 64) load_ring(ptr);
         const bool ahead = l_live && rbase + l_off >= ptr;
-        const uint64_t touched = __ballot(ahead && l_first != kNever);
-        const int tpos = touched ? __ffsll((unsigned long long)touched) - 1 : 64;
+        const uint64_t hard = __ballot(ahead && l_first != kNever && l_first > t_s);
+        const int tpos = hard ? __ffsll((unsigned long long)hard) - 1 : 64;
         const uint64_t usable = __ballot(ahead && l_first == kNever) & (tpos >= 64 ? ~0ull : ((1ull << tpos) - 1));
         const int d = ci - cbase;  // creation ci sits in lane d of its window
         const int ncr = __popcll(__ballot(lane >= d && c_t < t_h));
-        const int mm = min(__popcll(usable), ncr);
+        const int nu = __popcll(usable);
+        int mm = min(nu, ncr);
+        int32_t hf = kNever, hv = -1, hs = -1;
+        if (mm == nu && nu < ncr && tpos < 64) {
+          hf = __builtin_amdgcn_readlane(l_first, tpos);
+          if (__builtin_amdgcn_readlane(c_t, d + nu) < hf) {
+            hv = __builtin_amdgcn_readlane(l_vid, tpos);
+            hs = __builtin_amdgcn_readlane(l_seg, tpos);
+            ++mm;
+          }
+        }
+        const uint64_t vmask = usable | (hv >= 0 ? (1ull << tpos) : 0ull);
         if (mm > 0) {
-          // the r-th usable entry is the victim of creation ci + r
-          const bool mine = ((usable >> lane) & 1ull) != 0;
-          const int r = __popcll(usable & below);
+          // the r-th entry of vmask is the victim of creation ci + r
+          const bool mine = ((vmask >> lane) & 1ull) != 0;
+          const int r = __popcll(vmask & below);
           const bool act = mine && r < mm;
           const int src = d + (act ? r : 0);
           const int t = __shfl(c_t, src, 64);
@@ -727,9 +765,16 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
             err |= 8;
             break;
           }
-          uint64_t rem = usable;  // the ring position after the mm-th victim
+          uint64_t rem = vmask;  // the ring position after the mm-th victim
           for (int k = 1; k < mm; ++k) rem &= rem - 1;
           ptr = rbase + __builtin_amdgcn_readlane(l_off, __ffsll((unsigned long long)rem) - 1) + 1;
+          if (hv >= 0) {
+            if (hn >= kHeapCap) {
+              err |= 16;
+              break;
+            }
+            heap_push(hf, hv, hs);
+          }
           if (mm <= ftop) ftop -= mm;
           else {
             bump += mm - ftop;
@@ -741,11 +786,9 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
           gc += uint64_t(mm);
           continue;
         }
-        if (touched == 0 && usable == 0) {  // no live entry left in the window
-          ptr = min<int64_t>(rbase + 64, tail);
-          continue;
-        }
-        // a voxel touched in this scan comes first: the single step decides
+        // no victim left in the window: the rest was touched before t_s
+        ptr = min<int64_t>(rbase + 64, tail);
+        continue;
       }
     }
     // ---- single step: the earliest event, then its victim if over capacity
@@ -790,6 +833,7 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
       }
       __builtin_amdgcn_wave_barrier();
       --hn;
+      th = hn > 0 ? int(heap[0] >> 32) : kNever;
     }
     int32_t vid;
     if (ftop > 0) {
@@ -841,22 +885,7 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
           err |= 16;
           break;
         }
-        __builtin_amdgcn_wave_barrier();
-        if (lane == 0) {
-          int i = hn;
-          const uint64_t v = (uint64_t(uint32_t(vfirst)) << 32) | uint32_t(victim);
-          while (i > 0) {
-            const int p = (i - 1) >> 1;
-            if (heap[p] <= v) break;
-            heap[i] = heap[p];
-            heap_seg[i] = heap_seg[p];
-            i = p;
-          }
-          heap[i] = v;
-          heap_seg[i] = vseg;
-        }
-        __builtin_amdgcn_wave_barrier();
-        ++hn;
+        heap_push(vfirst, victim, vseg);
       }
     }
   }
