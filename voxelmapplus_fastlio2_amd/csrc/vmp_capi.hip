@@ -305,6 +305,9 @@ static int ws_reserve(vmp_ctx* c, int64_t points, int nsess, bool exact = false)
   ok &= dalloc(A, &b.ord_in, n) == hipSuccess;
   ok &= dalloc(A, &b.ord, n) == hipSuccess;
   ok &= dalloc(A, &b.lv, n * 2) == hipSuccess;
+  ok &= dalloc(A, &b.lv2, n * 2) == hipSuccess;
+  ok &= dalloc(A, &b.lv_cnt, (n * 2 >> 6) + size_t(ns) + 2) == hipSuccess;
+  ok &= dalloc(A, &b.lv_n, size_t(ns)) == hipSuccess;
   // taint-phase prefetch of up to 256k acting candidates per batch (the rest
   // take the replay's own lookups)
   b.pre_cap = int32_t(std::min<size_t>(n, size_t(1) << 18));
@@ -824,6 +827,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
       hipLaunchKernelGGL(k_create_fast, dim3(gn), dim3(T), 0, st, M, S, n, b);
       hipLaunchKernelGGL(k_create_fast_ctr, dim3(n), dim3(1), 0, st, M, n);
       hipLaunchKernelGGL(k_ring_live, dim3(kLvBlocks, n), dim3(64 * kLvWaves), 0, st, M, S, n, b);
+      hipLaunchKernelGGL(k_ring_pack, dim3(n), dim3(kLpThreads), 0, st, M, S, n, b);
       hipLaunchKernelGGL(k_create_evict, dim3(n), dim3(64), 0, st, M, S, n, b);
       hipLaunchKernelGGL(k_create_init, dim3(gn), dim3(T), 0, st, M, S, n, b);
       // replay order by segment length, capped at 10 bits (two radix passes;

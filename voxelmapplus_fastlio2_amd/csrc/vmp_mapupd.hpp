@@ -138,12 +138,17 @@ struct BatchBufs {
   int32_t* pre_tn;
   int32_t pre_cap;
   int32_t pad1;
-  // live LRU ring entries from the head of each evicting session, compacted
-  // within 64-entry chunks: (vid, first touch in this scan or kNever,
-  // segment, offset in the chunk), vid -1 after the chunk's last live entry;
-  // chunk k of a session at 2 * off + 64 k (two entries per point)
+  // live LRU ring entries from the head of each evicting session: (vid,
+  // first touch in this scan or kNever, segment, ring position - head).
+  // lv holds them per 64-entry ring chunk (chunk k of a session at
+  // 2 * off + 64 k, two entries per point; lv_cnt: its count, at
+  // lv_cnt_base()), lv2 all of them in ring order at 2 * off (lv_n: count)
   int4* lv;
+  int4* lv2;
+  int32_t* lv_cnt;
+  int32_t* lv_n;
 };
+__host__ __device__ inline int64_t lv_cnt_base(int off, int s) { return (int64_t(2) * off >> 6) + s; }
 
 // merge-candidate key: session (15 bits) | point time (24 bits) | voxel (25 bits)
 __host__ __device__ inline uint64_t cand_key(int s, int t, int32_t vid) {
@@ -494,7 +499,7 @@ constexpr int kLvWaves = 4;   // waves per block of k_ring_live
 constexpr int kLvBlocks = 32; // blocks per session
 
 // Liveness of the ring entries ahead of the head (grid: kLvBlocks x
-// sessions), one wave per 64-entry chunk.  An entry is live while its voxel
+// sessions), one wave per 64-entry chunk, compacted within the chunk.  An entry is live while its voxel
 // is alive, still carries the entry's stamp and has not been evicted in this
 // scan; nothing the ordered pass changes makes another entry live.
 __global__ void __launch_bounds__(64 * kLvWaves) k_ring_live(const MapDev* maps, const SessInfo* si, int nsess,
@@ -527,10 +532,47 @@ __global__ void __launch_bounds__(64 * kLvWaves) k_ring_live(const MapDev* maps,
       seg = st.seg;
     }
     const uint64_t bal = __ballot(live);
-    int4* out = out0 + k * 64;
-    if (live) out[__popcll(bal & below)] = make_int4(v, first, seg, lane);
-    if (lane >= __popcll(bal)) out[lane] = make_int4(-1, kNever, -1, 64);
+    if (live) out0[k * 64 + __popcll(bal & below)] = make_int4(v, first, seg, int32_t(k * 64 + lane));
+    if (lane == 0) b.lv_cnt[lv_cnt_base(S.off, s) + k] = __popcll(bal);
   }
+}
+
+constexpr int kLpThreads = 1024;  // k_ring_pack block
+
+// The chunks' live entries packed in ring order (one block per session):
+// counts scanned per group of 1024 chunks, then one wave copies a chunk.
+__global__ void __launch_bounds__(kLpThreads) k_ring_pack(const MapDev* maps, const SessInfo* si, int nsess,
+                                                          BatchBufs b) {
+  const int s = blockIdx.x;
+  if (s >= nsess) return;
+  const MapCounters* c = maps[s].ctr;
+  if (c->fast) return;
+  const SessInfo S = si[s];
+  const int64_t nch = lv_chunks(c, S);
+  const int4* src = b.lv + int64_t(2) * S.off;
+  int4* dst = b.lv2 + int64_t(2) * S.off;
+  const int32_t* cnt = b.lv_cnt + lv_cnt_base(S.off, s);
+  __shared__ int32_t sh[kLpThreads];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int32_t base = 0;
+  for (int64_t g0 = 0; g0 < nch; g0 += kLpThreads) {
+    const int32_t cj = g0 + tid < nch ? cnt[g0 + tid] : 0;
+    sh[tid] = cj;
+    __syncthreads();
+    for (int d = 1; d < kLpThreads; d <<= 1) {  // inclusive scan
+      const int32_t v = tid >= d ? sh[tid - d] : 0;
+      __syncthreads();
+      sh[tid] += v;
+      __syncthreads();
+    }
+    for (int q = wave; q < kLpThreads && g0 + q < nch; q += kLpThreads / 64) {
+      const int32_t hi = sh[q], lo = q > 0 ? sh[q - 1] : 0;
+      if (lane < hi - lo) dst[base + lo + lane] = src[(g0 + q) * 64 + lane];
+    }
+    base += sh[kLpThreads - 1];
+    __syncthreads();
+  }
+  if (tid == 0) b.lv_n[s] = base;
 }
 
 // Ordered creation + eviction, one wave per session that may overflow.  LRU
@@ -566,16 +608,21 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
   int ftop = c->free_top, bump = c->bump;
   int64_t ptr = c->ring_head;
   const int64_t tail = c->ring_tail;
-  // ring window [rbase, rbase + 64): per lane an entry's vid, liveness, first
-  // touch in this scan (kNever if untouched), segment, offset in the window,
-  // and whether the voxel state carries this tag.  The first npre chunks from
-  // the head come compacted from k_ring_live (the next one prefetched).
+  // ring window: per lane an entry's vid, liveness, first touch in this scan
+  // (kNever if untouched), segment, ring position - head0, and whether the
+  // voxel state carries this tag; wend is the ring position after the
+  // window.  The live entries of the first npre chunks come packed, 64 per
+  // window (the next window prefetched); the ring is read directly after them.
   const int64_t head0 = ptr;
   const int64_t npre = lv_chunks(c, si[s]);
-  const int4* lvs = b.lv + int64_t(2) * off;
-  int64_t rbase = -1, nk = -1;
-  int4 n_e = make_int4(-1, kNever, -1, 64);
-  int32_t l_vid = -1, l_first = kNever, l_seg = -1, l_off = 64;
+  const int nlive = b.lv_n[s];
+  const int nwin = (nlive + 63) >> 6;
+  const int4* lvs = b.lv2 + int64_t(2) * off;
+  const int4 none = make_int4(-1, kNever, -1, 0);
+  int64_t wend = head0;
+  int lw = 0, nlw = -1;  // next packed window, the prefetched one
+  int4 n_e = none;
+  int32_t l_vid = -1, l_first = kNever, l_seg = -1, l_rel = 0;
   bool l_live = false, l_cur = false;
   // creation window [cbase, cbase + 64); free-list window [fbase, fbase + 128)
   // in two registers (the list is only popped here, its entries do not change)
@@ -585,31 +632,34 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
   int err = 0;
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 
-  auto load_ring = [&](int64_t at) {
-    const int64_t k = (at - head0) >> 6;
-    if (k < npre) {
-      rbase = head0 + 64 * k;
-      const int4 e = nk == k ? n_e : lvs[k * 64 + lane];
-      if (k + 1 < npre) {
-        n_e = lvs[(k + 1) * 64 + lane];
-        nk = k + 1;
+  auto load_ring = [&]() {  // the next window, from ptr on
+    if (lw < nwin) {
+      const int i = 64 * lw + lane;
+      const int4 e = nlw == lw ? n_e : (i < nlive ? lvs[i] : none);
+      if (lw + 1 < nwin) {
+        n_e = i + 64 < nlive ? lvs[i + 64] : none;
+        nlw = lw + 1;
       }
       l_vid = e.x;
       l_first = e.y;
       l_seg = e.z;
-      l_off = e.w;
+      l_rel = e.w;
       l_live = e.x >= 0;
       l_cur = e.y != kNever;  // a live voxel carries this tag iff touched in this scan
+      wend = head0 + __builtin_amdgcn_readlane(e.w, min(63, nlive - 64 * lw - 1)) + 1;
+      ++lw;
       return;
     }
-    rbase = at;
-    const int64_t idx = at + lane;
+    const int64_t rbase = max(ptr, head0 + 64 * npre);
+    ptr = rbase;
+    wend = min<int64_t>(rbase + 64, tail);
+    const int64_t idx = rbase + lane;
     l_vid = -1;
     l_live = false;
     l_cur = false;
     l_first = kNever;
     l_seg = -1;
-    l_off = lane;
+    l_rel = int32_t(idx - head0);
     if (idx < tail) {
       const int64_t ph = idx % m.qcap;
       const int32_t v = m.ring_vid[ph];
@@ -722,8 +772,8 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
       //      as the victim of the next creation if that comes before the
       //      entry's first touch (its key then returns at that touch).
       if (size >= cap && ptr < tail) {
-        if (rbase < 0 || ptr >= rbase + 64) load_ring(ptr);
-        const bool ahead = l_live && rbase + l_off >= ptr;
+        if (ptr >= wend) load_ring();
+        const bool ahead = l_live && head0 + l_rel >= ptr;
         const uint64_t hard = __ballot(ahead && l_first != kNever && l_first > t_s);
         const int tpos = hard ? __ffsll((unsigned long long)hard) - 1 : 64;
         const uint64_t usable = __ballot(ahead && l_first == kNever) & (tpos >= 64 ? ~0ull : ((1ull << tpos) - 1));
@@ -767,7 +817,7 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
           }
           uint64_t rem = vmask;  // the ring position after the mm-th victim
           for (int k = 1; k < mm; ++k) rem &= rem - 1;
-          ptr = rbase + __builtin_amdgcn_readlane(l_off, __ffsll((unsigned long long)rem) - 1) + 1;
+          ptr = head0 + __builtin_amdgcn_readlane(l_rel, __ffsll((unsigned long long)rem) - 1) + 1;
           if (hv >= 0) {
             if (hn >= kHeapCap) {
               err |= 16;
@@ -787,7 +837,7 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
           continue;
         }
         // no victim left in the window: the rest was touched before t_s
-        ptr = min<int64_t>(rbase + 64, tail);
+        ptr = wend;
         continue;
       }
     }
@@ -859,19 +909,19 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
           err |= 4;
           break;
         }
-        if (rbase < 0 || ptr >= rbase + 64) load_ring(ptr);
-        const uint64_t bal = __ballot(l_live && rbase + l_off >= ptr && l_first > t);
+        if (ptr >= wend) load_ring();
+        const uint64_t bal = __ballot(l_live && head0 + l_rel >= ptr && l_first > t);
         if (bal) {
           const int w = __ffsll((unsigned long long)bal) - 1;
           victim = __builtin_amdgcn_readlane(l_vid, w);
           vfirst = __builtin_amdgcn_readlane(l_first, w);
           vseg = __builtin_amdgcn_readlane(l_seg, w);
           vcur = __builtin_amdgcn_readlane(int(l_cur), w) != 0;
-          ptr = rbase + __builtin_amdgcn_readlane(l_off, w) + 1;
+          ptr = head0 + __builtin_amdgcn_readlane(l_rel, w) + 1;
           if (lane == w) l_live = false;
           break;
         }
-        ptr = min<int64_t>(rbase + 64, tail);
+        ptr = wend;
       }
       if (victim < 0) break;
       if (lane == 0) {
