@@ -828,7 +828,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
       hipLaunchKernelGGL(k_create_fast_ctr, dim3(n), dim3(1), 0, st, M, n);
       hipLaunchKernelGGL(k_ring_live, dim3(kLvBlocks, n), dim3(64 * kLvWaves), 0, st, M, S, n, b);
       hipLaunchKernelGGL(k_ring_pack, dim3(n), dim3(kLpThreads), 0, st, M, S, n, b);
-      hipLaunchKernelGGL(k_create_evict, dim3(n), dim3(64), 0, st, M, S, n, b);
+      hipLaunchKernelGGL(k_create_evict, dim3(n), dim3(128), 0, st, M, S, n, b);
       hipLaunchKernelGGL(k_create_init, dim3(gn), dim3(T), 0, st, M, S, n, b);
       // replay order by segment length, capped at 10 bits (two radix passes;
       // segments of 1022 points and more share the first key)

@@ -575,21 +575,22 @@ __global__ void __launch_bounds__(kLpThreads) k_ring_pack(const MapDev* maps, co
   if (tid == 0) b.lv_n[s] = base;
 }
 
-// Ordered creation + eviction, one wave per session that may overflow.  LRU
-// order is the descending order of last-touch stamps; the ring holds
+// Ordered creation + eviction, one workgroup per session that may overflow.
+// LRU order is the descending order of last-touch stamps; the ring holds
 // (vid, stamp) in ascending stamp order with stale entries skipped lazily.
 // The victim of each creation is the oldest live voxel not yet touched by an
 // earlier point of this scan; a victim touched later comes back as a new
 // voxel at its first touch (a creation event pushed on a min-heap).
 //
-// The wave holds 64-entry windows of the ring (with each entry's voxel
-// state), of the creation events and of the free list in its lanes, so a
-// step reads global memory only to refill a window.  While the next
-// creations precede every pending re-creation and the next live ring entries
-// are untouched in this scan, up to 64 creations take their victims in one
-// step (such victims are eligible whatever the creation's time and never come
-// back in this scan).
-__global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
+// Wave 0 decides.  It holds 64-entry windows of the live ring entries and of
+// the creation events, and 128 entries of the free list, in its lanes, and
+// takes them from LDS stream buffers that wave 1 fills from global memory
+// ahead of it, so the deciding wave waits on no global load while its own
+// stores drain.  While the next creations precede every pending
+// re-creation, up to 64 of them take their victims in one step.
+constexpr int kStreamSlots = 8;  // chunks of each stream buffered in LDS
+
+__global__ void __launch_bounds__(128) k_create_evict(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
   const int s = blockIdx.x;
   if (s >= nsess) return;
   const MapDev m = maps[s];
@@ -597,49 +598,113 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
   if (c->fast) return;
   const int clo = si[s].cre_lo, ncre_ev = si[s].cre_hi - clo;
   const int off = si[s].off;
+  const int64_t npre = lv_chunks(c, si[s]);
+  const int nlive = b.lv_n[s];
+  const int ftop0 = c->free_top;
+  // streams: 0 packed live ring entries, 1 creation events (time, segment),
+  // 2 the free list from its top down; 64 entries per chunk
+  const int nwin = (nlive + 63) >> 6, ncch = (ncre_ev + 63) >> 6, nfch = (ftop0 + 63) >> 6;
   __shared__ uint64_t heap[kHeapCap];     // (first touch << 32) | evicted vid
   __shared__ int32_t heap_seg[kHeapCap];  // the key's segment in this scan
-  const int lane = threadIdx.x;
+  __shared__ int4 s_live[kStreamSlots][64];
+  __shared__ int32_t s_cre[kStreamSlots][128];
+  __shared__ int32_t s_free[kStreamSlots][64];
+  __shared__ int32_t s_seq[3];  // chunks produced per stream
+  __shared__ int32_t s_use[3];  // first chunk per stream the decider still reads
+  __shared__ int32_t s_done;
+  const int lane = threadIdx.x & 63;
+  if (threadIdx.x < 3) {
+    s_seq[threadIdx.x] = 0;
+    s_use[threadIdx.x] = 0;
+  }
+  if (threadIdx.x == 0) s_done = 0;
+  __syncthreads();
+
+  if (threadIdx.x >= 64) {
+    // ---- stream producer: keeps each stream up to kStreamSlots chunks
+    //      ahead of the decider (skipping chunks it no longer needs) and
+    //      stops when the decider is done or every chunk is out
+    const int4* lvs = b.lv2 + int64_t(2) * off;
+    const __attribute__((address_space(1))) int32_t* fl = as_global(m.freelist);
+    int p0 = 0, p1 = 0, p2 = 0;
+    while (__hip_atomic_load(&s_done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) {
+      const int u0 = __hip_atomic_load(&s_use[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      const int u1 = __hip_atomic_load(&s_use[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      const int u2 = __hip_atomic_load(&s_use[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      p0 = max(p0, u0);
+      p1 = max(p1, u1);
+      p2 = max(p2, u2);
+      const bool g0 = p0 < nwin && p0 < u0 + kStreamSlots;
+      const bool g1 = p1 < ncch && p1 < u1 + kStreamSlots;
+      const bool g2 = p2 < nfch && p2 < u2 + kStreamSlots;
+      int4 e = make_int4(-1, kNever, -1, 0);
+      int32_t ct = kNever, cs = -1, fv = -1;
+      const int i0 = 64 * p0 + lane, i1 = 64 * p1 + lane, i2 = 64 * p2 + lane;
+      if (g0 && i0 < nlive) e = lvs[i0];
+      if (g1 && i1 < ncre_ev) {
+        ct = b.cre_t[clo + i1];
+        cs = b.cre_seg[clo + i1];
+      }
+      if (g2 && i2 < ftop0) fv = fl[ftop0 - 1 - i2];
+      if (g0) s_live[p0 % kStreamSlots][lane] = e;
+      if (g1) {
+        s_cre[p1 % kStreamSlots][lane] = ct;
+        s_cre[p1 % kStreamSlots][64 + lane] = cs;
+      }
+      if (g2) s_free[p2 % kStreamSlots][lane] = fv;
+      p0 += g0;
+      p1 += g1;
+      p2 += g2;
+      if (lane == 0) {
+        __hip_atomic_store(&s_seq[0], p0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_store(&s_seq[1], p1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_store(&s_seq[2], p2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      if (p0 >= nwin && p1 >= ncch && p2 >= nfch) break;
+      if (!(g0 || g1 || g2)) __builtin_amdgcn_s_sleep(2);
+    }
+    return;
+  }
+
+  // ---- the decider (wave 0)
   const int tag = c->tag;
   const int64_t cap = m.capacity;
   int hn = 0;
   int64_t size = c->size;
   uint64_t gc = c->group_counter;
-  int ftop = c->free_top, bump = c->bump;
+  int ftop = ftop0, bump = c->bump;
   int64_t ptr = c->ring_head;
   const int64_t tail = c->ring_tail;
+  const int64_t head0 = ptr;
   // ring window: per lane an entry's vid, liveness, first touch in this scan
   // (kNever if untouched), segment, ring position - head0, and whether the
   // voxel state carries this tag; wend is the ring position after the
-  // window.  The live entries of the first npre chunks come packed, 64 per
-  // window (the next window prefetched); the ring is read directly after them.
-  const int64_t head0 = ptr;
-  const int64_t npre = lv_chunks(c, si[s]);
-  const int nlive = b.lv_n[s];
-  const int nwin = (nlive + 63) >> 6;
-  const int4* lvs = b.lv2 + int64_t(2) * off;
-  const int4 none = make_int4(-1, kNever, -1, 0);
+  // window.  The live entries of the first npre ring chunks come packed, 64
+  // per window; the ring is read directly after them.
   int64_t wend = head0;
-  int lw = 0, nlw = -1;  // next packed window, the prefetched one
-  int4 n_e = none;
+  int lw = 0;  // next packed window
   int32_t l_vid = -1, l_first = kNever, l_seg = -1, l_rel = 0;
   bool l_live = false, l_cur = false;
-  // creation window [cbase, cbase + 64); free-list window [fbase, fbase + 128)
-  // in two registers (the list is only popped here, its entries do not change)
-  int cbase = -1, fbase = -1;
+  // creation chunk cq in lanes; free-list chunks fq, fq + 1 (allocation
+  // indices 64 fq ... 64 fq + 127, allocation a taking freelist[ftop0 - 1 - a])
+  int cq = -1, fq = -1;
   int32_t c_t = kNever, c_seg = -1, f_lo = -1, f_hi = -1;
   int ci = 0, nco = 0, nev = 0;
   int err = 0;
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 
+  auto await = [&](int st, int need) {  // until chunk need - 1 of stream st is out
+    while (__hip_atomic_load(&s_seq[st], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need)
+      __builtin_amdgcn_s_sleep(1);
+  };
+  auto release = [&](int st, int first) {
+    if (lane == 0) __hip_atomic_store(&s_use[st], first, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
   auto load_ring = [&]() {  // the next window, from ptr on
     if (lw < nwin) {
-      const int i = 64 * lw + lane;
-      const int4 e = nlw == lw ? n_e : (i < nlive ? lvs[i] : none);
-      if (lw + 1 < nwin) {
-        n_e = i + 64 < nlive ? lvs[i + 64] : none;
-        nlw = lw + 1;
-      }
+      release(0, lw);
+      await(0, lw + 1);
+      const int4 e = s_live[lw % kStreamSlots][lane];
       l_vid = e.x;
       l_first = e.y;
       l_seg = e.z;
@@ -662,39 +727,40 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
     l_rel = int32_t(idx - head0);
     if (idx < tail) {
       const int64_t ph = idx % m.qcap;
-      const int32_t v = m.ring_vid[ph];
-      const uint64_t stp = m.ring_stamp[ph];
-      const VoxState& st = m.vs[v];
+      const int32_t v = *as_global(m.ring_vid + ph);
+      const uint64_t stp = *as_global(m.ring_stamp + ph);
+      const auto* st = as_global(m.vs + v);
       l_vid = v;
-      l_cur = st.tag == tag;
-      l_live = (st.flags & kAlive) && st.stamp == stp && !(l_cur && st.evict_t != kNever);
-      l_first = (l_cur && st.create_t < 0) ? st.first_t : kNever;
-      l_seg = st.seg;
+      l_cur = st->tag == tag;
+      l_live = (st->flags & kAlive) && st->stamp == stp && !(l_cur && st->evict_t != kNever);
+      l_first = (l_cur && st->create_t < 0) ? st->first_t : kNever;
+      l_seg = st->seg;
     }
   };
-  auto load_cre = [&](int at) {
-    cbase = at;
-    c_t = kNever;
-    c_seg = -1;
-    if (at + lane < ncre_ev) {
-      c_t = b.cre_t[clo + at + lane];
-      c_seg = b.cre_seg[clo + at + lane];
-    }
+  auto load_cre = [&]() {  // the chunk of creation ci
+    cq = ci >> 6;
+    release(1, cq);
+    await(1, cq + 1);
+    c_t = s_cre[cq % kStreamSlots][lane];
+    c_seg = s_cre[cq % kStreamSlots][64 + lane];
   };
-  // make the window hold the next k free-list entries (k <= 64, k <= ftop)
+  // make the lanes hold the next k free-list entries (k <= 64, k <= ftop)
   auto load_free = [&](int k) {
-    if (fbase >= 0 && ftop - k >= fbase) return;
-    fbase = max(0, ftop - 128);
-    f_lo = fbase + lane < ftop ? m.freelist[fbase + lane] : -1;
-    f_hi = fbase + 64 + lane < ftop ? m.freelist[fbase + 64 + lane] : -1;
+    const int a = ftop0 - ftop;
+    if (fq >= 0 && a >= 64 * fq && a + k <= 64 * fq + 128) return;
+    fq = a >> 6;
+    release(2, fq);
+    await(2, min(fq + 2, nfch));
+    f_lo = s_free[fq % kStreamSlots][lane];
+    f_hi = fq + 1 < nfch ? s_free[(fq + 1) % kStreamSlots][lane] : -1;
   };
-  auto free_at = [&](int i) {  // entry i of the list (inside the window), per lane
-    const int q = i - fbase;
-    const int32_t a = __shfl(f_lo, q & 63, 64), h = __shfl(f_hi, q & 63, 64);
-    return q < 64 ? a : h;
+  auto free_at = [&](int a) {  // entry of allocation a (inside the window), per lane
+    const int q = a - 64 * fq;
+    const int32_t x = __shfl(f_lo, q & 63, 64), y = __shfl(f_hi, q & 63, 64);
+    return q < 64 ? x : y;
   };
-  auto free_at_u = [&](int i) {  // the same for an index uniform over the wave
-    const int q = i - fbase;
+  auto free_at_u = [&](int a) {  // the same for an allocation uniform over the wave
+    const int q = a - 64 * fq;
     return q < 64 ? __builtin_amdgcn_readlane(f_lo, q) : __builtin_amdgcn_readlane(f_hi, q - 64);
   };
   auto evict_write = [&](int32_t victim, bool cur, int t) {  // caller: one lane
@@ -731,8 +797,9 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
   };
 
   while (true) {
-    if (ci < ncre_ev && (cbase < 0 || ci >= cbase + 64)) load_cre(ci);
-    const int t_s = ci < ncre_ev ? __builtin_amdgcn_readlane(c_t, ci - cbase) : kNever;
+    if (ci < ncre_ev && (ci >> 6) != cq) load_cre();
+    const int d = ci & 63;  // creation ci sits in lane d of its chunk
+    const int t_s = ci < ncre_ev ? __builtin_amdgcn_readlane(c_t, d) : kNever;
     const int t_h = th;
     if (t_s == kNever && t_h == kNever) break;
     if (t_s < t_h) {
@@ -744,7 +811,7 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
           const int j = j0 + lane;
           if (j < mm) {
             const int fj = ftop - 1 - j;  // the same allocation order as one by one
-            const int32_t vid = fj >= 0 ? m.freelist[fj] : bump + (j - ftop);
+            const int32_t vid = fj >= 0 ? *as_global(m.freelist + fj) : bump + (j - ftop);
             if (vid >= m.pool) err |= 8;
             b.co[off + nco + j] = make_int4(vid, b.cre_seg[clo + ci + j], b.cre_t[clo + ci + j], -1);
           }
@@ -777,8 +844,7 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
         const uint64_t hard = __ballot(ahead && l_first != kNever && l_first > t_s);
         const int tpos = hard ? __ffsll((unsigned long long)hard) - 1 : 64;
         const uint64_t usable = __ballot(ahead && l_first == kNever) & (tpos >= 64 ? ~0ull : ((1ull << tpos) - 1));
-        const int d = ci - cbase;  // creation ci sits in lane d of its window
-        const int ncr = __popcll(__ballot(lane >= d && c_t < t_h));
+        const int ncr = __popcll(__ballot(lane >= d && c_t < t_h));  // creations of this chunk before t_h
         const int nu = __popcll(usable);
         int mm = min(nu, ncr);
         int32_t hf = kNever, hv = -1, hs = -1;
@@ -801,10 +867,10 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
           const int sg = __shfl(c_seg, src, 64);
           const int nf = min(mm, ftop);  // victims' slots from the free list, then the bump
           if (nf > 0) load_free(nf);
-          const int fj = ftop - 1 - (act ? r : 0);
-          const int32_t fv = nf > 0 ? free_at(max(fj, fbase)) : -1;
+          const int a0 = ftop0 - ftop;  // allocation index of creation ci
+          const int32_t fv = nf > 0 ? free_at(a0 + min(act ? r : 0, nf - 1)) : -1;
           if (act) {
-            const int32_t vid = fj >= 0 ? fv : bump + (r - ftop);
+            const int32_t vid = r < ftop ? fv : bump + (r - ftop);
             if (vid >= m.pool) err |= 8;
             b.co[off + nco + r] = make_int4(vid, sg, t, -1);
             evict_write(l_vid, l_cur, t);
@@ -845,7 +911,7 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
     int t, seg, prev;
     if (t_s < t_h) {
       t = t_s;
-      seg = __builtin_amdgcn_readlane(c_seg, ci - cbase);
+      seg = __builtin_amdgcn_readlane(c_seg, d);
       prev = -1;
       ++ci;
     } else {
@@ -888,7 +954,7 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
     int32_t vid;
     if (ftop > 0) {
       load_free(1);
-      vid = free_at_u(ftop - 1);
+      vid = free_at_u(ftop0 - ftop);
       --ftop;
     } else {
       vid = bump++;
@@ -939,7 +1005,6 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
       }
     }
   }
-  __syncthreads();
   if (lane == 0) {
     c->size = size;
     c->group_counter = gc;
@@ -949,6 +1014,7 @@ __global__ void __launch_bounds__(64) k_create_evict(const MapDev* maps, const S
     c->ncre = nco;
     c->nevict = nev;
     if (err) atomicOr(&c->error, err);
+    __hip_atomic_store(&s_done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
 }
 
