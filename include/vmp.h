@@ -184,6 +184,10 @@ int vmp_ctx_reserve(vmp_ctx* ctx, int64_t points, int32_t sessions);
  *              frame anchored at the scan's pose voxel (the default),
  *              1 = plain fp32 world frame, a precision-study arm whose
  *              fallback counter counts keys that differ from the fp64 key.
+ *   "obs_time_reps" 1..64: while per-launch timing is on, each observe
+ *              launch runs this many times back to back between one event
+ *              pair (the kernel is idempotent), then once more for the
+ *              pass's counters; the timing reports the mean (default 1).
  * Applies to later calls. */
 int vmp_ctx_set_param(vmp_ctx* ctx, const char* name, int64_t value);
 

@@ -100,6 +100,46 @@ def test_filter_graph_matches_eager():
     ctx_e.close()
 
 
+def test_repeated_observe_timing_leaves_results_unchanged():
+    """Per-launch timing with obs_time_reps > 1 runs every observe launch
+    several times back to back (the kernel is idempotent) and once more for
+    the pass's counters: poses, P, maps and observe statistics stay bitwise
+    those of an untimed context."""
+    nscans = 4
+    ctx_t = Context(0)
+    ctx_r = Context(0)
+    ctx_t.set_param("obs_time_reps", 5)
+    timed = [make(ctx_t, *sp) for sp in SPECS[:3]]
+    ref = [make(ctx_r, *sp) for sp in SPECS[:3]]
+    for _, g in timed:
+        g.set_timing(True)
+    for k in range(1, nscans):
+        stats = []
+        for group in (timed, ref):
+            mot = [seq.motion(k) for seq, _ in group]
+            apply_motion_batch([g for _, g in group], [a for a, _ in mot], [b for _, b in mot], QDIAG)
+            for seq, g in group:
+                g.set_points(*seq.scan(k))
+            stats.append(register_batch([g for _, g in group]))
+        (ot, mt), (orf, mrf) = stats
+        for i, ((_, ga), (_, gb)) in enumerate(zip(timed, ref)):
+            sa, Pa = ga.get_state()
+            sb, Pb = gb.get_state()
+            assert np.array_equal(sa, sb), f"session {i} scan {k}: state differs with repeated timing"
+            assert np.array_equal(Pa, Pb), f"session {i} scan {k}: P differs with repeated timing"
+            for f in ("points", "hits", "plane_hits", "effective", "iterations"):
+                assert getattr(ot[i], f) == getattr(orf[i], f), f"session {i} scan {k}: observe stat {f} differs"
+            ea, eb = ga.export_map(), gb.export_map()
+            for key in ("keys", "ints", "group", "dbl"):
+                assert np.array_equal(ea[key], eb[key]), f"session {i} scan {k}: map '{key}' differs"
+    tt = timed[0][1].timing_totals()
+    assert tt["obs_launches"] > 0 and tt["obs_ms"] > 0.0
+    for _, g in timed + ref:
+        g.close()
+    ctx_t.close()
+    ctx_r.close()
+
+
 def test_autotune_observe_leaves_results_unchanged():
     """vmp_autotune_observe picks points per thread for the scan's size
     bucket and restores the residual slots its timing passes wrote: the next

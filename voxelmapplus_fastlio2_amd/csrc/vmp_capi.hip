@@ -140,6 +140,9 @@ struct vmp_ctx {
   hipEvent_t ev[4]{};
   // optional per-launch timing of the observe kernel / map pipeline
   bool timing = false;
+  // observe launches back to back between one event pair (the kernel is
+  // idempotent: the repetitions rewrite the same slots and partial rows)
+  int obs_time_reps = 1;
   hipEvent_t tev[2 * 64]{};
   int tev_used = 0;
   double acc_obs_ms = 0.0, acc_map_ms = 0.0;
@@ -732,19 +735,27 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
   auto observe = [&](int grid) {
     hipLaunchKernelGGL(k_zero_lists, dim3(n), dim3(64), 0, st, w.d_obsctr, 8);
     const bool tm = c->timing && c->tev_used + 2 <= 2 * 64;
-    if (tm) hipEventRecord(c->tev[c->tev_used], st);
     const bool w32 = c->obs_arith == 1;
-    if (ext && w32)
-      hipLaunchKernelGGL((observe_kernel<12, true>), dim3(grid), dim3(kObsThreads), 0, st, w.d_jobs, n);
-    else if (ext)
-      hipLaunchKernelGGL(observe_kernel<12>, dim3(grid), dim3(kObsThreads), 0, st, w.d_jobs, n);
-    else if (w32)
-      hipLaunchKernelGGL((observe_kernel<6, true>), dim3(grid), dim3(kObsThreads), 0, st, w.d_jobs, n);
-    else
-      hipLaunchKernelGGL(observe_kernel<6>, dim3(grid), dim3(kObsThreads), 0, st, w.d_jobs, n);
+    auto launch = [&]() {
+      if (ext && w32)
+        hipLaunchKernelGGL((observe_kernel<12, true>), dim3(grid), dim3(kObsThreads), 0, st, w.d_jobs, n);
+      else if (ext)
+        hipLaunchKernelGGL(observe_kernel<12>, dim3(grid), dim3(kObsThreads), 0, st, w.d_jobs, n);
+      else if (w32)
+        hipLaunchKernelGGL((observe_kernel<6, true>), dim3(grid), dim3(kObsThreads), 0, st, w.d_jobs, n);
+      else
+        hipLaunchKernelGGL(observe_kernel<6>, dim3(grid), dim3(kObsThreads), 0, st, w.d_jobs, n);
+    };
+    if (tm) hipEventRecord(c->tev[c->tev_used], st);
+    const int reps = tm ? c->obs_time_reps : 1;
+    for (int r = 0; r < reps; ++r) launch();
     if (tm) {
       hipEventRecord(c->tev[c->tev_used + 1], st);
       c->tev_used += 2;
+      if (reps > 1) {  // the counters of this pass once more, from zero
+        hipLaunchKernelGGL(k_zero_lists, dim3(n), dim3(64), 0, st, w.d_obsctr, 8);
+        launch();
+      }
     }
     if (ext)
       hipLaunchKernelGGL(k_obs_reduce<12>, dim3(n), dim3(kObsThreads), 0, st, w.d_jobs);
@@ -953,7 +964,7 @@ static void collect_timing(vmp_ctx* c, int active_max) {
   for (int k = 0; k + 1 < c->tev_used && k / 2 < active_max; k += 2) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->tev[k], c->tev[k + 1]) == hipSuccess && ms > 0.f) {
-      c->acc_obs_ms += ms;
+      c->acc_obs_ms += ms / c->obs_time_reps;
       c->acc_obs_n += 1;
     }
   }
@@ -1192,9 +1203,10 @@ static int autotune_locked(vmp_ctx* c, vmp_session** ss, int n, int reps, int32_
     for (hipEvent_t& e : c->tev)
       if (hipEventCreate(&e) != hipSuccess) rc = VMP_EDEVICE;
   const bool keep_timing = c->timing;
-  const int keep_ppt = c->obs_ppt;
+  const int keep_ppt = c->obs_ppt, keep_reps = c->obs_time_reps;
   c->timing = true;
   c->obs_ppt = 0;
+  c->obs_time_reps = 1;
   const int cands[4] = {1, 2, 4, 8};
   for (int bk = 0; bk < 4 && rc == VMP_OK; ++bk) {
     bool present = false;
@@ -1228,6 +1240,7 @@ static int autotune_locked(vmp_ctx* c, vmp_session** ss, int n, int reps, int32_
   c->tev_used = 0;
   c->timing = keep_timing;
   c->obs_ppt = keep_ppt;
+  c->obs_time_reps = keep_reps;
   o = 0;
   for (int i = 0; i < n; ++i) {
     if (hipMemcpyAsync(ss[i]->slots, saved + o, sizeof(float4) * size_t(ss[i]->n), hipMemcpyDeviceToDevice, st) !=
@@ -1367,6 +1380,11 @@ extern "C" int vmp_ctx_set_param(vmp_ctx* c, const char* name, int64_t value) {
   if (std::strcmp(name, "obs_arith") == 0) {
     if (value < 0 || value > 1) return VMP_EINVAL;
     c->obs_arith = int(value);
+    return VMP_OK;
+  }
+  if (std::strcmp(name, "obs_time_reps") == 0) {
+    if (value < 1 || value > 64) return VMP_EINVAL;
+    c->obs_time_reps = int(value);
     return VMP_OK;
   }
   return VMP_EINVAL;
