@@ -68,9 +68,15 @@ __global__ void __launch_bounds__(64) k_ieskf_begin(FilterDev** fs) {
   __shared__ vmp_state sx;
   const int lane = threadIdx.x;
   load_state(f->x, sx, lane);
-  for (int i = lane; i < 529; i += 64) M[(i % 23) * 23 + i / 23] = f->P[i];  // column-major -> row-major
+  double a[23];  // row `lane` of P (column-major in memory), factored in lanes
+#pragma unroll
+  for (int c = 0; c < 23; ++c) a[c] = lane < 23 ? f->P[c * 23 + lane] : 0.0;
+  chol_rows(a, lane);
+  if (lane < 23) {
+#pragma unroll
+    for (int c = 0; c < 23; ++c) M[lane * 23 + c] = a[c];
+  }
   __syncthreads();
-  chol_factor(M, lane);
   chol_inverse(M, Inv, lane);
   for (int i = lane; i < 529; i += 64) f->Pinv[i] = Inv[i];
   load_state(sx, f->xp, lane);
@@ -85,8 +91,8 @@ __global__ void __launch_bounds__(64) k_ieskf_begin(FilterDev** fs) {
 __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, int32_t** obs_ctr, int32_t** stat_out) {
   FilterDev* f = fs[blockIdx.x];
   if (!f->active) return;
-  __shared__ double Pi[529], H[529], JtPi[529];
-  __shared__ double dl[23], bb[23], xs[23];
+  __shared__ double Pi[529];
+  __shared__ double dl[23];
   __shared__ vmp_state sx, sxp;
   __shared__ JacBlocks jb;
   __shared__ double hobs[156];
@@ -98,33 +104,82 @@ __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, int32_t** obs
   __syncthreads();
   state_minus_par(sx, sxp, dl, lane);
   jac_blocks_par(sx, sxp, dl, jb, lane);
-  // JtPi = J^T P^-1 (J is the identity outside three small blocks)
-  for (int i = lane; i < 529; i += 64) {
-    const int r = i / 23, c = i % 23;
-    double s = jac_at(jb, 0, r) * Pi[c];
-    for (int k = 1; k < 23; ++k) s += jac_at(jb, k, r) * Pi[k * 23 + c];
-    JtPi[i] = s;
-  }
-  __syncthreads();
-  // H_ = JtPi J + [H_obs] ; b_ = JtPi delta + [b_obs]
-  for (int i = lane; i < 529; i += 64) {
-    const int r = i / 23, c = i % 23;
-    double s = JtPi[r * 23] * jac_at(jb, 0, c);
-    for (int k = 1; k < 23; ++k) s += JtPi[r * 23 + k] * jac_at(jb, k, c);
-    if (r < 12 && c < 12) s += hobs[c * 12 + r];
-    H[i] = s;
-  }
+  // Lane r (< 23) builds row r of JtPi = J^T P^-1 and of H_ = JtPi J + [H_obs]
+  // in registers (J is the identity outside three small blocks, so a block
+  // row or column adds up at most three products), then factors H_ and
+  // solves H_ delta = -b_ with the rows in lanes.
+  double a[23];
+  double br = 0.0;
+#pragma unroll
+  for (int c = 0; c < 23; ++c) a[c] = 0.0;
   if (lane < 23) {
-    double s = JtPi[lane * 23] * dl[0];
-    for (int k = 1; k < 23; ++k) s += JtPi[lane * 23 + k] * dl[k];
-    if (lane < 12) s += hobs[144 + lane];
-    bb[lane] = s;
+    const int r = lane;
+    double t[23];
+    // JtPi[r][c] = sum_k J[k][r] Pi[k][c]; a block row sums its nonzero terms
+    // in ascending k (the zero terms of the dense sum add nothing)
+    if (r >= 3 && r < 6) {
+#pragma unroll
+      for (int c = 0; c < 23; ++c) {
+        double s = jb.a[0 * 3 + (r - 3)] * Pi[3 * 23 + c];
+        s += jb.a[1 * 3 + (r - 3)] * Pi[4 * 23 + c];
+        s += jb.a[2 * 3 + (r - 3)] * Pi[5 * 23 + c];
+        t[c] = s;
+      }
+    } else if (r >= 6 && r < 9) {
+#pragma unroll
+      for (int c = 0; c < 23; ++c) {
+        double s = jb.b[0 * 3 + (r - 6)] * Pi[6 * 23 + c];
+        s += jb.b[1 * 3 + (r - 6)] * Pi[7 * 23 + c];
+        s += jb.b[2 * 3 + (r - 6)] * Pi[8 * 23 + c];
+        t[c] = s;
+      }
+    } else if (r >= 21) {
+#pragma unroll
+      for (int c = 0; c < 23; ++c) {
+        double s = jb.c[0 * 2 + (r - 21)] * Pi[21 * 23 + c];
+        s += jb.c[1 * 2 + (r - 21)] * Pi[22 * 23 + c];
+        t[c] = s;
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 23; ++c) t[c] = Pi[r * 23 + c];
+    }
+    // H_[r][c] = sum_k JtPi[r][k] J[k][c]
+#pragma unroll
+    for (int c = 0; c < 23; ++c) {
+      double s;
+      if (c >= 3 && c < 6) {
+        s = t[3] * jb.a[0 * 3 + (c - 3)];
+        s += t[4] * jb.a[1 * 3 + (c - 3)];
+        s += t[5] * jb.a[2 * 3 + (c - 3)];
+      } else if (c >= 6 && c < 9) {
+        s = t[6] * jb.b[0 * 3 + (c - 6)];
+        s += t[7] * jb.b[1 * 3 + (c - 6)];
+        s += t[8] * jb.b[2 * 3 + (c - 6)];
+      } else if (c >= 21) {
+        s = t[21] * jb.c[0 * 2 + (c - 21)];
+        s += t[22] * jb.c[1 * 2 + (c - 21)];
+      } else {
+        s = t[c];
+      }
+      if (r < 12 && c < 12) s += hobs[c * 12 + r];
+      a[c] = s;
+    }
+    // b_[r] = JtPi[r] . delta + [b_obs]
+    double s = t[0] * dl[0];
+#pragma unroll
+    for (int k = 1; k < 23; ++k) s += t[k] * dl[k];
+    if (r < 12) s += hobs[144 + r];
+    br = s;
   }
-  __syncthreads();
-  chol_factor(H, lane);
-  chol_solve_vec(H, bb, xs, lane);  // delta = -H_^-1 b_
-  if (lane < 23) dl[lane] = -xs[lane];
-  for (int i = lane; i < 529; i += 64) f->Hinv[i] = H[i];  // Cholesky factor of the last H_
+  chol_rows(a, lane);
+  const double xr = chol_solve_rows(a, br, lane);  // delta = -H_^-1 b_
+  if (lane < 23) {
+#pragma unroll
+    for (int c = 0; c < 23; ++c) f->Hinv[lane * 23 + c] = a[c];  // Cholesky factor of the last H_
+  }
+  __syncthreads();  // every lane has read dl
+  if (lane < 23) dl[lane] = -xr;
   __syncthreads();
   if (lane < 23) f->delta[lane] = dl[lane];
   state_plus_par(sx, dl, lane);
@@ -163,17 +218,46 @@ __global__ void __launch_bounds__(64) k_ieskf_end(FilterDev** fs) {
   __syncthreads();
   chol_inverse(Lc, Hi, lane);
   jac_blocks_par(sx, sxp, dl, jb, lane);
-  for (int i = lane; i < 529; i += 64) {
+  // P = L H_^-1 L^T with L = J; J is the identity outside three small
+  // blocks, so a block row or column adds up at most three products (in
+  // ascending k, as the dense sum would)
+  for (int i = lane; i < 529; i += 64) {  // T = J Hi
     const int r = i / 23, c = i % 23;
-    double s = jac_at(jb, r, 0) * Hi[c];
-    for (int k = 1; k < 23; ++k) s += jac_at(jb, r, k) * Hi[k * 23 + c];
+    double s;
+    if (r >= 3 && r < 6) {
+      s = jb.a[(r - 3) * 3 + 0] * Hi[3 * 23 + c];
+      s += jb.a[(r - 3) * 3 + 1] * Hi[4 * 23 + c];
+      s += jb.a[(r - 3) * 3 + 2] * Hi[5 * 23 + c];
+    } else if (r >= 6 && r < 9) {
+      s = jb.b[(r - 6) * 3 + 0] * Hi[6 * 23 + c];
+      s += jb.b[(r - 6) * 3 + 1] * Hi[7 * 23 + c];
+      s += jb.b[(r - 6) * 3 + 2] * Hi[8 * 23 + c];
+    } else if (r >= 21) {
+      s = jb.c[(r - 21) * 2 + 0] * Hi[21 * 23 + c];
+      s += jb.c[(r - 21) * 2 + 1] * Hi[22 * 23 + c];
+    } else {
+      s = Hi[r * 23 + c];
+    }
     T[i] = s;
   }
   __syncthreads();
-  for (int i = lane; i < 529; i += 64) {
+  for (int i = lane; i < 529; i += 64) {  // P = T J^T
     const int r = i / 23, c = i % 23;
-    double s = T[r * 23] * jac_at(jb, c, 0);
-    for (int k = 1; k < 23; ++k) s += T[r * 23 + k] * jac_at(jb, c, k);
+    double s;
+    if (c >= 3 && c < 6) {
+      s = T[r * 23 + 3] * jb.a[(c - 3) * 3 + 0];
+      s += T[r * 23 + 4] * jb.a[(c - 3) * 3 + 1];
+      s += T[r * 23 + 5] * jb.a[(c - 3) * 3 + 2];
+    } else if (c >= 6 && c < 9) {
+      s = T[r * 23 + 6] * jb.b[(c - 6) * 3 + 0];
+      s += T[r * 23 + 7] * jb.b[(c - 6) * 3 + 1];
+      s += T[r * 23 + 8] * jb.b[(c - 6) * 3 + 2];
+    } else if (c >= 21) {
+      s = T[r * 23 + 21] * jb.c[(c - 21) * 2 + 0];
+      s += T[r * 23 + 22] * jb.c[(c - 21) * 2 + 1];
+    } else {
+      s = T[r * 23 + c];
+    }
     f->P[c * 23 + r] = s;  // column-major
   }
   __syncthreads();

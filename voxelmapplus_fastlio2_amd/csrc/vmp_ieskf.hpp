@@ -265,22 +265,70 @@ __device__ inline void chol_solve_vec(const double* L, const double* b, double* 
 // Inv = A^-1 (row-major), one column per lane.
 __device__ inline void chol_inverse(const double* L, double* Inv, int lane) {
   constexpr int n = 23;
-  if (lane < n) {
+  if (lane < n) {  // column c = lane, solved in place in Inv's column c (LDS)
     const int c = lane;
-    double y[n];
-    for (int i = 0; i < n; ++i) {
+#pragma unroll 1
+    for (int i = 0; i < n; ++i) {  // L y = e_c
       double s = (i == c) ? 1.0 : 0.0;
-      for (int j = 0; j < i; ++j) s -= L[i * n + j] * y[j];
-      y[i] = s / L[i * n + i];
+#pragma unroll 4
+      for (int j = 0; j < i; ++j) s -= L[i * n + j] * Inv[j * n + c];
+      Inv[i * n + c] = s / L[i * n + i];
     }
-    for (int i = n - 1; i >= 0; --i) {
-      double s = y[i];
-      for (int j = i + 1; j < n; ++j) s -= L[j * n + i] * y[j];
-      y[i] = s / L[i * n + i];
+#pragma unroll 1
+    for (int i = n - 1; i >= 0; --i) {  // L^T x = y
+      double s = Inv[i * n + c];
+#pragma unroll 4
+      for (int j = i + 1; j < n; ++j) s -= L[j * n + i] * Inv[j * n + c];
+      Inv[i * n + c] = s / L[i * n + i];
     }
-    for (int i = 0; i < n; ++i) Inv[i * n + c] = y[i];
   }
   __syncthreads();
+}
+
+// ---- the same factorisation and solve on rows held in lanes ------------
+// Lane i (< 23) holds row i of a 23x23 matrix in a[0..22]; values cross lanes
+// by readlane (uniform index), so nothing goes through LDS and no barrier is
+// needed.  Each element sees the same operations in the same order as in
+// chol_factor / chol_solve_vec.
+__device__ inline double rl64(double v, int src) {
+  const long long x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(int(x), src);
+  const int hi = __builtin_amdgcn_readlane(int(x >> 32), src);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+// in place: row i becomes L[i][0..i], its upper part untouched
+__device__ inline void chol_rows(double (&a)[23], int lane) {
+#pragma unroll
+  for (int k = 0; k < 23; ++k) {
+    const double d = sqrt(rl64(a[k], k));
+    if (lane == k) a[k] = d;
+    else if (lane > k) a[k] = a[k] / d;
+#pragma unroll
+    for (int j = k + 1; j < 23; ++j) {
+      const double ljk = rl64(a[k], j);
+      if (lane >= j) a[j] -= a[k] * ljk;
+    }
+  }
+}
+// x = A^-1 b for A = L L^T from chol_rows; b and x one element per lane
+__device__ inline double chol_solve_rows(const double (&a)[23], double b, int lane) {
+  double y = b;
+#pragma unroll
+  for (int j = 0; j < 23; ++j) {  // L y = b
+    if (lane == j) y = y / a[j];
+    const double yj = rl64(y, j);
+    if (lane > j) y -= a[j] * yj;
+  }
+  double x = y;
+#pragma unroll
+  for (int i = 22; i >= 0; --i) {  // L^T x = y
+    double s = rl64(y, i);
+#pragma unroll
+    for (int j = i + 1; j < 23; ++j) s -= rl64(a[i], j) * rl64(x, j);
+    const double xi = s / rl64(a[i], i);
+    if (lane == i) x = xi;
+  }
+  return x;
 }
 
 // ---- lane-parallel pieces of the state algebra (inputs in LDS) ----------
