@@ -21,8 +21,9 @@ import time
 
 # The batched form runs four stream groups; HIP gives a process 4 hardware
 # queues by default, so streams beyond that would share queues and serialise.
-# Set before anything initialises HIP.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# Raised (never lowered) before anything initialises HIP.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 import numpy as np
 
