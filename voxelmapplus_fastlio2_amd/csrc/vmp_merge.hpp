@@ -599,60 +599,70 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
         }
         bool need = false;
         if (X >= 0) {
+          // what depends only on X (and, for a static entry, on its
+          // precomputed segment) is loaded in one round; nothing here is
+          // written before the commit
+          const VoxState* xv = m.vs + X;
+          const uint32_t xflags = xv->flags;
+          const uint64_t xkey = xv->key, xgrp = xv->group_id;
+          const int32_t xtag = xv->tag, xsg = xv->seg, xev = xv->evict_t, xmp = xv->merge_pos;
+          double px[kPlaneD];
+          for (int k = 0; k < kPlaneD; ++k) px[k] = m.plane[size_t(X) * kPlaneD + k];
+          int32_t nct = 0, ct[7];
+          if (pre_seg >= 0) {
+            nct = b.seg_nct[pre_seg];
+            for (int k = 0; k < 7; ++k) ct[k] = b.seg_ct[pre_seg * 7 + k];
+          }
           for (int k = 0; k < nc0; ++k) need |= (nbv[w][clos[w][k]] == X);
-          if (!need) {
-            const VoxState& xv = m.vs[X];
-            if ((xv.flags & kPlane) && (xv.flags & kInit)) {
-              double px[kPlaneD];
-              for (int k = 0; k < kPlaneD; ++k) px[k] = m.plane[size_t(X) * kPlaneD + k];
-              int64_t ax, ay, az;
-              unpack_key(xv.key, ax, ay, az);
-              for (int k = 0; k < nc0; ++k) {
-                const int cc = clos[w][k];
-                int64_t bx, by, bz;
-                unpack_key(rkey[w][cc], bx, by, bz);
-                const int64_t dist = (ax > bx ? ax - bx : bx - ax) + (ay > by ? ay - by : by - ay) +
-                                     (az > bz ? az - bz : bz - az);
-                if (dist == 1 && grp[w][cc] != xv.group_id && merge_gate(px, pl[w][cc], m.merge_a, m.merge_d))
-                  need = true;
+          if (!need && (xflags & kPlane) && (xflags & kInit)) {
+            int64_t ax, ay, az;
+            unpack_key(xkey, ax, ay, az);
+            for (int k = 0; k < nc0; ++k) {
+              const int cc = clos[w][k];
+              int64_t bx, by, bz;
+              unpack_key(rkey[w][cc], bx, by, bz);
+              const int64_t dist = (ax > bx ? ax - bx : bx - ax) + (ay > by ? ay - by : by - ay) +
+                                   (az > bz ? az - bz : bz - az);
+              if (dist == 1 && grp[w][cc] != xgrp && merge_gate(px, pl[w][cc], m.merge_a, m.merge_d)) need = true;
+            }
+          }
+          if (need) {
+            int xseg = pre_seg, tn = pre_tn;
+            if (pre < 0) {
+              xseg = (xtag == tag && xsg >= 0 && xev == kNever) ? xsg : -1;
+              tn = (xseg >= 0 && xmp != kNever) ? first_call_after(b, xsg, xmp, t, off) : kNever;
+              if (xseg >= 0) {
+                nct = b.seg_nct[xseg];
+                for (int k = 0; k < nct; ++k) ct[k] = b.seg_ct[xseg * 7 + k];
               }
             }
-          }
-        }
-        if (X >= 0 && need) {
-          const VoxState& xs = m.vs[X];
-          int xseg = pre_seg, tn = pre_tn;
-          if (pre < 0) {
-            xseg = (xs.tag == tag && xs.seg >= 0 && xs.evict_t == kNever) ? xs.seg : -1;
-            tn = (xseg >= 0 && xs.merge_pos != kNever) ? first_call_after(b, xs.seg, xs.merge_pos, t, off) : kNever;
-          }
-          if (xseg >= 0) {
-            uint64_t loc[8];
-            int nl = 0;
-            if (tn != kNever) loc[nl++] = cand_key(0, tn, X);
-            const int nct = b.seg_nct[xseg];
-            for (int k = 0; k < nct; ++k) {
-              const int tc = b.seg_ct[xseg * 7 + k];
-              if (tc > t) loc[nl++] = cand_key(0, tc, X);
-            }
-            if (nl > 0) {
-              const int at = atomicAdd(&sh_i[4], nl);
-              for (int k = 0; k < nl; ++k)
-                if (at + k < kNewCap) {
-                  nkey[at + k] = loc[k];
-                  nkown[at + k] = w;
-                }
-            }
-            const int f = atomicAdd(&sh_i[5], 1);
-            if (f < kFxCap) {
-              fx_v[f] = X;
-              fx_wk[f] = w * 2;
-            }
-          } else if (xs.flags & kAlive) {
-            const int f = atomicAdd(&sh_i[5], 1);
-            if (f < kFxCap) {
-              fx_v[f] = X;
-              fx_wk[f] = w * 2 + 1;
+            if (xseg >= 0) {
+              uint64_t loc[8];
+              int nl = 0;
+              if (tn != kNever) loc[nl++] = cand_key(0, tn, X);
+              for (int k = 0; k < nct; ++k) {
+                const int tc = ct[k];
+                if (tc > t) loc[nl++] = cand_key(0, tc, X);
+              }
+              if (nl > 0) {
+                const int at = atomicAdd(&sh_i[4], nl);
+                for (int k = 0; k < nl; ++k)
+                  if (at + k < kNewCap) {
+                    nkey[at + k] = loc[k];
+                    nkown[at + k] = w;
+                  }
+              }
+              const int f = atomicAdd(&sh_i[5], 1);
+              if (f < kFxCap) {
+                fx_v[f] = X;
+                fx_wk[f] = w * 2;
+              }
+            } else if (xflags & kAlive) {
+              const int f = atomicAdd(&sh_i[5], 1);
+              if (f < kFxCap) {
+                fx_v[f] = X;
+                fx_wk[f] = w * 2 + 1;
+              }
             }
           }
         }
