@@ -394,7 +394,7 @@ def main():
     ap.add_argument("--serve-sessions", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
-    ap.add_argument("--serve-executors", type=int, default=2)
+    ap.add_argument("--serve-executors", type=int, default=4)
     ap.add_argument("--serve-max-batch", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
