@@ -559,12 +559,12 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
     }
     __syncthreads();
     const int na = sh_i[3];
-    // ---- taint phase of every acting entry: for each changed voxel and each
-    //      voxel of its closed neighbourhood X: X re-evaluates its next call
-    //      and its later candidates if X changed itself, or if X (a plane not
-    //      in the group) passes the gates against a new record.  A voxel of
-    //      the scan records its dirty/taint times, one outside is marked dirty.
-    for (int i = tid; i < kFwCap; i += kMrThreads) {  // cleared for the commit below
+    // ---- conflicts through the records: the first writer of every voxel the
+    //      acting entries change, then for each entry whether one of its
+    //      seven records has an earlier writer.  The commit stops at the
+    //      first such entry at the latest, so only the acting entries before
+    //      it run the taint phase.
+    for (int i = tid; i < kFwCap; i += kMrThreads) {
       fw_key[i] = -1;
       fw_min[i] = kWin;
     }
@@ -572,7 +572,55 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
       sh_minkey[tid] = ~0ull;
       sh_conf[tid] = 0;
     }
-    for (int T = tid; T < na * kTaintPer; T += kMrThreads) {
+    __syncthreads();
+    if (tid < na * 8) {
+      const int a = tid >> 3, ci = tid & 7;
+      const int k = act[a];
+      if (ci < nclos[k]) {
+        const int32_t X = nbv[k][clos[k][ci]];
+        uint32_t h = fw_hash(X);
+        while (true) {  // at most half full: the probe ends
+          const int32_t old = atomicCAS(&fw_key[h], -1, X);
+          if (old == -1 || old == X) {
+            atomicMin(&fw_min[h], k);
+            break;
+          }
+          h = (h + 1) & (kFwCap - 1);
+        }
+      }
+    }
+    __syncthreads();
+    for (int q = tid; q < n * 7; q += kMrThreads) {
+      const int w = q / 7;
+      const int32_t v = nbv[w][q % 7];
+      if (v < 0) continue;
+      uint32_t h = fw_hash(v);
+      while (true) {
+        const int32_t kk = fw_key[h];
+        if (kk == -1) break;
+        if (kk == v) {
+          if (fw_min[h] < w) sh_conf[w] = 1;
+          break;
+        }
+        h = (h + 1) & (kFwCap - 1);
+      }
+    }
+    __syncthreads();
+    if (tid < 64) {  // acting entries before the first record conflict (act ascends)
+      const uint64_t cm = __ballot(tid < n && sh_conf[tid]);
+      const int pre = cm ? __ffsll((unsigned long long)cm) - 1 : n;
+      const uint64_t am = __ballot(tid < na && act[tid] < pre);
+      if (tid == 0) sh_i[6] = __popcll(am);
+    }
+    __syncthreads();
+    const int nat = sh_i[6];
+    // ---- taint phase of those acting entries: for each changed voxel and
+    //      each voxel of its closed neighbourhood X: X re-evaluates its next
+    //      call and its later candidates if X changed itself, or if X (a plane
+    //      not in the group) passes the gates against a new record.  A voxel
+    //      of the scan records its dirty/taint times, one outside is marked
+    //      dirty.
+    for (int T = tid; T < nat * kTaintPer; T += kMrThreads) {
       const int a = T / kTaintPer, r = T % kTaintPer, ci = r / 7, j = r % 7;
       const int w = act[a];
       const int nc0 = nclos[w];
@@ -677,37 +725,15 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
     }
     // ---- commit in point order, all at once: the first entry that read a
     //      voxel an earlier acting entry of the window writes (its seven
-    //      records, and if it acts the records of its taint phase), or that
-    //      an earlier entry's new key must precede, stops the commit.  The
-    //      entries before it commit together: their write sets are disjoint
-    //      (an entry that writes a voxel also read it).
-    if (tid < na * 8) {  // first writer of every voxel the acting entries change
-      const int a = tid >> 3, ci = tid & 7;
-      const int k = act[a];
-      if (ci < nclos[k]) {
-        const int32_t X = nbv[k][clos[k][ci]];
-        uint32_t h = fw_hash(X);
-        while (true) {  // at most half full: the probe ends
-          const int32_t old = atomicCAS(&fw_key[h], -1, X);
-          if (old == -1 || old == X) {
-            atomicMin(&fw_min[h], k);
-            break;
-          }
-          h = (h + 1) & (kFwCap - 1);
-        }
-      }
-    }
+    //      records, found above, and if it acts the records of its taint
+    //      phase), or that an earlier entry's new key must precede, stops the
+    //      commit.  The entries before it commit together: their write sets
+    //      are disjoint (an entry that writes a voxel also read it).
     for (int q = tid; q < nnew; q += kMrThreads)
       atomicMin(reinterpret_cast<unsigned long long*>(&sh_minkey[nkown[q]]), (unsigned long long)nkey[q]);
-    __syncthreads();
-    // reads: (entry, voxel) for the seven records of every entry and the
-    // taint reads of acting ones; conflict if the voxel's first writer of
-    // the window is an earlier entry
-    for (int q = tid; q < n * (7 + kTaintPer); q += kMrThreads) {
-      const int w = q / (7 + kTaintPer), r = q % (7 + kTaintPer);
-      int32_t v;
-      if (r < 7) v = nbv[w][r];
-      else v = nclos[w] > 0 ? r2[w][r - 7] : -1;
+    for (int q = tid; q < nat * kTaintPer; q += kMrThreads) {  // the taint reads
+      const int w = act[q / kTaintPer];
+      const int32_t v = r2[w][q % kTaintPer];
       if (v < 0) continue;
       uint32_t h = fw_hash(v);
       while (true) {
