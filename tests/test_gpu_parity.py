@@ -254,6 +254,51 @@ def test_lru_ring_compaction_parity(ctx):
     assert touched > 2 * qcap, "the ring must have been compacted at least twice"
 
 
+def test_lru_eviction_after_static_period(ctx):
+    """A sensor at rest rescans the same room: every scan touches the same
+    voxels, so the LRU ring fills with stale entries ahead of the live ones.
+    Scans of another scene then create voxels at capacity: the victims lie
+    behind more stale entries than the parallel liveness pre-scan covers
+    (two entries per point), so the ordered pass also reads the ring
+    directly.  LRU order, keys and group ids must stay bit-exact.
+
+    Sizes: the room scan makes about 5,200 voxels and the first street scan
+    about 5,000 more; six rescans leave about 31,000 stale ring entries ahead
+    of the live ones, below the compaction threshold of this session's ring
+    (45,024 entries less one scan), while the pre-scan covers 16,000."""
+    npts, cap = 8000, 6000
+    room = Sequence(ROOM, SEED0 + 31, npts)
+    street = Sequence(STREET, SEED0 + 32, npts)
+    cfg = LIOConfig(max_scan_points=npts, map_capacity=cap)
+    orc, s0, P0, (x0, y0, z0) = start(room, cfg)
+    still = (np.eye(3), np.zeros(3))
+    with Registrar(ctx, cfg) as g:
+        g.set_state(s0, P0)
+        g.set_points(x0, y0, z0)
+        g.build()
+        evicted = 0
+        scans = [(x0, y0, z0)] * 6 + [street.scan(k) for k in range(3)]
+        for k, (x, y, z) in enumerate(scans):
+            orc.apply_motion(*still, QDIAG)
+            sp, Pp = orc.get_state()
+            orc.register(x, y, z)
+            g.set_state(sp, Pp)
+            g.set_points(x, y, z)
+            _, mst = g.register()
+            evicted += mst.evicted
+            so, _ = orc.get_state()
+            sg, _ = g.get_state()
+            ep, er = state_err(so, sg)
+            a, b = orc.export_map(), g.export_map()
+            log(f"scan {k}: created {mst.created} evicted {mst.evicted} map {len(b['keys'])} pos {ep:.1e}")
+            assert ep <= 1e-4 and er <= 1e-5, (k, ep, er)
+            assert np.array_equal(a["keys"], b["keys"]), f"LRU order differs after scan {k}"
+            r = compare_maps(a, b)
+            assert r["plane_flag_diff"] <= max(2, r["n"] // 1000)
+        assert evicted > 1000
+        assert orc.stats()["evicted"] == evicted
+
+
 def test_capacity_below_scan_is_an_error(ctx):
     """A scan touching more voxels than the LRU can evict in point order is
     reported (VMP_ECAPACITY) instead of silently diverging."""
