@@ -732,8 +732,10 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
   if (hipMemcpyAsync(w.d_obsstat, w.h_obsstat, sizeof(int32_t*) * n, hipMemcpyHostToDevice, st) != hipSuccess)
     return VMP_EDEVICE;
   const bool ext = ss[0]->cfg.estimate_ext != 0;
-  auto observe = [&](int grid) {
-    hipLaunchKernelGGL(k_zero_lists, dim3(n), dim3(64), 0, st, w.d_obsctr, 8);
+  // zero: clear the sessions' observe counters first (inside the filter stage
+  // k_ieskf_begin and k_ieskf_step leave them cleared instead)
+  auto observe = [&](int grid, bool zero) {
+    if (zero) hipLaunchKernelGGL(k_zero_lists, dim3(n), dim3(64), 0, st, w.d_obsctr, 8);
     const bool tm = c->timing && c->tev_used + 2 <= 2 * 64;
     const bool w32 = c->obs_arith == 1;
     auto launch = [&]() {
@@ -765,7 +767,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
   };
   if (mode & kStepObserveOnce) {
     hipLaunchKernelGGL(k_obs_prep, dim3(n), dim3(64), 0, st, w.d_flist);
-    observe(nblocks);
+    observe(nblocks, true);
     return VMP_OK;
   }
   hipEventRecord(c->ev[0], st);
@@ -773,9 +775,9 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
     int max_iter = 1;
     for (int i = 0; i < n; ++i) max_iter = std::max(max_iter, ss[i]->cfg.max_iter);
     auto filter_stage = [&](int grid) {
-      hipLaunchKernelGGL(k_ieskf_begin, dim3(n), dim3(64), 0, st, w.d_flist);
+      hipLaunchKernelGGL(k_ieskf_begin, dim3(n), dim3(64), 0, st, w.d_flist, w.d_obsctr);
       for (int it = 0; it < max_iter; ++it) {
-        observe(grid);
+        observe(grid, false);
         hipLaunchKernelGGL(k_ieskf_step, dim3(n), dim3(64), 0, st, w.d_flist, w.d_obsctr, w.d_obsstat);
       }
       hipLaunchKernelGGL(k_ieskf_end, dim3(n), dim3(64), 0, st, w.d_flist);

@@ -62,8 +62,9 @@ __global__ void k_zero_lists(int32_t** lists, int n) {
 }
 
 // start of an update: predict_x = x, P^-1, first observe parameters
-__global__ void __launch_bounds__(64) k_ieskf_begin(FilterDev** fs) {
+__global__ void __launch_bounds__(64) k_ieskf_begin(FilterDev** fs, int32_t** obs_ctr) {
   FilterDev* f = fs[blockIdx.x];
+  if (threadIdx.x < 8) obs_ctr[blockIdx.x][threadIdx.x] = 0;  // counters of the first observe pass
   __shared__ double M[529], Inv[529];
   __shared__ vmp_state sx;
   const int lane = threadIdx.x;
@@ -88,6 +89,7 @@ __global__ void __launch_bounds__(64) k_ieskf_begin(FilterDev** fs) {
 }
 
 // one IESKF iteration after an observe pass (ieskf.cpp:133-149)
+// (the observe counters are read, then cleared for the next pass)
 __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, int32_t** obs_ctr, int32_t** stat_out) {
   FilterDev* f = fs[blockIdx.x];
   if (!f->active) return;
@@ -192,7 +194,7 @@ __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, int32_t** obs
     make_obs_params(sx, f->P, f->cfg, f->obs_prm);
     f->obs_prm.active = f->active;
     int32_t* so = stat_out[blockIdx.x];
-    const int32_t* oc = obs_ctr[blockIdx.x];
+    int32_t* oc = obs_ctr[blockIdx.x];
     if (f->iters == 1)
       for (int k = 4; k < 8; ++k) so[k] = 0;
     for (int k = 0; k < 4; ++k) so[k] = oc[k];
@@ -200,6 +202,7 @@ __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, int32_t** obs
     so[5] += oc[1];
     so[6] += oc[2];
     so[7] |= oc[4];  // hash probe overflow
+    for (int k = 0; k < 8; ++k) oc[k] = 0;
   }
 }
 
