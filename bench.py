@@ -276,14 +276,16 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000, groups=1):
 
 
 def run_serving(sessions, rate, duration, nmax=120000, nmin=2000, capacity=200000, budget_gb=200, executors=2,
-                max_batch_sessions=128, seed=0x5E27E):
+                max_batch_sessions=128, seed=0x5E27E, uniform_scans=4):
     """Continuous batching: `sessions` sequences (room / street alternating),
     Poisson scan arrivals at `rate` scans/s in aggregate for `duration` s,
     scan sizes log-uniform in [nmin, nmax], maps and states resident in HBM
     under `budget_gb`.  Every session's MAP_INIT scan is served first (all
     arriving at once, not timed); the timed trace holds the registered scans.
     A second replay of the same scans with every request arriving at t = 0
-    measures the serving loop's saturation throughput."""
+    measures the serving loop's saturation throughput.  A third replay at
+    saturation feeds every session `uniform_scans` scans with sizes uniform in
+    [nmin, nmax], the batched form's distribution, for a like-for-like rate."""
     import torch
 
     from voxelmapplus_fastlio2_amd import LIOConfig, initial_covariance, state_vector
@@ -358,6 +360,55 @@ def run_serving(sessions, rate, duration, nmax=120000, nmin=2000, capacity=20000
                 "resident_gb": st["resident_bytes"] / (1 << 30),
             }
             log(f"serving {mode}: {out[mode]}")
+        if uniform_scans > 0:
+            del dev
+            # per session: MAP_INIT then uniform_scans scans, sizes U[nmin, nmax]
+            usz = rng.integers(nmin, nmax + 1, size=(sessions, uniform_scans + 1))
+            utot = int(usz.sum())
+            host = np.empty((3, utot), np.float32)
+            uwhere = {}
+            off = 0
+            for i in range(sessions):
+                for k in range(uniform_scans + 1):
+                    n = int(usz[i, k])
+                    x, y, z = seqs[i].scan(k, n)
+                    host[0, off : off + n], host[1, off : off + n], host[2, off : off + n] = x, y, z
+                    uwhere[(i, k)] = (off, n)
+                    off += n
+            dev = torch.from_numpy(host).to("cuda:0")
+            del host
+            ubase, ustride = dev.data_ptr(), dev.stride(0) * 4
+
+            def uitem(i, k):
+                o, n = uwhere[(i, k)]
+                it = {"session": i, "n": n, "x": ubase + 4 * o, "y": ubase + ustride + 4 * o,
+                      "z": ubase + 2 * ustride + 4 * o, "arrival": 0.0}
+                if k > 0:
+                    it["dR"], it["dp"] = seqs[i].motion(k)
+                return it
+
+            for i in range(sessions):
+                R, t = seqs[i].pose(0)
+                sv.session_init(i, state_vector(R, t), initial_covariance())
+            sv.run(make_requests([uitem(i, 0) for i in range(sessions)]), QDIAG)
+            reqs = make_requests([uitem(i, k) for k in range(1, uniform_scans + 1) for i in range(sessions)])
+            torch.cuda.synchronize()
+            st0 = sv.stats()
+            sv.run(reqs, QDIAG)
+            st = sv.stats()
+            done = np.array([r.done for r in reqs])
+            npts = np.array([r.n for r in reqs])
+            span = done.max()
+            nb = st["batches"] - st0["batches"]
+            out["saturation_uniform"] = {
+                "value": float(npts.sum() / span),
+                "scans": len(reqs),
+                "scans_per_s": float(len(reqs) / span),
+                "mean_batch_sessions": float((st["requests"] - st0["requests"]) / max(nb, 1)),
+                "batches": nb,
+                "sizes": f"U[{nmin},{nmax}], {uniform_scans} scans per session after MAP_INIT",
+            }
+            log(f"serving saturation_uniform: {out['saturation_uniform']}")
     del dev
     offered = {
         "scans_per_s": rate,
@@ -370,6 +421,7 @@ def run_serving(sessions, rate, duration, nmax=120000, nmin=2000, capacity=20000
         "offered": offered,
         "poisson": out["poisson"],
         "saturation": out["saturation"],
+        "saturation_uniform": out.get("saturation_uniform"),
     }
 
 
