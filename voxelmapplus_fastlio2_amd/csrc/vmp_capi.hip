@@ -733,8 +733,9 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
     return VMP_EDEVICE;
   const bool ext = ss[0]->cfg.estimate_ext != 0;
   // zero: clear the sessions' observe counters first (inside the filter stage
-  // k_ieskf_begin and k_ieskf_step leave them cleared instead)
-  auto observe = [&](int grid, bool zero) {
+  // k_ieskf_begin and k_ieskf_step leave them cleared instead); reduce: add the
+  // partial rows here (inside the filter stage k_ieskf_step adds them)
+  auto observe = [&](int grid, bool zero, bool reduce) {
     if (zero) hipLaunchKernelGGL(k_zero_lists, dim3(n), dim3(64), 0, st, w.d_obsctr, 8);
     const bool tm = c->timing && c->tev_used + 2 <= 2 * 64;
     const bool w32 = c->obs_arith == 1;
@@ -759,15 +760,15 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         launch();
       }
     }
-    if (ext)
+    if (reduce && ext)
       hipLaunchKernelGGL(k_obs_reduce<12>, dim3(n), dim3(kObsThreads), 0, st, w.d_jobs);
-    else
+    else if (reduce)
       hipLaunchKernelGGL(k_obs_reduce<6>, dim3(n), dim3(kObsThreads), 0, st, w.d_jobs);
     c->obs_launches++;
   };
   if (mode & kStepObserveOnce) {
     hipLaunchKernelGGL(k_obs_prep, dim3(n), dim3(64), 0, st, w.d_flist);
-    observe(nblocks, true);
+    observe(nblocks, true, true);
     return VMP_OK;
   }
   hipEventRecord(c->ev[0], st);
@@ -777,8 +778,11 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
     auto filter_stage = [&](int grid) {
       hipLaunchKernelGGL(k_ieskf_begin, dim3(n), dim3(64), 0, st, w.d_flist, w.d_obsctr);
       for (int it = 0; it < max_iter; ++it) {
-        observe(grid, false);
-        hipLaunchKernelGGL(k_ieskf_step, dim3(n), dim3(64), 0, st, w.d_flist, w.d_obsctr, w.d_obsstat);
+        observe(grid, false, false);
+        if (ext)
+          hipLaunchKernelGGL(k_ieskf_step<12>, dim3(n), dim3(64), 0, st, w.d_flist, w.d_jobs, w.d_obsctr, w.d_obsstat);
+        else
+          hipLaunchKernelGGL(k_ieskf_step<6>, dim3(n), dim3(64), 0, st, w.d_flist, w.d_jobs, w.d_obsctr, w.d_obsstat);
       }
       hipLaunchKernelGGL(k_ieskf_end, dim3(n), dim3(64), 0, st, w.d_flist);
     };

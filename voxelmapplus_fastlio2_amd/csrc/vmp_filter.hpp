@@ -4,6 +4,7 @@
 #pragma once
 #include "vmp_common.hpp"
 #include "vmp_ieskf.hpp"
+#include "vmp_observe.hpp"
 
 namespace vmp {
 
@@ -89,8 +90,17 @@ __global__ void __launch_bounds__(64) k_ieskf_begin(FilterDev** fs, int32_t** ob
 }
 
 // one IESKF iteration after an observe pass (ieskf.cpp:133-149)
-// (the observe counters are read, then cleared for the next pass)
-__global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, int32_t** obs_ctr, int32_t** stat_out) {
+// The session's observe partial rows (one per observe block, jobs[session])
+// are reduced here in a fixed order: row b is added by lane b mod 64, then
+// the lanes are summed by the wave (H 12x12 column-major and b as the
+// standalone reduction writes them).  The observe counters are read, then
+// cleared for the next pass.
+template <int NJ>
+__global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, const ObsJob* __restrict__ jobs, int32_t** obs_ctr,
+                                                   int32_t** stat_out) {
+  constexpr int NH = NJ * (NJ + 1) / 2;
+  constexpr int NACC = NH + NJ;
+  constexpr int ROW = obs_row<NJ>();
   FilterDev* f = fs[blockIdx.x];
   if (!f->active) return;
   __shared__ double Pi[529];
@@ -98,11 +108,38 @@ __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, int32_t** obs
   __shared__ vmp_state sx, sxp;
   __shared__ JacBlocks jb;
   __shared__ double hobs[156];
+  __shared__ double red[ROW];
   const int lane = threadIdx.x;
   load_state(f->x, sx, lane);
   load_state(f->xp, sxp, lane);
   for (int i = lane; i < 529; i += 64) Pi[i] = f->Pinv[i];
-  for (int i = lane; i < 156; i += 64) hobs[i] = f->obs[i];
+  {
+    const ObsJob& J = jobs[blockIdx.x];
+    double fin[ROW];
+#pragma unroll
+    for (int k = 0; k < ROW; ++k) fin[k] = 0.0;
+    for (int b = lane; b < J.nblocks; b += 64) {
+      const double* src = J.partials + size_t(b) * ROW;
+#pragma unroll
+      for (int k = 0; k < ROW; ++k) fin[k] += src[k];
+    }
+#pragma unroll
+    for (int k = 0; k < ROW; ++k) {
+      const double v = wave_sum(fin[k]);
+      if (lane == 0) red[k] = v;
+    }
+  }
+  __syncthreads();
+  for (int i = lane; i < 144; i += 64) {  // H from the upper triangle
+    const int r = i % 12, c = i / 12;
+    double v = 0.0;
+    if (r < NJ && c < NJ) {
+      const int a = r < c ? r : c, b2 = r < c ? c : r;
+      v = red[a * NJ - a * (a - 1) / 2 + (b2 - a)];
+    }
+    hobs[i] = v;
+  }
+  if (lane < 12) hobs[144 + lane] = lane < NJ ? red[NH + lane] : 0.0;
   __syncthreads();
   state_minus_par(sx, sxp, dl, lane);
   jac_blocks_par(sx, sxp, dl, jb, lane);
@@ -197,10 +234,12 @@ __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, int32_t** obs
     int32_t* oc = obs_ctr[blockIdx.x];
     if (f->iters == 1)
       for (int k = 4; k < 8; ++k) so[k] = 0;
-    for (int k = 0; k < 4; ++k) so[k] = oc[k];
-    so[4] += oc[0];  // hits over all passes of this update
-    so[5] += oc[1];
-    so[6] += oc[2];
+    int32_t cnt[4];  // hits, plane hits, effective, fallbacks of this pass
+    for (int k = 0; k < 4; ++k) cnt[k] = int32_t(red[NACC + k]);
+    for (int k = 0; k < 4; ++k) so[k] = cnt[k];
+    so[4] += cnt[0];  // hits over all passes of this update
+    so[5] += cnt[1];
+    so[6] += cnt[2];
     so[7] |= oc[4];  // hash probe overflow
     for (int k = 0; k < 8; ++k) oc[k] = 0;
   }

@@ -1,7 +1,9 @@
 // Iterated error-state Kalman filter update on the device (kf::IESKF::update,
-// ieskf.cpp:125-156) — one 64-lane wave per session, FP64, 23x23 algebra in
-// LDS.  The measurement function is the observe kernel; each iteration is
-//   observe -> k_ieskf_step (H_, b_ assembly, Gauss-Jordan inverse, x += dx),
+// ieskf.cpp:125-156) — one 64-lane wave per session, FP64, 23x23 algebra on
+// rows held in lanes (and in LDS).  The measurement function is the observe
+// kernel; each iteration is
+//   observe -> k_ieskf_step (reduction of the observe partial rows, H_, b_
+//              assembly, Cholesky solve, x += dx),
 // and once converged (signed max(dx) < 1e-3, ieskf.cpp:148) or after max_iter
 // passes the session is marked inactive so later passes are no-ops, which is
 // what the reference's `break` does.  P = L H_^-1 L^T follows with L built
