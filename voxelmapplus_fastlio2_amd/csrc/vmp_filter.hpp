@@ -73,7 +73,8 @@ __global__ void __launch_bounds__(64) k_ieskf_begin(FilterDev** fs, int32_t** ob
   double a[23];  // row `lane` of P (column-major in memory), factored in lanes
 #pragma unroll
   for (int c = 0; c < 23; ++c) a[c] = lane < 23 ? f->P[c * 23 + lane] : 0.0;
-  chol_rows(a, lane);
+  double dinv;
+  chol_rows(a, dinv, lane);
   if (lane < 23) {
 #pragma unroll
     for (int c = 0; c < 23; ++c) M[lane * 23 + c] = a[c];
@@ -102,7 +103,9 @@ __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, const ObsJob*
   constexpr int NACC = NH + NJ;
   constexpr int ROW = obs_row<NJ>();
   FilterDev* f = fs[blockIdx.x];
-  if (!f->active) return;
+  // the loads below are valid whether or not the session is still active,
+  // so they are issued together with the flag's
+  const bool active = f->active;
   __shared__ double Pi[529];
   __shared__ double dl[23];
   __shared__ vmp_state sx, sxp;
@@ -129,6 +132,7 @@ __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, const ObsJob*
       if (lane == 0) red[k] = v;
     }
   }
+  if (!active) return;  // uniform over the block, before its first barrier
   __syncthreads();
   for (int i = lane; i < 144; i += 64) {  // H from the upper triangle
     const int r = i % 12, c = i / 12;
@@ -211,8 +215,9 @@ __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, const ObsJob*
     if (r < 12) s += hobs[144 + r];
     br = s;
   }
-  chol_rows(a, lane);
-  const double xr = chol_solve_rows(a, br, lane);  // delta = -H_^-1 b_
+  double dinv;
+  chol_rows(a, dinv, lane);
+  const double xr = chol_solve_rows(a, dinv, br, lane);  // delta = -H_^-1 b_
   if (lane < 23) {
 #pragma unroll
     for (int c = 0; c < 23; ++c) f->Hinv[lane * 23 + c] = a[c];  // Cholesky factor of the last H_

@@ -290,21 +290,28 @@ __device__ inline void chol_inverse(const double* L, double* Inv, int lane) {
 // ---- the same factorisation and solve on rows held in lanes ------------
 // Lane i (< 23) holds row i of a 23x23 matrix in a[0..22]; values cross lanes
 // by readlane (uniform index), so nothing goes through LDS and no barrier is
-// needed.  Each element sees the same operations in the same order as in
-// chol_factor / chol_solve_vec.
+// needed.  The order of operations per element is that of chol_factor /
+// chol_solve_vec, with divisions by a diagonal element done as products with
+// its reciprocal (kept per lane in dinv).
 __device__ inline double rl64(double v, int src) {
   const long long x = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane(int(x), src);
   const int hi = __builtin_amdgcn_readlane(int(x >> 32), src);
   return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
-// in place: row i becomes L[i][0..i], its upper part untouched
-__device__ inline void chol_rows(double (&a)[23], int lane) {
+// in place: row i becomes L[i][0..i], its upper part untouched; dinv = 1/L[i][i]
+__device__ inline void chol_rows(double (&a)[23], double& dinv, int lane) {
+  dinv = 0.0;
 #pragma unroll
   for (int k = 0; k < 23; ++k) {
     const double d = sqrt(rl64(a[k], k));
-    if (lane == k) a[k] = d;
-    else if (lane > k) a[k] = a[k] / d;
+    const double inv = 1.0 / d;
+    if (lane == k) {
+      a[k] = d;
+      dinv = inv;
+    } else if (lane > k) {
+      a[k] = a[k] * inv;
+    }
 #pragma unroll
     for (int j = k + 1; j < 23; ++j) {
       const double ljk = rl64(a[k], j);
@@ -313,11 +320,11 @@ __device__ inline void chol_rows(double (&a)[23], int lane) {
   }
 }
 // x = A^-1 b for A = L L^T from chol_rows; b and x one element per lane
-__device__ inline double chol_solve_rows(const double (&a)[23], double b, int lane) {
+__device__ inline double chol_solve_rows(const double (&a)[23], double dinv, double b, int lane) {
   double y = b;
 #pragma unroll
   for (int j = 0; j < 23; ++j) {  // L y = b
-    if (lane == j) y = y / a[j];
+    if (lane == j) y = y * dinv;
     const double yj = rl64(y, j);
     if (lane > j) y -= a[j] * yj;
   }
@@ -327,7 +334,7 @@ __device__ inline double chol_solve_rows(const double (&a)[23], double b, int la
     double s = rl64(y, i);
 #pragma unroll
     for (int j = i + 1; j < 23; ++j) s -= rl64(a[i], j) * rl64(x, j);
-    const double xi = s / rl64(a[i], i);
+    const double xi = s * rl64(dinv, i);
     if (lane == i) x = xi;
   }
   return x;
