@@ -103,9 +103,7 @@ __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, const ObsJob*
   constexpr int NACC = NH + NJ;
   constexpr int ROW = obs_row<NJ>();
   FilterDev* f = fs[blockIdx.x];
-  // the loads below are valid whether or not the session is still active,
-  // so they are issued together with the flag's
-  const bool active = f->active;
+  if (!f->active) return;  // converged: later passes are no-ops
   __shared__ double Pi[529];
   __shared__ double dl[23];
   __shared__ vmp_state sx, sxp;
@@ -132,7 +130,6 @@ __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, const ObsJob*
       if (lane == 0) red[k] = v;
     }
   }
-  if (!active) return;  // uniform over the block, before its first barrier
   __syncthreads();
   for (int i = lane; i < 144; i += 64) {  // H from the upper triangle
     const int r = i % 12, c = i / 12;
