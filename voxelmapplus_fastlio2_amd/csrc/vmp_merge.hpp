@@ -391,7 +391,6 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
   __shared__ int32_t rseg[kWin][7];   // record voxels: segment of this scan (-1 none), key, flags
   __shared__ uint64_t rkey[kWin][7];
   __shared__ uint32_t rflg[kWin][7];
-  __shared__ int pass[kWin];
   __shared__ int sh_mg;
   __shared__ int sh_rew;  // statics of this window that are read again next window
   __shared__ uint64_t sh_minkey[kWin];
@@ -520,29 +519,35 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
       }
     }
     __syncthreads();
-    // ---- gates of every neighbour against the entry's own (unchanged) plane:
-    //      an entry none passes does not act, the others run merge() below
-    if (tid < n) pass[tid] = 0;
-    __syncthreads();
-    if (tid < n * 7) {
-      const int w = tid / 7, j = tid % 7;
-      if (j < 6 && nbv[w][j] >= 0 && grp[w][j] != grp[w][6] && merge_gate(pl[w][6], pl[w][j], m.merge_a, m.merge_d))
-        pass[w] = 1;
-    }
-    __syncthreads();
-    // ---- merge(): each entry's six neighbours in order against its own plane
-    if (tid < n && !pass[tid]) nclos[tid] = 0;
-    if (tid < n && pass[tid]) {
+    // ---- merge(): each entry's six neighbours in order against its own
+    //      plane, one thread per entry with its seven records in registers
+    //      (an entry no neighbour passes changes nothing)
+    if (tid < n) {
       const int w = tid;
+      double pv[kPlaneD], pn[6][kPlaneD];
+#pragma unroll
+      for (int k = 0; k < kPlaneD; ++k) pv[k] = pl[w][6][k];
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+#pragma unroll
+        for (int k = 0; k < kPlaneD; ++k) pn[j][k] = pl[w][j][k];
+      const uint64_t g6 = grp[w][6];
       int nc = 0;
+#pragma unroll
       for (int j = 0; j < 6; ++j) {
-        if (nbv[w][j] < 0 || grp[w][j] == grp[w][6]) continue;
-        if (!merge_gate(pl[w][6], pl[w][j], m.merge_a, m.merge_d)) continue;
-        merge_fuse(pl[w][6], pl[w][j]);
-        grp[w][j] = grp[w][6];
+        if (nbv[w][j] < 0 || grp[w][j] == g6) continue;
+        if (!merge_gate(pv, pn[j], m.merge_a, m.merge_d)) continue;
+        merge_fuse(pv, pn[j]);
+#pragma unroll
+        for (int k = 0; k < kPlaneD; ++k) pl[w][j][k] = pn[j][k];
+        grp[w][j] = g6;
         clos[w][nc++] = j;
       }
-      if (nc > 0) clos[w][nc++] = 6;
+      if (nc > 0) {
+#pragma unroll
+        for (int k = 0; k < kPlaneD; ++k) pl[w][6][k] = pv[k];
+        clos[w][nc++] = 6;
+      }
       nclos[w] = nc;
     }
     __syncthreads();
