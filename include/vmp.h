@@ -153,7 +153,8 @@ int64_t vmp_map_size(vmp_session* s);
 /* Residual slots: valid flag, residual, normal (3 per point). */
 int vmp_slots_export(vmp_session* s, int32_t n, uint8_t* valid, float* resid, float* norm3);
 
-/* Batched forms: sessions advance together, one launch per stage. */
+/* Batched forms: sessions advance together, one launch per stage (at most
+ * 32768 sessions per batch). */
 int vmp_register_batch(vmp_session** s, int32_t n, vmp_obs_stats* ost, vmp_map_stats* mst);
 /* The same without waiting: the batch is launched and the call returns; the
  * next submit (or any call that needs the context) first completes it, so the

@@ -715,6 +715,10 @@ static hipGraphExec_t filter_graph(vmp_ctx* c, int n, int nblocks, int iters, bo
 }
 
 static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
+  if (n > kMaxBatchSessions) {
+    ss[0]->err = "more than 32768 sessions in one batch";
+    return VMP_EINVAL;
+  }
   BatchWs& w = c->ws;
   hipStream_t st = c->stream;
   int64_t ntot64 = 0;
@@ -987,6 +991,10 @@ static int step_finish(vmp_ctx* c, vmp_obs_stats* ost, vmp_map_stats* mst);
 // sure no session is in two batches at once); c->mu held, no batch in flight
 static int step_begin(vmp_ctx* c, vmp_session** ss, int n, int mode) {
   vmp_session* s = ss[0];
+  if (n > kMaxBatchSessions) {
+    s->err = "more than 32768 sessions in one batch";
+    return VMP_EINVAL;
+  }
   for (int i = 0; i < n; ++i) {
     if (!ss[i] || ss[i]->ctx->device != c->device) return VMP_EINVAL;
     if (!ss[i]->has_points) {

@@ -150,6 +150,9 @@ struct BatchBufs {
 };
 __host__ __device__ inline int64_t lv_cnt_base(int off, int s) { return (int64_t(2) * off >> 6) + s; }
 
+// sessions of one batch: the merge-candidate key holds a session in 15 bits
+constexpr int kMaxBatchSessions = 1 << 15;
+
 // merge-candidate key: session (15 bits) | point time (24 bits) | voxel (25 bits)
 __host__ __device__ inline uint64_t cand_key(int s, int t, int32_t vid) {
   return (uint64_t(uint32_t(s)) << 49) | (uint64_t(uint32_t(t)) << 25) | uint64_t(uint32_t(vid));
