@@ -66,7 +66,6 @@ __global__ void k_zero_lists(int32_t** lists, int n) {
 __global__ void __launch_bounds__(64) k_ieskf_begin(FilterDev** fs, int32_t** obs_ctr) {
   FilterDev* f = fs[blockIdx.x];
   if (threadIdx.x < 8) obs_ctr[blockIdx.x][threadIdx.x] = 0;  // counters of the first observe pass
-  __shared__ double M[529], Inv[529];
   __shared__ vmp_state sx;
   const int lane = threadIdx.x;
   load_state(f->x, sx, lane);
@@ -75,13 +74,8 @@ __global__ void __launch_bounds__(64) k_ieskf_begin(FilterDev** fs, int32_t** ob
   for (int c = 0; c < 23; ++c) a[c] = lane < 23 ? f->P[c * 23 + lane] : 0.0;
   double dinv;
   chol_rows(a, dinv, lane);
-  if (lane < 23) {
-#pragma unroll
-    for (int c = 0; c < 23; ++c) M[lane * 23 + c] = a[c];
-  }
+  chol_inverse_rows(a, dinv, f->Pinv, lane);  // P^-1
   __syncthreads();
-  chol_inverse(M, Inv, lane);
-  for (int i = lane; i < 529; i += 64) f->Pinv[i] = Inv[i];
   load_state(sx, f->xp, lane);
   if (lane == 0) {
     f->iters = 0;
@@ -250,17 +244,24 @@ __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, const ObsJob*
 // P = L H_^-1 L^T (ieskf.cpp:151-155) and the insertion parameters
 __global__ void __launch_bounds__(64) k_ieskf_end(FilterDev** fs) {
   FilterDev* f = fs[blockIdx.x];
-  __shared__ double Lc[529], T[529], Hi[529];
+  __shared__ double T[529], Hi[529];
   __shared__ double dl[23];
   __shared__ vmp_state sx, sxp;
   __shared__ JacBlocks jb;
   const int lane = threadIdx.x;
   load_state(f->x, sx, lane);
   load_state(f->xp, sxp, lane);
-  for (int i = lane; i < 529; i += 64) Lc[i] = f->Hinv[i];
+  // rows of the last H_'s Cholesky factor in lanes (its lower triangle)
+  double a[23];
+#pragma unroll
+  for (int c = 0; c < 23; ++c) a[c] = lane < 23 && c <= lane ? f->Hinv[lane * 23 + c] : 0.0;
+  double d = 1.0;
+#pragma unroll
+  for (int c = 0; c < 23; ++c)
+    if (lane == c) d = a[c];
   if (lane < 23) dl[lane] = f->delta[lane];
+  chol_inverse_rows(a, 1.0 / d, Hi, lane);  // H_^-1
   __syncthreads();
-  chol_inverse(Lc, Hi, lane);
   jac_blocks_par(sx, sxp, dl, jb, lane);
   // P = L H_^-1 L^T with L = J; J is the identity outside three small
   // blocks, so a block row or column adds up at most three products (in

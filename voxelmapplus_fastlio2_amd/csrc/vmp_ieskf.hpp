@@ -228,71 +228,13 @@ __device__ inline double jac_at(const JacBlocks& J, int r, int c) {
   return r == c ? 1.0 : 0.0;
 }
 
-// ---- Cholesky on a symmetric positive definite 23x23 (row-major, LDS) ----
-// Right-looking factorisation in place (lower triangle = L), one wave.
-__device__ inline void chol_factor(double* A, int lane) {
-  constexpr int n = 23;
-  for (int k = 0; k < n; ++k) {
-    if (lane == 0) A[k * n + k] = sqrt(A[k * n + k]);
-    __syncthreads();
-    const double d = A[k * n + k];
-    const int i = k + 1 + lane;
-    if (i < n) A[i * n + k] = A[i * n + k] / d;
-    __syncthreads();
-    if (i < n) {
-      const double lik = A[i * n + k];
-      for (int j = k + 1; j <= i; ++j) A[i * n + j] -= lik * A[j * n + k];
-    }
-    __syncthreads();
-  }
-}
-// x = A^-1 b with A = L L^T factored by chol_factor; lane 0 substitutes.
-__device__ inline void chol_solve_vec(const double* L, const double* b, double* x, int lane) {
-  constexpr int n = 23;
-  if (lane == 0) {
-    double y[n];
-    for (int i = 0; i < n; ++i) {
-      double s = b[i];
-      for (int j = 0; j < i; ++j) s -= L[i * n + j] * y[j];
-      y[i] = s / L[i * n + i];
-    }
-    for (int i = n - 1; i >= 0; --i) {
-      double s = y[i];
-      for (int j = i + 1; j < n; ++j) s -= L[j * n + i] * x[j];
-      x[i] = s / L[i * n + i];
-    }
-  }
-  __syncthreads();
-}
-// Inv = A^-1 (row-major), one column per lane.
-__device__ inline void chol_inverse(const double* L, double* Inv, int lane) {
-  constexpr int n = 23;
-  if (lane < n) {  // column c = lane, solved in place in Inv's column c (LDS)
-    const int c = lane;
-#pragma unroll 1
-    for (int i = 0; i < n; ++i) {  // L y = e_c
-      double s = (i == c) ? 1.0 : 0.0;
-#pragma unroll 4
-      for (int j = 0; j < i; ++j) s -= L[i * n + j] * Inv[j * n + c];
-      Inv[i * n + c] = s / L[i * n + i];
-    }
-#pragma unroll 1
-    for (int i = n - 1; i >= 0; --i) {  // L^T x = y
-      double s = Inv[i * n + c];
-#pragma unroll 4
-      for (int j = i + 1; j < n; ++j) s -= L[j * n + i] * Inv[j * n + c];
-      Inv[i * n + c] = s / L[i * n + i];
-    }
-  }
-  __syncthreads();
-}
-
-// ---- the same factorisation and solve on rows held in lanes ------------
-// Lane i (< 23) holds row i of a 23x23 matrix in a[0..22]; values cross lanes
-// by readlane (uniform index), so nothing goes through LDS and no barrier is
-// needed.  The order of operations per element is that of chol_factor /
-// chol_solve_vec, with divisions by a diagonal element done as products with
-// its reciprocal (kept per lane in dinv).
+// ---- Cholesky factorisation, solve and inverse of a symmetric positive
+//      definite 23x23 on rows held in lanes ---------------------------------
+// Lane i (< 23) holds row i of the matrix in a[0..22]; values cross lanes by
+// readlane (uniform index), so nothing goes through LDS and no barrier is
+// needed.  The factorisation is right-looking (column k scaled, then the
+// trailing lower triangle updated in ascending column order); divisions by a
+// diagonal element are products with its reciprocal (kept per lane in dinv).
 __device__ inline double rl64(double v, int src) {
   const long long x = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane(int(x), src);
@@ -338,6 +280,28 @@ __device__ inline double chol_solve_rows(const double (&a)[23], double dinv, dou
     if (lane == i) x = xi;
   }
   return x;
+}
+
+// A^-1 for A = L L^T from chol_rows (rows of L in lanes, dinv = 1/L[i][i]):
+// lane c forms column c of W = L^-1 by forward substitution, then
+// A^-1 = W^T W, whose row r is written by lane c into out[r * 23 + c]
+// (bitwise symmetric: both halves sum the same products in the same order).
+__device__ inline void chol_inverse_rows(const double (&a)[23], double dinv, double* out, int lane) {
+  double w[23];
+#pragma unroll
+  for (int i = 0; i < 23; ++i) {
+    double s = lane == i ? 1.0 : 0.0;
+#pragma unroll
+    for (int j = 0; j < i; ++j) s -= rl64(a[j], i) * w[j];
+    w[i] = s * rl64(dinv, i);
+  }
+#pragma unroll
+  for (int r = 0; r < 23; ++r) {
+    double s = rl64(w[0], r) * w[0];
+#pragma unroll
+    for (int k = 1; k < 23; ++k) s += rl64(w[k], r) * w[k];
+    if (lane < 23) out[r * 23 + lane] = s;
+  }
 }
 
 // ---- lane-parallel pieces of the state algebra (inputs in LDS) ----------
