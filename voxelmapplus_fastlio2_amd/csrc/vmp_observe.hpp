@@ -55,7 +55,7 @@ __device__ inline void two_sum(float a, float b, float& s, float& e) {
   e = (a - (s - bb)) + (b - bb);
 }
 
-// Wave-level fp64 sum (64 lanes)
+// Wave-level fp64 sum (64 lanes, result in every lane)
 __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -67,7 +67,53 @@ __device__ inline int wave_sum_i(int v) {
   return v;
 }
 
-__device__ inline int find_job(const ObsJob* jobs, int njobs, int blk) {
+// DPP lane moves within 16-lane rows (no LDS traffic): quad_perm [1,0,3,2]
+// (0xB1), quad_perm [2,3,0,1] (0x4E), row_half_mirror (0x141), row_mirror
+// (0x140).  Every source lane lies inside the row, so all lanes are written.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  return __hiloint2double(dpp_i32<CTRL>(__double2hiint(v)), dpp_i32<CTRL>(__double2loint(v)));
+}
+__device__ __forceinline__ double rl_f64(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+// 64-lane sums, valid in lane 0: two quad butterflies and the two mirrors
+// leave each 16-lane row's sum in all of its lanes (x + y and y + x round the
+// same); the row sums of lanes 0, 16, 32, 48 are added in that order.
+__device__ inline double wave_sum0(double v) {
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x141>(v);
+  v += dpp_f64<0x140>(v);
+  return ((rl_f64(v, 0) + rl_f64(v, 16)) + rl_f64(v, 32)) + rl_f64(v, 48);
+}
+__device__ inline int wave_sum0_i(int v) {
+  v += dpp_i32<0xB1>(v);
+  v += dpp_i32<0x4E>(v);
+  v += dpp_i32<0x141>(v);
+  v += dpp_i32<0x140>(v);
+  return ((__builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16)) + __builtin_amdgcn_readlane(v, 32)) +
+         __builtin_amdgcn_readlane(v, 48);
+}
+
+// Jobs and their parameter blocks do not change while a pass runs: reading
+// them through the constant address space lets the compiler use scalar loads
+// (one per workgroup, no per-lane vector loads inside the point loop).
+template <class T>
+using kconst = const __attribute__((address_space(4))) T;
+template <class T>
+__device__ __forceinline__ kconst<T>* as_const(const T* p) {
+  return (kconst<T>*)(p);
+}
+typedef float vmp_f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t vmp_u4 __attribute__((ext_vector_type(4)));
+
+__device__ inline int find_job(kconst<ObsJob>* jobs, int njobs, int blk) {
   int lo = 0, hi = njobs - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
@@ -77,12 +123,14 @@ __device__ inline int find_job(const ObsJob* jobs, int njobs, int blk) {
   return lo;
 }
 
+// Linear probe; each 16 B entry is read whole (key and id in one load).
 __device__ inline int32_t hash_find(const HashEntry* table, uint32_t mask, uint64_t key, int32_t* err) {
   uint32_t h = hash_slot(key, mask);
   for (uint32_t probe = 0; probe <= mask; ++probe) {
-    const uint4 e = *reinterpret_cast<const uint4*>(table + h);
+    const vmp_u4 e = *as_global(reinterpret_cast<const vmp_u4*>(table + h));
     const uint64_t k = uint64_t(e.x) | (uint64_t(e.y) << 32);
-    if (k == key) return int32_t(e.z);
+    const int32_t id = int32_t(e.z);
+    if (k == key) return id;
     if (k == kKeyEmpty) return -1;
     h = (h + 1) & mask;
   }
@@ -95,30 +143,38 @@ __device__ inline int32_t hash_find(const HashEntry* table, uint32_t mask, uint6
 // world mean); the fourth counter then counts keys that differ from the
 // fp64 key.  The registration path uses W32 = false.
 template <int NJ, bool W32 = false>
-__global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __restrict__ jobs, int njobs) {
+__global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __restrict__ jobs_in, int njobs) {
   constexpr int NH = NJ * (NJ + 1) / 2;
   constexpr int NACC = NH + NJ;
   __shared__ double red[kObsThreads / 64][NACC];
   __shared__ int redi[kObsThreads / 64][4];
 
+  kconst<ObsJob>* jobs = as_const(jobs_in);
   const int ji = find_job(jobs, njobs, blockIdx.x);
-  const ObsJob& J = jobs[ji];
-  const ObsParams& P = *J.prm;
+  kconst<ObsJob>& J = jobs[ji];
+  kconst<ObsParams>& P = *as_const(J.prm);
   if (!P.active) return;  // IESKF converged: reference leaves the loop
   const int lb = blockIdx.x - J.block0;
   if (lb >= J.nblocks) return;  // a grid rounded up past the last session's blocks
   const int tid = threadIdx.x;
+  const int n = J.n, ppt = J.ppt;
+  const auto X = as_global(J.x);
+  const auto Y = as_global(J.y);
+  const auto Z = as_global(J.z);
+  const auto SL = as_global(reinterpret_cast<vmp_f4*>(J.slots));
+  const auto HOT = as_global(reinterpret_cast<const vmp_f4*>(J.hot));
 
   double acc[NACC];
 #pragma unroll
   for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
   int c_hit = 0, c_plane = 0, c_eff = 0, c_fb = 0;
 
-  for (int k = 0; k < J.ppt; ++k) {
-    const int i = (lb * J.ppt + k) * kObsThreads + tid;
-    if (i >= J.n) break;
-    const float px = J.x[i], py = J.y[i];
-    float pz = J.z[i];
+  for (int k = 0; k < ppt; ++k) {
+    const int i = (lb * ppt + k) * kObsThreads + tid;
+    if (i >= n) break;
+    const float px = X[i], py = Y[i];
+    float pz = Z[i];
+    vmp_f4 slot = SL[i];
     const bool zfix = (pz == 0.0f);  // calcBodyCov mutation, commons.cpp:20-23
     if (zfix) pz = 0.001f;
     const float p[3] = {px, py, pz};
@@ -127,8 +183,7 @@ __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __re
     if constexpr (W32) {
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
-        const float* W = P.rwl32 + r * 3;
-        pw[r] = W[0] * px + W[1] * py + W[2] * pz + float(P.p_wl[r]);
+        pw[r] = P.rwl32[r * 3] * px + P.rwl32[r * 3 + 1] * py + P.rwl32[r * 3 + 2] * pz + float(P.p_wl[r]);
         key[r] = int64_t(floorf(pw[r] / P.vs));
       }
       const double pd[3] = {double(px), double(py), zfix ? 0.001 : double(pz)};
@@ -186,8 +241,7 @@ __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __re
         for (int r = 0; r < 3; ++r) key[r] = int64_t(kl[r]) + P.anchor[r];
       }
     }
-    float4 slot = J.slots[i];
-    bool in_range = key_ok(key[0]) && key_ok(key[1]) && key_ok(key[2]);
+    const bool in_range = key_ok(key[0]) && key_ok(key[1]) && key_ok(key[2]);
     const int32_t vid = in_range ? hash_find(J.table, J.hmask, pack_key(key[0], key[1], key[2]), J.counters + 4) : -1;
 
     const float r2 = px * px + py * py + pz * pz;
@@ -198,42 +252,42 @@ __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __re
     const float w2 = P.rwl32[6] * px + P.rwl32[7] * py + P.rwl32[8] * pz;
     if (vid >= 0) {
       ++c_hit;
-      const float4 h0 = *reinterpret_cast<const float4*>(J.hot + vid);
-      const float4 h1 = *(reinterpret_cast<const float4*>(J.hot + vid) + 1);
-      const uint32_t fl = __float_as_uint(h0.w);
-      if (fl & kPlane) {
-        ++c_plane;
-        const float nx = h1.x, ny = h1.y, nz = h1.z;
-        float res;
-        if constexpr (W32) {
-          // the plane mean held as an fp32 world point
-          const float mx = float(double(key[0]) * P.vs64 + double(h0.x));
-          const float my = float(double(key[1]) * P.vs64 + double(h0.y));
-          const float mz = float(double(key[2]) * P.vs64 + double(h0.z));
-          res = nx * (pw[0] - mx) + ny * (pw[1] - my) + nz * (pw[2] - mz);
-        } else {
-          const float qx = (uh[0] - kl[0] * P.vs) + ul[0];
-          const float qy = (uh[1] - kl[1] * P.vs) + ul[1];
-          const float qz = (uh[2] - kl[2] * P.vs) + ul[2];
-          res = nx * (qx - h0.x) + ny * (qy - h0.y) + nz * (qz - h0.z);
-        }
-        // sigma^2 = n^T (R C_l R^T + [p]x Prr [p]x^T + Ppp) n   (plane_cov term is 0)
-        const float nn = nx * nx + ny * ny + nz * nz;
-        const float nw = nx * w0 + ny * w1 + nz * w2;
-        const float cx = ny * pz - nz * py, cy = nz * px - nx * pz, cz = nx * py - ny * px;  // n x p
-        const float* A = P.Prr;
-        const float* B = P.Ppp;
-        const float qa = cx * (A[0] * cx + A[1] * cy + A[2] * cz) + cy * (A[1] * cx + A[3] * cy + A[4] * cz) +
-                         cz * (A[2] * cx + A[4] * cy + A[5] * cz);
-        const float qb = nx * (B[0] * nx + B[1] * ny + B[2] * nz) + ny * (B[1] * nx + B[3] * ny + B[4] * nz) +
-                         nz * (B[2] * nx + B[4] * ny + B[5] * nz);
-        const float sig2 = ca * nn + cb * nw * nw + qa + qb;
-        const bool valid = fabsf(res) < 3.0f * sqrtf(sig2);
-        slot = make_float4(nx, ny, nz, valid ? res : __int_as_float(0x7fc00000));
+      // both halves of the 32 B plane record are loaded together; the plane
+      // test below only selects the result
+      const vmp_f4 h0 = HOT[2 * size_t(vid)];
+      const vmp_f4 h1 = HOT[2 * size_t(vid) + 1];
+      const bool plane = (__float_as_uint(h0.w) & kPlane) != 0;
+      c_plane += plane ? 1 : 0;
+      const float nx = h1.x, ny = h1.y, nz = h1.z;
+      float res;
+      if constexpr (W32) {
+        // the plane mean held as an fp32 world point
+        const float mx = float(double(key[0]) * P.vs64 + double(h0.x));
+        const float my = float(double(key[1]) * P.vs64 + double(h0.y));
+        const float mz = float(double(key[2]) * P.vs64 + double(h0.z));
+        res = nx * (pw[0] - mx) + ny * (pw[1] - my) + nz * (pw[2] - mz);
       } else {
-        slot.w = __int_as_float(0x7fc00000);
+        const float qx = (uh[0] - kl[0] * P.vs) + ul[0];
+        const float qy = (uh[1] - kl[1] * P.vs) + ul[1];
+        const float qz = (uh[2] - kl[2] * P.vs) + ul[2];
+        res = nx * (qx - h0.x) + ny * (qy - h0.y) + nz * (qz - h0.z);
       }
-      J.slots[i] = slot;
+      // sigma^2 = n^T (R C_l R^T + [p]x Prr [p]x^T + Ppp) n   (plane_cov term is 0)
+      const float nn = nx * nx + ny * ny + nz * nz;
+      const float nw = nx * w0 + ny * w1 + nz * w2;
+      const float cx = ny * pz - nz * py, cy = nz * px - nx * pz, cz = nx * py - ny * px;  // n x p
+      kconst<float>* A = P.Prr;
+      kconst<float>* B = P.Ppp;
+      const float qa = cx * (A[0] * cx + A[1] * cy + A[2] * cz) + cy * (A[1] * cx + A[3] * cy + A[4] * cz) +
+                       cz * (A[2] * cx + A[4] * cy + A[5] * cz);
+      const float qb = nx * (B[0] * nx + B[1] * ny + B[2] * nz) + ny * (B[1] * nx + B[3] * ny + B[4] * nz) +
+                       nz * (B[2] * nx + B[4] * ny + B[5] * nz);
+      const float sig2 = ca * nn + cb * nw * nw + qa + qb;
+      const bool valid = fabsf(res) < 3.0f * sqrtf(sig2);
+      const float qnan = __int_as_float(0x7fc00000);
+      if (plane) slot = vmp_f4{nx, ny, nz, valid ? res : qnan};
+      else slot.w = qnan;
+      SL[i] = slot;
     }
     if (!isnan(slot.w)) {
       ++c_eff;
@@ -242,11 +296,11 @@ __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __re
       const float rc = ca * (nx * nx + ny * ny + nz * nz) + cb * nw * nw;
       const float wi = rc < 0.0002f ? 5000.0f : 1.0f / rc;
       // a' = rot^T n ; v = rot_ext p + pos_ext ; J_rot = v x a'
-      const float* Rt = P.rot;
+      kconst<float>* Rt = P.rot;
       const float ax = Rt[0] * nx + Rt[3] * ny + Rt[6] * nz;
       const float ay = Rt[1] * nx + Rt[4] * ny + Rt[7] * nz;
       const float az = Rt[2] * nx + Rt[5] * ny + Rt[8] * nz;
-      const float* E = P.ext_R;
+      kconst<float>* E = P.ext_R;
       const float vx = E[0] * px + E[1] * py + E[2] * pz + P.ext_t[0];
       const float vy = E[3] * px + E[4] * py + E[5] * pz + P.ext_t[1];
       const float vz = E[6] * px + E[7] * py + E[8] * pz + P.ext_t[2];
@@ -258,7 +312,7 @@ __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __re
       Jv[4] = vz * ax - vx * az;
       Jv[5] = vx * ay - vy * ax;
       if constexpr (NJ == 12) {
-        const float* W = P.rwl32;
+        kconst<float>* W = P.rwl32;
         const float bx = W[0] * nx + W[3] * ny + W[6] * nz;
         const float by = W[1] * nx + W[4] * ny + W[7] * nz;
         const float bz = W[2] * nx + W[5] * ny + W[8] * nz;
@@ -289,13 +343,13 @@ __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __re
   const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
   for (int k = 0; k < NACC; ++k) {
-    const double s = wave_sum(acc[k]);
+    const double s = wave_sum0(acc[k]);
     if (lane == 0) red[wv][k] = s;
   }
-  c_hit = wave_sum_i(c_hit);
-  c_plane = wave_sum_i(c_plane);
-  c_eff = wave_sum_i(c_eff);
-  c_fb = wave_sum_i(c_fb);
+  c_hit = wave_sum0_i(c_hit);
+  c_plane = wave_sum0_i(c_plane);
+  c_eff = wave_sum0_i(c_eff);
+  c_fb = wave_sum0_i(c_fb);
   if (lane == 0) {
     redi[wv][0] = c_hit;
     redi[wv][1] = c_plane;
@@ -303,7 +357,7 @@ __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __re
     redi[wv][3] = c_fb;
   }
   __syncthreads();
-  double* row = J.partials + size_t(lb) * obs_row<NJ>();
+  const auto row = as_global(J.partials + size_t(lb) * obs_row<NJ>());
   if (tid < NACC) {
     double s = red[0][tid];
     for (int w = 1; w < kObsThreads / 64; ++w) s += red[w][tid];
