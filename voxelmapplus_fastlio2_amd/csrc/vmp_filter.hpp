@@ -73,7 +73,8 @@ __global__ void __launch_bounds__(64) k_ieskf_begin(FilterDev** fs, int32_t** ob
 #pragma unroll
   for (int c = 0; c < 23; ++c) a[c] = lane < 23 ? f->P[c * 23 + lane] : 0.0;
   double dinv;
-  chol_rows(a, dinv, lane);
+  __shared__ double col[23];
+  chol_rows(a, dinv, lane, col);
   chol_inverse_rows(a, dinv, f->Pinv, lane);  // P^-1
   __syncthreads();
   load_state(sx, f->xp, lane);
@@ -120,7 +121,7 @@ __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, const ObsJob*
     }
 #pragma unroll
     for (int k = 0; k < ROW; ++k) {
-      const double v = wave_sum(fin[k]);
+      const double v = wave_sum0(fin[k]);
       if (lane == 0) red[k] = v;
     }
   }
@@ -207,8 +208,9 @@ __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, const ObsJob*
     br = s;
   }
   double dinv;
-  chol_rows(a, dinv, lane);
-  const double xr = chol_solve_rows(a, dinv, br, lane);  // delta = -H_^-1 b_
+  __syncthreads();  // Pi is no longer read: its LDS stages L from here on
+  chol_rows(a, dinv, lane, Pi);
+  const double xr = chol_solve_rows(a, dinv, br, lane, Pi);  // delta = -H_^-1 b_
   if (lane < 23) {
 #pragma unroll
     for (int c = 0; c < 23; ++c) f->Hinv[lane * 23 + c] = a[c];  // Cholesky factor of the last H_

@@ -241,8 +241,11 @@ __device__ inline double rl64(double v, int src) {
   const int hi = __builtin_amdgcn_readlane(int(x >> 32), src);
   return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
-// in place: row i becomes L[i][0..i], its upper part untouched; dinv = 1/L[i][i]
-__device__ inline void chol_rows(double (&a)[23], double& dinv, int lane) {
+// in place: row i becomes L[i][0..i], its upper part untouched; dinv = 1/L[i][i].
+// Column k of L, once scaled, is staged in col (23 doubles of LDS) and read
+// back by every lane as broadcasts.  Called by all threads of a 64-thread
+// workgroup (it synchronises the workgroup).
+__device__ inline void chol_rows(double (&a)[23], double& dinv, int lane, double* col) {
   dinv = 0.0;
 #pragma unroll
   for (int k = 0; k < 23; ++k) {
@@ -254,15 +257,21 @@ __device__ inline void chol_rows(double (&a)[23], double& dinv, int lane) {
     } else if (lane > k) {
       a[k] = a[k] * inv;
     }
+    if (lane < 23) col[lane] = a[k];  // L[lane][k] for lanes below the diagonal
+    __syncthreads();
 #pragma unroll
     for (int j = k + 1; j < 23; ++j) {
-      const double ljk = rl64(a[k], j);
+      const double ljk = col[j];
       if (lane >= j) a[j] -= a[k] * ljk;
     }
+    __syncthreads();  // col is rewritten by the next column
   }
 }
-// x = A^-1 b for A = L L^T from chol_rows; b and x one element per lane
-__device__ inline double chol_solve_rows(const double (&a)[23], double dinv, double b, int lane) {
+// x = A^-1 b for A = L L^T from chol_rows; b and x one element per lane.
+// Forward: L y = b by rows; backward: L^T x = y by columns, lane j taking
+// L[i][j] x_i off y_j as each x_i is found (L staged row-major in Ls, 529
+// doubles of LDS).  Called by all threads of a 64-thread workgroup.
+__device__ inline double chol_solve_rows(const double (&a)[23], double dinv, double b, int lane, double* Ls) {
   double y = b;
 #pragma unroll
   for (int j = 0; j < 23; ++j) {  // L y = b
@@ -270,16 +279,19 @@ __device__ inline double chol_solve_rows(const double (&a)[23], double dinv, dou
     const double yj = rl64(y, j);
     if (lane > j) y -= a[j] * yj;
   }
-  double x = y;
+  if (lane < 23) {
+#pragma unroll
+    for (int c = 0; c < 23; ++c) Ls[lane * 23 + c] = a[c];
+  }
+  __syncthreads();
 #pragma unroll
   for (int i = 22; i >= 0; --i) {  // L^T x = y
-    double s = rl64(y, i);
-#pragma unroll
-    for (int j = i + 1; j < 23; ++j) s -= rl64(a[i], j) * rl64(x, j);
-    const double xi = s * rl64(dinv, i);
-    if (lane == i) x = xi;
+    const double xi = rl64(y, i) * rl64(dinv, i);
+    if (lane == i) y = xi;
+    else if (lane < i) y -= Ls[i * 23 + lane] * xi;
   }
-  return x;
+  __syncthreads();  // Ls may be reused by the caller
+  return y;
 }
 
 // A^-1 for A = L L^T from chol_rows (rows of L in lanes, dinv = 1/L[i][i]):

@@ -22,6 +22,7 @@
 namespace vmp {
 
 constexpr int kObsThreads = 256;
+constexpr int kObsGroup = 2;  // points per lane in flight together
 
 // length of one block's partial row: the H upper triangle, b, four counters
 template <int NJ>
@@ -111,7 +112,9 @@ __device__ __forceinline__ kconst<T>* as_const(const T* p) {
   return (kconst<T>*)(p);
 }
 typedef float vmp_f4 __attribute__((ext_vector_type(4)));
-typedef uint32_t vmp_u4 __attribute__((ext_vector_type(4)));
+typedef float vmp_f3 __attribute__((ext_vector_type(3)));
+typedef uint32_t vmp_u3 __attribute__((ext_vector_type(3)));
+static_assert(sizeof(vmp_f3) == 16 && sizeof(vmp_u3) == 16, "3-vectors step by 16 bytes");
 
 __device__ inline int find_job(kconst<ObsJob>* jobs, int njobs, int blk) {
   int lo = 0, hi = njobs - 1;
@@ -123,16 +126,17 @@ __device__ inline int find_job(kconst<ObsJob>* jobs, int njobs, int blk) {
   return lo;
 }
 
-// Linear probe; each 16 B entry is read whole (key and id in one load).
-__device__ inline int32_t hash_find(const HashEntry* table, uint32_t mask, uint64_t key, int32_t* err) {
-  uint32_t h = hash_slot(key, mask);
-  for (uint32_t probe = 0; probe <= mask; ++probe) {
-    const vmp_u4 e = *as_global(reinterpret_cast<const vmp_u4*>(table + h));
+// Linear probe from slot h, whose entry e was already read; each further
+// entry's key and id are read in one load.
+__device__ inline int32_t hash_resolve(const HashEntry* table, uint32_t mask, uint64_t key, uint32_t h, vmp_u3 e,
+                                       int32_t* err) {
+  for (uint32_t probe = 0;; ++probe) {
     const uint64_t k = uint64_t(e.x) | (uint64_t(e.y) << 32);
-    const int32_t id = int32_t(e.z);
-    if (k == key) return id;
+    if (k == key) return int32_t(e.z);
     if (k == kKeyEmpty) return -1;
+    if (probe == mask) break;
     h = (h + 1) & mask;
+    e = *as_global(reinterpret_cast<const vmp_u3*>(table + h));
   }
   if (err) atomicOr(err, 1);
   return -1;
@@ -169,169 +173,214 @@ __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __re
   for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
   int c_hit = 0, c_plane = 0, c_eff = 0, c_fb = 0;
 
-  for (int k = 0; k < ppt; ++k) {
-    const int i = (lb * ppt + k) * kObsThreads + tid;
-    if (i >= n) break;
-    const float px = X[i], py = Y[i];
-    float pz = Z[i];
-    vmp_f4 slot = SL[i];
-    const bool zfix = (pz == 0.0f);  // calcBodyCov mutation, commons.cpp:20-23
-    if (zfix) pz = 0.001f;
-    const float p[3] = {px, py, pz};
-    float uh[3], ul[3], kl[3], pw[3];
-    int64_t key[3];
-    if constexpr (W32) {
+  // kObsGroup points per lane are in flight together: their loads, their
+  // hash probes and their plane gathers are each issued as one round, then
+  // the points are finished (slot store, accumulation) in index order.
+  for (int k0 = 0; k0 < ppt; k0 += kObsGroup) {
+    int idx[kObsGroup];
+    bool live[kObsGroup];
+    float px[kObsGroup], py[kObsGroup], pz[kObsGroup];
+    bool zfix[kObsGroup];
+    vmp_f4 slot[kObsGroup];
 #pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        pw[r] = P.rwl32[r * 3] * px + P.rwl32[r * 3 + 1] * py + P.rwl32[r * 3 + 2] * pz + float(P.p_wl[r]);
-        key[r] = int64_t(floorf(pw[r] / P.vs));
+    for (int g = 0; g < kObsGroup; ++g) {
+      idx[g] = (lb * ppt + k0 + g) * kObsThreads + tid;
+      live[g] = k0 + g < ppt && idx[g] < n;
+      px[g] = py[g] = pz[g] = 0.0f;
+      slot[g] = vmp_f4{0.0f, 0.0f, 0.0f, 0.0f};
+      if (live[g]) {
+        px[g] = X[idx[g]];
+        py[g] = Y[idx[g]];
+        pz[g] = Z[idx[g]];
+        slot[g] = SL[idx[g]];
       }
-      const double pd[3] = {double(px), double(py), zfix ? 0.001 : double(pz)};
-      bool differs = false;
+    }
+    if (!live[0]) break;
+    float uh[kObsGroup][3], ul[kObsGroup][3], kl[kObsGroup][3], pw[kObsGroup][3];
+    int64_t key[kObsGroup][3];
+    uint64_t pk[kObsGroup];
+    uint32_t hs[kObsGroup];
+    vmp_u3 he[kObsGroup];
 #pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        double s = P.r_wl[r * 3] * pd[0];
-        s += P.r_wl[r * 3 + 1] * pd[1];
-        s += P.r_wl[r * 3 + 2] * pd[2];
-        differs |= int64_t(floor((s + P.p_wl[r]) / P.vs64)) != key[r];
-      }
-      c_fb += differs ? 1 : 0;
-    } else {
-      // ---- compensated transform into the anchor frame
+    for (int g = 0; g < kObsGroup; ++g) {
+      zfix[g] = (pz[g] == 0.0f);  // calcBodyCov mutation, commons.cpp:20-23
+      if (zfix[g]) pz[g] = 0.001f;
+      const float p[3] = {px[g], py[g], pz[g]};
+      if constexpr (W32) {
 #pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        float s = P.th[r], c = P.tl[r];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          const float hi = P.Rh[r * 3 + q];
-          const float prod = hi * p[q];
-          const float perr = __builtin_fmaf(hi, p[q], -prod);
-          float s2, e2;
-          two_sum(s, prod, s2, e2);
-          s = s2;
-          c += perr + e2 + P.Rl[r * 3 + q] * p[q];
+        for (int r = 0; r < 3; ++r) {
+          pw[g][r] =
+              P.rwl32[r * 3] * p[0] + P.rwl32[r * 3 + 1] * p[1] + P.rwl32[r * 3 + 2] * p[2] + float(P.p_wl[r]);
+          key[g][r] = int64_t(floorf(pw[g][r] / P.vs));
         }
-        float h2, l2;
-        two_sum(s, c, h2, l2);
-        uh[r] = h2;
-        ul[r] = l2;
-      }
-      // ---- key (bit-exact with the reference's fp64 floor(p_w / vs))
-      bool near_face = false;
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        const float f = (uh[r] + ul[r]) * P.inv_vs;
-        kl[r] = floorf(f);
-        const float fr = f - kl[r];
-        near_face |= (fr < 1e-4f) || (fr > 1.0f - 1e-4f);
-      }
-      if (near_face) {
-        ++c_fb;
-        const double pd[3] = {double(px), double(py), zfix ? 0.001 : double(pz)};
+        const double pd[3] = {double(p[0]), double(p[1]), zfix[g] ? 0.001 : double(p[2])};
+        bool differs = false;
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
           double s = P.r_wl[r * 3] * pd[0];
           s += P.r_wl[r * 3 + 1] * pd[1];
           s += P.r_wl[r * 3 + 2] * pd[2];
-          key[r] = int64_t(floor((s + P.p_wl[r]) / P.vs64));
-          kl[r] = float(key[r] - P.anchor[r]);
+          differs |= int64_t(floor((s + P.p_wl[r]) / P.vs64)) != key[g][r];
         }
+        c_fb += (live[g] && differs) ? 1 : 0;
       } else {
+        // ---- compensated transform into the anchor frame
 #pragma unroll
-        for (int r = 0; r < 3; ++r) key[r] = int64_t(kl[r]) + P.anchor[r];
+        for (int r = 0; r < 3; ++r) {
+          float s = P.th[r], c = P.tl[r];
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            const float hi = P.Rh[r * 3 + q];
+            const float prod = hi * p[q];
+            const float perr = __builtin_fmaf(hi, p[q], -prod);
+            float s2, e2;
+            two_sum(s, prod, s2, e2);
+            s = s2;
+            c += perr + e2 + P.Rl[r * 3 + q] * p[q];
+          }
+          float h2, l2;
+          two_sum(s, c, h2, l2);
+          uh[g][r] = h2;
+          ul[g][r] = l2;
+        }
+        // ---- key (bit-exact with the reference's fp64 floor(p_w / vs))
+        bool near_face = false;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          const float f = (uh[g][r] + ul[g][r]) * P.inv_vs;
+          kl[g][r] = floorf(f);
+          const float fr = f - kl[g][r];
+          near_face |= (fr < 1e-4f) || (fr > 1.0f - 1e-4f);
+        }
+        if (live[g] && near_face) {
+          ++c_fb;
+          const double pd[3] = {double(p[0]), double(p[1]), zfix[g] ? 0.001 : double(p[2])};
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            double s = P.r_wl[r * 3] * pd[0];
+            s += P.r_wl[r * 3 + 1] * pd[1];
+            s += P.r_wl[r * 3 + 2] * pd[2];
+            key[g][r] = int64_t(floor((s + P.p_wl[r]) / P.vs64));
+            kl[g][r] = float(key[g][r] - P.anchor[r]);
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < 3; ++r) key[g][r] = int64_t(kl[g][r]) + P.anchor[r];
+        }
       }
+      // first probe of the voxel's hash slot
+      const bool in_range = live[g] && key_ok(key[g][0]) && key_ok(key[g][1]) && key_ok(key[g][2]);
+      pk[g] = in_range ? pack_key(key[g][0], key[g][1], key[g][2]) : kKeyEmpty;
+      // (read unconditionally: slot 0 stands in for points without a key, so
+      // the loads stay in flight together; every load's whole destination
+      // is used, so no register is reused while it is pending)
+      hs[g] = in_range ? hash_slot(pk[g], J.hmask) : 0u;
+      he[g] = *as_global(reinterpret_cast<const vmp_u3*>(J.table + hs[g]));
     }
-    const bool in_range = key_ok(key[0]) && key_ok(key[1]) && key_ok(key[2]);
-    const int32_t vid = in_range ? hash_find(J.table, J.hmask, pack_key(key[0], key[1], key[2]), J.counters + 4) : -1;
-
-    const float r2 = px * px + py * py + pz * pz;
-    const float ca = r2 * P.s2;
-    const float cb = (P.rv - ca) / r2;
-    const float w0 = P.rwl32[0] * px + P.rwl32[1] * py + P.rwl32[2] * pz;
-    const float w1 = P.rwl32[3] * px + P.rwl32[4] * py + P.rwl32[5] * pz;
-    const float w2 = P.rwl32[6] * px + P.rwl32[7] * py + P.rwl32[8] * pz;
-    if (vid >= 0) {
-      ++c_hit;
-      // both halves of the 32 B plane record are loaded together; the plane
-      // test below only selects the result
-      const vmp_f4 h0 = HOT[2 * size_t(vid)];
-      const vmp_f4 h1 = HOT[2 * size_t(vid) + 1];
-      const bool plane = (__float_as_uint(h0.w) & kPlane) != 0;
-      c_plane += plane ? 1 : 0;
-      const float nx = h1.x, ny = h1.y, nz = h1.z;
-      float res;
-      if constexpr (W32) {
-        // the plane mean held as an fp32 world point
-        const float mx = float(double(key[0]) * P.vs64 + double(h0.x));
-        const float my = float(double(key[1]) * P.vs64 + double(h0.y));
-        const float mz = float(double(key[2]) * P.vs64 + double(h0.z));
-        res = nx * (pw[0] - mx) + ny * (pw[1] - my) + nz * (pw[2] - mz);
-      } else {
-        const float qx = (uh[0] - kl[0] * P.vs) + ul[0];
-        const float qy = (uh[1] - kl[1] * P.vs) + ul[1];
-        const float qz = (uh[2] - kl[2] * P.vs) + ul[2];
-        res = nx * (qx - h0.x) + ny * (qy - h0.y) + nz * (qz - h0.z);
-      }
-      // sigma^2 = n^T (R C_l R^T + [p]x Prr [p]x^T + Ppp) n   (plane_cov term is 0)
-      const float nn = nx * nx + ny * ny + nz * nz;
-      const float nw = nx * w0 + ny * w1 + nz * w2;
-      const float cx = ny * pz - nz * py, cy = nz * px - nx * pz, cz = nx * py - ny * px;  // n x p
-      kconst<float>* A = P.Prr;
-      kconst<float>* B = P.Ppp;
-      const float qa = cx * (A[0] * cx + A[1] * cy + A[2] * cz) + cy * (A[1] * cx + A[3] * cy + A[4] * cz) +
-                       cz * (A[2] * cx + A[4] * cy + A[5] * cz);
-      const float qb = nx * (B[0] * nx + B[1] * ny + B[2] * nz) + ny * (B[1] * nx + B[3] * ny + B[4] * nz) +
-                       nz * (B[2] * nx + B[4] * ny + B[5] * nz);
-      const float sig2 = ca * nn + cb * nw * nw + qa + qb;
-      const bool valid = fabsf(res) < 3.0f * sqrtf(sig2);
-      const float qnan = __int_as_float(0x7fc00000);
-      if (plane) slot = vmp_f4{nx, ny, nz, valid ? res : qnan};
-      else slot.w = qnan;
-      SL[i] = slot;
+    int32_t vid[kObsGroup];
+    vmp_f4 h0[kObsGroup];
+    vmp_f3 h1[kObsGroup];
+#pragma unroll
+    for (int g = 0; g < kObsGroup; ++g)
+      vid[g] = pk[g] == kKeyEmpty ? -1 : hash_resolve(J.table, J.hmask, pk[g], hs[g], he[g], J.counters + 4);
+#pragma unroll
+    for (int g = 0; g < kObsGroup; ++g) {
+      // both halves of the 32 B plane record are loaded together (record 0
+      // stands in on a miss); the plane test below only selects the result
+      const size_t hv = vid[g] >= 0 ? size_t(vid[g]) : 0;
+      h0[g] = HOT[2 * hv];
+      h1[g] = *as_global(reinterpret_cast<const vmp_f3*>(J.hot) + 2 * hv + 1);
     }
-    if (!isnan(slot.w)) {
-      ++c_eff;
-      const float nx = slot.x, ny = slot.y, nz = slot.z, res = slot.w;
-      const float nw = nx * w0 + ny * w1 + nz * w2;
-      const float rc = ca * (nx * nx + ny * ny + nz * nz) + cb * nw * nw;
-      const float wi = rc < 0.0002f ? 5000.0f : 1.0f / rc;
-      // a' = rot^T n ; v = rot_ext p + pos_ext ; J_rot = v x a'
-      kconst<float>* Rt = P.rot;
-      const float ax = Rt[0] * nx + Rt[3] * ny + Rt[6] * nz;
-      const float ay = Rt[1] * nx + Rt[4] * ny + Rt[7] * nz;
-      const float az = Rt[2] * nx + Rt[5] * ny + Rt[8] * nz;
-      kconst<float>* E = P.ext_R;
-      const float vx = E[0] * px + E[1] * py + E[2] * pz + P.ext_t[0];
-      const float vy = E[3] * px + E[4] * py + E[5] * pz + P.ext_t[1];
-      const float vz = E[6] * px + E[7] * py + E[8] * pz + P.ext_t[2];
-      float Jv[NJ];
-      Jv[0] = nx;
-      Jv[1] = ny;
-      Jv[2] = nz;
-      Jv[3] = vy * az - vz * ay;
-      Jv[4] = vz * ax - vx * az;
-      Jv[5] = vx * ay - vy * ax;
-      if constexpr (NJ == 12) {
-        kconst<float>* W = P.rwl32;
-        const float bx = W[0] * nx + W[3] * ny + W[6] * nz;
-        const float by = W[1] * nx + W[4] * ny + W[7] * nz;
-        const float bz = W[2] * nx + W[5] * ny + W[8] * nz;
-        Jv[6] = py * bz - pz * by;
-        Jv[7] = pz * bx - px * bz;
-        Jv[8] = px * by - py * bx;
-        Jv[9] = ax;
-        Jv[10] = ay;
-        Jv[11] = az;
+#pragma unroll
+    for (int g = 0; g < kObsGroup; ++g) {
+      if (!live[g]) continue;
+      const float qx0 = px[g], qy0 = py[g], qz0 = pz[g];
+      const float r2 = qx0 * qx0 + qy0 * qy0 + qz0 * qz0;
+      const float ca = r2 * P.s2;
+      const float cb = (P.rv - ca) / r2;
+      const float w0 = P.rwl32[0] * qx0 + P.rwl32[1] * qy0 + P.rwl32[2] * qz0;
+      const float w1 = P.rwl32[3] * qx0 + P.rwl32[4] * qy0 + P.rwl32[5] * qz0;
+      const float w2 = P.rwl32[6] * qx0 + P.rwl32[7] * qy0 + P.rwl32[8] * qz0;
+      vmp_f4 sl = slot[g];
+      if (vid[g] >= 0) {
+        ++c_hit;
+        const bool plane = (__float_as_uint(h0[g].w) & kPlane) != 0;
+        c_plane += plane ? 1 : 0;
+        const float nx = h1[g].x, ny = h1[g].y, nz = h1[g].z;
+        float res;
+        if constexpr (W32) {
+          // the plane mean held as an fp32 world point
+          const float mx = float(double(key[g][0]) * P.vs64 + double(h0[g].x));
+          const float my = float(double(key[g][1]) * P.vs64 + double(h0[g].y));
+          const float mz = float(double(key[g][2]) * P.vs64 + double(h0[g].z));
+          res = nx * (pw[g][0] - mx) + ny * (pw[g][1] - my) + nz * (pw[g][2] - mz);
+        } else {
+          const float qx = (uh[g][0] - kl[g][0] * P.vs) + ul[g][0];
+          const float qy = (uh[g][1] - kl[g][1] * P.vs) + ul[g][1];
+          const float qz = (uh[g][2] - kl[g][2] * P.vs) + ul[g][2];
+          res = nx * (qx - h0[g].x) + ny * (qy - h0[g].y) + nz * (qz - h0[g].z);
+        }
+        // sigma^2 = n^T (R C_l R^T + [p]x Prr [p]x^T + Ppp) n   (plane_cov term is 0)
+        const float nn = nx * nx + ny * ny + nz * nz;
+        const float nw = nx * w0 + ny * w1 + nz * w2;
+        const float cx = ny * qz0 - nz * qy0, cy = nz * qx0 - nx * qz0, cz = nx * qy0 - ny * qx0;  // n x p
+        kconst<float>* A = P.Prr;
+        kconst<float>* B = P.Ppp;
+        const float qa = cx * (A[0] * cx + A[1] * cy + A[2] * cz) + cy * (A[1] * cx + A[3] * cy + A[4] * cz) +
+                         cz * (A[2] * cx + A[4] * cy + A[5] * cz);
+        const float qb = nx * (B[0] * nx + B[1] * ny + B[2] * nz) + ny * (B[1] * nx + B[3] * ny + B[4] * nz) +
+                         nz * (B[2] * nx + B[4] * ny + B[5] * nz);
+        const float sig2 = ca * nn + cb * nw * nw + qa + qb;
+        const bool valid = fabsf(res) < 3.0f * sqrtf(sig2);
+        const float qnan = __int_as_float(0x7fc00000);
+        if (plane) sl = vmp_f4{nx, ny, nz, valid ? res : qnan};
+        else sl.w = qnan;
+        SL[idx[g]] = sl;
       }
-      int o = 0;
+      if (!isnan(sl.w)) {
+        ++c_eff;
+        const float nx = sl.x, ny = sl.y, nz = sl.z, res = sl.w;
+        const float nw = nx * w0 + ny * w1 + nz * w2;
+        const float rc = ca * (nx * nx + ny * ny + nz * nz) + cb * nw * nw;
+        const float wi = rc < 0.0002f ? 5000.0f : 1.0f / rc;
+        // a' = rot^T n ; v = rot_ext p + pos_ext ; J_rot = v x a'
+        kconst<float>* Rt = P.rot;
+        const float ax = Rt[0] * nx + Rt[3] * ny + Rt[6] * nz;
+        const float ay = Rt[1] * nx + Rt[4] * ny + Rt[7] * nz;
+        const float az = Rt[2] * nx + Rt[5] * ny + Rt[8] * nz;
+        kconst<float>* E = P.ext_R;
+        const float vx = E[0] * qx0 + E[1] * qy0 + E[2] * qz0 + P.ext_t[0];
+        const float vy = E[3] * qx0 + E[4] * qy0 + E[5] * qz0 + P.ext_t[1];
+        const float vz = E[6] * qx0 + E[7] * qy0 + E[8] * qz0 + P.ext_t[2];
+        float Jv[NJ];
+        Jv[0] = nx;
+        Jv[1] = ny;
+        Jv[2] = nz;
+        Jv[3] = vy * az - vz * ay;
+        Jv[4] = vz * ax - vx * az;
+        Jv[5] = vx * ay - vy * ax;
+        if constexpr (NJ == 12) {
+          kconst<float>* W = P.rwl32;
+          const float bx = W[0] * nx + W[3] * ny + W[6] * nz;
+          const float by = W[1] * nx + W[4] * ny + W[7] * nz;
+          const float bz = W[2] * nx + W[5] * ny + W[8] * nz;
+          Jv[6] = qy0 * bz - qz0 * by;
+          Jv[7] = qz0 * bx - qx0 * bz;
+          Jv[8] = qx0 * by - qy0 * bx;
+          Jv[9] = ax;
+          Jv[10] = ay;
+          Jv[11] = az;
+        }
+        int o = 0;
 #pragma unroll
-      for (int a = 0; a < NJ; ++a) {
-        const double wa = double(wi) * double(Jv[a]);
+        for (int a = 0; a < NJ; ++a) {
+          const double wa = double(wi) * double(Jv[a]);
 #pragma unroll
-        for (int b2 = a; b2 < NJ; ++b2) acc[o++] += wa * double(Jv[b2]);
+          for (int b2 = a; b2 < NJ; ++b2) acc[o++] += wa * double(Jv[b2]);
+        }
+#pragma unroll
+        for (int a = 0; a < NJ; ++a) acc[NH + a] += double(wi) * double(Jv[a]) * double(res);
       }
-#pragma unroll
-      for (int a = 0; a < NJ; ++a) acc[NH + a] += double(wi) * double(Jv[a]) * double(res);
     }
   }
 
