@@ -73,9 +73,9 @@ __global__ void __launch_bounds__(64) k_ieskf_begin(FilterDev** fs, int32_t** ob
 #pragma unroll
   for (int c = 0; c < 23; ++c) a[c] = lane < 23 ? f->P[c * 23 + lane] : 0.0;
   double dinv;
-  __shared__ double col[23];
-  chol_rows(a, dinv, lane, col);
-  chol_inverse_rows(a, dinv, f->Pinv, lane);  // P^-1
+  __shared__ double S[23 * 24];
+  chol_rows(a, dinv, lane, S);
+  chol_inverse_rows(a, dinv, f->Pinv, lane, S);  // P^-1
   __syncthreads();
   load_state(sx, f->xp, lane);
   if (lane == 0) {
@@ -262,7 +262,8 @@ __global__ void __launch_bounds__(64) k_ieskf_end(FilterDev** fs) {
   for (int c = 0; c < 23; ++c)
     if (lane == c) d = a[c];
   if (lane < 23) dl[lane] = f->delta[lane];
-  chol_inverse_rows(a, 1.0 / d, Hi, lane);  // H_^-1
+  __shared__ double S[23 * 24];
+  chol_inverse_rows(a, 1.0 / d, Hi, lane, S);  // H_^-1
   __syncthreads();
   jac_blocks_par(sx, sxp, dl, jb, lane);
   // P = L H_^-1 L^T with L = J; J is the identity outside three small

@@ -298,22 +298,37 @@ __device__ inline double chol_solve_rows(const double (&a)[23], double dinv, dou
 // lane c forms column c of W = L^-1 by forward substitution, then
 // A^-1 = W^T W, whose row r is written by lane c into out[r * 23 + c]
 // (bitwise symmetric: both halves sum the same products in the same order).
-__device__ inline void chol_inverse_rows(const double (&a)[23], double dinv, double* out, int lane) {
+// L, 1/L[i][i] and then W are staged in S (23 * 24 doubles of LDS) and read
+// back as broadcasts.  Called by all threads of a 64-thread workgroup.
+__device__ inline void chol_inverse_rows(const double (&a)[23], double dinv, double* out, int lane, double* S) {
+  if (lane < 23) {
+#pragma unroll
+    for (int c = 0; c < 23; ++c) S[lane * 24 + c] = a[c];
+    S[lane * 24 + 23] = dinv;
+  }
+  __syncthreads();
   double w[23];
 #pragma unroll
   for (int i = 0; i < 23; ++i) {
     double s = lane == i ? 1.0 : 0.0;
 #pragma unroll
-    for (int j = 0; j < i; ++j) s -= rl64(a[j], i) * w[j];
-    w[i] = s * rl64(dinv, i);
+    for (int j = 0; j < i; ++j) s -= S[i * 24 + j] * w[j];
+    w[i] = s * S[i * 24 + 23];
   }
+  __syncthreads();
+  if (lane < 23) {
+#pragma unroll
+    for (int k = 0; k < 23; ++k) S[k * 24 + lane] = w[k];
+  }
+  __syncthreads();
 #pragma unroll
   for (int r = 0; r < 23; ++r) {
-    double s = rl64(w[0], r) * w[0];
+    double s = S[r] * w[0];
 #pragma unroll
-    for (int k = 1; k < 23; ++k) s += rl64(w[k], r) * w[k];
+    for (int k = 1; k < 23; ++k) s += S[k * 24 + r] * w[k];
     if (lane < 23) out[r * 23 + lane] = s;
   }
+  __syncthreads();  // S may be reused by the caller
 }
 
 // ---- lane-parallel pieces of the state algebra (inputs in LDS) ----------
