@@ -38,7 +38,8 @@ HBM_PEAK_GBS = 8000.0
 OBS_BYTES_POINT = 44
 OBS_BYTES_HIT = 48
 # observe launches timed back to back per event pair in the roofline pass
-OBS_TIME_REPS = 8
+# (the event pair and the first dispatch add a fixed ~15 us, spread over them)
+OBS_TIME_REPS = 32
 
 
 def log(*a):

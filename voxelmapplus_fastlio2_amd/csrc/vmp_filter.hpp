@@ -73,9 +73,10 @@ __global__ void __launch_bounds__(64) k_ieskf_begin(FilterDev** fs, int32_t** ob
 #pragma unroll
   for (int c = 0; c < 23; ++c) a[c] = lane < 23 ? f->P[c * 23 + lane] : 0.0;
   double dinv;
-  __shared__ double S[23 * 24];
-  chol_rows(a, dinv, lane, S);
-  chol_inverse_rows(a, dinv, f->Pinv, lane, S);  // P^-1
+  __shared__ double S[23 * 24], Pi[529];
+  chol_rows(a, dinv, lane);
+  chol_inverse_rows(a, dinv, Pi, lane, S);  // P^-1
+  for (int i = lane; i < 529; i += 64) as_global(f->Pinv)[i] = Pi[i];
   __syncthreads();
   load_state(sx, f->xp, lane);
   if (lane == 0) {
@@ -209,7 +210,7 @@ __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, const ObsJob*
   }
   double dinv;
   __syncthreads();  // Pi is no longer read: its LDS stages L from here on
-  chol_rows(a, dinv, lane, Pi);
+  chol_rows(a, dinv, lane);
   const double xr = chol_solve_rows(a, dinv, br, lane, Pi);  // delta = -H_^-1 b_
   if (lane < 23) {
 #pragma unroll

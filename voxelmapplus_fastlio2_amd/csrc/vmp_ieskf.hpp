@@ -241,11 +241,10 @@ __device__ inline double rl64(double v, int src) {
   const int hi = __builtin_amdgcn_readlane(int(x >> 32), src);
   return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
-// in place: row i becomes L[i][0..i], its upper part untouched; dinv = 1/L[i][i].
-// Column k of L, once scaled, is staged in col (23 doubles of LDS) and read
-// back by every lane as broadcasts.  Called by all threads of a 64-thread
-// workgroup (it synchronises the workgroup).
-__device__ inline void chol_rows(double (&a)[23], double& dinv, int lane, double* col) {
+// in place: row i becomes L[i][0..i], its upper part untouched; dinv = 1/L[i][i]
+// (column k of L is read by readlane as it is used: measured faster here
+// than staging each column through LDS)
+__device__ inline void chol_rows(double (&a)[23], double& dinv, int lane) {
   dinv = 0.0;
 #pragma unroll
   for (int k = 0; k < 23; ++k) {
@@ -257,14 +256,11 @@ __device__ inline void chol_rows(double (&a)[23], double& dinv, int lane, double
     } else if (lane > k) {
       a[k] = a[k] * inv;
     }
-    if (lane < 23) col[lane] = a[k];  // L[lane][k] for lanes below the diagonal
-    __syncthreads();
 #pragma unroll
     for (int j = k + 1; j < 23; ++j) {
-      const double ljk = col[j];
+      const double ljk = rl64(a[k], j);
       if (lane >= j) a[j] -= a[k] * ljk;
     }
-    __syncthreads();  // col is rewritten by the next column
   }
 }
 // x = A^-1 b for A = L L^T from chol_rows; b and x one element per lane.
@@ -299,7 +295,9 @@ __device__ inline double chol_solve_rows(const double (&a)[23], double dinv, dou
 // A^-1 = W^T W, whose row r is written by lane c into out[r * 23 + c]
 // (bitwise symmetric: both halves sum the same products in the same order).
 // L, 1/L[i][i] and then W are staged in S (23 * 24 doubles of LDS) and read
-// back as broadcasts.  Called by all threads of a 64-thread workgroup.
+// back as broadcasts; out should be LDS too (stores through a generic
+// pointer to global memory would wait together with those reads).  Called by
+// all threads of a 64-thread workgroup.
 __device__ inline void chol_inverse_rows(const double (&a)[23], double dinv, double* out, int lane, double* S) {
   if (lane < 23) {
 #pragma unroll
