@@ -96,6 +96,7 @@ typedef struct {
                           rebuilt, 2 = LRU ring compacted (bit set) */
   int32_t merge_windows;  /* evaluation windows of the merge replay */
   int32_t merge_requeued; /* window entries re-evaluated after a conflict */
+  int32_t merge_max_acting; /* most merge() calls that acted in one window */
 } vmp_map_stats;
 
 typedef struct vmp_ctx vmp_ctx;
@@ -123,7 +124,13 @@ int vmp_state_apply_motion(vmp_session* s, const double* dR, const double* dp, c
 int vmp_state_apply_motion_batch(vmp_session** s, int32_t n, const double* dR, const double* dp, const double* qdiag);
 
 /* Scan points in the LiDAR frame, SoA fp32, scan (time) order.  Host
- * pointers are copied; _device takes pointers that already live in HBM. */
+ * pointers are copied; _device takes pointers that already live in HBM (the
+ * buffers must hold the points when the call is made and stay unchanged
+ * until the registration call that uses them returns).  A scan holding a NaN
+ * or infinite coordinate is rejected with VMP_EINVAL and nothing is bound
+ * (the reference would index a voxel from an undefined key); device buffers
+ * are checked by one small kernel and a wait on a stream of the context's
+ * own (context parameter "check_points"). */
 int vmp_scan_set_points(vmp_session* s, const float* x, const float* y, const float* z, int32_t n);
 int vmp_scan_set_points_device(vmp_session* s, const float* dx, const float* dy, const float* dz, int32_t n);
 /* The same for n sessions in one call: device addresses and point counts. */
@@ -161,7 +168,9 @@ int vmp_register_batch(vmp_session** s, int32_t n, vmp_obs_stats* ost, vmp_map_s
  * host can prepare the next scans while the device works.  Returns the status
  * of the batch it completed (or of this launch).  vmp_register_batch_wait
  * completes the batch in flight and copies the statistics of the last
- * completed batch (n entries). */
+ * submitted batch (n entries) with its status; each submitted batch's
+ * statistics are handed out once, and a wait with no submitted batch since
+ * the previous wait returns VMP_ESTATE (synchronous calls leave none). */
 int vmp_register_batch_submit(vmp_session** s, int32_t n);
 int vmp_register_batch_wait(vmp_ctx* ctx, int32_t n, vmp_obs_stats* ost, vmp_map_stats* mst);
 
@@ -185,6 +194,10 @@ int vmp_ctx_reserve(vmp_ctx* ctx, int64_t points, int32_t sessions);
  *              frame anchored at the scan's pose voxel (the default),
  *              1 = plain fp32 world frame, a precision-study arm whose
  *              fallback counter counts keys that differ from the fp64 key.
+ *   "check_points" 1 = device point buffers are checked for non-finite
+ *              coordinates when they are bound (the default), 0 = the
+ *              caller guarantees finite points (host buffers are always
+ *              checked).
  *   "obs_time_reps" 1..64: while per-launch timing is on, each observe
  *              launch runs this many times back to back between one event
  *              pair (the kernel is idempotent), then once more for the

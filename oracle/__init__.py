@@ -76,6 +76,7 @@ def lib():
             "orc_stats",
             "orc_map_push",
             "orc_key",
+            "orc_lidar_to_world_f32",
         ):
             getattr(L, name).argtypes = None
         _lib = L
@@ -189,6 +190,13 @@ class Oracle:
         pts = np.ascontiguousarray(pts, np.float64)
         covs = np.ascontiguousarray(np.transpose(covs, (0, 2, 1)), np.float64)  # column-major 3x3
         self.L.orc_map_push(self.h, P(pts), P(covs), C.c_int(len(pts)), C.c_int(1 if is_build else 0))
+
+    def lidar_to_world_f32(self, x, y, z):
+        """Insertion points at the current state (fp32 PCL transform)."""
+        x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+        out = np.zeros((3, len(x)), np.float32)
+        self.L.orc_lidar_to_world_f32(self.h, P(x), P(y), P(z), C.c_int(len(x)), P(out[0]), P(out[1]), P(out[2]))
+        return out.T
 
     def key(self, p):
         k = np.zeros(3, np.int64)

@@ -211,6 +211,19 @@ int orc_map_push(void* h, const double* pts, const double* covs, int n, int is_b
   return 0;
 }
 
+// The insertion transform at the registrar's current state (fp32 out).
+void orc_lidar_to_world_f32(void* h, const float* x, const float* y, const float* z, int n, float* ox, float* oy,
+                            float* oz) {
+  Registrar* r = static_cast<Registrar*>(h);
+  std::vector<float> wx, wy, wz;
+  lidar_to_world_f32(r->kf.x, x, y, z, n, wx, wy, wz);
+  for (int i = 0; i < n; ++i) {
+    ox[i] = wx[i];
+    oy[i] = wy[i];
+    oz[i] = wz[i];
+  }
+}
+
 void orc_key(void* h, const double* p, int64_t* key) {
   V3 v;
   v3_in(p, v);

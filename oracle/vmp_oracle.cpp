@@ -573,7 +573,7 @@ Registrar::Registrar(const Config& c) : cfg(c) {
 }
 
 // PCL transformPointCloud with an fp32 Matrix4f (lio_builder.cpp:155-163).
-static void lidar_to_world_f32(const State& x, const float* px, const float* py, const float* pz, int n,
+void lidar_to_world_f32(const State& x, const float* px, const float* py, const float* pz, int n,
                                std::vector<float>& ox, std::vector<float>& oy, std::vector<float>& oz) {
   M3 r = x.rot * x.rot_ext;
   V3 t = (x.rot * x.pos_ext) + x.pos;

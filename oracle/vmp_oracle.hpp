@@ -227,6 +227,11 @@ class IESKF {
   void update();
 };
 
+// PCL transformPointCloud with the fp32 Matrix4f of lidarToWorld
+// (lio_builder.cpp:155-163): x' = m0*x + (m1*y + (m2*z + t)) in fp32.
+void lidar_to_world_f32(const State& x, const float* px, const float* py, const float* pz, int n,
+                        std::vector<float>& ox, std::vector<float>& oy, std::vector<float>& oz);
+
 // ---- registration driver (LIOBuilder restricted to the path) ----
 struct Config {
   double voxel_size = 0.5;

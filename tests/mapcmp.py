@@ -1,6 +1,8 @@
 """Map / state comparison helpers shared by the parity tests."""
 import numpy as np
 
+import obsref
+
 
 def rot_angle(Ra, Rb):
     d = Ra.T @ Rb
@@ -31,16 +33,19 @@ def compare_maps(ref, got, mean_tol=1e-6, norm_tol=1e-5, tr_rtol=1e-4, report=No
     out["merged_flag_diff"] = int(((fl_r & 8) != (fl_g & 8)).sum())
     out["group_diff"] = int((ref["group"] != got["group"]).sum())
     same = ~plane_diff
-    # merged planes: the fused mean and normal scale with the plane's distance
-    # from the origin (the reference divides only the second term), so their
+    # merged planes: the fused mean scales with the plane's distance from the
+    # origin (the reference divides only the second term, Q4), so its
     # tolerance is relative to |mean| (the cov-trace weights carry ~1e-6
-    # relative error from fp32 per-point covariances)
+    # relative error from fp32 per-point covariances); the fused normal is a
+    # weighted sum of unit normals and keeps an absolute tolerance, relative
+    # only to its own length when that exceeds 1
     merged = (fl_r & 8) != 0
     scale = np.where(merged, 10.0 * np.maximum(1.0, np.abs(ref["dbl"][:, 0:3]).max(axis=1)), 1.0)
     dm = np.abs(ref["dbl"][:, 0:3] - got["dbl"][:, 0:3]).max(axis=1) / scale
     out["mean_err_max"] = float(dm[same].max()) if same.any() else 0.0
     pl = same & ((fl_r & 2) != 0)
-    dn = np.abs(ref["dbl"][:, 3:6] - got["dbl"][:, 3:6]).max(axis=1) / scale
+    nscale = np.maximum(1.0, np.linalg.norm(ref["dbl"][:, 3:6], axis=1))
+    dn = np.abs(ref["dbl"][:, 3:6] - got["dbl"][:, 3:6]).max(axis=1) / nscale
     out["norm_err_max"] = float(dn[pl].max()) if pl.any() else 0.0
     out["norm_bad"] = int((dn[pl] > norm_tol).sum())
     out["mean_bad"] = int((dm[same] > mean_tol).sum())
@@ -53,3 +58,25 @@ def compare_maps(ref, got, mean_tol=1e-6, norm_tol=1e-5, tr_rtol=1e-4, report=No
     if report is not None:
         report(out)
     return out
+
+
+def hb_on_common_set(Ho, bo, s, x, y, z, oslots, gslots, estimate_ext=False):
+    """The oracle's H, b moved onto the device's valid set: slots valid only
+    on the oracle's side are taken out (with the oracle's normal and
+    residual), slots valid only on the device's side are added (with the
+    device's), each with the Jacobian row and weight of lio_builder.cpp:283-306
+    at state s.  Equal to the device's H, b up to rounding whenever the two
+    sides agree on every slot they share."""
+    pts = obsref.lidar_points(x, y, z)
+    vo, ro, no = oslots
+    vg, rg, ng = gslots
+    H, b = np.array(Ho, copy=True), np.array(bo, copy=True)
+    for i in np.flatnonzero(vo & ~vg):
+        J, w = obsref.jac_weight(s, pts[i], np.asarray(no[i], np.float64), estimate_ext)
+        H -= w * np.outer(J, J)
+        b -= w * J * float(ro[i])
+    for i in np.flatnonzero(vg & ~vo):
+        J, w = obsref.jac_weight(s, pts[i], np.asarray(ng[i], np.float64), estimate_ext)
+        H += w * np.outer(J, J)
+        b += w * J * float(rg[i])
+    return H, b

@@ -5,17 +5,20 @@ All calls go through the C ABI (libvmp.so).  Tolerances:
   plane mean                                          1e-6 m
   plane normal (per component)                        1e-5
   Plane::cov block traces                             1e-4 relative
-  H, b of one observe pass                            1e-6 relative (Frobenius)
+  H, b of one observe pass                            1e-6 / 1e-4 relative (Frobenius),
+                                                      on the slots valid on both sides
   per-scan pose after the IESKF update                1e-4 m, 1e-5 rad
+  posterior P                                         1e-9 relative to max |P|
 Margin cases (plane threshold or merge gates decided by the last bits of a
-float) are counted and must stay rare (< 0.1 % of voxels).
+float) are counted and must stay rare (< 0.1 % of voxels); the voxels they
+touch are the only ones allowed to differ in normal, mean or trace.
 """
 import sys
 
 import numpy as np
 import pytest
 
-from mapcmp import compare_maps, state_err
+from mapcmp import compare_maps, hb_on_common_set, state_err
 from oracle import Oracle
 from voxelmapplus_fastlio2_amd import Context, LIOConfig, Registrar, initial_covariance, state_vector
 from voxelmapplus_fastlio2_amd.synth import QDIAG, ROOM, SEED0, STREET, Sequence
@@ -46,6 +49,16 @@ def start(seq, cfg):
     return orc, s0, P0, (x, y, z)
 
 
+def assert_map_close(r):
+    """Map comparison bounds: margin cases stay rare and are the only voxels
+    whose group, mean, normal or Plane::cov traces may differ."""
+    margin = r["plane_flag_diff"] + r["merged_flag_diff"]
+    assert r["plane_flag_diff"] <= max(2, r["n"] // 1000)
+    assert r["group_diff"] <= margin * 8
+    assert r["mean_bad"] <= margin * 8 and r["norm_bad"] <= margin * 8
+    assert r["trace_bad"] <= margin * 8
+
+
 def test_build_parity(ctx):
     seq = Sequence(ROOM, SEED0, 20000)
     cfg = LIOConfig(max_scan_points=20000)
@@ -56,8 +69,7 @@ def test_build_parity(ctx):
         st = g.build()
         log("build stats", {f: getattr(st, f) for f, _ in st._fields_})
         r = compare_maps(orc.export_map(), g.export_map(), report=lambda o: log("build map cmp", o))
-        assert r["plane_flag_diff"] <= max(2, r["n"] // 1000)
-        assert r["mean_bad"] == 0 and r["norm_bad"] <= r["plane_flag_diff"] + 2
+        assert_map_close(r)
 
 
 def test_observe_parity(ctx):
@@ -85,12 +97,12 @@ def test_observe_parity(ctx):
     assert flips <= max(3, len(x) // 2000)
     both = vo & v
     assert np.abs(ro[both] - r[both]).max() < 1e-5
-    eH = np.linalg.norm(H - Ho) / np.linalg.norm(Ho)
-    eb = np.linalg.norm(b - bo) / max(np.linalg.norm(bo), 1e-12)
+    Hc, bc = hb_on_common_set(Ho, bo, sp, x, y, z, (vo, ro, no), (v, r, n))
+    eH = np.linalg.norm(H - Hc) / np.linalg.norm(Hc)
+    eb = np.linalg.norm(b - bc) / max(np.linalg.norm(bc), 1e-12)
     log("H rel err", eH, "b rel err", eb)
-    if flips == 0:
-        assert eH < 1e-6
-        assert eb < 1e-4
+    assert eH < 1e-6
+    assert eb < 1e-4
 
 
 @pytest.mark.parametrize("scene,npts,nscans", [(ROOM, 20000, 12), (STREET, 30000, 8)])
@@ -120,10 +132,10 @@ def test_sequence_teacher_forced(ctx, scene, npts, nscans):
             log(f"scan {k}: iters o/g {it_o}/{ost.iterations} pos {ep:.2e} rot {er:.2e} P {eP:.2e}",
                 {f: getattr(mst, f) for f, _ in mst._fields_}, orc.stats())
             assert ep <= 1e-4 and er <= 1e-5
+            if it_o == ost.iterations:
+                assert eP <= 1e-9, "posterior covariance"
             r = compare_maps(orc.export_map(), g.export_map(), report=lambda o: log("  map", o))
-            assert r["plane_flag_diff"] <= max(2, r["n"] // 1000)
-            assert r["group_diff"] <= r["plane_flag_diff"] * 8
-            assert r["mean_bad"] <= r["plane_flag_diff"] * 8 and r["norm_bad"] <= r["plane_flag_diff"] * 8
+            assert_map_close(r)
 
 
 def test_extrinsic_parity(ctx):
@@ -160,10 +172,12 @@ def test_extrinsic_parity(ctx):
             g.set_state(sp, Pp)
             g.set_points(x, y, z)
             H, b, _ = g.observe(sp)
-            v, _, _ = g.export_slots(npts)
+            v, rg, ng = g.export_slots(npts)
             flips = int((vo != v).sum())
-            eH = np.linalg.norm(H - Ho) / np.linalg.norm(Ho)
-            eb = np.linalg.norm(b - bo) / max(np.linalg.norm(bo), 1e-12)
+            _, ro, no = orc.export_slots(npts)
+            Hc, bc = hb_on_common_set(Ho, bo, sp, x, y, z, (vo, ro, no), (v, rg, ng), estimate_ext=True)
+            eH = np.linalg.norm(H - Hc) / np.linalg.norm(Hc)
+            eb = np.linalg.norm(b - bc) / max(np.linalg.norm(bc), 1e-12)
             assert np.abs(Ho[6:12, 6:12]).max() > 0, "extrinsic block populated"
             it_o = orc.register(x, y, z)
             so, Po = orc.get_state()
@@ -174,11 +188,10 @@ def test_extrinsic_parity(ctx):
             log(f"ext scan {k}: flips {flips} H rel {eH:.2e} b rel {eb:.2e} iters o/g {it_o}/{ost.iterations} "
                 f"pos {ep:.2e} rot {er:.2e} pos_ext {ex:.2e}")
             assert flips <= max(3, npts // 2000)
-            if flips == 0:
-                assert eH < 1e-6 and eb < 1e-4
+            assert eH < 1e-6 and eb < 1e-4
             assert ep <= 1e-4 and er <= 1e-5 and ex <= 1e-4
             r = compare_maps(orc.export_map(), g.export_map())
-            assert r["plane_flag_diff"] <= max(2, r["n"] // 1000)
+            assert_map_close(r)
 
 
 @pytest.mark.parametrize("capacity", [30000, 18000])
@@ -212,7 +225,7 @@ def test_lru_eviction_parity(ctx, capacity):
                 f"oracle {orc.stats()} pos {ep:.2e} rot {er:.2e}")
             assert ep <= 1e-4 and er <= 1e-5
             r = compare_maps(orc.export_map(), g.export_map(), report=lambda o: log("  map", o))
-            assert r["plane_flag_diff"] <= max(2, r["n"] // 1000)
+            assert_map_close(r)
         assert evicted > 0
         assert orc.stats()["evicted"] == evicted
 
@@ -250,7 +263,7 @@ def test_lru_ring_compaction_parity(ctx):
                 log(f"scan {k}: ring entries written {touched} (ring {qcap}), map {len(b['keys'])}, pos {ep:.1e}")
                 assert np.array_equal(a["keys"], b["keys"]), f"LRU order differs after scan {k}"
                 r = compare_maps(a, b)
-                assert r["plane_flag_diff"] <= max(2, r["n"] // 1000)
+                assert_map_close(r)
     assert touched > 2 * qcap, "the ring must have been compacted at least twice"
 
 
@@ -294,7 +307,7 @@ def test_lru_eviction_after_static_period(ctx):
             assert ep <= 1e-4 and er <= 1e-5, (k, ep, er)
             assert np.array_equal(a["keys"], b["keys"]), f"LRU order differs after scan {k}"
             r = compare_maps(a, b)
-            assert r["plane_flag_diff"] <= max(2, r["n"] // 1000)
+            assert_map_close(r)
         assert evicted > 1000
         assert orc.stats()["evicted"] == evicted
 

@@ -67,6 +67,7 @@ class MapStats(C.Structure):
             "maintenance",
             "merge_windows",
             "merge_requeued",
+            "merge_max_acting",
         )
     ]
 

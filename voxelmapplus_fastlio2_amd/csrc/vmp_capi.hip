@@ -37,6 +37,26 @@ __global__ void k_init_vs(VoxState* vs, int n) {
   }
 }
 
+// Input validation: a scan with a NaN or infinite coordinate is rejected
+// before any stage reads it (its key would be undefined).  One launch covers
+// many point sets; block row y checks set y (grid-stride over sets).
+struct PtsRef {
+  const float* x;
+  const float* y;
+  const float* z;
+  int32_t n;
+  int32_t pad;
+};
+__global__ void k_check_points(const PtsRef* refs, int nref, int32_t* bad) {
+  for (int r = blockIdx.y; r < nref; r += gridDim.y) {
+    const PtsRef R = refs[r];
+    int b = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < R.n; i += gridDim.x * blockDim.x)
+      b |= !(isfinite(R.x[i]) && isfinite(R.y[i]) && isfinite(R.z[i]));
+    if (__ballot(b) != 0ull && (threadIdx.x & 63) == 0) bad[r] = 1;
+  }
+}
+
 // gather the per-session counters and posterior states of a batch into one
 // array (one copy back)
 struct BatchOut {
@@ -174,9 +194,19 @@ struct vmp_ctx {
   int fl_mode = 0;
   std::vector<vmp_session*> fl_ss;
   std::vector<int32_t> fl_n;
-  std::vector<vmp_obs_stats> res_ost;  // results of the last finished batch
+  std::vector<vmp_obs_stats> res_ost;  // results of the last submitted batch, once finished
   std::vector<vmp_map_stats> res_mst;
   int res_rc = 0;
+  bool res_valid = false;  // those results have not been collected by vmp_register_batch_wait
+  // validation of device point buffers (k_check_points) on a stream of its
+  // own, so it does not wait for a batch in flight on `stream`
+  bool check_points = true;
+  hipStream_t vstream = nullptr;
+  int vcap = 0;
+  PtsRef* h_refs = nullptr;
+  PtsRef* d_refs = nullptr;
+  int32_t* h_bad = nullptr;
+  int32_t* d_bad = nullptr;
 };
 
 // Points per thread of the observe kernel for a scan of n points.  It depends
@@ -230,6 +260,11 @@ extern "C" int vmp_ctx_create(int device, vmp_ctx** out) {
     delete c;
     return VMP_EDEVICE;
   }
+  if (hipStreamCreateWithFlags(&c->vstream, hipStreamNonBlocking) != hipSuccess) {
+    hipStreamDestroy(c->stream);
+    delete c;
+    return VMP_EDEVICE;
+  }
   for (hipEvent_t& e : c->ev) hipEventCreate(&e);
   *out = c;
   return VMP_OK;
@@ -254,9 +289,64 @@ extern "C" int vmp_ctx_destroy(vmp_ctx* ctx) {
     if (e) hipEventDestroy(e);
   for (hipEvent_t e : ctx->tev)
     if (e) hipEventDestroy(e);
+  hipStreamSynchronize(ctx->vstream);
+  if (ctx->h_refs) hipHostFree(ctx->h_refs);
+  if (ctx->h_bad) hipHostFree(ctx->h_bad);
+  if (ctx->d_refs) hipFree(ctx->d_refs);
+  if (ctx->d_bad) hipFree(ctx->d_bad);
+  hipStreamDestroy(ctx->vstream);
   hipStreamDestroy(ctx->stream);
   delete ctx;
   return VMP_OK;
+}
+
+// Check device point sets for non-finite coordinates (c->mu held).  Returns
+// the index of the first bad set, -1 if all are finite, or -2 on a device
+// error.  The caller's buffers must hold the points when it is called.
+static int check_device_points(vmp_ctx* c, const PtsRef* refs, int nref) {
+  if (nref <= 0 || !c->check_points) return -1;
+  if (nref > c->vcap) {
+    hipStreamSynchronize(c->vstream);
+    if (c->h_refs) hipHostFree(c->h_refs);
+    if (c->h_bad) hipHostFree(c->h_bad);
+    if (c->d_refs) hipFree(c->d_refs);
+    if (c->d_bad) hipFree(c->d_bad);
+    c->h_refs = nullptr;
+    c->h_bad = nullptr;
+    c->d_refs = nullptr;
+    c->d_bad = nullptr;
+    c->vcap = 0;
+    const int cap = std::max(nref, 64);
+    if (hipHostMalloc(&c->h_refs, sizeof(PtsRef) * cap) != hipSuccess ||
+        hipHostMalloc(&c->h_bad, sizeof(int32_t) * cap) != hipSuccess ||
+        hipMalloc(&c->d_refs, sizeof(PtsRef) * cap) != hipSuccess ||
+        hipMalloc(&c->d_bad, sizeof(int32_t) * cap) != hipSuccess)
+      return -2;
+    c->vcap = cap;
+  }
+  int64_t most = 0;
+  for (int i = 0; i < nref; ++i) {
+    c->h_refs[i] = refs[i];
+    most = std::max<int64_t>(most, refs[i].n);
+  }
+  hipStream_t st = c->vstream;
+  if (hipMemcpyAsync(c->d_refs, c->h_refs, sizeof(PtsRef) * nref, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemsetAsync(c->d_bad, 0, sizeof(int32_t) * nref, st) != hipSuccess)
+    return -2;
+  const int gx = int(std::max<int64_t>(1, std::min<int64_t>((most + 1023) / 1024, 256)));
+  hipLaunchKernelGGL(k_check_points, dim3(gx, std::min(nref, 65535)), dim3(256), 0, st, c->d_refs, nref, c->d_bad);
+  if (hipMemcpyAsync(c->h_bad, c->d_bad, sizeof(int32_t) * nref, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return -2;
+  for (int i = 0; i < nref; ++i)
+    if (c->h_bad[i]) return i;
+  return -1;
+}
+
+static bool host_points_finite(const float* x, const float* y, const float* z, int32_t n) {
+  for (int32_t i = 0; i < n; ++i)
+    if (!(std::isfinite(x[i]) && std::isfinite(y[i]) && std::isfinite(z[i]))) return false;
+  return true;
 }
 
 static int settle(vmp_ctx* c);
@@ -572,6 +662,10 @@ extern "C" int vmp_scan_set_points(vmp_session* s, const float* x, const float* 
     if (s) s->err = "scan size exceeds max_scan_points or null input";
     return VMP_EINVAL;
   }
+  if (!host_points_finite(x, y, z, n)) {
+    s->err = "scan holds a non-finite coordinate";
+    return VMP_EINVAL;
+  }
   std::lock_guard<std::mutex> lk(s->ctx->mu);
   VMP_HIP(hipSetDevice(s->ctx->device));
   hipStream_t st = s->ctx->stream;
@@ -589,8 +683,8 @@ extern "C" int vmp_scan_set_points(vmp_session* s, const float* x, const float* 
   return VMP_OK;
 }
 
-extern "C" int vmp_scan_set_points_device(vmp_session* s, const float* dx, const float* dy, const float* dz,
-                                          int32_t n) {
+// bind device points without validation (the caller has checked them)
+static int bind_points_device(vmp_session* s, const float* dx, const float* dy, const float* dz, int32_t n) {
   if (!s || n < 0 || n > s->cfg.max_scan_points || (n > 0 && (!dx || !dy || !dz))) {
     if (s) s->err = "scan size exceeds max_scan_points or null input";
     return VMP_EINVAL;
@@ -603,13 +697,52 @@ extern "C" int vmp_scan_set_points_device(vmp_session* s, const float* dx, const
   return VMP_OK;
 }
 
+extern "C" int vmp_scan_set_points_device(vmp_session* s, const float* dx, const float* dy, const float* dz,
+                                          int32_t n) {
+  if (!s || n < 0 || n > s->cfg.max_scan_points || (n > 0 && (!dx || !dy || !dz))) {
+    if (s) s->err = "scan size exceeds max_scan_points or null input";
+    return VMP_EINVAL;
+  }
+  vmp_ctx* c = s->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  VMP_HIP(hipSetDevice(c->device));
+  const PtsRef r{dx, dy, dz, n, 0};
+  const int bad = check_device_points(c, &r, 1);
+  if (bad == -2) {
+    s->err = "point validation failed on the device";
+    return VMP_EDEVICE;
+  }
+  if (bad >= 0) {
+    s->err = "scan holds a non-finite coordinate";
+    return VMP_EINVAL;
+  }
+  return bind_points_device(s, dx, dy, dz, n);
+}
+
 extern "C" int vmp_scan_set_points_device_batch(vmp_session** ss, int32_t n, const uint64_t* dx, const uint64_t* dy,
                                                 const uint64_t* dz, const int32_t* counts) {
-  if (!ss || n <= 0 || !dx || !dy || !dz || !counts) return VMP_EINVAL;
+  if (!ss || n <= 0 || !dx || !dy || !dz || !counts || !ss[0]) return VMP_EINVAL;
+  vmp_ctx* c = ss[0]->ctx;
+  std::vector<PtsRef> refs(static_cast<size_t>(n));
   for (int i = 0; i < n; ++i) {
-    const int rc = vmp_scan_set_points_device(ss[i], reinterpret_cast<const float*>(dx[i]),
-                                              reinterpret_cast<const float*>(dy[i]),
-                                              reinterpret_cast<const float*>(dz[i]), counts[i]);
+    if (!ss[i] || ss[i]->ctx->device != c->device || counts[i] < 0 || counts[i] > ss[i]->cfg.max_scan_points)
+      return VMP_EINVAL;
+    refs[size_t(i)] = PtsRef{reinterpret_cast<const float*>(dx[i]), reinterpret_cast<const float*>(dy[i]),
+                             reinterpret_cast<const float*>(dz[i]), counts[i], 0};
+  }
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return VMP_EDEVICE;
+    const int bad = check_device_points(c, refs.data(), n);
+    if (bad == -2) return VMP_EDEVICE;
+    if (bad >= 0) {
+      ss[bad]->err = "scan holds a non-finite coordinate";
+      return VMP_EINVAL;
+    }
+  }
+  // all or nothing: nothing is bound when one set is rejected
+  for (int i = 0; i < n; ++i) {
+    const int rc = bind_points_device(ss[i], refs[size_t(i)].x, refs[size_t(i)].y, refs[size_t(i)].z, counts[i]);
     if (rc != VMP_OK) return rc;
   }
   return VMP_OK;
@@ -1110,6 +1243,7 @@ static int step_finish(vmp_ctx* c, vmp_obs_stats* ost, vmp_map_stats* mst) {
         mst[i].merge_evals = m.merge_evals;
         mst[i].merge_windows = m.merge_windows;
         mst[i].merge_requeued = m.merge_requeued;
+        mst[i].merge_max_acting = m.merge_max_acting;
         mst[i].merges = m.merges;
         mst[i].dropped = m.dropped;
         mst[i].map_size = int32_t(m.size);
@@ -1131,6 +1265,7 @@ static int settle(vmp_ctx* c) {
   c->res_ost.assign(n, vmp_obs_stats{});
   c->res_mst.assign(n, vmp_map_stats{});
   c->res_rc = step_finish(c, c->res_ost.data(), c->res_mst.data());
+  c->res_valid = true;
   return c->res_rc;
 }
 
@@ -1336,6 +1471,11 @@ extern "C" int vmp_register_batch_wait(vmp_ctx* c, int32_t n, vmp_obs_stats* ost
   if (!c) return VMP_EINVAL;
   std::lock_guard<std::mutex> lk(c->mu);
   settle(c);
+  // only a submitted batch leaves results here, and they are handed out once:
+  // a wait with nothing submitted since the last one reports that instead of
+  // returning an older batch's statistics
+  if (!c->res_valid) return VMP_ESTATE;
+  c->res_valid = false;
   const int m = std::min<int>(n, int(c->res_mst.size()));
   for (int i = 0; i < m; ++i) {
     if (ost) ost[i] = c->res_ost[size_t(i)];
@@ -1394,6 +1534,11 @@ extern "C" int vmp_ctx_set_param(vmp_ctx* c, const char* name, int64_t value) {
   if (std::strcmp(name, "obs_arith") == 0) {
     if (value < 0 || value > 1) return VMP_EINVAL;
     c->obs_arith = int(value);
+    return VMP_OK;
+  }
+  if (std::strcmp(name, "check_points") == 0) {
+    if (value < 0 || value > 1) return VMP_EINVAL;
+    c->check_points = value != 0;
     return VMP_OK;
   }
   if (std::strcmp(name, "obs_time_reps") == 0) {

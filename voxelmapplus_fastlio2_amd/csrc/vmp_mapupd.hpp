@@ -52,6 +52,8 @@ struct MapCounters {
   int32_t nlru;            // voxels appended to the LRU queue by this scan
   int32_t merge_windows;   // windows of the merge replay
   int32_t merge_requeued;  // window entries sent back for re-evaluation
+  int32_t merge_max_acting;  // most acting entries in one window of the merge replay
+  int32_t pad_ctr;
 };
 
 // one session's map (device pointers)
@@ -333,7 +335,7 @@ __global__ void k_begin_scan(const MapDev* maps, int nsess) {
   c->fast = 0;
   c->nseg = c->ncreate_ev = c->ncre = c->nevict = c->ncand = c->nlru = 0;
   c->refits = c->merge_calls = c->merge_evals = c->merges = c->dropped = 0;
-  c->merge_windows = c->merge_requeued = 0;
+  c->merge_windows = c->merge_requeued = c->merge_max_acting = 0;
 }
 __global__ void k_fill_i32(int32_t* p, int32_t v, int n) {
   VMP_GRID_LOOP(i, n) p[i] = v;

@@ -317,6 +317,9 @@ constexpr int kNewCap = 2048;    // keys created by one window's taint phase
 constexpr int kFxCap = 2048;     // deferred taint side effects of one window
 constexpr int kTaintPer = 49;    // up to 7 changed voxels x their closed neighbourhood
 constexpr int kFwCap = 512;      // first-writer hash: >= 2 x the kWin x 7 voxels one window writes
+static_assert(kWin * 7 <= kMrThreads, "one thread per window record");
+static_assert(2 * kWin * 7 <= kFwCap, "first-writer hash at most half full");
+static_assert(kWin <= 64, "window bookkeeping runs in one wave");
 
 __device__ inline uint32_t fw_hash(int32_t v) { return (uint32_t(v) * 2654435761u) >> (32 - 9); }
 
@@ -398,7 +401,7 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
   __shared__ int32_t fw_key[kFwCap];  // voxels written in this window -> first writing entry
   __shared__ int32_t fw_min[kFwCap];
   int sp = si[s].spl_lo, pb = 0, ph = 0, pn = 0;
-  int evals = 0, merges = 0, windows = 0, requeued = 0, err = 0;
+  int evals = 0, merges = 0, windows = 0, requeued = 0, max_acting = 0, err = 0;
   uint64_t last = ~0ull;
   while (true) {
     // ---- fill the window: the loaded statics and pool entries merged in
@@ -564,6 +567,7 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
     }
     __syncthreads();
     const int na = sh_i[3];
+    max_acting = max(max_acting, na);
     // ---- conflicts through the records: the first writer of every voxel the
     //      acting entries change, then for each entry whether one of its
     //      seven records has an earlier writer.  The commit stops at the
@@ -578,8 +582,9 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
       sh_conf[tid] = 0;
     }
     __syncthreads();
-    if (tid < na * 8) {
-      const int a = tid >> 3, ci = tid & 7;
+    // na * 8 can exceed the block (kWin * 8 > kMrThreads): strided
+    for (int q = tid; q < na * 8; q += kMrThreads) {
+      const int a = q >> 3, ci = q & 7;
       const int k = act[a];
       if (ci < nclos[k]) {
         const int32_t X = nbv[k][clos[k][ci]];
@@ -862,6 +867,7 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
     c->merges += merges;
     c->merge_windows += windows;
     c->merge_requeued += requeued;
+    c->merge_max_acting = max(c->merge_max_acting, max_acting);
     if (err) atomicOr(&c->error, err);
   }
 }

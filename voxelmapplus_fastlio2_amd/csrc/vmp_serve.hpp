@@ -171,7 +171,7 @@ static void serve_worker(vmp_server* sv, int e) {
         bs.push_back(ss.s);
         r.kind = ss.seen ? 3 : 1;
       }
-      vmp_scan_set_points_device(ss.s, r.x, r.y, r.z, r.n);
+      bind_points_device(ss.s, r.x, r.y, r.z, r.n);  // checked by vmp_server_run
     }
     const double tstart = sv->now();
     lk.unlock();
@@ -312,6 +312,22 @@ extern "C" int vmp_server_run(vmp_server* sv, vmp_request* reqs, int32_t n, cons
     if (r.session < 0 || r.session >= int32_t(sv->sess.size()) || !sv->sess[size_t(r.session)].has_state ||
         r.n < 0 || r.n > sv->scfg.max_scan_points || (i > 0 && r.arrival < reqs[i - 1].arrival))
       return VMP_EINVAL;
+  }
+  {
+    // every request's points are checked once, before the run's clock starts
+    std::vector<PtsRef> refs(static_cast<size_t>(n));
+    for (int32_t i = 0; i < n; ++i) refs[size_t(i)] = PtsRef{reqs[i].x, reqs[i].y, reqs[i].z, reqs[i].n, 0};
+    vmp_ctx* c = sv->home;
+    std::lock_guard<std::mutex> lk(c->mu);
+    hipSetDevice(c->device);
+    const int bad = check_device_points(c, refs.data(), n);
+    if (bad == -2) return VMP_EDEVICE;
+    if (bad >= 0) {
+      reqs[bad].status = VMP_EINVAL;
+      std::lock_guard<std::mutex> g(sv->mu);
+      sv->err = "request " + std::to_string(bad) + ": scan holds a non-finite coordinate";
+      return VMP_EINVAL;
+    }
   }
   {
     std::lock_guard<std::mutex> lk(sv->mu);
