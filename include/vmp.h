@@ -58,7 +58,20 @@ typedef struct {
   int32_t max_scan_points;          /* residual slots = largest scan */
   int32_t max_iter;                 /* IESKF iterations, 5 */
   int32_t estimate_ext;             /* 0 */
+  double scan_resolution;           /* VoxelGrid leaf of registered scans
+                                       (lio_builder.cpp:13-14,215-223); <= 0:
+                                       none (the yaml's 0.1; 0 in the configs
+                                       this path is measured on) */
 } vmp_map_cfg;
+
+/* IMU side of LIOConfig (lio_builder.h:19-28). */
+typedef struct {
+  double na, ng, nba, nbg;  /* noise of Q's blocks: 0.01, 0.01, 1e-4, 1e-4 */
+  int32_t imu_init_num;     /* samples averaged at IMU_INIT: 20 */
+  int32_t gravity_align;    /* 1 */
+  double r_il[9];           /* LiDAR -> IMU rotation, column-major: I */
+  double p_il[3];           /* LiDAR -> IMU translation: 0 */
+} vmp_imu_cfg;
 
 /* kf::State (ieskf.h:31-62); rot and rot_ext column-major. */
 typedef struct {
@@ -136,6 +149,44 @@ int vmp_scan_set_points_device(vmp_session* s, const float* dx, const float* dy,
 /* The same for n sessions in one call: device addresses and point counts. */
 int vmp_scan_set_points_device_batch(vmp_session** s, int32_t n, const uint64_t* dx, const uint64_t* dy,
                                      const uint64_t* dz, const int32_t* counts);
+
+/* IMU_INIT (LIOBuilder::initializeImu, lio_builder.cpp:28-63, with
+ * lio_builder.cpp:177-184): the package's samples (t seconds, acc and gyro 3
+ * per sample, host arrays) join the session's cache; once it holds
+ * imu_init_num samples the device sets rot (gravity alignment), g, bg,
+ * rot_ext, pos_ext and P, and *initialized = 1.  The scan of an IMU_INIT
+ * package is not used (as in the reference). */
+int vmp_imu_init(vmp_session* s, const vmp_imu_cfg* cfg, const double* t, const double* acc, const double* gyro,
+                 int32_t m, double cloud_end_time, int32_t* initialized);
+/* A raw SyncPackage (lio_node.cpp:137-165) for the next registration call:
+ * points in the LiDAR frame at their own times, t_ms per point in ms since
+ * cloud_start (ascending), and the package's 1..512 IMU samples (host
+ * arrays).  The next vmp_map_update(build) / vmp_register_scan / batch call
+ * first runs undistortCloud on the device (lio_builder.cpp:65-153: IMU
+ * propagation of x and P over the samples, then every point compensated
+ * into the frame at cloud_end) and, with scan_resolution > 0 and not at
+ * MAP_INIT, the VoxelGrid filter.  Replaces vmp_scan_set_points and the
+ * motion prior for that call.  _device takes points and times already in
+ * HBM (read during the call that processes them). */
+int vmp_scan_set_raw(vmp_session* s, const float* x, const float* y, const float* z, const float* t_ms, int32_t n,
+                     const double* imu_t, const double* imu_acc, const double* imu_gyro, int32_t m,
+                     double cloud_start, double cloud_end);
+int vmp_scan_set_raw_device(vmp_session* s, const float* dx, const float* dy, const float* dz, const float* dt_ms,
+                            int32_t n, const double* imu_t, const double* imu_acc, const double* imu_gyro, int32_t m,
+                            double cloud_start, double cloud_end);
+/* The points the last registration used (undistorted and filtered), SoA;
+ * returns their count (at most cap are copied). */
+int32_t vmp_scan_export(vmp_session* s, int32_t cap, float* x, float* y, float* z);
+/* The LiDAR cloud of the last processed package, undistorted and before
+ * filtering, in the body (IMU) frame: lidarToBody's fp32 transform with
+ * rot_ext, pos_ext (lio_builder.cpp:165-173), the cloud PointCloudWithOdom
+ * carries (lio_node.cpp:176-181,200-214).  Returns the count. */
+int32_t vmp_cloud_body_export(vmp_session* s, int32_t cap, float* x, float* y, float* z);
+/* PointCloudWithOdom's pose (interface/msg/PointCloudWithOdom.msg:1-3,
+ * lio_node.cpp:219-227): position, orientation quaternion (w, x, y, z) of
+ * rot, and the 6x6 covariance P[0:6, 0:6] row-major (position, then
+ * rotation error). */
+int vmp_odom_export(vmp_session* s, double pos[3], double quat_wxyz[4], double cov6x6[36]);
 
 /* One measurement pass at state x with the session's P as prior:
  * H (12x12) and b (12) exactly as SharedState (ieskf.h:22-29). */

@@ -56,6 +56,21 @@ int vmp_synth_scan_n(const vmp_synth* s, int32_t k, int32_t n, float* x, float* 
  * 2 cm / 0.2 deg per axis stands in for IMU propagation error. */
 int vmp_synth_motion(const vmp_synth* s, int32_t k, double dR[9], double dp[3]);
 
+/* IMU mode: the same scene and path, traversed from rest (0.5 s still, then
+ * a 1 s smooth speed ramp to the trajectory's 1 m/s), with a 200 Hz IMU
+ * (gyro / accelerometer white noise 1e-3 rad/s / 1e-2 m/s^2 and random-walk
+ * biases) and raw scans whose points are taken along the 0.1 s sweep. */
+int vmp_synth_create_imu(int scene, uint64_t seed, int32_t pts_per_scan,
+                         const double origin_offset[3], vmp_synth** out);
+/* Ground-truth pose (world <- lidar = imu) at clock time t seconds. */
+int vmp_synth_pose_at(const vmp_synth* s, double t, double R[9], double p[3]);
+/* IMU samples j0 .. j0+count-1, taken at t_j = 0.005 j: acc (specific
+ * force, m/s^2) and gyro (rad/s) in the body frame, 3 per sample. */
+int vmp_synth_imu(const vmp_synth* s, int32_t j0, int32_t count, double* t, double* acc, double* gyro);
+/* Raw scan k over [0.1 k, 0.1 k + 0.1): n returns, each in the LiDAR frame
+ * at its own time; t_ms = time since 0.1 k in ms (strictly ascending). */
+int vmp_synth_scan_raw(const vmp_synth* s, int32_t k, int32_t n, float* x, float* y, float* z, float* t_ms);
+
 #ifdef __cplusplus
 }
 #endif

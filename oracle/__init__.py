@@ -41,7 +41,7 @@ class Cfg(C.Structure):
     ] + [
         (n, C.c_int32)
         for n in ("max_point_thresh", "update_size_thresh", "map_capacity", "max_scan_points", "max_iter", "estimate_ext")
-    ]
+    ] + [("scan_resolution", C.c_double)]
 
 
 _lib = None
@@ -77,14 +77,40 @@ def lib():
             "orc_map_push",
             "orc_key",
             "orc_lidar_to_world_f32",
+            "orc_imu_config",
+            "orc_set_scan_resolution",
+            "orc_undistort",
+            "orc_predict",
+            "orc_imu_state",
+            "orc_scan_points",
+            "orc_voxel_grid",
+            "orc_rot_from_two_vectors",
+            "orc_imu_poses",
         ):
             getattr(L, name).argtypes = None
+        L.orc_process.restype = C.c_int
+        L.orc_process.argtypes = [C.c_void_p] * 5 + [C.c_int] + [C.c_void_p] * 3 + [C.c_int, C.c_double, C.c_double,
+                                                                                     C.c_void_p]
+        L.orc_initialize_imu.restype = C.c_int
+        L.orc_initialize_imu.argtypes = [C.c_void_p] * 4 + [C.c_int]
         _lib = L
     return _lib
 
 
+def _imu_arrays(imu):
+    """(t[m], acc[m,3], gyro[m,3]) as contiguous float64."""
+    t, acc, gyro = imu
+    return (np.ascontiguousarray(t, np.float64), np.ascontiguousarray(acc, np.float64).reshape(-1, 3),
+            np.ascontiguousarray(gyro, np.float64).reshape(-1, 3))
+
+
 def P(a):
-    return C.c_void_p(a.ctypes.data)
+    """Address of a numpy array as a call argument; the argument keeps the
+    array alive for the call (temporaries made in the argument list would
+    otherwise be freed before the C function reads them)."""
+    v = C.c_void_p(a.ctypes.data)
+    v._keep = a
+    return v
 
 
 def cfg_from(lio_cfg):
@@ -198,6 +224,62 @@ class Oracle:
         self.L.orc_lidar_to_world_f32(self.h, P(x), P(y), P(z), C.c_int(len(x)), P(out[0]), P(out[1]), P(out[2]))
         return out.T
 
+    # ---- IMU side of LIOBuilder (lio_builder.cpp:5-153)
+    def imu_config(self, na=0.01, ng=0.01, nba=0.0001, nbg=0.0001, imu_init_num=20, gravity_align=True,
+                   r_il=None, p_il=None):
+        r = np.asfortranarray(np.eye(3) if r_il is None else r_il, np.float64)
+        p = np.ascontiguousarray(np.zeros(3) if p_il is None else p_il, np.float64)
+        noise = np.array([na, ng, nba, nbg], np.float64)
+        self.L.orc_imu_config(self.h, P(noise), C.c_int(imu_init_num), C.c_int(1 if gravity_align else 0),
+                              C.c_void_p(r.ctypes.data), P(p))
+
+    def set_scan_resolution(self, leaf):
+        self.L.orc_set_scan_resolution(self.h, C.c_double(leaf))
+
+    def process(self, x, y, z, t_ms, imu, cloud_start, cloud_end):
+        """One SyncPackage through LIOBuilder::process; returns (status, iterations)."""
+        x, y, z, t_ms = (np.ascontiguousarray(a, np.float32) for a in (x, y, z, t_ms))
+        it, acc, gyro = _imu_arrays(imu)
+        iters = C.c_int()
+        st = self.L.orc_process(self.h, P(x), P(y), P(z), P(t_ms), len(x), P(it), P(acc), P(gyro), len(it),
+                                float(cloud_start), float(cloud_end), C.byref(iters))
+        return st, iters.value
+
+    def undistort(self, x, y, z, t_ms, imu, cloud_start, cloud_end):
+        x, y, z, t_ms = (np.array(a, np.float32) for a in (x, y, z, t_ms))
+        it, acc, gyro = _imu_arrays(imu)
+        self.L.orc_undistort(self.h, P(x), P(y), P(z), P(t_ms), C.c_int(len(x)), P(it), P(acc), P(gyro),
+                             C.c_int(len(it)), C.c_double(cloud_start), C.c_double(cloud_end))
+        return x, y, z, t_ms
+
+    def initialize_imu(self, imu):
+        it, acc, gyro = _imu_arrays(imu)
+        return bool(self.L.orc_initialize_imu(self.h, P(it), P(acc), P(gyro), len(it)))
+
+    def predict(self, acc, gyro, dt):
+        a, g = np.ascontiguousarray(acc, np.float64), np.ascontiguousarray(gyro, np.float64)
+        self.L.orc_predict(self.h, P(a), P(g), C.c_double(dt))
+
+    def imu_state(self):
+        o = np.zeros(10)
+        self.L.orc_imu_state(self.h, P(o))
+        return {"last_acc": o[0:3], "last_gyro": o[3:6], "last_cloud_end_time": o[6], "gravity_norm": o[7],
+                "last_imu_t": o[8], "status": int(o[9])}
+
+    def imu_poses(self):
+        """IMU pose table of the last undistortion: list of dicts."""
+        n = self.L.orc_imu_poses(self.h, C.c_int(0), None)
+        o = np.zeros((max(n, 1), 22))
+        self.L.orc_imu_poses(self.h, C.c_int(n), P(o))
+        return [{"offset": r[0], "acc": r[1:4], "gyro": r[4:7], "vel": r[7:10], "pos": r[10:13],
+                 "rot": r[13:22].reshape(3, 3).T} for r in o[:n]]
+
+    def scan_points(self, cap=1 << 24):
+        n = self.L.orc_scan_points(self.h, C.c_int(0), None, None, None)
+        out = np.zeros((3, max(n, 1)), np.float32)
+        self.L.orc_scan_points(self.h, C.c_int(n), P(out[0]), P(out[1]), P(out[2]))
+        return out[:, :n]
+
     def key(self, p):
         k = np.zeros(3, np.int64)
         self.L.orc_key(self.h, P(np.ascontiguousarray(p, np.float64)), P(k))
@@ -267,3 +349,20 @@ def boxminus(a, b):
     out = np.zeros(23)
     L.orc_boxminus(P(np.ascontiguousarray(a, np.float64)), P(np.ascontiguousarray(b, np.float64)), P(out))
     return out
+
+
+def voxel_grid(x, y, z, leaf):
+    """PCL VoxelGrid restated (oracle): the downsampled cloud, 3 x n_out."""
+    L = lib()
+    x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+    out = np.zeros((3, max(len(x), 1)), np.float32)
+    n = L.orc_voxel_grid(P(x), P(y), P(z), C.c_int(len(x)), C.c_float(leaf), P(out[0]), P(out[1]), P(out[2]))
+    return out[:, :n]
+
+
+def rot_from_two_vectors(a, b):
+    L = lib()
+    R = np.zeros((3, 3), order="F")
+    av, bv = np.ascontiguousarray(a, np.float64), np.ascontiguousarray(b, np.float64)
+    L.orc_rot_from_two_vectors(P(av), P(bv), C.c_void_p(R.ctypes.data))
+    return np.ascontiguousarray(R)

@@ -2,8 +2,10 @@
 // restatement so pytest (ctypes) and bench.py's cpu_baseline leg can drive it.
 // Layouts: state = 36 doubles {pos[3], rot[9], rot_ext[9], pos_ext[3], vel[3],
 // bg[3], ba[3], g[3]}, 3x3 and 23x23 matrices column-major (Eigen layout).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "vmp_oracle.hpp"
 
@@ -14,6 +16,7 @@ extern "C" {
 typedef struct {
   double voxel_size, plane_thresh, merge_thresh_for_angle, merge_thresh_for_distance, ranging_cov, angle_cov;
   int32_t max_point_thresh, update_size_thresh, map_capacity, max_scan_points, max_iter, estimate_ext;
+  double scan_resolution;
 } orc_cfg;
 
 static void m3_in(const double* cm, M3& m) {
@@ -64,6 +67,7 @@ void* orc_create(const orc_cfg* c) {
   cfg.map_capacity = c->map_capacity;
   cfg.max_iter = c->max_iter;
   cfg.estimate_ext = c->estimate_ext;
+  cfg.scan_resolution = c->scan_resolution;
   return new Registrar(cfg);
 }
 void orc_destroy(void* h) { delete static_cast<Registrar*>(h); }
@@ -231,6 +235,124 @@ void orc_key(void* h, const double* p, int64_t* key) {
   key[0] = k.x;
   key[1] = k.y;
   key[2] = k.z;
+}
+
+// ---- IMU side of LIOBuilder ----
+static std::vector<ImuData> imus_in(const double* t, const double* acc, const double* gyro, int m) {
+  std::vector<ImuData> v(static_cast<size_t>(m));
+  for (int i = 0; i < m; ++i) {
+    v3_in(acc + 3 * i, v[size_t(i)].acc);
+    v3_in(gyro + 3 * i, v[size_t(i)].gyro);
+    v[size_t(i)].t = t[i];
+  }
+  return v;
+}
+// noise (na, ng, nba, nbg), imu_init_num, gravity_align, r_il (column-major), p_il
+void orc_imu_config(void* h, const double* noise4, int init_num, int gravity_align, const double* r_il,
+                    const double* p_il) {
+  Registrar* r = static_cast<Registrar*>(h);
+  ImuConfig c;
+  c.na = noise4[0];
+  c.ng = noise4[1];
+  c.nba = noise4[2];
+  c.nbg = noise4[3];
+  c.imu_init_num = init_num;
+  c.gravity_align = gravity_align != 0;
+  m3_in(r_il, c.r_il);
+  v3_in(p_il, c.p_il);
+  r->set_imu_config(c);
+}
+void orc_set_scan_resolution(void* h, double leaf) { static_cast<Registrar*>(h)->cfg.scan_resolution = leaf; }
+// One SyncPackage (lio_builder.cpp:175-248): returns the status after it
+// (0 IMU_INIT, 1 MAP_INIT, 2 LIO_MAPPING); iters gets the IESKF iterations.
+int orc_process(void* h, const float* x, const float* y, const float* z, const float* t_ms, int n, const double* it,
+                const double* iacc, const double* igyro, int m, double cloud_start, double cloud_end, int* iters) {
+  Registrar* r = static_cast<Registrar*>(h);
+  r->kf.last_iters = 0;
+  const LioStatus st = r->process(std::vector<float>(x, x + n), std::vector<float>(y, y + n),
+                                  std::vector<float>(z, z + n), std::vector<float>(t_ms, t_ms + n),
+                                  imus_in(it, iacc, igyro, m), cloud_start, cloud_end);
+  if (iters) *iters = r->kf.last_iters;
+  return int(st);
+}
+// undistortCloud alone (points in/out, sorted by time on return)
+void orc_undistort(void* h, float* x, float* y, float* z, float* t_ms, int n, const double* it, const double* iacc,
+                   const double* igyro, int m, double cloud_start, double cloud_end) {
+  Registrar* r = static_cast<Registrar*>(h);
+  std::vector<float> a(x, x + n), b(y, y + n), c(z, z + n), d(t_ms, t_ms + n);
+  r->undistort(a, b, c, d, imus_in(it, iacc, igyro, m), cloud_start, cloud_end);
+  for (int i = 0; i < n; ++i) {
+    x[i] = a[size_t(i)];
+    y[i] = b[size_t(i)];
+    z[i] = c[size_t(i)];
+    t_ms[i] = d[size_t(i)];
+  }
+}
+int orc_initialize_imu(void* h, const double* it, const double* iacc, const double* igyro, int m) {
+  return static_cast<Registrar*>(h)->initialize_imu(imus_in(it, iacc, igyro, m)) ? 1 : 0;
+}
+void orc_predict(void* h, const double* acc, const double* gyro, double dt) {
+  Registrar* r = static_cast<Registrar*>(h);
+  V3 a, g;
+  v3_in(acc, a);
+  v3_in(gyro, g);
+  r->kf.predict(a, g, dt, r->Q);
+}
+// IMU bookkeeping: last_acc (3), last_gyro (3), last_cloud_end_time,
+// gravity_norm, last_imu time, status
+void orc_imu_state(void* h, double* out10) {
+  Registrar* r = static_cast<Registrar*>(h);
+  v3_out(r->last_acc, out10);
+  v3_out(r->last_gyro, out10 + 3);
+  out10[6] = r->last_cloud_end_time;
+  out10[7] = r->gravity_norm;
+  out10[8] = r->last_imu.t;
+  out10[9] = double(int(r->status));
+}
+// IMU pose table of the last undistortion: per entry offset, acc(3),
+// gyro(3), vel(3), pos(3), rot(9, column-major) = 22 doubles
+int orc_imu_poses(void* h, int cap, double* out) {
+  Registrar* r = static_cast<Registrar*>(h);
+  const int n = int(r->imu_poses.size());
+  for (int i = 0; i < n && i < cap; ++i) {
+    const ImuPose& q = r->imu_poses[size_t(i)];
+    double* o = out + 22 * i;
+    o[0] = q.offset;
+    v3_out(q.acc, o + 1);
+    v3_out(q.gyro, o + 4);
+    v3_out(q.vel, o + 7);
+    v3_out(q.pos, o + 10);
+    m3_out(q.rot, o + 13);
+  }
+  return n;
+}
+
+// points of the last registered scan (after undistortion and downsampling)
+int orc_scan_points(void* h, int cap, float* x, float* y, float* z) {
+  Registrar* r = static_cast<Registrar*>(h);
+  const int n = std::min(cap, r->size);
+  for (int i = 0; i < n; ++i) {
+    if (x) x[i] = r->cx[size_t(i)];
+    if (y) y[i] = r->cy[size_t(i)];
+    if (z) z[i] = r->cz[size_t(i)];
+  }
+  return r->size;
+}
+int orc_voxel_grid(const float* x, const float* y, const float* z, int n, float leaf, float* ox, float* oy, float* oz) {
+  std::vector<float> a, b, c;
+  voxel_grid_filter(x, y, z, n, leaf, a, b, c);
+  for (size_t i = 0; i < a.size(); ++i) {
+    ox[i] = a[i];
+    oy[i] = b[i];
+    oz[i] = c[i];
+  }
+  return int(a.size());
+}
+void orc_rot_from_two_vectors(const double* a, const double* b, double* R) {
+  V3 va, vb;
+  v3_in(a, va);
+  v3_in(b, vb);
+  m3_out(rot_from_two_vectors(va, vb), R);
 }
 
 // ---- math exposed for pinning against numpy / scipy ----

@@ -500,6 +500,10 @@ Mat<3, 2> State::getBx() const {  // ieskf.cpp:57-65
   return r;
 }
 
+Mat<3, 2> State::getMx() const {  // ieskf.cpp:79-83
+  return ((-1.0) * hat(g)) * getBx();
+}
+
 Mat<3, 2> State::getMx(const Mat<2, 1>& res) const {  // ieskf.cpp:73-78
   Mat<3, 2> bx = getBx();
   V3 bu = bx * res;
@@ -704,6 +708,312 @@ void Registrar::map_update() {  // lio_builder.cpp:231-246
     pv_list.push_back(pv);
   }
   map->update(pv_list);
+}
+
+
+// ---------------- IMU propagation (ieskf.cpp:101-123) ----------------
+// F and G are built from the state before the step; x += delta with the
+// 24-vector whose rot_ext, pos_ext, bias and gravity parts are zero, so
+// those stay exactly as they were.
+void IESKF::predict(const V3& acc, const V3& gyro, double dt, const Mat<12, 12>& Q) {
+  const V3 wb = gyro - x.bg;
+  const V3 wdt = dt * wb;                               // (gyro - bg) * dt
+  const V3 dpos = dt * x.vel;                           // vel * dt
+  const V3 dvel = dt * ((x.rot * (acc - x.ba)) + x.g);  // (rot (acc - ba) + g) * dt
+  M23 F = M23::ident();
+  Mat<23, 12> G = Mat<23, 12>::zero();
+  const M3 Jr = right_jacobian(wdt);
+  const M3 Rexp = so3_exp((-1.0) * wdt);
+  const M3 Ahat = ((-1.0) * x.rot) * hat(acc - x.ba);
+  const Mat<3, 2> Mx = x.getMx();
+  const Mat<2, 2> NM = x.getNx() * Mx;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      F(i, 12 + j) = (i == j ? 1.0 : 0.0) * dt;
+      F(3 + i, 3 + j) = Rexp(i, j);
+      F(3 + i, 15 + j) = ((-1.0) * Jr(i, j)) * dt;
+      F(12 + i, 3 + j) = Ahat(i, j) * dt;
+      F(12 + i, 18 + j) = ((-1.0) * x.rot(i, j)) * dt;
+      G(3 + i, j) = ((-1.0) * Jr(i, j)) * dt;
+      G(12 + i, 3 + j) = ((-1.0) * x.rot(i, j)) * dt;
+      G(15 + i, 6 + j) = (i == j ? 1.0 : 0.0) * dt;
+      G(18 + i, 9 + j) = (i == j ? 1.0 : 0.0) * dt;
+    }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 2; ++j) F(12 + i, 21 + j) = Mx(i, j) * dt;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) F(21 + i, 21 + j) = NM(i, j);
+  x.pos = x.pos + dpos;
+  x.rot = x.rot * so3_exp(wdt);
+  x.rot_ext = x.rot_ext * so3_exp(V3::zero());
+  x.vel = x.vel + dvel;
+  x.g = so3_exp(V3::zero()) * x.g;
+  P = (((F * P) * tr(F)) + ((G * Q) * tr(G)));
+}
+
+// Eigen's Quaternion::setFromTwoVectors for a, b not opposite: the quaternion
+// half-way between them, then the rotation matrix of that quaternion.
+M3 rot_from_two_vectors(const V3& a, const V3& b) {
+  auto unit = [](const V3& v) {
+    const double n = std::sqrt(dot(v, v));
+    return v3(v(0, 0) / n, v(1, 0) / n, v(2, 0) / n);
+  };
+  const V3 v0 = unit(a), v1 = unit(b);
+  const double c = dot(v1, v0);
+  double qw, qx, qy, qz;
+  if (c < -1.0 + 1e-12) {  // opposite: a half turn about an axis orthogonal to a
+    V3 ax = std::fabs(v0(0, 0)) < 0.9 ? v3(0.0, -v0(2, 0), v0(1, 0)) : v3(-v0(2, 0), 0.0, v0(0, 0));
+    ax = unit(ax);
+    qw = 0.0;
+    qx = ax(0, 0);
+    qy = ax(1, 0);
+    qz = ax(2, 0);
+  } else {
+    const V3 axis = v3(v0(1, 0) * v1(2, 0) - v0(2, 0) * v1(1, 0), v0(2, 0) * v1(0, 0) - v0(0, 0) * v1(2, 0),
+                       v0(0, 0) * v1(1, 0) - v0(1, 0) * v1(0, 0));
+    const double sq = std::sqrt((1.0 + c) * 2.0);
+    const double invs = 1.0 / sq;
+    qx = axis(0, 0) * invs;
+    qy = axis(1, 0) * invs;
+    qz = axis(2, 0) * invs;
+    qw = sq * 0.5;
+  }
+  const double tx = 2 * qx, ty = 2 * qy, tz = 2 * qz;
+  const double twx = tx * qw, twy = ty * qw, twz = tz * qw;
+  const double txx = tx * qx, txy = ty * qx, txz = tz * qx;
+  const double tyy = ty * qy, tyz = tz * qy, tzz = tz * qz;
+  M3 R;
+  R(0, 0) = 1 - (tyy + tzz);
+  R(0, 1) = txy - twz;
+  R(0, 2) = txz + twy;
+  R(1, 0) = txy + twz;
+  R(1, 1) = 1 - (txx + tzz);
+  R(1, 2) = tyz - twx;
+  R(2, 0) = txz - twy;
+  R(2, 1) = tyz + twx;
+  R(2, 2) = 1 - (txx + tyy);
+  return R;
+}
+
+void voxel_grid_filter(const float* x, const float* y, const float* z, int n, float leaf, std::vector<float>& ox,
+                       std::vector<float>& oy, std::vector<float>& oz) {
+  ox.clear();
+  oy.clear();
+  oz.clear();
+  if (n <= 0) return;
+  const float* P[3] = {x, y, z};
+  float mn[3], mx[3];
+  for (int a = 0; a < 3; ++a) {
+    mn[a] = P[a][0];
+    mx[a] = P[a][0];
+    for (int i = 1; i < n; ++i) {
+      mn[a] = std::min(mn[a], P[a][i]);
+      mx[a] = std::max(mx[a], P[a][i]);
+    }
+  }
+  const float inv = 1.0f / leaf;
+  // index range check: too many leaves for 32-bit indices -> the cloud
+  // passes unfiltered
+  int64_t dxyz = 1;
+  for (int a = 0; a < 3; ++a) dxyz *= int64_t((mx[a] - mn[a]) * inv) + 1;
+  if (dxyz > int64_t(2147483647)) {
+    ox.assign(x, x + n);
+    oy.assign(y, y + n);
+    oz.assign(z, z + n);
+    return;
+  }
+  int minb[3], maxb[3], divb[3], mul[3];
+  for (int a = 0; a < 3; ++a) {
+    minb[a] = int(std::floor(mn[a] * inv));
+    maxb[a] = int(std::floor(mx[a] * inv));
+    divb[a] = maxb[a] - minb[a] + 1;
+  }
+  mul[0] = 1;
+  mul[1] = divb[0];
+  mul[2] = divb[0] * divb[1];
+  std::vector<std::pair<int, int>> idx(static_cast<size_t>(n));
+  for (int i = 0; i < n; ++i) {
+    int lin = 0;
+    for (int a = 0; a < 3; ++a) {
+      const int ijk = int(std::floor(P[a][i] * inv) - float(minb[a]));
+      lin += ijk * mul[a];
+    }
+    idx[size_t(i)] = {lin, i};
+  }
+  std::stable_sort(idx.begin(), idx.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+    return a.first < b.first;
+  });
+  size_t q = 0;
+  while (q < idx.size()) {
+    size_t e = q + 1;
+    while (e < idx.size() && idx[e].first == idx[q].first) ++e;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (size_t k = q; k < e; ++k) {
+      sx += x[idx[k].second];
+      sy += y[idx[k].second];
+      sz += z[idx[k].second];
+    }
+    const float cnt = float(e - q);
+    ox.push_back(sx / cnt);
+    oy.push_back(sy / cnt);
+    oz.push_back(sz / cnt);
+    q = e;
+  }
+}
+
+// ---------------- LIOBuilder IMU side (lio_builder.cpp:5-153) ----------------
+void Registrar::set_imu_config(const ImuConfig& c) {  // lio_builder.cpp:9-12
+  icfg = c;
+  Q = Mat<12, 12>::ident();
+  for (int i = 0; i < 3; ++i) {
+    Q(i, i) = c.ng;
+    Q(3 + i, 3 + i) = c.na;
+    Q(6 + i, 6 + i) = c.nbg;
+    Q(9 + i, 9 + i) = c.nba;
+  }
+}
+
+bool Registrar::initialize_imu(const std::vector<ImuData>& imus) {  // lio_builder.cpp:28-63
+  imu_cache.insert(imu_cache.end(), imus.begin(), imus.end());
+  if (int(imu_cache.size()) < icfg.imu_init_num) return false;
+  V3 acc_mean = V3::zero(), gyro_mean = V3::zero();
+  for (const ImuData& d : imu_cache) {
+    acc_mean = acc_mean + d.acc;
+    gyro_mean = gyro_mean + d.gyro;
+  }
+  const double cnt = double(imu_cache.size());
+  for (int i = 0; i < 3; ++i) {
+    acc_mean(i, 0) = acc_mean(i, 0) / cnt;
+    gyro_mean(i, 0) = gyro_mean(i, 0) / cnt;
+  }
+  gravity_norm = std::sqrt(dot(acc_mean, acc_mean));
+  kf.x.rot_ext = icfg.r_il;
+  kf.x.pos_ext = icfg.p_il;
+  kf.x.bg = gyro_mean;
+  auto init_g = [&](const V3& dir) {  // State::initG (ieskf.h:44-47)
+    const double nn = std::sqrt(dot(dir, dir));
+    kf.x.g = v3(dir(0, 0) / nn * GRAVITY, dir(1, 0) / nn * GRAVITY, dir(2, 0) / nn * GRAVITY);
+  };
+  if (icfg.gravity_align) {
+    kf.x.rot = rot_from_two_vectors((-1.0) * acc_mean, v3(0.0, 0.0, -1.0));
+    init_g(v3(0.0, 0.0, -1.0));
+  } else {
+    init_g((-1.0) * acc_mean);
+  }
+  kf.P = M23::ident();
+  for (int i = 0; i < 3; ++i) {
+    kf.P(6 + i, 6 + i) = 0.00001;
+    kf.P(9 + i, 9 + i) = 0.00001;
+    kf.P(15 + i, 15 + i) = 0.0001;
+    kf.P(18 + i, 18 + i) = 0.0001;
+  }
+  kf.P(21, 21) = 0.00001;
+  kf.P(22, 22) = 0.00001;
+  last_imu = imus.back();
+  return true;
+}
+
+void Registrar::undistort(std::vector<float>& x, std::vector<float>& y, std::vector<float>& z,
+                          std::vector<float>& t_ms, const std::vector<ImuData>& imus, double cloud_start,
+                          double cloud_end) {  // lio_builder.cpp:65-153
+  imu_cache.clear();
+  imu_cache.push_back(last_imu);
+  imu_cache.insert(imu_cache.end(), imus.begin(), imus.end());
+  const double imu_time_end = imu_cache.back().t;
+  const size_t n = x.size();
+  {  // points by time (the generator's times are unique, so any sort agrees)
+    std::vector<size_t> ord(n);
+    for (size_t i = 0; i < n; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return t_ms[a] < t_ms[b]; });
+    std::vector<float> a(n), b(n), c(n), d(n);
+    for (size_t i = 0; i < n; ++i) {
+      a[i] = x[ord[i]];
+      b[i] = y[ord[i]];
+      c[i] = z[ord[i]];
+      d[i] = t_ms[ord[i]];
+    }
+    x.swap(a);
+    y.swap(b);
+    z.swap(c);
+    t_ms.swap(d);
+  }
+  imu_poses.clear();
+  imu_poses.push_back(ImuPose{0.0, last_acc, last_gyro, kf.x.vel, kf.x.pos, kf.x.rot});
+  V3 acc_val = V3::zero(), gyro_val = V3::zero();
+  // deviation: the reference predicts its final step with an uninitialised
+  // input when no IMU pair falls after the last scan end; zero here
+  V3 inp_acc = V3::zero(), inp_gyro = V3::zero();
+  double dt = 0.0;
+  for (size_t k = 0; k + 1 < imu_cache.size(); ++k) {
+    const ImuData& head = imu_cache[k];
+    const ImuData& tail = imu_cache[k + 1];
+    if (tail.t < last_cloud_end_time) continue;
+    gyro_val = 0.5 * (head.gyro + tail.gyro);
+    acc_val = 0.5 * (head.acc + tail.acc);
+    for (int i = 0; i < 3; ++i) acc_val(i, 0) = acc_val(i, 0) * 9.81 / gravity_norm;
+    if (head.t < last_cloud_end_time) dt = tail.t - last_cloud_end_time;
+    else dt = tail.t - head.t;
+    inp_acc = acc_val;
+    inp_gyro = gyro_val;
+    kf.predict(inp_acc, inp_gyro, dt, Q);
+    last_gyro = gyro_val - kf.x.bg;
+    last_acc = (kf.x.rot * (acc_val - kf.x.ba)) + kf.x.g;
+    const double offset = tail.t - cloud_start;
+    imu_poses.push_back(ImuPose{offset, last_acc, last_gyro, kf.x.vel, kf.x.pos, kf.x.rot});
+  }
+  dt = cloud_end - imu_time_end;
+  kf.predict(inp_acc, inp_gyro, dt, Q);
+  last_imu = imus.back();
+  last_cloud_end_time = cloud_end;
+  const M3 cur_rot = kf.x.rot, cur_rot_ext = kf.x.rot_ext;
+  const V3 cur_pos = kf.x.pos, cur_pos_ext = kf.x.pos_ext;
+  if (n == 0) return;
+  size_t ip = n - 1;
+  for (size_t kp = imu_poses.size() - 1; kp > 0; --kp) {
+    const ImuPose& head = imu_poses[kp - 1];
+    const ImuPose& tail = imu_poses[kp];
+    for (; double(t_ms[ip]) / double(1000) > head.offset; ip--) {
+      dt = double(t_ms[ip]) / double(1000) - head.offset;
+      const V3 point = v3(double(x[ip]), double(y[ip]), double(z[ip]));
+      const M3 point_rot = head.rot * so3_exp(dt * tail.gyro);
+      const V3 point_pos = (head.pos + dt * head.vel) + dt * (dt * (0.5 * tail.acc));
+      const V3 pc = tr(cur_rot_ext) *
+                    ((tr(cur_rot) * (((point_rot * ((cur_rot_ext * point) + cur_pos_ext)) + point_pos) - cur_pos)) -
+                     cur_pos_ext);
+      x[ip] = float(pc(0, 0));
+      y[ip] = float(pc(1, 0));
+      z[ip] = float(pc(2, 0));
+      if (ip == 0) break;
+    }
+  }
+}
+
+LioStatus Registrar::process(std::vector<float> x, std::vector<float> y, std::vector<float> z, std::vector<float> t_ms,
+                             const std::vector<ImuData>& imus, double cloud_start, double cloud_end) {
+  if (status == LioStatus::IMU_INIT) {  // lio_builder.cpp:177-184
+    if (initialize_imu(imus)) {
+      status = LioStatus::MAP_INIT;
+      last_cloud_end_time = cloud_end;
+    }
+  } else if (status == LioStatus::MAP_INIT) {  // lio_builder.cpp:185-211
+    undistort(x, y, z, t_ms, imus, cloud_start, cloud_end);
+    map_init(x.data(), y.data(), z.data(), int(x.size()));
+    status = LioStatus::LIO_MAPPING;
+  } else {  // lio_builder.cpp:212-247
+    undistort(x, y, z, t_ms, imus, cloud_start, cloud_end);
+    std::vector<float> fx, fy, fz;
+    if (cfg.scan_resolution > 0.0) {
+      voxel_grid_filter(x.data(), y.data(), z.data(), int(x.size()), float(cfg.scan_resolution), fx, fy, fz);
+    } else {
+      fx = x;
+      fy = y;
+      fz = z;
+    }
+    set_scan(fx.data(), fy.data(), fz.data(), int(fx.size()));
+    kf.update();
+    map_update();
+  }
+  return status;
 }
 
 }  // namespace orc

@@ -208,6 +208,7 @@ struct State {
   void plus(const V23& d);        // operator+=(Vector23d)
   V23 minus(const State& o) const;  // operator-
   Mat<3, 2> getBx() const;
+  Mat<3, 2> getMx() const;  // -hat(g) Bx (ieskf.cpp:79-83)
   Mat<3, 2> getMx(const Mat<2, 1>& res) const;
   Mat<2, 3> getNx() const;
 };
@@ -225,7 +226,38 @@ class IESKF {
   double eps = 0.001;
   int last_iters = 0;
   void update();
+  // IMU propagation (ieskf.cpp:101-123); Q is the 12x12 noise of LIODataGroup
+  void predict(const V3& acc, const V3& gyro, double dt, const Mat<12, 12>& Q);
 };
+
+// ---- IMU side of LIOBuilder (lio_builder.cpp:28-153) ----
+struct ImuData {  // commons.h:12-20
+  V3 acc, gyro;
+  double t;
+};
+struct ImuPose {  // commons.h:30-43
+  double offset;
+  V3 acc, gyro, vel, pos;
+  M3 rot;
+};
+struct ImuConfig {  // LIOConfig fields of the IMU side (lio_builder.h:19-28)
+  double na = 0.01, ng = 0.01, nba = 0.0001, nbg = 0.0001;
+  int imu_init_num = 20;
+  bool gravity_align = true;
+  M3 r_il = M3::ident();
+  V3 p_il = V3::zero();
+};
+// Eigen::Quaterniond::FromTwoVectors(a, b).matrix(): the rotation by the
+// smallest angle that takes direction a to direction b (half-way quaternion
+// w = s/2, v = (a x b)/s, s = sqrt(2 (1 + a.b)) on the normalised vectors).
+M3 rot_from_two_vectors(const V3& a, const V3& b);
+
+// PCL VoxelGrid<PointXYZINormal> (lio_builder.cpp:13-14, 215-223) on xyz:
+// leaf indices from fp32 floor(p * (1/leaf)) minus the cloud's minimum
+// index, points sorted by the linear index (ties in input order), every
+// occupied leaf emitted as the fp32 mean of its points in that order.
+void voxel_grid_filter(const float* x, const float* y, const float* z, int n, float leaf, std::vector<float>& ox,
+                       std::vector<float>& oy, std::vector<float>& oz);
 
 // PCL transformPointCloud with the fp32 Matrix4f of lidarToWorld
 // (lio_builder.cpp:155-163): x' = m0*x + (m1*y + (m2*z + t)) in fp32.
@@ -245,7 +277,10 @@ struct Config {
   int map_capacity = 100000;
   int max_iter = 5;
   int estimate_ext = 0;
+  double scan_resolution = 0.0;  // VoxelGrid leaf of LIO_MAPPING scans (<= 0: none)
 };
+
+enum class LioStatus { IMU_INIT, MAP_INIT, LIO_MAPPING };  // lio_builder.h:10-16
 
 class Registrar {
  public:
@@ -256,7 +291,27 @@ class Registrar {
   void set_scan(const float* x, const float* y, const float* z, int n);
   void observe(State& s, SharedState& d);  // sharedUpdateFunc (lio_builder.cpp:250-311)
   void map_update();                        // lio_builder.cpp:231-246
+  // IMU side (lio_builder.cpp:28-153)
+  void set_imu_config(const ImuConfig& c);                 // lio_builder.cpp:9-12
+  bool initialize_imu(const std::vector<ImuData>& imus);   // lio_builder.cpp:28-63
+  // predict over the package's IMU samples, then backward compensation of
+  // the points (sorted by time, t_ms per point) into the frame at cloud_end
+  void undistort(std::vector<float>& x, std::vector<float>& y, std::vector<float>& z, std::vector<float>& t_ms,
+                 const std::vector<ImuData>& imus, double cloud_start, double cloud_end);
+  // one SyncPackage through the state machine of LIOBuilder::process
+  // (lio_builder.cpp:175-248); returns the status after it
+  LioStatus process(std::vector<float> x, std::vector<float> y, std::vector<float> z, std::vector<float> t_ms,
+                    const std::vector<ImuData>& imus, double cloud_start, double cloud_end);
   Config cfg;
+  ImuConfig icfg;
+  Mat<12, 12> Q = Mat<12, 12>::ident();
+  std::vector<ImuData> imu_cache;
+  std::vector<ImuPose> imu_poses;
+  ImuData last_imu{};
+  V3 last_acc = V3::zero(), last_gyro = V3::zero();
+  double last_cloud_end_time = 0.0;
+  double gravity_norm = 0.0;
+  LioStatus status = LioStatus::IMU_INIT;
   IESKF kf;
   std::shared_ptr<VoxelMap> map;
   std::vector<ResidualData> residual_info;

@@ -18,7 +18,7 @@ import sys
 import numpy as np
 import pytest
 
-from mapcmp import compare_maps, hb_on_common_set, state_err
+from mapcmp import compare_maps, hb_on_common_set, register_both, state_err
 from oracle import Oracle
 from voxelmapplus_fastlio2_amd import Context, LIOConfig, Registrar, initial_covariance, state_vector
 from voxelmapplus_fastlio2_amd.synth import QDIAG, ROOM, SEED0, STREET, Sequence
@@ -119,14 +119,8 @@ def test_sequence_teacher_forced(ctx, scene, npts, nscans):
         for k in range(1, nscans):
             dR, dp = seq.motion(k)
             orc.apply_motion(dR, dp, QDIAG)
-            sp, Pp = orc.get_state()
             x, y, z = seq.scan(k)
-            it_o = orc.register(x, y, z)
-            so, Po = orc.get_state()
-            g.set_state(sp, Pp)
-            g.set_points(x, y, z)
-            ost, mst = g.register()
-            sg, Pg = g.get_state()
+            it_o, so, Po, ost, mst, sg, Pg = register_both(orc, g, x, y, z)
             ep, er = state_err(so, sg)
             eP = np.abs(Po - Pg).max() / np.abs(Po).max()
             log(f"scan {k}: iters o/g {it_o}/{ost.iterations} pos {ep:.2e} rot {er:.2e} P {eP:.2e}",
@@ -211,15 +205,9 @@ def test_lru_eviction_parity(ctx, capacity):
         for k in range(1, nscans):
             dR, dp = seq.motion(k)
             orc.apply_motion(dR, dp, QDIAG)
-            sp, Pp = orc.get_state()
             x, y, z = seq.scan(k)
-            orc.register(x, y, z)
-            g.set_state(sp, Pp)
-            g.set_points(x, y, z)
-            ost, mst = g.register()
+            _, so, _, ost, mst, sg, _ = register_both(orc, g, x, y, z)
             evicted += mst.evicted
-            so, _ = orc.get_state()
-            sg, _ = g.get_state()
             ep, er = state_err(so, sg)
             log(f"cap {capacity} scan {k}: evicted {mst.evicted} created {mst.created} size {mst.map_size} "
                 f"oracle {orc.stats()} pos {ep:.2e} rot {er:.2e}")
@@ -247,15 +235,9 @@ def test_lru_ring_compaction_parity(ctx):
         for k in range(1, nscans):
             dR, dp = seq.motion(k)
             orc.apply_motion(dR, dp, QDIAG)
-            sp, Pp = orc.get_state()
             x, y, z = seq.scan(k)
-            orc.register(x, y, z)
-            g.set_state(sp, Pp)
-            g.set_points(x, y, z)
-            _, mst = g.register()
+            _, so, _, _, mst, sg, _ = register_both(orc, g, x, y, z)
             touched += mst.touched
-            so, _ = orc.get_state()
-            sg, _ = g.get_state()
             ep, er = state_err(so, sg)
             assert ep <= 1e-4 and er <= 1e-5, (k, ep, er)
             if k % 5 == 0 or k == nscans - 1:
@@ -293,14 +275,8 @@ def test_lru_eviction_after_static_period(ctx):
         scans = [(x0, y0, z0)] * 6 + [street.scan(k) for k in range(3)]
         for k, (x, y, z) in enumerate(scans):
             orc.apply_motion(*still, QDIAG)
-            sp, Pp = orc.get_state()
-            orc.register(x, y, z)
-            g.set_state(sp, Pp)
-            g.set_points(x, y, z)
-            _, mst = g.register()
+            _, so, _, _, mst, sg, _ = register_both(orc, g, x, y, z)
             evicted += mst.evicted
-            so, _ = orc.get_state()
-            sg, _ = g.get_state()
             ep, er = state_err(so, sg)
             a, b = orc.export_map(), g.export_map()
             log(f"scan {k}: created {mst.created} evicted {mst.evicted} map {len(b['keys'])} pos {ep:.1e}")

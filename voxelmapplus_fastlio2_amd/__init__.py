@@ -7,6 +7,8 @@ sequence generator (synth.py).
 """
 from .registration import (
     Context,
+    ImuConfig,
+    LIOBuilder,
     LIOConfig,
     Registrar,
     VmpError,
@@ -22,6 +24,8 @@ from .registration import (
 
 __all__ = [
     "Context",
+    "ImuConfig",
+    "LIOBuilder",
     "LIOConfig",
     "Registrar",
     "VmpError",

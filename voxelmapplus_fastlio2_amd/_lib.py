@@ -28,6 +28,22 @@ class MapCfg(C.Structure):
         ("max_scan_points", C.c_int32),
         ("max_iter", C.c_int32),
         ("estimate_ext", C.c_int32),
+        ("scan_resolution", C.c_double),
+    ]
+
+
+class ImuCfg(C.Structure):
+    """vmp_imu_cfg — the IMU side of LIOConfig (lio_builder.h:19-28)."""
+
+    _fields_ = [
+        ("na", C.c_double),
+        ("ng", C.c_double),
+        ("nba", C.c_double),
+        ("nbg", C.c_double),
+        ("imu_init_num", C.c_int32),
+        ("gravity_align", C.c_int32),
+        ("r_il", C.c_double * 9),
+        ("p_il", C.c_double * 3),
     ]
 
 
@@ -137,6 +153,15 @@ def vmp():
            [pp, C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_float)])
     i64p = C.POINTER(C.c_int64)
     _proto(lib, "vmp_session_timing_totals", C.c_int, [P, dp, i64p, dp, i64p, i64p, i64p])
+    i32p = C.POINTER(C.c_int32)
+    _proto(lib, "vmp_imu_init", C.c_int, [P, C.POINTER(ImuCfg), dp, dp, dp, C.c_int32, C.c_double, i32p])
+    _proto(lib, "vmp_scan_set_raw", C.c_int,
+           [P, fp, fp, fp, fp, C.c_int32, dp, dp, dp, C.c_int32, C.c_double, C.c_double])
+    _proto(lib, "vmp_scan_set_raw_device", C.c_int,
+           [P, P, P, P, P, C.c_int32, dp, dp, dp, C.c_int32, C.c_double, C.c_double])
+    _proto(lib, "vmp_scan_export", C.c_int32, [P, C.c_int32, fp, fp, fp])
+    _proto(lib, "vmp_cloud_body_export", C.c_int32, [P, C.c_int32, fp, fp, fp])
+    _proto(lib, "vmp_odom_export", C.c_int, [P, dp, dp, dp])
     _vmp = lib
     return lib
 
@@ -158,5 +183,9 @@ def synth():
     _proto(lib, "vmp_synth_scan", C.c_int, [P, C.c_int32, fp, fp, fp, fp])
     _proto(lib, "vmp_synth_scan_n", C.c_int, [P, C.c_int32, C.c_int32, fp, fp, fp, fp])
     _proto(lib, "vmp_synth_motion", C.c_int, [P, C.c_int32, dp, dp])
+    _proto(lib, "vmp_synth_create_imu", C.c_int, [C.c_int, C.c_uint64, C.c_int32, dp, C.POINTER(C.c_void_p)])
+    _proto(lib, "vmp_synth_pose_at", C.c_int, [P, C.c_double, dp, dp])
+    _proto(lib, "vmp_synth_imu", C.c_int, [P, C.c_int32, C.c_int32, dp, dp, dp])
+    _proto(lib, "vmp_synth_scan_raw", C.c_int, [P, C.c_int32, C.c_int32, fp, fp, fp, fp])
     _synth = lib
     return lib

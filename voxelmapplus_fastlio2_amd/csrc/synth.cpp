@@ -3,6 +3,7 @@
 // index), so results do not depend on thread count or call order.
 #include "vmp_synth.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -85,6 +86,15 @@ M3 expso3(V3 w) {  // Rodrigues
   return R;
 }
 
+V3 logso3(const M3& R) {  // rotation vector of a rotation matrix
+  const V3 v{R.m[2][1] - R.m[1][2], R.m[0][2] - R.m[2][0], R.m[1][0] - R.m[0][1]};
+  double c = 0.5 * (R.m[0][0] + R.m[1][1] + R.m[2][2] - 1.0);
+  c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
+  const double th = std::acos(c);
+  const double f = th < 1e-8 ? 0.5 : th / (2.0 * std::sin(th));
+  return f * v;
+}
+
 struct OBox {  // box with yaw about its vertical axis
   V3 c;        // centre
   V3 h;        // half extents
@@ -115,11 +125,21 @@ bool slab(V3 o, V3 d, V3 lo, V3 hi, double& tnear) {
 
 }  // namespace
 
+// IMU-mode clock: the sensor rests for kRest seconds (IMU initialisation
+// needs a static start), then its speed ramps smoothly to the trajectory's
+// 1 m/s over kRamp seconds; the path itself is the trajectory's, traversed
+// at path parameter tau(t).
+constexpr double kRest = 0.5;
+constexpr double kRamp = 1.0;
+constexpr double kImuDt = 0.005;  // 200 Hz
+
 struct vmp_synth {
   int scene;
   uint64_t seed;
   int n;
+  int imu_mode = 0;
   V3 offset;
+  std::vector<V3> bg, ba;  // IMU biases per sample (random walks)
   std::vector<OBox> boxes;
   std::vector<OBox> facades;
   std::vector<Pole> poles;
@@ -151,6 +171,85 @@ struct vmp_synth {
       }
       xy_table[k + 1] = p;
     }
+  }
+
+  // path parameter and its first two derivatives at clock time t
+  void tau_at(double t, double& tau, double& s, double& sd) const {
+    if (!imu_mode) {
+      tau = t;
+      s = 1.0;
+      sd = 0.0;
+      return;
+    }
+    if (t <= kRest) {
+      tau = 0.0;
+      s = 0.0;
+      sd = 0.0;
+      return;
+    }
+    const double u = (t - kRest) / kRamp;
+    if (u >= 1.0) {
+      tau = 0.5 * kRamp + (t - kRest - kRamp);  // the ramp covers half its length
+      s = 1.0;
+      sd = 0.0;
+      return;
+    }
+    // speed 3u^2 - 2u^3 (smoothstep); its integral over [0, u] times kRamp
+    tau = kRamp * (u * u * u - 0.5 * u * u * u * u);
+    s = 3.0 * u * u - 2.0 * u * u * u;
+    sd = (6.0 * u - 6.0 * u * u) / kRamp;
+  }
+  double heading_rate(double tau) const {
+    if (scene == VMP_SCENE_ROOM) return 0.2 + 0.075 * std::cos(0.5 * tau);
+    return 0.015 * std::cos(0.3 * tau);
+  }
+  // planar position at path parameter tau (table entries every 0.1, then
+  // Simpson over the remainder)
+  V3 xy_at(double tau) const {
+    int k0 = int(std::floor(tau / kScanDt));
+    if (k0 < 0) k0 = 0;
+    if (size_t(k0) >= xy_table.size()) k0 = int(xy_table.size()) - 1;
+    V3 p = xy_table[size_t(k0)];
+    const double a0 = k0 * kScanDt, len = tau - a0;
+    if (len > 0.0) {
+      const int ns = 8;
+      const double h = len / ns;
+      for (int q = 0; q < ns; ++q) {
+        const double a = a0 + q * h, m = a + 0.5 * h, b = a + h;
+        p.x += h * (std::cos(heading(a)) + 4.0 * std::cos(heading(m)) + std::cos(heading(b))) / 6.0;
+        p.y += h * (std::sin(heading(a)) + 4.0 * std::sin(heading(m)) + std::sin(heading(b))) / 6.0;
+      }
+    }
+    return p;
+  }
+  // pose (world <- lidar) at path parameter tau, local frame
+  void pose_tau(double tau, M3& R, V3& t) const {
+    V3 p = xy_at(tau);
+    const double roll = 0.03 * std::sin(0.9 * tau + 0.3), pitch = 0.03 * std::sin(0.6 * tau + 1.1);
+    if (scene == VMP_SCENE_ROOM) p.z = 1.2 + 0.05 * std::sin(0.7 * tau);
+    else p.z = ground(p.x, p.y) + 2.0;
+    R = rpy(roll, pitch, heading(tau));
+    t = p;
+  }
+  void pose_clock(double t, M3& R, V3& p) const {
+    double tau, sp, sd;
+    tau_at(t, tau, sp, sd);
+    pose_tau(tau, R, p);
+  }
+  // world acceleration at clock time t (second derivative of the position)
+  V3 accel_clock(double t) const {
+    double tau, sp, sd;
+    tau_at(t, tau, sp, sd);
+    const double h = heading(tau), hr = heading_rate(tau);
+    const double c = std::cos(h), sn = std::sin(h);
+    V3 a{sd * c - sp * sp * hr * sn, sd * sn + sp * sp * hr * c, 0.0};
+    if (scene == VMP_SCENE_ROOM) {
+      const double dz = 0.035 * std::cos(0.7 * tau), ddz = -0.0245 * std::sin(0.7 * tau);
+      a.z = sd * dz + sp * sp * ddz;
+    } else {
+      a.z = 0.02 * a.x + 0.005 * a.y;
+    }
+    return a;
   }
 
   // pose at time t = kScanDt * (k + 1), local (un-offset) frame
@@ -384,4 +483,141 @@ extern "C" int vmp_synth_motion(const vmp_synth* s, int32_t k, double dR[9], dou
     dp[2] = p.z + ep.z;
   }
   return 0;
+}
+
+extern "C" int vmp_synth_create_imu(int scene, uint64_t seed, int32_t pts_per_scan, const double origin_offset[3],
+                                    vmp_synth** out) {
+  const int rc = vmp_synth_create(scene, seed, pts_per_scan, origin_offset, out);
+  if (rc != 0) return rc;
+  vmp_synth* s = *out;
+  s->imu_mode = 1;
+  // bias random walks per IMU sample over the trajectory's length
+  const size_t m = size_t(double(s->xy_table.size()) * kScanDt / kImuDt) + 64;
+  s->bg.resize(m);
+  s->ba.resize(m);
+  const uint64_t bs = hash3(seed, 0xB1A5ull, 0);
+  auto g = [&](uint64_t a, uint64_t b) { return gauss(hash3(bs, a, 2 * b), hash3(bs, a, 2 * b + 1)); };
+  V3 bgc{2e-3 * g(0, 0), 2e-3 * g(0, 1), 2e-3 * g(0, 2)};
+  V3 bac{1e-2 * g(1, 0), 1e-2 * g(1, 1), 1e-2 * g(1, 2)};
+  for (size_t j = 0; j < m; ++j) {
+    s->bg[j] = bgc;
+    s->ba[j] = bac;
+    bgc = bgc + V3{1e-5 * g(10 + j, 0), 1e-5 * g(10 + j, 1), 1e-5 * g(10 + j, 2)};
+    bac = bac + V3{1e-4 * g(10 + m + j, 0), 1e-4 * g(10 + m + j, 1), 1e-4 * g(10 + m + j, 2)};
+  }
+  return 0;
+}
+
+extern "C" int vmp_synth_pose_at(const vmp_synth* s, double t, double R[9], double p[3]) {
+  if (!s || t < 0.0 || t > double(s->xy_table.size() - 1) * kScanDt) return -1;
+  M3 Rl;
+  V3 tl;
+  s->pose_clock(t, Rl, tl);
+  if (R) store_cm(Rl, R);
+  if (p) {
+    p[0] = tl.x + s->offset.x;
+    p[1] = tl.y + s->offset.y;
+    p[2] = tl.z + s->offset.z;
+  }
+  return 0;
+}
+
+extern "C" int vmp_synth_imu(const vmp_synth* s, int32_t j0, int32_t count, double* t, double* acc, double* gyro) {
+  if (!s || !s->imu_mode || j0 < 0 || count < 0 || size_t(j0) + size_t(count) > s->bg.size()) return -1;
+  const double h = 1e-4;
+  for (int32_t q = 0; q < count; ++q) {
+    const int32_t j = j0 + q;
+    const double tj = kImuDt * j;
+    M3 R, Rm, Rp;
+    V3 p, pm, pp;
+    s->pose_clock(tj, R, p);
+    s->pose_clock(tj > h ? tj - h : 0.0, Rm, pm);
+    s->pose_clock(tj + h, Rp, pp);
+    const double span = tj > h ? 2.0 * h : tj + h;
+    const V3 w = (1.0 / span) * logso3(mulm(transpose(Rm), Rp));  // body rate
+    V3 f = s->accel_clock(tj);
+    f.z += 9.81;  // specific force: a - g, g = (0, 0, -9.81)
+    const V3 fb = mul(transpose(R), f);
+    const uint64_t ns = hash3(s->seed, 0x1A7Aull, uint64_t(j));
+    auto g = [&](uint64_t a) { return gauss(hash3(ns, a, 0), hash3(ns, a, 1)); };
+    if (t) t[q] = tj;
+    if (gyro) {
+      gyro[3 * q + 0] = w.x + s->bg[size_t(j)].x + 1e-3 * g(1);
+      gyro[3 * q + 1] = w.y + s->bg[size_t(j)].y + 1e-3 * g(2);
+      gyro[3 * q + 2] = w.z + s->bg[size_t(j)].z + 1e-3 * g(3);
+    }
+    if (acc) {
+      acc[3 * q + 0] = fb.x + s->ba[size_t(j)].x + 1e-2 * g(4);
+      acc[3 * q + 1] = fb.y + s->ba[size_t(j)].y + 1e-2 * g(5);
+      acc[3 * q + 2] = fb.z + s->ba[size_t(j)].z + 1e-2 * g(6);
+    }
+  }
+  return 0;
+}
+
+// Raw scan k: ray j leaves at clock time 0.1 k + 0.1 j / J from the pose at
+// that time; a return is expressed in the LiDAR frame of its own time (what
+// undistortCloud receives).  J rays span the 0.1 s sweep and are sized from
+// the return rate of a first sweep so that the n-th return comes late in it;
+// a sweep with fewer than n returns is redone with more rays.
+extern "C" int vmp_synth_scan_raw(const vmp_synth* s, int32_t k, int32_t n, float* x, float* y, float* z,
+                                  float* t_ms) {
+  if (!s || k < 0 || n <= 0 || size_t(k) + 1 >= s->xy_table.size()) return -1;
+  const double a1 = 0.7548776662466927, a2 = 0.5698402909980532;
+  const double b1 = 0.1370 * k, b2 = 0.4210 * k;
+  const double el_lo = -7.0 * kPi / 180.0, el_span = 59.0 * kPi / 180.0;
+  const uint64_t sseed = hash3(s->seed, 0x5CA17ull, uint64_t(k));
+  const double t0 = kScanDt * k;
+  long long J = 2LL * n + 64;
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    std::vector<float> px(static_cast<size_t>(J)), py(static_cast<size_t>(J)), pz(static_cast<size_t>(J));
+    std::vector<unsigned char> ok(static_cast<size_t>(J));
+#pragma omp parallel for schedule(static)
+    for (long long j = 0; j < J; ++j) {
+      const double tr = t0 + kScanDt * double(j) / double(J);
+      M3 R;
+      V3 o;
+      s->pose_clock(tr, R, o);
+      double fa = a1 * double(j) + b1, fe = a2 * double(j) + b2;
+      double az = 2.0 * kPi * (fa - std::floor(fa));
+      double el = el_lo + el_span * (fe - std::floor(fe));
+      V3 dl{std::cos(el) * std::cos(az), std::cos(el) * std::sin(az), std::sin(el)};
+      V3 dw = mul(R, dl);
+      double r = s->cast(o, dw);
+      uint64_t h0 = hash3(sseed, uint64_t(j), 1), h1 = hash3(sseed, uint64_t(j), 2);
+      if (s->scene == VMP_SCENE_STREET && u01(hash3(sseed, uint64_t(j), 3)) < 0.15) {
+        double rmax = (r > 0 && r < kMaxRange) ? r : kMaxRange;
+        r = rmax * (0.05 + 0.95 * u01(hash3(sseed, uint64_t(j), 4)));
+      }
+      ok[size_t(j)] = 0;
+      if (r > 0 && r <= kMaxRange) {
+        double rm = r + kRangeSigma * gauss(h0, h1);
+        if (rm >= kMinRange && rm <= kMaxRange) {
+          px[size_t(j)] = float(rm * dl.x);
+          py[size_t(j)] = float(rm * dl.y);
+          pz[size_t(j)] = float(rm * dl.z);
+          ok[size_t(j)] = 1;
+        }
+      }
+    }
+    long long returns = 0;
+    for (long long j = 0; j < J; ++j) returns += ok[size_t(j)];
+    if (attempt == 0 || returns < n) {  // size the sweep: n returns at ~99 % of it
+      const double rate = double(std::max(returns, 1LL)) / double(J);
+      const long long want = (long long)(double(n) / rate * 1.01) + 32;
+      J = returns < n ? std::max(want, J + J / 20) : want;
+      continue;
+    }
+    int filled = 0;
+    for (long long j = 0; j < J && filled < n; ++j) {
+      if (!ok[size_t(j)]) continue;
+      if (x) x[filled] = px[size_t(j)];
+      if (y) y[filled] = py[size_t(j)];
+      if (z) z[filled] = pz[size_t(j)];
+      if (t_ms) t_ms[filled] = float(1e3 * kScanDt * double(j) / double(J));
+      ++filled;
+    }
+    return 0;
+  }
+  return -2;
 }

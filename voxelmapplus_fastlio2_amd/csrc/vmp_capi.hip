@@ -21,6 +21,7 @@
 #include "vmp_ieskf.hpp"
 #include "vmp_mapupd.hpp"
 #include "vmp_observe.hpp"
+#include "vmp_imu.hpp"
 
 using namespace vmp;
 
@@ -44,6 +45,7 @@ struct PtsRef {
   const float* x;
   const float* y;
   const float* z;
+  const float* t;  // optional per-point times, which must also ascend
   int32_t n;
   int32_t pad;
 };
@@ -51,8 +53,10 @@ __global__ void k_check_points(const PtsRef* refs, int nref, int32_t* bad) {
   for (int r = blockIdx.y; r < nref; r += gridDim.y) {
     const PtsRef R = refs[r];
     int b = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < R.n; i += gridDim.x * blockDim.x)
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < R.n; i += gridDim.x * blockDim.x) {
       b |= !(isfinite(R.x[i]) && isfinite(R.y[i]) && isfinite(R.z[i]));
+      if (R.t) b |= !isfinite(R.t[i]) || (i > 0 && !(R.t[i] >= R.t[i - 1]));
+    }
     if (__ballot(b) != 0ull && (threadIdx.x & 63) == 0) bad[r] = 1;
   }
 }
@@ -122,6 +126,12 @@ struct BatchWs {
   BatchOut* h_out = nullptr;
   MotionRec* d_motion = nullptr;
   MotionRec* h_motion = nullptr;
+  PreJob* d_pre = nullptr;  // sessions of a step with a raw package
+  PreJob* h_pre = nullptr;
+  VgJob* d_vg = nullptr;    // sessions of a step whose scan is filtered
+  VgJob* h_vg = nullptr;
+  int32_t* d_vgst = nullptr;
+  int32_t* h_vgst = nullptr;
   hipEvent_t motion_done = nullptr;  // the last motion table upload finished
   bool motion_pending = false;
   std::vector<void*> host_allocs;
@@ -236,6 +246,19 @@ struct vmp_session {
   int32_t* live = nullptr;
   int32_t* nlive = nullptr;
   bool has_points = false;
+  // IMU side: device state, the raw package bound for the next step
+  ImuDev* imu = nullptr;
+  float* own_t = nullptr;
+  bool imu_inited = false;
+  std::vector<double> init_t, init_acc, init_gyro;  // IMU_INIT cache (host)
+  bool raw_pending = false;
+  const float *rx = nullptr, *ry = nullptr, *rz = nullptr, *rt = nullptr;
+  int32_t raw_n = 0;      // points of the pending raw scan
+  int32_t cloud_n = 0;    // undistorted points of the last processed package
+  ImuPkg* stage = nullptr;  // pinned staging of one package's IMU samples
+  float *own_fx = nullptr, *own_fy = nullptr, *own_fz = nullptr;  // VoxelGrid output (scan_resolution > 0)
+  hipEvent_t stage_done = nullptr;
+  bool stage_pending = false;
 };
 
 static uint32_t next_pow2(uint64_t v) {
@@ -415,6 +438,9 @@ static int ws_reserve(vmp_ctx* c, int64_t points, int nsess, bool exact = false)
   ok &= dalloc(A, &w.d_obsstat, size_t(ns)) == hipSuccess;
   ok &= dalloc(A, &w.d_out, size_t(ns)) == hipSuccess;
   ok &= dalloc(A, &w.d_motion, size_t(ns)) == hipSuccess;
+  ok &= dalloc(A, &w.d_pre, size_t(ns)) == hipSuccess;
+  ok &= dalloc(A, &w.d_vg, size_t(ns)) == hipSuccess;
+  ok &= dalloc(A, &w.d_vgst, size_t(ns) * kVgStat) == hipSuccess;
   auto halloc = [&](auto** p, size_t bytes) {
     void* q = nullptr;
     if (hipHostMalloc(&q, std::max<size_t>(bytes, 16)) != hipSuccess) return false;
@@ -430,6 +456,9 @@ static int ws_reserve(vmp_ctx* c, int64_t points, int nsess, bool exact = false)
   ok &= halloc(&w.h_obsstat, sizeof(int32_t*) * ns);
   ok &= halloc(&w.h_out, sizeof(BatchOut) * ns);
   ok &= halloc(&w.h_motion, sizeof(MotionRec) * ns);
+  ok &= halloc(&w.h_pre, sizeof(PreJob) * ns);
+  ok &= halloc(&w.h_vg, sizeof(VgJob) * ns);
+  ok &= halloc(&w.h_vgst, sizeof(int32_t) * kVgStat * ns);
   if (ok && !w.motion_done) ok = hipEventCreateWithFlags(&w.motion_done, hipEventDisableTiming) == hipSuccess;
   w.motion_pending = false;
   if (!ok) {
@@ -476,6 +505,8 @@ extern "C" int64_t vmp_session_bytes(const vmp_map_cfg* cfg) {
   b += z.maxn * (3 * 4 + 16);
   b += z.max_blocks * kObsRowMax * 8;
   b += int64_t(sizeof(FilterDev));
+  b += int64_t(sizeof(ImuDev)) + z.maxn * 4;
+  if (cfg->scan_resolution > 0.0) b += z.maxn * 12;
   return b;
 }
 
@@ -502,6 +533,8 @@ extern "C" int vmp_session_destroy(vmp_session* s) {
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
   for (void* p : s->allocs) hipFree(p);
+  if (s->stage) hipHostFree(s->stage);
+  if (s->stage_done) hipEventDestroy(s->stage_done);
   delete s;
   return VMP_OK;
 }
@@ -543,6 +576,15 @@ extern "C" int vmp_session_create(vmp_ctx* ctx, const vmp_map_cfg* cfg, vmp_sess
   ok &= dalloc(A, &s->obs_stat, 8) == hipSuccess;
   ok &= dalloc(A, &s->live, size_t(z.pool)) == hipSuccess;
   ok &= dalloc(A, &s->nlive, 1) == hipSuccess;
+  ok &= dalloc(A, &s->imu, 1) == hipSuccess;
+  ok &= dalloc(A, &s->own_t, maxn) == hipSuccess;
+  if (cfg->scan_resolution > 0.0) {
+    ok &= dalloc(A, &s->own_fx, maxn) == hipSuccess;
+    ok &= dalloc(A, &s->own_fy, maxn) == hipSuccess;
+    ok &= dalloc(A, &s->own_fz, maxn) == hipSuccess;
+  }
+  ok &= hipHostMalloc(&s->stage, sizeof(ImuPkg)) == hipSuccess;
+  ok &= hipEventCreateWithFlags(&s->stage_done, hipEventDisableTiming) == hipSuccess;
   if (!ok) return fail(VMP_ENOMEM);
   m.hmask = uint32_t(z.hcap - 1);
   m.hbits = 0;
@@ -576,6 +618,7 @@ extern "C" int vmp_session_create(vmp_ctx* ctx, const vmp_map_cfg* cfg, vmp_sess
   for (int i = 0; i < 23; ++i) f0.P[i * 23 + i] = 1.0;
   if (hipMemcpyAsync(s->f, &f0, sizeof(f0), hipMemcpyHostToDevice, st) != hipSuccess) return fail(VMP_EDEVICE);
   if (hipMemsetAsync(s->obs_ctr, 0, 8 * sizeof(int32_t), st) != hipSuccess) return fail(VMP_EDEVICE);
+  if (hipMemsetAsync(s->imu, 0, sizeof(ImuDev), st) != hipSuccess) return fail(VMP_EDEVICE);
   if (hipMemsetAsync(s->obs_stat, 0, 8 * sizeof(int32_t), st) != hipSuccess) return fail(VMP_EDEVICE);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return fail(VMP_EDEVICE);
   *out = s;
@@ -706,7 +749,7 @@ extern "C" int vmp_scan_set_points_device(vmp_session* s, const float* dx, const
   vmp_ctx* c = s->ctx;
   std::lock_guard<std::mutex> lk(c->mu);
   VMP_HIP(hipSetDevice(c->device));
-  const PtsRef r{dx, dy, dz, n, 0};
+  const PtsRef r{dx, dy, dz, nullptr, n, 0};
   const int bad = check_device_points(c, &r, 1);
   if (bad == -2) {
     s->err = "point validation failed on the device";
@@ -728,7 +771,7 @@ extern "C" int vmp_scan_set_points_device_batch(vmp_session** ss, int32_t n, con
     if (!ss[i] || ss[i]->ctx->device != c->device || counts[i] < 0 || counts[i] > ss[i]->cfg.max_scan_points)
       return VMP_EINVAL;
     refs[size_t(i)] = PtsRef{reinterpret_cast<const float*>(dx[i]), reinterpret_cast<const float*>(dy[i]),
-                             reinterpret_cast<const float*>(dz[i]), counts[i], 0};
+                             reinterpret_cast<const float*>(dz[i]), nullptr, counts[i], 0};
   }
   {
     std::lock_guard<std::mutex> lk(c->mu);
@@ -745,6 +788,291 @@ extern "C" int vmp_scan_set_points_device_batch(vmp_session** ss, int32_t n, con
     const int rc = bind_points_device(ss[i], refs[size_t(i)].x, refs[size_t(i)].y, refs[size_t(i)].z, counts[i]);
     if (rc != VMP_OK) return rc;
   }
+  return VMP_OK;
+}
+
+// ----------------------------------------------------------- IMU side -----
+extern "C" int vmp_imu_init(vmp_session* s, const vmp_imu_cfg* cfg, const double* t, const double* acc,
+                            const double* gyro, int32_t m, double cloud_end_time, int32_t* initialized) {
+  if (!s || !cfg || m < 0 || (m > 0 && (!t || !acc || !gyro)) || cfg->imu_init_num < 1) return VMP_EINVAL;
+  if (initialized) *initialized = s->imu_inited ? 1 : 0;
+  if (s->imu_inited) return VMP_OK;
+  for (int32_t i = 0; i < m; ++i) {
+    if (!std::isfinite(t[i])) return VMP_EINVAL;
+    for (int k = 0; k < 3; ++k)
+      if (!std::isfinite(acc[3 * i + k]) || !std::isfinite(gyro[3 * i + k])) return VMP_EINVAL;
+  }
+  s->init_t.insert(s->init_t.end(), t, t + m);
+  s->init_acc.insert(s->init_acc.end(), acc, acc + 3 * size_t(m));
+  s->init_gyro.insert(s->init_gyro.end(), gyro, gyro + 3 * size_t(m));
+  const int total = int(s->init_t.size());
+  if (total < cfg->imu_init_num || m == 0) return VMP_OK;  // lio_builder.cpp:31-32
+  vmp_ctx* c = s->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  VMP_HIP(hipSetDevice(c->device));
+  settle(c);
+  double* buf = nullptr;
+  VMP_HIP(hipMalloc(&buf, sizeof(double) * 7 * size_t(total)));
+  hipStream_t st = c->stream;
+  int rc = VMP_OK;
+  if (hipMemcpyAsync(buf, s->init_t.data(), sizeof(double) * total, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(buf + total, s->init_acc.data(), sizeof(double) * 3 * total, hipMemcpyHostToDevice, st) !=
+          hipSuccess ||
+      hipMemcpyAsync(buf + 4 * total, s->init_gyro.data(), sizeof(double) * 3 * total, hipMemcpyHostToDevice, st) !=
+          hipSuccess)
+    rc = VMP_EDEVICE;
+  ImuCfgDev d{};
+  const D33 ril = d33_from_cm(cfg->r_il);
+  for (int i = 0; i < 9; ++i) d.r_il[i] = ril.m[i];
+  for (int i = 0; i < 3; ++i) d.p_il[i] = cfg->p_il[i];
+  d.gravity_align = cfg->gravity_align;
+  d.n = total;
+  // Q (lio_builder.cpp:9-12) and the IMU_INIT -> MAP_INIT bookkeeping
+  // (lio_builder.cpp:181-182): the scan end of this package
+  const double q[4] = {cfg->ng, cfg->na, cfg->nbg, cfg->nba};
+  if (rc == VMP_OK) {
+    hipLaunchKernelGGL(k_imu_init, dim3(1), dim3(64), 0, st, s->imu, s->f, buf, buf + total, buf + 4 * total, d);
+    if (hipMemcpyAsync(s->imu->q, q, sizeof(q), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(&s->imu->last_cloud_end, &cloud_end_time, sizeof(double), hipMemcpyHostToDevice, st) !=
+            hipSuccess)
+      rc = VMP_EDEVICE;
+  }
+  if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess) rc = VMP_EDEVICE;
+  hipFree(buf);
+  if (rc != VMP_OK) {
+    s->err = "IMU initialisation failed on the device";
+    return rc;
+  }
+  s->imu_inited = true;
+  s->init_t.clear();
+  s->init_acc.clear();
+  s->init_gyro.clear();
+  if (initialized) *initialized = 1;
+  return VMP_OK;
+}
+
+// stage a package's IMU samples (c->mu held); the copy is ordered on the
+// context's stream before the step that reads them
+static int stage_imu(vmp_session* s, const double* it, const double* iacc, const double* igyro, int32_t m,
+                     double cloud_start, double cloud_end) {
+  if (s->stage_pending) VMP_HIP(hipEventSynchronize(s->stage_done));
+  ImuPkg& p = *s->stage;
+  p.cloud_start = cloud_start;
+  p.cloud_end = cloud_end;
+  p.m = m;
+  p.pad = 0;
+  std::memcpy(p.t, it, sizeof(double) * m);
+  std::memcpy(p.acc, iacc, sizeof(double) * 3 * m);
+  std::memcpy(p.gyro, igyro, sizeof(double) * 3 * m);
+  VMP_HIP(hipMemcpyAsync(&s->imu->pkg, s->stage, sizeof(ImuPkg), hipMemcpyHostToDevice, s->ctx->stream));
+  VMP_HIP(hipEventRecord(s->stage_done, s->ctx->stream));
+  s->stage_pending = true;
+  return VMP_OK;
+}
+
+static int check_raw_args(vmp_session* s, int32_t n, const double* it, const double* iacc, const double* igyro,
+                          int32_t m, double cloud_start, double cloud_end) {
+  if (!s->imu_inited) {
+    s->err = "raw package before IMU initialisation (vmp_imu_init)";
+    return VMP_ESTATE;
+  }
+  if (n < 0 || n > s->cfg.max_scan_points || m < 1 || m > kMaxImu || !it || !iacc || !igyro ||
+      !std::isfinite(cloud_start) || !std::isfinite(cloud_end)) {
+    s->err = "raw package: point count, 1..512 IMU samples or cloud times out of range";
+    return VMP_EINVAL;
+  }
+  for (int32_t i = 0; i < m; ++i) {
+    bool ok = std::isfinite(it[i]) && (i == 0 || it[i] >= it[i - 1]);
+    for (int k = 0; k < 3; ++k) ok = ok && std::isfinite(iacc[3 * i + k]) && std::isfinite(igyro[3 * i + k]);
+    if (!ok) {
+      s->err = "raw package: non-finite or unordered IMU sample";
+      return VMP_EINVAL;
+    }
+  }
+  return VMP_OK;
+}
+
+extern "C" int vmp_scan_set_raw(vmp_session* s, const float* x, const float* y, const float* z, const float* t_ms,
+                                int32_t n, const double* it, const double* iacc, const double* igyro, int32_t m,
+                                double cloud_start, double cloud_end) {
+  if (!s) return VMP_EINVAL;
+  int rc = check_raw_args(s, n, it, iacc, igyro, m, cloud_start, cloud_end);
+  if (rc != VMP_OK) return rc;
+  if (n > 0 && (!x || !y || !z || !t_ms)) return VMP_EINVAL;
+  if (!host_points_finite(x, y, z, n)) {
+    s->err = "scan holds a non-finite coordinate";
+    return VMP_EINVAL;
+  }
+  for (int32_t i = 0; i < n; ++i)
+    if (!std::isfinite(t_ms[i]) || (i > 0 && !(t_ms[i] >= t_ms[i - 1]))) {
+      s->err = "raw scan: point times must be finite and ascending";
+      return VMP_EINVAL;
+    }
+  std::lock_guard<std::mutex> lk(s->ctx->mu);
+  VMP_HIP(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  if (n > 0) {
+    VMP_HIP(hipMemcpyAsync(s->own_x, x, sizeof(float) * n, hipMemcpyHostToDevice, st));
+    VMP_HIP(hipMemcpyAsync(s->own_y, y, sizeof(float) * n, hipMemcpyHostToDevice, st));
+    VMP_HIP(hipMemcpyAsync(s->own_z, z, sizeof(float) * n, hipMemcpyHostToDevice, st));
+    VMP_HIP(hipMemcpyAsync(s->own_t, t_ms, sizeof(float) * n, hipMemcpyHostToDevice, st));
+  }
+  rc = stage_imu(s, it, iacc, igyro, m, cloud_start, cloud_end);
+  if (rc != VMP_OK) return rc;
+  VMP_HIP(hipStreamSynchronize(st));
+  s->rx = s->own_x;
+  s->ry = s->own_y;
+  s->rz = s->own_z;
+  s->rt = s->own_t;
+  s->raw_n = n;
+  s->raw_pending = true;
+  s->cx = s->own_x;
+  s->cy = s->own_y;
+  s->cz = s->own_z;
+  s->n = n;
+  s->has_points = true;
+  return VMP_OK;
+}
+
+extern "C" int vmp_scan_set_raw_device(vmp_session* s, const float* dx, const float* dy, const float* dz,
+                                       const float* dt_ms, int32_t n, const double* it, const double* iacc,
+                                       const double* igyro, int32_t m, double cloud_start, double cloud_end) {
+  if (!s) return VMP_EINVAL;
+  int rc = check_raw_args(s, n, it, iacc, igyro, m, cloud_start, cloud_end);
+  if (rc != VMP_OK) return rc;
+  if (n > 0 && (!dx || !dy || !dz || !dt_ms)) return VMP_EINVAL;
+  vmp_ctx* c = s->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  VMP_HIP(hipSetDevice(c->device));
+  const PtsRef r{dx, dy, dz, dt_ms, n, 0};
+  const int bad = check_device_points(c, &r, 1);
+  if (bad == -2) {
+    s->err = "point validation failed on the device";
+    return VMP_EDEVICE;
+  }
+  if (bad >= 0) {
+    s->err = "raw scan: a non-finite coordinate or time, or times out of order";
+    return VMP_EINVAL;
+  }
+  rc = stage_imu(s, it, iacc, igyro, m, cloud_start, cloud_end);
+  if (rc != VMP_OK) return rc;
+  s->rx = dx;
+  s->ry = dy;
+  s->rz = dz;
+  s->rt = dt_ms;
+  s->raw_n = n;
+  s->raw_pending = true;
+  s->cx = s->own_x;
+  s->cy = s->own_y;
+  s->cz = s->own_z;
+  s->n = n;
+  s->has_points = true;
+  return VMP_OK;
+}
+
+extern "C" int32_t vmp_scan_export(vmp_session* s, int32_t cap, float* x, float* y, float* z) {
+  if (!s || cap < 0) return VMP_EINVAL;
+  std::lock_guard<std::mutex> lk(s->ctx->mu);
+  if (hipSetDevice(s->ctx->device) != hipSuccess) return VMP_EDEVICE;
+  settle(s->ctx);
+  const int32_t n = s->has_points ? s->n : 0;
+  const int32_t k = std::min(cap, n);
+  hipStream_t st = s->ctx->stream;
+  if (k > 0) {
+    if ((x && hipMemcpyAsync(x, s->cx, sizeof(float) * k, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+        (y && hipMemcpyAsync(y, s->cy, sizeof(float) * k, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+        (z && hipMemcpyAsync(z, s->cz, sizeof(float) * k, hipMemcpyDeviceToHost, st) != hipSuccess))
+      return VMP_EDEVICE;
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) return VMP_EDEVICE;
+  return n;
+}
+
+// lidarToBody: fp32 transform with the extrinsic, in PCL's order
+__global__ void k_to_body(const FilterDev* f, const float* x, const float* y, const float* z, int n, float* ox,
+                          float* oy, float* oz) {
+  __shared__ float m[9], t[3];
+  if (threadIdx.x == 0) {
+    const D33 r = d33_from_cm(f->x.rot_ext);
+    for (int i = 0; i < 9; ++i) m[i] = float(r.m[i]);
+    for (int i = 0; i < 3; ++i) t[i] = float(f->x.pos_ext[i]);
+  }
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float o[3];
+    ins_transform(m, t, x[i], y[i], z[i], o);
+    ox[i] = o[0];
+    oy[i] = o[1];
+    oz[i] = o[2];
+  }
+}
+
+extern "C" int32_t vmp_cloud_body_export(vmp_session* s, int32_t cap, float* x, float* y, float* z) {
+  if (!s || cap < 0) return VMP_EINVAL;
+  std::lock_guard<std::mutex> lk(s->ctx->mu);
+  if (hipSetDevice(s->ctx->device) != hipSuccess) return VMP_EDEVICE;
+  settle(s->ctx);
+  const int32_t n = s->cloud_n;
+  const int32_t k = std::min(cap, n);
+  if (k <= 0) return n;
+  hipStream_t st = s->ctx->stream;
+  float* buf = nullptr;
+  if (hipMalloc(&buf, sizeof(float) * 3 * size_t(k)) != hipSuccess) return VMP_ENOMEM;
+  hipLaunchKernelGGL(k_to_body, dim3(std::max(1, std::min((k + 255) / 256, 1024))), dim3(256), 0, st, s->f,
+                     s->own_x, s->own_y, s->own_z, k, buf, buf + k, buf + 2 * k);
+  int rc = n;
+  if ((x && hipMemcpyAsync(x, buf, sizeof(float) * k, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+      (y && hipMemcpyAsync(y, buf + k, sizeof(float) * k, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+      (z && hipMemcpyAsync(z, buf + 2 * k, sizeof(float) * k, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+      hipStreamSynchronize(st) != hipSuccess)
+    rc = VMP_EDEVICE;
+  hipFree(buf);
+  return rc;
+}
+
+// Eigen::Quaterniond(const Matrix3d&): Shepperd's method (largest of the
+// trace and the diagonal picks the branch)
+static void quat_from_rot(const D33& R, double q[4]) {
+  const double* m = R.m;
+  const double t = m[0] + m[4] + m[8];
+  double w, v[3];
+  if (t > 0) {
+    double r = std::sqrt(t + 1.0);
+    w = 0.5 * r;
+    r = 0.5 / r;
+    v[0] = (m[7] - m[5]) * r;
+    v[1] = (m[2] - m[6]) * r;
+    v[2] = (m[3] - m[1]) * r;
+  } else {
+    int i = 0;
+    if (m[4] > m[0]) i = 1;
+    if (m[8] > m[i * 4]) i = 2;
+    const int j = (i + 1) % 3, k = (j + 1) % 3;
+    double r = std::sqrt(m[i * 4] - m[j * 4] - m[k * 4] + 1.0);
+    v[i] = 0.5 * r;
+    r = 0.5 / r;
+    w = (m[k * 3 + j] - m[j * 3 + k]) * r;
+    v[j] = (m[j * 3 + i] + m[i * 3 + j]) * r;
+    v[k] = (m[k * 3 + i] + m[i * 3 + k]) * r;
+  }
+  q[0] = w;
+  q[1] = v[0];
+  q[2] = v[1];
+  q[3] = v[2];
+}
+
+extern "C" int vmp_odom_export(vmp_session* s, double pos[3], double quat_wxyz[4], double cov6x6[36]) {
+  if (!s) return VMP_EINVAL;
+  vmp_state x;
+  std::vector<double> P(529);
+  const int rc = vmp_state_get(s, &x, P.data());
+  if (rc != VMP_OK) return rc;
+  if (pos)
+    for (int i = 0; i < 3; ++i) pos[i] = x.pos[i];
+  if (quat_wxyz) quat_from_rot(d33_from_cm(x.rot), quat_wxyz);
+  if (cov6x6)
+    for (int r = 0; r < 6; ++r)
+      for (int c = 0; c < 6; ++c) cov6x6[r * 6 + c] = P[size_t(c) * 23 + size_t(r)];
   return VMP_OK;
 }
 
@@ -1119,6 +1447,87 @@ static void collect_timing(vmp_ctx* c, int active_max) {
 // sure no session is in two batches at once) and distribute the results
 static int step_finish(vmp_ctx* c, vmp_obs_stats* ost, vmp_map_stats* mst);
 
+// undistortCloud of the sessions that carry a raw package (IMU propagation of
+// x and P, then the points compensated into the sweep's end frame) and, for
+// registered scans with scan_resolution > 0, the VoxelGrid filter, whose
+// output counts are read back before the step is sized (c->mu held, the
+// workspace reserved for the raw points)
+static int prepare_scans(vmp_ctx* c, vmp_session** ss, int n, bool is_build) {
+  BatchWs& w = c->ws;
+  hipStream_t st = c->stream;
+  int npre = 0;
+  int64_t most = 0;
+  for (int i = 0; i < n; ++i) {
+    vmp_session* q = ss[i];
+    if (!q->raw_pending) continue;
+    w.h_pre[npre++] = PreJob{q->imu, q->f, q->rx, q->ry, q->rz, q->rt, q->own_x, q->own_y, q->own_z, q->raw_n, 0};
+    most = std::max<int64_t>(most, q->raw_n);
+  }
+  if (npre == 0) return VMP_OK;
+  if (hipMemcpyAsync(w.d_pre, w.h_pre, sizeof(PreJob) * npre, hipMemcpyHostToDevice, st) != hipSuccess)
+    return VMP_EDEVICE;
+  hipLaunchKernelGGL(k_imu_predict, dim3(npre), dim3(64), 0, st, w.d_pre, npre);
+  const int gx = int(std::max<int64_t>(1, std::min<int64_t>((most + 255) / 256, 1024)));
+  hipLaunchKernelGGL(k_undistort, dim3(gx, std::min(npre, 65535)), dim3(256), 0, st, w.d_pre, npre);
+  // the filter (never at MAP_INIT, lio_builder.cpp:185-211)
+  std::vector<vmp_session*> fs;
+  int64_t off = 0;
+  most = 0;
+  for (int i = 0; i < n && !is_build; ++i) {
+    vmp_session* q = ss[i];
+    if (!q->raw_pending || !(q->cfg.scan_resolution > 0.0) || !q->own_fx) continue;
+    const float leaf = float(q->cfg.scan_resolution);
+    w.h_vg[fs.size()] = VgJob{q->own_x, q->own_y, q->own_z, q->own_fx, q->own_fy, q->own_fz, q->raw_n, int32_t(off),
+                              1.0f / leaf, 0};
+    off += q->raw_n;
+    most = std::max<int64_t>(most, q->raw_n);
+    fs.push_back(q);
+  }
+  for (int i = 0; i < n; ++i)
+    if (ss[i]->raw_pending) {
+      ss[i]->raw_pending = false;
+      ss[i]->cloud_n = ss[i]->raw_n;
+    }
+  const int nf = int(fs.size());
+  if (nf == 0) return VMP_OK;
+  BatchBufs& b = w.b;
+  const int ntot = int(off);
+  if (hipMemcpyAsync(w.d_vg, w.h_vg, sizeof(VgJob) * nf, hipMemcpyHostToDevice, st) != hipSuccess) return VMP_EDEVICE;
+  const int gy = std::min(nf, 65535);
+  const int gxp = int(std::max<int64_t>(1, std::min<int64_t>((most + 255) / 256, 256)));
+  hipLaunchKernelGGL(k_vg_init, dim3((nf + 63) / 64), dim3(64), 0, st, w.d_vgst, nf);
+  hipLaunchKernelGGL(k_vg_minmax, dim3(gxp, gy), dim3(256), 0, st, w.d_vg, nf, w.d_vgst);
+  hipLaunchKernelGGL(k_vg_keys, dim3(gxp, gy), dim3(256), 0, st, w.d_vg, nf, w.d_vgst, b.skey_in, b.sval_in);
+  int sbits = 0;
+  while ((1 << sbits) < nf) ++sbits;
+  size_t tb = w.temp_bytes;
+  if (ntot > 0) {
+    if (rocprim::radix_sort_pairs(w.temp, tb, b.skey_in, b.skey_out, b.sval_in, b.sval_out, size_t(ntot), 0,
+                                  32 + sbits, st) != hipSuccess)
+      return VMP_EDEVICE;
+    const int gn = grid_for(ntot, 256);
+    hipLaunchKernelGGL(k_vg_heads, dim3(gn), dim3(256), 0, st, b.skey_out, ntot, b.flags);
+    tb = w.temp_bytes;
+    if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt + 6, size_t(ntot),
+                        st) != hipSuccess)
+      return VMP_EDEVICE;
+    hipLaunchKernelGGL(k_vg_ranges, dim3((nf + 63) / 64), dim3(64), 0, st, w.d_vg, nf, b.sel, b.cnt + 6, w.d_vgst);
+    hipLaunchKernelGGL(k_vg_centroid, dim3(gn), dim3(256), 0, st, w.d_vg, b.skey_out, b.sval_out, b.sel, b.cnt + 6,
+                       ntot, w.d_vgst);
+  }
+  if (hipMemcpyAsync(w.h_vgst, w.d_vgst, sizeof(int32_t) * kVgStat * nf, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return VMP_EDEVICE;
+  for (int k = 0; k < nf; ++k) {
+    vmp_session* q = fs[size_t(k)];
+    q->n = ntot > 0 ? w.h_vgst[k * kVgStat + 8] : 0;
+    q->cx = q->own_fx;
+    q->cy = q->own_fy;
+    q->cz = q->own_fz;
+  }
+  return VMP_OK;
+}
+
 // validate and launch a batch on the stream and workspace of context `c` (the
 // sessions may belong to other contexts of the same device; the caller makes
 // sure no session is in two batches at once); c->mu held, no batch in flight
@@ -1158,6 +1567,11 @@ static int step_begin(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         ss[i]->err = "build (MAP_INIT) requires an empty map";
         return VMP_ESTATE;
       }
+  }
+  rc = prepare_scans(c, ss, n, (mode & kStepBuild) != 0);
+  if (rc != VMP_OK) {
+    s->err = "undistortion / scan filter failed";
+    return rc;
   }
   c->obs_launches = 0;
   rc = enqueue_step(c, ss, n, mode);
@@ -1300,6 +1714,7 @@ extern "C" int vmp_observe(vmp_session* s, const vmp_state* x, double* H, double
   if (ws_reserve(c, std::max(s->n, 1), 1) != VMP_OK) return VMP_ENOMEM;
   VMP_HIP(hipMemcpyAsync(&s->f->x, x, sizeof(vmp_state), hipMemcpyHostToDevice, c->stream));
   vmp_session* one[1] = {s};
+  if (prepare_scans(c, one, 1, false) != VMP_OK) return VMP_EDEVICE;
   if (enqueue_step(c, one, 1, kStepObserveOnce) != VMP_OK) return VMP_EDEVICE;
   double out[156];
   int32_t ctr[8];
@@ -1333,6 +1748,11 @@ extern "C" int vmp_observe(vmp_session* s, const vmp_state* x, double* H, double
 // registrations are the same as without tuning.  c->mu held.
 static int autotune_locked(vmp_ctx* c, vmp_session** ss, int n, int reps, int32_t* table, float* cand_ms) {
   if (hipSetDevice(c->device) != hipSuccess) return VMP_EDEVICE;
+  for (int i = 0; i < n; ++i)
+    if (ss[i]->raw_pending) {  // its timing passes would undistort the package again and again
+      ss[i]->err = "vmp_autotune_observe with a raw package pending";
+      return VMP_ESTATE;
+    }
   settle(c);
   int64_t ntot = 0;
   for (int i = 0; i < n; ++i) ntot += ss[i]->n;

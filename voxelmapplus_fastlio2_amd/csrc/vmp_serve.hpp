@@ -316,7 +316,7 @@ extern "C" int vmp_server_run(vmp_server* sv, vmp_request* reqs, int32_t n, cons
   {
     // every request's points are checked once, before the run's clock starts
     std::vector<PtsRef> refs(static_cast<size_t>(n));
-    for (int32_t i = 0; i < n; ++i) refs[size_t(i)] = PtsRef{reqs[i].x, reqs[i].y, reqs[i].z, reqs[i].n, 0};
+    for (int32_t i = 0; i < n; ++i) refs[size_t(i)] = PtsRef{reqs[i].x, reqs[i].y, reqs[i].z, nullptr, reqs[i].n, 0};
     vmp_ctx* c = sv->home;
     std::lock_guard<std::mutex> lk(c->mu);
     hipSetDevice(c->device);

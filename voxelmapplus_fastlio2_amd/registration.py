@@ -10,6 +10,10 @@ voxel_plus/src/map_builder/):
   Registrar.build       VoxelMap::build via MAP_INIT, lio_builder.cpp:185-211
   Registrar.map_update  VoxelMap::update via lio_builder.cpp:231-246
   Registrar.register    the LIO_MAPPING branch after undistortion
+  Registrar.imu_init    LIOBuilder::initializeImu, lio_builder.cpp:28-63
+  Registrar.set_raw     a SyncPackage (raw points + IMU) whose undistortCloud
+                        (lio_builder.cpp:65-153) the next call runs on the device
+  LIOBuilder.process    LIOBuilder::process, lio_builder.cpp:175-248
 
 State vectors use one flat layout of 36 float64 values:
 pos[3], rot[9], rot_ext[9], pos_ext[3], vel[3], bg[3], ba[3], g[3], with the
@@ -48,6 +52,7 @@ class LIOConfig:
     max_scan_points: int = 20000
     max_iter: int = 5
     estimate_ext: int = 0
+    scan_resolution: float = 0.0
 
     def to_c(self):
         return _lib.MapCfg(
@@ -63,7 +68,34 @@ class LIOConfig:
             self.max_scan_points,
             self.max_iter,
             self.estimate_ext,
+            self.scan_resolution,
         )
+
+
+@dataclass
+class ImuConfig:
+    """IMU side of LIOConfig (lio_builder.h:19-28, lio_node.cpp:76-88)."""
+
+    na: float = 0.01
+    ng: float = 0.01
+    nba: float = 0.0001
+    nbg: float = 0.0001
+    imu_init_num: int = 20
+    gravity_align: bool = True
+    r_il: tuple = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)  # row-major, as the yaml lists it
+    p_il: tuple = (0.0, 0.0, 0.0)
+
+    def to_c(self):
+        c = _lib.ImuCfg()
+        c.na, c.ng, c.nba, c.nbg = self.na, self.ng, self.nba, self.nbg
+        c.imu_init_num = self.imu_init_num
+        c.gravity_align = 1 if self.gravity_align else 0
+        R = np.asarray(self.r_il, np.float64).reshape(3, 3)
+        for i, v in enumerate(R.T.reshape(-1)):  # column-major
+            c.r_il[i] = float(v)
+        for i in range(3):
+            c.p_il[i] = float(self.p_il[i])
+        return c
 
 
 def state_vector(R=None, t=None):
@@ -221,6 +253,67 @@ class Registrar:
             "vmp_scan_set_points_device",
         )
         self.n = int(n)
+
+    # ---- IMU side
+    def imu_init(self, imu, cloud_end, imu_cfg=None):
+        """Add an IMU_INIT package's samples; True once initialised."""
+        t, acc, gyro = (np.ascontiguousarray(a, np.float64) for a in imu)
+        done = C.c_int32()
+        c = (imu_cfg or ImuConfig()).to_c()
+        self._check(
+            self.lib.vmp_imu_init(self.h, C.byref(c), _dptr(t), _dptr(acc), _dptr(gyro), len(t), float(cloud_end),
+                                  C.byref(done)),
+            "vmp_imu_init",
+        )
+        return bool(done.value)
+
+    def set_raw(self, x, y, z, t_ms, imu, cloud_start, cloud_end):
+        """A raw package from host arrays (points in scan time order)."""
+        x, y, z, t_ms = (np.ascontiguousarray(a, dtype=np.float32) for a in (x, y, z, t_ms))
+        it, acc, gyro = (np.ascontiguousarray(a, np.float64) for a in imu)
+        self._check(
+            self.lib.vmp_scan_set_raw(self.h, _fptr(x), _fptr(y), _fptr(z), _fptr(t_ms), len(x), _dptr(it),
+                                      _dptr(acc), _dptr(gyro), len(it), float(cloud_start), float(cloud_end)),
+            "vmp_scan_set_raw",
+        )
+        self.n = len(x)
+
+    def set_raw_device(self, px, py, pz, pt, n, imu, cloud_start, cloud_end):
+        """A raw package whose points and times already live in HBM."""
+        it, acc, gyro = (np.ascontiguousarray(a, np.float64) for a in imu)
+        self._check(
+            self.lib.vmp_scan_set_raw_device(self.h, C.c_void_p(px), C.c_void_p(py), C.c_void_p(pz), C.c_void_p(pt),
+                                             int(n), _dptr(it), _dptr(acc), _dptr(gyro), len(it), float(cloud_start),
+                                             float(cloud_end)),
+            "vmp_scan_set_raw_device",
+        )
+        self.n = int(n)
+
+    def export_scan(self):
+        """The points of the last registration (undistorted, filtered), 3 x n."""
+        n = int(self.lib.vmp_scan_export(self.h, 0, None, None, None))
+        out = np.zeros((3, max(n, 1)), np.float32)
+        k = self.lib.vmp_scan_export(self.h, n, _fptr(out[0]), _fptr(out[1]), _fptr(out[2]))
+        if k < 0:
+            raise VmpError(f"vmp_scan_export failed ({k})")
+        return out[:, :n]
+
+    def export_cloud_body(self):
+        """The last package's undistorted cloud in the body frame, 3 x n."""
+        n = int(self.lib.vmp_cloud_body_export(self.h, 0, None, None, None))
+        out = np.zeros((3, max(n, 1)), np.float32)
+        k = self.lib.vmp_cloud_body_export(self.h, n, _fptr(out[0]), _fptr(out[1]), _fptr(out[2]))
+        if k < 0:
+            raise VmpError(f"vmp_cloud_body_export failed ({k})")
+        return out[:, :n]
+
+    def odom(self):
+        """(position, quaternion w x y z, 6x6 covariance) of PointCloudWithOdom."""
+        pos = np.zeros(3)
+        q = np.zeros(4)
+        cov = np.zeros(36)
+        self._check(self.lib.vmp_odom_export(self.h, _dptr(pos), _dptr(q), _dptr(cov)), "vmp_odom_export")
+        return pos, q, cov.reshape(6, 6)
 
     # ---- path entry points
     def observe(self, s):
@@ -396,3 +489,50 @@ def register_batch_wait(ctx, n):
     if rc != 0:
         raise VmpError(f"vmp_register_batch_wait failed ({rc})")
     return o, m
+
+
+IMU_INIT, MAP_INIT, LIO_MAPPING = 0, 1, 2  # LIOStatus (lio_builder.h:10-16)
+
+
+class LIOBuilder:
+    """lio::LIOBuilder over one device session: process() takes one
+    SyncPackage (raw points with per-point times, the IMU samples up to the
+    scan's end, cloud start/end times) through the reference's state machine
+    (lio_builder.cpp:175-248): IMU_INIT averages IMU samples, MAP_INIT
+    undistorts and builds the map, LIO_MAPPING undistorts, filters (with
+    scan_resolution > 0), updates the IESKF and the map.  Everything after
+    IMU_INIT runs in one device call per package."""
+
+    def __init__(self, ctx, cfg=None, imu_cfg=None):
+        self.cfg = cfg or LIOConfig()
+        self.imu_cfg = imu_cfg or ImuConfig()
+        self.reg = Registrar(ctx, self.cfg)
+        self.status = IMU_INIT
+        self.last_obs = None
+        self.last_map = None
+
+    def close(self):
+        self.reg.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def process(self, x, y, z, t_ms, imu, cloud_start, cloud_end, device=None):
+        """device = (px, py, pz, pt) HBM addresses of the same points, if any."""
+        if self.status == IMU_INIT:
+            if self.reg.imu_init(imu, cloud_end, self.imu_cfg):
+                self.status = MAP_INIT
+            return self.status
+        if device is None:
+            self.reg.set_raw(x, y, z, t_ms, imu, cloud_start, cloud_end)
+        else:
+            self.reg.set_raw_device(*device, len(x), imu, cloud_start, cloud_end)
+        if self.status == MAP_INIT:
+            self.last_map = self.reg.build()
+            self.status = LIO_MAPPING
+        else:
+            self.last_obs, self.last_map = self.reg.register()
+        return self.status
