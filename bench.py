@@ -1,29 +1,30 @@
 #!/usr/bin/env python3
 """Registered points/s of the voxel_plus registration path on one MI355X.
 
-A step is one registered scan: the IESKF update (<= 5 observe passes of
-sharedUpdateFunc + the FP64 filter algebra) followed by the map update
-(VoxelMap::update), on a seeded synthetic street sequence of 120,000 points
-per scan (0.5 m voxels, 2,000,000-voxel LRU).  Scans are generated on the
-host and copied to HBM before timing; the timed region is the device path
-with one host synchronisation per scan (the plain, single-scan form), so the
-per-scan wall time is also the latency reported as p50 / p99.
+A step is one registered scan, the reference's LIO_MAPPING work per
+SyncPackage (lio_builder.cpp:212-247): undistortCloud (IMU propagation of the
+filter state over the package's 200 Hz samples and backward compensation of
+every point), the IESKF update (<= 5 observe passes of sharedUpdateFunc + the
+FP64 filter algebra) and the map update (VoxelMap::update).  The workload is
+a seeded synthetic street sequence of 120,000 points per raw scan (0.5 m
+voxels, 2,000,000-voxel LRU).  Before the warmup the sequence is registered
+from its start until the map is at its capacity and evicting (the steady
+state of a long street run); the timed scans follow on.  Raw scans and their
+per-point times are copied to HBM before timing; the IMU samples of a package
+(about 20) go with each step.  The timed region is the device path with one
+host synchronisation per scan (the plain, single-scan form), so the per-scan
+wall time is also the latency reported as p50 / p99.
 
-The same JSON line carries the live roofline figure of the observe kernel
-(HIP events around every launch on its stream) and the FP64 oracle timed on
-the host over a bounded sample of the same workload.
+The same JSON line carries the roofline figure of the kernel that takes the
+most time per scan (HIP events around every launch on the library's stream,
+over scans that follow the timed region) and the FP64 oracle timed on the
+host over a bounded sample of the same workload.
 """
 import argparse
 import json
 import os
 import sys
 import time
-
-# The batched form runs four stream groups; HIP gives a process 4 hardware
-# queues by default, so streams beyond that would share queues and serialise.
-# Raised (never lowered) before anything initialises HIP.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 import numpy as np
 
@@ -32,55 +33,78 @@ sys.path.insert(0, ROOT)
 
 METRIC = "registered points/s (batched vs plain, same GPU); p50/p99 scan latency"
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes of one observe pass (see the observe kernel's header):
-# every point reads xyz (12 B), its residual slot (16 B) and one hash slot
-# (16 B); a hit also reads the 32 B plane record and writes the slot (16 B).
-OBS_BYTES_POINT = 44
-OBS_BYTES_HIT = 48
+# Algorithmic bytes per unit of work (the per-scan byte model of the path):
+# an observe pass moves 76 B per point (xyz 12, hash probe 16, plane record
+# 32, residual slot 16); an insertion 64 B per point (xyz 12, hash probe 16,
+# retained point + symmetric cov 36); a voxel replayed through a refit
+# 168 B (accumulator read + write 2 x 80, LRU stamp 8); a merge() evaluation
+# 288 B (6 x (16 B probe + 32 B plane)); an LRU creation or eviction
+# decision 32 B (ring entry 16, state and stamp 16).
+OBS_BYTES_POINT = 76
+INS_BYTES_POINT = 64
+VOXEL_BYTES = 168
+MERGE_BYTES = 288
+LRU_BYTES = 32
 # observe launches timed back to back per event pair in the roofline pass
-# (the event pair and the first dispatch add a fixed ~15 us, spread over them)
-OBS_TIME_REPS = 32
+OBS_TIME_REPS = 8
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scans, motions, cfg, s0, P0, n_warm, n_timed):
-    """FP64 oracle (one host thread) on the first scans of the same workload."""
-    from oracle import Oracle
-    from voxelmapplus_fastlio2_amd.synth import QDIAG
+def host_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)), "cpu_model": model}
 
-    orc = Oracle(cfg)
-    orc.set_state(s0, P0)
-    orc.build(*scans[0])
+
+def make_oracle(cfg):
+    from oracle import Oracle
+
+    o = Oracle(cfg)
+    o.imu_config()
+    o.set_scan_resolution(cfg.scan_resolution)
+    return o
+
+
+def cpu_baseline(pkgs, cfg, n_warm, n_timed):
+    """FP64 oracle (one host thread), LIOBuilder::process per package, on the
+    first packages of the same sequence: IMU_INIT, MAP_INIT, n_warm
+    registered scans, then n_timed timed ones."""
+    o = make_oracle(cfg)
     t_reg = 0.0
-    for k in range(1, 1 + n_warm + n_timed):
-        orc.apply_motion(*motions[k], QDIAG)
+    for k in range(2 + n_warm + n_timed):
         t0 = time.perf_counter()
-        orc.register(*scans[k])
+        o.process(*pkgs[k])
         dt = time.perf_counter() - t0
-        if k > n_warm:
+        if k >= 2 + n_warm:
             t_reg += dt
-    n = len(scans[0][0])
+    n = len(pkgs[0][0])
     return {
         "value": n * n_timed / t_reg,
         "unit": "points/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"street {n} pts/scan, scans {n_warm + 1}-{n_warm + n_timed} after MAP_INIT "
-        f"({n_timed} registered scans, {t_reg:.1f} s), FP64 oracle, 1 thread",
+        "sample": f"street {n} pts/scan, packages {2 + n_warm}-{1 + n_warm + n_timed} of the sequence (after IMU_INIT, "
+        f"MAP_INIT and {n_warm} registered scans; map {o.map_size()} voxels at the end; {n_timed} timed, "
+        f"{t_reg:.1f} s), FP64 oracle incl. undistortion, 1 thread; the timed GPU scans run on a full "
+        "2M-voxel map, so the per-point CPU cost here is a lower bound",
     }
 
 
-def cpu_baseline_sessions(scans, motions, cfg, s0, P0, n_warm, n_timed, threads):
+def cpu_baseline_sessions(pkgs, cfg, n_warm, n_timed, threads):
     """FP64 oracle with one independent session per host thread, all threads
     registering at once (the host-CPU form of the batched / serving workload):
     aggregate points/s over the timed scans."""
     import threading
-
-    from oracle import Oracle
-    from voxelmapplus_fastlio2_amd.synth import QDIAG
 
     ready = threading.Barrier(threads + 1)
     done = threading.Barrier(threads + 1)
@@ -88,18 +112,14 @@ def cpu_baseline_sessions(scans, motions, cfg, s0, P0, n_warm, n_timed, threads)
 
     def worker():
         try:
-            orc = Oracle(cfg)
-            orc.set_state(s0, P0)
-            orc.build(*scans[0])
-            for k in range(1, 1 + n_warm):
-                orc.apply_motion(*motions[k], QDIAG)
-                orc.register(*scans[k])
+            o = make_oracle(cfg)
+            for k in range(2 + n_warm):
+                o.process(*pkgs[k])
             ready.wait()
-            for k in range(1 + n_warm, 1 + n_warm + n_timed):
-                orc.apply_motion(*motions[k], QDIAG)
-                orc.register(*scans[k])
+            for k in range(2 + n_warm, 2 + n_warm + n_timed):
+                o.process(*pkgs[k])
             done.wait()
-            orc.close()
+            o.close()
         except Exception as e:  # reported, never hangs the barrier
             errors.append(repr(e))
             ready.abort()
@@ -116,97 +136,104 @@ def cpu_baseline_sessions(scans, motions, cfg, s0, P0, n_warm, n_timed, threads)
         t.join()
     if errors:
         raise RuntimeError(errors[0])
-    n = len(scans[0][0])
+    n = len(pkgs[0][0])
     return {
         "value": n * n_timed * threads / wall,
         "unit": "points/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{threads} independent street sessions of {n} pts/scan, one per host thread, scans "
-        f"{n_warm + 1}-{n_warm + n_timed} after MAP_INIT registered concurrently ({wall:.1f} s), FP64 oracle",
+        "sample": f"{threads} independent street sessions of {n} pts/scan, one per host thread (all the process's "
+        f"cores), packages {2 + n_warm}-{1 + n_warm + n_timed} registered concurrently ({wall:.1f} s), FP64 oracle",
     }
 
 
+def upload_packages(pkgs, first, dev_dtype="float32"):
+    """Points and times of packages [first, len) into one HBM tensor
+    [k, 4, n] (x, y, z, t_ms); packages must share n."""
+    import torch
+
+    n = len(pkgs[first][0])
+    dev = torch.empty((len(pkgs) - first, 4, n), dtype=torch.float32, device="cuda:0")
+    for k in range(first, len(pkgs)):
+        x, y, z, t = pkgs[k][:4]
+        dev[k - first].copy_(torch.from_numpy(np.stack([x, y, z, t])))
+    return dev
+
+
 def run_batched(ctx, B, W, K, capacity, nmax=120000, groups=1):
-    """B independent sequences (room / street alternating), per-scan sizes
-    uniform in [2k, nmax], advanced together by vmp_register_batch.  With
-    groups > 1 the sessions are split round-robin into that many groups, each
-    on its own context (HIP stream) and host thread, so one group's
-    latency-bound stages overlap another group's bandwidth-bound ones."""
+    """B independent sequences (room / street alternating), raw scans of
+    sizes uniform in [2k, nmax] with their IMU packages, advanced together by
+    vmp_register_batch (undistortion, IESKF and map update of every session
+    in one set of launches per stage).  With groups > 1 the sessions are
+    split round-robin into that many groups, each on its own context (HIP
+    stream) and host thread, so one group's latency-bound stages overlap
+    another group's bandwidth-bound ones."""
     import threading
 
     import torch
 
     from voxelmapplus_fastlio2_amd import (
+        Context,
         LIOConfig,
         Registrar,
-        apply_motion_batch,
-        initial_covariance,
         register_batch,
         register_batch_submit,
         register_batch_wait,
-        set_points_device_batch,
-        state_vector,
+        set_raw_device_batch,
     )
-    from voxelmapplus_fastlio2_amd.synth import QDIAG, ROOM, SEED0, STREET, Sequence
+    from voxelmapplus_fastlio2_amd.synth import ROOM, SEED0, STREET, ImuSequence
 
     rng = np.random.default_rng(0xBA7C4)
-    total = W + K + 1
+    total = W + K + 2  # IMU_INIT and MAP_INIT packages first
     sizes = rng.integers(2000, nmax + 1, size=(total, B))
-    seqs = [Sequence(ROOM if i % 2 == 0 else STREET, SEED0 + 1000 + i, nmax) for i in range(B)]
+    seqs = [ImuSequence(ROOM if i % 2 == 0 else STREET, SEED0 + 1000 + i, nmax) for i in range(B)]
     t0 = time.perf_counter()
     steps = []
     for k in range(total):
         offs = np.concatenate([[0], np.cumsum(sizes[k])])
-        buf = torch.empty((3, int(offs[-1])), dtype=torch.float32)
+        buf = torch.empty((4, int(offs[-1])), dtype=torch.float32)
+        meta = []
         for i in range(B):
-            x, y, z = seqs[i].scan(k, sizes[k, i])
-            buf[0, offs[i] : offs[i + 1]] = torch.from_numpy(x)
-            buf[1, offs[i] : offs[i + 1]] = torch.from_numpy(y)
-            buf[2, offs[i] : offs[i + 1]] = torch.from_numpy(z)
-        steps.append((buf.to("cuda:0"), offs))
-    motions = [[None] * B] + [[seqs[i].motion(k) for i in range(B)] for k in range(1, total)]
-    log(f"batched: generated {total} steps x {B} sessions ({int(sizes.sum())} points) in {time.perf_counter() - t0:.1f} s")
+            x, y, z, t, imu, cs, ce = seqs[i].package(k, sizes[k, i])
+            buf[:, offs[i] : offs[i + 1]] = torch.from_numpy(np.stack([x, y, z, t]))
+            meta.append((imu, cs, ce))
+        steps.append((buf.to("cuda:0"), offs, meta))
+    log(f"batched: generated {total} packages x {B} sessions ({int(sizes.sum())} points) in "
+        f"{time.perf_counter() - t0:.1f} s")
     cfg = LIOConfig(max_scan_points=nmax, map_capacity=capacity)
-    from voxelmapplus_fastlio2_amd import Context
-
     ctxs = [ctx] + [Context(0) for _ in range(groups - 1)]
-
-    # sessions alternate room / street; each group gets both kinds (or, with
-    # VMP_BENCH_GROUPING=stride, every groups-th session)
-    stride = os.environ.get("VMP_BENCH_GROUPING") == "stride"
-    gof = [(i % groups) if stride else ((i // 2) % groups) for i in range(B)]
+    # sessions alternate room / street; each group gets both kinds
+    gof = [(i // 2) % groups for i in range(B)]
     members = [[i for i in range(B) if gof[i] == gi] for gi in range(groups)]
     regs = [Registrar(ctxs[gof[i]], cfg) for i in range(B)]
     for gi, cx in enumerate(ctxs):  # the largest step's workspace, allocated before timing
         cx.reserve(int(sizes[:, members[gi]].sum(axis=1).max()), len(members[gi]))
 
     def bind(k, idx):
-        dev, offs = steps[k]
+        dev, offs, meta = steps[k]
         base = dev.data_ptr()
         stride = dev.stride(0) * 4
         idx = np.asarray(list(idx))
         xo = base + offs[idx].astype(np.uint64) * 4
-        set_points_device_batch([regs[i] for i in idx], xo, xo + stride, xo + 2 * stride, sizes[k, idx])
+        set_raw_device_batch([regs[i] for i in idx], xo, xo + stride, xo + 2 * stride, xo + 3 * stride,
+                             sizes[k, idx], [meta[i][0] for i in idx], [meta[i][1] for i in idx],
+                             [meta[i][2] for i in idx])
 
-    bind(0, range(B))
-    for i, g in enumerate(regs):
-        R, t = seqs[i].pose(0)
-        g.set_state(state_vector(R, t), initial_covariance())
+    for i, g in enumerate(regs):  # IMU_INIT
+        imu, cs, ce = steps[0][2][i]
+        assert g.imu_init(imu, ce), "one package holds imu_init_num samples"
+    bind(1, range(B))
+    for g in regs:  # MAP_INIT: undistortion + build
         g.build()
 
     def step(k, idx, wait=True):
         rg = [regs[i] for i in idx]
-        apply_motion_batch(rg, [motions[k][i][0] for i in idx], [motions[k][i][1] for i in idx], QDIAG)
         bind(k, idx)
         if wait:
             return register_batch(rg)
-        # launched without waiting: the host prepares the next step meanwhile
-        register_batch_submit(rg)
+        register_batch_submit(rg)  # launched without waiting: the host prepares the next step meanwhile
         return None
 
-    # observe launch shape per scan-size bucket (< 8k, < 32k, < 128k, more
-    # points), tuned on the batch itself (scan 0 of every session)
     from voxelmapplus_fastlio2_amd import autotune_observe_batch
 
     tuned = {}
@@ -214,7 +241,7 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000, groups=1):
         table, ms = autotune_observe_batch([regs[i] for i in members[gi]], reps=5)
         tuned[f"group{gi}"] = {"points_per_thread": table, "candidates_ms": ms}
     log(f"batched autotune: {tuned}")
-    for k in range(1, W + 1):
+    for k in range(2, W + 2):
         for idx in members:
             step(k, idx)
     torch.cuda.synchronize()
@@ -222,7 +249,7 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000, groups=1):
     last_mst = [None] * groups
 
     def run_group(gi):
-        for k in range(W + 1, W + K + 1):
+        for k in range(W + 2, W + K + 2):
             ts = time.perf_counter()
             step(k, members[gi], wait=False)
             lat[gi].append(time.perf_counter() - ts)
@@ -239,20 +266,21 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000, groups=1):
             t.join()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
-    pts = int(sizes[W + 1 : W + K + 1].sum())
+    pts = int(sizes[W + 2 : W + K + 2].sum())
     ev = np.zeros(B, np.int64)
     mg = np.zeros(B, np.int64)
     wn = np.zeros(B, np.int64)
     rq = np.zeros(B, np.int64)
+    evi = np.zeros(B, np.int64)
     for gi, idx in enumerate(members):
         for j, i in enumerate(idx):
             m = last_mst[gi][j]
-            ev[i], mg[i], wn[i], rq[i] = m.merge_evals, m.merges, m.merge_windows, m.merge_requeued
+            ev[i], mg[i], wn[i], rq[i], evi[i] = m.merge_evals, m.merges, m.merge_windows, m.merge_requeued, m.evicted
     h = int(ev.argmax())
     log(f"batched last step: merge evals room max {ev[0::2].max()} mean {ev[0::2].mean():.0f}, "
         f"street max {ev[1::2].max()} mean {ev[1::2].mean():.0f}; merges room mean {mg[0::2].mean():.0f} "
         f"street mean {mg[1::2].mean():.1f}; heaviest session: {ev[h]} evals, {mg[h]} merges, {wn[h]} windows, "
-        f"{rq[h]} requeued")
+        f"{rq[h]} requeued; evictions street mean {evi[1::2].mean():.0f}")
     lat = [x for grp in lat for x in grp]
     for g in regs:
         g.close()
@@ -271,8 +299,10 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000, groups=1):
         "step_host_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "submission": "asynchronous (vmp_register_batch_submit): step k+1 is prepared while step k runs",
         "map_capacity_per_session": capacity,
+        "evicted_per_street_session_last_step": float(evi[1::2].mean()),
         "observe_autotune": tuned,
-        "workload": f"{B} sequences room/street alternating, per-scan points U[2000,{nmax}]",
+        "workload": f"{B} sequences room/street alternating, raw scans of U[2000,{nmax}] points with their 200 Hz "
+        "IMU packages (undistortion on the device)",
     }
 
 
@@ -426,28 +456,81 @@ def run_serving(sessions, rate, duration, nmax=120000, nmin=2000, capacity=20000
     }
 
 
+def load_traffic():
+    """Measured HBM bytes per dispatch of each kernel, from the committed
+    rocprofv3 PMC summaries (lines "traffic <kernel> <bytes>" in
+    profiles/*.md); empty when none is there."""
+    import glob
+    import re
+
+    out = {}
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"^traffic (\S+) (\d+(?:\.\d+)?)\s*$", line.strip())
+                if m:
+                    out[m.group(1)] = (float(m.group(2)), os.path.relpath(path, ROOT))
+    return out
+
+
+def kernel_rooflines(tt, kt, stats, N, scans):
+    """Per-kernel figures over the roofline scans: time per scan, launches,
+    algorithmic bytes per launch (the model above) and achieved GB/s."""
+    rows = {}
+    if tt["obs_launches"] > 0:
+        b = OBS_BYTES_POINT * tt["obs_points"] / tt["obs_launches"]
+        rows["observe_kernel<6>"] = (tt["obs_ms"], tt["obs_launches"], b, f"{OBS_BYTES_POINT} B x points")
+    units = {
+        "k_ins_keys": (INS_BYTES_POINT * stats["points"], f"{INS_BYTES_POINT} B x points"),
+        "k_replay_heavy": (VOXEL_BYTES * stats["refits"], f"{VOXEL_BYTES} B x refits"),
+        "k_merge_replay": (MERGE_BYTES * stats["merge_evals"], f"{MERGE_BYTES} B x merge() evaluations"),
+        "k_create_evict": (LRU_BYTES * (stats["created"] + stats["evicted"]), f"{LRU_BYTES} B x (creations + evictions)"),
+        "k_undistort": (28 * stats["raw_points"], "28 B x raw points (xyz + time in, xyz out)"),
+    }
+    for name, (ms, n) in kt.items():
+        if n == 0:
+            continue
+        tot, desc = units.get(name, (0.0, "latency-bound, no byte model"))
+        rows[name] = (ms, n, tot / n, desc)
+    out = {}
+    for name, (ms, n, bpl, desc) in rows.items():
+        avg = ms / n
+        out[name] = {
+            "ms_per_scan": ms / scans,
+            "launches": n,
+            "avg_launch_ms": avg,
+            "bytes_per_launch": bpl,
+            "bytes_model": desc,
+            "achieved_gbs": bpl / (avg * 1e-3) / 1e9 if avg > 0 else 0.0,
+        }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--points", type=int, default=120000)
     ap.add_argument("--capacity", type=int, default=2000000)
-    ap.add_argument("--roofline-scans", type=int, default=20)
+    ap.add_argument("--fill", type=int, default=300,
+                    help="scans registered before the warmup so the map reaches its LRU capacity")
+    ap.add_argument("--roofline-scans", type=int, default=10)
     ap.add_argument("--cpu-warm", type=int, default=3)
-    ap.add_argument("--cpu-timed", type=int, default=10)
+    ap.add_argument("--cpu-timed", type=int, default=8)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the all-cores CPU leg (0 = all)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-sessions", type=int, default=128)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-warmup", type=int, default=5)
     ap.add_argument("--batch-capacity", type=int, default=200000)
-    ap.add_argument("--batch-groups", type=int, default=4,
+    ap.add_argument("--batch-groups", type=int, default=3,
                     help="session groups, each on its own context (HIP stream) and host thread")
     ap.add_argument("--obs-ppt", type=int, default=0, help="observe points per thread (0 = by scan size)")
     ap.add_argument("--serve-sessions", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
-    ap.add_argument("--serve-executors", type=int, default=4)
+    ap.add_argument("--serve-executors", type=int, default=3)
     ap.add_argument("--serve-max-batch", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -460,23 +543,21 @@ def main():
 
     import torch
 
-    from voxelmapplus_fastlio2_amd import Context, LIOConfig, Registrar, initial_covariance, state_vector
-    from voxelmapplus_fastlio2_amd.synth import QDIAG, SEED0, STREET, Sequence
+    from voxelmapplus_fastlio2_amd import Context, LIOConfig, Registrar
+    from voxelmapplus_fastlio2_amd.synth import SEED0, STREET, ImuSequence
 
-    N, W, K = args.points, args.warmup, args.steps
-    RF = args.roofline_scans
-    total = W + K + RF + 1
-    seq = Sequence(STREET, SEED0, N)
-    log(f"generating {total} scans x {N} points")
+    N, W, K, F, RF = args.points, args.warmup, args.steps, args.fill, args.roofline_scans
+    total = 2 + F + W + K + RF
+    seq = ImuSequence(STREET, SEED0, N)
+    log(f"generating {total} raw packages x {N} points")
     t0 = time.perf_counter()
-    host = [seq.scan(k) for k in range(total)]
-    motions = [None] + [seq.motion(k) for k in range(1, total)]
+    pkgs = []
+    for k in range(total):
+        pkgs.append(seq.package(k))
+        if (k + 1) % 100 == 0:
+            log(f"  {k + 1} packages ({time.perf_counter() - t0:.1f} s)")
     log(f"generated in {time.perf_counter() - t0:.1f} s")
-    dev = torch.empty((total, 3, N), dtype=torch.float32, device="cuda:0")
-    for k, (x, y, z) in enumerate(host):
-        dev[k, 0].copy_(torch.from_numpy(x))
-        dev[k, 1].copy_(torch.from_numpy(y))
-        dev[k, 2].copy_(torch.from_numpy(z))
+    dev = upload_packages(pkgs, 1)
     # one elementwise pass of known size (256 MiB read, 256 MiB written): the
     # reference launch that scales the FETCH_SIZE / WRITE_SIZE counters
     cal = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
@@ -485,43 +566,51 @@ def main():
     torch.cuda.synchronize()
 
     cfg = LIOConfig(max_scan_points=N, map_capacity=args.capacity)
-    R, t = seq.pose(0)
-    s0, P0 = state_vector(R, t), initial_covariance()
     ctx = Context(0)
     if args.obs_ppt:
         ctx.set_param("obs_ppt", args.obs_ppt)
     g = Registrar(ctx, cfg)
+    stride = dev.stride(1) * 4
 
     def bind(k):
-        g.set_points_device(dev[k, 0].data_ptr(), dev[k, 1].data_ptr(), dev[k, 2].data_ptr(), N)
+        base = dev[k - 1].data_ptr()
+        _, _, _, _, imu, cs, ce = pkgs[k]
+        g.set_raw_device(base, base + stride, base + 2 * stride, base + 3 * stride, N, imu, cs, ce)
 
-    g.set_state(s0, P0)
-    bind(0)
-    g.build()
+    assert g.imu_init(pkgs[0][4], pkgs[0][6]), "IMU_INIT within the first package"
+    bind(1)
+    g.build()  # MAP_INIT
     tune = None
     if not args.obs_ppt:
         # observe launch shape for this scan size, picked by timing on the
-        # session after MAP_INIT (its residual slots are restored afterwards),
-        # before the warmup so the warmup also captures the filter graphs
+        # session after MAP_INIT (its residual slots are restored afterwards)
         best, ms = ctx.autotune_observe(g, reps=10)
         tune = {"points_per_thread": best, "candidates_ms": dict(zip(["1", "2", "4", "8"], ms))}
         log(f"autotune observe ({N} points): {best} points per thread, medians {ms}")
-    for k in range(1, W + 1):
-        g.apply_motion(*motions[k], QDIAG)
+    t0 = time.perf_counter()
+    created = evicted = 0
+    for k in range(2, 2 + F):
+        bind(k)
+        _, m = g.register()
+        created += m.created
+        evicted += m.evicted
+        if (k - 1) % 50 == 0:
+            log(f"  fill: scan {k}, map {m.map_size} voxels, created {created}, evicted {evicted} "
+                f"({time.perf_counter() - t0:.1f} s)")
+    for k in range(2 + F, 2 + F + W):
         bind(k)
         g.register()
     torch.cuda.synchronize()
-    log(f"warmup done: map {g.map_size()} voxels")
+    log(f"fill + warmup done: map {g.map_size()} voxels, {evicted} evictions so far")
 
     lat = []
     maint = []
     iters = 0
-    mstat = {"created": 0, "evicted": 0, "merges": 0, "merge_evals": 0, "refits": 0}
+    mstat = {"created": 0, "evicted": 0, "merges": 0, "merge_evals": 0, "refits": 0, "touched": 0}
     torch.cuda.synchronize()
     t_start = time.perf_counter()
-    for k in range(W + 1, W + K + 1):
+    for k in range(2 + F + W, 2 + F + W + K):
         ts = time.perf_counter()
-        g.apply_motion(*motions[k], QDIAG)
         bind(k)
         o, m = g.register()
         lat.append(time.perf_counter() - ts)
@@ -536,12 +625,17 @@ def main():
     # by kernel instead of replaying its graph; the kernels are the same)
     g.set_timing(True)
     ctx.set_param("obs_time_reps", OBS_TIME_REPS)
-    for k in range(W + K + 1, W + K + RF + 1):
-        g.apply_motion(*motions[k], QDIAG)
+    rstat = {"points": 0, "raw_points": 0, "refits": 0, "merge_evals": 0, "created": 0, "evicted": 0}
+    for k in range(2 + F + W + K, total):
         bind(k)
-        g.register()
+        o, m = g.register()
+        rstat["points"] += m.points
+        rstat["raw_points"] += N
+        for f in ("refits", "merge_evals", "created", "evicted"):
+            rstat[f] += getattr(m, f)
     torch.cuda.synchronize()
     tt = g.timing_totals()
+    kt = ctx.kernel_times()
     g.set_timing(False)
     ctx.set_param("obs_time_reps", 1)
     map_size = g.map_size()
@@ -549,11 +643,15 @@ def main():
     for j in np.argsort(lat_ms)[::-1][:3]:
         log(f"slow step {j}: {lat_ms[j]:.2f} ms, maintenance flags {maint[j]}")
 
-    obs_launches = max(tt["obs_launches"], 1)
-    obs_avg_ms = tt["obs_ms"] / obs_launches
-    obs_bytes = (tt["obs_points"] * OBS_BYTES_POINT + tt["obs_hits"] * OBS_BYTES_HIT) / obs_launches
-    achieved = obs_bytes / (obs_avg_ms * 1e-3) / 1e9
+    kr = kernel_rooflines(tt, kt, rstat, N, RF)
+    for name, r in sorted(kr.items(), key=lambda kv: -kv[1]["ms_per_scan"]):
+        log(f"kernel {name}: {r['ms_per_scan'] * 1e3:.1f} us/scan over {r['launches']} launches, "
+            f"{r['achieved_gbs']:.1f} GB/s ({r['bytes_model']})")
+    dom = max(kr, key=lambda k: kr[k]["ms_per_scan"])
+    traffic = load_traffic()
     value = N * K / elapsed
+    tr = traffic.get(dom)
+    d = kr[dom]
 
     line = {
         "metric": METRIC,
@@ -566,20 +664,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 per point, f64 reductions and filter",
+        "dtype": "f32 per point, f64 reductions, filter and IMU propagation",
         "data": "synthetic",
         "config": {
-            "workload": f"street-{N // 1000}k-plain",
-            "form": "plain (one session, one scan per step, host sync per scan)",
+            "workload": f"street-{N // 1000}k-plain-steady",
+            "form": "plain (one session, one raw scan + IMU package per step, host sync per scan)",
             "points_per_scan": N,
             "voxel_size": cfg.voxel_size,
             "map_capacity": cfg.map_capacity,
+            "fill_scans": F,
             "map_voxels_end": map_size,
             "ieskf_iterations_mean": iters / K,
             "per_scan_mean": {f: v / K for f, v in mstat.items()},
-            "maintenance_events": {"hash_rebuilds": sum(1 for f in maint if f & 1), "ring_compactions": sum(1 for f in maint if f & 2)},
+            "maintenance_events": {"hash_rebuilds": sum(1 for f in maint if f & 1),
+                                   "ring_compactions": sum(1 for f in maint if f & 2)},
             "observe_autotune": tune,
-            "filter_stage": "hipGraph replay (IESKF begin, 5 x observe/reduce/step, end)",
+            "filter_stage": "hipGraph replay (IESKF begin, 5 x observe/step, end)",
         },
         "latency_ms": {
             "p50": float(np.percentile(lat_ms, 50)),
@@ -592,19 +692,22 @@ def main():
             "measured_over": f"{RF} scans after the timed region, per-launch HIP events",
         },
         "roofline": {
-            "kernel": "observe_kernel<6>",
+            "kernel": dom,
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": d["achieved_gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "bytes_per_launch": obs_bytes,
-            "avg_launch_ms": obs_avg_ms,
-            "launches": tt["obs_launches"],
-            "timed_over": f"{RF} scans following the timed region: HIP events around {OBS_TIME_REPS} back-to-back "
-            "launches of each observe pass (the kernel is idempotent), mean per launch",
+            "frac": d["achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": tr[0] if tr else None,
+            "traffic_source": tr[1] if tr else None,
+            "bytes_per_launch": d["bytes_per_launch"],
+            "bytes_model": d["bytes_model"],
+            "avg_launch_ms": d["avg_launch_ms"],
+            "launches": d["launches"],
+            "timed_over": f"{RF} scans following the timed region, HIP events around each launch on the "
+            "library's stream (observe: around back-to-back repeats of each launch, mean per launch)",
         },
+        "kernels": kr,
     }
     g.close()
     del dev
@@ -620,12 +723,15 @@ def main():
         sv["saturation_vs_plain"] = sv["saturation"]["value"] / value
         line["serving"] = sv
     if not args.no_cpu:
-        log("timing the FP64 oracle on the host")
-        line["cpu_baseline"] = cpu_baseline(host, motions, cfg, s0, P0, args.cpu_warm, args.cpu_timed)
+        info = host_info()
+        log(f"timing the FP64 oracle on the host ({info})")
+        line["cpu_baseline"] = cpu_baseline(pkgs, cfg, args.cpu_warm, args.cpu_timed)
+        line["cpu_baseline"]["host"] = info
         line["vs_cpu"] = value / line["cpu_baseline"]["value"]
-        threads = min(len(os.sched_getaffinity(0)), 16)
+        threads = args.cpu_threads or info["affinity_cores"]
         log(f"timing the FP64 oracle with {threads} concurrent sessions")
-        mt = cpu_baseline_sessions(host, motions, cfg, s0, P0, 2, 4, threads)
+        mt = cpu_baseline_sessions(pkgs, cfg, 2, 3, threads)
+        mt["host"] = info
         line["cpu_baseline_all_cores"] = mt
         if "batched" in line:
             line["batched"]["vs_cpu_all_cores"] = line["batched"]["value"] / mt["value"]

@@ -138,12 +138,13 @@ int vmp_state_apply_motion_batch(vmp_session** s, int32_t n, const double* dR, c
 
 /* Scan points in the LiDAR frame, SoA fp32, scan (time) order.  Host
  * pointers are copied; _device takes pointers that already live in HBM (the
- * buffers must hold the points when the call is made and stay unchanged
- * until the registration call that uses them returns).  A scan holding a NaN
- * or infinite coordinate is rejected with VMP_EINVAL and nothing is bound
- * (the reference would index a voxel from an undefined key); device buffers
- * are checked by one small kernel and a wait on a stream of the context's
- * own (context parameter "check_points"). */
+ * buffers must hold the points from the registration call that uses them
+ * until it returns).  A scan holding a NaN or infinite coordinate is
+ * rejected with VMP_EINVAL (the reference would index a voxel from an
+ * undefined key): host buffers when they are bound; device buffers when the
+ * next registration call starts, by one small kernel on the context's
+ * stream, before anything is launched, so the failed call changes nothing
+ * and the binding is dropped (context parameter "check_points"). */
 int vmp_scan_set_points(vmp_session* s, const float* x, const float* y, const float* z, int32_t n);
 int vmp_scan_set_points_device(vmp_session* s, const float* dx, const float* dy, const float* dz, int32_t n);
 /* The same for n sessions in one call: device addresses and point counts. */
@@ -174,6 +175,13 @@ int vmp_scan_set_raw(vmp_session* s, const float* x, const float* y, const float
 int vmp_scan_set_raw_device(vmp_session* s, const float* dx, const float* dy, const float* dz, const float* dt_ms,
                             int32_t n, const double* imu_t, const double* imu_acc, const double* imu_gyro, int32_t m,
                             double cloud_start, double cloud_end);
+/* The same for n sessions of one context: device addresses of points and
+ * times, point counts, the sessions' IMU samples concatenated (imu_counts
+ * per session) and their cloud times; one check of all point sets. */
+int vmp_scan_set_raw_device_batch(vmp_session** s, int32_t n, const uint64_t* dx, const uint64_t* dy,
+                                  const uint64_t* dz, const uint64_t* dt_ms, const int32_t* counts,
+                                  const double* imu_t, const double* imu_acc, const double* imu_gyro,
+                                  const int32_t* imu_counts, const double* cloud_start, const double* cloud_end);
 /* The points the last registration used (undistorted and filtered), SoA;
  * returns their count (at most cap are copied). */
 int32_t vmp_scan_export(vmp_session* s, int32_t cap, float* x, float* y, float* z);
@@ -246,9 +254,9 @@ int vmp_ctx_reserve(vmp_ctx* ctx, int64_t points, int32_t sessions);
  *              1 = plain fp32 world frame, a precision-study arm whose
  *              fallback counter counts keys that differ from the fp64 key.
  *   "check_points" 1 = device point buffers are checked for non-finite
- *              coordinates when they are bound (the default), 0 = the
- *              caller guarantees finite points (host buffers are always
- *              checked).
+ *              coordinates (and raw scans for ascending times) when the
+ *              call that uses them starts (the default), 0 = the caller
+ *              guarantees them (host buffers are always checked).
  *   "obs_time_reps" 1..64: while per-launch timing is on, each observe
  *              launch runs this many times back to back between one event
  *              pair (the kernel is idempotent), then once more for the
@@ -278,6 +286,13 @@ int vmp_session_timing(vmp_session* s, double* observe_ms, double* map_ms, int32
 int vmp_session_set_timing(vmp_session* s, int on);
 int vmp_session_timing_totals(vmp_session* s, double* obs_ms, int64_t* obs_launches, double* map_ms,
                               int64_t* map_calls, int64_t* obs_points, int64_t* obs_hits);
+/* Per-kernel HIP-event totals over the same period (instrumentation, no
+ * reference counterpart): milliseconds and launches of, in this order,
+ * k_ins_keys, k_create_evict, k_replay_heavy, k_replay_light,
+ * k_merge_replay, k_candidates, k_ring_live, k_ieskf_step, k_undistort,
+ * k_imu_predict, k_spec_eval.  Returns the number of kernels (up to cap
+ * entries are written). */
+int32_t vmp_ctx_kernel_times(vmp_ctx* ctx, int32_t cap, double* total_ms, int64_t* launches);
 
 #ifdef __cplusplus
 }

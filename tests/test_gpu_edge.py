@@ -7,8 +7,9 @@
   z == 0 points       calcBodyCov's z = 0 -> 0.001 mutation (commons.cpp:20-23)
                       applies to observation and insertion covariance, not to
                       the insertion point
-  non-finite input    NaN / Inf coordinates are rejected (VMP_EINVAL) from host
-                      and device buffers; the session is unchanged
+  non-finite input    NaN / Inf coordinates are rejected (VMP_EINVAL): host
+                      buffers when bound, device buffers by the registration
+                      call, before it launches anything; the session is unchanged
   dense merges        a flat 40 x 40-voxel floor whose voxels all freeze in one
                       scan, then one merge() call per voxel in a strided order:
                       merge-replay windows with more than 32 acting entries
@@ -143,15 +144,21 @@ def test_non_finite_points_are_rejected(ctx):
             xb[1234] = bad
             with pytest.raises(VmpError, match="-1"):
                 g.set_points(xb, y, z)
+            # device buffers: checked when the registration call starts,
+            # which then fails before launching anything
             dev = torch.from_numpy(np.stack([xb, y, z])).to("cuda:0")
-            with pytest.raises(VmpError, match="-1"):
-                g.set_points_device(dev[0].data_ptr(), dev[1].data_ptr(), dev[2].data_ptr(), len(x))
             torch.cuda.synchronize()
+            g.set_points_device(dev[0].data_ptr(), dev[1].data_ptr(), dev[2].data_ptr(), len(x))
+            with pytest.raises(VmpError, match="-1"):
+                g.register()
+            with pytest.raises(VmpError, match="-6"):  # the binding was dropped
+                g.register()
         after, Pa = g.get_state()
         np.testing.assert_array_equal(before, after)
         np.testing.assert_array_equal(Pb, Pa)
         # finite points bound through the device path register as usual
         dev = torch.from_numpy(np.stack([x, y, z])).to("cuda:0")
+        torch.cuda.synchronize()
         orc.apply_motion(*seq.motion(1), QDIAG)
         sp, Pp = orc.get_state()
         orc.register(x, y, z)

@@ -19,6 +19,7 @@ from .registration import (
     register_batch_submit,
     register_batch_wait,
     set_points_device_batch,
+    set_raw_device_batch,
     state_vector,
 )
 
@@ -36,5 +37,6 @@ __all__ = [
     "register_batch_submit",
     "register_batch_wait",
     "set_points_device_batch",
+    "set_raw_device_batch",
     "state_vector",
 ]

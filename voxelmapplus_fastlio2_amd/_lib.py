@@ -159,9 +159,12 @@ def vmp():
            [P, fp, fp, fp, fp, C.c_int32, dp, dp, dp, C.c_int32, C.c_double, C.c_double])
     _proto(lib, "vmp_scan_set_raw_device", C.c_int,
            [P, P, P, P, P, C.c_int32, dp, dp, dp, C.c_int32, C.c_double, C.c_double])
+    _proto(lib, "vmp_scan_set_raw_device_batch", C.c_int,
+           [pp, C.c_int32, u64p, u64p, u64p, u64p, i32p, dp, dp, dp, i32p, dp, dp])
     _proto(lib, "vmp_scan_export", C.c_int32, [P, C.c_int32, fp, fp, fp])
     _proto(lib, "vmp_cloud_body_export", C.c_int32, [P, C.c_int32, fp, fp, fp])
     _proto(lib, "vmp_odom_export", C.c_int, [P, dp, dp, dp])
+    _proto(lib, "vmp_ctx_kernel_times", C.c_int32, [P, C.c_int32, dp, i64p])
     _vmp = lib
     return lib
 

@@ -175,6 +175,12 @@ struct vmp_ctx {
   int obs_time_reps = 1;
   hipEvent_t tev[2 * 64]{};
   int tev_used = 0;
+  // per-kernel event pairs of the timed kernels below (while timing is on)
+  hipEvent_t kev[2 * 48]{};
+  int kev_id[48]{};
+  int kev_used = 0;
+  double acc_k_ms[16]{};
+  int64_t acc_k_n[16]{};
   double acc_obs_ms = 0.0, acc_map_ms = 0.0;
   int64_t acc_obs_n = 0, acc_map_n = 0;
   int64_t acc_obs_points = 0, acc_obs_hits = 0;
@@ -208,10 +214,9 @@ struct vmp_ctx {
   std::vector<vmp_map_stats> res_mst;
   int res_rc = 0;
   bool res_valid = false;  // those results have not been collected by vmp_register_batch_wait
-  // validation of device point buffers (k_check_points) on a stream of its
-  // own, so it does not wait for a batch in flight on `stream`
+  // validation of device point buffers (k_check_points), run on `stream`
+  // when a step starts (no batch is in flight then)
   bool check_points = true;
-  hipStream_t vstream = nullptr;
   int vcap = 0;
   PtsRef* h_refs = nullptr;
   PtsRef* d_refs = nullptr;
@@ -226,6 +231,35 @@ struct vmp_ctx {
 static int ppt_bucket(int n) { return n < 8192 ? 0 : (n < 32768 ? 1 : (n < 131072 ? 2 : 3)); }
 static int obs_ppt_for(const vmp_ctx* c, int n) {
   return c->obs_ppt > 0 ? c->obs_ppt : c->ppt_tab[ppt_bucket(n)];
+}
+
+// kernels timed one by one while per-launch timing is on, in the order
+// vmp_ctx_kernel_times reports them
+enum {
+  kTkInsKeys = 0,
+  kTkCreateEvict,
+  kTkReplayHeavy,
+  kTkReplayLight,
+  kTkMergeReplay,
+  kTkCandidates,
+  kTkRingLive,
+  kTkIeskfStep,
+  kTkUndistort,
+  kTkImuPredict,
+  kTkSpecEval,
+  kTkCount
+};
+template <class F>
+static void timed_launch(vmp_ctx* c, int id, F&& launch) {
+  if (!c->timing || c->kev_used >= 48 || !c->kev[0]) {
+    launch();
+    return;
+  }
+  const int k = c->kev_used++;
+  hipEventRecord(c->kev[2 * k], c->stream);
+  launch();
+  hipEventRecord(c->kev[2 * k + 1], c->stream);
+  c->kev_id[k] = id;
 }
 
 struct vmp_session {
@@ -246,6 +280,7 @@ struct vmp_session {
   int32_t* live = nullptr;
   int32_t* nlive = nullptr;
   bool has_points = false;
+  bool needs_check = false;  // device buffers bound, checked when the next step starts
   // IMU side: device state, the raw package bound for the next step
   ImuDev* imu = nullptr;
   float* own_t = nullptr;
@@ -283,11 +318,7 @@ extern "C" int vmp_ctx_create(int device, vmp_ctx** out) {
     delete c;
     return VMP_EDEVICE;
   }
-  if (hipStreamCreateWithFlags(&c->vstream, hipStreamNonBlocking) != hipSuccess) {
-    hipStreamDestroy(c->stream);
-    delete c;
-    return VMP_EDEVICE;
-  }
+
   for (hipEvent_t& e : c->ev) hipEventCreate(&e);
   *out = c;
   return VMP_OK;
@@ -310,26 +341,27 @@ extern "C" int vmp_ctx_destroy(vmp_ctx* ctx) {
   ctx->rt.release();
   for (hipEvent_t e : ctx->ev)
     if (e) hipEventDestroy(e);
+  for (hipEvent_t e : ctx->kev)
+    if (e) hipEventDestroy(e);
   for (hipEvent_t e : ctx->tev)
     if (e) hipEventDestroy(e);
-  hipStreamSynchronize(ctx->vstream);
   if (ctx->h_refs) hipHostFree(ctx->h_refs);
   if (ctx->h_bad) hipHostFree(ctx->h_bad);
   if (ctx->d_refs) hipFree(ctx->d_refs);
   if (ctx->d_bad) hipFree(ctx->d_bad);
-  hipStreamDestroy(ctx->vstream);
   hipStreamDestroy(ctx->stream);
   delete ctx;
   return VMP_OK;
 }
 
-// Check device point sets for non-finite coordinates (c->mu held).  Returns
-// the index of the first bad set, -1 if all are finite, or -2 on a device
-// error.  The caller's buffers must hold the points when it is called.
+// Check device point sets for non-finite coordinates (and times out of
+// order) on the context's stream (c->mu held, no batch in flight).  Returns
+// the index of the first bad set, -1 if all are good, or -2 on a device
+// error.
 static int check_device_points(vmp_ctx* c, const PtsRef* refs, int nref) {
   if (nref <= 0 || !c->check_points) return -1;
   if (nref > c->vcap) {
-    hipStreamSynchronize(c->vstream);
+    hipStreamSynchronize(c->stream);
     if (c->h_refs) hipHostFree(c->h_refs);
     if (c->h_bad) hipHostFree(c->h_bad);
     if (c->d_refs) hipFree(c->d_refs);
@@ -352,7 +384,7 @@ static int check_device_points(vmp_ctx* c, const PtsRef* refs, int nref) {
     c->h_refs[i] = refs[i];
     most = std::max<int64_t>(most, refs[i].n);
   }
-  hipStream_t st = c->vstream;
+  hipStream_t st = c->stream;
   if (hipMemcpyAsync(c->d_refs, c->h_refs, sizeof(PtsRef) * nref, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipMemsetAsync(c->d_bad, 0, sizeof(int32_t) * nref, st) != hipSuccess)
     return -2;
@@ -737,6 +769,7 @@ static int bind_points_device(vmp_session* s, const float* dx, const float* dy, 
   s->cz = dz;
   s->n = n;
   s->has_points = true;
+  s->raw_pending = false;
   return VMP_OK;
 }
 
@@ -746,46 +779,25 @@ extern "C" int vmp_scan_set_points_device(vmp_session* s, const float* dx, const
     if (s) s->err = "scan size exceeds max_scan_points or null input";
     return VMP_EINVAL;
   }
-  vmp_ctx* c = s->ctx;
-  std::lock_guard<std::mutex> lk(c->mu);
-  VMP_HIP(hipSetDevice(c->device));
-  const PtsRef r{dx, dy, dz, nullptr, n, 0};
-  const int bad = check_device_points(c, &r, 1);
-  if (bad == -2) {
-    s->err = "point validation failed on the device";
-    return VMP_EDEVICE;
+  const int rc = bind_points_device(s, dx, dy, dz, n);
+  if (rc == VMP_OK) {
+    s->rt = nullptr;
+    s->needs_check = s->ctx->check_points;
   }
-  if (bad >= 0) {
-    s->err = "scan holds a non-finite coordinate";
-    return VMP_EINVAL;
-  }
-  return bind_points_device(s, dx, dy, dz, n);
+  return rc;
 }
 
 extern "C" int vmp_scan_set_points_device_batch(vmp_session** ss, int32_t n, const uint64_t* dx, const uint64_t* dy,
                                                 const uint64_t* dz, const int32_t* counts) {
   if (!ss || n <= 0 || !dx || !dy || !dz || !counts || !ss[0]) return VMP_EINVAL;
-  vmp_ctx* c = ss[0]->ctx;
-  std::vector<PtsRef> refs(static_cast<size_t>(n));
-  for (int i = 0; i < n; ++i) {
-    if (!ss[i] || ss[i]->ctx->device != c->device || counts[i] < 0 || counts[i] > ss[i]->cfg.max_scan_points)
+  for (int i = 0; i < n; ++i)
+    if (!ss[i] || ss[i]->ctx->device != ss[0]->ctx->device || counts[i] < 0 ||
+        counts[i] > ss[i]->cfg.max_scan_points)
       return VMP_EINVAL;
-    refs[size_t(i)] = PtsRef{reinterpret_cast<const float*>(dx[i]), reinterpret_cast<const float*>(dy[i]),
-                             reinterpret_cast<const float*>(dz[i]), nullptr, counts[i], 0};
-  }
-  {
-    std::lock_guard<std::mutex> lk(c->mu);
-    if (hipSetDevice(c->device) != hipSuccess) return VMP_EDEVICE;
-    const int bad = check_device_points(c, refs.data(), n);
-    if (bad == -2) return VMP_EDEVICE;
-    if (bad >= 0) {
-      ss[bad]->err = "scan holds a non-finite coordinate";
-      return VMP_EINVAL;
-    }
-  }
-  // all or nothing: nothing is bound when one set is rejected
   for (int i = 0; i < n; ++i) {
-    const int rc = bind_points_device(ss[i], refs[size_t(i)].x, refs[size_t(i)].y, refs[size_t(i)].z, counts[i]);
+    const int rc = vmp_scan_set_points_device(ss[i], reinterpret_cast<const float*>(dx[i]),
+                                              reinterpret_cast<const float*>(dy[i]),
+                                              reinterpret_cast<const float*>(dz[i]), counts[i]);
     if (rc != VMP_OK) return rc;
   }
   return VMP_OK;
@@ -861,10 +873,14 @@ static int stage_imu(vmp_session* s, const double* it, const double* iacc, const
   p.cloud_end = cloud_end;
   p.m = m;
   p.pad = 0;
-  std::memcpy(p.t, it, sizeof(double) * m);
-  std::memcpy(p.acc, iacc, sizeof(double) * 3 * m);
-  std::memcpy(p.gyro, igyro, sizeof(double) * 3 * m);
-  VMP_HIP(hipMemcpyAsync(&s->imu->pkg, s->stage, sizeof(ImuPkg), hipMemcpyHostToDevice, s->ctx->stream));
+  for (int32_t i = 0; i < m; ++i) {
+    p.s[i][0] = it[i];
+    for (int k = 0; k < 3; ++k) {
+      p.s[i][1 + k] = iacc[3 * i + k];
+      p.s[i][4 + k] = igyro[3 * i + k];
+    }
+  }
+  VMP_HIP(hipMemcpyAsync(&s->imu->pkg, s->stage, imu_pkg_bytes(m), hipMemcpyHostToDevice, s->ctx->stream));
   VMP_HIP(hipEventRecord(s->stage_done, s->ctx->stream));
   s->stage_pending = true;
   return VMP_OK;
@@ -926,6 +942,7 @@ extern "C" int vmp_scan_set_raw(vmp_session* s, const float* x, const float* y, 
   s->rt = s->own_t;
   s->raw_n = n;
   s->raw_pending = true;
+  s->needs_check = false;
   s->cx = s->own_x;
   s->cy = s->own_y;
   s->cz = s->own_z;
@@ -944,18 +961,9 @@ extern "C" int vmp_scan_set_raw_device(vmp_session* s, const float* dx, const fl
   vmp_ctx* c = s->ctx;
   std::lock_guard<std::mutex> lk(c->mu);
   VMP_HIP(hipSetDevice(c->device));
-  const PtsRef r{dx, dy, dz, dt_ms, n, 0};
-  const int bad = check_device_points(c, &r, 1);
-  if (bad == -2) {
-    s->err = "point validation failed on the device";
-    return VMP_EDEVICE;
-  }
-  if (bad >= 0) {
-    s->err = "raw scan: a non-finite coordinate or time, or times out of order";
-    return VMP_EINVAL;
-  }
   rc = stage_imu(s, it, iacc, igyro, m, cloud_start, cloud_end);
   if (rc != VMP_OK) return rc;
+  s->needs_check = c->check_points;
   s->rx = dx;
   s->ry = dy;
   s->rz = dz;
@@ -967,6 +975,52 @@ extern "C" int vmp_scan_set_raw_device(vmp_session* s, const float* dx, const fl
   s->cz = s->own_z;
   s->n = n;
   s->has_points = true;
+  return VMP_OK;
+}
+
+extern "C" int vmp_scan_set_raw_device_batch(vmp_session** ss, int32_t n, const uint64_t* dx, const uint64_t* dy,
+                                             const uint64_t* dz, const uint64_t* dt, const int32_t* counts,
+                                             const double* imu_t, const double* imu_acc, const double* imu_gyro,
+                                             const int32_t* imu_counts, const double* cloud_start,
+                                             const double* cloud_end) {
+  if (!ss || n <= 0 || !dx || !dy || !dz || !dt || !counts || !imu_t || !imu_acc || !imu_gyro || !imu_counts ||
+      !cloud_start || !cloud_end || !ss[0])
+    return VMP_EINVAL;
+  vmp_ctx* c = ss[0]->ctx;
+  std::vector<PtsRef> refs(static_cast<size_t>(n));
+  std::vector<int64_t> ioff(static_cast<size_t>(n) + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    if (!ss[i] || ss[i]->ctx != c) return VMP_EINVAL;
+    const int64_t o = ioff[size_t(i)];
+    const int rc = check_raw_args(ss[i], counts[i], imu_t + o, imu_acc + 3 * o, imu_gyro + 3 * o, imu_counts[i],
+                                  cloud_start[i], cloud_end[i]);
+    if (rc != VMP_OK) return rc;
+    ioff[size_t(i) + 1] = o + imu_counts[i];
+    refs[size_t(i)] = PtsRef{reinterpret_cast<const float*>(dx[i]), reinterpret_cast<const float*>(dy[i]),
+                             reinterpret_cast<const float*>(dz[i]), reinterpret_cast<const float*>(dt[i]), counts[i],
+                             0};
+  }
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return VMP_EDEVICE;
+  for (int i = 0; i < n; ++i) {
+    vmp_session* s = ss[i];
+    const int64_t o = ioff[size_t(i)];
+    const int rc = stage_imu(s, imu_t + o, imu_acc + 3 * o, imu_gyro + 3 * o, imu_counts[i], cloud_start[i],
+                             cloud_end[i]);
+    if (rc != VMP_OK) return rc;
+    s->rx = refs[size_t(i)].x;
+    s->ry = refs[size_t(i)].y;
+    s->rz = refs[size_t(i)].z;
+    s->rt = refs[size_t(i)].t;
+    s->raw_n = counts[i];
+    s->raw_pending = true;
+    s->needs_check = c->check_points;
+    s->cx = s->own_x;
+    s->cy = s->own_y;
+    s->cz = s->own_z;
+    s->n = counts[i];
+    s->has_points = true;
+  }
   return VMP_OK;
 }
 
@@ -1244,10 +1298,14 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
       hipLaunchKernelGGL(k_ieskf_begin, dim3(n), dim3(64), 0, st, w.d_flist, w.d_obsctr);
       for (int it = 0; it < max_iter; ++it) {
         observe(grid, false, false);
-        if (ext)
-          hipLaunchKernelGGL(k_ieskf_step<12>, dim3(n), dim3(64), 0, st, w.d_flist, w.d_jobs, w.d_obsctr, w.d_obsstat);
-        else
-          hipLaunchKernelGGL(k_ieskf_step<6>, dim3(n), dim3(64), 0, st, w.d_flist, w.d_jobs, w.d_obsctr, w.d_obsstat);
+        timed_launch(c, kTkIeskfStep, [&] {
+          if (ext)
+            hipLaunchKernelGGL(k_ieskf_step<12>, dim3(n), dim3(64), 0, st, w.d_flist, w.d_jobs, w.d_obsctr,
+                               w.d_obsstat);
+          else
+            hipLaunchKernelGGL(k_ieskf_step<6>, dim3(n), dim3(64), 0, st, w.d_flist, w.d_jobs, w.d_obsctr,
+                               w.d_obsstat);
+        });
       }
       hipLaunchKernelGGL(k_ieskf_end, dim3(n), dim3(64), 0, st, w.d_flist);
     };
@@ -1286,7 +1344,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
     hipLaunchKernelGGL(k_begin_scan, dim3(n), dim3(1), 0, st, M, n);
     if (ntot > 0) {
       size_t tb = w.temp_bytes;
-      hipLaunchKernelGGL(k_ins_keys, dim3(gn), dim3(T), 0, st, M, S, n, b);
+      timed_launch(c, kTkInsKeys, [&] { hipLaunchKernelGGL(k_ins_keys, dim3(gn), dim3(T), 0, st, M, S, n, b); });
       // one stable radix sort over (session, hash slot): every key's points
       // become a contiguous segment in point order, sessions stay in order
       if (rocprim::radix_sort_pairs(w.temp, tb, b.skey_in, b.skey_out, b.sval_in, b.sval_out, size_t(ntot), 0,
@@ -1308,9 +1366,10 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
       hipLaunchKernelGGL(k_cre_split, dim3(gn), dim3(T), 0, st, S, n, b);
       hipLaunchKernelGGL(k_create_fast, dim3(gn), dim3(T), 0, st, M, S, n, b);
       hipLaunchKernelGGL(k_create_fast_ctr, dim3(n), dim3(1), 0, st, M, n);
-      hipLaunchKernelGGL(k_ring_live, dim3(kLvBlocks, n), dim3(64 * kLvWaves), 0, st, M, S, n, b);
+      timed_launch(c, kTkRingLive,
+                   [&] { hipLaunchKernelGGL(k_ring_live, dim3(kLvBlocks, n), dim3(64 * kLvWaves), 0, st, M, S, n, b); });
       hipLaunchKernelGGL(k_ring_pack, dim3(n), dim3(kLpThreads), 0, st, M, S, n, b);
-      hipLaunchKernelGGL(k_create_evict, dim3(n), dim3(128), 0, st, M, S, n, b);
+      timed_launch(c, kTkCreateEvict, [&] { hipLaunchKernelGGL(k_create_evict, dim3(n), dim3(128), 0, st, M, S, n, b); });
       hipLaunchKernelGGL(k_create_init, dim3(gn), dim3(T), 0, st, M, S, n, b);
       // replay order by segment length, capped at 10 bits (two radix passes;
       // segments of 1022 points and more share the first key)
@@ -1323,11 +1382,12 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         return VMP_EDEVICE;
       // the moment path runs one wave per SIMD: a grid of four waves per CU
       // strides over the refitting voxels; the light path covers the rest
-      hipLaunchKernelGGL(k_replay_heavy, dim3(std::min(gn, 256)), dim3(T), 0, st, M, S, b);
-      hipLaunchKernelGGL(k_replay_light, dim3(gn), dim3(T), 0, st, M, S, b);
+      timed_launch(c, kTkReplayHeavy,
+                   [&] { hipLaunchKernelGGL(k_replay_heavy, dim3(std::min(gn, 256)), dim3(T), 0, st, M, S, b); });
+      timed_launch(c, kTkReplayLight, [&] { hipLaunchKernelGGL(k_replay_light, dim3(gn), dim3(T), 0, st, M, S, b); });
       if (!(mode & kStepBuild)) {
         hipLaunchKernelGGL(k_fill_u64, dim3(gn), dim3(T), 0, st, b.cand_in, ~0ull, ntot);
-        hipLaunchKernelGGL(k_candidates, dim3(gn), dim3(T), 0, st, M, S, b);
+        timed_launch(c, kTkCandidates, [&] { hipLaunchKernelGGL(k_candidates, dim3(gn), dim3(T), 0, st, M, S, b); });
         hipLaunchKernelGGL(k_dirty_untouched, dim3(gn), dim3(T), 0, st, M, b);
         tb = w.temp_bytes;
         // sorted on (session, point time) only: the call at point time t is
@@ -1338,7 +1398,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
           return VMP_EDEVICE;
         hipLaunchKernelGGL(k_cand_starts, dim3(1), dim3(kCsThreads), 0, st, M, n, b);
         hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.cand_spec, 0, ntot);
-        hipLaunchKernelGGL(k_spec_eval, dim3(gn), dim3(T), 0, st, M, b);
+        timed_launch(c, kTkSpecEval, [&] { hipLaunchKernelGGL(k_spec_eval, dim3(gn), dim3(T), 0, st, M, b); });
         tb = w.temp_bytes;
         if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.cand_spec, b.spl, b.cnt + 4,
                             size_t(ntot), st) != hipSuccess)
@@ -1346,7 +1406,8 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         hipLaunchKernelGGL(k_spl_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b);
         hipLaunchKernelGGL(k_spl_keys, dim3(gn), dim3(T), 0, st, b);
         hipLaunchKernelGGL(k_spl_prefetch, dim3(std::min(gn * kPreItems, 4096)), dim3(T), 0, st, M, S, b);
-        hipLaunchKernelGGL(k_merge_replay, dim3(n), dim3(kMrThreads), 0, st, M, S, n, b);
+        timed_launch(c, kTkMergeReplay,
+                     [&] { hipLaunchKernelGGL(k_merge_replay, dim3(n), dim3(kMrThreads), 0, st, M, S, n, b); });
       }
       hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.flags, 0, ntot);
       hipLaunchKernelGGL(k_finalize_segs, dim3(gn), dim3(T), 0, st, M, S, b);
@@ -1431,6 +1492,17 @@ static int maintain(vmp_ctx* ex, vmp_session* s, int32_t* did) {
   return VMP_OK;
 }
 
+static void collect_kernel_timing(vmp_ctx* c) {
+  for (int k = 0; k < c->kev_used; ++k) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->kev[2 * k], c->kev[2 * k + 1]) == hipSuccess) {
+      c->acc_k_ms[c->kev_id[k]] += ms;
+      c->acc_k_n[c->kev_id[k]] += 1;
+    }
+  }
+  c->kev_used = 0;
+}
+
 static void collect_timing(vmp_ctx* c, int active_max) {
   for (int k = 0; k + 1 < c->tev_used && k / 2 < active_max; k += 2) {
     float ms = 0.f;
@@ -1446,6 +1518,38 @@ static void collect_timing(vmp_ctx* c, int active_max) {
 // sessions may belong to other contexts of the same device; the caller makes
 // sure no session is in two batches at once) and distribute the results
 static int step_finish(vmp_ctx* c, vmp_obs_stats* ost, vmp_map_stats* mst);
+
+// the deferred check of device buffers bound since the last step (c->mu
+// held, nothing in flight): a session with a non-finite coordinate or time,
+// or times out of order, fails the call before anything is launched, and its
+// binding is dropped
+static int check_bound_points(vmp_ctx* c, vmp_session** ss, int n) {
+  std::vector<PtsRef> refs;
+  std::vector<int> who;
+  for (int i = 0; i < n; ++i) {
+    vmp_session* q = ss[i];
+    if (!q->needs_check) continue;
+    if (q->raw_pending) refs.push_back(PtsRef{q->rx, q->ry, q->rz, q->rt, q->raw_n, 0});
+    else refs.push_back(PtsRef{q->cx, q->cy, q->cz, nullptr, q->n, 0});
+    who.push_back(i);
+  }
+  if (refs.empty()) return VMP_OK;
+  const int bad = check_device_points(c, refs.data(), int(refs.size()));
+  if (bad == -2) {
+    ss[0]->err = "point validation failed on the device";
+    return VMP_EDEVICE;
+  }
+  if (bad >= 0) {
+    vmp_session* q = ss[who[size_t(bad)]];
+    q->err = "scan holds a non-finite coordinate or time, or point times out of order";
+    q->has_points = false;
+    q->raw_pending = false;
+    q->needs_check = false;
+    return VMP_EINVAL;
+  }
+  for (int i : who) ss[i]->needs_check = false;
+  return VMP_OK;
+}
 
 // undistortCloud of the sessions that carry a raw package (IMU propagation of
 // x and P, then the points compensated into the sweep's end frame) and, for
@@ -1466,9 +1570,10 @@ static int prepare_scans(vmp_ctx* c, vmp_session** ss, int n, bool is_build) {
   if (npre == 0) return VMP_OK;
   if (hipMemcpyAsync(w.d_pre, w.h_pre, sizeof(PreJob) * npre, hipMemcpyHostToDevice, st) != hipSuccess)
     return VMP_EDEVICE;
-  hipLaunchKernelGGL(k_imu_predict, dim3(npre), dim3(64), 0, st, w.d_pre, npre);
+  timed_launch(c, kTkImuPredict, [&] { hipLaunchKernelGGL(k_imu_predict, dim3(npre), dim3(64), 0, st, w.d_pre, npre); });
   const int gx = int(std::max<int64_t>(1, std::min<int64_t>((most + 255) / 256, 1024)));
-  hipLaunchKernelGGL(k_undistort, dim3(gx, std::min(npre, 65535)), dim3(256), 0, st, w.d_pre, npre);
+  timed_launch(c, kTkUndistort,
+               [&] { hipLaunchKernelGGL(k_undistort, dim3(gx, std::min(npre, 65535)), dim3(256), 0, st, w.d_pre, npre); });
   // the filter (never at MAP_INIT, lio_builder.cpp:185-211)
   std::vector<vmp_session*> fs;
   int64_t off = 0;
@@ -1549,10 +1654,12 @@ static int step_begin(vmp_ctx* c, vmp_session** ss, int n, int mode) {
     }
   }
   VMP_HIP(hipSetDevice(c->device));
+  int rc = check_bound_points(c, ss, n);
+  if (rc != VMP_OK) return rc;
   int64_t ntot = 0;
   for (int i = 0; i < n; ++i) ntot += ss[i]->n;
   if (ntot > (int64_t(1) << 30)) return VMP_EINVAL;
-  int rc = ws_reserve(c, std::max<int64_t>(ntot, 1), n);
+  rc = ws_reserve(c, std::max<int64_t>(ntot, 1), n);
   if (rc != VMP_OK) {
     s->err = "batch workspace allocation failed";
     return rc;
@@ -1608,6 +1715,7 @@ static int step_finish(vmp_ctx* c, vmp_obs_stats* ost, vmp_map_stats* mst) {
   }
   if (c->timing) {
     collect_timing(c, (mode & kStepIeskf) ? max_iters : 0);
+    collect_kernel_timing(c);
     c->acc_obs_points += pts_iters;
     c->acc_obs_hits += hits;
     if (mode & kStepMap) {
@@ -1711,9 +1819,11 @@ extern "C" int vmp_observe(vmp_session* s, const vmp_state* x, double* H, double
   std::lock_guard<std::mutex> lk(c->mu);
   VMP_HIP(hipSetDevice(c->device));
   settle(c);
+  vmp_session* one[1] = {s};
+  const int crc = check_bound_points(c, one, 1);
+  if (crc != VMP_OK) return crc;
   if (ws_reserve(c, std::max(s->n, 1), 1) != VMP_OK) return VMP_ENOMEM;
   VMP_HIP(hipMemcpyAsync(&s->f->x, x, sizeof(vmp_state), hipMemcpyHostToDevice, c->stream));
-  vmp_session* one[1] = {s};
   if (prepare_scans(c, one, 1, false) != VMP_OK) return VMP_EDEVICE;
   if (enqueue_step(c, one, 1, kStepObserveOnce) != VMP_OK) return VMP_EDEVICE;
   double out[156];
@@ -1754,6 +1864,10 @@ static int autotune_locked(vmp_ctx* c, vmp_session** ss, int n, int reps, int32_
       return VMP_ESTATE;
     }
   settle(c);
+  {
+    const int crc = check_bound_points(c, ss, n);
+    if (crc != VMP_OK) return crc;
+  }
   int64_t ntot = 0;
   for (int i = 0; i < n; ++i) ntot += ss[i]->n;
   if (ws_reserve(c, std::max<int64_t>(ntot, 1), n) != VMP_OK) return VMP_ENOMEM;
@@ -1976,6 +2090,13 @@ extern "C" int vmp_session_set_timing(vmp_session* s, int on) {
   VMP_HIP(hipSetDevice(c->device));
   if (on && !c->tev[0])
     for (hipEvent_t& e : c->tev) VMP_HIP(hipEventCreate(&e));
+  if (on && !c->kev[0])
+    for (hipEvent_t& e : c->kev) VMP_HIP(hipEventCreate(&e));
+  c->kev_used = 0;
+  for (int k = 0; k < 16; ++k) {
+    c->acc_k_ms[k] = 0.0;
+    c->acc_k_n[k] = 0;
+  }
   c->timing = on != 0;
   c->tev_used = 0;
   c->acc_obs_ms = c->acc_map_ms = 0.0;
@@ -1995,6 +2116,16 @@ extern "C" int vmp_session_timing_totals(vmp_session* s, double* obs_ms, int64_t
   if (obs_points) *obs_points = c->acc_obs_points;
   if (obs_hits) *obs_hits = c->acc_obs_hits;
   return VMP_OK;
+}
+
+extern "C" int32_t vmp_ctx_kernel_times(vmp_ctx* c, int32_t cap, double* total_ms, int64_t* launches) {
+  if (!c || cap < 0) return VMP_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  for (int k = 0; k < kTkCount && k < cap; ++k) {
+    if (total_ms) total_ms[k] = c->acc_k_ms[k];
+    if (launches) launches[k] = c->acc_k_n[k];
+  }
+  return kTkCount;
 }
 
 extern "C" int64_t vmp_map_size(vmp_session* s) {

@@ -27,13 +27,17 @@ namespace vmp {
 constexpr int kMaxImu = 512;  // IMU samples of one package
 constexpr int kPoseD = 22;    // offset, acc 3, gyro 3, vel 3, pos 3, rot 9 (row-major)
 
-// one SyncPackage's IMU side (lio_node.cpp:137-165): uploaded as a whole
+// one SyncPackage's IMU side (lio_node.cpp:137-165); the samples are
+// interleaved (t, acc xyz, gyro xyz) so a package uploads as one copy of
+// its used bytes
+constexpr int kImuS = 7;
 struct ImuPkg {
   double cloud_start, cloud_end;
   int32_t m;  // IMU samples
   int32_t pad;
-  double t[kMaxImu], acc[kMaxImu][3], gyro[kMaxImu][3];
+  double s[kMaxImu][kImuS];
 };
+__host__ __device__ inline size_t imu_pkg_bytes(int m) { return offsetof(ImuPkg, s) + sizeof(double) * kImuS * m; }
 
 struct ImuDev {
   // LIODataGroup (lio_builder.h:43-54)
@@ -320,7 +324,14 @@ __device__ inline void predict_cov(const PredBlocks& B, const double q[4], doubl
   __syncthreads();
 }
 
-// undistortCloud's IMU part for one session per workgroup (64 lanes)
+// undistortCloud's IMU part for one session per workgroup (64 lanes).  The
+// biases and gravity do not change while a package is propagated, so every
+// step's rotation increment exp((gyro - bg) dt) and right Jacobian depend on
+// the IMU data alone: lanes compute them for up to 64 steps at once (the
+// transcendental part), then the steps run in order: the state on lane 0
+// (products only), the rows of P on lanes 0-22.  exp(-w) is exp(w)^T bit
+// for bit (the quaternion form only flips signs), so F's rotation block is
+// the transpose of the step's increment.
 __global__ void __launch_bounds__(64) k_imu_predict(const PreJob* jobs, int nsess) {
   if (int(blockIdx.x) >= nsess) return;
   const PreJob J = jobs[blockIdx.x];
@@ -331,80 +342,156 @@ __global__ void __launch_bounds__(64) k_imu_predict(const PreJob* jobs, int nses
   __shared__ PredBlocks B;
   __shared__ vmp_state sx;
   __shared__ double q[4];
+  __shared__ double cE[64][9], cJ[64][9], cA[64][3], cG[64][3], cDt[64], cT[64];
+  __shared__ double Mx[6], NM[4];
   for (int i = lane; i < 529; i += 64) P[(i % 23) * 23 + i / 23] = f->P[i];  // column-major -> rows
   load_state(f->x, sx, lane);
   if (lane < 4) q[lane] = im->q[lane];
   __syncthreads();
   const int m = im->pkg.m;
   const double lce = im->last_cloud_end;
-  double acc_val[3] = {0.0, 0.0, 0.0}, gyro_val[3] = {0.0, 0.0, 0.0};
-  double inp_acc[3] = {0.0, 0.0, 0.0}, inp_gyro[3] = {0.0, 0.0, 0.0};
-  if (lane == 0) {  // pose 0: the start of the sweep
-    double* p = im->pose[0];
+  // active IMU pairs (head k-1 or last_imu, tail k) are those whose tail is
+  // not before the last scan end: a prefix of the ascending samples is skipped
+  int k0 = 0;
+  while (k0 < m && im->pkg.s[k0][0] < lce) ++k0;
+  const int nsteps = (m - k0) + 1;  // the pairs, then the step to the sweep's end
+  if (lane == 0) {
+    double Bx[6];
+    getBx_d(sx.g, Bx);
+    const D33 hg = hat_d(sx.g);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 2; ++c) {
+        double v = ((-1.0) * hg.m[r * 3]) * Bx[c];
+        v += ((-1.0) * hg.m[r * 3 + 1]) * Bx[2 + c];
+        v += ((-1.0) * hg.m[r * 3 + 2]) * Bx[4 + c];
+        Mx[r * 2 + c] = v;
+      }
+    double N[6];
+    getNx_d(sx.g, N);
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 2; ++c) {
+        double v = N[r * 3] * Mx[c];
+        v += N[r * 3 + 1] * Mx[2 + c];
+        v += N[r * 3 + 2] * Mx[4 + c];
+        NM[r * 2 + c] = v;
+      }
+    double* p = im->pose[0];  // pose 0: the start of the sweep
     const D33 rot = d33_from_cm(sx.rot);
     p[0] = 0.0;
-    for (int k = 0; k < 3; ++k) {
-      p[1 + k] = im->last_acc[k];
-      p[4 + k] = im->last_gyro[k];
-      p[7 + k] = sx.vel[k];
-      p[10 + k] = sx.pos[k];
+    for (int c = 0; c < 3; ++c) {
+      p[1 + c] = im->last_acc[c];
+      p[4 + c] = im->last_gyro[c];
+      p[7 + c] = sx.vel[c];
+      p[10 + c] = sx.pos[c];
     }
-    for (int k = 0; k < 9; ++k) p[13 + k] = rot.m[k];
+    for (int c = 0; c < 9; ++c) p[13 + c] = rot.m[c];
   }
-  int np = 1;
-  // pairs (head, tail) of [last_imu, package samples...]
-  for (int k = 0; k < m; ++k) {
-    const double ht = k == 0 ? im->last_imu_t : im->pkg.t[k - 1];
-    const double tt = im->pkg.t[k];
-    if (tt < lce) continue;  // uniform: every lane reads the same
-    if (lane == 0) {
-      const double* ha = k == 0 ? im->last_imu_acc : im->pkg.acc[k - 1];
-      const double* hg = k == 0 ? im->last_imu_gyro : im->pkg.gyro[k - 1];
+  // input of step st (the last pair's input for the final step; zero when
+  // there is no pair: the reference's input is then uninitialised)
+  auto step_input = [&](int st, double acc[3], double gyro[3], double& dt, double& tt) {
+    const int k = k0 + (st < nsteps - 1 ? st : nsteps - 2);
+    if (k < k0) {
+      for (int c = 0; c < 3; ++c) acc[c] = gyro[c] = 0.0;
+    } else {
+      const double* hs = k == 0 ? nullptr : im->pkg.s[k - 1];
       for (int c = 0; c < 3; ++c) {
-        gyro_val[c] = 0.5 * (hg[c] + im->pkg.gyro[k][c]);
-        acc_val[c] = 0.5 * (ha[c] + im->pkg.acc[k][c]);
-        acc_val[c] = acc_val[c] * 9.81 / im->gravity_norm;
-        inp_acc[c] = acc_val[c];
-        inp_gyro[c] = gyro_val[c];
+        const double ha = k == 0 ? im->last_imu_acc[c] : hs[1 + c];
+        const double hgy = k == 0 ? im->last_imu_gyro[c] : hs[4 + c];
+        gyro[c] = 0.5 * (hgy + im->pkg.s[k][4 + c]);
+        acc[c] = 0.5 * (ha + im->pkg.s[k][1 + c]);
+        acc[c] = acc[c] * 9.81 / im->gravity_norm;
       }
-      const double dt = ht < lce ? tt - lce : tt - ht;
-      predict_state(sx, inp_acc, inp_gyro, dt, B);
+    }
+    if (st < nsteps - 1) {
+      const double ht = k == 0 ? im->last_imu_t : im->pkg.s[k - 1][0];
+      tt = im->pkg.s[k][0];
+      dt = ht < lce ? tt - lce : tt - ht;
+    } else {
+      const double imu_end = m > 0 ? im->pkg.s[m - 1][0] : im->last_imu_t;
+      tt = 0.0;
+      dt = im->pkg.cloud_end - imu_end;
+    }
+  };
+  int np = 1;
+  for (int c0 = 0; c0 < nsteps; c0 += 64) {
+    const int cn = min(64, nsteps - c0);
+    if (lane < cn) {  // the steps' rotation increments and right Jacobians, in parallel
+      double a[3], g[3], dt, tt;
+      step_input(c0 + lane, a, g, dt, tt);
+      double wdt[3];
+      for (int c = 0; c < 3; ++c) wdt[c] = dt * (g[c] - sx.bg[c]);
+      const D33 E = so3_exp_d(wdt);
+      const D33 Jr = d33_t(left_jac_d(wdt));
+      for (int i = 0; i < 9; ++i) {
+        cE[lane][i] = E.m[i];
+        cJ[lane][i] = ((-1.0) * Jr.m[i]) * dt;
+      }
+      for (int c = 0; c < 3; ++c) {
+        cA[lane][c] = a[c];
+        cG[lane][c] = g[c];
+      }
+      cDt[lane] = dt;
+      cT[lane] = tt;
     }
     __syncthreads();
-    predict_cov(B, q, P, FP, lane);
-    if (lane == 0) {
-      const D33 rot = d33_from_cm(sx.rot);
-      double ab[3] = {acc_val[0] - sx.ba[0], acc_val[1] - sx.ba[1], acc_val[2] - sx.ba[2]};
-      const D3 ra = d33_mulv(rot, D3{{ab[0], ab[1], ab[2]}});
-      double* p = im->pose[np];
-      p[0] = tt - im->pkg.cloud_start;
-      for (int c = 0; c < 3; ++c) {
-        im->last_gyro[c] = gyro_val[c] - sx.bg[c];
-        im->last_acc[c] = ra.v[c] + sx.g[c];
-        p[1 + c] = im->last_acc[c];
-        p[4 + c] = im->last_gyro[c];
-        p[7 + c] = sx.vel[c];
-        p[10 + c] = sx.pos[c];
+    for (int j = 0; j < cn; ++j) {
+      const bool last = c0 + j == nsteps - 1;
+      if (lane == 0) {  // IESKF::predict on the state: blocks from the state before the step, then x += delta
+        const double dt = cDt[j];
+        const D33 rot = d33_from_cm(sx.rot);
+        const double ab[3] = {cA[j][0] - sx.ba[0], cA[j][1] - sx.ba[1], cA[j][2] - sx.ba[2]};
+        const D3 ra = d33_mulv(rot, D3{{ab[0], ab[1], ab[2]}});
+        D33 nrot;
+        for (int i = 0; i < 9; ++i) nrot.m[i] = (-1.0) * rot.m[i];
+        const D33 Ah = d33_mul(nrot, hat_d(ab));
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) B.Rexp[r * 3 + c] = cE[j][c * 3 + r];
+        for (int i = 0; i < 9; ++i) {
+          B.Jdt[i] = cJ[j][i];
+          B.Adt[i] = Ah.m[i] * dt;
+          B.Rdt[i] = nrot.m[i] * dt;
+        }
+        for (int i = 0; i < 6; ++i) B.Mdt[i] = Mx[i] * dt;
+        for (int i = 0; i < 4; ++i) B.NM[i] = NM[i];
+        B.dt = dt;
+        for (int c = 0; c < 3; ++c) {
+          const double dvel = dt * (ra.v[c] + sx.g[c]);
+          const double dpos = dt * sx.vel[c];
+          sx.pos[c] = sx.pos[c] + dpos;
+          sx.vel[c] = sx.vel[c] + dvel;
+        }
+        D33 E;
+        for (int i = 0; i < 9; ++i) E.m[i] = cE[j][i];
+        const D33 rn = d33_mul(rot, E);
+        d33_to_cm(rn, sx.rot);
+        if (!last) {  // the IMU pose at the pair's tail
+          const D3 rb = d33_mulv(rn, D3{{ab[0], ab[1], ab[2]}});
+          double* p = im->pose[np];
+          p[0] = cT[j] - im->pkg.cloud_start;
+          for (int c = 0; c < 3; ++c) {
+            im->last_gyro[c] = cG[j][c] - sx.bg[c];
+            im->last_acc[c] = rb.v[c] + sx.g[c];
+            p[1 + c] = im->last_acc[c];
+            p[4 + c] = im->last_gyro[c];
+            p[7 + c] = sx.vel[c];
+            p[10 + c] = sx.pos[c];
+          }
+          for (int c = 0; c < 9; ++c) p[13 + c] = rn.m[c];
+        }
       }
-      for (int c = 0; c < 9; ++c) p[13 + c] = rot.m[c];
+      __syncthreads();
+      predict_cov(B, q, P, FP, lane);
+      if (!last) ++np;
     }
-    ++np;
   }
-  // the last step to the end of the sweep, with the last input
-  if (lane == 0) {
-    const double imu_end = m > 0 ? im->pkg.t[m - 1] : im->last_imu_t;
-    predict_state(sx, inp_acc, inp_gyro, im->pkg.cloud_end - imu_end, B);
-  }
-  __syncthreads();
-  predict_cov(B, q, P, FP, lane);
   for (int i = lane; i < 529; i += 64) f->P[i] = P[(i % 23) * 23 + i / 23];
   if (lane == 0) {
     f->x = sx;
     if (m > 0) {
-      im->last_imu_t = im->pkg.t[m - 1];
+      im->last_imu_t = im->pkg.s[m - 1][0];
       for (int c = 0; c < 3; ++c) {
-        im->last_imu_acc[c] = im->pkg.acc[m - 1][c];
-        im->last_imu_gyro[c] = im->pkg.gyro[m - 1][c];
+        im->last_imu_acc[c] = im->pkg.s[m - 1][1 + c];
+        im->last_imu_gyro[c] = im->pkg.s[m - 1][4 + c];
       }
     }
     im->last_cloud_end = im->pkg.cloud_end;

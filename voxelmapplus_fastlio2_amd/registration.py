@@ -172,6 +172,18 @@ class Context:
         if rc != 0:
             raise VmpError(f"vmp_ctx_reserve({points}, {sessions}) failed ({rc})")
 
+    TIMED_KERNELS = ("k_ins_keys", "k_create_evict", "k_replay_heavy", "k_replay_light", "k_merge_replay",
+                     "k_candidates", "k_ring_live", "k_ieskf_step", "k_undistort", "k_imu_predict", "k_spec_eval")
+
+    def kernel_times(self):
+        """{kernel: (total ms, launches)} of the per-kernel event timing."""
+        ms = (C.c_double * 16)()
+        n = (C.c_int64 * 16)()
+        k = self.lib.vmp_ctx_kernel_times(self.h, 16, ms, n)
+        if k < 0:
+            raise VmpError(f"vmp_ctx_kernel_times failed ({k})")
+        return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(self.TIMED_KERNELS[:k])}
+
     def autotune_observe(self, registrar, reps=10):
         """Time the observe pass on the registrar's current scan for 1, 2, 4
         and 8 points per thread; the fastest becomes this context's choice
@@ -451,6 +463,32 @@ def set_points_device_batch(registrars, xs, ys, zs, counts):
     )
     if rc != 0:
         raise VmpError(f"vmp_scan_set_points_device_batch failed ({rc})")
+    for r, c in zip(registrars, cn):
+        r.n = int(c)
+
+
+def set_raw_device_batch(registrars, xs, ys, zs, ts, counts, imus, cloud_starts, cloud_ends):
+    """Bind raw packages of many sessions (device addresses of points and
+    times; imus: one (t, acc, gyro) per session) in one call."""
+    lib = registrars[0].lib
+    n = len(registrars)
+    arr = (C.c_void_p * n)(*[r.h.value for r in registrars])
+    u64 = C.POINTER(C.c_uint64)
+    a = [np.ascontiguousarray(v, dtype=np.uint64) for v in (xs, ys, zs, ts)]
+    cn = np.ascontiguousarray(counts, dtype=np.int32)
+    it = np.ascontiguousarray(np.concatenate([np.asarray(i[0], np.float64) for i in imus]))
+    ia = np.ascontiguousarray(np.concatenate([np.asarray(i[1], np.float64).reshape(-1, 3) for i in imus]))
+    ig = np.ascontiguousarray(np.concatenate([np.asarray(i[2], np.float64).reshape(-1, 3) for i in imus]))
+    ic = np.ascontiguousarray([len(i[0]) for i in imus], dtype=np.int32)
+    cs = np.ascontiguousarray(cloud_starts, dtype=np.float64)
+    ce = np.ascontiguousarray(cloud_ends, dtype=np.float64)
+    i32 = C.POINTER(C.c_int32)
+    rc = lib.vmp_scan_set_raw_device_batch(
+        arr, n, *(v.ctypes.data_as(u64) for v in a), cn.ctypes.data_as(i32), _dptr(it), _dptr(ia), _dptr(ig),
+        ic.ctypes.data_as(i32), _dptr(cs), _dptr(ce),
+    )
+    if rc != 0:
+        raise VmpError(f"vmp_scan_set_raw_device_batch failed ({rc})")
     for r, c in zip(registrars, cn):
         r.n = int(c)
 
