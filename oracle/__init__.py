@@ -61,6 +61,8 @@ def lib():
         L.orc_map_size.restype = C.c_int64
         L.orc_map_size.argtypes = [C.c_void_p]
         L.orc_map_export.restype = C.c_int64
+        L.orc_map_fit.restype = C.c_int64
+        L.orc_map_fit.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
         L.orc_map_export.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         for name in (
             "orc_set_state",
@@ -198,7 +200,9 @@ class Oracle:
         group = np.zeros(max(n, 1), np.uint64)
         dbl = np.zeros((max(n, 1), 8))
         cnt = self.L.orc_map_export(self.h, n, P(keys), P(ints), P(group), P(dbl))
-        return {"keys": keys[:cnt], "ints": ints[:cnt], "group": group[:cnt], "dbl": dbl[:cnt]}
+        fit = np.zeros((max(n, 1), 6))
+        self.L.orc_map_fit(self.h, n, P(fit))
+        return {"keys": keys[:cnt], "ints": ints[:cnt], "group": group[:cnt], "dbl": dbl[:cnt], "fit": fit[:cnt]}
 
     def export_slots(self, n):
         valid = np.zeros(n, np.uint8)

@@ -147,6 +147,23 @@ int64_t orc_map_size(void* h) { return int64_t(static_cast<Registrar*>(h)->map->
 // Voxels in LRU order (front = most recently touched).
 // ints[5*i]: n, flags (1 init, 2 plane, 4 update_enable, 8 merged), newly_add_point, temp size, 0
 // dbl[8*i]: mean xyz, norm xyz, trace(cov[0:3,0:3]), trace(cov[3:6,3:6])
+// Per voxel in LRU order: eigenvalues (ascending) and major axis of the last
+// refit (6 doubles), for conditioning-aware normal comparisons.
+int64_t orc_map_fit(void* h, int64_t cap, double* out) {
+  Registrar* r = static_cast<Registrar*>(h);
+  int64_t i = 0;
+  for (const VoxelKey& k : r->map->cache) {
+    if (i >= cap) break;
+    const VoxelGrid& g = *r->map->featmap.at(k);
+    for (int a = 0; a < 3; ++a) {
+      out[6 * i + a] = g.fit_evals[a];
+      out[6 * i + 3 + a] = g.fit_major(a, 0);
+    }
+    ++i;
+  }
+  return i;
+}
+
 int64_t orc_map_export(void* h, int64_t cap, int64_t* keys, int32_t* ints, uint64_t* group, double* dbl) {
   Registrar* r = static_cast<Registrar*>(h);
   int64_t i = 0;

@@ -295,6 +295,8 @@ void VoxelGrid::updatePlane() {  // voxel_map.cpp:97-136
   double evals[3];
   M3 evecs;
   eigh3(cov, evals, evecs);
+  for (int i = 0; i < 3; ++i) fit_evals[i] = evals[i];
+  fit_major = v3(evecs(0, 2), evecs(1, 2), evecs(2, 2));
   if (evals[0] > plane_thresh) {
     is_plane = false;
     return;

@@ -170,6 +170,11 @@ class VoxelGrid {
   std::shared_ptr<Plane> plane;
   V3 center = V3::zero();
   std::list<VoxelKey>::iterator cache_it;
+  // eigenvalues and major axis of the last refit's covariance: test
+  // diagnostics only (the conditioning of the fitted normal), no part of
+  // the restated algorithm
+  double fit_evals[3] = {0.0, 0.0, 0.0};
+  V3 fit_major = V3::zero();
 };
 
 class VoxelMap {

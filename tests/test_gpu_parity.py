@@ -57,6 +57,7 @@ def assert_map_close(r):
     assert r["group_diff"] <= margin * 8
     assert r["mean_bad"] <= margin * 8 and r["norm_bad"] <= margin * 8
     assert r["trace_bad"] <= margin * 8
+    assert r.get("ill_conditioned_bad", 0) == 0
 
 
 def test_build_parity(ctx):
