@@ -130,6 +130,7 @@ class Oracle:
         self.c = cfg_from(lio_cfg)
         self.h = C.c_void_p(self.L.orc_create(C.byref(self.c)))
         self.n = 0
+        self.scan_resolution = float(getattr(lio_cfg, "scan_resolution", 0.0))
 
     def close(self):
         if self.h:
@@ -238,6 +239,7 @@ class Oracle:
                               C.c_void_p(r.ctypes.data), P(p))
 
     def set_scan_resolution(self, leaf):
+        self.scan_resolution = float(leaf)
         self.L.orc_set_scan_resolution(self.h, C.c_double(leaf))
 
     def process(self, x, y, z, t_ms, imu, cloud_start, cloud_end):

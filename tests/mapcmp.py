@@ -17,12 +17,15 @@ def state_err(sa, sb):
 
 
 def ulp_err(a, b):
-    """Largest difference in units of the fp32 spacing at the coordinate:
-    the compensation runs in double and rounds once, and a state that
-    differs by ~1e-9 between the two sides moves a rounding by one unit."""
+    """Largest coordinate difference of two (3, n) point arrays in units of
+    the fp32 spacing at each point's largest coordinate: the compensation
+    rotates and translates the whole point in double and rounds once, so its
+    error scales with the point's size, and a state that differs by ~1e-9
+    between the two sides moves a rounding by one unit."""
     if a.size == 0:
         return 0.0
-    u = np.spacing(np.maximum(np.abs(a), np.abs(b)).astype(np.float32)).astype(np.float64)
+    mag = np.maximum(np.abs(a).max(axis=0), np.abs(b).max(axis=0)).astype(np.float32)
+    u = np.spacing(mag).astype(np.float64)
     return float((np.abs(a.astype(np.float64) - b) / u).max())
 
 
@@ -139,3 +142,46 @@ def register_both(orc, g, x, y, z):
     orc.set_state(sg, Pg)
     orc.map_update()
     return it, so, Po, ost, mst, sg, Pg
+
+
+def lio_both(orc, lb, pkg):
+    """One SyncPackage (x, y, z, t_ms, imu, cloud_start, cloud_end) through
+    LIOBuilder::process on both sides, the oracle starting from the device's
+    state (teacher forcing): at LIO_MAPPING the oracle undistorts, updates
+    and then inserts with the device's posterior, as register_both does, so
+    every package checks one step of the device against the oracle from
+    the same state and map.  Returns a dict: status (device, oracle),
+    iterations (oracle, device), prior (the oracle's propagated state),
+    oracle posterior so, Po, device posterior sg, Pg, and the undistorted
+    points of both sides (None before LIO_MAPPING)."""
+    x, y, z, t, imu, cs, ce = pkg
+    out = {"points": None, "it": None}
+    if lb.status != 0:
+        s, P = lb.reg.get_state()
+        orc.set_state(s, P)
+    if lb.status in (0, 1):  # IMU_INIT, MAP_INIT: the whole package on both sides
+        st_o, _ = orc.process(*pkg)
+        st_g = lb.process(*pkg)
+    else:
+        ux, uy, uz, _ = orc.undistort(*pkg)
+        out["prior"] = orc.get_state()
+        if orc.scan_resolution > 0.0:
+            import oracle
+
+            ux, uy, uz = oracle.voxel_grid(ux, uy, uz, float(np.float32(orc.scan_resolution)))
+        orc.set_points(ux, uy, uz)
+        it_o = orc.update()
+        so, Po = orc.get_state()
+        st_g = lb.process(*pkg)
+        st_o = st_g if lb.status == 2 else -1
+        sg, Pg = lb.reg.get_state()
+        orc.set_state(sg, Pg)
+        orc.map_update()
+        out["it"] = (it_o, lb.last_obs.iterations)
+        out["points"] = (orc.scan_points(), lb.reg.export_scan())
+        out["so"], out["Po"] = so, Po
+    out["status"] = (st_g, st_o)
+    out["sg"], out["Pg"] = lb.reg.get_state()
+    if "so" not in out:
+        out["so"], out["Po"] = orc.get_state()
+    return out

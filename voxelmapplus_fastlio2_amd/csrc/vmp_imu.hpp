@@ -7,11 +7,10 @@
 //   k_imu_init      one thread per session: means of the cached samples,
 //                   gravity alignment, bg, P
 //   k_imu_predict   one 64-lane workgroup per session: the package's IMU
-//                   pairs in order; the state step and the small F/G blocks
-//                   on lane 0, the 23 rows of P = F P F^T + G Q G^T on lanes
-//                   0-22 (only the structurally non-zero products, added in
-//                   ascending order, which is what the dense sums give); the
-//                   IMU pose table and its running minimum of offsets
+//                   pairs in order (per-step increments across lanes, the
+//                   state chain on one lane, P = F P F^T + G Q G^T over the
+//                   structurally non-zero products across lanes); the IMU
+//                   pose table and its running minimum of offsets
 //   k_undistort     one lane per point: the pose interval it falls in (a
 //                   binary search over the running minimum, which is the
 //                   interval the reference's backward walk reaches it in),
@@ -180,176 +179,159 @@ __global__ void k_imu_init(ImuDev* imu, FilterDev* f, const double* t, const dou
   }
 }
 
-// the small blocks of F and G of one predict step, from the state before it
-struct PredBlocks {
-  double Rexp[9];  // exp(-(gyro - bg) dt)
-  double Jdt[9];   // -Jr((gyro - bg) dt) dt (F and G)
-  double Adt[9];   // (-rot hat(acc - ba)) dt
-  double Rdt[9];   // (-rot) dt (F and G)
-  double Mdt[6];   // getMx() dt, 3x2
-  double NM[4];    // getNx() getMx(), 2x2
-  double dt;
-};
-
-// IESKF::predict on the state in `x` (lane 0): the blocks, then x += delta
-__device__ inline void predict_state(vmp_state& x, const double acc[3], const double gyro[3], double dt,
-                                     PredBlocks& B) {
-  double wdt[3], dvel[3], dpos[3];
-  for (int k = 0; k < 3; ++k) wdt[k] = dt * (gyro[k] - x.bg[k]);
-  const D33 rot = d33_from_cm(x.rot);
-  double ab[3] = {acc[0] - x.ba[0], acc[1] - x.ba[1], acc[2] - x.ba[2]};
-  const D3 ra = d33_mulv(rot, D3{{ab[0], ab[1], ab[2]}});
-  for (int k = 0; k < 3; ++k) {
-    dvel[k] = dt * (ra.v[k] + x.g[k]);
-    dpos[k] = dt * x.vel[k];
-  }
-  const double nw[3] = {(-1.0) * wdt[0], (-1.0) * wdt[1], (-1.0) * wdt[2]};
-  const D33 Re = so3_exp_d(nw);
-  const D33 Jr = d33_t(left_jac_d(wdt));
-  D33 nrot;
-  for (int i = 0; i < 9; ++i) nrot.m[i] = (-1.0) * rot.m[i];
-  const D33 Ah = d33_mul(nrot, hat_d(ab));
-  for (int i = 0; i < 9; ++i) {
-    B.Rexp[i] = Re.m[i];
-    B.Jdt[i] = ((-1.0) * Jr.m[i]) * dt;
-    B.Adt[i] = Ah.m[i] * dt;
-    B.Rdt[i] = nrot.m[i] * dt;
-  }
-  double Bx[6];
-  getBx_d(x.g, Bx);
-  const D33 hg = hat_d(x.g);
-  double Mx[6];  // (-hat(g)) Bx
-  for (int r = 0; r < 3; ++r)
-    for (int c = 0; c < 2; ++c) {
-      double s = ((-1.0) * hg.m[r * 3]) * Bx[c];
-      s += ((-1.0) * hg.m[r * 3 + 1]) * Bx[2 + c];
-      s += ((-1.0) * hg.m[r * 3 + 2]) * Bx[4 + c];
-      Mx[r * 2 + c] = s;
-    }
-  for (int i = 0; i < 6; ++i) B.Mdt[i] = Mx[i] * dt;
-  double N[6];
-  getNx_d(x.g, N);
-  for (int r = 0; r < 2; ++r)
-    for (int c = 0; c < 2; ++c) {
-      double s = N[r * 3] * Mx[c];
-      s += N[r * 3 + 1] * Mx[2 + c];
-      s += N[r * 3 + 2] * Mx[4 + c];
-      B.NM[r * 2 + c] = s;
-    }
-  B.dt = dt;
-  // x += delta: rot_ext, pos_ext, bg, ba and g take zero increments
-  for (int k = 0; k < 3; ++k) {
-    x.pos[k] = x.pos[k] + dpos[k];
-    x.vel[k] = x.vel[k] + dvel[k];
-  }
-  d33_to_cm(d33_mul(rot, so3_exp_d(wdt)), x.rot);
-}
-
-// column indices of the structurally non-zero entries of row r of F, in
-// ascending order (returns the count)
-__device__ inline int f_row_cols(int r, int* cols) {
-  int n = 0;
-  if (r < 3) {
-    cols[n++] = r;
-    cols[n++] = 12 + r;
-  } else if (r < 6) {
-    for (int k = 3; k < 6; ++k) cols[n++] = k;
-    for (int k = 15; k < 18; ++k) cols[n++] = k;
-  } else if (r >= 12 && r < 15) {
-    for (int k = 3; k < 6; ++k) cols[n++] = k;
-    cols[n++] = r;
-    for (int k = 18; k < 23; ++k) cols[n++] = k;
-  } else if (r >= 21) {
-    cols[n++] = 21;
-    cols[n++] = 22;
-  } else {
-    cols[n++] = r;
-  }
-  return n;
-}
-__device__ inline double f_at(const PredBlocks& B, int r, int c) {
-  if (r < 3) return c == r ? 1.0 : (c == 12 + r ? B.dt : 0.0);  // (I dt) off the diagonal is 0 dt
-  if (r < 6) {
-    if (c >= 3 && c < 6) return B.Rexp[(r - 3) * 3 + (c - 3)];
-    if (c >= 15 && c < 18) return B.Jdt[(r - 3) * 3 + (c - 15)];
-    return 0.0;
-  }
-  if (r >= 12 && r < 15) {
-    if (c >= 3 && c < 6) return B.Adt[(r - 12) * 3 + (c - 3)];
-    if (c >= 18 && c < 21) return B.Rdt[(r - 12) * 3 + (c - 18)];
-    if (c >= 21) return B.Mdt[(r - 12) * 2 + (c - 21)];
-    return c == r ? 1.0 : 0.0;
-  }
-  if (r >= 21) return c >= 21 ? B.NM[(r - 21) * 2 + (c - 21)] : 0.0;
-  return c == r ? 1.0 : 0.0;
-}
-// G (23x12): rows 3-5 <- cols 0-2 (-Jr dt), 12-14 <- 3-5 (-rot dt),
-// 15-17 <- 6-8 (I dt), 18-20 <- 9-11 (I dt)
+// IMU propagation of one package (IESKF::predict per IMU pair, ieskf.cpp:
+// 83-111, from lio_builder.cpp:80-119): one 64-lane workgroup per session,
+// the pairs in chunks of up to 64 steps, each chunk in four phases.
+//   A  lanes: each step's input (mid-point acc and gyro, dt), its rotation
+//      increment exp((gyro - bg) dt) and -Jr dt (IMU data and the biases,
+//      which do not change while a package is propagated)
+//   B  lane 0: the state chain in order (velocity, position, rotation),
+//      keeping the rotation before each step and the state after it
+//   C  lanes: each step's blocks that depend on the rotation before it,
+//      (-rot hat(acc - ba)) dt and (-rot) dt, and the IMU pose table entry
+//      at the step's tail
+//   D  all lanes, step after step: P = F P F^T + G Q G^T.  F is the identity
+//      except in rows 0-5, 12-14 and 21-22, so F P is formed for those rows
+//      only (its other rows are rows of P) and (F P) F^T for those columns
+//      only (its other columns are columns of F P).  Every sum runs over the
+//      structurally non-zero terms in ascending index order, the order of
+//      the dense product's sums; G Q G^T adds to the diagonal blocks 3-5,
+//      12-14, 15-17 and 18-20.
+// exp(-w) is exp(w)^T bit for bit (the quaternion form only flips signs),
+// so F's rotation block is the transpose of the step's increment.
+constexpr int kPredChunk = 64;
+constexpr int kFpRows = 11;  // rows of F that differ from the identity's
+__device__ inline int fp_slot(int r) { return r < 6 ? r : (r >= 12 && r < 15) ? r - 6 : r >= 21 ? r - 12 : -1; }
+__device__ inline int fp_row(int slot) { return slot < 6 ? slot : slot < 9 ? slot + 6 : slot + 12; }
+// G's row groups: rows 3-5 (ng), 12-14 (na), 15-17 (nbg), 18-20 (nba)
 __device__ inline int g_group(int r) {
   return (r >= 3 && r < 6) ? 0 : (r >= 12 && r < 15) ? 1 : (r >= 15 && r < 18) ? 2 : (r >= 18 && r < 21) ? 3 : -1;
 }
-__device__ inline double g_at(const PredBlocks& B, int r, int c) {
-  const int gr = g_group(r);
-  if (gr < 0 || c / 3 != gr) return 0.0;
-  const int i = r - (gr == 0 ? 3 : gr == 1 ? 12 : gr == 2 ? 15 : 18), j = c % 3;
-  if (gr == 0) return B.Jdt[i * 3 + j];
-  if (gr == 1) return B.Rdt[i * 3 + j];
-  return (i == j ? 1.0 : 0.0) * B.dt;
-}
 
-// one predict step of P on lanes 0..22 (row per lane); P, FP row-major in LDS
-__device__ inline void predict_cov(const PredBlocks& B, const double q[4], double* P, double* FP, int lane) {
-  int cols[12];
-  if (lane < 23) {  // FP = F P
-    const int nc = f_row_cols(lane, cols);
-    for (int j = 0; j < 23; ++j) {
-      double s = 0.0;
-      for (int t = 0; t < nc; ++t) s += f_at(B, lane, cols[t]) * P[cols[t] * 23 + j];
-      FP[lane * 23 + j] = s;
+struct PredShared {
+  double P[2][529];         // row-major, the current one and the next
+  double FP[kFpRows * 23];  // rows fp_row(0..10) of F P
+  double E[kPredChunk][9];  // per step of the chunk: exp((gyro - bg) dt), row-major
+  double J[kPredChunk][9];  // -Jr((gyro - bg) dt) dt (F and G)
+  double A[kPredChunk][9];  // (-rot hat(acc - ba)) dt
+  double Rd[kPredChunk][9]; // (-rot) dt (F and G)
+  double acc[kPredChunk][3], gyro[kPredChunk][3];
+  double dt[kPredChunk], tt[kPredChunk];
+  double R[kPredChunk + 1][9];  // rotation before each step, and after the last
+  double vel[kPredChunk + 1][3], pos[kPredChunk + 1][3];  // after step j at j + 1
+  double Mx[6], NM[4], q[4];
+  double off[kMaxImu + 1];  // pose table offsets
+  vmp_state sx;
+};
+
+// phase D for step j of the chunk: S.P[cur] -> S.P[cur ^ 1]
+__device__ inline void predict_cov_step(PredShared& S, int j, int cur, int lane) {
+  const double* P = S.P[cur];
+  double* Pn = S.P[cur ^ 1];
+  const double dt = S.dt[j];
+  const double* E = S.E[j];
+  const double* Jd = S.J[j];
+  const double* Ad = S.A[j];
+  const double* Rd = S.Rd[j];
+  for (int o = lane; o < kFpRows * 23; o += 64) {  // F P, the non-identity rows
+    const int slot = o / 23, c = o % 23, r = fp_row(slot);
+    double s;
+    if (r < 3) {
+      s = P[r * 23 + c];
+      s += dt * P[(12 + r) * 23 + c];
+    } else if (r < 6) {
+      const int i = r - 3;  // exp(-w) row i = exp(w) column i
+      s = E[i] * P[3 * 23 + c];
+      s += E[3 + i] * P[4 * 23 + c];
+      s += E[6 + i] * P[5 * 23 + c];
+      s += Jd[i * 3] * P[15 * 23 + c];
+      s += Jd[i * 3 + 1] * P[16 * 23 + c];
+      s += Jd[i * 3 + 2] * P[17 * 23 + c];
+    } else if (r < 15) {
+      const int i = r - 12;
+      s = Ad[i * 3] * P[3 * 23 + c];
+      s += Ad[i * 3 + 1] * P[4 * 23 + c];
+      s += Ad[i * 3 + 2] * P[5 * 23 + c];
+      s += P[r * 23 + c];
+      s += Rd[i * 3] * P[18 * 23 + c];
+      s += Rd[i * 3 + 1] * P[19 * 23 + c];
+      s += Rd[i * 3 + 2] * P[20 * 23 + c];
+      s += (S.Mx[i * 2] * dt) * P[21 * 23 + c];
+      s += (S.Mx[i * 2 + 1] * dt) * P[22 * 23 + c];
+    } else {
+      const int i = r - 21;
+      s = S.NM[i * 2] * P[21 * 23 + c];
+      s += S.NM[i * 2 + 1] * P[22 * 23 + c];
     }
+    S.FP[slot * 23 + c] = s;
   }
   __syncthreads();
-  if (lane < 23) {  // P = FP F^T + (G Q) G^T
-    const int gi = g_group(lane);
-    for (int j = 0; j < 23; ++j) {
-      const int nc = f_row_cols(j, cols);
-      double s = 0.0;
-      for (int t = 0; t < nc; ++t) s += FP[lane * 23 + cols[t]] * f_at(B, j, cols[t]);
-      double g = 0.0;
-      if (gi >= 0 && g_group(j) == gi)
-        for (int c = 3 * gi; c < 3 * gi + 3; ++c) g += (g_at(B, lane, c) * q[gi]) * g_at(B, j, c);
-      P[lane * 23 + j] = s + g;
+  for (int o = lane; o < 529; o += 64) {  // (F P) F^T + G Q G^T, column by column
+    const int c = o / 23, r = o % 23;
+    const int rs = fp_slot(r);
+    const double* fr = rs >= 0 ? &S.FP[rs * 23] : &P[r * 23];  // row r of F P
+    double s;
+    if (c < 3) {
+      s = fr[c];
+      s += fr[12 + c] * dt;
+    } else if (c < 6) {
+      const int i = c - 3;
+      s = fr[3] * E[i];
+      s += fr[4] * E[3 + i];
+      s += fr[5] * E[6 + i];
+      s += fr[15] * Jd[i * 3];
+      s += fr[16] * Jd[i * 3 + 1];
+      s += fr[17] * Jd[i * 3 + 2];
+    } else if (c >= 12 && c < 15) {
+      const int i = c - 12;
+      s = fr[3] * Ad[i * 3];
+      s += fr[4] * Ad[i * 3 + 1];
+      s += fr[5] * Ad[i * 3 + 2];
+      s += fr[c];
+      s += fr[18] * Rd[i * 3];
+      s += fr[19] * Rd[i * 3 + 1];
+      s += fr[20] * Rd[i * 3 + 2];
+      s += fr[21] * (S.Mx[i * 2] * dt);
+      s += fr[22] * (S.Mx[i * 2 + 1] * dt);
+    } else if (c >= 21) {
+      const int i = c - 21;
+      s = fr[21] * S.NM[i * 2];
+      s += fr[22] * S.NM[i * 2 + 1];
+    } else {
+      s = fr[c];
     }
+    const int gr = g_group(r);
+    double g = 0.0;
+    if (gr >= 0 && gr == g_group(c)) {
+      const double q = S.q[gr];
+      if (gr >= 2) {
+        if (r == c) g = (dt * q) * dt;
+      } else {
+        const double* Gr = gr == 0 ? &Jd[(r - 3) * 3] : &Rd[(r - 12) * 3];
+        const double* Gc = gr == 0 ? &Jd[(c - 3) * 3] : &Rd[(c - 12) * 3];
+        g = (Gr[0] * q) * Gc[0];
+        g += (Gr[1] * q) * Gc[1];
+        g += (Gr[2] * q) * Gc[2];
+      }
+    }
+    Pn[r * 23 + c] = s + g;
   }
-  __syncthreads();
 }
 
-// undistortCloud's IMU part for one session per workgroup (64 lanes).  The
-// biases and gravity do not change while a package is propagated, so every
-// step's rotation increment exp((gyro - bg) dt) and right Jacobian depend on
-// the IMU data alone: lanes compute them for up to 64 steps at once (the
-// transcendental part), then the steps run in order: the state on lane 0
-// (products only), the rows of P on lanes 0-22.  exp(-w) is exp(w)^T bit
-// for bit (the quaternion form only flips signs), so F's rotation block is
-// the transpose of the step's increment.
 __global__ void __launch_bounds__(64) k_imu_predict(const PreJob* jobs, int nsess) {
   if (int(blockIdx.x) >= nsess) return;
-  const PreJob J = jobs[blockIdx.x];
-  ImuDev* im = J.imu;
-  FilterDev* f = J.f;
+  const PreJob Jb = jobs[blockIdx.x];
+  ImuDev* im = Jb.imu;
+  FilterDev* f = Jb.f;
   const int lane = threadIdx.x;
-  __shared__ double P[529], FP[529];
-  __shared__ PredBlocks B;
-  __shared__ vmp_state sx;
-  __shared__ double q[4];
-  __shared__ double cE[64][9], cJ[64][9], cA[64][3], cG[64][3], cDt[64], cT[64];
-  __shared__ double Mx[6], NM[4];
-  for (int i = lane; i < 529; i += 64) P[(i % 23) * 23 + i / 23] = f->P[i];  // column-major -> rows
-  load_state(f->x, sx, lane);
-  if (lane < 4) q[lane] = im->q[lane];
+  __shared__ PredShared S;
+  for (int i = lane; i < 529; i += 64) S.P[0][(i % 23) * 23 + i / 23] = f->P[i];  // column-major -> rows
+  load_state(f->x, S.sx, lane);
+  if (lane < 4) S.q[lane] = im->q[lane];
   __syncthreads();
   const int m = im->pkg.m;
   const double lce = im->last_cloud_end;
+  const double cstart = im->pkg.cloud_start;
   // active IMU pairs (head k-1 or last_imu, tail k) are those whose tail is
   // not before the last scan end: a prefix of the ascending samples is skipped
   int k0 = 0;
@@ -357,32 +339,33 @@ __global__ void __launch_bounds__(64) k_imu_predict(const PreJob* jobs, int nses
   const int nsteps = (m - k0) + 1;  // the pairs, then the step to the sweep's end
   if (lane == 0) {
     double Bx[6];
-    getBx_d(sx.g, Bx);
-    const D33 hg = hat_d(sx.g);
+    getBx_d(S.sx.g, Bx);
+    const D33 hg = hat_d(S.sx.g);
     for (int r = 0; r < 3; ++r)
       for (int c = 0; c < 2; ++c) {
         double v = ((-1.0) * hg.m[r * 3]) * Bx[c];
         v += ((-1.0) * hg.m[r * 3 + 1]) * Bx[2 + c];
         v += ((-1.0) * hg.m[r * 3 + 2]) * Bx[4 + c];
-        Mx[r * 2 + c] = v;
+        S.Mx[r * 2 + c] = v;
       }
     double N[6];
-    getNx_d(sx.g, N);
+    getNx_d(S.sx.g, N);
     for (int r = 0; r < 2; ++r)
       for (int c = 0; c < 2; ++c) {
-        double v = N[r * 3] * Mx[c];
-        v += N[r * 3 + 1] * Mx[2 + c];
-        v += N[r * 3 + 2] * Mx[4 + c];
-        NM[r * 2 + c] = v;
+        double v = N[r * 3] * S.Mx[c];
+        v += N[r * 3 + 1] * S.Mx[2 + c];
+        v += N[r * 3 + 2] * S.Mx[4 + c];
+        S.NM[r * 2 + c] = v;
       }
     double* p = im->pose[0];  // pose 0: the start of the sweep
-    const D33 rot = d33_from_cm(sx.rot);
+    const D33 rot = d33_from_cm(S.sx.rot);
     p[0] = 0.0;
+    S.off[0] = 0.0;
     for (int c = 0; c < 3; ++c) {
       p[1 + c] = im->last_acc[c];
       p[4 + c] = im->last_gyro[c];
-      p[7 + c] = sx.vel[c];
-      p[10 + c] = sx.pos[c];
+      p[7 + c] = S.sx.vel[c];
+      p[10 + c] = S.sx.pos[c];
     }
     for (int c = 0; c < 9; ++c) p[13 + c] = rot.m[c];
   }
@@ -412,81 +395,102 @@ __global__ void __launch_bounds__(64) k_imu_predict(const PreJob* jobs, int nses
       dt = im->pkg.cloud_end - imu_end;
     }
   };
-  int np = 1;
-  for (int c0 = 0; c0 < nsteps; c0 += 64) {
-    const int cn = min(64, nsteps - c0);
-    if (lane < cn) {  // the steps' rotation increments and right Jacobians, in parallel
+  int cur = 0;
+  int np = 1;  // pose table entries
+  for (int c0 = 0; c0 < nsteps; c0 += kPredChunk) {
+    const int cn = min(kPredChunk, nsteps - c0);
+    if (lane < cn) {  // A
       double a[3], g[3], dt, tt;
       step_input(c0 + lane, a, g, dt, tt);
       double wdt[3];
-      for (int c = 0; c < 3; ++c) wdt[c] = dt * (g[c] - sx.bg[c]);
+      for (int c = 0; c < 3; ++c) wdt[c] = dt * (g[c] - S.sx.bg[c]);
       const D33 E = so3_exp_d(wdt);
       const D33 Jr = d33_t(left_jac_d(wdt));
       for (int i = 0; i < 9; ++i) {
-        cE[lane][i] = E.m[i];
-        cJ[lane][i] = ((-1.0) * Jr.m[i]) * dt;
+        S.E[lane][i] = E.m[i];
+        S.J[lane][i] = ((-1.0) * Jr.m[i]) * dt;
       }
       for (int c = 0; c < 3; ++c) {
-        cA[lane][c] = a[c];
-        cG[lane][c] = g[c];
+        S.acc[lane][c] = a[c];
+        S.gyro[lane][c] = g[c];
       }
-      cDt[lane] = dt;
-      cT[lane] = tt;
+      S.dt[lane] = dt;
+      S.tt[lane] = tt;
     }
     __syncthreads();
-    for (int j = 0; j < cn; ++j) {
-      const bool last = c0 + j == nsteps - 1;
-      if (lane == 0) {  // IESKF::predict on the state: blocks from the state before the step, then x += delta
-        const double dt = cDt[j];
-        const D33 rot = d33_from_cm(sx.rot);
-        const double ab[3] = {cA[j][0] - sx.ba[0], cA[j][1] - sx.ba[1], cA[j][2] - sx.ba[2]};
+    if (lane == 0) {  // B: x += delta step after step (the blocks come from the state before it)
+      vmp_state& x = S.sx;
+      D33 rot = d33_from_cm(x.rot);
+      for (int j = 0; j < cn; ++j) {
+        for (int i = 0; i < 9; ++i) S.R[j][i] = rot.m[i];
+        const double dt = S.dt[j];
+        const double ab[3] = {S.acc[j][0] - x.ba[0], S.acc[j][1] - x.ba[1], S.acc[j][2] - x.ba[2]};
         const D3 ra = d33_mulv(rot, D3{{ab[0], ab[1], ab[2]}});
-        D33 nrot;
-        for (int i = 0; i < 9; ++i) nrot.m[i] = (-1.0) * rot.m[i];
-        const D33 Ah = d33_mul(nrot, hat_d(ab));
-        for (int r = 0; r < 3; ++r)
-          for (int c = 0; c < 3; ++c) B.Rexp[r * 3 + c] = cE[j][c * 3 + r];
-        for (int i = 0; i < 9; ++i) {
-          B.Jdt[i] = cJ[j][i];
-          B.Adt[i] = Ah.m[i] * dt;
-          B.Rdt[i] = nrot.m[i] * dt;
-        }
-        for (int i = 0; i < 6; ++i) B.Mdt[i] = Mx[i] * dt;
-        for (int i = 0; i < 4; ++i) B.NM[i] = NM[i];
-        B.dt = dt;
         for (int c = 0; c < 3; ++c) {
-          const double dvel = dt * (ra.v[c] + sx.g[c]);
-          const double dpos = dt * sx.vel[c];
-          sx.pos[c] = sx.pos[c] + dpos;
-          sx.vel[c] = sx.vel[c] + dvel;
+          const double dvel = dt * (ra.v[c] + x.g[c]);
+          const double dpos = dt * x.vel[c];
+          x.pos[c] = x.pos[c] + dpos;
+          x.vel[c] = x.vel[c] + dvel;
+          S.vel[j + 1][c] = x.vel[c];
+          S.pos[j + 1][c] = x.pos[c];
         }
         D33 E;
-        for (int i = 0; i < 9; ++i) E.m[i] = cE[j][i];
-        const D33 rn = d33_mul(rot, E);
-        d33_to_cm(rn, sx.rot);
-        if (!last) {  // the IMU pose at the pair's tail
-          const D3 rb = d33_mulv(rn, D3{{ab[0], ab[1], ab[2]}});
-          double* p = im->pose[np];
-          p[0] = cT[j] - im->pkg.cloud_start;
-          for (int c = 0; c < 3; ++c) {
-            im->last_gyro[c] = cG[j][c] - sx.bg[c];
-            im->last_acc[c] = rb.v[c] + sx.g[c];
-            p[1 + c] = im->last_acc[c];
-            p[4 + c] = im->last_gyro[c];
-            p[7 + c] = sx.vel[c];
-            p[10 + c] = sx.pos[c];
-          }
-          for (int c = 0; c < 9; ++c) p[13 + c] = rn.m[c];
-        }
+        for (int i = 0; i < 9; ++i) E.m[i] = S.E[j][i];
+        rot = d33_mul(rot, E);
       }
-      __syncthreads();
-      predict_cov(B, q, P, FP, lane);
-      if (!last) ++np;
+      for (int i = 0; i < 9; ++i) S.R[cn][i] = rot.m[i];
+      d33_to_cm(rot, x.rot);
     }
+    __syncthreads();
+    if (lane < cn) {  // C
+      const int j = lane;
+      const double dt = S.dt[j];
+      const double ab[3] = {S.acc[j][0] - S.sx.ba[0], S.acc[j][1] - S.sx.ba[1], S.acc[j][2] - S.sx.ba[2]};
+      D33 nrot, rot;
+      for (int i = 0; i < 9; ++i) {
+        rot.m[i] = S.R[j][i];
+        nrot.m[i] = (-1.0) * rot.m[i];
+      }
+      const D33 Ah = d33_mul(nrot, hat_d(ab));
+      for (int i = 0; i < 9; ++i) {
+        S.A[j][i] = Ah.m[i] * dt;
+        S.Rd[j][i] = nrot.m[i] * dt;
+      }
+      if (c0 + j != nsteps - 1) {  // the IMU pose at the pair's tail
+        D33 rn;
+        for (int i = 0; i < 9; ++i) rn.m[i] = S.R[j + 1][i];
+        const D3 rb = d33_mulv(rn, D3{{ab[0], ab[1], ab[2]}});
+        double* p = im->pose[np + j];
+        p[0] = S.tt[j] - cstart;
+        S.off[np + j] = p[0];
+        double la[3], lg[3];
+        for (int c = 0; c < 3; ++c) {
+          lg[c] = S.gyro[j][c] - S.sx.bg[c];
+          la[c] = rb.v[c] + S.sx.g[c];
+          p[1 + c] = la[c];
+          p[4 + c] = lg[c];
+          p[7 + c] = S.vel[j + 1][c];
+          p[10 + c] = S.pos[j + 1][c];
+        }
+        for (int c = 0; c < 9; ++c) p[13 + c] = rn.m[c];
+        if (c0 + j == nsteps - 2)  // the last pair: LIODataGroup's last acc / gyro
+          for (int c = 0; c < 3; ++c) {
+            im->last_acc[c] = la[c];
+            im->last_gyro[c] = lg[c];
+          }
+      }
+    }
+    for (int j = 0; j < cn; ++j) {  // D
+      __syncthreads();
+      predict_cov_step(S, j, cur, lane);
+      cur ^= 1;
+    }
+    np += (c0 + cn == nsteps) ? cn - 1 : cn;
+    __syncthreads();
   }
-  for (int i = lane; i < 529; i += 64) f->P[i] = P[(i % 23) * 23 + i / 23];
+  for (int i = lane; i < 529; i += 64) f->P[i] = S.P[cur][(i % 23) * 23 + i / 23];
   if (lane == 0) {
-    f->x = sx;
+    f->x = S.sx;
     if (m > 0) {
       im->last_imu_t = im->pkg.s[m - 1][0];
       for (int c = 0; c < 3; ++c) {
@@ -499,7 +503,7 @@ __global__ void __launch_bounds__(64) k_imu_predict(const PreJob* jobs, int nses
     // running minimum of the offsets of the interval heads (entries 0..np-2)
     double mn = 1e300;
     for (int i = np - 2; i >= 0; --i) {
-      mn = fmin(mn, im->pose[i][0]);
+      mn = fmin(mn, S.off[i]);
       im->pose_min[i] = mn;
     }
   }
