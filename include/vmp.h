@@ -210,6 +210,22 @@ int vmp_ieskf_update(vmp_session* s, vmp_obs_stats* st);
 /* set points (device) + IESKF update + map update, no host round trip */
 int vmp_register_scan(vmp_session* s, vmp_obs_stats* ost, vmp_map_stats* mst);
 
+/* vmp_register_scan that allocates nothing (no counterpart in the
+ * reference): for callers that cannot allocate device memory in the call
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The step's work arrays live in a 64 MiB array per GPU that the
+ * library's code object holds until the process ends, set up by
+ * vmp_ctx_create.  Limits: the bound scan (raw package or points) holds at
+ * most 131072 points, and the session's LRU ring, 2 (map_capacity + 2
+ * max_scan_points) + 1024 entries, fits the scratch (about 3.2M entries:
+ * map_capacity up to about 1.3M at max_scan_points 131072); beyond either
+ * the call returns VMP_EINVAL before launching anything.  The filter stage
+ * is launched kernel by kernel (no hipGraph).  Results are bitwise those of
+ * vmp_register_scan.  Every context on a GPU shares its scratch, whose
+ * contents the next call overwrites: no-allocation calls that overlap on one
+ * GPU (from contexts on different host threads) race. */
+int vmp_register_scan_noalloc(vmp_session* s, vmp_obs_stats* ost, vmp_map_stats* mst);
+
 /* Map export, LRU order (most recent first).  Any output may be NULL.
  * keys[3i]; ints[5i] = {n, flags(1 init,2 plane,4 update_enable,8 merged),
  * newly_add_point, points held, 0}; group[i]; dbl[8i] = {mean xyz, norm xyz,

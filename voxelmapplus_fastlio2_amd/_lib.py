@@ -133,6 +133,7 @@ def vmp():
     _proto(lib, "vmp_map_update", C.c_int, [P, C.c_int, C.POINTER(MapStats)])
     _proto(lib, "vmp_ieskf_update", C.c_int, [P, C.POINTER(ObsStats)])
     _proto(lib, "vmp_register_scan", C.c_int, [P, C.POINTER(ObsStats), C.POINTER(MapStats)])
+    _proto(lib, "vmp_register_scan_noalloc", C.c_int, [P, C.POINTER(ObsStats), C.POINTER(MapStats)])
     _proto(lib, "vmp_register_batch", C.c_int, [pp, C.c_int32, C.POINTER(ObsStats), C.POINTER(MapStats)])
     _proto(lib, "vmp_register_batch_submit", C.c_int, [pp, C.c_int32])
     _proto(lib, "vmp_register_batch_wait", C.c_int, [P, C.c_int32, C.POINTER(ObsStats), C.POINTER(MapStats)])

@@ -38,6 +38,22 @@ __global__ void k_init_vs(VoxState* vs, int n) {
   }
 }
 
+// Device scratch of vmp_register_scan_noalloc: the map pipeline's per-point
+// work arrays, the rocPRIM temporary storage and the job tables of a
+// one-session step (and, between steps, the LRU ring compaction's arrays),
+// for scans of up to kNoAllocMaxPoints points.  One array per GPU, loaded
+// with the code object and held until the process ends; every context on a
+// GPU shares it, so no-allocation calls that overlap on one GPU race.  Host
+// code reaches it only through the address hipGetSymbolAddress returns for
+// the current device (g_scratch below), never through the name.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int32_t kNoAllocMaxPoints = 131072;
+constexpr int32_t kNoAllocPreCap = 16384;  // taint-phase prefetch entries (the rest look up directly)
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static uint8_t* g_scratch[kMaxDevices] = {};
+
 // Input validation: a scan with a NaN or infinite coordinate is rejected
 // before any stage reads it (its key would be undefined).  One launch covers
 // many point sets; block row y checks set y (grid-stride over sets).
@@ -105,6 +121,7 @@ static hipError_t dalloc(std::vector<void*>& allocs, T** p, size_t count) {
 struct BatchWs {
   int64_t cap_points = 0;
   int cap_sess = 0;
+  bool fixed = false;  // carved from the device scratch: never reallocated
   std::vector<void*> allocs;
   BatchBufs b{};
   void* temp = nullptr;
@@ -149,6 +166,7 @@ struct BatchWs {
 // Scratch of the device-side LRU ring compaction (grown on demand).
 struct RingTmp {
   int64_t cap = 0;
+  bool fixed = false;  // carved from the device scratch: never reallocated
   std::vector<void*> allocs;
   int32_t *tv = nullptr, *keep = nullptr, *sel = nullptr, *cnt = nullptr;
   uint64_t* ts = nullptr;
@@ -217,6 +235,12 @@ struct vmp_ctx {
   // validation of device point buffers (k_check_points), run on `stream`
   // when a step starts (no batch is in flight then)
   bool check_points = true;
+  // vmp_register_scan_noalloc: a workspace and ring scratch carved from the
+  // device's scratch array at context creation (na_ready once that
+  // succeeded); the call swaps them in for its duration
+  BatchWs ws_na;
+  RingTmp rt_na;
+  bool na_ready = false;
   int vcap = 0;
   PtsRef* h_refs = nullptr;
   PtsRef* d_refs = nullptr;
@@ -307,6 +331,8 @@ static int grid_for(int64_t n, int threads) {
 
 extern "C" const char* vmp_version(void) { return "voxelmapplus_fastlio2_amd 0.2 (gfx950)"; }
 
+static void noalloc_setup(vmp_ctx* c);
+
 extern "C" int vmp_ctx_create(int device, vmp_ctx** out) {
   if (!out) return VMP_EINVAL;
   int n = 0;
@@ -320,6 +346,7 @@ extern "C" int vmp_ctx_create(int device, vmp_ctx** out) {
   }
 
   for (hipEvent_t& e : c->ev) hipEventCreate(&e);
+  noalloc_setup(c);
   *out = c;
   return VMP_OK;
 }
@@ -338,6 +365,8 @@ extern "C" int vmp_ctx_destroy(vmp_ctx* ctx) {
     if (g.exec) hipGraphExecDestroy(g.exec);
   ctx->ws.release();
   if (ctx->ws.motion_done) hipEventDestroy(ctx->ws.motion_done);
+  ctx->ws_na.release();  // its pinned mirrors (the device arrays are the scratch's)
+  if (ctx->ws_na.motion_done) hipEventDestroy(ctx->ws_na.motion_done);
   ctx->rt.release();
   for (hipEvent_t e : ctx->ev)
     if (e) hipEventDestroy(e);
@@ -358,27 +387,32 @@ extern "C" int vmp_ctx_destroy(vmp_ctx* ctx) {
 // order) on the context's stream (c->mu held, no batch in flight).  Returns
 // the index of the first bad set, -1 if all are good, or -2 on a device
 // error.
+// room for nref point sets in the check's tables (at least 64)
+static bool grow_checks(vmp_ctx* c, int nref) {
+  if (nref <= c->vcap) return true;
+  hipStreamSynchronize(c->stream);
+  if (c->h_refs) hipHostFree(c->h_refs);
+  if (c->h_bad) hipHostFree(c->h_bad);
+  if (c->d_refs) hipFree(c->d_refs);
+  if (c->d_bad) hipFree(c->d_bad);
+  c->h_refs = nullptr;
+  c->h_bad = nullptr;
+  c->d_refs = nullptr;
+  c->d_bad = nullptr;
+  c->vcap = 0;
+  const int cap = std::max(nref, 64);
+  if (hipHostMalloc(&c->h_refs, sizeof(PtsRef) * cap) != hipSuccess ||
+      hipHostMalloc(&c->h_bad, sizeof(int32_t) * cap) != hipSuccess ||
+      hipMalloc(&c->d_refs, sizeof(PtsRef) * cap) != hipSuccess ||
+      hipMalloc(&c->d_bad, sizeof(int32_t) * cap) != hipSuccess)
+    return false;
+  c->vcap = cap;
+  return true;
+}
+
 static int check_device_points(vmp_ctx* c, const PtsRef* refs, int nref) {
   if (nref <= 0 || !c->check_points) return -1;
-  if (nref > c->vcap) {
-    hipStreamSynchronize(c->stream);
-    if (c->h_refs) hipHostFree(c->h_refs);
-    if (c->h_bad) hipHostFree(c->h_bad);
-    if (c->d_refs) hipFree(c->d_refs);
-    if (c->d_bad) hipFree(c->d_bad);
-    c->h_refs = nullptr;
-    c->h_bad = nullptr;
-    c->d_refs = nullptr;
-    c->d_bad = nullptr;
-    c->vcap = 0;
-    const int cap = std::max(nref, 64);
-    if (hipHostMalloc(&c->h_refs, sizeof(PtsRef) * cap) != hipSuccess ||
-        hipHostMalloc(&c->h_bad, sizeof(int32_t) * cap) != hipSuccess ||
-        hipMalloc(&c->d_refs, sizeof(PtsRef) * cap) != hipSuccess ||
-        hipMalloc(&c->d_bad, sizeof(int32_t) * cap) != hipSuccess)
-      return -2;
-    c->vcap = cap;
-  }
+  if (!grow_checks(c, nref)) return -2;
   int64_t most = 0;
   for (int i = 0; i < nref; ++i) {
     c->h_refs[i] = refs[i];
@@ -406,10 +440,59 @@ static bool host_points_finite(const float* x, const float* y, const float* z, i
 
 static int settle(vmp_ctx* c);
 
+// Lays out the batch workspace for n points and ns sessions: `dev(&p,
+// count)` provides each device array, `host(&p, bytes)` each pinned host
+// mirror; both return false on failure.  The rocPRIM temporary storage is
+// sized for the largest sort or compaction at n and provided last.
+template <class Dev, class Host>
+static bool ws_layout(vmp_ctx* c, BatchWs& w, size_t n, int ns, size_t pre_cap, Dev&& dev, Host&& host) {
+  BatchBufs& b = w.b;
+  bool ok = true;
+  ok = ok && dev(&b.skey_in, n) && dev(&b.sval_in, n) && dev(&b.skey_out, n) && dev(&b.sval_out, n);
+  ok = ok && dev(&b.flags, n) && dev(&b.sel, n) && dev(&b.cnt, size_t(8));
+  ok = ok && dev(&b.seg_start, n) && dev(&b.seg_end, n) && dev(&b.seg_sess, n) && dev(&b.seg_vid, n) &&
+       dev(&b.seg_entry, n);
+  ok = ok && dev(&b.cre_t, n) && dev(&b.cre_seg, n) && dev(&b.co, n) && dev(&b.ev_vid, n);
+  ok = ok && dev(&b.cand_in, n) && dev(&b.cand_out, n) && dev(&b.cand_nbr, n * 6) && dev(&b.cand_spec, n) &&
+       dev(&b.cand_start, size_t(ns)) && dev(&b.spl, n);
+  ok = ok && dev(&b.seg_ct, n * 7) && dev(&b.seg_nct, n) && dev(&b.seg_nbr, n * 6);
+  ok = ok && dev(&b.len_in, n) && dev(&b.len_out, n) && dev(&b.ord_in, n) && dev(&b.ord, n);
+  ok = ok && dev(&b.lv, n * 2) && dev(&b.lv2, n * 2) && dev(&b.lv_cnt, (n * 2 >> 6) + size_t(ns) + 2) &&
+       dev(&b.lv_n, size_t(ns));
+  // taint-phase prefetch of up to pre_cap acting candidates per batch (the
+  // rest take the replay's own lookups)
+  b.pre_cap = int32_t(std::min(n, pre_cap));
+  ok = ok && dev(&b.pre_x, size_t(b.pre_cap) * kPreItems) && dev(&b.pre_seg, size_t(b.pre_cap) * kPreItems) &&
+       dev(&b.pre_tn, size_t(b.pre_cap) * kPreItems);
+  const size_t S = size_t(ns);
+  ok = ok && dev(&w.d_maps, S) && dev(&w.d_si, S) && dev(&w.d_jobs, S) && dev(&w.d_flist, S) &&
+       dev(&w.d_obsctr, S) && dev(&w.d_obsstat, S) && dev(&w.d_out, S) && dev(&w.d_motion, S) && dev(&w.d_pre, S) &&
+       dev(&w.d_vg, S) && dev(&w.d_vgst, S * kVgStat);
+  ok = ok && host(&w.h_maps, sizeof(MapDev) * S) && host(&w.h_si, sizeof(SessInfo) * S) &&
+       host(&w.h_jobs, sizeof(ObsJob) * S) && host(&w.h_flist, sizeof(FilterDev*) * S) &&
+       host(&w.h_obsctr, sizeof(int32_t*) * S) && host(&w.h_obsstat, sizeof(int32_t*) * S) &&
+       host(&w.h_out, sizeof(BatchOut) * S) && host(&w.h_motion, sizeof(MotionRec) * S) &&
+       host(&w.h_pre, sizeof(PreJob) * S) && host(&w.h_vg, sizeof(VgJob) * S) &&
+       host(&w.h_vgst, sizeof(int32_t) * kVgStat * S);
+  if (ok && !w.motion_done) ok = hipEventCreateWithFlags(&w.motion_done, hipEventDisableTiming) == hipSuccess;
+  if (!ok) return false;
+  size_t t[4] = {0, 0, 0, 0};
+  rocprim::radix_sort_pairs(nullptr, t[0], b.skey_in, b.skey_out, b.sval_in, b.sval_out, n, 0, 64, c->stream);
+  rocprim::radix_sort_keys(nullptr, t[1], b.cand_in, b.cand_out, n, 0, 64, c->stream);
+  rocprim::select(nullptr, t[2], rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt, n, c->stream);
+  rocprim::radix_sort_pairs(nullptr, t[3], b.len_in, b.len_out, b.ord_in, b.ord, n, 0, 32, c->stream);
+  w.temp_bytes = *std::max_element(t, t + 4) + 256;
+  uint8_t* temp = nullptr;
+  if (!dev(&temp, w.temp_bytes)) return false;
+  w.temp = temp;
+  return true;
+}
+
 // grow the batch workspace to hold `points` points and `nsess` sessions
 static int ws_reserve(vmp_ctx* c, int64_t points, int nsess, bool exact = false) {
   BatchWs& w = c->ws;
   if (points <= w.cap_points && nsess <= w.cap_sess) return VMP_OK;
+  if (w.fixed) return VMP_ENOMEM;  // the no-allocation workspace never grows
   settle(c);  // a batch in flight still reads the old workspace
   hipStreamSynchronize(c->stream);
   c->ws_gen++;  // captured filter graphs hold the old workspace pointers
@@ -419,98 +502,105 @@ static int ws_reserve(vmp_ctx* c, int64_t points, int nsess, bool exact = false)
   const int64_t np = std::max<int64_t>(grown, w.cap_points);
   const int ns = std::max(nsess, w.cap_sess);
   w.release();
-  BatchBufs& b = w.b;
-  const size_t n = size_t(np);
-  bool ok = true;
   auto& A = w.allocs;
-  ok &= dalloc(A, &b.skey_in, n) == hipSuccess;
-  ok &= dalloc(A, &b.sval_in, n) == hipSuccess;
-  ok &= dalloc(A, &b.skey_out, n) == hipSuccess;
-  ok &= dalloc(A, &b.sval_out, n) == hipSuccess;
-  ok &= dalloc(A, &b.flags, n) == hipSuccess;
-  ok &= dalloc(A, &b.sel, n) == hipSuccess;
-  ok &= dalloc(A, &b.cnt, 8) == hipSuccess;
-  ok &= dalloc(A, &b.seg_start, n) == hipSuccess;
-  ok &= dalloc(A, &b.seg_end, n) == hipSuccess;
-  ok &= dalloc(A, &b.seg_sess, n) == hipSuccess;
-  ok &= dalloc(A, &b.seg_vid, n) == hipSuccess;
-  ok &= dalloc(A, &b.seg_entry, n) == hipSuccess;
-  ok &= dalloc(A, &b.cre_t, n) == hipSuccess;
-  ok &= dalloc(A, &b.cre_seg, n) == hipSuccess;
-  ok &= dalloc(A, &b.co, n) == hipSuccess;
-  ok &= dalloc(A, &b.ev_vid, n) == hipSuccess;
-  ok &= dalloc(A, &b.cand_in, n) == hipSuccess;
-  ok &= dalloc(A, &b.cand_out, n) == hipSuccess;
-  ok &= dalloc(A, &b.cand_nbr, n * 6) == hipSuccess;
-  ok &= dalloc(A, &b.cand_spec, n) == hipSuccess;
-  ok &= dalloc(A, &b.cand_start, size_t(ns)) == hipSuccess;
-  ok &= dalloc(A, &b.spl, n) == hipSuccess;
-  ok &= dalloc(A, &b.seg_ct, n * 7) == hipSuccess;
-  ok &= dalloc(A, &b.seg_nct, n) == hipSuccess;
-  ok &= dalloc(A, &b.seg_nbr, n * 6) == hipSuccess;
-  ok &= dalloc(A, &b.len_in, n) == hipSuccess;
-  ok &= dalloc(A, &b.len_out, n) == hipSuccess;
-  ok &= dalloc(A, &b.ord_in, n) == hipSuccess;
-  ok &= dalloc(A, &b.ord, n) == hipSuccess;
-  ok &= dalloc(A, &b.lv, n * 2) == hipSuccess;
-  ok &= dalloc(A, &b.lv2, n * 2) == hipSuccess;
-  ok &= dalloc(A, &b.lv_cnt, (n * 2 >> 6) + size_t(ns) + 2) == hipSuccess;
-  ok &= dalloc(A, &b.lv_n, size_t(ns)) == hipSuccess;
-  // taint-phase prefetch of up to 256k acting candidates per batch (the rest
-  // take the replay's own lookups)
-  b.pre_cap = int32_t(std::min<size_t>(n, size_t(1) << 18));
-  ok &= dalloc(A, &b.pre_x, size_t(b.pre_cap) * kPreItems) == hipSuccess;
-  ok &= dalloc(A, &b.pre_seg, size_t(b.pre_cap) * kPreItems) == hipSuccess;
-  ok &= dalloc(A, &b.pre_tn, size_t(b.pre_cap) * kPreItems) == hipSuccess;
-  ok &= dalloc(A, &w.d_maps, size_t(ns)) == hipSuccess;
-  ok &= dalloc(A, &w.d_si, size_t(ns)) == hipSuccess;
-  ok &= dalloc(A, &w.d_jobs, size_t(ns)) == hipSuccess;
-  ok &= dalloc(A, &w.d_flist, size_t(ns)) == hipSuccess;
-  ok &= dalloc(A, &w.d_obsctr, size_t(ns)) == hipSuccess;
-  ok &= dalloc(A, &w.d_obsstat, size_t(ns)) == hipSuccess;
-  ok &= dalloc(A, &w.d_out, size_t(ns)) == hipSuccess;
-  ok &= dalloc(A, &w.d_motion, size_t(ns)) == hipSuccess;
-  ok &= dalloc(A, &w.d_pre, size_t(ns)) == hipSuccess;
-  ok &= dalloc(A, &w.d_vg, size_t(ns)) == hipSuccess;
-  ok &= dalloc(A, &w.d_vgst, size_t(ns) * kVgStat) == hipSuccess;
-  auto halloc = [&](auto** p, size_t bytes) {
+  auto dev = [&](auto** p, size_t count) { return dalloc(A, p, count) == hipSuccess; };
+  auto host = [&](auto** p, size_t bytes) {
     void* q = nullptr;
     if (hipHostMalloc(&q, std::max<size_t>(bytes, 16)) != hipSuccess) return false;
     w.host_allocs.push_back(q);
     *p = static_cast<std::remove_reference_t<decltype(**p)>*>(q);
     return true;
   };
-  ok &= halloc(&w.h_maps, sizeof(MapDev) * ns);
-  ok &= halloc(&w.h_si, sizeof(SessInfo) * ns);
-  ok &= halloc(&w.h_jobs, sizeof(ObsJob) * ns);
-  ok &= halloc(&w.h_flist, sizeof(FilterDev*) * ns);
-  ok &= halloc(&w.h_obsctr, sizeof(int32_t*) * ns);
-  ok &= halloc(&w.h_obsstat, sizeof(int32_t*) * ns);
-  ok &= halloc(&w.h_out, sizeof(BatchOut) * ns);
-  ok &= halloc(&w.h_motion, sizeof(MotionRec) * ns);
-  ok &= halloc(&w.h_pre, sizeof(PreJob) * ns);
-  ok &= halloc(&w.h_vg, sizeof(VgJob) * ns);
-  ok &= halloc(&w.h_vgst, sizeof(int32_t) * kVgStat * ns);
-  if (ok && !w.motion_done) ok = hipEventCreateWithFlags(&w.motion_done, hipEventDisableTiming) == hipSuccess;
   w.motion_pending = false;
-  if (!ok) {
-    w.release();
-    return VMP_ENOMEM;
-  }
-  // rocprim scratch: the largest of the sorts and compactions at this size
-  size_t t[4] = {0, 0, 0, 0};
-  rocprim::radix_sort_pairs(nullptr, t[0], b.skey_in, b.skey_out, b.sval_in, b.sval_out, n, 0, 64, c->stream);
-  rocprim::radix_sort_keys(nullptr, t[1], b.cand_in, b.cand_out, n, 0, 64, c->stream);
-  rocprim::select(nullptr, t[2], rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt, n, c->stream);
-  rocprim::radix_sort_pairs(nullptr, t[3], b.len_in, b.len_out, b.ord_in, b.ord, n, 0, 32, c->stream);
-  w.temp_bytes = *std::max_element(t, t + 4) + 256;
-  if (dalloc(A, reinterpret_cast<uint8_t**>(&w.temp), w.temp_bytes) != hipSuccess) {
+  if (!ws_layout(c, w, size_t(np), ns, size_t(1) << 18, dev, host)) {
     w.release();
     return VMP_ENOMEM;
   }
   w.cap_points = np;
   w.cap_sess = ns;
   return VMP_OK;
+}
+
+// Bump placement inside the device scratch (256-byte aligned pieces).
+struct ScratchCarver {
+  uint8_t* base;
+  size_t cap;
+  size_t used = 0;
+  template <class T>
+  bool operator()(T** p, size_t count) {
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+    const size_t off = (used + 255) & ~size_t(255);
+    if (off + bytes > cap) return false;
+    *p = reinterpret_cast<T*>(base + off);
+    used = off + bytes;
+    return true;
+  }
+};
+
+// Set-up of vmp_register_scan_noalloc for a new context (its device
+// current): the device's scratch address, once per device, then the
+// one-session workspace for kNoAllocMaxPoints points carved from it (its
+// pinned host mirrors and the other host-side tables are allocated here, so
+// the entry point allocates nothing), and the LRU ring compaction's arrays
+// over the same bytes (they are used only between steps, after the step's
+// kernels have finished).  A failure leaves na_ready false; the entry point
+// then reports it.
+static void noalloc_setup(vmp_ctx* c) {
+  uint8_t* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (c->device >= kMaxDevices) return;
+    if (!g_scratch[c->device]) {
+      void* p = nullptr;
+      if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return;
+      g_scratch[c->device] = static_cast<uint8_t*>(p);
+    }
+    base = g_scratch[c->device];
+  }
+  BatchWs& w = c->ws_na;
+  ScratchCarver carve{base, kScratchBytes};
+  auto host = [&](auto** p, size_t bytes) {
+    void* q = nullptr;
+    if (hipHostMalloc(&q, std::max<size_t>(bytes, 16)) != hipSuccess) return false;
+    w.host_allocs.push_back(q);
+    *p = static_cast<std::remove_reference_t<decltype(**p)>*>(q);
+    return true;
+  };
+  if (!ws_layout(c, w, size_t(kNoAllocMaxPoints), 1, size_t(kNoAllocPreCap), carve, host)) {
+    w.release();
+    return;
+  }
+  w.cap_points = kNoAllocMaxPoints;
+  w.cap_sess = 1;
+  w.fixed = true;
+  // ring compaction: 20 bytes per entry plus the compaction's temporary
+  // storage, sized for as many entries as the scratch holds
+  RingTmp& t = c->rt_na;
+  size_t n = kScratchBytes / 20;
+  for (;;) {
+    size_t tb = 0;
+    if (rocprim::select(nullptr, tb, rocprim::counting_iterator<int32_t>(0), t.keep, t.sel, t.cnt, n, c->stream) !=
+        hipSuccess)
+      return;
+    ScratchCarver rc{base, kScratchBytes};
+    uint8_t* temp = nullptr;
+    if (rc(&t.tv, n) && rc(&t.ts, n) && rc(&t.keep, n) && rc(&t.sel, n) && rc(&t.cnt, size_t(1)) &&
+        rc(&temp, tb + 256)) {
+      t.temp = temp;
+      t.temp_bytes = tb + 256;
+      break;
+    }
+    n -= n / 16;
+  }
+  t.cap = int64_t(n);
+  t.fixed = true;
+  // host-side tables a one-session step would otherwise grow on first use
+  if (!grow_checks(c, 64)) return;
+  c->fl_ss.reserve(64);
+  c->fl_n.reserve(64);
+  c->res_ost.reserve(64);
+  c->res_mst.reserve(64);
+  c->na_ready = true;
 }
 
 struct Sizes {
@@ -1453,6 +1543,10 @@ static int maintain(vmp_ctx* ex, vmp_session* s, int32_t* did) {
   const int64_t used = s->h_ctr.ring_tail - s->h_ctr.ring_head;
   if (used + int64_t(s->cfg.max_scan_points) > m.qcap) {
     RingTmp& t = ex->rt;
+    if (t.cap < used && t.fixed) {
+      s->err = "LRU ring compaction: the ring exceeds the no-allocation scratch";
+      return VMP_ECAPACITY;
+    }
     if (t.cap < used) {
       VMP_HIP(hipStreamSynchronize(st));
       t.release();
@@ -1524,30 +1618,43 @@ static int step_finish(vmp_ctx* c, vmp_obs_stats* ost, vmp_map_stats* mst);
 // or times out of order, fails the call before anything is launched, and its
 // binding is dropped
 static int check_bound_points(vmp_ctx* c, vmp_session** ss, int n) {
-  std::vector<PtsRef> refs;
-  std::vector<int> who;
+  // tables on the stack for small batches (the no-allocation entry point
+  // passes one session), on the heap beyond
+  PtsRef refs_small[16];
+  int who_small[16];
+  std::vector<PtsRef> refs_big;
+  std::vector<int> who_big;
+  PtsRef* refs = refs_small;
+  int* who = who_small;
+  if (n > 16) {
+    refs_big.resize(size_t(n));
+    who_big.resize(size_t(n));
+    refs = refs_big.data();
+    who = who_big.data();
+  }
+  int nr = 0;
   for (int i = 0; i < n; ++i) {
     vmp_session* q = ss[i];
     if (!q->needs_check) continue;
-    if (q->raw_pending) refs.push_back(PtsRef{q->rx, q->ry, q->rz, q->rt, q->raw_n, 0});
-    else refs.push_back(PtsRef{q->cx, q->cy, q->cz, nullptr, q->n, 0});
-    who.push_back(i);
+    if (q->raw_pending) refs[nr] = PtsRef{q->rx, q->ry, q->rz, q->rt, q->raw_n, 0};
+    else refs[nr] = PtsRef{q->cx, q->cy, q->cz, nullptr, q->n, 0};
+    who[nr++] = i;
   }
-  if (refs.empty()) return VMP_OK;
-  const int bad = check_device_points(c, refs.data(), int(refs.size()));
+  if (nr == 0) return VMP_OK;
+  const int bad = check_device_points(c, refs, nr);
   if (bad == -2) {
     ss[0]->err = "point validation failed on the device";
     return VMP_EDEVICE;
   }
   if (bad >= 0) {
-    vmp_session* q = ss[who[size_t(bad)]];
+    vmp_session* q = ss[who[bad]];
     q->err = "scan holds a non-finite coordinate or time, or point times out of order";
     q->has_points = false;
     q->raw_pending = false;
     q->needs_check = false;
     return VMP_EINVAL;
   }
-  for (int i : who) ss[i]->needs_check = false;
+  for (int k = 0; k < nr; ++k) ss[who[k]]->needs_check = false;
   return VMP_OK;
 }
 
@@ -1574,26 +1681,33 @@ static int prepare_scans(vmp_ctx* c, vmp_session** ss, int n, bool is_build) {
   const int gx = int(std::max<int64_t>(1, std::min<int64_t>((most + 255) / 256, 1024)));
   timed_launch(c, kTkUndistort,
                [&] { hipLaunchKernelGGL(k_undistort, dim3(gx, std::min(npre, 65535)), dim3(256), 0, st, w.d_pre, npre); });
-  // the filter (never at MAP_INIT, lio_builder.cpp:185-211)
-  std::vector<vmp_session*> fs;
+  // the filter (never at MAP_INIT, lio_builder.cpp:185-211); the filtered
+  // sessions listed on the stack for small batches, on the heap beyond
+  vmp_session* fs_small[16];
+  std::vector<vmp_session*> fs_big;
+  vmp_session** fs = fs_small;
+  if (n > 16) {
+    fs_big.resize(size_t(n));
+    fs = fs_big.data();
+  }
+  int nf = 0;
   int64_t off = 0;
   most = 0;
   for (int i = 0; i < n && !is_build; ++i) {
     vmp_session* q = ss[i];
     if (!q->raw_pending || !(q->cfg.scan_resolution > 0.0) || !q->own_fx) continue;
     const float leaf = float(q->cfg.scan_resolution);
-    w.h_vg[fs.size()] = VgJob{q->own_x, q->own_y, q->own_z, q->own_fx, q->own_fy, q->own_fz, q->raw_n, int32_t(off),
-                              1.0f / leaf, 0};
+    w.h_vg[nf] = VgJob{q->own_x, q->own_y, q->own_z, q->own_fx, q->own_fy, q->own_fz, q->raw_n, int32_t(off),
+                       1.0f / leaf, 0};
     off += q->raw_n;
     most = std::max<int64_t>(most, q->raw_n);
-    fs.push_back(q);
+    fs[nf++] = q;
   }
   for (int i = 0; i < n; ++i)
     if (ss[i]->raw_pending) {
       ss[i]->raw_pending = false;
       ss[i]->cloud_n = ss[i]->raw_n;
     }
-  const int nf = int(fs.size());
   if (nf == 0) return VMP_OK;
   BatchBufs& b = w.b;
   const int ntot = int(off);
@@ -1624,7 +1738,7 @@ static int prepare_scans(vmp_ctx* c, vmp_session** ss, int n, bool is_build) {
       hipStreamSynchronize(st) != hipSuccess)
     return VMP_EDEVICE;
   for (int k = 0; k < nf; ++k) {
-    vmp_session* q = fs[size_t(k)];
+    vmp_session* q = fs[k];
     q->n = ntot > 0 ? w.h_vgst[k * kVgStat + 8] : 0;
     q->cx = q->own_fx;
     q->cy = q->own_fy;
@@ -1988,6 +2102,49 @@ extern "C" int vmp_register_scan(vmp_session* s, vmp_obs_stats* ost, vmp_map_sta
   if (!s) return VMP_EINVAL;
   vmp_session* one[1] = {s};
   return run_step(one, 1, kStepIeskf | kStepMap, ost, mst);
+}
+
+// vmp_register_scan with the context's workspace and ring scratch swapped for
+// the ones carved from the device scratch, the filter stage launched kernel
+// by kernel (instantiating a graph allocates), and every size checked
+// against the scratch before anything is launched.
+extern "C" int vmp_register_scan_noalloc(vmp_session* s, vmp_obs_stats* ost, vmp_map_stats* mst) {
+  if (!s) return VMP_EINVAL;
+  vmp_ctx* c = s->ctx;
+  if (!c->na_ready) {
+    s->err = "no-allocation scratch unavailable on this device (set-up at context creation failed)";
+    return VMP_EDEVICE;
+  }
+  const int64_t np = s->raw_pending ? int64_t(s->raw_n) : int64_t(s->n);
+  if (np > kNoAllocMaxPoints) {
+    s->err = "scan larger than the no-allocation limit of 131072 points";
+    return VMP_EINVAL;
+  }
+  if (s->map.qcap > c->rt_na.cap) {
+    s->err = "LRU ring (map_capacity, max_scan_points) larger than the no-allocation scratch holds";
+    return VMP_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(c->mu);
+  int rc = settle(c);  // a submitted batch completes first (on the regular workspace)
+  if (rc != VMP_OK) return rc;
+  struct Swap {
+    vmp_ctx* c;
+    bool graph;
+    explicit Swap(vmp_ctx* cc) : c(cc), graph(cc->use_graph) {
+      std::swap(c->ws, c->ws_na);
+      std::swap(c->rt, c->rt_na);
+      c->use_graph = false;
+    }
+    ~Swap() {
+      std::swap(c->ws, c->ws_na);
+      std::swap(c->rt, c->rt_na);
+      c->use_graph = graph;
+    }
+  } swap(c);
+  vmp_session* one[1] = {s};
+  rc = step_begin(c, one, 1, kStepIeskf | kStepMap);
+  if (rc != VMP_OK) return rc;
+  return step_finish(c, ost, mst);
 }
 
 extern "C" int vmp_register_batch_submit(vmp_session** ss, int32_t n) {

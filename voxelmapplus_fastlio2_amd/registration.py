@@ -360,6 +360,14 @@ class Registrar:
         self._check(self.lib.vmp_register_scan(self.h, C.byref(o), C.byref(m)), "vmp_register_scan")
         return o, m
 
+    def register_noalloc(self):
+        """register() through the entry point that allocates nothing (device
+        scratch of the library; scans up to 131072 points)."""
+        o = _lib.ObsStats()
+        m = _lib.MapStats()
+        self._check(self.lib.vmp_register_scan_noalloc(self.h, C.byref(o), C.byref(m)), "vmp_register_scan_noalloc")
+        return o, m
+
     def timing(self):
         a = C.c_double()
         b = C.c_double()
