@@ -265,6 +265,11 @@ int vmp_ctx_reserve(vmp_ctx* ctx, int64_t points, int32_t sessions);
  *              iterations' observe/reduce/step launches, end) replays a
  *              captured hipGraph (the default), 0 = launched one by one
  *              (also while per-launch timing is on).
+ *   "map_graph" 1 = the map stage of a step replays a captured hipGraph
+ *              once the step's shape (points, sessions, hash bits, build or
+ *              not) recurs on consecutive steps (the default), 0 = always
+ *              launched one by one (also while timing is on or when
+ *              "filter_graph" is 0).
  *   "obs_arith" arithmetic of the observe pass: 0 = compensated fp32 in a
  *              frame anchored at the scan's pose voxel (the default),
  *              1 = plain fp32 world frame, a precision-study arm whose
@@ -279,6 +284,10 @@ int vmp_ctx_reserve(vmp_ctx* ctx, int64_t points, int32_t sessions);
  *              pass's counters; the timing reports the mean (default 1).
  * Applies to later calls. */
 int vmp_ctx_set_param(vmp_ctx* ctx, const char* name, int64_t value);
+/* Graph use of a context so far (no reference counterpart): out[0] filter
+ * stage replays, out[1] filter graphs captured, out[2] map stage replays,
+ * out[3] map graphs captured. */
+int vmp_ctx_graph_stats(vmp_ctx* ctx, int64_t* out);
 
 /* Autotuner of the observe launch shape (no reference counterpart): times
  * the observe kernel on the session's current scan and state for 1, 2, 4

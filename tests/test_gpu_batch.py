@@ -100,6 +100,58 @@ def test_filter_graph_matches_eager():
     ctx_e.close()
 
 
+def test_map_graph_matches_eager():
+    """The map stage replayed from its captured hipGraph (captured once a
+    step's shape recurs: scans of a fixed size) gives bitwise the results of
+    launching its kernels one by one, scan after scan, with the program's own
+    eager work (motion prior, point upload, state and map reads) between the
+    replays.  The 36k-voxel LRU fills after a few scans, so the replays cover
+    eviction too."""
+    nscans = 10
+    n = 20000
+    ctx_g = Context(0)
+    ctx_e = Context(0)
+    ctx_e.set_param("map_graph", 0)
+    seq = Sequence(STREET, SEED0 + 71, n)
+    cfg = LIOConfig(max_scan_points=n, map_capacity=36000)
+    runs = []
+    for ctx in (ctx_g, ctx_e):
+        g = Registrar(ctx, cfg)
+        R, t = seq.pose(0)
+        g.set_state(state_vector(R, t), initial_covariance())
+        g.set_points(*seq.scan(0))
+        g.build()
+        runs.append(g)
+    ga, ge = runs
+    evicted = 0
+    for k in range(1, nscans):
+        x, y, z = seq.scan(k)
+        res = []
+        for g in runs:
+            g.apply_motion(*seq.motion(k), QDIAG)
+            g.set_points(x, y, z)
+            res.append(g.register())
+        (oa, ma), (ob, mb) = res
+        for f, _ in ma._fields_:
+            assert getattr(ma, f) == getattr(mb, f), f"scan {k}: map stat {f} differs graph vs eager"
+        evicted += ma.evicted
+        sa, Pa = ga.get_state()
+        sb, Pb = ge.get_state()
+        assert np.array_equal(sa, sb) and np.array_equal(Pa, Pb), f"scan {k}: posterior differs graph vs eager"
+        ea, eb = ga.export_map(), ge.export_map()
+        for key in ("keys", "ints", "group", "dbl"):
+            assert np.array_equal(ea[key], eb[key]), f"scan {k}: map '{key}' differs graph vs eager"
+        print(f"scan {k}: map stage {ctx_g.graph_stats()}, evicted {ma.evicted}: identical", flush=True)
+    st = ctx_g.graph_stats()
+    assert st["map_replays"] >= nscans - 3 and st["map_builds"] >= 1, st
+    assert ctx_e.graph_stats()["map_replays"] == 0
+    assert evicted > 0
+    for g in runs:
+        g.close()
+    ctx_g.close()
+    ctx_e.close()
+
+
 def test_repeated_observe_timing_leaves_results_unchanged():
     """Per-launch timing with obs_time_reps > 1 runs every observe launch
     several times back to back (the kernel is idempotent) and once more for

@@ -148,6 +148,7 @@ def vmp():
     _proto(lib, "vmp_session_timing", C.c_int, [P, dp, dp, C.POINTER(C.c_int32)])
     _proto(lib, "vmp_session_set_timing", C.c_int, [P, C.c_int])
     _proto(lib, "vmp_ctx_set_param", C.c_int, [P, C.c_char_p, C.c_int64])
+    _proto(lib, "vmp_ctx_graph_stats", C.c_int, [P, C.POINTER(C.c_int64)])
     _proto(lib, "vmp_ctx_reserve", C.c_int, [P, C.c_int64, C.c_int32])
     _proto(lib, "vmp_autotune_observe", C.c_int, [P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_float)])
     _proto(lib, "vmp_autotune_observe_batch", C.c_int,

@@ -220,6 +220,21 @@ struct vmp_ctx {
     uint64_t last_use = 0;
   };
   FilterGraph graphs[8];
+  // the map stage of a step captured as one hipGraph per shape (points,
+  // sessions, hash / session / time / length bits, build flag), once a shape
+  // recurs on consecutive steps (scans of a fixed size); a shape whose
+  // capture holds anything but kernel nodes runs eagerly from then on
+  bool use_map_graph = true;
+  struct MapGraph {
+    hipGraphExec_t exec = nullptr;
+    int64_t key[3] = {-1, -1, -1};
+    uint64_t ws_gen = 0;
+    uint64_t last_use = 0;
+  };
+  MapGraph mgraphs[4];
+  int64_t mkey_prev[3] = {-1, -1, -1};
+  int64_t mkey_bad[3] = {-1, -1, -1};
+  int64_t map_graph_launches = 0, map_graph_builds = 0;
   uint64_t graph_clock = 0;
   uint64_t ws_gen = 1;  // bumped when the batch workspace is reallocated
   int64_t graph_launches = 0, graph_builds = 0;
@@ -362,6 +377,8 @@ extern "C" int vmp_ctx_destroy(vmp_ctx* ctx) {
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   for (auto& g : ctx->graphs)
+    if (g.exec) hipGraphExecDestroy(g.exec);
+  for (auto& g : ctx->mgraphs)
     if (g.exec) hipGraphExecDestroy(g.exec);
   ctx->ws.release();
   if (ctx->ws.motion_done) hipEventDestroy(ctx->ws.motion_done);
@@ -1319,6 +1336,64 @@ static hipGraphExec_t filter_graph(vmp_ctx* c, int n, int nblocks, int iters, bo
   return hit->exec;
 }
 
+// Instantiated map-stage graph for this step's shape, or nullptr (then the
+// caller launches the stage itself).  A shape is captured the second time in
+// a row it occurs, so steps whose sizes change every time (batches of uneven
+// scans) never pay for a capture.  Only graphs made of kernel nodes are
+// kept: a memset node (which a library routine may record) has given wrong
+// results on replays here.
+template <class Stage>
+static hipGraphExec_t map_graph(vmp_ctx* c, const int64_t key[3], Stage&& stage) {
+  auto same = [&](const int64_t* a) { return a[0] == key[0] && a[1] == key[1] && a[2] == key[2]; };
+  vmp_ctx::MapGraph* victim = &c->mgraphs[0];
+  for (auto& g : c->mgraphs) {
+    if (g.exec && g.ws_gen == c->ws_gen && same(g.key)) {
+      g.last_use = ++c->graph_clock;
+      return g.exec;
+    }
+    if (!g.exec || g.last_use < victim->last_use) victim = &g;
+  }
+  if (same(c->mkey_bad)) return nullptr;
+  if (!same(c->mkey_prev)) {
+    for (int k = 0; k < 3; ++k) c->mkey_prev[k] = key[k];
+    return nullptr;
+  }
+  hipStream_t st = c->stream;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+  const int saved_kev = c->kev_used;
+  const int rc = stage();
+  c->kev_used = saved_kev;
+  hipGraph_t graph = nullptr;
+  const hipError_t ce = hipStreamEndCapture(st, &graph);
+  bool ok = rc == VMP_OK && ce == hipSuccess && graph;
+  if (ok) {
+    size_t nn = 0;
+    ok = hipGraphGetNodes(graph, nullptr, &nn) == hipSuccess && nn > 0;
+    std::vector<hipGraphNode_t> nodes(ok ? nn : 0);
+    if (ok) ok = hipGraphGetNodes(graph, nodes.data(), &nn) == hipSuccess;
+    for (size_t i = 0; ok && i < nn; ++i) {
+      hipGraphNodeType t;
+      ok = hipGraphNodeGetType(nodes[i], &t) == hipSuccess && t == hipGraphNodeTypeKernel;
+    }
+  }
+  hipGraphExec_t exec = nullptr;
+  if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess && exec;
+  if (graph) hipGraphDestroy(graph);
+  hipGetLastError();  // a failed capture leaves nothing behind for the eager launches
+  if (!ok) {  // this shape runs eagerly from now on
+    for (int k = 0; k < 3; ++k) c->mkey_bad[k] = key[k];
+    return nullptr;
+  }
+  if (victim->exec) hipGraphExecDestroy(victim->exec);
+  *victim = vmp_ctx::MapGraph{};
+  victim->exec = exec;
+  for (int k = 0; k < 3; ++k) victim->key[k] = key[k];
+  victim->ws_gen = c->ws_gen;
+  victim->last_use = ++c->graph_clock;
+  c->map_graph_builds++;
+  return exec;
+}
+
 static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
   if (n > kMaxBatchSessions) {
     ss[0]->err = "more than 32768 sessions in one batch";
@@ -1431,86 +1506,102 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
     for (int i = 0; i < n; ++i)
       while ((int64_t(1) << tbits) < int64_t(ss[i]->n)) ++tbits;
     b.cand_sshift = 25 + tbits;
-    hipLaunchKernelGGL(k_begin_scan, dim3(n), dim3(1), 0, st, M, n);
-    if (ntot > 0) {
-      size_t tb = w.temp_bytes;
-      timed_launch(c, kTkInsKeys, [&] { hipLaunchKernelGGL(k_ins_keys, dim3(gn), dim3(T), 0, st, M, S, n, b); });
-      // one stable radix sort over (session, hash slot): every key's points
-      // become a contiguous segment in point order, sessions stay in order
-      if (rocprim::radix_sort_pairs(w.temp, tb, b.skey_in, b.skey_out, b.sval_in, b.sval_out, size_t(ntot), 0,
-                                    hbits + sbits, st) != hipSuccess)
-        return VMP_EDEVICE;
-      hipLaunchKernelGGL(k_seg_heads, dim3(gn), dim3(T), 0, st, S, n, b);
-      tb = w.temp_bytes;
-      if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt + 0, size_t(ntot),
-                          st) != hipSuccess)
-        return VMP_EDEVICE;
-      hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 0, 0);
-      hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.flags, 0, ntot);
-      hipLaunchKernelGGL(k_seg_info, dim3(gn), dim3(T), 0, st, M, S, n, b);
-      tb = w.temp_bytes;
-      if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt + 1, size_t(ntot),
-                          st) != hipSuccess)
-        return VMP_EDEVICE;
-      hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 1, 1);
-      hipLaunchKernelGGL(k_cre_split, dim3(gn), dim3(T), 0, st, S, n, b);
-      hipLaunchKernelGGL(k_create_fast, dim3(gn), dim3(T), 0, st, M, S, n, b);
-      hipLaunchKernelGGL(k_create_fast_ctr, dim3(n), dim3(1), 0, st, M, n);
-      timed_launch(c, kTkRingLive,
-                   [&] { hipLaunchKernelGGL(k_ring_live, dim3(kLvBlocks, n), dim3(64 * kLvWaves), 0, st, M, S, n, b); });
-      hipLaunchKernelGGL(k_ring_pack, dim3(n), dim3(kLpThreads), 0, st, M, S, n, b);
-      timed_launch(c, kTkCreateEvict, [&] { hipLaunchKernelGGL(k_create_evict, dim3(n), dim3(128), 0, st, M, S, n, b); });
-      hipLaunchKernelGGL(k_create_init, dim3(gn), dim3(T), 0, st, M, S, n, b);
-      // replay order by segment length, capped at 10 bits (two radix passes;
-      // segments of 1022 points and more share the first key)
-      int lbits = 1;
-      while ((int64_t(1) << lbits) <= int64_t(ntot) && lbits < 10) ++lbits;
-      hipLaunchKernelGGL(k_seg_lenkeys, dim3(gn), dim3(T), 0, st, M, S, b, lbits);
-      tb = w.temp_bytes;
-      if (rocprim::radix_sort_pairs(w.temp, tb, b.len_in, b.len_out, b.ord_in, b.ord, size_t(ntot), 0, lbits + 1, st) !=
-          hipSuccess)
-        return VMP_EDEVICE;
-      // the moment path runs one wave per SIMD: a grid of four waves per CU
-      // strides over the refitting voxels; the light path covers the rest
-      timed_launch(c, kTkReplayHeavy,
-                   [&] { hipLaunchKernelGGL(k_replay_heavy, dim3(std::min(gn, 256)), dim3(T), 0, st, M, S, b); });
-      timed_launch(c, kTkReplayLight, [&] { hipLaunchKernelGGL(k_replay_light, dim3(gn), dim3(T), 0, st, M, S, b); });
-      if (!(mode & kStepBuild)) {
-        hipLaunchKernelGGL(k_fill_u64, dim3(gn), dim3(T), 0, st, b.cand_in, ~0ull, ntot);
-        timed_launch(c, kTkCandidates, [&] { hipLaunchKernelGGL(k_candidates, dim3(gn), dim3(T), 0, st, M, S, b); });
-        hipLaunchKernelGGL(k_dirty_untouched, dim3(gn), dim3(T), 0, st, M, b);
+    // the stage's launches; `lbits` etc. depend only on the step's shape
+    int lbits = 1;  // replay order by segment length, capped at 10 bits
+    while ((int64_t(1) << lbits) <= int64_t(ntot) && lbits < 10) ++lbits;
+    auto map_stage = [&]() -> int {
+      hipLaunchKernelGGL(k_begin_scan, dim3(n), dim3(1), 0, st, M, n);
+      if (ntot > 0) {
+        size_t tb = w.temp_bytes;
+        timed_launch(c, kTkInsKeys, [&] { hipLaunchKernelGGL(k_ins_keys, dim3(gn), dim3(T), 0, st, M, S, n, b); });
+        // one stable radix sort over (session, hash slot): every key's points
+        // become a contiguous segment in point order, sessions stay in order
+        if (rocprim::radix_sort_pairs(w.temp, tb, b.skey_in, b.skey_out, b.sval_in, b.sval_out, size_t(ntot), 0,
+                                      hbits + sbits, st) != hipSuccess)
+          return VMP_EDEVICE;
+        hipLaunchKernelGGL(k_seg_heads, dim3(gn), dim3(T), 0, st, S, n, b);
         tb = w.temp_bytes;
-        // sorted on (session, point time) only: the call at point time t is
-        // made by the voxel that received point t, so t fixes the voxel bits;
-        // unused slots hold ~0 and sort last
-        if (rocprim::radix_sort_keys(w.temp, tb, b.cand_in, b.cand_out, size_t(ntot), 25, b.cand_sshift + sbits, st) !=
+        if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt + 0, size_t(ntot),
+                            st) != hipSuccess)
+          return VMP_EDEVICE;
+        hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 0, 0);
+        hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.flags, 0, ntot);
+        hipLaunchKernelGGL(k_seg_info, dim3(gn), dim3(T), 0, st, M, S, n, b);
+        tb = w.temp_bytes;
+        if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt + 1, size_t(ntot),
+                            st) != hipSuccess)
+          return VMP_EDEVICE;
+        hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 1, 1);
+        hipLaunchKernelGGL(k_cre_split, dim3(gn), dim3(T), 0, st, S, n, b);
+        hipLaunchKernelGGL(k_create_fast, dim3(gn), dim3(T), 0, st, M, S, n, b);
+        hipLaunchKernelGGL(k_create_fast_ctr, dim3(n), dim3(1), 0, st, M, n);
+        timed_launch(c, kTkRingLive,
+                     [&] { hipLaunchKernelGGL(k_ring_live, dim3(kLvBlocks, n), dim3(64 * kLvWaves), 0, st, M, S, n, b); });
+        hipLaunchKernelGGL(k_ring_pack, dim3(n), dim3(kLpThreads), 0, st, M, S, n, b);
+        timed_launch(c, kTkCreateEvict, [&] { hipLaunchKernelGGL(k_create_evict, dim3(n), dim3(128), 0, st, M, S, n, b); });
+        hipLaunchKernelGGL(k_create_init, dim3(gn), dim3(T), 0, st, M, S, n, b);
+        // replay order by segment length, capped at 10 bits (two radix passes;
+        // segments of 1022 points and more share the first key)
+        hipLaunchKernelGGL(k_seg_lenkeys, dim3(gn), dim3(T), 0, st, M, S, b, lbits);
+        tb = w.temp_bytes;
+        if (rocprim::radix_sort_pairs(w.temp, tb, b.len_in, b.len_out, b.ord_in, b.ord, size_t(ntot), 0, lbits + 1, st) !=
             hipSuccess)
           return VMP_EDEVICE;
-        hipLaunchKernelGGL(k_cand_starts, dim3(1), dim3(kCsThreads), 0, st, M, n, b);
-        hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.cand_spec, 0, ntot);
-        timed_launch(c, kTkSpecEval, [&] { hipLaunchKernelGGL(k_spec_eval, dim3(gn), dim3(T), 0, st, M, b); });
+        // the moment path runs one wave per SIMD: a grid of four waves per CU
+        // strides over the refitting voxels; the light path covers the rest
+        timed_launch(c, kTkReplayHeavy,
+                     [&] { hipLaunchKernelGGL(k_replay_heavy, dim3(std::min(gn, 256)), dim3(T), 0, st, M, S, b); });
+        timed_launch(c, kTkReplayLight, [&] { hipLaunchKernelGGL(k_replay_light, dim3(gn), dim3(T), 0, st, M, S, b); });
+        if (!(mode & kStepBuild)) {
+          hipLaunchKernelGGL(k_fill_u64, dim3(gn), dim3(T), 0, st, b.cand_in, ~0ull, ntot);
+          timed_launch(c, kTkCandidates, [&] { hipLaunchKernelGGL(k_candidates, dim3(gn), dim3(T), 0, st, M, S, b); });
+          hipLaunchKernelGGL(k_dirty_untouched, dim3(gn), dim3(T), 0, st, M, b);
+          tb = w.temp_bytes;
+          // sorted on (session, point time) only: the call at point time t is
+          // made by the voxel that received point t, so t fixes the voxel bits;
+          // unused slots hold ~0 and sort last
+          if (rocprim::radix_sort_keys(w.temp, tb, b.cand_in, b.cand_out, size_t(ntot), 25, b.cand_sshift + sbits, st) !=
+              hipSuccess)
+            return VMP_EDEVICE;
+          hipLaunchKernelGGL(k_cand_starts, dim3(1), dim3(kCsThreads), 0, st, M, n, b);
+          hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.cand_spec, 0, ntot);
+          timed_launch(c, kTkSpecEval, [&] { hipLaunchKernelGGL(k_spec_eval, dim3(gn), dim3(T), 0, st, M, b); });
+          tb = w.temp_bytes;
+          if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.cand_spec, b.spl, b.cnt + 4,
+                              size_t(ntot), st) != hipSuccess)
+            return VMP_EDEVICE;
+          hipLaunchKernelGGL(k_spl_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b);
+          hipLaunchKernelGGL(k_spl_keys, dim3(gn), dim3(T), 0, st, b);
+          hipLaunchKernelGGL(k_spl_prefetch, dim3(std::min(gn * kPreItems, 4096)), dim3(T), 0, st, M, S, b);
+          timed_launch(c, kTkMergeReplay,
+                       [&] { hipLaunchKernelGGL(k_merge_replay, dim3(n), dim3(kMrThreads), 0, st, M, S, n, b); });
+        }
+        hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.flags, 0, ntot);
+        hipLaunchKernelGGL(k_finalize_segs, dim3(gn), dim3(T), 0, st, M, S, b);
+        hipLaunchKernelGGL(k_evict_finalize, dim3(gn), dim3(T), 0, st, M, S, n, b);
+        hipLaunchKernelGGL(k_evict_finalize_ctr, dim3(n), dim3(1), 0, st, M, n);
         tb = w.temp_bytes;
-        if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.cand_spec, b.spl, b.cnt + 4,
-                            size_t(ntot), st) != hipSuccess)
+        if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt + 2, size_t(ntot),
+                            st) != hipSuccess)
           return VMP_EDEVICE;
-        hipLaunchKernelGGL(k_spl_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b);
-        hipLaunchKernelGGL(k_spl_keys, dim3(gn), dim3(T), 0, st, b);
-        hipLaunchKernelGGL(k_spl_prefetch, dim3(std::min(gn * kPreItems, 4096)), dim3(T), 0, st, M, S, b);
-        timed_launch(c, kTkMergeReplay,
-                     [&] { hipLaunchKernelGGL(k_merge_replay, dim3(n), dim3(kMrThreads), 0, st, M, S, n, b); });
+        hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 2, 2);
+        hipLaunchKernelGGL(k_lru_append, dim3(gn), dim3(T), 0, st, M, S, n, b);
       }
-      hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.flags, 0, ntot);
-      hipLaunchKernelGGL(k_finalize_segs, dim3(gn), dim3(T), 0, st, M, S, b);
-      hipLaunchKernelGGL(k_evict_finalize, dim3(gn), dim3(T), 0, st, M, S, n, b);
-      hipLaunchKernelGGL(k_evict_finalize_ctr, dim3(n), dim3(1), 0, st, M, n);
-      tb = w.temp_bytes;
-      if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt + 2, size_t(ntot),
-                          st) != hipSuccess)
-        return VMP_EDEVICE;
-      hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 2, 2);
-      hipLaunchKernelGGL(k_lru_append, dim3(gn), dim3(T), 0, st, M, S, n, b);
+      hipLaunchKernelGGL(k_end_scan, dim3(n), dim3(1), 0, st, M, S, n);
+  
+      return VMP_OK;
+    };
+    const int64_t key[3] = {int64_t(ntot), int64_t(n),
+                            int64_t(hbits) | (int64_t(sbits) << 8) | (int64_t(tbits) << 16) |
+                                (int64_t(lbits) << 24) | (int64_t(mode & kStepBuild) << 32)};
+    hipGraphExec_t mexec = (c->use_graph && c->use_map_graph && !c->timing) ? map_graph(c, key, map_stage) : nullptr;
+    if (mexec) {
+      if (hipGraphLaunch(mexec, st) != hipSuccess) return VMP_EDEVICE;
+      c->map_graph_launches++;
+    } else {
+      const int rc = map_stage();
+      if (rc != VMP_OK) return rc;
     }
-    hipLaunchKernelGGL(k_end_scan, dim3(n), dim3(1), 0, st, M, S, n);
   }
   hipEventRecord(c->ev[3], st);
   hipLaunchKernelGGL(k_gather, dim3(n), dim3(64), 0, st, w.d_maps, w.d_flist, w.d_obsstat, w.d_out, n);
@@ -2222,6 +2313,11 @@ extern "C" int vmp_ctx_set_param(vmp_ctx* c, const char* name, int64_t value) {
     c->use_graph = value != 0;
     return VMP_OK;
   }
+  if (std::strcmp(name, "map_graph") == 0) {
+    if (value < 0 || value > 1) return VMP_EINVAL;
+    c->use_map_graph = value != 0;
+    return VMP_OK;
+  }
   if (std::strcmp(name, "obs_arith") == 0) {
     if (value < 0 || value > 1) return VMP_EINVAL;
     c->obs_arith = int(value);
@@ -2238,6 +2334,16 @@ extern "C" int vmp_ctx_set_param(vmp_ctx* c, const char* name, int64_t value) {
     return VMP_OK;
   }
   return VMP_EINVAL;
+}
+
+extern "C" int vmp_ctx_graph_stats(vmp_ctx* c, int64_t* out) {
+  if (!c || !out) return VMP_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  out[0] = c->graph_launches;
+  out[1] = c->graph_builds;
+  out[2] = c->map_graph_launches;
+  out[3] = c->map_graph_builds;
+  return VMP_OK;
 }
 
 extern "C" int vmp_session_set_timing(vmp_session* s, int on) {

@@ -209,7 +209,7 @@ __device__ inline int g_group(int r) {
 }
 
 struct PredShared {
-  double P[2][529];         // row-major, the current one and the next
+  double P[529];            // row-major
   double FP[kFpRows * 23];  // rows fp_row(0..10) of F P
   double E[kPredChunk][9];  // per step of the chunk: exp((gyro - bg) dt), row-major
   double J[kPredChunk][9];  // -Jr((gyro - bg) dt) dt (F and G)
@@ -224,97 +224,136 @@ struct PredShared {
   vmp_state sx;
 };
 
-// phase D for step j of the chunk: S.P[cur] -> S.P[cur ^ 1]
-__device__ inline void predict_cov_step(PredShared& S, int j, int cur, int lane) {
-  const double* P = S.P[cur];
-  double* Pn = S.P[cur ^ 1];
+// the columns (and rows) of F that differ from the identity's
+__device__ constexpr int kFK[17] = {0, 1, 2, 3, 4, 5, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22};
+
+// phase D for step j of the chunk, P in place.  F P: lane c holds column c
+// of P (its 17 entries in the non-identity rows) and forms column c of the 11
+// non-identity rows of F P.  (F P) F^T + G Q G^T: lane r holds row r of F P
+// (a row of P for an identity row of F) and rewrites row r of P: its
+// non-identity columns from the sums, its identity columns from F P (for a
+// non-identity row), and the diagonal of G Q G^T's identity blocks.  Every
+// lane follows the same sequence of operations on its own column or row.
+__device__ inline void predict_cov_step(PredShared& S, int j, int lane) {
+  double* P = S.P;
   const double dt = S.dt[j];
   const double* E = S.E[j];
   const double* Jd = S.J[j];
   const double* Ad = S.A[j];
   const double* Rd = S.Rd[j];
-  for (int o = lane; o < kFpRows * 23; o += 64) {  // F P, the non-identity rows
-    const int slot = o / 23, c = o % 23, r = fp_row(slot);
-    double s;
-    if (r < 3) {
-      s = P[r * 23 + c];
-      s += dt * P[(12 + r) * 23 + c];
-    } else if (r < 6) {
-      const int i = r - 3;  // exp(-w) row i = exp(w) column i
-      s = E[i] * P[3 * 23 + c];
-      s += E[3 + i] * P[4 * 23 + c];
-      s += E[6 + i] * P[5 * 23 + c];
-      s += Jd[i * 3] * P[15 * 23 + c];
-      s += Jd[i * 3 + 1] * P[16 * 23 + c];
-      s += Jd[i * 3 + 2] * P[17 * 23 + c];
-    } else if (r < 15) {
-      const int i = r - 12;
-      s = Ad[i * 3] * P[3 * 23 + c];
-      s += Ad[i * 3 + 1] * P[4 * 23 + c];
-      s += Ad[i * 3 + 2] * P[5 * 23 + c];
-      s += P[r * 23 + c];
-      s += Rd[i * 3] * P[18 * 23 + c];
-      s += Rd[i * 3 + 1] * P[19 * 23 + c];
-      s += Rd[i * 3 + 2] * P[20 * 23 + c];
-      s += (S.Mx[i * 2] * dt) * P[21 * 23 + c];
-      s += (S.Mx[i * 2 + 1] * dt) * P[22 * 23 + c];
-    } else {
-      const int i = r - 21;
-      s = S.NM[i * 2] * P[21 * 23 + c];
-      s += S.NM[i * 2 + 1] * P[22 * 23 + c];
+  const double md[6] = {S.Mx[0] * dt, S.Mx[1] * dt, S.Mx[2] * dt, S.Mx[3] * dt, S.Mx[4] * dt, S.Mx[5] * dt};
+  if (lane < 23) {  // F P
+    const int c = lane;
+    double p[23];
+#pragma unroll
+    for (int t = 0; t < 17; ++t) p[kFK[t]] = P[kFK[t] * 23 + c];
+    double* fp = S.FP;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      double s = p[r];
+      s += dt * p[12 + r];
+      fp[r * 23 + c] = s;
     }
-    S.FP[slot * 23 + c] = s;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {  // rows 3-5: exp(-w) row i = exp(w) column i
+      double s = E[i] * p[3];
+      s += E[3 + i] * p[4];
+      s += E[6 + i] * p[5];
+      s += Jd[i * 3] * p[15];
+      s += Jd[i * 3 + 1] * p[16];
+      s += Jd[i * 3 + 2] * p[17];
+      fp[(3 + i) * 23 + c] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {  // rows 12-14
+      double s = Ad[i * 3] * p[3];
+      s += Ad[i * 3 + 1] * p[4];
+      s += Ad[i * 3 + 2] * p[5];
+      s += p[12 + i];
+      s += Rd[i * 3] * p[18];
+      s += Rd[i * 3 + 1] * p[19];
+      s += Rd[i * 3 + 2] * p[20];
+      s += md[i * 2] * p[21];
+      s += md[i * 2 + 1] * p[22];
+      fp[(6 + i) * 23 + c] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {  // rows 21-22
+      double s = S.NM[i * 2] * p[21];
+      s += S.NM[i * 2 + 1] * p[22];
+      fp[(9 + i) * 23 + c] = s;
+    }
   }
   __syncthreads();
-  for (int o = lane; o < 529; o += 64) {  // (F P) F^T + G Q G^T, column by column
-    const int c = o / 23, r = o % 23;
+  if (lane < 23) {  // (F P) F^T + G Q G^T
+    const int r = lane;
     const int rs = fp_slot(r);
-    const double* fr = rs >= 0 ? &S.FP[rs * 23] : &P[r * 23];  // row r of F P
-    double s;
-    if (c < 3) {
-      s = fr[c];
-      s += fr[12 + c] * dt;
-    } else if (c < 6) {
-      const int i = c - 3;
-      s = fr[3] * E[i];
-      s += fr[4] * E[3 + i];
-      s += fr[5] * E[6 + i];
-      s += fr[15] * Jd[i * 3];
-      s += fr[16] * Jd[i * 3 + 1];
-      s += fr[17] * Jd[i * 3 + 2];
-    } else if (c >= 12 && c < 15) {
-      const int i = c - 12;
-      s = fr[3] * Ad[i * 3];
-      s += fr[4] * Ad[i * 3 + 1];
-      s += fr[5] * Ad[i * 3 + 2];
-      s += fr[c];
-      s += fr[18] * Rd[i * 3];
-      s += fr[19] * Rd[i * 3 + 1];
-      s += fr[20] * Rd[i * 3 + 2];
-      s += fr[21] * (S.Mx[i * 2] * dt);
-      s += fr[22] * (S.Mx[i * 2 + 1] * dt);
-    } else if (c >= 21) {
-      const int i = c - 21;
-      s = fr[21] * S.NM[i * 2];
-      s += fr[22] * S.NM[i * 2 + 1];
-    } else {
-      s = fr[c];
+    const double* fr = rs >= 0 ? &S.FP[rs * 23] : &P[r * 23];
+    double f[23];
+#pragma unroll
+    for (int t = 0; t < 17; ++t) f[kFK[t]] = fr[kFK[t]];
+    double o[23];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double s = f[c];
+      s += f[12 + c] * dt;
+      o[c] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double s = f[3] * E[i];
+      s += f[4] * E[3 + i];
+      s += f[5] * E[6 + i];
+      s += f[15] * Jd[i * 3];
+      s += f[16] * Jd[i * 3 + 1];
+      s += f[17] * Jd[i * 3 + 2];
+      o[3 + i] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double s = f[3] * Ad[i * 3];
+      s += f[4] * Ad[i * 3 + 1];
+      s += f[5] * Ad[i * 3 + 2];
+      s += f[12 + i];
+      s += f[18] * Rd[i * 3];
+      s += f[19] * Rd[i * 3 + 1];
+      s += f[20] * Rd[i * 3 + 2];
+      s += f[21] * md[i * 2];
+      s += f[22] * md[i * 2 + 1];
+      o[12 + i] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      double s = f[21] * S.NM[i * 2];
+      s += f[22] * S.NM[i * 2 + 1];
+      o[21 + i] = s;
     }
     const int gr = g_group(r);
-    double g = 0.0;
-    if (gr >= 0 && gr == g_group(c)) {
+    if (gr == 0 || gr == 1) {  // G Q G^T on the blocks 3-5 and 12-14
       const double q = S.q[gr];
-      if (gr >= 2) {
-        if (r == c) g = (dt * q) * dt;
-      } else {
-        const double* Gr = gr == 0 ? &Jd[(r - 3) * 3] : &Rd[(r - 12) * 3];
-        const double* Gc = gr == 0 ? &Jd[(c - 3) * 3] : &Rd[(c - 12) * 3];
-        g = (Gr[0] * q) * Gc[0];
-        g += (Gr[1] * q) * Gc[1];
-        g += (Gr[2] * q) * Gc[2];
+      const double* G = gr == 0 ? Jd : Rd;
+      const double* Gr = &G[(r - (gr == 0 ? 3 : 12)) * 3];
+      const int c0 = gr == 0 ? 3 : 12;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        double g = (Gr[0] * q) * G[i * 3];
+        g += (Gr[1] * q) * G[i * 3 + 1];
+        g += (Gr[2] * q) * G[i * 3 + 2];
+        o[c0 + i] = o[c0 + i] + g;
       }
     }
-    Pn[r * 23 + c] = s + g;
+    double* pr = &P[r * 23];
+#pragma unroll
+    for (int t = 0; t < 17; ++t)
+      if (kFK[t] < 6 || (kFK[t] >= 12 && kFK[t] < 15) || kFK[t] >= 21) pr[kFK[t]] = o[kFK[t]];
+    if (rs >= 0) {  // identity columns of a non-identity row: F P's entries
+#pragma unroll
+      for (int c = 6; c < 12; ++c) pr[c] = fr[c];
+#pragma unroll
+      for (int c = 15; c < 21; ++c) pr[c] = fr[c];
+    } else if (gr >= 2) {  // identity row and column: only G Q G^T's diagonal
+      pr[r] = pr[r] + (dt * S.q[gr]) * dt;
+    }
   }
 }
 
@@ -325,7 +364,7 @@ __global__ void __launch_bounds__(64) k_imu_predict(const PreJob* jobs, int nses
   FilterDev* f = Jb.f;
   const int lane = threadIdx.x;
   __shared__ PredShared S;
-  for (int i = lane; i < 529; i += 64) S.P[0][(i % 23) * 23 + i / 23] = f->P[i];  // column-major -> rows
+  for (int i = lane; i < 529; i += 64) S.P[(i % 23) * 23 + i / 23] = f->P[i];  // column-major -> rows
   load_state(f->x, S.sx, lane);
   if (lane < 4) S.q[lane] = im->q[lane];
   __syncthreads();
@@ -395,7 +434,6 @@ __global__ void __launch_bounds__(64) k_imu_predict(const PreJob* jobs, int nses
       dt = im->pkg.cloud_end - imu_end;
     }
   };
-  int cur = 0;
   int np = 1;  // pose table entries
   for (int c0 = 0; c0 < nsteps; c0 += kPredChunk) {
     const int cn = min(kPredChunk, nsteps - c0);
@@ -418,28 +456,49 @@ __global__ void __launch_bounds__(64) k_imu_predict(const PreJob* jobs, int nses
       S.tt[lane] = tt;
     }
     __syncthreads();
-    if (lane == 0) {  // B: x += delta step after step (the blocks come from the state before it)
+    {  // B: x += delta step after step, the rotation in lanes 0-8 (entry
+       // lane), velocity and position in lanes 0-2; the blocks of a step come
+       // from the state before it
       vmp_state& x = S.sx;
-      D33 rot = d33_from_cm(x.rot);
-      for (int j = 0; j < cn; ++j) {
-        for (int i = 0; i < 9; ++i) S.R[j][i] = rot.m[i];
-        const double dt = S.dt[j];
-        const double ab[3] = {S.acc[j][0] - x.ba[0], S.acc[j][1] - x.ba[1], S.acc[j][2] - x.ba[2]};
-        const D3 ra = d33_mulv(rot, D3{{ab[0], ab[1], ab[2]}});
-        for (int c = 0; c < 3; ++c) {
-          const double dvel = dt * (ra.v[c] + x.g[c]);
-          const double dpos = dt * x.vel[c];
-          x.pos[c] = x.pos[c] + dpos;
-          x.vel[c] = x.vel[c] + dvel;
-          S.vel[j + 1][c] = x.vel[c];
-          S.pos[j + 1][c] = x.pos[c];
-        }
-        D33 E;
-        for (int i = 0; i < 9; ++i) E.m[i] = S.E[j][i];
-        rot = d33_mul(rot, E);
+      double rv = 0.0, vel = 0.0, pos = 0.0;
+      if (lane < 9) rv = x.rot[(lane % 3) * 3 + lane / 3];  // column-major storage
+      if (lane < 3) {
+        vel = x.vel[lane];
+        pos = x.pos[lane];
       }
-      for (int i = 0; i < 9; ++i) S.R[cn][i] = rot.m[i];
-      d33_to_cm(rot, x.rot);
+      for (int j = 0; j < cn; ++j) {
+        if (lane < 9) S.R[j][lane] = rv;
+        __syncthreads();
+        const double* R = S.R[j];
+        const double* E = S.E[j];
+        if (lane < 3) {
+          const double dt = S.dt[j];
+          const double ab0 = S.acc[j][0] - x.ba[0], ab1 = S.acc[j][1] - x.ba[1], ab2 = S.acc[j][2] - x.ba[2];
+          double ra = R[lane * 3] * ab0;
+          ra += R[lane * 3 + 1] * ab1;
+          ra += R[lane * 3 + 2] * ab2;
+          const double dvel = dt * (ra + x.g[lane]);
+          const double dpos = dt * vel;
+          pos = pos + dpos;
+          vel = vel + dvel;
+          S.vel[j + 1][lane] = vel;
+          S.pos[j + 1][lane] = pos;
+        }
+        if (lane < 9) {
+          const int r = lane / 3, c = lane % 3;
+          double v = R[r * 3] * E[c];
+          v += R[r * 3 + 1] * E[3 + c];
+          v += R[r * 3 + 2] * E[6 + c];
+          rv = v;
+        }
+      }
+      if (lane < 9) S.R[cn][lane] = rv;
+      __syncthreads();
+      if (lane < 9) x.rot[(lane % 3) * 3 + lane / 3] = rv;
+      if (lane < 3) {
+        x.vel[lane] = vel;
+        x.pos[lane] = pos;
+      }
     }
     __syncthreads();
     if (lane < cn) {  // C
@@ -482,13 +541,12 @@ __global__ void __launch_bounds__(64) k_imu_predict(const PreJob* jobs, int nses
     }
     for (int j = 0; j < cn; ++j) {  // D
       __syncthreads();
-      predict_cov_step(S, j, cur, lane);
-      cur ^= 1;
+      predict_cov_step(S, j, lane);
     }
     np += (c0 + cn == nsteps) ? cn - 1 : cn;
     __syncthreads();
   }
-  for (int i = lane; i < 529; i += 64) f->P[i] = S.P[cur][(i % 23) * 23 + i / 23];
+  for (int i = lane; i < 529; i += 64) f->P[i] = S.P[(i % 23) * 23 + i / 23];
   if (lane == 0) {
     f->x = S.sx;
     if (m > 0) {

@@ -165,6 +165,14 @@ class Context:
         if rc != 0:
             raise VmpError(f"vmp_ctx_set_param({name}, {value}) failed ({rc})")
 
+    def graph_stats(self):
+        """{filter_replays, filter_builds, map_replays, map_builds} so far."""
+        out = (C.c_int64 * 4)()
+        rc = self.lib.vmp_ctx_graph_stats(self.h, out)
+        if rc != 0:
+            raise VmpError(f"vmp_ctx_graph_stats failed ({rc})")
+        return dict(zip(("filter_replays", "filter_builds", "map_replays", "map_builds"), list(out)))
+
     def reserve(self, points, sessions):
         """Size the batch workspace for batches of up to `points` points over
         up to `sessions` sessions (no later batch then allocates)."""
