@@ -608,6 +608,7 @@ def main():
     iters = 0
     mstat = {"created": 0, "evicted": 0, "merges": 0, "merge_evals": 0, "refits": 0, "touched": 0}
     torch.cuda.synchronize()
+    gs0 = ctx.graph_stats()
     t_start = time.perf_counter()
     for k in range(2 + F + W, 2 + F + W + K):
         ts = time.perf_counter()
@@ -620,6 +621,8 @@ def main():
             mstat[f] += getattr(m, f)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
+    gs1 = ctx.graph_stats()
+    graphs = {k: gs1[k] - gs0[k] for k in gs1}
     # roofline pass: the sequence continues for RF scans with per-launch HIP
     # events on the library's stream (the filter stage then launches kernel
     # by kernel instead of replaying its graph; the kernels are the same)
@@ -680,6 +683,8 @@ def main():
                                    "ring_compactions": sum(1 for f in maint if f & 2)},
             "observe_autotune": tune,
             "filter_stage": "hipGraph replay (IESKF begin, 5 x observe/step, end)",
+            "map_stage": "hipGraph replay once the scan shape recurs (one capture per shape)",
+            "graphs_in_timed_region": graphs,
         },
         "latency_ms": {
             "p50": float(np.percentile(lat_ms, 50)),

@@ -22,6 +22,7 @@
 #include "vmp_mapupd.hpp"
 #include "vmp_observe.hpp"
 #include "vmp_imu.hpp"
+#include "vmp_prim.hpp"
 
 using namespace vmp;
 
@@ -476,6 +477,7 @@ static bool ws_layout(vmp_ctx* c, BatchWs& w, size_t n, int ns, size_t pre_cap, 
   ok = ok && dev(&b.len_in, n) && dev(&b.len_out, n) && dev(&b.ord_in, n) && dev(&b.ord, n);
   ok = ok && dev(&b.lv, n * 2) && dev(&b.lv2, n * 2) && dev(&b.lv_cnt, (n * 2 >> 6) + size_t(ns) + 2) &&
        dev(&b.lv_n, size_t(ns));
+  ok = ok && dev(&b.prim, prim_ws_entries(int64_t(n)));
   // taint-phase prefetch of up to pre_cap acting candidates per batch (the
   // rest take the replay's own lookups)
   b.pre_cap = int32_t(std::min(n, pre_cap));
@@ -1507,30 +1509,42 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
       while ((int64_t(1) << tbits) < int64_t(ss[i]->n)) ++tbits;
     b.cand_sshift = 25 + tbits;
     // the stage's launches; `lbits` etc. depend only on the step's shape
-    int lbits = 1;  // replay order by segment length, capped at 10 bits
-    while ((int64_t(1) << lbits) <= int64_t(ntot) && lbits < 10) ++lbits;
+    int lbits = 1;  // replay order by segment length, capped at 6 bits
+    while ((int64_t(1) << lbits) <= int64_t(ntot) && lbits < 6) ++lbits;
+    // batches up to kPrimMaxItems points use this library's compaction and
+    // radix sort (kernels only, so the stage can be captured); larger ones
+    // rocPRIM's device-wide routines
+    const bool prim = ntot <= kPrimMaxItems;
+    auto select_flags = [&](int32_t* flags, int32_t* out, int32_t* count) -> bool {
+      if (prim) {
+        select_stable(st, ntot, PredFlag{flags}, OutIndex{out}, b.prim, count);
+        return true;
+      }
+      size_t tb = w.temp_bytes;
+      return rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), flags, out, count, size_t(ntot),
+                             st) == hipSuccess;
+    };
+    auto sort_pairs = [&](auto* kin, int32_t* vin, auto* kout, int32_t* vout, int bits) -> bool {
+      if (prim) {
+        radix_sort_stable(st, kin, vin, kout, vout, ntot, bits, b.prim);
+        return true;
+      }
+      size_t tb = w.temp_bytes;
+      return rocprim::radix_sort_pairs(w.temp, tb, kin, kout, vin, vout, size_t(ntot), 0, bits, st) == hipSuccess;
+    };
     auto map_stage = [&]() -> int {
       hipLaunchKernelGGL(k_begin_scan, dim3(n), dim3(1), 0, st, M, n);
       if (ntot > 0) {
-        size_t tb = w.temp_bytes;
         timed_launch(c, kTkInsKeys, [&] { hipLaunchKernelGGL(k_ins_keys, dim3(gn), dim3(T), 0, st, M, S, n, b); });
         // one stable radix sort over (session, hash slot): every key's points
         // become a contiguous segment in point order, sessions stay in order
-        if (rocprim::radix_sort_pairs(w.temp, tb, b.skey_in, b.skey_out, b.sval_in, b.sval_out, size_t(ntot), 0,
-                                      hbits + sbits, st) != hipSuccess)
-          return VMP_EDEVICE;
+        if (!sort_pairs(b.skey_in, b.sval_in, b.skey_out, b.sval_out, hbits + sbits)) return VMP_EDEVICE;
         hipLaunchKernelGGL(k_seg_heads, dim3(gn), dim3(T), 0, st, S, n, b);
-        tb = w.temp_bytes;
-        if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt + 0, size_t(ntot),
-                            st) != hipSuccess)
-          return VMP_EDEVICE;
+        if (!select_flags(b.flags, b.sel, b.cnt + 0)) return VMP_EDEVICE;
         hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 0, 0);
         hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.flags, 0, ntot);
         hipLaunchKernelGGL(k_seg_info, dim3(gn), dim3(T), 0, st, M, S, n, b);
-        tb = w.temp_bytes;
-        if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt + 1, size_t(ntot),
-                            st) != hipSuccess)
-          return VMP_EDEVICE;
+        if (!select_flags(b.flags, b.sel, b.cnt + 1)) return VMP_EDEVICE;
         hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 1, 1);
         hipLaunchKernelGGL(k_cre_split, dim3(gn), dim3(T), 0, st, S, n, b);
         hipLaunchKernelGGL(k_create_fast, dim3(gn), dim3(T), 0, st, M, S, n, b);
@@ -1540,13 +1554,11 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         hipLaunchKernelGGL(k_ring_pack, dim3(n), dim3(kLpThreads), 0, st, M, S, n, b);
         timed_launch(c, kTkCreateEvict, [&] { hipLaunchKernelGGL(k_create_evict, dim3(n), dim3(128), 0, st, M, S, n, b); });
         hipLaunchKernelGGL(k_create_init, dim3(gn), dim3(T), 0, st, M, S, n, b);
-        // replay order by segment length, capped at 10 bits (two radix passes;
-        // segments of 1022 points and more share the first key)
+        // replay order by segment length in 6 bits (one radix pass; segments
+        // of 62 points and more share the first key): the order only spreads
+        // the work evenly over waves, every segment replays on its own
         hipLaunchKernelGGL(k_seg_lenkeys, dim3(gn), dim3(T), 0, st, M, S, b, lbits);
-        tb = w.temp_bytes;
-        if (rocprim::radix_sort_pairs(w.temp, tb, b.len_in, b.len_out, b.ord_in, b.ord, size_t(ntot), 0, lbits + 1, st) !=
-            hipSuccess)
-          return VMP_EDEVICE;
+        if (!sort_pairs(b.len_in, b.ord_in, b.len_out, b.ord, lbits + 1)) return VMP_EDEVICE;
         // the moment path runs one wave per SIMD: a grid of four waves per CU
         // strides over the refitting voxels; the light path covers the rest
         timed_launch(c, kTkReplayHeavy,
@@ -1556,20 +1568,21 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
           hipLaunchKernelGGL(k_fill_u64, dim3(gn), dim3(T), 0, st, b.cand_in, ~0ull, ntot);
           timed_launch(c, kTkCandidates, [&] { hipLaunchKernelGGL(k_candidates, dim3(gn), dim3(T), 0, st, M, S, b); });
           hipLaunchKernelGGL(k_dirty_untouched, dim3(gn), dim3(T), 0, st, M, b);
-          tb = w.temp_bytes;
-          // sorted on (session, point time) only: the call at point time t is
-          // made by the voxel that received point t, so t fixes the voxel bits;
-          // unused slots hold ~0 and sort last
-          if (rocprim::radix_sort_keys(w.temp, tb, b.cand_in, b.cand_out, size_t(ntot), 25, b.cand_sshift + sbits, st) !=
-              hipSuccess)
-            return VMP_EDEVICE;
+          // the call at point time t is made by the voxel that received point
+          // t, so each candidate sits at its session offset plus t: the
+          // non-empty slots in order are the list sorted on (session, time)
+          if (prim) {
+            select_stable(st, ntot, PredCand{b.cand_in}, OutCand{b.cand_in, b.cand_out}, b.prim, b.cnt + 7);
+          } else {
+            size_t tb = w.temp_bytes;
+            if (rocprim::radix_sort_keys(w.temp, tb, b.cand_in, b.cand_out, size_t(ntot), 25, b.cand_sshift + sbits,
+                                         st) != hipSuccess)
+              return VMP_EDEVICE;
+          }
           hipLaunchKernelGGL(k_cand_starts, dim3(1), dim3(kCsThreads), 0, st, M, n, b);
           hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.cand_spec, 0, ntot);
           timed_launch(c, kTkSpecEval, [&] { hipLaunchKernelGGL(k_spec_eval, dim3(gn), dim3(T), 0, st, M, b); });
-          tb = w.temp_bytes;
-          if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.cand_spec, b.spl, b.cnt + 4,
-                              size_t(ntot), st) != hipSuccess)
-            return VMP_EDEVICE;
+          if (!select_flags(b.cand_spec, b.spl, b.cnt + 4)) return VMP_EDEVICE;
           hipLaunchKernelGGL(k_spl_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b);
           hipLaunchKernelGGL(k_spl_keys, dim3(gn), dim3(T), 0, st, b);
           hipLaunchKernelGGL(k_spl_prefetch, dim3(std::min(gn * kPreItems, 4096)), dim3(T), 0, st, M, S, b);
@@ -1580,10 +1593,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         hipLaunchKernelGGL(k_finalize_segs, dim3(gn), dim3(T), 0, st, M, S, b);
         hipLaunchKernelGGL(k_evict_finalize, dim3(gn), dim3(T), 0, st, M, S, n, b);
         hipLaunchKernelGGL(k_evict_finalize_ctr, dim3(n), dim3(1), 0, st, M, n);
-        tb = w.temp_bytes;
-        if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt + 2, size_t(ntot),
-                            st) != hipSuccess)
-          return VMP_EDEVICE;
+        if (!select_flags(b.flags, b.sel, b.cnt + 2)) return VMP_EDEVICE;
         hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 2, 2);
         hipLaunchKernelGGL(k_lru_append, dim3(gn), dim3(T), 0, st, M, S, n, b);
       }
@@ -1812,15 +1822,22 @@ static int prepare_scans(vmp_ctx* c, vmp_session** ss, int n, bool is_build) {
   while ((1 << sbits) < nf) ++sbits;
   size_t tb = w.temp_bytes;
   if (ntot > 0) {
-    if (rocprim::radix_sort_pairs(w.temp, tb, b.skey_in, b.skey_out, b.sval_in, b.sval_out, size_t(ntot), 0,
-                                  32 + sbits, st) != hipSuccess)
+    const bool prim = ntot <= kPrimMaxItems;
+    if (prim) {
+      radix_sort_stable(st, b.skey_in, b.sval_in, b.skey_out, b.sval_out, ntot, 32 + sbits, b.prim);
+    } else if (rocprim::radix_sort_pairs(w.temp, tb, b.skey_in, b.skey_out, b.sval_in, b.sval_out, size_t(ntot), 0,
+                                         32 + sbits, st) != hipSuccess) {
       return VMP_EDEVICE;
+    }
     const int gn = grid_for(ntot, 256);
     hipLaunchKernelGGL(k_vg_heads, dim3(gn), dim3(256), 0, st, b.skey_out, ntot, b.flags);
     tb = w.temp_bytes;
-    if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt + 6, size_t(ntot),
-                        st) != hipSuccess)
+    if (prim) {
+      select_stable(st, ntot, PredFlag{b.flags}, OutIndex{b.sel}, b.prim, b.cnt + 6);
+    } else if (rocprim::select(w.temp, tb, rocprim::counting_iterator<int32_t>(0), b.flags, b.sel, b.cnt + 6,
+                               size_t(ntot), st) != hipSuccess) {
       return VMP_EDEVICE;
+    }
     hipLaunchKernelGGL(k_vg_ranges, dim3((nf + 63) / 64), dim3(64), 0, st, w.d_vg, nf, b.sel, b.cnt + 6, w.d_vgst);
     hipLaunchKernelGGL(k_vg_centroid, dim3(gn), dim3(256), 0, st, w.d_vg, b.skey_out, b.sval_out, b.sel, b.cnt + 6,
                        ntot, w.d_vgst);

@@ -149,6 +149,7 @@ struct BatchBufs {
   int4* lv2;
   int32_t* lv_cnt;
   int32_t* lv_n;
+  int32_t* prim;       // work entries of the compaction / radix sort (prim_ws_entries)
 };
 __host__ __device__ inline int64_t lv_cnt_base(int off, int s) { return (int64_t(2) * off >> 6) + s; }
 

@@ -174,12 +174,14 @@ __global__ void k_candidates(const MapDev* maps, const SessInfo* si, BatchBufs b
       }
       st.dirty_t = dirty;
     }
-    // slots in the session's candidate list, one atomic per wave and session
-    const int slot = wave_fetch_add(&c->ncand, ne);
+    // the session's candidate count, one atomic per wave and session; each
+    // candidate goes to its session's offset plus its point time (a call
+    // time belongs to one voxel), so the filled slots are in sorted order
+    wave_fetch_add(&c->ncand, ne);
     if (ne > 0) {
       const int off = si[s].off, ns_pts = si[s].n;
       for (int k = 0; k < ne; ++k) {
-        if (slot + k < ns_pts) b.cand_in[off + slot + k] = cand_key_batch(s, emitted[k], vid, b.cand_sshift);
+        if (emitted[k] < ns_pts) b.cand_in[off + emitted[k]] = cand_key_batch(s, emitted[k], vid, b.cand_sshift);
         else atomicOr(&c->error, 16);
       }
       // candidate times, ascending (insertion sort of <= 7)
@@ -215,6 +217,17 @@ __global__ void k_dirty_untouched(const MapDev* maps, BatchBufs b) {
     }
   }
 }
+
+// the filled slots of the dense candidate array, and their compaction
+struct PredCand {
+  const uint64_t* c;
+  __device__ bool operator()(int i) const { return c[i] != ~uint64_t(0); }
+};
+struct OutCand {
+  const uint64_t* in;
+  uint64_t* out;
+  __device__ void operator()(int k, int i) const { out[k] = in[i]; }
+};
 
 // ------------------------------------------------------------------ K8 ----
 __global__ void k_spec_eval(const MapDev* maps, BatchBufs b) {
