@@ -105,15 +105,15 @@ def test_map_graph_matches_eager():
     step's shape recurs: scans of a fixed size) gives bitwise the results of
     launching its kernels one by one, scan after scan, with the program's own
     eager work (motion prior, point upload, state and map reads) between the
-    replays.  The 36k-voxel LRU fills after a few scans, so the replays cover
-    eviction too."""
+    replays.  The 14k-voxel LRU fills at the second scan, so the replays
+    cover eviction too."""
     nscans = 10
     n = 20000
     ctx_g = Context(0)
     ctx_e = Context(0)
     ctx_e.set_param("map_graph", 0)
     seq = Sequence(STREET, SEED0 + 71, n)
-    cfg = LIOConfig(max_scan_points=n, map_capacity=36000)
+    cfg = LIOConfig(max_scan_points=n, map_capacity=14000)
     runs = []
     for ctx in (ctx_g, ctx_e):
         g = Registrar(ctx, cfg)

@@ -3,8 +3,11 @@
 // Both run as fixed launch sequences whose sizes come from the host and
 // whose data-dependent counts stay on the device, and neither records
 // anything but kernels, so the map stage can be captured as a hipGraph.
-// Tiles of kPrimTile items: thread t of a tile holds items t*kPrimIpt ..
-// t*kPrimIpt + kPrimIpt - 1 (blocked order), so in-tile ranks are stable.
+// Tiles of kPrimTile items.  The compaction gives thread t of a tile items
+// t*kPrimIpt .. t*kPrimIpt + kPrimIpt - 1 (blocked order); the radix passes
+// give wave w, lane l items w*64*kPrimIpt + k*64 + l for k = 0..kPrimIpt-1
+// (wave-striped order: coalesced loads, and the wave-matching rank walks
+// the items in index order), so in-tile ranks are stable either way.
 #pragma once
 #include <rocprim/block/block_radix_rank.hpp>
 #include <rocprim/block/block_reduce.hpp>
@@ -21,23 +24,52 @@ __host__ __device__ inline int prim_tiles(int64_t n) { return int((n + kPrimTile
 // counts of a radix pass (256 per tile) plus two counters
 __host__ __device__ inline size_t prim_ws_entries(int64_t n) { return size_t(prim_tiles(n)) * 256 + 64; }
 
-// Exclusive scan in place of v[0..m) by one workgroup (each thread a
-// contiguous run); the total goes to *total when total is not null.
+// Exclusive scan in place of v[0..m) by one workgroup, in chunks of
+// kPrimScanThreads * 8 entries: each chunk is loaded and stored coalesced
+// through LDS and scanned in blocked order; the total goes to *total when
+// total is not null.
+constexpr int kPrimScanIpt = 8;
+constexpr int kPrimScanChunk = kPrimScanThreads * kPrimScanIpt;
 __global__ void __launch_bounds__(kPrimScanThreads) k_prim_scan(int32_t* v, int m, int32_t* total) {
   using Scan = rocprim::block_scan<int32_t, kPrimScanThreads>;
   __shared__ typename Scan::storage_type st;
-  const int per = (m + kPrimScanThreads - 1) / kPrimScanThreads;
-  const int lo = min(m, int(threadIdx.x) * per), hi = min(m, lo + per);
-  int32_t sum = 0;
-  for (int i = lo; i < hi; ++i) sum += v[i];
-  int32_t base = 0;
-  Scan().exclusive_scan(sum, base, 0, st);
-  for (int i = lo; i < hi; ++i) {
-    const int32_t x = v[i];
-    v[i] = base;
-    base += x;
+  __shared__ int32_t buf[kPrimScanChunk];
+  __shared__ int32_t carry_s;
+  const int t = threadIdx.x;
+  int32_t carry = 0;
+  for (int c = 0; c < m; c += kPrimScanChunk) {
+#pragma unroll
+    for (int k = 0; k < kPrimScanIpt; ++k) {
+      const int i = c + t + k * kPrimScanThreads;
+      buf[t + k * kPrimScanThreads] = i < m ? v[i] : 0;
+    }
+    __syncthreads();
+    int32_t x[kPrimScanIpt];
+    int32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kPrimScanIpt; ++k) {
+      x[k] = buf[t * kPrimScanIpt + k];
+      sum += x[k];
+    }
+    int32_t base = 0;
+    Scan().exclusive_scan(sum, base, 0, st);
+    base += carry;
+#pragma unroll
+    for (int k = 0; k < kPrimScanIpt; ++k) {
+      buf[t * kPrimScanIpt + k] = base;
+      base += x[k];
+    }
+    if (t == kPrimScanThreads - 1) carry_s = base;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPrimScanIpt; ++k) {
+      const int i = c + t + k * kPrimScanThreads;
+      if (i < m) v[i] = buf[t + k * kPrimScanThreads];
+    }
+    carry = carry_s;
+    __syncthreads();
   }
-  if (total && int(threadIdx.x) == kPrimScanThreads - 1) *total = base;
+  if (total && t == 0) *total = carry;
 }
 
 // ---- stable compaction: out[k] = val(i) for the k-th i < n with pred(i)
@@ -109,33 +141,39 @@ __global__ void __launch_bounds__(kPrimThreads) k_rs_hist(const K* keys, int n, 
   __shared__ int32_t cnt[256];
   cnt[threadIdx.x] = 0;
   __syncthreads();
-  const int base = blockIdx.x * kPrimTile + threadIdx.x * kPrimIpt;
+  const int base = blockIdx.x * kPrimTile + (threadIdx.x >> 6) * 64 * kPrimIpt + (threadIdx.x & 63);
 #pragma unroll
   for (int k = 0; k < kPrimIpt; ++k)
-    if (base + k < n) atomicAdd(&cnt[uint32_t(keys[base + k] >> shift) & mask], 1);
+    if (base + k * 64 < n) atomicAdd(&cnt[uint32_t(keys[base + k * 64] >> shift) & mask], 1);
   __syncthreads();
   hist[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
 }
-// stable scatter of one pass: in-tile ranks by digit (blocked order), then
-// the digit's offset for this tile from the scanned histogram
+// stable scatter of one pass: in-tile ranks by digit (blocked order), the
+// tile sorted by digit in LDS, then written in LDS order so that each
+// digit's run goes out contiguously (coalesced) at that digit's offset for
+// this tile from the scanned histogram
 template <class K>
 __global__ void __launch_bounds__(kPrimThreads) k_rs_scatter(const K* kin, const int32_t* vin, K* kout, int32_t* vout,
                                                             int n, int shift, uint32_t mask, const int32_t* offs) {
-  using Rank = rocprim::block_radix_rank<kPrimThreads, 8, rocprim::block_radix_rank_algorithm::basic>;
+  using Rank = rocprim::block_radix_rank<kPrimThreads, 8, rocprim::block_radix_rank_algorithm::match>;
   static_assert(Rank::digits_per_thread == 1, "one digit per thread");
   __shared__ typename Rank::storage_type st;
   __shared__ int32_t dstart[256];  // first in-tile rank of each digit
   __shared__ int32_t dbase[256];   // the digit's output offset for this tile
-  const int base = blockIdx.x * kPrimTile + threadIdx.x * kPrimIpt;
+  __shared__ K skey[kPrimTile];
+  __shared__ int32_t sval[kPrimTile];
+  const int tile0 = blockIdx.x * kPrimTile;
+  const int base = tile0 + (threadIdx.x >> 6) * 64 * kPrimIpt + (threadIdx.x & 63);
+  const int valid = min(kPrimTile, n - tile0);
   K key[kPrimIpt];
   int32_t val[kPrimIpt];
 #pragma unroll
   for (int k = 0; k < kPrimIpt; ++k) {
-    const bool in = base + k < n;
-    // items past n rank after every valid item of their digit (they are the
-    // tile's last items in blocked order) and are not written
-    key[k] = in ? kin[base + k] : ~K(0);
-    val[k] = in ? vin[base + k] : 0;
+    const bool in = base + k * 64 < n;
+    // items past n carry the largest digit and the tile's largest indices,
+    // so they rank after every valid item and are not written
+    key[k] = in ? kin[base + k * 64] : ~K(0);
+    val[k] = in ? vin[base + k * 64] : 0;
   }
   unsigned int rank[kPrimIpt];
   unsigned int prefix[1], counts[1];
@@ -143,14 +181,19 @@ __global__ void __launch_bounds__(kPrimThreads) k_rs_scatter(const K* kin, const
       key, rank, st, [&](const K& x) { return uint32_t(x >> shift) & mask; }, prefix, counts);
   dstart[threadIdx.x] = int32_t(prefix[0]);
   dbase[threadIdx.x] = offs[size_t(threadIdx.x) * gridDim.x + blockIdx.x];
-  __syncthreads();
 #pragma unroll
   for (int k = 0; k < kPrimIpt; ++k) {
-    if (base + k >= n) continue;
-    const uint32_t d = uint32_t(key[k] >> shift) & mask;
-    const int pos = dbase[d] + (int(rank[k]) - dstart[d]);
-    kout[pos] = key[k];
-    vout[pos] = val[k];
+    skey[rank[k]] = key[k];
+    sval[rank[k]] = val[k];
+  }
+  __syncthreads();
+  // the valid items occupy ranks 0..valid-1 (invalid ones rank last)
+  for (int i = threadIdx.x; i < valid; i += kPrimThreads) {
+    const K kk = skey[i];
+    const uint32_t d = uint32_t(kk >> shift) & mask;
+    const int pos = dbase[d] + (i - dstart[d]);
+    kout[pos] = kk;
+    vout[pos] = sval[i];
   }
 }
 
