@@ -66,15 +66,22 @@ struct PtsRef {
   int32_t n;
   int32_t pad;
 };
-__global__ void k_check_points(const PtsRef* refs, int nref, int32_t* bad) {
+// The point sets come by value (up to kCheckInline) or from a device table;
+// a bad set r gets bad[r] = gen, the call's generation number, so the flags
+// need no clearing between calls.
+constexpr int kCheckInline = 8;
+struct PtsRefs {
+  PtsRef r[kCheckInline];
+};
+__global__ void k_check_points(PtsRefs inl, const PtsRef* refs, int nref, int32_t* bad, int32_t gen) {
   for (int r = blockIdx.y; r < nref; r += gridDim.y) {
-    const PtsRef R = refs[r];
+    const PtsRef R = nref <= kCheckInline ? inl.r[r] : refs[r];
     int b = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < R.n; i += gridDim.x * blockDim.x) {
       b |= !(isfinite(R.x[i]) && isfinite(R.y[i]) && isfinite(R.z[i]));
       if (R.t) b |= !isfinite(R.t[i]) || (i > 0 && !(R.t[i] >= R.t[i - 1]));
     }
-    if (__ballot(b) != 0ull && (threadIdx.x & 63) == 0) bad[r] = 1;
+    if (__ballot(b) != 0ull && (threadIdx.x & 63) == 0) bad[r] = gen;
   }
 }
 
@@ -127,6 +134,12 @@ struct BatchWs {
   BatchBufs b{};
   void* temp = nullptr;
   size_t temp_bytes = 0;
+  // the per-session job tables (maps, session info, observe jobs, filter /
+  // counter pointers) sit in one device block with one pinned mirror, so a
+  // step uploads them with one copy
+  uint8_t* d_tab = nullptr;
+  uint8_t* h_tab = nullptr;
+  size_t tab_obsstat = 0;  // offset of the last table (d_obsstat)
   MapDev* d_maps = nullptr;
   SessInfo* d_si = nullptr;
   ObsJob* d_jobs = nullptr;
@@ -251,6 +264,7 @@ struct vmp_ctx {
   // validation of device point buffers (k_check_points), run on `stream`
   // when a step starts (no batch is in flight then)
   bool check_points = true;
+  int32_t check_gen = 0;  // generation of the last point check (bad-set flags)
   // vmp_register_scan_noalloc: a workspace and ring scratch carved from the
   // device's scratch array at context creation (na_ready once that
   // succeeded); the call swaps them in for its duration
@@ -422,7 +436,8 @@ static bool grow_checks(vmp_ctx* c, int nref) {
   if (hipHostMalloc(&c->h_refs, sizeof(PtsRef) * cap) != hipSuccess ||
       hipHostMalloc(&c->h_bad, sizeof(int32_t) * cap) != hipSuccess ||
       hipMalloc(&c->d_refs, sizeof(PtsRef) * cap) != hipSuccess ||
-      hipMalloc(&c->d_bad, sizeof(int32_t) * cap) != hipSuccess)
+      hipMalloc(&c->d_bad, sizeof(int32_t) * cap) != hipSuccess ||
+      hipMemset(c->d_bad, 0, sizeof(int32_t) * cap) != hipSuccess)  // no stale generation matches
     return false;
   c->vcap = cap;
   return true;
@@ -432,21 +447,25 @@ static int check_device_points(vmp_ctx* c, const PtsRef* refs, int nref) {
   if (nref <= 0 || !c->check_points) return -1;
   if (!grow_checks(c, nref)) return -2;
   int64_t most = 0;
+  PtsRefs inl{};
   for (int i = 0; i < nref; ++i) {
     c->h_refs[i] = refs[i];
+    if (i < kCheckInline) inl.r[i] = refs[i];
     most = std::max<int64_t>(most, refs[i].n);
   }
   hipStream_t st = c->stream;
-  if (hipMemcpyAsync(c->d_refs, c->h_refs, sizeof(PtsRef) * nref, hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemsetAsync(c->d_bad, 0, sizeof(int32_t) * nref, st) != hipSuccess)
+  if (nref > kCheckInline &&
+      hipMemcpyAsync(c->d_refs, c->h_refs, sizeof(PtsRef) * nref, hipMemcpyHostToDevice, st) != hipSuccess)
     return -2;
-  const int gx = int(std::max<int64_t>(1, std::min<int64_t>((most + 1023) / 1024, 256)));
-  hipLaunchKernelGGL(k_check_points, dim3(gx, std::min(nref, 65535)), dim3(256), 0, st, c->d_refs, nref, c->d_bad);
+  const int32_t gen = ++c->check_gen;
+  const int gx = int(std::max<int64_t>(1, std::min<int64_t>((most + 255) / 256, 1024)));
+  hipLaunchKernelGGL(k_check_points, dim3(gx, std::min(nref, 65535)), dim3(256), 0, st, inl, c->d_refs, nref,
+                     c->d_bad, gen);
   if (hipMemcpyAsync(c->h_bad, c->d_bad, sizeof(int32_t) * nref, hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
     return -2;
   for (int i = 0; i < nref; ++i)
-    if (c->h_bad[i]) return i;
+    if (c->h_bad[i] == gen) return i;
   return -1;
 }
 
@@ -484,13 +503,34 @@ static bool ws_layout(vmp_ctx* c, BatchWs& w, size_t n, int ns, size_t pre_cap, 
   ok = ok && dev(&b.pre_x, size_t(b.pre_cap) * kPreItems) && dev(&b.pre_seg, size_t(b.pre_cap) * kPreItems) &&
        dev(&b.pre_tn, size_t(b.pre_cap) * kPreItems);
   const size_t S = size_t(ns);
-  ok = ok && dev(&w.d_maps, S) && dev(&w.d_si, S) && dev(&w.d_jobs, S) && dev(&w.d_flist, S) &&
-       dev(&w.d_obsctr, S) && dev(&w.d_obsstat, S) && dev(&w.d_out, S) && dev(&w.d_motion, S) && dev(&w.d_pre, S) &&
-       dev(&w.d_vg, S) && dev(&w.d_vgst, S * kVgStat);
-  ok = ok && host(&w.h_maps, sizeof(MapDev) * S) && host(&w.h_si, sizeof(SessInfo) * S) &&
-       host(&w.h_jobs, sizeof(ObsJob) * S) && host(&w.h_flist, sizeof(FilterDev*) * S) &&
-       host(&w.h_obsctr, sizeof(int32_t*) * S) && host(&w.h_obsstat, sizeof(int32_t*) * S) &&
-       host(&w.h_out, sizeof(BatchOut) * S) && host(&w.h_motion, sizeof(MotionRec) * S) &&
+  size_t tab = 0;
+  auto tab_at = [&](size_t bytes) {
+    const size_t o = tab;
+    tab = (tab + bytes + 255) & ~size_t(255);
+    return o;
+  };
+  const size_t o_maps = tab_at(sizeof(MapDev) * S), o_si = tab_at(sizeof(SessInfo) * S);
+  const size_t o_jobs = tab_at(sizeof(ObsJob) * S), o_fl = tab_at(sizeof(FilterDev*) * S);
+  const size_t o_oc = tab_at(sizeof(int32_t*) * S), o_os = tab_at(sizeof(int32_t*) * S);
+  ok = ok && dev(&w.d_tab, tab) && host(&w.h_tab, tab);
+  if (ok) {
+    w.tab_obsstat = o_os;
+    w.d_maps = reinterpret_cast<MapDev*>(w.d_tab + o_maps);
+    w.d_si = reinterpret_cast<SessInfo*>(w.d_tab + o_si);
+    w.d_jobs = reinterpret_cast<ObsJob*>(w.d_tab + o_jobs);
+    w.d_flist = reinterpret_cast<FilterDev**>(w.d_tab + o_fl);
+    w.d_obsctr = reinterpret_cast<int32_t**>(w.d_tab + o_oc);
+    w.d_obsstat = reinterpret_cast<int32_t**>(w.d_tab + o_os);
+    w.h_maps = reinterpret_cast<MapDev*>(w.h_tab + o_maps);
+    w.h_si = reinterpret_cast<SessInfo*>(w.h_tab + o_si);
+    w.h_jobs = reinterpret_cast<ObsJob*>(w.h_tab + o_jobs);
+    w.h_flist = reinterpret_cast<FilterDev**>(w.h_tab + o_fl);
+    w.h_obsctr = reinterpret_cast<int32_t**>(w.h_tab + o_oc);
+    w.h_obsstat = reinterpret_cast<int32_t**>(w.h_tab + o_os);
+  }
+  ok = ok && dev(&w.d_out, S) && dev(&w.d_motion, S) && dev(&w.d_pre, S) && dev(&w.d_vg, S) &&
+       dev(&w.d_vgst, S * kVgStat);
+  ok = ok && host(&w.h_out, sizeof(BatchOut) * S) && host(&w.h_motion, sizeof(MotionRec) * S) &&
        host(&w.h_pre, sizeof(PreJob) * S) && host(&w.h_vg, sizeof(VgJob) * S) &&
        host(&w.h_vgst, sizeof(int32_t) * kVgStat * S);
   if (ok && !w.motion_done) ok = hipEventCreateWithFlags(&w.motion_done, hipEventDisableTiming) == hipSuccess;
@@ -1408,14 +1448,9 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
   stage_batch(c, ss, n, &ntot64, &nblocks);
   const int ntot = int(ntot64);
   for (int i = 0; i < n; ++i) w.h_si[i].is_build = (mode & kStepBuild) ? 1 : 0;
-  if (hipMemcpyAsync(w.d_maps, w.h_maps, sizeof(MapDev) * n, hipMemcpyHostToDevice, st) != hipSuccess) return VMP_EDEVICE;
-  if (hipMemcpyAsync(w.d_si, w.h_si, sizeof(SessInfo) * n, hipMemcpyHostToDevice, st) != hipSuccess) return VMP_EDEVICE;
-  if (hipMemcpyAsync(w.d_jobs, w.h_jobs, sizeof(ObsJob) * n, hipMemcpyHostToDevice, st) != hipSuccess) return VMP_EDEVICE;
-  if (hipMemcpyAsync(w.d_flist, w.h_flist, sizeof(FilterDev*) * n, hipMemcpyHostToDevice, st) != hipSuccess)
-    return VMP_EDEVICE;
-  if (hipMemcpyAsync(w.d_obsctr, w.h_obsctr, sizeof(int32_t*) * n, hipMemcpyHostToDevice, st) != hipSuccess)
-    return VMP_EDEVICE;
-  if (hipMemcpyAsync(w.d_obsstat, w.h_obsstat, sizeof(int32_t*) * n, hipMemcpyHostToDevice, st) != hipSuccess)
+  // the job tables in one copy: from the first table through the last
+  // one's n entries (the unused tails of the earlier tables ride along)
+  if (hipMemcpyAsync(w.d_tab, w.h_tab, w.tab_obsstat + sizeof(int32_t*) * n, hipMemcpyHostToDevice, st) != hipSuccess)
     return VMP_EDEVICE;
   const bool ext = ss[0]->cfg.estimate_ext != 0;
   // zero: clear the sessions' observe counters first (inside the filter stage
