@@ -94,10 +94,12 @@ struct BatchOut {
   int32_t iters;
   int32_t pad[3];
 };
-__global__ void k_gather(const MapDev* maps, FilterDev** fs, int32_t** obs_stat, BatchOut* out, int nsess) {
+__global__ void k_gather(const MapDev* maps, FilterDev** fs, int32_t** obs_stat, BatchOut* out, int nsess,
+                         const int32_t* prim_err) {
   const int s = blockIdx.x;
   if (s >= nsess || threadIdx.x != 0) return;
   out[s].ctr = *maps[s].ctr;
+  if (*prim_err) out[s].ctr.error |= 64;  // a bounded wait of the compaction / sort ran out
   for (int k = 0; k < 8; ++k) out[s].obs[k] = obs_stat[s][k];
   out[s].iters = fs[s]->iters;
   out[s].x = fs[s]->x;
@@ -571,7 +573,8 @@ static int ws_reserve(vmp_ctx* c, int64_t points, int nsess, bool exact = false)
     return true;
   };
   w.motion_pending = false;
-  if (!ws_layout(c, w, size_t(np), ns, size_t(1) << 18, dev, host)) {
+  if (!ws_layout(c, w, size_t(np), ns, size_t(1) << 18, dev, host) ||
+      hipMemsetAsync(w.b.prim, 0, prim_ws_entries(np) * sizeof(int32_t), c->stream) != hipSuccess) {
     w.release();
     return VMP_ENOMEM;
   }
@@ -626,6 +629,11 @@ static void noalloc_setup(vmp_ctx* c) {
     return true;
   };
   if (!ws_layout(c, w, size_t(kNoAllocMaxPoints), 1, size_t(kNoAllocPreCap), carve, host)) {
+    w.release();
+    return;
+  }
+  // the single-pass primitives' control words start from zero
+  if (hipMemset(w.b.prim, 0, prim_ws_entries(kNoAllocMaxPoints) * sizeof(int32_t)) != hipSuccess) {
     w.release();
     return;
   }
@@ -1649,7 +1657,8 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
     }
   }
   hipEventRecord(c->ev[3], st);
-  hipLaunchKernelGGL(k_gather, dim3(n), dim3(64), 0, st, w.d_maps, w.d_flist, w.d_obsstat, w.d_out, n);
+  hipLaunchKernelGGL(k_gather, dim3(n), dim3(64), 0, st, w.d_maps, w.d_flist, w.d_obsstat, w.d_out, n,
+                     &prim_ws(w.b.prim).ctl->err);
   if (hipMemcpyAsync(w.h_out, w.d_out, sizeof(BatchOut) * n, hipMemcpyDeviceToHost, st) != hipSuccess)
     return VMP_EDEVICE;
   return hipGetLastError() == hipSuccess ? VMP_OK : VMP_EDEVICE;
@@ -2006,7 +2015,7 @@ static int step_finish(vmp_ctx* c, vmp_obs_stats* ost, vmp_map_stats* mst) {
         char buf[200];
         std::snprintf(buf, sizeof(buf),
                       "map update error 0x%x (1 hash probe, 2 key range, 4 LRU exhausted in one scan, 8 voxel "
-                      "pool, 16 replay queue, 32 replay class)",
+                      "pool, 16 replay queue, 32 replay class, 64 compaction / sort wait)",
                       unsigned(m.error));
         si->err = buf;
         if (first_err == VMP_OK) first_err = (m.error & 2) ? VMP_ERANGE : ((m.error & 4) ? VMP_ECAPACITY : VMP_EDEVICE);

@@ -1,14 +1,16 @@
-// Device-wide primitives of the map stage: a stable compaction and a stable
-// LSD radix sort of (64-bit key, 32-bit value) pairs on a key's low bits.
-// Both run as fixed launch sequences whose sizes come from the host and
-// whose data-dependent counts stay on the device, and neither records
-// anything but kernels, so the map stage can be captured as a hipGraph.
+// Device-wide primitives of the map stage: a stable compaction (one launch)
+// and a stable LSD radix sort of (key, 32-bit value) pairs on a key's low
+// bits (one histogram launch, then one launch per pass).  Their sizes come
+// from the host, their data-dependent counts stay on the device, and they
+// launch nothing but kernels, so the map stage can be captured as a hipGraph.
 // Tiles of kPrimTile items.  The compaction gives thread t of a tile items
 // t*kPrimIpt .. t*kPrimIpt + kPrimIpt - 1 (blocked order); the radix passes
 // give wave w, lane l items w*64*kPrimIpt + k*64 + l for k = 0..kPrimIpt-1
 // (wave-striped order: coalesced loads, and the wave-matching rank walks
 // the items in index order), so in-tile ranks are stable either way.
 #pragma once
+#include <algorithm>
+
 #include <rocprim/block/block_radix_rank.hpp>
 #include <rocprim/block/block_reduce.hpp>
 #include <rocprim/block/block_scan.hpp>
@@ -18,78 +20,99 @@ namespace vmp {
 constexpr int kPrimThreads = 256;
 constexpr int kPrimIpt = 8;
 constexpr int kPrimTile = kPrimThreads * kPrimIpt;
-constexpr int kPrimScanThreads = 1024;
 __host__ __device__ inline int prim_tiles(int64_t n) { return int((n + kPrimTile - 1) / kPrimTile); }
-// int32 work entries the primitives need for n items: per-tile digit
-// counts of a radix pass (256 per tile) plus two counters
-__host__ __device__ inline size_t prim_ws_entries(int64_t n) { return size_t(prim_tiles(n)) * 256 + 64; }
+// ---- single-pass control.  A launch takes its tiles in ticket order (a
+// tile's predecessors have all started, so waiting on them always ends),
+// publishes per tile its aggregate and then its inclusive prefix in status
+// words (bits 30-31: 1 aggregate, 2 inclusive; bits 0-29 the value), and
+// finds its own prefix by looking back over the predecessors' words.  The
+// tile that takes the last ticket resets the ticket counter; the last tile to
+// finish clears the status words and the done counter, so every launch
+// starts from zeros (the work area is zeroed once when it is allocated).
+struct PrimCtl {
+  int32_t ticket;
+  int32_t done;
+  int32_t err;  // a bounded wait ran out (not expected; the results are then void)
+  int32_t pad[61];
+};
+constexpr uint32_t kStAgg = 1u << 30, kStInc = 2u << 30, kStVal = (1u << 30) - 1;
+constexpr int kSpinMax = 1 << 22;
+constexpr int kPrimMaxPasses = 9;  // radix passes of up to 64 key bits (odd count)
+// work area of n items, in int32 entries: control, the radix histograms of
+// every pass, then 256 status words per tile
+__host__ __device__ inline size_t prim_ws_entries(int64_t n) {
+  return 64 + size_t(kPrimMaxPasses) * 256 + size_t(prim_tiles(n)) * 256;
+}
+struct PrimWs {
+  PrimCtl* ctl;
+  int32_t* ghist;    // [kPrimMaxPasses][256]
+  uint32_t* status;  // [tiles][256] (the compaction uses [tiles])
+};
+__host__ __device__ inline PrimWs prim_ws(int32_t* w) {
+  return PrimWs{reinterpret_cast<PrimCtl*>(w), w + 64, reinterpret_cast<uint32_t*>(w + 64 + kPrimMaxPasses * 256)};
+}
 
-// Exclusive scan in place of v[0..m) by one workgroup, in chunks of
-// kPrimScanThreads * 8 entries: each chunk is loaded and stored coalesced
-// through LDS and scanned in blocked order; the total goes to *total when
-// total is not null.
-constexpr int kPrimScanIpt = 8;
-constexpr int kPrimScanChunk = kPrimScanThreads * kPrimScanIpt;
-__global__ void __launch_bounds__(kPrimScanThreads) k_prim_scan(int32_t* v, int m, int32_t* total) {
-  using Scan = rocprim::block_scan<int32_t, kPrimScanThreads>;
-  __shared__ typename Scan::storage_type st;
-  __shared__ int32_t buf[kPrimScanChunk];
-  __shared__ int32_t carry_s;
-  const int t = threadIdx.x;
-  int32_t carry = 0;
-  for (int c = 0; c < m; c += kPrimScanChunk) {
-#pragma unroll
-    for (int k = 0; k < kPrimScanIpt; ++k) {
-      const int i = c + t + k * kPrimScanThreads;
-      buf[t + k * kPrimScanThreads] = i < m ? v[i] : 0;
+__device__ inline uint32_t st_load(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_store(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// the tile's ticket (thread 0; broadcast by the caller)
+__device__ inline int take_ticket(PrimCtl* ctl, int nt) {
+  const int t = atomicAdd(&ctl->ticket, 1);
+  if (t == nt - 1) atomicExch(&ctl->ticket, 0);
+  return t;
+}
+// exclusive prefix of a tile's slot from the predecessors' status words at
+// stride `stride` (slot + j * stride for tile j)
+__device__ inline uint32_t look_back(const uint32_t* status, int tile, int stride, int slot, PrimCtl* ctl) {
+  uint32_t prefix = 0;
+  int j = tile - 1, spins = 0;
+  while (j >= 0) {
+    const uint32_t w = st_load(status + size_t(j) * stride + slot);
+    if ((w >> 30) == 0) {
+      if (++spins > kSpinMax) {
+        atomicOr(&ctl->err, 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
     }
-    __syncthreads();
-    int32_t x[kPrimScanIpt];
-    int32_t sum = 0;
-#pragma unroll
-    for (int k = 0; k < kPrimScanIpt; ++k) {
-      x[k] = buf[t * kPrimScanIpt + k];
-      sum += x[k];
-    }
-    int32_t base = 0;
-    Scan().exclusive_scan(sum, base, 0, st);
-    base += carry;
-#pragma unroll
-    for (int k = 0; k < kPrimScanIpt; ++k) {
-      buf[t * kPrimScanIpt + k] = base;
-      base += x[k];
-    }
-    if (t == kPrimScanThreads - 1) carry_s = base;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kPrimScanIpt; ++k) {
-      const int i = c + t + k * kPrimScanThreads;
-      if (i < m) v[i] = buf[t + k * kPrimScanThreads];
-    }
-    carry = carry_s;
-    __syncthreads();
+    prefix += w & kStVal;
+    if ((w & kStInc) == kStInc) break;
+    --j;
   }
-  if (total && t == 0) *total = carry;
+  return prefix;
+}
+// end of a tile: the last to finish clears `nclear` status words (and the
+// extra words, if any) and the done counter; returns true in that tile
+// (every thread of it)
+__device__ inline bool finish_tile(PrimCtl* ctl, int nt, uint32_t* status, int nclear, int32_t* extra, int nextra,
+                                   int* s_last) {
+  __syncthreads();
+  if (threadIdx.x == 0) *s_last = atomicAdd(&ctl->done, 1) == nt - 1;
+  __syncthreads();
+  if (!*s_last) return false;
+  for (int i = threadIdx.x; i < nclear; i += blockDim.x) status[i] = 0u;
+  for (int i = threadIdx.x; i < nextra; i += blockDim.x) extra[i] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicExch(&ctl->done, 0);
+  return true;
 }
 
-// ---- stable compaction: out[k] = val(i) for the k-th i < n with pred(i)
-template <class Pred>
-__global__ void __launch_bounds__(kPrimThreads) k_sel_count(int n, Pred pred, int32_t* tile_cnt) {
-  using Reduce = rocprim::block_reduce<int32_t, kPrimThreads>;
-  __shared__ typename Reduce::storage_type st;
-  const int base = blockIdx.x * kPrimTile + threadIdx.x * kPrimIpt;
-  int32_t c = 0;
-#pragma unroll
-  for (int k = 0; k < kPrimIpt; ++k) c += (base + k < n && pred(base + k)) ? 1 : 0;
-  int32_t tot;
-  Reduce().reduce(c, tot, st);
-  if (threadIdx.x == 0) tile_cnt[blockIdx.x] = tot;
-}
+// ---- stable compaction: out(k, i) for the k-th i < n (ascending) with pred(i)
 template <class Pred, class Out>
-__global__ void __launch_bounds__(kPrimThreads) k_sel_scatter(int n, Pred pred, const int32_t* tile_off, Out out) {
+__global__ void __launch_bounds__(kPrimThreads) k_select1(int n, Pred pred, Out out, int32_t* count, PrimWs ws) {
   using Scan = rocprim::block_scan<int32_t, kPrimThreads>;
   __shared__ typename Scan::storage_type st;
-  const int base = blockIdx.x * kPrimTile + threadIdx.x * kPrimIpt;
+  __shared__ int s_tile, s_last;
+  __shared__ uint32_t s_prefix;
+  const int nt = gridDim.x;
+  if (threadIdx.x == 0) s_tile = take_ticket(ws.ctl, nt);
+  __syncthreads();
+  const int tile = s_tile;
+  const int base = tile * kPrimTile + threadIdx.x * kPrimIpt;
   bool f[kPrimIpt];
   int32_t c = 0;
 #pragma unroll
@@ -97,12 +120,26 @@ __global__ void __launch_bounds__(kPrimThreads) k_sel_scatter(int n, Pred pred, 
     f[k] = base + k < n && pred(base + k);
     c += f[k] ? 1 : 0;
   }
-  int32_t pos = 0;
-  Scan().exclusive_scan(c, pos, 0, st);
-  pos += tile_off[blockIdx.x];
+  int32_t pos = 0, agg = 0;
+  Scan().exclusive_scan(c, pos, 0, agg, st);
+  if (threadIdx.x == 0) {
+    uint32_t prefix = 0;
+    if (tile == 0) {
+      st_store(ws.status, kStInc | uint32_t(agg));
+    } else {
+      st_store(ws.status + tile, kStAgg | uint32_t(agg));
+      prefix = look_back(ws.status, tile, 1, 0, ws.ctl);
+      st_store(ws.status + tile, kStInc | (prefix + uint32_t(agg)));
+    }
+    s_prefix = prefix;
+  }
+  __syncthreads();
+  pos += int32_t(s_prefix);
 #pragma unroll
   for (int k = 0; k < kPrimIpt; ++k)
     if (f[k]) out(pos++, base + k);
+  if (threadIdx.x == 0 && tile == nt - 1) *count = int32_t(s_prefix) + agg;
+  finish_tile(ws.ctl, nt, ws.status, nt, nullptr, 0, &s_last);
 }
 
 // flags[i] != 0 -> the ascending indices, as rocprim::select over a counting
@@ -116,53 +153,63 @@ struct OutIndex {
   __device__ void operator()(int k, int i) const { o[k] = i; }
 };
 
-// stable compaction; work: prim_ws_entries(n) int32 entries, *count the total
+// stable compaction, one launch; work: prim_ws_entries(n) int32 entries,
+// *count the total
 template <class Pred, class Out>
 inline void select_stable(hipStream_t st, int n, Pred pred, Out out, int32_t* work, int32_t* count) {
-  const int nt = prim_tiles(n);
-  if (nt == 0) {
-    hipLaunchKernelGGL(k_prim_scan, dim3(1), dim3(kPrimScanThreads), 0, st, work, 0, count);
-    return;
-  }
-  hipLaunchKernelGGL((k_sel_count<Pred>), dim3(nt), dim3(kPrimThreads), 0, st, n, pred, work);
-  hipLaunchKernelGGL(k_prim_scan, dim3(1), dim3(kPrimScanThreads), 0, st, work, nt, count);
-  hipLaunchKernelGGL((k_sel_scatter<Pred, Out>), dim3(nt), dim3(kPrimThreads), 0, st, n, pred, work, out);
+  const int nt = std::max(1, prim_tiles(n));
+  hipLaunchKernelGGL((k_select1<Pred, Out>), dim3(nt), dim3(kPrimThreads), 0, st, n, pred, out, count,
+                     prim_ws(work));
 }
 
-// largest input the primitives take (the tile scans run in one workgroup);
-// larger batches use rocPRIM's device-wide routines
+// largest input the primitives take; larger batches use rocPRIM's
+// device-wide routines
 constexpr int kPrimMaxItems = 1 << 20;
 
 // ---- stable LSD radix sort of (key, value) on key bits [0, bits)
-// digit histogram of each tile, stored digit-major: hist[d * ntile + tile]
+// the digit histograms of every pass over the whole input, added into
+// ghist[pass][256] (which the sort's last pass clears again)
 template <class K>
-__global__ void __launch_bounds__(kPrimThreads) k_rs_hist(const K* keys, int n, int shift, uint32_t mask,
-                                                         int32_t* hist) {
-  __shared__ int32_t cnt[256];
-  cnt[threadIdx.x] = 0;
+__global__ void __launch_bounds__(kPrimThreads) k_rs_ghist(const K* keys, int n, int passes, int width,
+                                                          int32_t* ghist) {
+  __shared__ int32_t h[kPrimMaxPasses * 256];
+  for (int i = threadIdx.x; i < passes * 256; i += kPrimThreads) h[i] = 0;
   __syncthreads();
+  const uint32_t mask = (1u << width) - 1u;
   const int base = blockIdx.x * kPrimTile + (threadIdx.x >> 6) * 64 * kPrimIpt + (threadIdx.x & 63);
 #pragma unroll
-  for (int k = 0; k < kPrimIpt; ++k)
-    if (base + k * 64 < n) atomicAdd(&cnt[uint32_t(keys[base + k * 64] >> shift) & mask], 1);
+  for (int k = 0; k < kPrimIpt; ++k) {
+    if (base + k * 64 >= n) continue;
+    const K key = keys[base + k * 64];
+    for (int p = 0; p < passes; ++p) atomicAdd(&h[p * 256 + (uint32_t(key >> (p * width)) & mask)], 1);
+  }
   __syncthreads();
-  hist[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
+  for (int i = threadIdx.x; i < passes * 256; i += kPrimThreads)
+    if (h[i]) atomicAdd(&ghist[i], h[i]);
 }
-// stable scatter of one pass: in-tile ranks by digit (blocked order), the
-// tile sorted by digit in LDS, then written in LDS order so that each
-// digit's run goes out contiguously (coalesced) at that digit's offset for
-// this tile from the scanned histogram
+// One pass: in-tile ranks by digit (wave matching over wave-striped items),
+// each digit's offset = its global start (scan of the pass's histogram) +
+// the digit's count in earlier tiles (look-back per digit, one thread per
+// digit), the tile sorted by digit in LDS and written out coalesced.
 template <class K>
-__global__ void __launch_bounds__(kPrimThreads) k_rs_scatter(const K* kin, const int32_t* vin, K* kout, int32_t* vout,
-                                                            int n, int shift, uint32_t mask, const int32_t* offs) {
+__global__ void __launch_bounds__(kPrimThreads) k_rs_pass(const K* kin, const int32_t* vin, K* kout, int32_t* vout,
+                                                         int n, int shift, uint32_t mask, const int32_t* hist,
+                                                         PrimWs ws, int clear_hist) {
   using Rank = rocprim::block_radix_rank<kPrimThreads, 8, rocprim::block_radix_rank_algorithm::match>;
+  using Scan = rocprim::block_scan<int32_t, kPrimThreads>;
   static_assert(Rank::digits_per_thread == 1, "one digit per thread");
   __shared__ typename Rank::storage_type st;
+  __shared__ typename Scan::storage_type sst;
   __shared__ int32_t dstart[256];  // first in-tile rank of each digit
   __shared__ int32_t dbase[256];   // the digit's output offset for this tile
   __shared__ K skey[kPrimTile];
   __shared__ int32_t sval[kPrimTile];
-  const int tile0 = blockIdx.x * kPrimTile;
+  __shared__ int s_tile, s_last;
+  const int nt = gridDim.x;
+  if (threadIdx.x == 0) s_tile = take_ticket(ws.ctl, nt);
+  __syncthreads();
+  const int tile = s_tile;
+  const int tile0 = tile * kPrimTile;
   const int base = tile0 + (threadIdx.x >> 6) * 64 * kPrimIpt + (threadIdx.x & 63);
   const int valid = min(kPrimTile, n - tile0);
   K key[kPrimIpt];
@@ -179,22 +226,37 @@ __global__ void __launch_bounds__(kPrimThreads) k_rs_scatter(const K* kin, const
   unsigned int prefix[1], counts[1];
   Rank().rank_keys(
       key, rank, st, [&](const K& x) { return uint32_t(x >> shift) & mask; }, prefix, counts);
-  dstart[threadIdx.x] = int32_t(prefix[0]);
-  dbase[threadIdx.x] = offs[size_t(threadIdx.x) * gridDim.x + blockIdx.x];
+  const int d = threadIdx.x;
+  // this tile's count of digit d (the invalid items, all of digit `mask`,
+  // are not counted)
+  const uint32_t cnt_d = counts[0] - (d == int(mask) ? uint32_t(kPrimTile - valid) : 0u);
+  uint32_t* sd = ws.status + d;
+  uint32_t before = 0;
+  if (tile == 0) {
+    st_store(sd, kStInc | cnt_d);
+  } else {
+    st_store(sd + size_t(tile) * 256, kStAgg | cnt_d);
+    before = look_back(ws.status, tile, 256, d, ws.ctl);
+    st_store(sd + size_t(tile) * 256, kStInc | (before + cnt_d));
+  }
+  int32_t gstart = 0;
+  Scan().exclusive_scan(hist[d], gstart, 0, sst);
+  dstart[d] = int32_t(prefix[0]);
+  dbase[d] = gstart + int32_t(before);
 #pragma unroll
   for (int k = 0; k < kPrimIpt; ++k) {
     skey[rank[k]] = key[k];
     sval[rank[k]] = val[k];
   }
   __syncthreads();
-  // the valid items occupy ranks 0..valid-1 (invalid ones rank last)
   for (int i = threadIdx.x; i < valid; i += kPrimThreads) {
     const K kk = skey[i];
-    const uint32_t d = uint32_t(kk >> shift) & mask;
-    const int pos = dbase[d] + (i - dstart[d]);
-    kout[pos] = kk;
-    vout[pos] = sval[i];
+    const uint32_t dd = uint32_t(kk >> shift) & mask;
+    kout[dbase[dd] + (i - dstart[dd])] = kk;
+    vout[dbase[dd] + (i - dstart[dd])] = sval[i];
   }
+  finish_tile(ws.ctl, nt, ws.status, nt * 256, clear_hist ? ws.ghist : nullptr,
+              clear_hist ? kPrimMaxPasses * 256 : 0, &s_last);
 }
 
 // passes for `bits` key bits: an odd number (so the result lands in the
@@ -205,8 +267,8 @@ inline int rs_passes(int bits) {
   return p;
 }
 // Sorts (kin, vin) into (kout, vout) by the low `bits` bits of the keys,
-// stably; kin / vin are overwritten (ping-pong buffers).  work:
-// prim_ws_entries(n) int32 entries.
+// stably; kin / vin are overwritten (ping-pong buffers).  One histogram
+// launch, then one launch per pass.  work: prim_ws_entries(n) int32 entries.
 template <class K>
 inline void radix_sort_stable(hipStream_t st, K* kin, int32_t* vin, K* kout, int32_t* vout, int n, int bits,
                               int32_t* work) {
@@ -215,16 +277,16 @@ inline void radix_sort_stable(hipStream_t st, K* kin, int32_t* vin, K* kout, int
   const int passes = rs_passes(bits);
   const int width = bits <= 0 ? 1 : (bits + passes - 1) / passes;
   const uint32_t mask = (1u << width) - 1u;
+  const PrimWs ws = prim_ws(work);
+  hipLaunchKernelGGL(k_rs_ghist<K>, dim3(nt), dim3(kPrimThreads), 0, st, kin, n, passes, width, ws.ghist);
   for (int p = 0; p < passes; ++p) {
     const bool fwd = (p & 1) == 0;  // even passes in -> out, odd ones out -> in
     const K* ks = fwd ? kin : kout;
     const int32_t* vs = fwd ? vin : vout;
     K* kd = fwd ? kout : kin;
     int32_t* vd = fwd ? vout : vin;
-    const int shift = p * width;
-    hipLaunchKernelGGL(k_rs_hist<K>, dim3(nt), dim3(kPrimThreads), 0, st, ks, n, shift, mask, work);
-    hipLaunchKernelGGL(k_prim_scan, dim3(1), dim3(kPrimScanThreads), 0, st, work, nt * 256, nullptr);
-    hipLaunchKernelGGL(k_rs_scatter<K>, dim3(nt), dim3(kPrimThreads), 0, st, ks, vs, kd, vd, n, shift, mask, work);
+    hipLaunchKernelGGL(k_rs_pass<K>, dim3(nt), dim3(kPrimThreads), 0, st, ks, vs, kd, vd, n, p * width, mask,
+                       ws.ghist + p * 256, ws, p == passes - 1 ? 1 : 0);
   }
 }
 
