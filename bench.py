@@ -186,6 +186,9 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000, groups=1):
     rng = np.random.default_rng(0xBA7C4)
     total = W + K + 2  # IMU_INIT and MAP_INIT packages first
     sizes = rng.integers(2000, nmax + 1, size=(total, B))
+    # the IMU_INIT package is a full sweep, so it holds the imu_init_num
+    # samples (a scan of fewer points ends earlier in its sweep)
+    sizes[0, :] = nmax
     seqs = [ImuSequence(ROOM if i % 2 == 0 else STREET, SEED0 + 1000 + i, nmax) for i in range(B)]
     t0 = time.perf_counter()
     steps = []

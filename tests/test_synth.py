@@ -20,6 +20,8 @@ def test_digests_match_the_committed_fixture():
     want = json.load(open(os.path.join(HERE, "golden", "synth_digest.json")))
     for name, sc, seed, n, ks in mk.CASES:
         assert mk.digest(sc, seed, n, ks) == want[name]["digests"], name
+    for name, sc, seed, n, npkg in mk.IMU_CASES:
+        assert mk.digest_imu(sc, seed, n, npkg) == want[name]["digests"], name
 
 
 def test_scan_shape_range_and_prefix():
