@@ -225,7 +225,8 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000, groups=1):
     for i, g in enumerate(regs):  # IMU_INIT
         imu, cs, ce = steps[0][2][i]
         assert g.imu_init(imu, ce), "one package holds imu_init_num samples"
-    bind(1, range(B))
+    for gi in range(groups):  # a batch binding takes the sessions of one context
+        bind(1, members[gi])
     for g in regs:  # MAP_INIT: undistortion + build
         g.build()
 
