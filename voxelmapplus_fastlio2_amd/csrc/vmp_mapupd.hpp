@@ -150,6 +150,10 @@ struct BatchBufs {
   int32_t* lv_cnt;
   int32_t* lv_n;
   int32_t* prim;       // work entries of the compaction / radix sort (prim_ws_entries)
+  // per point (global index): its insertion point relative to its voxel's
+  // centre (fp32, as the replays fold it in) and its fp32 world covariance,
+  // (q.x, q.y, q.z, -, C0..C5, -, -), formed once by k_ins_keys
+  float4* qc;
 };
 __host__ __device__ inline int64_t lv_cnt_base(int off, int s) { return (int64_t(2) * off >> 6) + s; }
 
@@ -401,6 +405,17 @@ __global__ void k_ins_keys(const MapDev* maps, const SessInfo* si, int nsess, Ba
     else atomicOr(&m.ctr->error, 2);
     b.skey_in[g] = (uint64_t(s) << b.hbits) | uint64_t(uint32_t(e < 0 ? 0 : e));
     b.sval_in[g] = g;
+    // the replays' inputs: the point relative to its voxel's centre and its
+    // world covariance (calcBodyCov's z = 0 -> 0.001 applies to the
+    // covariance only, commons.cpp:20-23)
+    float qv[3];
+    for (int r = 0; r < 3; ++r) qv[r] = float(double(pw[r]) - (double(k[r]) + 0.5) * m.vs64);
+    const float pz = S.z[i] == 0.0f ? 0.001f : S.z[i];
+    float C[6];
+    point_world_cov(P.rwl32, P.Prr, P.Ppp, P.s2, P.rv, S.x[i], S.y[i], pz, C);
+    b.qc[3 * size_t(g)] = make_float4(qv[0], qv[1], qv[2], 0.0f);
+    b.qc[3 * size_t(g) + 1] = make_float4(C[0], C[1], C[2], C[3]);
+    b.qc[3 * size_t(g) + 2] = make_float4(C[4], C[5], 0.0f, 0.0f);
     }
     wave_fetch_add(&maps[s].ctr->used_entries, ins);
   }
@@ -1282,7 +1297,6 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_heavy(const MapDev* maps
       unpack_key(st.key, kx, ky, kz);
       const double center[3] = {(double(kx) + 0.5) * m.vs64, (double(ky) + 0.5) * m.vs64,
                                 (double(kz) + 0.5) * m.vs64};
-      const UpdParams& P = *S.prm;
       uint32_t flags = st.flags;
       int n = st.n, newly = st.newly_add;
       int merge_pos = kNever, freeze_t = -1;
@@ -1293,13 +1307,13 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_heavy(const MapDev* maps
       // reference's: add the point, refit, then the freeze check.
       int pos = start;
       bool active = true;
-      // prefetch of the next point (its index, then its coordinates)
-      int i_nx = pos < end ? b.sval_out[pos] - S.off : 0;
-      float x_nx = 0.f, y_nx = 0.f, z_nx = 0.f;
+      // prefetch of the next point (its index, then its q and C)
+      int g_nx = pos < end ? b.sval_out[pos] : 0;
+      float4 qa_nx = make_float4(0.f, 0.f, 0.f, 0.f), qb_nx = qa_nx, qc_nx = qa_nx;
       if (pos < end) {
-        x_nx = S.x[i_nx];
-        y_nx = S.y[i_nx];
-        z_nx = S.z[i_nx];
+        qa_nx = b.qc[3 * size_t(g_nx)];
+        qb_nx = b.qc[3 * size_t(g_nx) + 1];
+        qc_nx = b.qc[3 * size_t(g_nx) + 2];
       }
       while (__any(active)) {
         bool need = false;
@@ -1318,22 +1332,17 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_heavy(const MapDev* maps
             active = false;
             break;
           }
-          const int i = i_nx;
-          const float px = x_nx, py = y_nx, pz0 = z_nx;
+          const int i = g_nx - S.off;
+          const float4 qa = qa_nx, qb = qb_nx, qcc = qc_nx;
           if (pos + 1 < end) {
-            i_nx = b.sval_out[pos + 1] - S.off;
-            x_nx = S.x[i_nx];
-            y_nx = S.y[i_nx];
-            z_nx = S.z[i_nx];
+            g_nx = b.sval_out[pos + 1];
+            qa_nx = b.qc[3 * size_t(g_nx)];
+            qb_nx = b.qc[3 * size_t(g_nx) + 1];
+            qc_nx = b.qc[3 * size_t(g_nx) + 2];
           }
           ++pos;
-          float pw[3];
-          ins_transform(P.m, P.t, px, py, pz0, pw);
-          const float qv[3] = {float(double(pw[0]) - center[0]), float(double(pw[1]) - center[1]),
-                               float(double(pw[2]) - center[2])};
-          const float pz = pz0 == 0.0f ? 0.001f : pz0;
-          float C[6];
-          point_world_cov(P.rwl32, P.Prr, P.Ppp, P.s2, P.rv, px, py, pz, C);
+          const float qv[3] = {qa.x, qa.y, qa.z};
+          const float C[6] = {qb.x, qb.y, qb.z, qb.w, qcc.x, qcc.y};
           mom_add(M, qv, C);
           ++n;
           if (S.is_build) continue;
@@ -1401,7 +1410,6 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_light(const MapDev* maps
       }
     } else if (vid >= 0) {
       const SessInfo& S = si[s];
-      const UpdParams& P = *S.prm;
       const int start = b.seg_start[q], end = b.seg_end[q];
       VoxState st = m.vs[vid];
       int64_t kx, ky, kz;
@@ -1419,7 +1427,7 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_light(const MapDev* maps
       uint32_t flags = st.flags;
       int n = st.n, newly = st.newly_add;
       int merge_pos = kNever, freeze_t = -1;
-      int i_nx = start < end ? b.sval_out[start] - S.off : 0;
+      int g_nx = start < end ? b.sval_out[start] : 0;
       for (int pos = start; pos < end; ++pos) {
         if ((flags & kInit) && !(flags & kEnable) && !S.is_build) {
           if (flags & kPlane) {
@@ -1430,16 +1438,12 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_light(const MapDev* maps
           }
           break;
         }
-        const int i = i_nx;
-        if (pos + 1 < end) i_nx = b.sval_out[pos + 1] - S.off;
-        const float px = S.x[i], py = S.y[i], pz0 = S.z[i];
-        float pw[3];
-        ins_transform(P.m, P.t, px, py, pz0, pw);
-        const float qv[3] = {float(double(pw[0]) - center[0]), float(double(pw[1]) - center[1]),
-                             float(double(pw[2]) - center[2])};
-        const float pz = pz0 == 0.0f ? 0.001f : pz0;
-        float C[6];
-        point_world_cov(P.rwl32, P.Prr, P.Ppp, P.s2, P.rv, px, py, pz, C);
+        const int g = g_nx;
+        const int i = g - S.off;
+        if (pos + 1 < end) g_nx = b.sval_out[pos + 1];
+        const float4 qa = b.qc[3 * size_t(g)], qb = b.qc[3 * size_t(g) + 1], qcc = b.qc[3 * size_t(g) + 2];
+        const float qv[3] = {qa.x, qa.y, qa.z};
+        const float C[6] = {qb.x, qb.y, qb.z, qb.w, qcc.x, qcc.y};
         if (cnt < m.pend_cap) {
           for (int a = 0; a < 3; ++a) pd[cnt * kPendF + a] = qv[a];
           for (int a = 0; a < 6; ++a) pd[cnt * kPendF + 3 + a] = C[a];
