@@ -23,6 +23,7 @@
 #include "vmp_observe.hpp"
 #include "vmp_imu.hpp"
 #include "vmp_prim.hpp"
+static_assert(vmp::kCsThreads == vmp::kPrimThreads, "the candidate starts run as the tail of a compaction tile");
 
 using namespace vmp;
 
@@ -1583,13 +1584,26 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         // become a contiguous segment in point order, sessions stay in order
         if (!sort_pairs(b.skey_in, b.sval_in, b.skey_out, b.sval_out, hbits + sbits)) return VMP_EDEVICE;
         hipLaunchKernelGGL(k_seg_heads, dim3(gn), dim3(T), 0, st, S, n, b);
-        if (!select_flags(b.flags, b.sel, b.cnt + 0)) return VMP_EDEVICE;
-        hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 0, 0);
-        hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.flags, 0, ntot);
+        // the library's compaction also clears the flags it reads, computes
+        // the per-session ranges in its last tile and (creation events)
+        // splits each event into its time and segment
+        if (prim) {
+          select_stable(st, ntot, PredFlagClear{b.flags}, OutIndex{b.sel}, b.prim, b.cnt + 0,
+                        TailRanges{M, S, n, b.sel, 0});
+        } else {
+          if (!select_flags(b.flags, b.sel, b.cnt + 0)) return VMP_EDEVICE;
+          hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 0, 0);
+          hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.flags, 0, ntot);
+        }
         hipLaunchKernelGGL(k_seg_info, dim3(gn), dim3(T), 0, st, M, S, n, b);
-        if (!select_flags(b.flags, b.sel, b.cnt + 1)) return VMP_EDEVICE;
-        hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 1, 1);
-        hipLaunchKernelGGL(k_cre_split, dim3(gn), dim3(T), 0, st, S, n, b);
+        if (prim) {
+          select_stable(st, ntot, PredFlagClear{b.flags}, OutCre{b.sel, b.cre_t, b.cre_seg, S, n}, b.prim,
+                        b.cnt + 1, TailRanges{M, S, n, b.sel, 1});
+        } else {
+          if (!select_flags(b.flags, b.sel, b.cnt + 1)) return VMP_EDEVICE;
+          hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 1, 1);
+          hipLaunchKernelGGL(k_cre_split, dim3(gn), dim3(T), 0, st, S, n, b);
+        }
         hipLaunchKernelGGL(k_create_fast, dim3(gn), dim3(T), 0, st, M, S, n, b);
         hipLaunchKernelGGL(k_create_fast_ctr, dim3(n), dim3(1), 0, st, M, n);
         timed_launch(c, kTkRingLive,
@@ -1608,36 +1622,47 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
                      [&] { hipLaunchKernelGGL(k_replay_heavy, dim3(std::min(gn, 256)), dim3(T), 0, st, M, S, b); });
         timed_launch(c, kTkReplayLight, [&] { hipLaunchKernelGGL(k_replay_light, dim3(gn), dim3(T), 0, st, M, S, b); });
         if (!(mode & kStepBuild)) {
-          hipLaunchKernelGGL(k_fill_u64, dim3(gn), dim3(T), 0, st, b.cand_in, ~0ull, ntot);
+          hipLaunchKernelGGL(k_fill_cand, dim3(gn), dim3(T), 0, st, b, ntot);
           timed_launch(c, kTkCandidates, [&] { hipLaunchKernelGGL(k_candidates, dim3(gn), dim3(T), 0, st, M, S, b); });
           hipLaunchKernelGGL(k_dirty_untouched, dim3(gn), dim3(T), 0, st, M, b);
           // the call at point time t is made by the voxel that received point
           // t, so each candidate sits at its session offset plus t: the
           // non-empty slots in order are the list sorted on (session, time)
           if (prim) {
-            select_stable(st, ntot, PredCand{b.cand_in}, OutCand{b.cand_in, b.cand_out}, b.prim, b.cnt + 7);
+            select_stable(st, ntot, PredCand{b.cand_in}, OutCand{b.cand_in, b.cand_out}, b.prim, b.cnt + 7,
+                          TailCandStarts{M, n, b});
           } else {
             size_t tb = w.temp_bytes;
             if (rocprim::radix_sort_keys(w.temp, tb, b.cand_in, b.cand_out, size_t(ntot), 25, b.cand_sshift + sbits,
                                          st) != hipSuccess)
               return VMP_EDEVICE;
+            hipLaunchKernelGGL(k_cand_starts, dim3(1), dim3(kCsThreads), 0, st, M, n, b);
           }
-          hipLaunchKernelGGL(k_cand_starts, dim3(1), dim3(kCsThreads), 0, st, M, n, b);
-          hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.cand_spec, 0, ntot);
           timed_launch(c, kTkSpecEval, [&] { hipLaunchKernelGGL(k_spec_eval, dim3(gn), dim3(T), 0, st, M, b); });
-          if (!select_flags(b.cand_spec, b.spl, b.cnt + 4)) return VMP_EDEVICE;
-          hipLaunchKernelGGL(k_spl_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b);
-          hipLaunchKernelGGL(k_spl_keys, dim3(gn), dim3(T), 0, st, b);
+          if (prim) {
+            select_stable(st, ntot, PredFlag{b.cand_spec}, OutSpl{b.spl, b.cand_in, b.cand_out, b.cand_sshift},
+                          b.prim, b.cnt + 4, TailSplRanges{M, S, n, b});
+          } else {
+            if (!select_flags(b.cand_spec, b.spl, b.cnt + 4)) return VMP_EDEVICE;
+            hipLaunchKernelGGL(k_spl_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b);
+            hipLaunchKernelGGL(k_spl_keys, dim3(gn), dim3(T), 0, st, b);
+          }
           hipLaunchKernelGGL(k_spl_prefetch, dim3(std::min(gn * kPreItems, 4096)), dim3(T), 0, st, M, S, b);
           timed_launch(c, kTkMergeReplay,
                        [&] { hipLaunchKernelGGL(k_merge_replay, dim3(n), dim3(kMrThreads), 0, st, M, S, n, b); });
         }
-        hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.flags, 0, ntot);
+        // (the library's compaction of the creation events left the flags cleared)
+        if (!prim) hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.flags, 0, ntot);
         hipLaunchKernelGGL(k_finalize_segs, dim3(gn), dim3(T), 0, st, M, S, b);
         hipLaunchKernelGGL(k_evict_finalize, dim3(gn), dim3(T), 0, st, M, S, n, b);
         hipLaunchKernelGGL(k_evict_finalize_ctr, dim3(n), dim3(1), 0, st, M, n);
-        if (!select_flags(b.flags, b.sel, b.cnt + 2)) return VMP_EDEVICE;
-        hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 2, 2);
+        if (prim) {
+          select_stable(st, ntot, PredFlag{b.flags}, OutIndex{b.sel}, b.prim, b.cnt + 2,
+                        TailRanges{M, S, n, b.sel, 2});
+        } else {
+          if (!select_flags(b.flags, b.sel, b.cnt + 2)) return VMP_EDEVICE;
+          hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 2, 2);
+        }
         hipLaunchKernelGGL(k_lru_append, dim3(gn), dim3(T), 0, st, M, S, n, b);
       }
       hipLaunchKernelGGL(k_end_scan, dim3(n), dim3(1), 0, st, M, S, n);
@@ -1727,6 +1752,12 @@ static int maintain(vmp_ctx* ex, vmp_session* s, int32_t* did) {
     VMP_HIP(hipStreamSynchronize(st));
     s->h_ctr = cc;
     *did |= 2;
+    if (t.fixed) {
+      // these arrays share the device scratch with the workspace of
+      // vmp_register_scan_noalloc, whose primitives' control words must
+      // start from zero at its next step
+      VMP_HIP(hipMemsetAsync(ex->ws.b.prim, 0, prim_ws_entries(ex->ws.cap_points) * sizeof(int32_t), st));
+    }
   }
   return VMP_OK;
 }

@@ -345,17 +345,11 @@ __global__ void k_begin_scan(const MapDev* maps, int nsess) {
 __global__ void k_fill_i32(int32_t* p, int32_t v, int n) {
   VMP_GRID_LOOP(i, n) p[i] = v;
 }
-__global__ void k_fill_u64(uint64_t* p, uint64_t v, int n) {
-  VMP_GRID_LOOP(i, n) p[i] = v;
-}
 
-// per-session [lo, hi) of an ascending list of global indices
+// per-session [lo, hi) of an ascending list of n global indices
 // which: 0 segments, 1 creation events, 2 LRU appends
-__global__ void k_ranges(const MapDev* maps, SessInfo* si, int nsess, const int32_t* list, const int32_t* count,
-                         int which) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsess) return;
-  const int n = *count;
+__device__ inline void session_range(const MapDev* maps, SessInfo* si, int s, const int32_t* list, int n,
+                                     int which) {
   auto lb = [&](int v) {
     int lo = 0, hi = n;
     while (lo < hi) {
@@ -381,6 +375,23 @@ __global__ void k_ranges(const MapDev* maps, SessInfo* si, int nsess, const int3
     c->nlru = hi - lo;
   }
 }
+__global__ void k_ranges(const MapDev* maps, SessInfo* si, int nsess, const int32_t* list, const int32_t* count,
+                         int which) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nsess) session_range(maps, si, s, list, *count, which);
+}
+// the same as the tail of the compaction that produced the list
+struct TailRanges {
+  const MapDev* maps;
+  SessInfo* si;
+  int nsess;
+  const int32_t* list;
+  int which;
+  static constexpr bool kActive = true;
+  __device__ void operator()(int n) const {
+    for (int s = threadIdx.x; s < nsess; s += blockDim.x) session_range(maps, si, s, list, n, which);
+  }
+};
 
 // ------------------------------------------------------------------ K1 ----
 __global__ void k_ins_keys(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
@@ -468,6 +479,20 @@ __global__ void k_cre_split(const SessInfo* si, int nsess, BatchBufs b) {
     b.cre_seg[r] = b.flags[g] - 1;
   }
 }
+// the same as the output of the compaction of the creation events (v: the
+// event's flag, its segment + 1)
+struct OutCre {
+  int32_t* sel;
+  int32_t* cre_t;
+  int32_t* cre_seg;
+  const SessInfo* si;
+  int nsess;
+  __device__ void operator()(int r, int g, int32_t v) const {
+    sel[r] = g;
+    cre_t[r] = g - si[sess_of(si, nsess, g)].off;
+    cre_seg[r] = v - 1;
+  }
+};
 
 // ------------------------------------------------------------------ K5 ----
 // Parallel creation for sessions whose LRU cannot overflow in this scan.
@@ -1565,7 +1590,7 @@ __global__ void k_end_scan(const MapDev* maps, const SessInfo* si, int nsess) {
 // (one workgroup: every session's count is loaded at once, then an
 // exclusive scan in LDS, kCsThreads sessions per round)
 constexpr int kCsThreads = 256;
-__global__ void __launch_bounds__(kCsThreads) k_cand_starts(const MapDev* maps, int nsess, BatchBufs b) {
+__device__ inline void cand_starts_block(const MapDev* maps, int nsess, const BatchBufs& b) {
   __shared__ int sc[kCsThreads];
   __shared__ int carry;
   const int tid = threadIdx.x;
@@ -1587,6 +1612,25 @@ __global__ void __launch_bounds__(kCsThreads) k_cand_starts(const MapDev* maps, 
     __syncthreads();
   }
   if (tid == 0) b.cnt[3] = carry;
+}
+__global__ void __launch_bounds__(kCsThreads) k_cand_starts(const MapDev* maps, int nsess, BatchBufs b) {
+  cand_starts_block(maps, nsess, b);
+}
+// the same as the tail of the candidate compaction (a block of kCsThreads)
+struct TailCandStarts {
+  const MapDev* maps;
+  int nsess;
+  BatchBufs b;
+  static constexpr bool kActive = true;
+  __device__ void operator()(int) const { cand_starts_block(maps, nsess, b); }
+};
+
+// merge candidates' key list cleared and their speculative results zeroed
+__global__ void k_fill_cand(BatchBufs b, int n) {
+  VMP_GRID_LOOP(i, n) {
+    b.cand_in[i] = ~0ull;
+    b.cand_spec[i] = 0;
+  }
 }
 
 // ---------------------------------------------------- housekeeping -----

@@ -221,12 +221,12 @@ __global__ void k_dirty_untouched(const MapDev* maps, BatchBufs b) {
 // the filled slots of the dense candidate array, and their compaction
 struct PredCand {
   const uint64_t* c;
-  __device__ bool operator()(int i) const { return c[i] != ~uint64_t(0); }
+  __device__ int32_t operator()(int i) const { return c[i] != ~uint64_t(0) ? 1 : 0; }
 };
 struct OutCand {
   const uint64_t* in;
   uint64_t* out;
-  __device__ void operator()(int k, int i) const { out[k] = in[i]; }
+  __device__ void operator()(int k, int i, int32_t) const { out[k] = in[i]; }
 };
 
 // ------------------------------------------------------------------ K8 ----
@@ -264,10 +264,7 @@ __global__ void k_spl_keys(BatchBufs b) {
   VMP_GRID_LOOP(k, n) b.cand_in[k] = cand_local(b.cand_out[b.spl[k]], b.cand_sshift);
 }
 
-__global__ void k_spl_ranges(const MapDev* maps, SessInfo* si, int nsess, BatchBufs b) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsess) return;
-  const int n = b.cnt[4];
+__device__ inline void spl_range(const MapDev* maps, SessInfo* si, int s, const BatchBufs& b, int n) {
   auto lb = [&](int v) {
     int lo = 0, hi = n;
     while (lo < hi) {
@@ -281,6 +278,32 @@ __global__ void k_spl_ranges(const MapDev* maps, SessInfo* si, int nsess, BatchB
   si[s].spl_lo = lb(c0);
   si[s].spl_hi = lb(c0 + maps[s].ctr->ncand);
 }
+__global__ void k_spl_ranges(const MapDev* maps, SessInfo* si, int nsess, BatchBufs b) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nsess) spl_range(maps, si, s, b, b.cnt[4]);
+}
+// k_spl_keys and k_spl_ranges as the output and the tail of the compaction
+// of the acting candidates
+struct OutSpl {
+  int32_t* spl;
+  uint64_t* keys;
+  const uint64_t* cand;
+  int sshift;
+  __device__ void operator()(int k, int p, int32_t) const {
+    spl[k] = p;
+    keys[k] = cand_local(cand[p], sshift);
+  }
+};
+struct TailSplRanges {
+  const MapDev* maps;
+  SessInfo* si;
+  int nsess;
+  BatchBufs b;
+  static constexpr bool kActive = true;
+  __device__ void operator()(int n) const {
+    for (int s = threadIdx.x; s < nsess; s += blockDim.x) spl_range(maps, si, s, b, n);
+  }
+};
 
 // The merge-independent part of the taint phase of every acting candidate,
 // in parallel ahead of the replay (see BatchBufs::pre_x).

@@ -64,14 +64,39 @@ __device__ inline int take_ticket(PrimCtl* ctl, int nt) {
   if (t == nt - 1) atomicExch(&ctl->ticket, 0);
   return t;
 }
-// exclusive prefix of a tile's slot from the predecessors' status words at
-// stride `stride` (slot + j * stride for tile j)
+// Look-back: the exclusive prefix of a tile from its predecessors' status
+// words, read a window at a time (the words of one window are loaded
+// together, so a chain of aggregates costs one memory round trip per window
+// rather than one per tile).  The ready words nearest the tile are consumed
+// up to the first inclusive one; a window whose nearest word is not yet
+// published is read again.  A bounded wait that runs out sets ctl->err.
+//
+// One thread per slot, a window of kLbWin predecessors (the radix passes:
+// slot = digit, stride 256)
+constexpr int kLbWin = 8;
 __device__ inline uint32_t look_back(const uint32_t* status, int tile, int stride, int slot, PrimCtl* ctl) {
   uint32_t prefix = 0;
   int j = tile - 1, spins = 0;
   while (j >= 0) {
-    const uint32_t w = st_load(status + size_t(j) * stride + slot);
-    if ((w >> 30) == 0) {
+    uint32_t w[kLbWin];
+#pragma unroll
+    for (int k = 0; k < kLbWin; ++k) w[k] = j - k >= 0 ? st_load(status + size_t(j - k) * stride + slot) : kStInc;
+    bool stop = false, done = false;
+    int adv = 0;
+#pragma unroll
+    for (int k = 0; k < kLbWin; ++k) {
+      if (!stop) {
+        if ((w[k] >> 30) == 0) {
+          stop = true;
+        } else {
+          prefix += w[k] & kStVal;
+          ++adv;
+          if ((w[k] & kStInc) == kStInc) stop = done = true;
+        }
+      }
+    }
+    if (done) break;
+    if (adv == 0) {
       if (++spins > kSpinMax) {
         atomicOr(&ctl->err, 1);
         break;
@@ -79,19 +104,58 @@ __device__ inline uint32_t look_back(const uint32_t* status, int tile, int strid
       __builtin_amdgcn_s_sleep(1);
       continue;
     }
-    prefix += w & kStVal;
-    if ((w & kStInc) == kStInc) break;
-    --j;
+    j -= adv;
+  }
+  return prefix;
+}
+__device__ inline uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += uint32_t(__shfl_xor(int(v), o, 64));
+  return v;
+}
+// One wave, lane l reading predecessor j - l (the compaction: one word per
+// tile); every lane of the wave calls it and gets the prefix.
+__device__ inline uint32_t look_back_wave(const uint32_t* status, int tile, PrimCtl* ctl) {
+  const int lane = threadIdx.x & 63;
+  uint32_t prefix = 0;
+  int j = tile - 1, spins = 0;
+  while (j >= 0) {
+    const int idx = j - lane;
+    const uint32_t w = idx >= 0 ? st_load(status + idx) : kStInc;  // before tile 0: an inclusive 0
+    const uint64_t nready = __ballot((w >> 30) == 0);
+    const uint64_t inc = __ballot((w & kStInc) == kStInc);
+    const int nr = nready ? __ffsll((unsigned long long)nready) - 1 : 64;
+    const int fi = inc ? __ffsll((unsigned long long)inc) - 1 : 64;
+    if (fi < nr) {
+      prefix += wave_sum_u32(lane <= fi ? (w & kStVal) : 0u);
+      break;
+    }
+    if (nr == 0) {
+      if (++spins > kSpinMax) {
+        if (lane == 0) atomicOr(&ctl->err, 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    prefix += wave_sum_u32(lane < nr ? (w & kStVal) : 0u);
+    j -= nr;
   }
   return prefix;
 }
 // end of a tile: the last to finish clears `nclear` status words (and the
 // extra words, if any) and the done counter; returns true in that tile
-// (every thread of it)
+// (every thread of it).  With `publish`, each tile's stores are made visible
+// device-wide before it counts itself done, and the last tile acquires them,
+// so it may read what every tile wrote.
 __device__ inline bool finish_tile(PrimCtl* ctl, int nt, uint32_t* status, int nclear, int32_t* extra, int nextra,
-                                   int* s_last) {
+                                   int* s_last, bool publish = false) {
   __syncthreads();
-  if (threadIdx.x == 0) *s_last = atomicAdd(&ctl->done, 1) == nt - 1;
+  if (threadIdx.x == 0) {
+    if (publish) __threadfence();
+    *s_last = atomicAdd(&ctl->done, 1) == nt - 1;
+    if (publish && *s_last) __threadfence();
+  }
   __syncthreads();
   if (!*s_last) return false;
   for (int i = threadIdx.x; i < nclear; i += blockDim.x) status[i] = 0u;
@@ -101,9 +165,19 @@ __device__ inline bool finish_tile(PrimCtl* ctl, int nt, uint32_t* status, int n
   return true;
 }
 
-// ---- stable compaction: out(k, i) for the k-th i < n (ascending) with pred(i)
-template <class Pred, class Out>
-__global__ void __launch_bounds__(kPrimThreads) k_select1(int n, Pred pred, Out out, int32_t* count, PrimWs ws) {
+// no work after the compaction
+struct NoTail {
+  static constexpr bool kActive = false;
+  __device__ void operator()(int) const {}
+};
+
+// ---- stable compaction: for the k-th i < n (ascending) whose pred(i) = v is
+// non-zero, out(k, i, v).  pred is called once per item and may clear what it
+// read.  tail(count), if active, runs in every thread of the last tile to
+// finish, after the outputs of all tiles are visible.
+template <class Pred, class Out, class Tail>
+__global__ void __launch_bounds__(kPrimThreads) k_select1(int n, Pred pred, Out out, Tail tail, int32_t* count,
+                                                          PrimWs ws) {
   using Scan = rocprim::block_scan<int32_t, kPrimThreads>;
   __shared__ typename Scan::storage_type st;
   __shared__ int s_tile, s_last;
@@ -113,52 +187,63 @@ __global__ void __launch_bounds__(kPrimThreads) k_select1(int n, Pred pred, Out 
   __syncthreads();
   const int tile = s_tile;
   const int base = tile * kPrimTile + threadIdx.x * kPrimIpt;
-  bool f[kPrimIpt];
+  int32_t v[kPrimIpt];
   int32_t c = 0;
 #pragma unroll
   for (int k = 0; k < kPrimIpt; ++k) {
-    f[k] = base + k < n && pred(base + k);
-    c += f[k] ? 1 : 0;
+    v[k] = base + k < n ? pred(base + k) : 0;
+    c += v[k] != 0 ? 1 : 0;
   }
   int32_t pos = 0, agg = 0;
   Scan().exclusive_scan(c, pos, 0, agg, st);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < 64) {
+    const bool l0 = threadIdx.x == 0;
     uint32_t prefix = 0;
     if (tile == 0) {
-      st_store(ws.status, kStInc | uint32_t(agg));
+      if (l0) st_store(ws.status, kStInc | uint32_t(agg));
     } else {
-      st_store(ws.status + tile, kStAgg | uint32_t(agg));
-      prefix = look_back(ws.status, tile, 1, 0, ws.ctl);
-      st_store(ws.status + tile, kStInc | (prefix + uint32_t(agg)));
+      if (l0) st_store(ws.status + tile, kStAgg | uint32_t(agg));
+      prefix = look_back_wave(ws.status, tile, ws.ctl);
+      if (l0) st_store(ws.status + tile, kStInc | (prefix + uint32_t(agg)));
     }
-    s_prefix = prefix;
+    if (l0) s_prefix = prefix;
   }
   __syncthreads();
   pos += int32_t(s_prefix);
 #pragma unroll
   for (int k = 0; k < kPrimIpt; ++k)
-    if (f[k]) out(pos++, base + k);
+    if (v[k] != 0) out(pos++, base + k, v[k]);
   if (threadIdx.x == 0 && tile == nt - 1) *count = int32_t(s_prefix) + agg;
-  finish_tile(ws.ctl, nt, ws.status, nt, nullptr, 0, &s_last);
+  if (finish_tile(ws.ctl, nt, ws.status, nt, nullptr, 0, &s_last, Tail::kActive) && Tail::kActive) tail(*count);
 }
 
 // flags[i] != 0 -> the ascending indices, as rocprim::select over a counting
 // iterator would give them
 struct PredFlag {
   const int32_t* f;
-  __device__ bool operator()(int i) const { return f[i] != 0; }
+  __device__ int32_t operator()(int i) const { return f[i]; }
+};
+// the same, leaving the flags it read cleared
+struct PredFlagClear {
+  int32_t* f;
+  __device__ int32_t operator()(int i) const {
+    const int32_t v = f[i];
+    if (v != 0) f[i] = 0;
+    return v;
+  }
 };
 struct OutIndex {
   int32_t* o;
-  __device__ void operator()(int k, int i) const { o[k] = i; }
+  __device__ void operator()(int k, int i, int32_t) const { o[k] = i; }
 };
 
 // stable compaction, one launch; work: prim_ws_entries(n) int32 entries,
 // *count the total
-template <class Pred, class Out>
-inline void select_stable(hipStream_t st, int n, Pred pred, Out out, int32_t* work, int32_t* count) {
+template <class Pred, class Out, class Tail = NoTail>
+inline void select_stable(hipStream_t st, int n, Pred pred, Out out, int32_t* work, int32_t* count,
+                          Tail tail = Tail()) {
   const int nt = std::max(1, prim_tiles(n));
-  hipLaunchKernelGGL((k_select1<Pred, Out>), dim3(nt), dim3(kPrimThreads), 0, st, n, pred, out, count,
+  hipLaunchKernelGGL((k_select1<Pred, Out, Tail>), dim3(nt), dim3(kPrimThreads), 0, st, n, pred, out, tail, count,
                      prim_ws(work));
 }
 
