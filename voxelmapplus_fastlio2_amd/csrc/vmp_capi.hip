@@ -1853,7 +1853,7 @@ static int prepare_scans(vmp_ctx* c, vmp_session** ss, int n, bool is_build) {
   if (npre == 0) return VMP_OK;
   if (hipMemcpyAsync(w.d_pre, w.h_pre, sizeof(PreJob) * npre, hipMemcpyHostToDevice, st) != hipSuccess)
     return VMP_EDEVICE;
-  timed_launch(c, kTkImuPredict, [&] { hipLaunchKernelGGL(k_imu_predict, dim3(npre), dim3(64), 0, st, w.d_pre, npre); });
+  timed_launch(c, kTkImuPredict, [&] { hipLaunchKernelGGL(k_imu_predict, dim3(npre), dim3(kPredThreads), 0, st, w.d_pre, npre); });
   const int gx = int(std::max<int64_t>(1, std::min<int64_t>((most + 255) / 256, 1024)));
   timed_launch(c, kTkUndistort,
                [&] { hipLaunchKernelGGL(k_undistort, dim3(gx, std::min(npre, 65535)), dim3(256), 0, st, w.d_pre, npre); });

@@ -92,9 +92,11 @@ __global__ void __launch_bounds__(64) k_ieskf_begin(FilterDev** fs, int32_t** ob
 // the lanes are summed by the wave (H 12x12 column-major and b as the
 // standalone reduction writes them).  The observe counters are read, then
 // cleared for the next pass.
+// One wave per session: the registers of the whole SIMD are its own
+// (waves_per_eu 1), so a lane keeps several partial rows in flight.
 template <int NJ>
-__global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, const ObsJob* __restrict__ jobs, int32_t** obs_ctr,
-                                                   int32_t** stat_out) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_ieskf_step(FilterDev** fs, const ObsJob* __restrict__ jobs, int32_t** obs_ctr, int32_t** stat_out) {
   constexpr int NH = NJ * (NJ + 1) / 2;
   constexpr int NACC = NH + NJ;
   constexpr int ROW = obs_row<NJ>();
@@ -112,13 +114,28 @@ __global__ void __launch_bounds__(64) k_ieskf_step(FilterDev** fs, const ObsJob*
   for (int i = lane; i < 529; i += 64) Pi[i] = f->Pinv[i];
   {
     const ObsJob& J = jobs[blockIdx.x];
+    // lane l adds rows l, l + 64, l + 128, ... in that order; the loads of
+    // kRows of them are issued together (one memory round trip per group)
+    constexpr int kRows = ROW <= 32 ? 4 : 1;
+    const auto part = as_global(J.partials);
+    const int nb = J.nblocks;
     double fin[ROW];
 #pragma unroll
     for (int k = 0; k < ROW; ++k) fin[k] = 0.0;
-    for (int b = lane; b < J.nblocks; b += 64) {
-      const double* src = J.partials + size_t(b) * ROW;
+    for (int b0 = lane; b0 < nb; b0 += 64 * kRows) {
+      double v[kRows][ROW];
 #pragma unroll
-      for (int k = 0; k < ROW; ++k) fin[k] += src[k];
+      for (int u = 0; u < kRows; ++u) {
+        const int b = min(b0 + 64 * u, nb - 1);  // rows past the end are read but not added
+#pragma unroll
+        for (int k = 0; k < ROW; ++k) v[u][k] = part[size_t(b) * ROW + k];
+      }
+#pragma unroll
+      for (int u = 0; u < kRows; ++u)
+        if (b0 + 64 * u < nb) {
+#pragma unroll
+          for (int k = 0; k < ROW; ++k) fin[k] += v[u][k];
+        }
     }
 #pragma unroll
     for (int k = 0; k < ROW; ++k) {

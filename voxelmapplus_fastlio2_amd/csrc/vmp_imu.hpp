@@ -180,8 +180,9 @@ __global__ void k_imu_init(ImuDev* imu, FilterDev* f, const double* t, const dou
 }
 
 // IMU propagation of one package (IESKF::predict per IMU pair, ieskf.cpp:
-// 83-111, from lio_builder.cpp:80-119): one 64-lane workgroup per session,
-// the pairs in chunks of up to 64 steps, each chunk in four phases.
+// 83-111, from lio_builder.cpp:80-119): one four-wave workgroup per session,
+// the pairs in chunks of up to 64 steps, each chunk in four phases (A-C on
+// the first wave's lanes, D on all four waves).
 //   A  lanes: each step's input (mid-point acc and gyro, dt), its rotation
 //      increment exp((gyro - bg) dt) and -Jr dt (IMU data and the biases,
 //      which do not change while a package is propagated)
@@ -190,7 +191,7 @@ __global__ void k_imu_init(ImuDev* imu, FilterDev* f, const double* t, const dou
 //   C  lanes: each step's blocks that depend on the rotation before it,
 //      (-rot hat(acc - ba)) dt and (-rot) dt, and the IMU pose table entry
 //      at the step's tail
-//   D  all lanes, step after step: P = F P F^T + G Q G^T.  F is the identity
+//   D  all waves, step after step: P = F P F^T + G Q G^T.  F is the identity
 //      except in rows 0-5, 12-14 and 21-22, so F P is formed for those rows
 //      only (its other rows are rows of P) and (F P) F^T for those columns
 //      only (its other columns are columns of F P).  Every sum runs over the
@@ -224,147 +225,206 @@ struct PredShared {
   vmp_state sx;
 };
 
-// the columns (and rows) of F that differ from the identity's
-__device__ constexpr int kFK[17] = {0, 1, 2, 3, 4, 5, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22};
-
-// phase D for step j of the chunk, P in place.  F P: lane c holds column c
-// of P (its 17 entries in the non-identity rows) and forms column c of the 11
-// non-identity rows of F P.  (F P) F^T + G Q G^T: lane r holds row r of F P
-// (a row of P for an identity row of F) and rewrites row r of P: its
-// non-identity columns from the sums, its identity columns from F P (for a
-// non-identity row), and the diagonal of G Q G^T's identity blocks.  Every
-// lane follows the same sequence of operations on its own column or row.
-__device__ inline void predict_cov_step(PredShared& S, int j, int lane) {
+// phase D for step j of the chunk, P in place, on the block's four waves.
+// F P: lane c of every wave holds column c of P (the entries its rows need)
+// and forms column c of some of the 11 non-identity rows of F P: wave 0 rows
+// 0-2, wave 1 rows 3-5, wave 2 rows 12-13, wave 3 rows 14, 21 and 22.
+// (F P) F^T + G Q G^T: lane r of every wave reads row r of F P (a row of P
+// for an identity row of F) into registers; after a barrier each wave
+// rewrites its columns of row r of P (wave 0 columns 0-2, wave 1 columns 3-5,
+// wave 2 columns 12-13, wave 3 columns 14, 21, 22), wave 0 also the identity
+// columns of a non-identity row (from F P) and the diagonal of G Q G^T's
+// identity blocks.  Every entry is the same expression, summed in the same
+// order, whichever wave forms it.
+constexpr int kPredThreads = 256;
+__device__ inline void predict_cov_step(PredShared& S, int j, int w, int l) {
   double* P = S.P;
   const double dt = S.dt[j];
   const double* E = S.E[j];
   const double* Jd = S.J[j];
   const double* Ad = S.A[j];
   const double* Rd = S.Rd[j];
-  const double md[6] = {S.Mx[0] * dt, S.Mx[1] * dt, S.Mx[2] * dt, S.Mx[3] * dt, S.Mx[4] * dt, S.Mx[5] * dt};
-  if (lane < 23) {  // F P
-    const int c = lane;
-    double p[23];
+  double* fp = S.FP;
+  if (l < 23) {  // F P
+    const int c = l;
+    if (w == 0) {  // rows 0-2
 #pragma unroll
-    for (int t = 0; t < 17; ++t) p[kFK[t]] = P[kFK[t] * 23 + c];
-    double* fp = S.FP;
+      for (int r = 0; r < 3; ++r) {
+        double s = P[r * 23 + c];
+        s += dt * P[(12 + r) * 23 + c];
+        fp[r * 23 + c] = s;
+      }
+    } else if (w == 1) {  // rows 3-5: exp(-w) row i = exp(w) column i
+      double p[6];
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      double s = p[r];
-      s += dt * p[12 + r];
-      fp[r * 23 + c] = s;
-    }
+      for (int t = 0; t < 3; ++t) {
+        p[t] = P[(3 + t) * 23 + c];
+        p[3 + t] = P[(15 + t) * 23 + c];
+      }
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {  // rows 3-5: exp(-w) row i = exp(w) column i
-      double s = E[i] * p[3];
-      s += E[3 + i] * p[4];
-      s += E[6 + i] * p[5];
-      s += Jd[i * 3] * p[15];
-      s += Jd[i * 3 + 1] * p[16];
-      s += Jd[i * 3 + 2] * p[17];
-      fp[(3 + i) * 23 + c] = s;
-    }
+      for (int i = 0; i < 3; ++i) {
+        double s = E[i] * p[0];
+        s += E[3 + i] * p[1];
+        s += E[6 + i] * p[2];
+        s += Jd[i * 3] * p[3];
+        s += Jd[i * 3 + 1] * p[4];
+        s += Jd[i * 3 + 2] * p[5];
+        fp[(3 + i) * 23 + c] = s;
+      }
+    } else {  // rows 12-14 (waves 2, 3) and 21-22 (wave 3)
+      const double md[6] = {S.Mx[0] * dt, S.Mx[1] * dt, S.Mx[2] * dt, S.Mx[3] * dt, S.Mx[4] * dt, S.Mx[5] * dt};
+      double p[3], q[3], u[5];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {  // rows 12-14
-      double s = Ad[i * 3] * p[3];
-      s += Ad[i * 3 + 1] * p[4];
-      s += Ad[i * 3 + 2] * p[5];
-      s += p[12 + i];
-      s += Rd[i * 3] * p[18];
-      s += Rd[i * 3 + 1] * p[19];
-      s += Rd[i * 3 + 2] * p[20];
-      s += md[i * 2] * p[21];
-      s += md[i * 2 + 1] * p[22];
-      fp[(6 + i) * 23 + c] = s;
-    }
+      for (int t = 0; t < 3; ++t) {
+        p[t] = P[(3 + t) * 23 + c];
+        q[t] = P[(12 + t) * 23 + c];
+      }
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {  // rows 21-22
-      double s = S.NM[i * 2] * p[21];
-      s += S.NM[i * 2 + 1] * p[22];
-      fp[(9 + i) * 23 + c] = s;
+      for (int t = 0; t < 5; ++t) u[t] = P[(18 + t) * 23 + c];
+      const int i0 = w == 2 ? 0 : 2, i1 = w == 2 ? 2 : 3;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {  // rows 12-14
+        if (i < i0 || i >= i1) continue;
+        double s = Ad[i * 3] * p[0];
+        s += Ad[i * 3 + 1] * p[1];
+        s += Ad[i * 3 + 2] * p[2];
+        s += q[i];
+        s += Rd[i * 3] * u[0];
+        s += Rd[i * 3 + 1] * u[1];
+        s += Rd[i * 3 + 2] * u[2];
+        s += md[i * 2] * u[3];
+        s += md[i * 2 + 1] * u[4];
+        fp[(6 + i) * 23 + c] = s;
+      }
+      if (w == 3) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {  // rows 21-22
+          double s = S.NM[i * 2] * u[3];
+          s += S.NM[i * 2 + 1] * u[4];
+          fp[(9 + i) * 23 + c] = s;
+        }
+      }
     }
   }
   __syncthreads();
-  if (lane < 23) {  // (F P) F^T + G Q G^T
-    const int r = lane;
-    const int rs = fp_slot(r);
-    const double* fr = rs >= 0 ? &S.FP[rs * 23] : &P[r * 23];
-    double f[23];
+  // (F P) F^T + G Q G^T: every wave first reads the entries of row r it needs
+  const int r = l;
+  const int rs = r < 23 ? fp_slot(r) : -1;
+  const double* fr = rs >= 0 ? &S.FP[rs * 23] : &P[(r < 23 ? r : 0) * 23];
+  double f[11];
+  if (r < 23) {
+    if (w == 0) {
 #pragma unroll
-    for (int t = 0; t < 17; ++t) f[kFK[t]] = fr[kFK[t]];
-    double o[23];
+      for (int t = 0; t < 3; ++t) {
+        f[t] = fr[t];
+        f[3 + t] = fr[12 + t];
+      }
+    } else if (w == 1) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      double s = f[c];
-      s += f[12 + c] * dt;
-      o[c] = s;
+      for (int t = 0; t < 3; ++t) {
+        f[t] = fr[3 + t];
+        f[3 + t] = fr[15 + t];
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < 3; ++t) {
+        f[t] = fr[3 + t];
+        f[3 + t] = fr[12 + t];
+      }
+#pragma unroll
+      for (int t = 0; t < 5; ++t) f[6 + t] = fr[18 + t];
     }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      double s = f[3] * E[i];
-      s += f[4] * E[3 + i];
-      s += f[5] * E[6 + i];
-      s += f[15] * Jd[i * 3];
-      s += f[16] * Jd[i * 3 + 1];
-      s += f[17] * Jd[i * 3 + 2];
-      o[3 + i] = s;
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      double s = f[3] * Ad[i * 3];
-      s += f[4] * Ad[i * 3 + 1];
-      s += f[5] * Ad[i * 3 + 2];
-      s += f[12 + i];
-      s += f[18] * Rd[i * 3];
-      s += f[19] * Rd[i * 3 + 1];
-      s += f[20] * Rd[i * 3 + 2];
-      s += f[21] * md[i * 2];
-      s += f[22] * md[i * 2 + 1];
-      o[12 + i] = s;
-    }
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      double s = f[21] * S.NM[i * 2];
-      s += f[22] * S.NM[i * 2 + 1];
-      o[21 + i] = s;
-    }
+  }
+  __syncthreads();  // every read of P's row r comes before the writes below
+  if (r < 23) {
+    double* pr = &P[r * 23];
     const int gr = g_group(r);
-    if (gr == 0 || gr == 1) {  // G Q G^T on the blocks 3-5 and 12-14
-      const double q = S.q[gr];
-      const double* G = gr == 0 ? Jd : Rd;
-      const double* Gr = &G[(r - (gr == 0 ? 3 : 12)) * 3];
-      const int c0 = gr == 0 ? 3 : 12;
+    if (w == 0) {  // columns 0-2
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double s = f[c];
+        s += f[3 + c] * dt;
+        pr[c] = s;
+      }
+      if (rs >= 0) {  // identity columns of a non-identity row: F P's entries
+#pragma unroll
+        for (int c = 6; c < 12; ++c) pr[c] = fr[c];
+#pragma unroll
+        for (int c = 15; c < 21; ++c) pr[c] = fr[c];
+      } else if (gr >= 2) {  // identity row and column: only G Q G^T's diagonal
+        pr[r] = pr[r] + (dt * S.q[gr]) * dt;
+      }
+    } else if (w == 1) {  // columns 3-5 (+ G Q G^T's block 3-5)
+      double o[3];
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        double g = (Gr[0] * q) * G[i * 3];
-        g += (Gr[1] * q) * G[i * 3 + 1];
-        g += (Gr[2] * q) * G[i * 3 + 2];
-        o[c0 + i] = o[c0 + i] + g;
+        double s = f[0] * E[i];
+        s += f[1] * E[3 + i];
+        s += f[2] * E[6 + i];
+        s += f[3] * Jd[i * 3];
+        s += f[4] * Jd[i * 3 + 1];
+        s += f[5] * Jd[i * 3 + 2];
+        o[i] = s;
       }
-    }
-    double* pr = &P[r * 23];
+      if (gr == 0) {
+        const double q = S.q[0];
+        const double* Gr = &Jd[(r - 3) * 3];
 #pragma unroll
-    for (int t = 0; t < 17; ++t)
-      if (kFK[t] < 6 || (kFK[t] >= 12 && kFK[t] < 15) || kFK[t] >= 21) pr[kFK[t]] = o[kFK[t]];
-    if (rs >= 0) {  // identity columns of a non-identity row: F P's entries
+        for (int i = 0; i < 3; ++i) {
+          double g = (Gr[0] * q) * Jd[i * 3];
+          g += (Gr[1] * q) * Jd[i * 3 + 1];
+          g += (Gr[2] * q) * Jd[i * 3 + 2];
+          o[i] = o[i] + g;
+        }
+      }
 #pragma unroll
-      for (int c = 6; c < 12; ++c) pr[c] = fr[c];
+      for (int i = 0; i < 3; ++i) pr[3 + i] = o[i];
+    } else {  // columns 12-13 (wave 2), 14, 21, 22 (wave 3) (+ G Q G^T's block 12-14)
+      const double md[6] = {S.Mx[0] * dt, S.Mx[1] * dt, S.Mx[2] * dt, S.Mx[3] * dt, S.Mx[4] * dt, S.Mx[5] * dt};
+      const int i0 = w == 2 ? 0 : 2, i1 = w == 2 ? 2 : 3;
 #pragma unroll
-      for (int c = 15; c < 21; ++c) pr[c] = fr[c];
-    } else if (gr >= 2) {  // identity row and column: only G Q G^T's diagonal
-      pr[r] = pr[r] + (dt * S.q[gr]) * dt;
+      for (int i = 0; i < 3; ++i) {
+        if (i < i0 || i >= i1) continue;
+        double s = f[0] * Ad[i * 3];
+        s += f[1] * Ad[i * 3 + 1];
+        s += f[2] * Ad[i * 3 + 2];
+        s += f[3 + i];
+        s += f[6] * Rd[i * 3];
+        s += f[7] * Rd[i * 3 + 1];
+        s += f[8] * Rd[i * 3 + 2];
+        s += f[9] * md[i * 2];
+        s += f[10] * md[i * 2 + 1];
+        if (gr == 1) {
+          const double q = S.q[1];
+          const double* Gr = &Rd[(r - 12) * 3];
+          double g = (Gr[0] * q) * Rd[i * 3];
+          g += (Gr[1] * q) * Rd[i * 3 + 1];
+          g += (Gr[2] * q) * Rd[i * 3 + 2];
+          s = s + g;
+        }
+        pr[12 + i] = s;
+      }
+      if (w == 3) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          double s = f[9] * S.NM[i * 2];
+          s += f[10] * S.NM[i * 2 + 1];
+          pr[21 + i] = s;
+        }
+      }
     }
   }
 }
 
-__global__ void __launch_bounds__(64) k_imu_predict(const PreJob* jobs, int nsess) {
+__global__ void __launch_bounds__(kPredThreads) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_imu_predict(const PreJob* jobs, int nsess) {
   if (int(blockIdx.x) >= nsess) return;
   const PreJob Jb = jobs[blockIdx.x];
   ImuDev* im = Jb.imu;
   FilterDev* f = Jb.f;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x;  // phases A-C and the set-up run on the first wave's lanes
   __shared__ PredShared S;
-  for (int i = lane; i < 529; i += 64) S.P[(i % 23) * 23 + i / 23] = f->P[i];  // column-major -> rows
+  for (int i = lane; i < 529; i += kPredThreads) S.P[(i % 23) * 23 + i / 23] = f->P[i];  // column-major -> rows
   load_state(f->x, S.sx, lane);
   if (lane < 4) S.q[lane] = im->q[lane];
   __syncthreads();
@@ -541,12 +601,12 @@ __global__ void __launch_bounds__(64) k_imu_predict(const PreJob* jobs, int nses
     }
     for (int j = 0; j < cn; ++j) {  // D
       __syncthreads();
-      predict_cov_step(S, j, lane);
+      predict_cov_step(S, j, lane >> 6, lane & 63);
     }
     np += (c0 + cn == nsteps) ? cn - 1 : cn;
     __syncthreads();
   }
-  for (int i = lane; i < 529; i += 64) f->P[i] = S.P[(i % 23) * 23 + i / 23];
+  for (int i = lane; i < 529; i += kPredThreads) f->P[i] = S.P[(i % 23) * 23 + i / 23];
   if (lane == 0) {
     f->x = S.sx;
     if (m > 0) {
