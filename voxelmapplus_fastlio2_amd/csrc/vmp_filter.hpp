@@ -114,33 +114,40 @@ k_ieskf_step(FilterDev** fs, const ObsJob* __restrict__ jobs, int32_t** obs_ctr,
   for (int i = lane; i < 529; i += 64) Pi[i] = f->Pinv[i];
   {
     const ObsJob& J = jobs[blockIdx.x];
-    // lane l adds rows l, l + 64, l + 128, ... in that order; the loads of
-    // kRows of them are issued together (one memory round trip per group)
-    constexpr int kRows = ROW <= 32 ? 4 : 1;
     const auto part = as_global(J.partials);
     const int nb = J.nblocks;
-    double fin[ROW];
+    if constexpr (obs_stride<NJ>() == 32) {
+      // lane l adds column l & 31 of rows (l >> 5), (l >> 5) + 2, ... in that
+      // order (each load of the wave reads two whole rows; kU loads of a lane
+      // in flight together), then lanes l and l + 32 are added
+      constexpr int kU = 112;
+      const int col = lane & 31, half = lane >> 5;
+      double acc = 0.0;
+      for (int r0 = half; r0 < nb; r0 += 2 * kU) {
+        double v[kU];
 #pragma unroll
-    for (int k = 0; k < ROW; ++k) fin[k] = 0.0;
-    for (int b0 = lane; b0 < nb; b0 += 64 * kRows) {
-      double v[kRows][ROW];
+        for (int u = 0; u < kU; ++u) v[u] = part[size_t(min(r0 + 2 * u, nb - 1)) * 32 + col];
 #pragma unroll
-      for (int u = 0; u < kRows; ++u) {
-        const int b = min(b0 + 64 * u, nb - 1);  // rows past the end are read but not added
+        for (int u = 0; u < kU; ++u)
+          if (r0 + 2 * u < nb) acc += v[u];
+      }
+      const double other = __shfl_down(acc, 32, 64);
+      if (lane < ROW) red[lane] = acc + other;
+    } else {
+      // lane l adds rows l, l + 64, l + 128, ... in that order, then the
+      // wave adds the lanes
+      double fin[ROW];
 #pragma unroll
-        for (int k = 0; k < ROW; ++k) v[u][k] = part[size_t(b) * ROW + k];
+      for (int k = 0; k < ROW; ++k) fin[k] = 0.0;
+      for (int b = lane; b < nb; b += 64) {
+#pragma unroll
+        for (int k = 0; k < ROW; ++k) fin[k] += part[size_t(b) * ROW + k];
       }
 #pragma unroll
-      for (int u = 0; u < kRows; ++u)
-        if (b0 + 64 * u < nb) {
-#pragma unroll
-          for (int k = 0; k < ROW; ++k) fin[k] += v[u][k];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < ROW; ++k) {
-      const double v = wave_sum0(fin[k]);
-      if (lane == 0) red[k] = v;
+      for (int k = 0; k < ROW; ++k) {
+        const double v = wave_sum0(fin[k]);
+        if (lane == 0) red[k] = v;
+      }
     }
   }
   __syncthreads();

@@ -87,6 +87,64 @@ __device__ inline int32_t nbr_at(const MapDev& m, uint64_t vkey, int j, int tag,
   return eligible_at(m, lookup_vid(m, key_neighbour(vkey, j)), tag, t);
 }
 
+// The six face neighbours' voxel ids (-1: none), as lookup_vid gives them;
+// the six home-slot entries are loaded together, and only a collision
+// continues its probe.
+__device__ inline void lookup_nbrs6(const MapDev& m, uint64_t vkey, int32_t nv[6]) {
+  uint64_t nk[6];
+  uint32_t h[6];
+  HashEntry e[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    nk[j] = key_neighbour_ok(vkey, j) ? key_neighbour(vkey, j) : kKeyEmpty;
+    h[j] = hash_slot(nk[j], m.hmask);
+    e[j] = m.table[h[j]];
+  }
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    int32_t v = -1;
+    if (nk[j] != kKeyEmpty) {
+      if (e[j].key == nk[j]) {
+        v = e[j].vid;
+      } else if (e[j].key != kKeyEmpty) {
+        uint32_t hh = h[j];
+        for (uint32_t probe = 1; probe <= m.hmask; ++probe) {
+          hh = (hh + 1) & m.hmask;
+          const HashEntry ee = m.table[hh];
+          if (ee.key == nk[j]) {
+            v = ee.vid;
+            break;
+          }
+          if (ee.key == kKeyEmpty) break;
+        }
+      }
+    }
+    nv[j] = v;
+  }
+}
+
+// eligible_at for six voxel ids, their records read in three rounds of six
+// loads (the incarnation's creation fields, then the eligibility fields of
+// the incarnation at time t) instead of a chain per neighbour
+__device__ inline void eligible6(const MapDev& m, const int32_t cur[6], int tag, int t, int32_t out[6]) {
+  int32_t v[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const VoxState& r = m.vs[cur[j] < 0 ? 0 : cur[j]];
+    const int rt = r.tag, ct = r.create_t, pv = r.prev_vid;
+    v[j] = cur[j] < 0 ? -1 : ((rt == tag && ct > t) ? pv : cur[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const VoxState& r = m.vs[v[j] < 0 ? 0 : v[j]];
+    const int rt = r.tag, ev = r.evict_t, fz = r.freeze_t;
+    const uint32_t fl = r.flags;
+    const bool gone = v[j] < 0 || (rt == tag && ev <= t);
+    const bool frozen = !(fl & kEnable) && !(rt == tag && fz > t);
+    out[j] = (!gone && frozen && (fl & kPlane) && (fl & kInit)) ? v[j] : -1;
+  }
+}
+
 // angle and distance gates (voxel_map.cpp:147-155); planes as kPlaneD records
 __device__ inline bool merge_gate(const double* pv, const double* pn, double ma, double md) {
   double dn = pn[3] * pv[3];
@@ -162,15 +220,25 @@ __global__ void k_candidates(const MapDev* maps, const SessInfo* si, BatchBufs b
         emitted[ne++] = t;
       };
       if (has_calls && ((st.flags & kDirty) || own_freeze)) emit(-1);
+      // the six neighbours' entries, then their records, six loads at a time
+      int32_t nvs[6];
+      lookup_nbrs6(m, st.key, nvs);
+      int ntag[6], nfrz[6];
+      uint32_t nflg[6];
+#pragma unroll
       for (int j = 0; j < 6; ++j) {
-        int32_t nv = -1;
-        if (key_neighbour_ok(st.key, j)) nv = lookup_vid(m, key_neighbour(st.key, j));
+        const VoxState& nbs = m.vs[nvs[j] < 0 ? 0 : nvs[j]];
+        ntag[j] = nbs.tag;
+        nfrz[j] = nbs.freeze_t;
+        nflg[j] = nbs.flags;
+      }
+      for (int j = 0; j < 6; ++j) {
+        const int32_t nv = nvs[j];
         b.seg_nbr[q * 6 + j] = nv;
         if (nv < 0) continue;
-        const VoxState& nbs = m.vs[nv];
-        if (nbs.tag != tag || nbs.freeze_t < 0 || !(nbs.flags & kPlane)) continue;
-        if (nbs.freeze_t > dirty) dirty = nbs.freeze_t;
-        if (has_calls) emit(nbs.freeze_t);
+        if (ntag[j] != tag || nfrz[j] < 0 || !(nflg[j] & kPlane)) continue;
+        if (nfrz[j] > dirty) dirty = nfrz[j];
+        if (has_calls) emit(nfrz[j]);
       }
       st.dirty_t = dirty;
     }
@@ -244,12 +312,23 @@ __global__ void k_spec_eval(const MapDev* maps, BatchBufs b) {
     double pv[kPlaneD];
     for (int k = 0; k < kPlaneD; ++k) pv[k] = m.plane[size_t(V) * kPlaneD + k];
     const bool cached = vst.tag == tag && vst.seg >= 0;
+    // the six neighbours (cached ids of a voxel touched in this scan, else
+    // looked up), their eligibility at t, then the gate, each step with the
+    // six neighbours' loads together
+    int32_t nc[6], nv[6];
+    if (cached) {
+#pragma unroll
+      for (int j = 0; j < 6; ++j) nc[j] = b.seg_nbr[vst.seg * 6 + j];
+    } else {
+      lookup_nbrs6(m, vst.key, nc);
+    }
+    eligible6(m, nc, tag, t, nv);
     int spec = 0;
+#pragma unroll
     for (int j = 0; j < 6; ++j) {
-      const int32_t nv = cached ? eligible_at(m, b.seg_nbr[vst.seg * 6 + j], tag, t) : nbr_at(m, vst.key, j, tag, t);
-      b.cand_nbr[size_t(p) * 6 + j] = nv;
-      if (nv >= 0 && !spec && m.vs[nv].group_id != gV &&
-          merge_gate(pv, m.plane + size_t(nv) * kPlaneD, m.merge_a, m.merge_d))
+      b.cand_nbr[size_t(p) * 6 + j] = nv[j];
+      if (nv[j] >= 0 && m.vs[nv[j]].group_id != gV &&
+          merge_gate(pv, m.plane + size_t(nv[j]) * kPlaneD, m.merge_a, m.merge_d))
         spec = 1;
     }
     b.cand_spec[p] = spec;

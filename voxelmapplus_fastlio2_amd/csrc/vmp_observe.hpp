@@ -30,6 +30,13 @@ __host__ __device__ constexpr int obs_row() {
   return NJ * (NJ + 1) / 2 + NJ + 4;
 }
 constexpr int kObsRowMax = 96;  // >= obs_row<12>() = 94
+// distance between consecutive partial rows: rows of up to 32 entries are
+// padded to 32 (zeros), so a wave reads two whole rows per load, each lane
+// the same column every time
+template <int NJ>
+__host__ __device__ constexpr int obs_stride() {
+  return obs_row<NJ>() <= 32 ? 32 : obs_row<NJ>();
+}
 
 struct ObsJob {
   const float* x;
@@ -406,12 +413,13 @@ __global__ void __launch_bounds__(kObsThreads) observe_kernel(const ObsJob* __re
     redi[wv][3] = c_fb;
   }
   __syncthreads();
-  const auto row = as_global(J.partials + size_t(lb) * obs_row<NJ>());
+  const auto row = as_global(J.partials + size_t(lb) * obs_stride<NJ>());
   if (tid < NACC) {
     double s = red[0][tid];
     for (int w = 1; w < kObsThreads / 64; ++w) s += red[w][tid];
     row[tid] = s;
   }
+  if (tid >= obs_row<NJ>() && tid < obs_stride<NJ>()) row[tid] = 0.0;
   if (tid < 4) {
     int s = 0;
     for (int w = 0; w < kObsThreads / 64; ++w) s += redi[w][tid];
@@ -435,7 +443,7 @@ __global__ void __launch_bounds__(kObsThreads) k_obs_reduce(const ObsJob* __rest
 #pragma unroll
   for (int k = 0; k < ROW; ++k) fin[k] = 0.0;
   for (int b = tid; b < J.nblocks; b += kObsThreads) {
-    const double* src = J.partials + size_t(b) * ROW;
+    const double* src = J.partials + size_t(b) * obs_stride<NJ>();
 #pragma unroll
     for (int k = 0; k < ROW; ++k) fin[k] += src[k];
   }
