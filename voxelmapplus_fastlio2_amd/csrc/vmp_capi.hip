@@ -1511,10 +1511,10 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         observe(grid, false, false);
         timed_launch(c, kTkIeskfStep, [&] {
           if (ext)
-            hipLaunchKernelGGL(k_ieskf_step<12>, dim3(n), dim3(64), 0, st, w.d_flist, w.d_jobs, w.d_obsctr,
+            hipLaunchKernelGGL(k_ieskf_step<12>, dim3(n), dim3(kStepThreads), 0, st, w.d_flist, w.d_jobs, w.d_obsctr,
                                w.d_obsstat);
           else
-            hipLaunchKernelGGL(k_ieskf_step<6>, dim3(n), dim3(64), 0, st, w.d_flist, w.d_jobs, w.d_obsctr,
+            hipLaunchKernelGGL(k_ieskf_step<6>, dim3(n), dim3(kStepThreads), 0, st, w.d_flist, w.d_jobs, w.d_obsctr,
                                w.d_obsstat);
         });
       }
@@ -1583,7 +1583,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         // one stable radix sort over (session, hash slot): every key's points
         // become a contiguous segment in point order, sessions stay in order
         if (!sort_pairs(b.skey_in, b.sval_in, b.skey_out, b.sval_out, hbits + sbits)) return VMP_EDEVICE;
-        hipLaunchKernelGGL(k_seg_heads, dim3(gn), dim3(T), 0, st, S, n, b);
+        hipLaunchKernelGGL(k_seg_heads, dim3(gn), dim3(T), 0, st, M, S, n, b);
         // the library's compaction also clears the flags it reads, computes
         // the per-session ranges in its last tile and (creation events)
         // splits each event into its time and segment

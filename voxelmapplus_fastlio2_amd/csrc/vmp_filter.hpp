@@ -92,10 +92,12 @@ __global__ void __launch_bounds__(64) k_ieskf_begin(FilterDev** fs, int32_t** ob
 // the lanes are summed by the wave (H 12x12 column-major and b as the
 // standalone reduction writes them).  The observe counters are read, then
 // cleared for the next pass.
-// One wave per session: the registers of the whole SIMD are its own
-// (waves_per_eu 1), so a lane keeps several partial rows in flight.
+// Four waves per session: all of them read and add the partial rows (many
+// loads in flight); the filter algebra then runs on the first wave's lanes
+// (the other waves pass the same barriers and store nothing).
+constexpr int kStepThreads = 256;
 template <int NJ>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_ieskf_step(FilterDev** fs, const ObsJob* __restrict__ jobs, int32_t** obs_ctr, int32_t** stat_out) {
   constexpr int NH = NJ * (NJ + 1) / 2;
   constexpr int NACC = NH + NJ;
@@ -108,32 +110,39 @@ k_ieskf_step(FilterDev** fs, const ObsJob* __restrict__ jobs, int32_t** obs_ctr,
   __shared__ JacBlocks jb;
   __shared__ double hobs[156];
   __shared__ double red[ROW];
-  const int lane = threadIdx.x;
+  __shared__ double gsum[kStepThreads / 32][32];
+  const int lane = threadIdx.x;  // lanes of the first wave carry the algebra
   load_state(f->x, sx, lane);
   load_state(f->xp, sxp, lane);
-  for (int i = lane; i < 529; i += 64) Pi[i] = f->Pinv[i];
+  for (int i = lane; i < 529; i += kStepThreads) Pi[i] = f->Pinv[i];
   {
     const ObsJob& J = jobs[blockIdx.x];
     const auto part = as_global(J.partials);
     const int nb = J.nblocks;
     if constexpr (obs_stride<NJ>() == 32) {
-      // lane l adds column l & 31 of rows (l >> 5), (l >> 5) + 2, ... in that
-      // order (each load of the wave reads two whole rows; kU loads of a lane
-      // in flight together), then lanes l and l + 32 are added
-      constexpr int kU = 112;
-      const int col = lane & 31, half = lane >> 5;
+      // thread t adds column t & 31 of rows g, g + 8, g + 16, ... (g = t >> 5)
+      // in that order, kU loads in flight together (each wave load reads two
+      // whole rows); then the eight groups are added in order
+      constexpr int kU = 16, kG = kStepThreads / 32;
+      const int col = lane & 31, grp = lane >> 5;
       double acc = 0.0;
-      for (int r0 = half; r0 < nb; r0 += 2 * kU) {
+      for (int r0 = grp; r0 < nb; r0 += kG * kU) {
         double v[kU];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) v[u] = part[size_t(min(r0 + 2 * u, nb - 1)) * 32 + col];
+        for (int u = 0; u < kU; ++u) v[u] = part[size_t(min(r0 + kG * u, nb - 1)) * 32 + col];
 #pragma unroll
         for (int u = 0; u < kU; ++u)
-          if (r0 + 2 * u < nb) acc += v[u];
+          if (r0 + kG * u < nb) acc += v[u];
       }
-      const double other = __shfl_down(acc, 32, 64);
-      if (lane < ROW) red[lane] = acc + other;
-    } else {
+      gsum[grp][col] = acc;
+      __syncthreads();
+      if (lane < ROW) {
+        double t = gsum[0][lane];
+#pragma unroll
+        for (int g = 1; g < kG; ++g) t += gsum[g][lane];
+        red[lane] = t;
+      }
+    } else if (lane < 64) {
       // lane l adds rows l, l + 64, l + 128, ... in that order, then the
       // wave adds the lanes
       double fin[ROW];
@@ -151,7 +160,7 @@ k_ieskf_step(FilterDev** fs, const ObsJob* __restrict__ jobs, int32_t** obs_ctr,
     }
   }
   __syncthreads();
-  for (int i = lane; i < 144; i += 64) {  // H from the upper triangle
+  for (int i = lane; i < 144; i += kStepThreads) {  // H from the upper triangle
     const int r = i % 12, c = i / 12;
     double v = 0.0;
     if (r < NJ && c < NJ) {

@@ -150,9 +150,10 @@ struct BatchBufs {
   int32_t* lv_cnt;
   int32_t* lv_n;
   int32_t* prim;       // work entries of the compaction / radix sort (prim_ws_entries)
-  // per point (global index): its insertion point relative to its voxel's
-  // centre (fp32, as the replays fold it in) and its fp32 world covariance,
-  // (q.x, q.y, q.z, -, C0..C5, -, -), formed once by k_ins_keys
+  // per point in key-sorted order: its insertion point relative to its
+  // voxel's centre (fp32, as the replays fold it in), its index in the scan
+  // and its fp32 world covariance, (q.x, q.y, q.z, index bits, C0..C5, -, -),
+  // formed once by k_seg_heads
   float4* qc;
 };
 __host__ __device__ inline int64_t lv_cnt_base(int off, int s) { return (int64_t(2) * off >> 6) + s; }
@@ -416,28 +417,42 @@ __global__ void k_ins_keys(const MapDev* maps, const SessInfo* si, int nsess, Ba
     else atomicOr(&m.ctr->error, 2);
     b.skey_in[g] = (uint64_t(s) << b.hbits) | uint64_t(uint32_t(e < 0 ? 0 : e));
     b.sval_in[g] = g;
-    // the replays' inputs: the point relative to its voxel's centre and its
-    // world covariance (calcBodyCov's z = 0 -> 0.001 applies to the
-    // covariance only, commons.cpp:20-23)
-    float qv[3];
-    for (int r = 0; r < 3; ++r) qv[r] = float(double(pw[r]) - (double(k[r]) + 0.5) * m.vs64);
-    const float pz = S.z[i] == 0.0f ? 0.001f : S.z[i];
-    float C[6];
-    point_world_cov(P.rwl32, P.Prr, P.Ppp, P.s2, P.rv, S.x[i], S.y[i], pz, C);
-    b.qc[3 * size_t(g)] = make_float4(qv[0], qv[1], qv[2], 0.0f);
-    b.qc[3 * size_t(g) + 1] = make_float4(C[0], C[1], C[2], C[3]);
-    b.qc[3 * size_t(g) + 2] = make_float4(C[4], C[5], 0.0f, 0.0f);
     }
     wave_fetch_add(&maps[s].ctr->used_entries, ins);
   }
 }
 
 // ------------------------------------------------------------------ K3 ----
-__global__ void k_seg_heads(const SessInfo* si, int nsess, BatchBufs b) {
+// Segment heads of the key-sorted points, and the replays' input of every
+// point in that order (so a voxel's points are contiguous): the point
+// relative to its voxel's centre, its world covariance (calcBodyCov's z = 0
+// -> 0.001 applies to the covariance only, commons.cpp:20-23) and its index
+// in the scan, (q.x, q.y, q.z, index bits, C0..C5, -, -).  The transform and
+// the key are k_ins_keys' expressions, so the voxel is the one the point was
+// keyed to.
+__global__ void k_seg_heads(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
   if (blockIdx.x == 0 && threadIdx.x == 0) b.cnt[5] = 0;  // refitting voxels, counted by k_seg_lenkeys
   VMP_GRID_LOOP(j, b.ntot) {
     const int s = sess_of(si, nsess, j);
-    b.flags[j] = (j == si[s].off || b.skey_out[j] != b.skey_out[j - 1]) ? 1 : 0;
+    const SessInfo& S = si[s];
+    b.flags[j] = (j == S.off || b.skey_out[j] != b.skey_out[j - 1]) ? 1 : 0;
+    const double vs64 = maps[s].vs64;
+    const UpdParams& P = *S.prm;
+    const int i = b.sval_out[j] - S.off;
+    const float x = S.x[i], y = S.y[i], z = S.z[i];
+    float pw[3];
+    ins_transform(P.m, P.t, x, y, z, pw);
+    float qv[3];
+    for (int r = 0; r < 3; ++r) {
+      const int64_t k = int64_t(floor(double(pw[r]) / vs64));
+      qv[r] = float(double(pw[r]) - (double(k) + 0.5) * vs64);
+    }
+    const float pz = z == 0.0f ? 0.001f : z;
+    float C[6];
+    point_world_cov(P.rwl32, P.Prr, P.Ppp, P.s2, P.rv, x, y, pz, C);
+    b.qc[3 * size_t(j)] = make_float4(qv[0], qv[1], qv[2], __int_as_float(i));
+    b.qc[3 * size_t(j) + 1] = make_float4(C[0], C[1], C[2], C[3]);
+    b.qc[3 * size_t(j) + 2] = make_float4(C[4], C[5], 0.0f, 0.0f);
   }
 }
 
@@ -1140,7 +1155,11 @@ __device__ inline void mom_store(double* dst, const Moments& M) {
   for (int a = 0; a < 6; ++a)
     for (int l = 0; l < 6; ++l) dst[o++] = M.c2[a][l];
 }
+// The products are fused into the sums (one rounding per term instead of
+// two): the moments are this library's own intermediate, not a quantity
+// the reference rounds in a particular order.
 __device__ inline void mom_add(Moments& M, const float q[3], const float C[6]) {
+#pragma clang fp contract(fast)
   const double qd[3] = {double(q[0]), double(q[1]), double(q[2])};
   double qq[6];
   for (int a = 0; a < 3; ++a)
@@ -1285,6 +1304,8 @@ __global__ void __launch_bounds__(kMapThreads) k_seg_lenkeys(const MapDev* maps,
 // order): one lane per voxel replays its pending points and then its points
 // of this scan in order through the pushPoint state machine, with the FP64
 // moments in registers and refits from them.
+constexpr int kHeavyChunk = 4;  // points of a heavy voxel's walk loaded together
+constexpr int kLightChunk = 8;  // the same for the light walk
 __global__ void __launch_bounds__(kMapThreads) k_replay_heavy(const MapDev* maps, const SessInfo* si, BatchBufs b) {
   const int nh = b.cnt[5];
   VMP_UNIFORM_LOOP(qi, valid, nh) {
@@ -1312,11 +1333,18 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_heavy(const MapDev* maps
       else mom_zero(M);  // no point has reached the moments yet
       if (cnt > m.pend_cap) atomicOr(&m.ctr->error, 32);
       const float* pd = m.pend + size_t(vid) * m.pend_cap * kPendF;
-      for (int k = 0; k < cnt && k < m.pend_cap; ++k) {
-        const float qv[3] = {pd[k * kPendF], pd[k * kPendF + 1], pd[k * kPendF + 2]};
-        const float C[6] = {pd[k * kPendF + 3], pd[k * kPendF + 4], pd[k * kPendF + 5],
-                            pd[k * kPendF + 6], pd[k * kPendF + 7], pd[k * kPendF + 8]};
-        mom_add(M, qv, C);
+      const int npd = min(cnt, m.pend_cap);
+      for (int k0 = 0; k0 < npd; k0 += kHeavyChunk) {  // a chunk of pending points loaded together
+        float v[kHeavyChunk][kPendF];
+#pragma unroll
+        for (int u = 0; u < kHeavyChunk; ++u) {
+          const int k = min(k0 + u, npd - 1);
+#pragma unroll
+          for (int a = 0; a < kPendF; ++a) v[u][a] = pd[k * kPendF + a];
+        }
+#pragma unroll
+        for (int u = 0; u < kHeavyChunk; ++u)
+          if (k0 + u < npd) mom_add(M, v[u], v[u] + 3);
       }
       int64_t kx, ky, kz;
       unpack_key(st.key, kx, ky, kz);
@@ -1332,58 +1360,57 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_heavy(const MapDev* maps
       // reference's: add the point, refit, then the freeze check.
       int pos = start;
       bool active = true;
-      // prefetch of the next point (its index, then its q and C)
-      int g_nx = pos < end ? b.sval_out[pos] : 0;
-      float4 qa_nx = make_float4(0.f, 0.f, 0.f, 0.f), qb_nx = qa_nx, qc_nx = qa_nx;
-      if (pos < end) {
-        qa_nx = b.qc[3 * size_t(g_nx)];
-        qb_nx = b.qc[3 * size_t(g_nx) + 1];
-        qc_nx = b.qc[3 * size_t(g_nx) + 2];
-      }
       while (__any(active)) {
         bool need = false;
         while (active && !need) {
-          if (pos >= end) {
-            active = false;
-            break;
+          // the next kHeavyChunk points of the segment, loaded together
+          // (contiguous in key order); a refit ends the chunk early
+          float4 qa[kHeavyChunk], qb[kHeavyChunk], qd[kHeavyChunk];
+#pragma unroll
+          for (int u = 0; u < kHeavyChunk; ++u) {
+            const size_t p3 = 3 * size_t(min(pos + u, end - 1));
+            qa[u] = b.qc[p3];
+            qb[u] = b.qc[p3 + 1];
+            qd[u] = b.qc[p3 + 2];
           }
-          if ((flags & kInit) && !(flags & kEnable) && !S.is_build) {
-            if (flags & kPlane) {
-              merge_pos = pos - start;
-              calls += end - pos;
-            } else {
-              dropped += end - pos;
+#pragma unroll
+          for (int u = 0; u < kHeavyChunk; ++u) {
+            if (!active || need) break;
+            if (pos >= end) {
+              active = false;
+              break;
             }
-            active = false;
-            break;
-          }
-          const int i = g_nx - S.off;
-          const float4 qa = qa_nx, qb = qb_nx, qcc = qc_nx;
-          if (pos + 1 < end) {
-            g_nx = b.sval_out[pos + 1];
-            qa_nx = b.qc[3 * size_t(g_nx)];
-            qb_nx = b.qc[3 * size_t(g_nx) + 1];
-            qc_nx = b.qc[3 * size_t(g_nx) + 2];
-          }
-          ++pos;
-          const float qv[3] = {qa.x, qa.y, qa.z};
-          const float C[6] = {qb.x, qb.y, qb.z, qb.w, qcc.x, qcc.y};
-          mom_add(M, qv, C);
-          ++n;
-          if (S.is_build) continue;
-          if (!(flags & kInit)) {
-            // updatePlane after every point until the voxel initialises
-            need = n >= m.upd_pt;
-            continue;
-          }
-          ++newly;
-          if (newly >= m.upd_pt) {
-            need = true;
-            newly = 0;
-          }
-          if (n >= m.max_pt) {
-            flags &= ~kEnable;
-            freeze_t = i;
+            if ((flags & kInit) && !(flags & kEnable) && !S.is_build) {
+              if (flags & kPlane) {
+                merge_pos = pos - start;
+                calls += end - pos;
+              } else {
+                dropped += end - pos;
+              }
+              active = false;
+              break;
+            }
+            const int i = __float_as_int(qa[u].w);
+            ++pos;
+            const float qv[3] = {qa[u].x, qa[u].y, qa[u].z};
+            const float C[6] = {qb[u].x, qb[u].y, qb[u].z, qb[u].w, qd[u].x, qd[u].y};
+            mom_add(M, qv, C);
+            ++n;
+            if (S.is_build) continue;
+            if (!(flags & kInit)) {
+              // updatePlane after every point until the voxel initialises
+              need = n >= m.upd_pt;
+              continue;
+            }
+            ++newly;
+            if (newly >= m.upd_pt) {
+              need = true;
+              newly = 0;
+            }
+            if (n >= m.max_pt) {
+              flags &= ~kEnable;
+              freeze_t = i;
+            }
           }
         }
         if (need) refit(m, M, n, flags, plv, center, refits);
@@ -1452,8 +1479,21 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_light(const MapDev* maps
       uint32_t flags = st.flags;
       int n = st.n, newly = st.newly_add;
       int merge_pos = kNever, freeze_t = -1;
-      int g_nx = start < end ? b.sval_out[start] : 0;
-      for (int pos = start; pos < end; ++pos) {
+      bool walking = true;
+      for (int p0 = start; p0 < end && walking; p0 += kLightChunk) {
+       // a chunk of the segment's points loaded together (contiguous in key order)
+       float4 qa[kLightChunk], qb[kLightChunk], qd[kLightChunk];
+#pragma unroll
+       for (int u = 0; u < kLightChunk; ++u) {
+         const size_t p3 = 3 * size_t(min(p0 + u, end - 1));
+         qa[u] = b.qc[p3];
+         qb[u] = b.qc[p3 + 1];
+         qd[u] = b.qc[p3 + 2];
+       }
+#pragma unroll
+       for (int u = 0; u < kLightChunk; ++u) {
+        const int pos = p0 + u;
+        if (pos >= end || !walking) break;
         if ((flags & kInit) && !(flags & kEnable) && !S.is_build) {
           if (flags & kPlane) {
             merge_pos = pos - start;
@@ -1461,14 +1501,12 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_light(const MapDev* maps
           } else {
             dropped += end - pos;
           }
+          walking = false;
           break;
         }
-        const int g = g_nx;
-        const int i = g - S.off;
-        if (pos + 1 < end) g_nx = b.sval_out[pos + 1];
-        const float4 qa = b.qc[3 * size_t(g)], qb = b.qc[3 * size_t(g) + 1], qcc = b.qc[3 * size_t(g) + 2];
-        const float qv[3] = {qa.x, qa.y, qa.z};
-        const float C[6] = {qb.x, qb.y, qb.z, qb.w, qcc.x, qcc.y};
+        const int i = __float_as_int(qa[u].w);
+        const float qv[3] = {qa[u].x, qa[u].y, qa[u].z};
+        const float C[6] = {qb[u].x, qb[u].y, qb[u].z, qb[u].w, qd[u].x, qd[u].y};
         if (cnt < m.pend_cap) {
           for (int a = 0; a < 3; ++a) pd[cnt * kPendF + a] = qv[a];
           for (int a = 0; a < 6; ++a) pd[cnt * kPendF + 3 + a] = C[a];
@@ -1489,6 +1527,7 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_light(const MapDev* maps
           flags &= ~kEnable;
           freeze_t = i;
         }
+       }
       }
       for (int a = 0; a < 3; ++a) mo[kMomRun + a] = s1[a];
       mo[kMomPend] = double(cnt);
