@@ -19,10 +19,13 @@
 #include "vmp_common.hpp"
 #include "vmp_filter.hpp"
 #include "vmp_ieskf.hpp"
+#include "vmp_prim.hpp"
 #include "vmp_mapupd.hpp"
 #include "vmp_observe.hpp"
 #include "vmp_imu.hpp"
-#include "vmp_prim.hpp"
+// entries of the per-tile first-session table: every tile of the largest
+// input the library's compaction takes, and one past the last
+constexpr int kSfirstEntries = vmp::kPrimMaxItems / vmp::kPrimTile + 2;
 static_assert(vmp::kCsThreads == vmp::kPrimThreads, "the candidate starts run as the tail of a compaction tile");
 
 using namespace vmp;
@@ -142,7 +145,12 @@ struct BatchWs {
   // step uploads them with one copy
   uint8_t* d_tab = nullptr;
   uint8_t* h_tab = nullptr;
-  size_t tab_obsstat = 0;  // offset of the last table (d_obsstat)
+  size_t tab_obsstat = 0;  // offset of d_obsstat
+  size_t tab_sfirst = 0;   // offset of the last table (d_sfirst)
+  // first session of each compaction tile of the step's points (sessions
+  // starting in tile t: [sfirst[t], sfirst[t + 1]))
+  int32_t* d_sfirst = nullptr;
+  int32_t* h_sfirst = nullptr;
   MapDev* d_maps = nullptr;
   SessInfo* d_si = nullptr;
   ObsJob* d_jobs = nullptr;
@@ -515,9 +523,13 @@ static bool ws_layout(vmp_ctx* c, BatchWs& w, size_t n, int ns, size_t pre_cap, 
   const size_t o_maps = tab_at(sizeof(MapDev) * S), o_si = tab_at(sizeof(SessInfo) * S);
   const size_t o_jobs = tab_at(sizeof(ObsJob) * S), o_fl = tab_at(sizeof(FilterDev*) * S);
   const size_t o_oc = tab_at(sizeof(int32_t*) * S), o_os = tab_at(sizeof(int32_t*) * S);
+  const size_t o_sf = tab_at(sizeof(int32_t) * kSfirstEntries);
   ok = ok && dev(&w.d_tab, tab) && host(&w.h_tab, tab);
   if (ok) {
     w.tab_obsstat = o_os;
+    w.tab_sfirst = o_sf;
+    w.d_sfirst = reinterpret_cast<int32_t*>(w.d_tab + o_sf);
+    w.h_sfirst = reinterpret_cast<int32_t*>(w.h_tab + o_sf);
     w.d_maps = reinterpret_cast<MapDev*>(w.d_tab + o_maps);
     w.d_si = reinterpret_cast<SessInfo*>(w.d_tab + o_si);
     w.d_jobs = reinterpret_cast<ObsJob*>(w.d_tab + o_jobs);
@@ -1335,6 +1347,17 @@ static int stage_batch(vmp_ctx* c, vmp_session** ss, int n, int64_t* ntot_out, i
   }
   *ntot_out = off;
   *nblocks_out = blocks;
+  // the sessions starting in each compaction tile (only when the library's
+  // compaction runs: at most kPrimMaxItems points)
+  if (off <= kPrimMaxItems) {
+    const int tiles = prim_tiles(off);
+    int sidx = 0;
+    for (int t = 0; t <= tiles; ++t) {
+      const int64_t lo = int64_t(t) * kPrimTile;
+      while (sidx < n && w.h_si[sidx].off < lo) ++sidx;
+      w.h_sfirst[t] = sidx;
+    }
+  }
   return VMP_OK;
 }
 
@@ -1459,7 +1482,10 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
   for (int i = 0; i < n; ++i) w.h_si[i].is_build = (mode & kStepBuild) ? 1 : 0;
   // the job tables in one copy: from the first table through the last
   // one's n entries (the unused tails of the earlier tables ride along)
-  if (hipMemcpyAsync(w.d_tab, w.h_tab, w.tab_obsstat + sizeof(int32_t*) * n, hipMemcpyHostToDevice, st) != hipSuccess)
+  const size_t tab_bytes = ntot <= kPrimMaxItems
+                               ? w.tab_sfirst + sizeof(int32_t) * size_t(prim_tiles(ntot) + 1)
+                               : w.tab_obsstat + sizeof(int32_t*) * n;
+  if (hipMemcpyAsync(w.d_tab, w.h_tab, tab_bytes, hipMemcpyHostToDevice, st) != hipSuccess)
     return VMP_EDEVICE;
   const bool ext = ss[0]->cfg.estimate_ext != 0;
   // zero: clear the sessions' observe counters first (inside the filter stage
@@ -1589,7 +1615,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         // splits each event into its time and segment
         if (prim) {
           select_stable(st, ntot, PredFlagClear{b.flags}, OutIndex{b.sel}, b.prim, b.cnt + 0,
-                        TailRanges{M, S, n, b.sel, 0});
+                        TailRanges{M, S, n, b.sel, 0, w.d_sfirst, ntot});
         } else {
           if (!select_flags(b.flags, b.sel, b.cnt + 0)) return VMP_EDEVICE;
           hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 0, 0);
@@ -1598,7 +1624,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         hipLaunchKernelGGL(k_seg_info, dim3(gn), dim3(T), 0, st, M, S, n, b);
         if (prim) {
           select_stable(st, ntot, PredFlagClear{b.flags}, OutCre{b.sel, b.cre_t, b.cre_seg, S, n}, b.prim,
-                        b.cnt + 1, TailRanges{M, S, n, b.sel, 1});
+                        b.cnt + 1, TailRanges{M, S, n, b.sel, 1, w.d_sfirst, ntot});
         } else {
           if (!select_flags(b.flags, b.sel, b.cnt + 1)) return VMP_EDEVICE;
           hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 1, 1);
@@ -1658,7 +1684,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         hipLaunchKernelGGL(k_evict_finalize_ctr, dim3(n), dim3(1), 0, st, M, n);
         if (prim) {
           select_stable(st, ntot, PredFlag{b.flags}, OutIndex{b.sel}, b.prim, b.cnt + 2,
-                        TailRanges{M, S, n, b.sel, 2});
+                        TailRanges{M, S, n, b.sel, 2, w.d_sfirst, ntot});
         } else {
           if (!select_flags(b.flags, b.sel, b.cnt + 2)) return VMP_EDEVICE;
           hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 2, 2);

@@ -381,16 +381,54 @@ __global__ void k_ranges(const MapDev* maps, SessInfo* si, int nsess, const int3
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < nsess) session_range(maps, si, s, list, *count, which);
 }
-// the same as the tail of the compaction that produced the list
+// The same, inside the compaction that produces the list: a session's first
+// list position is the number of items selected before its first point,
+// which the thread holding that point knows once its tile's prefix is known
+// (sfirst[t] .. sfirst[t + 1] - 1 are the sessions starting in tile t); the
+// last tile to finish sets each session's end (the next one's start, or the
+// count) and its counter.  Sessions starting at or past nitems (empty
+// trailing sessions) start at the count.
+__device__ inline int32_t* range_lo(SessInfo& S, int which) {
+  return which == 0 ? &S.seg_lo : (which == 1 ? &S.cre_lo : &S.lru_lo);
+}
 struct TailRanges {
   const MapDev* maps;
   SessInfo* si;
   int nsess;
   const int32_t* list;
   int which;
+  const int32_t* sfirst;
+  int nitems;
   static constexpr bool kActive = true;
+  __device__ void at_tile(int tile, int base, int pos, const int32_t (&v)[kPrimIpt]) const {
+    const int s1 = sfirst[tile + 1];
+    for (int s = sfirst[tile]; s < s1; ++s) {
+      const int o = si[s].off;
+      if (o < base || o >= base + kPrimIpt) continue;
+      int p = pos;
+#pragma unroll
+      for (int k = 0; k < kPrimIpt; ++k)
+        if (k < o - base && v[k] != 0) ++p;
+      *range_lo(si[s], which) = p;
+    }
+  }
   __device__ void operator()(int n) const {
-    for (int s = threadIdx.x; s < nsess; s += blockDim.x) session_range(maps, si, s, list, n, which);
+    for (int s = threadIdx.x; s < nsess; s += blockDim.x) {
+      const int lo = si[s].off >= nitems ? n : *range_lo(si[s], which);
+      const int hi = s + 1 < nsess ? (si[s + 1].off >= nitems ? n : *range_lo(si[s + 1], which)) : n;
+      if (si[s].off >= nitems) *range_lo(si[s], which) = lo;
+      MapCounters* c = maps[s].ctr;
+      if (which == 0) {
+        si[s].seg_hi = hi;
+        c->nseg = hi - lo;
+      } else if (which == 1) {
+        si[s].cre_hi = hi;
+        c->ncreate_ev = hi - lo;
+      } else {
+        si[s].lru_hi = hi;
+        c->nlru = hi - lo;
+      }
+    }
   }
 };
 
@@ -1661,6 +1699,7 @@ struct TailCandStarts {
   int nsess;
   BatchBufs b;
   static constexpr bool kActive = true;
+  __device__ void at_tile(int, int, int, const int32_t (&)[kPrimIpt]) const {}
   __device__ void operator()(int) const { cand_starts_block(maps, nsess, b); }
 };
 

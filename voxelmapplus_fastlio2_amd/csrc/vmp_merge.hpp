@@ -379,6 +379,7 @@ struct TailSplRanges {
   int nsess;
   BatchBufs b;
   static constexpr bool kActive = true;
+  __device__ void at_tile(int, int, int, const int32_t (&)[kPrimIpt]) const {}
   __device__ void operator()(int n) const {
     for (int s = threadIdx.x; s < nsess; s += blockDim.x) spl_range(maps, si, s, b, n);
   }

@@ -165,9 +165,13 @@ __device__ inline bool finish_tile(PrimCtl* ctl, int nt, uint32_t* status, int n
   return true;
 }
 
-// no work after the compaction
+// Hooks of a compaction: at_tile(tile, base, pos, v) runs in every thread
+// once the tile's prefix is known (its items base .. base + kPrimIpt - 1
+// with values v, pos the output position of its first selected item);
+// operator()(count) runs in the last tile to finish (kActive).
 struct NoTail {
   static constexpr bool kActive = false;
+  __device__ void at_tile(int, int, int, const int32_t (&)[kPrimIpt]) const {}
   __device__ void operator()(int) const {}
 };
 
@@ -210,6 +214,7 @@ __global__ void __launch_bounds__(kPrimThreads) k_select1(int n, Pred pred, Out 
   }
   __syncthreads();
   pos += int32_t(s_prefix);
+  tail.at_tile(tile, base, pos, v);
 #pragma unroll
   for (int k = 0; k < kPrimIpt; ++k)
     if (v[k] != 0) out(pos++, base + k, v[k]);
