@@ -45,6 +45,18 @@ INS_BYTES_POINT = 64
 VOXEL_BYTES = 168
 MERGE_BYTES = 288
 LRU_BYTES = 32
+# an IESKF iteration reads the observe pass's partial rows (248 B per block
+# of 256 x points-per-thread points) and moves P^-1, H_'s factor, the state
+# and the prior state (9,500 B); an IMU propagation reads and writes P and
+# the state (9,040 B) and per IMU sample reads it (56 B) and writes a pose
+# table entry (176 B); a candidate pass reads per touched voxel its record
+# (80 B) and its six neighbours' hash entries and fields (6 x 32 B) and
+# writes their ids and its dirty time (32 B)
+PART_ROW_BYTES = 248
+IESKF_ITER_BYTES = 9500
+PRED_BYTES = 9040
+PRED_SAMPLE_BYTES = 56 + 176
+CAND_VOXEL_BYTES = 80 + 6 * 32 + 32
 # observe launches timed back to back per event pair in the roofline pass
 OBS_TIME_REPS = 8
 
@@ -485,6 +497,10 @@ def kernel_rooflines(tt, kt, stats, N, scans):
         b = OBS_BYTES_POINT * tt["obs_points"] / tt["obs_launches"]
         rows["observe_kernel<6>"] = (tt["obs_ms"], tt["obs_launches"], b, f"{OBS_BYTES_POINT} B x points")
     units = {
+        "k_ieskf_step": (stats["ieskf_bytes"], f"{PART_ROW_BYTES} B x observe blocks + {IESKF_ITER_BYTES} B, "
+                         "per iteration"),
+        "k_imu_predict": (stats["pred_bytes"], f"{PRED_BYTES} B + {PRED_SAMPLE_BYTES} B x IMU samples"),
+        "k_candidates": (CAND_VOXEL_BYTES * stats["touched"], f"{CAND_VOXEL_BYTES} B x touched voxels"),
         "k_ins_keys": (INS_BYTES_POINT * stats["points"], f"{INS_BYTES_POINT} B x points"),
         "k_replay_heavy": (VOXEL_BYTES * stats["refits"], f"{VOXEL_BYTES} B x refits"),
         "k_merge_replay": (MERGE_BYTES * stats["merge_evals"], f"{MERGE_BYTES} B x merge() evaluations"),
@@ -632,12 +648,18 @@ def main():
     # by kernel instead of replaying its graph; the kernels are the same)
     g.set_timing(True)
     ctx.set_param("obs_time_reps", OBS_TIME_REPS)
-    rstat = {"points": 0, "raw_points": 0, "refits": 0, "merge_evals": 0, "created": 0, "evicted": 0}
+    rstat = {"points": 0, "raw_points": 0, "refits": 0, "merge_evals": 0, "created": 0, "evicted": 0,
+             "touched": 0, "ieskf_bytes": 0, "pred_bytes": 0}
+    ppt = tune["points_per_thread"] if tune else args.obs_ppt
     for k in range(2 + F + W + K, total):
         bind(k)
         o, m = g.register()
         rstat["points"] += m.points
         rstat["raw_points"] += N
+        rstat["touched"] += m.touched
+        blocks = -(-m.points // (256 * ppt))
+        rstat["ieskf_bytes"] += o.iterations * (PART_ROW_BYTES * blocks + IESKF_ITER_BYTES)
+        rstat["pred_bytes"] += PRED_BYTES + PRED_SAMPLE_BYTES * len(pkgs[k][4][0])
         for f in ("refits", "merge_evals", "created", "evicted"):
             rstat[f] += getattr(m, f)
     torch.cuda.synchronize()
