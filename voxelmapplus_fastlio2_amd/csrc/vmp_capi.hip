@@ -211,6 +211,14 @@ struct vmp_ctx {
   BatchWs ws;
   RingTmp rt;
   hipEvent_t ev[4]{};
+  // the IMU packages of a batch of raw scans in one upload: move records and
+  // packages in pinned staging, copied whole, then scattered on the device
+  // into the sessions' package slots
+  uint8_t* h_pkstage = nullptr;
+  uint8_t* d_pkstage = nullptr;
+  size_t pkstage_cap = 0;
+  hipEvent_t pkstage_done = nullptr;
+  bool pkstage_pending = false;
   // optional per-launch timing of the observe kernel / map pipeline
   bool timing = false;
   // observe launches back to back between one event pair (the kernel is
@@ -417,6 +425,9 @@ extern "C" int vmp_ctx_destroy(vmp_ctx* ctx) {
     if (e) hipEventDestroy(e);
   for (hipEvent_t e : ctx->tev)
     if (e) hipEventDestroy(e);
+  if (ctx->h_pkstage) hipHostFree(ctx->h_pkstage);
+  if (ctx->d_pkstage) hipFree(ctx->d_pkstage);
+  if (ctx->pkstage_done) hipEventDestroy(ctx->pkstage_done);
   if (ctx->h_refs) hipHostFree(ctx->h_refs);
   if (ctx->h_bad) hipHostFree(ctx->h_bad);
   if (ctx->d_refs) hipFree(ctx->d_refs);
@@ -1056,6 +1067,77 @@ static int stage_imu(vmp_session* s, const double* it, const double* iacc, const
   return VMP_OK;
 }
 
+// one package's place in the batch staging: its session's slot, its offset
+// in the staging and its length (a multiple of 8 bytes)
+struct PkgMove {
+  ImuPkg* dst;
+  int64_t src_off;
+  int32_t bytes;
+  int32_t pad;
+};
+__global__ void k_scatter_pkgs(const uint8_t* stage) {
+  const PkgMove mv = reinterpret_cast<const PkgMove*>(stage)[blockIdx.x];
+  const uint2* src = reinterpret_cast<const uint2*>(stage + mv.src_off);
+  uint2* dst = reinterpret_cast<uint2*>(mv.dst);
+  for (int i = threadIdx.x; i < mv.bytes / 8; i += blockDim.x) dst[i] = src[i];
+}
+
+// Stage the IMU packages of n sessions of one context with one copy and one
+// scatter launch (c->mu held); ioff[i] is session i's first sample.
+static int stage_imu_batch(vmp_ctx* c, vmp_session** ss, int n, const int64_t* ioff, const double* it,
+                           const double* iacc, const double* igyro, const int32_t* m, const double* cstart,
+                           const double* cend) {
+  vmp_session* s = ss[0];  // a failure is reported on the batch's first session
+  const size_t head = (sizeof(PkgMove) * size_t(n) + 255) & ~size_t(255);
+  size_t total = head;
+  for (int i = 0; i < n; ++i) total += imu_pkg_bytes(m[i]);
+  hipStream_t st = c->stream;
+  if (c->pkstage_pending) VMP_HIP(hipEventSynchronize(c->pkstage_done));  // its copy has read the staging
+  c->pkstage_pending = false;
+  if (total > c->pkstage_cap) {
+    // the last scatter may still read the device staging
+    VMP_HIP(hipStreamSynchronize(st));
+    if (c->h_pkstage) hipHostFree(c->h_pkstage);
+    if (c->d_pkstage) hipFree(c->d_pkstage);
+    c->h_pkstage = nullptr;
+    c->d_pkstage = nullptr;
+    c->pkstage_cap = 0;
+    const size_t cap = total + total / 4;
+    if (hipHostMalloc(&c->h_pkstage, cap) != hipSuccess || hipMalloc(&c->d_pkstage, cap) != hipSuccess) {
+      if (c->h_pkstage) hipHostFree(c->h_pkstage);
+      c->h_pkstage = nullptr;
+      c->d_pkstage = nullptr;
+      return VMP_ENOMEM;
+    }
+    c->pkstage_cap = cap;
+  }
+  if (!c->pkstage_done) VMP_HIP(hipEventCreateWithFlags(&c->pkstage_done, hipEventDisableTiming));
+  PkgMove* mv = reinterpret_cast<PkgMove*>(c->h_pkstage);
+  size_t o = head;
+  for (int i = 0; i < n; ++i) {
+    ImuPkg& p = *reinterpret_cast<ImuPkg*>(c->h_pkstage + o);
+    p.cloud_start = cstart[i];
+    p.cloud_end = cend[i];
+    p.m = m[i];
+    p.pad = 0;
+    const int64_t b = ioff[i];
+    for (int32_t k = 0; k < m[i]; ++k) {
+      p.s[k][0] = it[b + k];
+      for (int a = 0; a < 3; ++a) {
+        p.s[k][1 + a] = iacc[3 * (b + k) + a];
+        p.s[k][4 + a] = igyro[3 * (b + k) + a];
+      }
+    }
+    mv[i] = PkgMove{&ss[i]->imu->pkg, int64_t(o), int32_t(imu_pkg_bytes(m[i])), 0};
+    o += imu_pkg_bytes(m[i]);
+  }
+  VMP_HIP(hipMemcpyAsync(c->d_pkstage, c->h_pkstage, total, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_scatter_pkgs, dim3(n), dim3(256), 0, st, c->d_pkstage);
+  VMP_HIP(hipEventRecord(c->pkstage_done, st));
+  c->pkstage_pending = true;
+  return hipGetLastError() == hipSuccess ? VMP_OK : VMP_EDEVICE;
+}
+
 static int check_raw_args(vmp_session* s, int32_t n, const double* it, const double* iacc, const double* igyro,
                           int32_t m, double cloud_start, double cloud_end) {
   if (!s->imu_inited) {
@@ -1172,12 +1254,10 @@ extern "C" int vmp_scan_set_raw_device_batch(vmp_session** ss, int32_t n, const 
   }
   std::lock_guard<std::mutex> lk(c->mu);
   if (hipSetDevice(c->device) != hipSuccess) return VMP_EDEVICE;
+  const int rc = stage_imu_batch(c, ss, n, ioff.data(), imu_t, imu_acc, imu_gyro, imu_counts, cloud_start, cloud_end);
+  if (rc != VMP_OK) return rc;
   for (int i = 0; i < n; ++i) {
     vmp_session* s = ss[i];
-    const int64_t o = ioff[size_t(i)];
-    const int rc = stage_imu(s, imu_t + o, imu_acc + 3 * o, imu_gyro + 3 * o, imu_counts[i], cloud_start[i],
-                             cloud_end[i]);
-    if (rc != VMP_OK) return rc;
     s->rx = refs[size_t(i)].x;
     s->ry = refs[size_t(i)].y;
     s->rz = refs[size_t(i)].z;
