@@ -254,7 +254,7 @@ inline void select_stable(hipStream_t st, int n, Pred pred, Out out, int32_t* wo
 
 // largest input the primitives take; larger batches use rocPRIM's
 // device-wide routines
-constexpr int kPrimMaxItems = 1 << 23;
+constexpr int kPrimMaxItems = 1 << 20;
 
 // ---- stable LSD radix sort of (key, value) on key bits [0, bits)
 // the digit histograms of every pass over the whole input, added into
