@@ -172,6 +172,58 @@ def upload_packages(pkgs, first, dev_dtype="float32"):
     return dev
 
 
+def run_plain_evicting(ctx, dev, pkgs, N, capacity, F, W, K):
+    """The plain form on the same street packages with an LRU small enough
+    that the fill leaves it full: every timed scan creates and evicts voxels
+    (the LRU-eviction regime of the reference's map, voxel_map.cpp:232-256)."""
+    import torch
+
+    from voxelmapplus_fastlio2_amd import LIOConfig, Registrar
+
+    g = Registrar(ctx, LIOConfig(max_scan_points=N, map_capacity=capacity))
+    stride = dev.stride(1) * 4
+
+    def bind(k):
+        base = dev[k - 1].data_ptr()
+        _, _, _, _, imu, cs, ce = pkgs[k]
+        g.set_raw_device(base, base + stride, base + 2 * stride, base + 3 * stride, N, imu, cs, ce)
+
+    assert g.imu_init(pkgs[0][4], pkgs[0][6]), "IMU_INIT within the first package"
+    bind(1)
+    g.build()
+    for k in range(2, 2 + F + W):
+        bind(k)
+        g.register()
+    torch.cuda.synchronize()
+    size0 = g.map_size()
+    lat, evicted, created = [], 0, 0
+    t0 = time.perf_counter()
+    for k in range(2 + F + W, 2 + F + W + K):
+        ts = time.perf_counter()
+        bind(k)
+        _, m = g.register()
+        lat.append(time.perf_counter() - ts)
+        evicted += m.evicted
+        created += m.created
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    size1 = g.map_size()
+    g.close()
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "value": N * K / elapsed,
+        "unit": "points/s",
+        "ms_per_step": elapsed / K * 1e3,
+        "map_capacity": capacity,
+        "map_voxels": [size0, size1],
+        "per_scan_mean": {"created": created / K, "evicted": evicted / K},
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+    }
+    log(f"plain, LRU evicting ({capacity} voxels): {out['ms_per_step']:.3f} ms/scan, "
+        f"{out['per_scan_mean']['evicted']:.0f} evictions per scan")
+    return out
+
+
 def run_batched(ctx, B, W, K, capacity, nmax=120000, groups=1):
     """B independent sequences (room / street alternating), raw scans of
     sizes uniform in [2k, nmax] with their IMU packages, advanced together by
@@ -533,6 +585,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--points", type=int, default=120000)
     ap.add_argument("--capacity", type=int, default=2000000)
+    ap.add_argument("--evict-capacity", type=int, default=500000,
+                    help="LRU capacity of the second plain run, full after the fill (0: skip)")
     ap.add_argument("--fill", type=int, default=300,
                     help="scans registered before the warmup so the map reaches its LRU capacity")
     ap.add_argument("--roofline-scans", type=int, default=10)
@@ -567,7 +621,8 @@ def main():
     from voxelmapplus_fastlio2_amd.synth import SEED0, STREET, ImuSequence
 
     N, W, K, F, RF = args.points, args.warmup, args.steps, args.fill, args.roofline_scans
-    total = 2 + F + W + K + RF
+    RG = 10  # scans after the timed region whose graph-replayed stages are timed by their HIP events
+    total = 2 + F + W + K + RG + RF
     seq = ImuSequence(STREET, SEED0, N)
     log(f"generating {total} raw packages x {N} points")
     t0 = time.perf_counter()
@@ -643,6 +698,16 @@ def main():
     elapsed = time.perf_counter() - t_start
     gs1 = ctx.graph_stats()
     graphs = {k: gs1[k] - gs0[k] for k in gs1}
+    # the filter stage (IESKF begin, iterations, end) and the map stage as the
+    # timed region runs them (hipGraph replays), each between the library's
+    # HIP event pair, over RG more scans
+    stage_graph = {"filter": [], "map": []}
+    for k in range(2 + F + W + K, 2 + F + W + K + RG):
+        bind(k)
+        g.register()
+        f_ms, m_ms, _ = g.timing()
+        stage_graph["filter"].append(f_ms)
+        stage_graph["map"].append(m_ms)
     # roofline pass: the sequence continues for RF scans with per-launch HIP
     # events on the library's stream (the filter stage then launches kernel
     # by kernel instead of replaying its graph; the kernels are the same)
@@ -651,7 +716,7 @@ def main():
     rstat = {"points": 0, "raw_points": 0, "refits": 0, "merge_evals": 0, "created": 0, "evicted": 0,
              "touched": 0, "ieskf_bytes": 0, "pred_bytes": 0}
     ppt = tune["points_per_thread"] if tune else args.obs_ppt
-    for k in range(2 + F + W + K, total):
+    for k in range(2 + F + W + K + RG, total):
         bind(k)
         o, m = g.register()
         rstat["points"] += m.points
@@ -718,9 +783,13 @@ def main():
             "max": float(lat_ms.max()),
         },
         "stage_ms_per_scan": {
+            "filter_stage_graph": float(np.mean(stage_graph["filter"])),
+            "map_stage_graph": float(np.mean(stage_graph["map"])),
+            "graph_measured_over": f"{RG} scans after the timed region, hipGraph replays between the library's "
+            "HIP events (filter: IESKF begin, iterations and end; map: the whole map update)",
             "observe_kernels": tt["obs_ms"] / max(RF, 1),
-            "map_update": tt["map_ms"] / max(tt["map_calls"], 1),
-            "measured_over": f"{RF} scans after the timed region, per-launch HIP events",
+            "map_update_eager": tt["map_ms"] / max(tt["map_calls"], 1),
+            "measured_over": f"{RF} scans after those, launched kernel by kernel with per-launch HIP events",
         },
         "roofline": {
             "kernel": dom,
@@ -741,6 +810,8 @@ def main():
         "kernels": kr,
     }
     g.close()
+    if args.evict_capacity > 0:
+        line["plain_evicting"] = run_plain_evicting(ctx, dev, pkgs, N, args.evict_capacity, F, W, K)
     del dev
     if args.batch_sessions > 0:
         bt = run_batched(ctx, args.batch_sessions, args.batch_warmup, args.batch_steps, args.batch_capacity,
