@@ -512,7 +512,8 @@ static bool ws_layout(vmp_ctx* c, BatchWs& w, size_t n, int ns, size_t pre_cap, 
   ok = ok && dev(&b.seg_start, n) && dev(&b.seg_end, n) && dev(&b.seg_sess, n) && dev(&b.seg_vid, n) &&
        dev(&b.seg_entry, n);
   ok = ok && dev(&b.cre_t, n) && dev(&b.cre_seg, n) && dev(&b.co, n) && dev(&b.ev_vid, n);
-  ok = ok && dev(&b.cand_in, n) && dev(&b.cand_out, n) && dev(&b.cand_nbr, n * 6) && dev(&b.cand_spec, n) &&
+  ok = ok && dev(&b.cand_in, n) && dev(&b.cand_out, n) && dev(&b.cand_nbr, n * 6) && dev(&b.tnbr, n * 6) &&
+       dev(&b.cand_spec, n) &&
        dev(&b.cand_start, size_t(ns)) && dev(&b.spl, n);
   ok = ok && dev(&b.seg_ct, n * 7) && dev(&b.seg_nct, n) && dev(&b.seg_nbr, n * 6);
   ok = ok && dev(&b.len_in, n) && dev(&b.len_out, n) && dev(&b.ord_in, n) && dev(&b.ord, n);

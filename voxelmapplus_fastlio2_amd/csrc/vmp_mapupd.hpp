@@ -120,6 +120,11 @@ struct BatchBufs {
   uint64_t* cand_in;   // merge candidates, see cand_key(); per session at its offset
   uint64_t* cand_out;  // sorted: session-major, then point time, then voxel
   int32_t* cand_nbr;   // per sorted candidate: 6 eligible neighbour ids at its time
+  // per point (session offset + point time): the 6 eligible neighbour ids of
+  // the merge() call at that time, + 2 (0: not looked up yet in this scan);
+  // they cannot change during the merge replay, so a call that returns to
+  // the pool reads them back instead of repeating the lookups
+  int32_t* tnbr;
   int32_t* cand_spec;  // per sorted candidate: 1 if it merges on the pre-replay state
   int32_t* cand_start; // per session: first sorted candidate
   int32_t* spl;        // positions of candidates with cand_spec = 1, ascending
@@ -1703,11 +1708,16 @@ struct TailCandStarts {
   __device__ void operator()(int) const { cand_starts_block(maps, nsess, b); }
 };
 
-// merge candidates' key list cleared and their speculative results zeroed
+// merge candidates' key list cleared, their speculative results zeroed and
+// the per-call neighbour cache emptied
 __global__ void k_fill_cand(BatchBufs b, int n) {
   VMP_GRID_LOOP(i, n) {
     b.cand_in[i] = ~0ull;
     b.cand_spec[i] = 0;
+    int2* t = reinterpret_cast<int2*>(b.tnbr + size_t(i) * 6);
+    t[0] = make_int2(0, 0);
+    t[1] = make_int2(0, 0);
+    t[2] = make_int2(0, 0);
   }
 }
 

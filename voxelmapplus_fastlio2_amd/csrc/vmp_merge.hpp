@@ -622,10 +622,17 @@ __global__ void __launch_bounds__(kMrThreads) k_merge_replay(const MapDev* maps,
         nv = V;
       } else if (wpos[w] >= 0) {
         nv = b.cand_nbr[size_t(wpos[w]) * 6 + j];
-      } else {
-        const VoxState& vst = m.vs[V];
-        nv = (vst.tag == tag && vst.seg >= 0) ? eligible_at(m, b.seg_nbr[vst.seg * 6 + j], tag, t)
-                                              : nbr_at(m, vst.key, j, tag, t);
+      } else {  // a re-evaluation: the ids from its first evaluation in this scan, if any
+        int32_t* slot = b.tnbr + (size_t(off) + size_t(t)) * 6 + j;
+        const int32_t cached = *slot;
+        if (cached != 0) {
+          nv = cached - 2;
+        } else {
+          const VoxState& vst = m.vs[V];
+          nv = (vst.tag == tag && vst.seg >= 0) ? eligible_at(m, b.seg_nbr[vst.seg * 6 + j], tag, t)
+                                                : nbr_at(m, vst.key, j, tag, t);
+          *slot = nv + 2;
+        }
       }
       nbv[w][j] = nv;
       if (nv >= 0) {
