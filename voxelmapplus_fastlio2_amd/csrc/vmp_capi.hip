@@ -1613,7 +1613,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
     int max_iter = 1;
     for (int i = 0; i < n; ++i) max_iter = std::max(max_iter, ss[i]->cfg.max_iter);
     auto filter_stage = [&](int grid) {
-      hipLaunchKernelGGL(k_ieskf_begin, dim3(n), dim3(64), 0, st, w.d_flist, w.d_obsctr);
+      hipLaunchKernelGGL(k_ieskf_begin, dim3(n), dim3(kBeginEndThreads), 0, st, w.d_flist, w.d_obsctr);
       for (int it = 0; it < max_iter; ++it) {
         observe(grid, false, false);
         timed_launch(c, kTkIeskfStep, [&] {
@@ -1625,7 +1625,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
                                w.d_obsstat);
         });
       }
-      hipLaunchKernelGGL(k_ieskf_end, dim3(n), dim3(64), 0, st, w.d_flist);
+      hipLaunchKernelGGL(k_ieskf_end, dim3(n), dim3(kBeginEndThreads), 0, st, w.d_flist);
     };
     // per-launch timing (vmp_session_set_timing) records events between the
     // launches, so it takes the launch-by-launch path

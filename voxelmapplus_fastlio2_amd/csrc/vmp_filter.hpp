@@ -62,8 +62,11 @@ __global__ void k_zero_lists(int32_t** lists, int n) {
   if (int(threadIdx.x) < n) lists[s][threadIdx.x] = 0;
 }
 
-// start of an update: predict_x = x, P^-1, first observe parameters
-__global__ void __launch_bounds__(64) k_ieskf_begin(FilterDev** fs, int32_t** obs_ctr) {
+// start of an update: predict_x = x, P^-1, first observe parameters.  The
+// factor runs in the first wave's lanes; the inverse's product W^T W is
+// spread over all the workgroup's waves.
+constexpr int kBeginEndThreads = 256;
+__global__ void __launch_bounds__(kBeginEndThreads) k_ieskf_begin(FilterDev** fs, int32_t** obs_ctr) {
   FilterDev* f = fs[blockIdx.x];
   if (threadIdx.x < 8) obs_ctr[blockIdx.x][threadIdx.x] = 0;  // counters of the first observe pass
   __shared__ vmp_state sx;
@@ -76,7 +79,7 @@ __global__ void __launch_bounds__(64) k_ieskf_begin(FilterDev** fs, int32_t** ob
   __shared__ double S[23 * 24], Pi[529];
   chol_rows(a, dinv, lane);
   chol_inverse_rows(a, dinv, Pi, lane, S);  // P^-1
-  for (int i = lane; i < 529; i += 64) as_global(f->Pinv)[i] = Pi[i];
+  for (int i = lane; i < 529; i += kBeginEndThreads) as_global(f->Pinv)[i] = Pi[i];
   __syncthreads();
   load_state(sx, f->xp, lane);
   if (lane == 0) {
@@ -111,10 +114,16 @@ k_ieskf_step(FilterDev** fs, const ObsJob* __restrict__ jobs, int32_t** obs_ctr,
   __shared__ double hobs[156];
   __shared__ double red[ROW];
   __shared__ double gsum[kStepThreads / 32][32];
+  // P's first six columns (the blocks the observe parameters take) and the
+  // configuration, loaded with the rest up front for the final lane-0 work
+  __shared__ double Pc[6 * 23];
+  __shared__ vmp_map_cfg scfg;
   const int lane = threadIdx.x;  // lanes of the first wave carry the algebra
   load_state(f->x, sx, lane);
   load_state(f->xp, sxp, lane);
   for (int i = lane; i < 529; i += kStepThreads) Pi[i] = f->Pinv[i];
+  if (lane < 6 * 23) Pc[lane] = f->P[lane];
+  if (lane == 0) scfg = f->cfg;
   {
     const ObsJob& J = jobs[blockIdx.x];
     const auto part = as_global(J.partials);
@@ -259,8 +268,8 @@ k_ieskf_step(FilterDev** fs, const ObsJob* __restrict__ jobs, int32_t** obs_ctr,
     f->iters += 1;
     double mx = dl[0];
     for (int k = 1; k < 23; ++k) mx = mx > dl[k] ? mx : dl[k];
-    if (mx < 0.001 || f->iters >= f->cfg.max_iter) f->active = 0;  // signed maxCoeff (ieskf.cpp:148)
-    make_obs_params(sx, f->P, f->cfg, f->obs_prm);
+    if (mx < 0.001 || f->iters >= scfg.max_iter) f->active = 0;  // signed maxCoeff (ieskf.cpp:148)
+    make_obs_params(sx, Pc, scfg, f->obs_prm);
     f->obs_prm.active = f->active;
     int32_t* so = stat_out[blockIdx.x];
     int32_t* oc = obs_ctr[blockIdx.x];
@@ -277,8 +286,9 @@ k_ieskf_step(FilterDev** fs, const ObsJob* __restrict__ jobs, int32_t** obs_ctr,
   }
 }
 
-// P = L H_^-1 L^T (ieskf.cpp:151-155) and the insertion parameters
-__global__ void __launch_bounds__(64) k_ieskf_end(FilterDev** fs) {
+// P = L H_^-1 L^T (ieskf.cpp:151-155) and the insertion parameters; the
+// inverse's product and the two J products on all the workgroup's waves
+__global__ void __launch_bounds__(kBeginEndThreads) k_ieskf_end(FilterDev** fs) {
   FilterDev* f = fs[blockIdx.x];
   __shared__ double T[529], Hi[529];
   __shared__ double dl[23];
@@ -303,7 +313,7 @@ __global__ void __launch_bounds__(64) k_ieskf_end(FilterDev** fs) {
   // P = L H_^-1 L^T with L = J; J is the identity outside three small
   // blocks, so a block row or column adds up at most three products (in
   // ascending k, as the dense sum would)
-  for (int i = lane; i < 529; i += 64) {  // T = J Hi
+  for (int i = lane; i < 529; i += kBeginEndThreads) {  // T = J Hi
     const int r = i / 23, c = i % 23;
     double s;
     if (r >= 3 && r < 6) {
@@ -323,7 +333,7 @@ __global__ void __launch_bounds__(64) k_ieskf_end(FilterDev** fs) {
     T[i] = s;
   }
   __syncthreads();
-  for (int i = lane; i < 529; i += 64) {  // P = T J^T
+  for (int i = lane; i < 529; i += kBeginEndThreads) {  // P = T J^T
     const int r = i / 23, c = i % 23;
     double s;
     if (c >= 3 && c < 6) {

@@ -290,14 +290,15 @@ __device__ inline double chol_solve_rows(const double (&a)[23], double dinv, dou
   return y;
 }
 
-// A^-1 for A = L L^T from chol_rows (rows of L in lanes, dinv = 1/L[i][i]):
-// lane c forms column c of W = L^-1 by forward substitution, then
-// A^-1 = W^T W, whose row r is written by lane c into out[r * 23 + c]
-// (bitwise symmetric: both halves sum the same products in the same order).
-// L, 1/L[i][i] and then W are staged in S (23 * 24 doubles of LDS) and read
-// back as broadcasts; out should be LDS too (stores through a generic
-// pointer to global memory would wait together with those reads).  Called by
-// all threads of a 64-thread workgroup.
+// A^-1 for A = L L^T from chol_rows (rows of L in the first wave's lanes,
+// dinv = 1/L[i][i]): lane c of the first wave forms column c of W = L^-1 by
+// forward substitution, then A^-1 = W^T W, whose row r is written by lane c
+// of wave r mod (waves) into out[r * 23 + c] (bitwise symmetric: both halves
+// sum the same products in the same order).  L, 1/L[i][i] and then W are
+// staged in S (23 * 24 doubles of LDS) and read back as broadcasts; out
+// should be LDS too (stores through a generic pointer to global memory would
+// wait together with those reads).  Called by all threads of the workgroup
+// (one or more waves).
 __device__ inline void chol_inverse_rows(const double (&a)[23], double dinv, double* out, int lane, double* S) {
   if (lane < 23) {
 #pragma unroll
@@ -306,12 +307,14 @@ __device__ inline void chol_inverse_rows(const double (&a)[23], double dinv, dou
   }
   __syncthreads();
   double w[23];
+  if (lane < 64) {
 #pragma unroll
-  for (int i = 0; i < 23; ++i) {
-    double s = lane == i ? 1.0 : 0.0;
+    for (int i = 0; i < 23; ++i) {
+      double s = lane == i ? 1.0 : 0.0;
 #pragma unroll
-    for (int j = 0; j < i; ++j) s -= S[i * 24 + j] * w[j];
-    w[i] = s * S[i * 24 + 23];
+      for (int j = 0; j < i; ++j) s -= S[i * 24 + j] * w[j];
+      w[i] = s * S[i * 24 + 23];
+    }
   }
   __syncthreads();
   if (lane < 23) {
@@ -319,12 +322,17 @@ __device__ inline void chol_inverse_rows(const double (&a)[23], double dinv, dou
     for (int k = 0; k < 23; ++k) S[k * 24 + lane] = w[k];
   }
   __syncthreads();
+  const int l = lane & 63, nw = int(blockDim.x) >> 6;
+  if (l < 23) {
+    double wc[23];
 #pragma unroll
-  for (int r = 0; r < 23; ++r) {
-    double s = S[r] * w[0];
+    for (int k = 0; k < 23; ++k) wc[k] = S[k * 24 + l];
+    for (int r = lane >> 6; r < 23; r += nw) {
+      double s = S[r] * wc[0];
 #pragma unroll
-    for (int k = 1; k < 23; ++k) s += S[k * 24 + r] * w[k];
-    if (lane < 23) out[r * 23 + lane] = s;
+      for (int k = 1; k < 23; ++k) s += S[k * 24 + r] * wc[k];
+      out[r * 23 + l] = s;
+    }
   }
   __syncthreads();  // S may be reused by the caller
 }

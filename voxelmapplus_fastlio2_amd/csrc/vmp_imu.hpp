@@ -186,8 +186,9 @@ __global__ void k_imu_init(ImuDev* imu, FilterDev* f, const double* t, const dou
 //   A  lanes: each step's input (mid-point acc and gyro, dt), its rotation
 //      increment exp((gyro - bg) dt) and -Jr dt (IMU data and the biases,
 //      which do not change while a package is propagated)
-//   B  lane 0: the state chain in order (velocity, position, rotation),
-//      keeping the rotation before each step and the state after it
+//   B  the first wave: the state chain in order (velocity, position,
+//      rotation; the rotation's entries cross lanes by shuffles), keeping the
+//      rotation before each step and the state after it
 //   C  lanes: each step's blocks that depend on the rotation before it,
 //      (-rot hat(acc - ba)) dt and (-rot) dt, and the IMU pose table entry
 //      at the step's tail
@@ -526,33 +527,35 @@ k_imu_predict(const PreJob* jobs, int nsess) {
         vel = x.vel[lane];
         pos = x.pos[lane];
       }
-      for (int j = 0; j < cn; ++j) {
-        if (lane < 9) S.R[j][lane] = rv;
-        __syncthreads();
-        const double* R = S.R[j];
-        const double* E = S.E[j];
-        if (lane < 3) {
-          const double dt = S.dt[j];
-          const double ab0 = S.acc[j][0] - x.ba[0], ab1 = S.acc[j][1] - x.ba[1], ab2 = S.acc[j][2] - x.ba[2];
-          double ra = R[lane * 3] * ab0;
-          ra += R[lane * 3 + 1] * ab1;
-          ra += R[lane * 3 + 2] * ab2;
-          const double dvel = dt * (ra + x.g[lane]);
-          const double dpos = dt * vel;
-          pos = pos + dpos;
-          vel = vel + dvel;
-          S.vel[j + 1][lane] = vel;
-          S.pos[j + 1][lane] = pos;
+      if (lane < 64) {  // the first wave alone: the rotation's entries cross lanes by shuffles
+        const int r = lane < 9 ? lane / 3 : 0, c = lane % 3, vr = lane < 3 ? lane : 0;
+        for (int j = 0; j < cn; ++j) {
+          if (lane < 9) S.R[j][lane] = rv;  // the rotation before step j (lane k: entry k, row-major)
+          const double Rv0 = __shfl(rv, vr * 3, 64), Rv1 = __shfl(rv, vr * 3 + 1, 64), Rv2 = __shfl(rv, vr * 3 + 2, 64);
+          const double Rr0 = __shfl(rv, r * 3, 64), Rr1 = __shfl(rv, r * 3 + 1, 64), Rr2 = __shfl(rv, r * 3 + 2, 64);
+          const double* E = S.E[j];
+          if (lane < 3) {
+            const double dt = S.dt[j];
+            const double ab0 = S.acc[j][0] - x.ba[0], ab1 = S.acc[j][1] - x.ba[1], ab2 = S.acc[j][2] - x.ba[2];
+            double ra = Rv0 * ab0;
+            ra += Rv1 * ab1;
+            ra += Rv2 * ab2;
+            const double dvel = dt * (ra + x.g[lane]);
+            const double dpos = dt * vel;
+            pos = pos + dpos;
+            vel = vel + dvel;
+            S.vel[j + 1][lane] = vel;
+            S.pos[j + 1][lane] = pos;
+          }
+          if (lane < 9) {
+            double v = Rr0 * E[c];
+            v += Rr1 * E[3 + c];
+            v += Rr2 * E[6 + c];
+            rv = v;
+          }
         }
-        if (lane < 9) {
-          const int r = lane / 3, c = lane % 3;
-          double v = R[r * 3] * E[c];
-          v += R[r * 3 + 1] * E[3 + c];
-          v += R[r * 3 + 2] * E[6 + c];
-          rv = v;
-        }
+        if (lane < 9) S.R[cn][lane] = rv;
       }
-      if (lane < 9) S.R[cn][lane] = rv;
       __syncthreads();
       if (lane < 9) x.rot[(lane % 3) * 3 + lane / 3] = rv;
       if (lane < 3) {

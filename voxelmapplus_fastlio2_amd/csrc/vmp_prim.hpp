@@ -145,14 +145,16 @@ __device__ inline uint32_t look_back_wave(const uint32_t* status, int tile, Prim
 }
 // end of a tile: the last to finish clears `nclear` status words (and the
 // extra words, if any) and the done counter; returns true in that tile
-// (every thread of it).  With `publish`, every thread makes its stores
-// visible device-wide before the tile counts itself done, and the last tile
-// acquires them, so it may read what every tile wrote.
+// (every thread of it).  With `publish`, the tile's stores are released
+// device-wide before it counts itself done (the workgroup barrier orders
+// every thread's stores before thread 0's agent-scope release fence, which
+// the memory model makes cumulative), and the last tile acquires them, so it
+// may read what every tile wrote.
 __device__ inline bool finish_tile(PrimCtl* ctl, int nt, uint32_t* status, int nclear, int32_t* extra, int nextra,
                                    int* s_last, bool publish = false) {
-  if (publish) __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) {
+    if (publish) __threadfence();
     *s_last = atomicAdd(&ctl->done, 1) == nt - 1;
     if (publish && *s_last) __threadfence();
   }
