@@ -282,6 +282,12 @@ int vmp_ctx_reserve(vmp_ctx* ctx, int64_t points, int32_t sessions);
  *              launch runs this many times back to back between one event
  *              pair (the kernel is idempotent), then once more for the
  *              pass's counters; the timing reports the mean (default 1).
+ *   "replay_wave_max" 0..2^20: a map update with at most this many
+ *              refitting voxels replays each of them on its own wave
+ *              (default 4096); with more, only the voxels of
+ *              "replay_long_seg" points or more do, and the others walk one
+ *              per lane.  Both forms give the same bits.
+ *   "replay_long_seg" 2..62, or 0 for none (default 24).
  * Applies to later calls. */
 int vmp_ctx_set_param(vmp_ctx* ctx, const char* name, int64_t value);
 /* Graph use of a context so far (no reference counterpart): out[0] filter

@@ -152,6 +152,52 @@ def test_map_graph_matches_eager():
     ctx_e.close()
 
 
+def test_replay_forms_match():
+    """The refitting voxels' replay in its two forms, one wave per voxel
+    (moments per lane, refits side by side from snapshots) and one lane per
+    voxel (a sequential walk), gives bitwise the same map: a batch of mixed
+    sessions registered with every voxel on its own wave (the default at
+    these sizes), with every voxel on a lane, and with a split at 8 points.
+    The MAP_INIT builds take the same forms."""
+    nscans = 5
+    forms = {"wave": {}, "lane": {"replay_wave_max": 0, "replay_long_seg": 0},
+             "split": {"replay_wave_max": 0, "replay_long_seg": 8}}
+    ctxs, sess = {}, {}
+    for name, params in forms.items():
+        ctxs[name] = Context(0)
+        for k, v in params.items():
+            ctxs[name].set_param(k, v)
+        sess[name] = [make(ctxs[name], *sp) for sp in SPECS]
+    refits = 0
+    for k in range(1, nscans):
+        stats = {}
+        for name, group in sess.items():
+            mot = [seq.motion(k) for seq, _ in group]
+            apply_motion_batch([g for _, g in group], [a for a, _ in mot], [b for _, b in mot], QDIAG)
+            for (seq, g) in group:
+                g.set_points(*seq.scan(k))
+            stats[name] = register_batch([g for _, g in group])[1]
+        for i, (_, ref) in enumerate(sess["wave"]):
+            sa, Pa = ref.get_state()
+            ea = ref.export_map()
+            for name in ("lane", "split"):
+                g = sess[name][i][1]
+                sb, Pb = g.get_state()
+                assert np.array_equal(sa, sb) and np.array_equal(Pa, Pb), f"session {i} scan {k}: {name} differs"
+                eb = g.export_map()
+                for key in ("keys", "ints", "group", "dbl"):
+                    assert np.array_equal(ea[key], eb[key]), f"session {i} scan {k}: {name} map '{key}' differs"
+                for f, _ in stats["wave"][i]._fields_:
+                    assert getattr(stats["wave"][i], f) == getattr(stats[name][i], f), f"{name} stat {f}"
+            refits += stats["wave"][i].refits
+        print(f"scan {k}: replay forms identical; refits", [m.refits for m in stats["wave"]], flush=True)
+    assert refits > 0
+    for name in sess:
+        for _, g in sess[name]:
+            g.close()
+        ctxs[name].close()
+
+
 def test_repeated_observe_timing_leaves_results_unchanged():
     """Per-launch timing with obs_time_reps > 1 runs every observe launch
     several times back to back (the kernel is idempotent) and once more for

@@ -224,6 +224,8 @@ struct vmp_ctx {
   // observe launches back to back between one event pair (the kernel is
   // idempotent: the repetitions rewrite the same slots and partial rows)
   int obs_time_reps = 1;
+  int replay_wave_max = 4096;  // refitting voxels per step up to which each replays on its own wave
+  int replay_long_seg = 24;    // beyond that, segments of this many points still do (0: none)
   hipEvent_t tev[2 * 64]{};
   int tev_used = 0;
   // per-kernel event pairs of the timed kernels below (while timing is on)
@@ -1662,6 +1664,9 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
     // the stage's launches; `lbits` etc. depend only on the step's shape
     int lbits = 1;  // replay order by segment length, capped at 6 bits
     while ((int64_t(1) << lbits) <= int64_t(ntot) && lbits < 6) ++lbits;
+    // long segments sort first only below the length clamp (2^lbits - 2)
+    const int long_seg = c->replay_long_seg < (1 << lbits) - 1 ? c->replay_long_seg : 0;
+    const int wave_max = c->replay_wave_max;
     // batches up to kPrimMaxItems points use this library's compaction and
     // radix sort (kernels only, so the stage can be captured); larger ones
     // rocPRIM's device-wide routines
@@ -1721,12 +1726,16 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         // replay order by segment length in 6 bits (one radix pass; segments
         // of 62 points and more share the first key): the order only spreads
         // the work evenly over waves, every segment replays on its own
-        hipLaunchKernelGGL(k_seg_lenkeys, dim3(gn), dim3(T), 0, st, M, S, b, lbits);
+        hipLaunchKernelGGL(k_seg_lenkeys, dim3(gn), dim3(T), 0, st, M, S, b, lbits, long_seg);
         if (!sort_pairs(b.len_in, b.ord_in, b.len_out, b.ord, lbits + 1)) return VMP_EDEVICE;
-        // the moment path runs one wave per SIMD: a grid of four waves per CU
-        // strides over the refitting voxels; the light path covers the rest
-        timed_launch(c, kTkReplayHeavy,
-                     [&] { hipLaunchKernelGGL(k_replay_heavy, dim3(std::min(gn, 256)), dim3(T), 0, st, M, S, b); });
+        // the moment path: a voxel per wave up to wave_max refitting voxels
+        // (a grid of up to 4096 waves strides over them), else for the long
+        // segments only and 64 voxels per wave for the rest; the light path
+        // covers the voxels that do not refit
+        timed_launch(c, kTkReplayHeavy, [&] {
+          hipLaunchKernelGGL(k_replay_heavy_wave, dim3(std::min(gn, 1024)), dim3(T), 0, st, M, S, b, wave_max);
+          hipLaunchKernelGGL(k_replay_heavy, dim3(std::min(gn, 256)), dim3(T), 0, st, M, S, b, wave_max);
+        });
         timed_launch(c, kTkReplayLight, [&] { hipLaunchKernelGGL(k_replay_light, dim3(gn), dim3(T), 0, st, M, S, b); });
         if (!(mode & kStepBuild)) {
           hipLaunchKernelGGL(k_fill_cand, dim3(gn), dim3(T), 0, st, b, ntot);
@@ -1778,7 +1787,8 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
     };
     const int64_t key[3] = {int64_t(ntot), int64_t(n),
                             int64_t(hbits) | (int64_t(sbits) << 8) | (int64_t(tbits) << 16) |
-                                (int64_t(lbits) << 24) | (int64_t(mode & kStepBuild) << 32)};
+                                (int64_t(lbits) << 24) | (int64_t(mode & kStepBuild) << 32) |
+                                (int64_t(long_seg) << 36) | (int64_t(wave_max) << 42)};
     hipGraphExec_t mexec = (c->use_graph && c->use_map_graph && !c->timing) ? map_graph(c, key, map_stage) : nullptr;
     if (mexec) {
       if (hipGraphLaunch(mexec, st) != hipSuccess) return VMP_EDEVICE;
@@ -2530,6 +2540,16 @@ extern "C" int vmp_ctx_set_param(vmp_ctx* c, const char* name, int64_t value) {
   if (std::strcmp(name, "obs_time_reps") == 0) {
     if (value < 1 || value > 64) return VMP_EINVAL;
     c->obs_time_reps = int(value);
+    return VMP_OK;
+  }
+  if (std::strcmp(name, "replay_wave_max") == 0) {
+    if (value < 0 || value > (int64_t(1) << 20)) return VMP_EINVAL;
+    c->replay_wave_max = int(value);
+    return VMP_OK;
+  }
+  if (std::strcmp(name, "replay_long_seg") == 0) {
+    if (value != 0 && (value < 2 || value > 62)) return VMP_EINVAL;
+    c->replay_long_seg = int(value);
     return VMP_OK;
   }
   return VMP_EINVAL;

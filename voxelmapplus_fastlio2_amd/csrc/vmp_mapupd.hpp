@@ -474,7 +474,10 @@ __global__ void k_ins_keys(const MapDev* maps, const SessInfo* si, int nsess, Ba
 // the key are k_ins_keys' expressions, so the voxel is the one the point was
 // keyed to.
 __global__ void k_seg_heads(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) b.cnt[5] = 0;  // refitting voxels, counted by k_seg_lenkeys
+  if (blockIdx.x == 0 && threadIdx.x == 0) {  // refitting voxels, counted by k_seg_lenkeys
+    b.cnt[5] = 0;
+    b.cnt[6] = 0;
+  }
   VMP_GRID_LOOP(j, b.ntot) {
     const int s = sess_of(si, nsess, j);
     const SessInfo& S = si[s];
@@ -1198,35 +1201,55 @@ __device__ inline void mom_store(double* dst, const Moments& M) {
   for (int a = 0; a < 6; ++a)
     for (int l = 0; l < 6; ++l) dst[o++] = M.c2[a][l];
 }
-// The products are fused into the sums (one rounding per term instead of
-// two): the moments are this library's own intermediate, not a quantity
-// the reference rounds in a particular order.
+// A point's contribution to every moment is one fused multiply-add of two
+// of 16 factors, F = (1, q, q q^T (rounded, xx xy xz yy yz zz), C): each
+// moment c takes F[mom_i1(c)] * F[mom_i2(c)].  Both the per-lane walk and
+// the per-wave walk accumulate exactly these terms in point order, so the
+// two give the same bits.  The moments are this library's own intermediate,
+// not a quantity the reference rounds in a particular order.
+constexpr int kMomC = 69;  // moment components (S1q S2q S0C S1C S2C)
+constexpr int kMomF = 16;  // factors per point
+__host__ __device__ constexpr int mom_i1(int c) {
+  return c < 3 ? 1 + c
+               : (c < 9 ? 1 + (c - 3 < 3 ? 0 : (c - 3 < 5 ? 1 : 2))
+                        : (c < 15 ? 10 + (c - 9) : (c < 33 ? 1 + (c - 15) / 6 : 4 + (c - 33) / 6)));
+}
+__host__ __device__ constexpr int mom_i2(int c) {
+  return c < 3 ? 0
+               : (c < 9 ? 1 + (c - 3 < 3 ? c - 3 : (c - 3 < 5 ? c - 5 : 2))
+                        : (c < 15 ? 0 : 10 + (c < 33 ? (c - 15) % 6 : (c - 33) % 6)));
+}
+__device__ inline void mom_factors(const float q[3], const float C[6], double F[kMomF]) {
+  F[0] = 1.0;
+  for (int a = 0; a < 3; ++a) F[1 + a] = double(q[a]);
+  for (int a = 0; a < 3; ++a)
+    for (int bb = a; bb < 3; ++bb) F[4 + s6(a, bb)] = F[1 + a] * F[1 + bb];
+  for (int l = 0; l < 6; ++l) F[10 + l] = double(C[l]);
+}
 __device__ inline void mom_add(Moments& M, const float q[3], const float C[6]) {
-#pragma clang fp contract(fast)
-  const double qd[3] = {double(q[0]), double(q[1]), double(q[2])};
-  double qq[6];
+  double F[kMomF];
+  mom_factors(q, C, F);
+  for (int a = 0; a < 3; ++a) M.s1[a] = fma(F[mom_i1(a)], F[mom_i2(a)], M.s1[a]);
+  for (int a = 0; a < 6; ++a) M.s2[a] = fma(F[mom_i1(3 + a)], F[mom_i2(3 + a)], M.s2[a]);
+  for (int l = 0; l < 6; ++l) M.c0[l] = fma(F[mom_i1(9 + l)], F[mom_i2(9 + l)], M.c0[l]);
   for (int a = 0; a < 3; ++a)
-    for (int bb = a; bb < 3; ++bb) qq[s6(a, bb)] = qd[a] * qd[bb];
-  double Cd[6];
-  for (int l = 0; l < 6; ++l) Cd[l] = double(C[l]);
-  for (int a = 0; a < 3; ++a) M.s1[a] += qd[a];
-  for (int a = 0; a < 6; ++a) M.s2[a] += qq[a];
-  for (int l = 0; l < 6; ++l) M.c0[l] += Cd[l];
-  for (int a = 0; a < 3; ++a)
-    for (int l = 0; l < 6; ++l) M.c1[a][l] += qd[a] * Cd[l];
+    for (int l = 0; l < 6; ++l) M.c1[a][l] = fma(F[mom_i1(15 + 6 * a + l)], F[mom_i2(15 + 6 * a + l)], M.c1[a][l]);
   for (int a = 0; a < 6; ++a)
-    for (int l = 0; l < 6; ++l) M.c2[a][l] += qq[a] * Cd[l];
+    for (int l = 0; l < 6; ++l) M.c2[a][l] = fma(F[mom_i1(33 + 6 * a + l)], F[mom_i2(33 + 6 * a + l)], M.c2[a][l]);
 }
 
 // updatePlane (voxel_map.cpp:97-136) from moments.  Plane::cov is kept as the
 // traces of its two diagonal blocks: the normal block's trace
 // sum_i tr(F_i C_i F_i^T) is a quadratic form of d_i = p_i - mean, rebuilt
 // from sum q q C, sum q C, sum C; the centre block adds tr(C_i)/n^2.
-__device__ inline void refit(const MapDev& m, Moments& M, int n, uint32_t& flags, double* pl,
-                             const double center[3], int& refits) {
-  if (n < m.upd_pt) return;
-  flags |= kInit;
-  ++refits;
+struct RefitOut {
+  double tn, tm;   // increments of the two traces
+  double nrm[3];   // the oriented normal
+  bool plane;
+};
+__device__ inline RefitOut refit_eval(const MapDev& m, const Moments& M, int n, const double center[3]) {
+  RefitOut o;
+  o.plane = false;
   const double invn = 1.0 / double(n);
   double mu[3];
   for (int a = 0; a < 3; ++a) mu[a] = M.s1[a] * invn;
@@ -1235,60 +1258,78 @@ __device__ inline void refit(const MapDev& m, Moments& M, int n, uint32_t& flags
     for (int bb = 0; bb < 3; ++bb) cov[a * 3 + bb] = M.s2[s6(a, bb)] * invn - mu[a] * mu[bb];
   double ev[3], V[9];
   eigh3_dev(cov, ev, V);
-  if (ev[0] > m.plane_thresh) {
-    flags &= ~kPlane;
-    return;
-  }
-  flags |= kPlane;
+  if (ev[0] > m.plane_thresh) return o;
+  o.plane = true;
+  // the covariance moments are read after the eigen solve, not hoisted
+  // above it (they would hold 120 registers through the Jacobi sweeps)
+  asm volatile("" ::: "memory");
   // With d_i = q_i - mu and T[ab][lm] = sum_i d_a d_b C_lm
   //   = S2C[ab][lm] - mu_a S1C[b][lm] - mu_b S1C[a][lm] + mu_a mu_b S0C[lm],
-  // G(w,x)[ab] = sum_lm T[ab][lm] w_l x_m is formed from the moments
-  // contracted with the symmetric weights of w x^T (no T is materialised).
+  // and W(w,x) the symmetric weights of w x^T (xx xy xz yy yz zz; the
+  // off-diagonal ones w_a x_b + w_b x_a), the numerator for eigenvector e is
+  //   sum_{ab,lm} T[ab][lm] (W(n,n)[lm] W(e,e)[ab] + 2 W(e,n)[lm] W(e,n)[ab]
+  //                          + W(e,e)[lm] W(n,n)[ab]),
+  // formed one row ab of T at a time (T is not stored).
   const double nn[3] = {V[0], V[3], V[6]};
-  auto G = [&](const double* w, const double* x, double g[6]) {
-    double W6[6];
+  auto W = [](const double* w, const double* x, double w6[6]) {
     for (int l = 0; l < 3; ++l)
-      for (int mm = l; mm < 3; ++mm) W6[s6(l, mm)] = l == mm ? w[l] * x[l] : w[l] * x[mm] + w[mm] * x[l];
-    double c1w[3], c0w = 0.0;
-    for (int l = 0; l < 6; ++l) c0w += M.c0[l] * W6[l];
-    for (int a = 0; a < 3; ++a) {
-      double sacc = 0.0;
-      for (int l = 0; l < 6; ++l) sacc += M.c1[a][l] * W6[l];
-      c1w[a] = sacc;
-    }
-    for (int a = 0; a < 3; ++a)
-      for (int bb = a; bb < 3; ++bb) {
-        double sacc = 0.0;
-        for (int l = 0; l < 6; ++l) sacc += M.c2[s6(a, bb)][l] * W6[l];
-        g[s6(a, bb)] = sacc - mu[a] * c1w[bb] - mu[bb] * c1w[a] + mu[a] * mu[bb] * c0w;
-      }
+      for (int mm = l; mm < 3; ++mm) w6[s6(l, mm)] = l == mm ? w[l] * x[l] : w[l] * x[mm] + w[mm] * x[l];
   };
-  auto Q = [&](const double* g, const double* u, const double* v) {
-    double s = 0.0;
-    for (int a = 0; a < 3; ++a)
-      for (int bb = 0; bb < 3; ++bb) s += g[s6(a, bb)] * u[a] * v[bb];
-    return s;
-  };
-  double gnn[6];
-  G(nn, nn, gnn);
-  double tn_inc = 0.0;
-  for (int mm = 1; mm < 3; ++mm) {
-    const double e[3] = {V[mm], V[3 + mm], V[6 + mm]};
-    double gen[6], gee[6];
-    G(e, nn, gen);
-    G(e, e, gee);
-    const double num = Q(gnn, e, e) + 2.0 * Q(gen, e, nn) + Q(gee, nn, nn);
-    const double den = double(n) * (ev[0] - ev[mm]);
-    tn_inc += num / (den * den);
+  double Wnn[6], We[2][6], Wen[2][6];
+  W(nn, nn, Wnn);
+  for (int k = 0; k < 2; ++k) {
+    const double e[3] = {V[1 + k], V[4 + k], V[7 + k]};
+    W(e, e, We[k]);
+    W(e, nn, Wen[k]);
   }
-  const double tm_inc = (M.c0[0] + M.c0[3] + M.c0[5]) * invn * invn;
-  pl[6] += tn_inc;
-  pl[7] += tm_inc;
+  double num[2] = {0.0, 0.0};
+  for (int a = 0; a < 3; ++a)
+    for (int bb = a; bb < 3; ++bb) {
+      const int ab = s6(a, bb);
+      double r1 = 0.0, r2[2] = {0.0, 0.0}, r3[2] = {0.0, 0.0};
+      for (int l = 0; l < 6; ++l) {
+        const double t = M.c2[ab][l] - mu[a] * M.c1[bb][l] - mu[bb] * M.c1[a][l] + mu[a] * mu[bb] * M.c0[l];
+        r1 += t * Wnn[l];
+        for (int k = 0; k < 2; ++k) {
+          r2[k] += t * Wen[k][l];
+          r3[k] += t * We[k][l];
+        }
+      }
+      for (int k = 0; k < 2; ++k) num[k] += We[k][ab] * r1 + 2.0 * Wen[k][ab] * r2[k] + Wnn[ab] * r3[k];
+    }
+  double tn_inc = 0.0;
+  for (int k = 0; k < 2; ++k) {
+    const double den = double(n) * (ev[0] - ev[1 + k]);
+    tn_inc += num[k] / (den * den);
+  }
+  o.tn = tn_inc;
+  o.tm = (M.c0[0] + M.c0[3] + M.c0[5]) * invn * invn;
   double mw[3];
   for (int a = 0; a < 3; ++a) mw[a] = center[a] + mu[a];
   double axis = -(mw[0] * nn[0] + mw[1] * nn[1] + mw[2] * nn[2]);
   const double sg = axis < 0.0 ? -1.0 : 1.0;
-  for (int a = 0; a < 3; ++a) pl[3 + a] = sg * nn[a];
+  for (int a = 0; a < 3; ++a) o.nrm[a] = sg * nn[a];
+  return o;
+}
+// A refit's effect on the voxel: every refit initialises it; a plane refit
+// adds to the two traces and replaces the normal, a non-plane one clears
+// the plane flag.  A voxel's refits apply in walk order.
+__device__ inline void refit_apply(const RefitOut& o, uint32_t& flags, double* pl) {
+  flags |= kInit;
+  if (!o.plane) {
+    flags &= ~kPlane;
+    return;
+  }
+  flags |= kPlane;
+  pl[6] += o.tn;
+  pl[7] += o.tm;
+  for (int a = 0; a < 3; ++a) pl[3 + a] = o.nrm[a];
+}
+__device__ inline void refit(const MapDev& m, Moments& M, int n, uint32_t& flags, double* pl,
+                             const double center[3], int& refits) {
+  if (n < m.upd_pt) return;
+  ++refits;
+  refit_apply(refit_eval(m, M, n, center), flags, pl);
 }
 
 // Whether this scan's walk of a voxel reaches a refit (updatePlane with
@@ -1306,19 +1347,67 @@ __device__ inline bool walk_refits(int n0, int newly0, uint32_t flags, int len, 
   return jr < (len < f + 1 ? len : f + 1);
 }
 
+// The whole walk of a voxel in this scan from counts alone (the state
+// machine of the lane walk in k_replay_heavy, voxel_map.cpp:42-95): the
+// walk index of the refit that initialises the voxel, then of the first
+// refit of the initialised voxel (one every update_size_thresh points after
+// it), of the point that freezes it, how many points the walk adds before a
+// frozen voxel stops it, and newly_add after it.  A build adds every point
+// and refits once at the end (not listed here).
+struct WalkPlan {
+  int walk;      // points the walk adds
+  int init_j;    // refit that initialises the voxel (-1: none)
+  int per_j;     // first refit of the initialised voxel (kNever: none)
+  int freeze_j;  // point that freezes the voxel (-1: none)
+  int newly;     // newly_add after the walk
+  bool stopped;  // the segment's rest reaches a frozen voxel
+};
+__device__ inline WalkPlan walk_plan(int n0, int newly0, uint32_t flags, int len, int upd, int mx, bool build) {
+  WalkPlan p{len, -1, kNever, -1, newly0, false};
+  if (build) return p;
+  int js = 0;  // first walk index on the initialised path
+  if (!(flags & kInit)) {
+    const int ji = max(0, upd - n0 - 1);  // n reaches update_size_thresh
+    if (ji >= len) return p;
+    p.init_j = ji;
+    js = ji + 1;
+  }
+  if (!(flags & kEnable)) {  // frozen once initialised
+    p.walk = min(len, js);
+    p.stopped = js < len;
+    return p;
+  }
+  const int jf = max(js, mx - n0 - 1);  // n reaches max_point_thresh
+  p.walk = min(len, jf + 1);
+  p.stopped = jf + 1 < len;
+  if (jf < len) p.freeze_j = jf;
+  if (p.walk > js) {
+    const int j1 = js + max(0, upd - 1 - newly0);
+    if (j1 < p.walk) {
+      p.per_j = j1;
+      p.newly = (p.walk - 1 - j1) % upd;
+    } else {
+      p.newly = newly0 + (p.walk - js);
+    }
+  }
+  return p;
+}
+
 // Replay order: the voxels that refit first (the moment path), then the
 // rest, each part by descending segment length so waves walk segments of
 // similar length; slots past the segment count sort last.  Lengths are
 // clamped below 2^lbits - 1; the class is bit lbits.  Also counts the
-// refitting voxels (cnt[5], cleared by k_seg_heads).
+// refitting voxels (cnt[5]) and those of them with at least long_seg points
+// (cnt[6], the first ones of the order when long_seg < 2^lbits - 1; 0 counts
+// none); both cleared by k_seg_heads.
 __global__ void __launch_bounds__(kMapThreads) k_seg_lenkeys(const MapDev* maps, const SessInfo* si, BatchBufs b,
-                                                             int lbits) {
-  __shared__ int bsum;
+                                                             int lbits, int long_seg) {
+  __shared__ int bsum, lsum;
   const int nseg = b.cnt[0];
   const uint32_t top = uint32_t((uint64_t(1) << lbits) - 1);
-  if (threadIdx.x == 0) bsum = 0;
+  if (threadIdx.x == 0) bsum = lsum = 0;
   __syncthreads();
-  int heavy_n = 0;
+  int heavy_n = 0, long_n = 0;
   VMP_GRID_LOOP(q, b.ntot) {
     uint32_t key = (1u << lbits) | top;
     if (q < nseg) {
@@ -1332,147 +1421,358 @@ __global__ void __launch_bounds__(kMapThreads) k_seg_lenkeys(const MapDev* maps,
         heavy = walk_refits(st.n, st.newly_add, st.flags, len, m.upd_pt, m.max_pt, si[s].is_build != 0) ? 1 : 0;
       }
       heavy_n += heavy;
+      long_n += heavy & int(long_seg > 0 && len >= long_seg);
       key = (uint32_t(1 - heavy) << lbits) | (top - min(uint32_t(len), top - 1));
     }
     b.len_in[q] = key;
     b.ord_in[q] = q;
   }
   heavy_n = wave_sum_i(heavy_n);
+  long_n = wave_sum_i(long_n);
   if ((threadIdx.x & 63) == 0 && heavy_n) atomicAdd(&bsum, heavy_n);
+  if ((threadIdx.x & 63) == 0 && long_n) atomicAdd(&lsum, long_n);
   __syncthreads();
   if (threadIdx.x == 0 && bsum) atomicAdd(&b.cnt[5], bsum);
+  if (threadIdx.x == 0 && lsum) atomicAdd(&b.cnt[6], lsum);
 }
 
 // The voxels that refit in this scan (the first cnt[5] of the replay
-// order): one lane per voxel replays its pending points and then its points
-// of this scan in order through the pushPoint state machine, with the FP64
-// moments in registers and refits from them.
-constexpr int kHeavyChunk = 4;  // points of a heavy voxel's walk loaded together
+// order) replay their pending points and then their points of this scan in
+// order through the pushPoint state machine, with FP64 moments and refits
+// from them, in one of two forms that give the same bits:
+//  - one wave per voxel (heavy_wave): lane c keeps moment c (lanes 0-4 also
+//    moment 64 + c) and adds the points' terms in order from factors that
+//    the wave stages in LDS 64 points at a time; at each refit point (known
+//    from counts, walk_plan) it stores a snapshot of the moments, and up to
+//    kSnap refits then run side by side, one per lane, applied in walk order.
+//    The chain is a few memory round trips and one refit, whatever the
+//    segment's length.
+//  - one lane per voxel (heavy_lane): a walk per lane, the refits of the
+//    wave's lanes taken together in rounds.  For steps with more than
+//    wave_max refitting voxels, those whose segment has fewer than long_seg
+//    points walk this way, so the FP64 refits run for 64 voxels at once.
+constexpr int kHeavyChunk = 4;  // points of a heavy voxel's lane walk loaded together
 constexpr int kLightChunk = 8;  // the same for the light walk
-__global__ void __launch_bounds__(kMapThreads) k_replay_heavy(const MapDev* maps, const SessInfo* si, BatchBufs b) {
-  const int nh = b.cnt[5];
-  VMP_UNIFORM_LOOP(qi, valid, nh) {
-    int refits = 0, calls = 0, dropped = 0;
-    const int q = valid ? b.ord[qi] : 0;
-    const int s = valid ? b.seg_sess[q] : 0;
-    const MapDev& m = maps[s];
-    const int32_t vid = valid ? b.seg_vid[q] : -1;
-    if (vid >= 0) {
-      const SessInfo& S = si[s];
-      const int start = b.seg_start[q];
-      const int end = b.seg_end[q];
-      VoxState st = m.vs[vid];
-      double* pl = m.plane + size_t(vid) * kPlaneD;
-      double* mo = m.mom + size_t(vid) * kMomD;
-      double plv[kPlaneD];
-      Moments M;
-      const int cnt = st.n > 0 ? int(mo[kMomPend]) : 0;
-      if (st.n > 0) {
-        for (int k = 0; k < kPlaneD; ++k) plv[k] = pl[k];
-      } else {  // a new voxel (its records were not cleared at creation)
-        for (int k = 0; k < kPlaneD; ++k) plv[k] = 0.0;
+constexpr int kSnap = 8;        // refits evaluated together by heavy_wave
+struct HeavyLds {
+  double F[64][kMomF];        // factors of 64 consecutive points of the walk
+  double snap[kSnap][kMomC];  // moments at refit points
+  int snapn[kSnap];           // point count at each of those refits
+};
+
+// LDS written by some lanes of a wave and read by others of the same wave
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ inline double readlane_d(double v, int k) {
+  const uint64_t u = uint64_t(__double_as_longlong(v));
+  const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u)), k));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u >> 32)), k));
+  return __longlong_as_double((long long)(uint64_t(lo) | (uint64_t(hi) << 32)));
+}
+
+__device__ inline void write_hot_pl(const MapDev& m, int32_t vid, uint32_t flags, const double* pl, int64_t kx,
+                                    int64_t ky, int64_t kz) {
+  VoxHot h;
+  h.mx = float(pl[0] - double(kx) * m.vs64);
+  h.my = float(pl[1] - double(ky) * m.vs64);
+  h.mz = float(pl[2] - double(kz) * m.vs64);
+  h.flags = flags;
+  h.nx = float(pl[3]);
+  h.ny = float(pl[4]);
+  h.nz = float(pl[5]);
+  h.pad = 0.f;
+  m.hot[vid] = h;
+}
+
+__device__ inline void heavy_lane(const MapDev& m, const SessInfo& S, const BatchBufs& b, int q, int32_t vid,
+                                  int& refits, int& calls, int& dropped) {
+  if (vid >= 0) {
+    const int start = b.seg_start[q];
+    const int end = b.seg_end[q];
+    VoxState st = m.vs[vid];
+    double* pl = m.plane + size_t(vid) * kPlaneD;
+    double* mo = m.mom + size_t(vid) * kMomD;
+    double plv[kPlaneD];
+    Moments M;
+    const int cnt = st.n > 0 ? int(mo[kMomPend]) : 0;
+    if (st.n > 0) {
+      for (int k = 0; k < kPlaneD; ++k) plv[k] = pl[k];
+    } else {  // a new voxel (its records were not cleared at creation)
+      for (int k = 0; k < kPlaneD; ++k) plv[k] = 0.0;
+    }
+    if (st.n - cnt > 0) mom_load(mo, M);
+    else mom_zero(M);  // no point has reached the moments yet
+    if (cnt > m.pend_cap) atomicOr(&m.ctr->error, 32);
+    const float* pd = m.pend + size_t(vid) * m.pend_cap * kPendF;
+    const int npd = min(cnt, m.pend_cap);
+    for (int k0 = 0; k0 < npd; k0 += kHeavyChunk) {  // a chunk of pending points loaded together
+      float v[kHeavyChunk][kPendF];
+#pragma unroll
+      for (int u = 0; u < kHeavyChunk; ++u) {
+        const int k = min(k0 + u, npd - 1);
+#pragma unroll
+        for (int a = 0; a < kPendF; ++a) v[u][a] = pd[k * kPendF + a];
       }
-      if (st.n - cnt > 0) mom_load(mo, M);
-      else mom_zero(M);  // no point has reached the moments yet
-      if (cnt > m.pend_cap) atomicOr(&m.ctr->error, 32);
-      const float* pd = m.pend + size_t(vid) * m.pend_cap * kPendF;
-      const int npd = min(cnt, m.pend_cap);
-      for (int k0 = 0; k0 < npd; k0 += kHeavyChunk) {  // a chunk of pending points loaded together
-        float v[kHeavyChunk][kPendF];
+#pragma unroll
+      for (int u = 0; u < kHeavyChunk; ++u)
+        if (k0 + u < npd) mom_add(M, v[u], v[u] + 3);
+    }
+    int64_t kx, ky, kz;
+    unpack_key(st.key, kx, ky, kz);
+    const double center[3] = {(double(kx) + 0.5) * m.vs64, (double(ky) + 0.5) * m.vs64,
+                              (double(kz) + 0.5) * m.vs64};
+    uint32_t flags = st.flags;
+    int n = st.n, newly = st.newly_add;
+    int merge_pos = kNever, freeze_t = -1;
+    // Points are replayed in rounds: each lane advances to its next refit
+    // (or the end of its segment), then the lanes that reached one refit
+    // together, so the wave runs the FP64 refit once per round instead of
+    // whenever any lane needs it.  Per lane the order of operations is the
+    // reference's: add the point, refit, then the freeze check.
+    int pos = start;
+    bool active = true;
+    while (__any(active)) {
+      bool need = false;
+      while (active && !need) {
+        // the next kHeavyChunk points of the segment, loaded together
+        // (contiguous in key order); a refit ends the chunk early
+        float4 qa[kHeavyChunk], qb[kHeavyChunk], qd[kHeavyChunk];
 #pragma unroll
         for (int u = 0; u < kHeavyChunk; ++u) {
-          const int k = min(k0 + u, npd - 1);
-#pragma unroll
-          for (int a = 0; a < kPendF; ++a) v[u][a] = pd[k * kPendF + a];
+          const size_t p3 = 3 * size_t(min(pos + u, end - 1));
+          qa[u] = b.qc[p3];
+          qb[u] = b.qc[p3 + 1];
+          qd[u] = b.qc[p3 + 2];
         }
 #pragma unroll
-        for (int u = 0; u < kHeavyChunk; ++u)
-          if (k0 + u < npd) mom_add(M, v[u], v[u] + 3);
-      }
-      int64_t kx, ky, kz;
-      unpack_key(st.key, kx, ky, kz);
-      const double center[3] = {(double(kx) + 0.5) * m.vs64, (double(ky) + 0.5) * m.vs64,
-                                (double(kz) + 0.5) * m.vs64};
-      uint32_t flags = st.flags;
-      int n = st.n, newly = st.newly_add;
-      int merge_pos = kNever, freeze_t = -1;
-      // Points are replayed in rounds: each lane advances to its next refit
-      // (or the end of its segment), then the lanes that reached one refit
-      // together, so the wave runs the FP64 refit once per round instead of
-      // whenever any lane needs it.  Per lane the order of operations is the
-      // reference's: add the point, refit, then the freeze check.
-      int pos = start;
-      bool active = true;
-      while (__any(active)) {
-        bool need = false;
-        while (active && !need) {
-          // the next kHeavyChunk points of the segment, loaded together
-          // (contiguous in key order); a refit ends the chunk early
-          float4 qa[kHeavyChunk], qb[kHeavyChunk], qd[kHeavyChunk];
-#pragma unroll
-          for (int u = 0; u < kHeavyChunk; ++u) {
-            const size_t p3 = 3 * size_t(min(pos + u, end - 1));
-            qa[u] = b.qc[p3];
-            qb[u] = b.qc[p3 + 1];
-            qd[u] = b.qc[p3 + 2];
+        for (int u = 0; u < kHeavyChunk; ++u) {
+          if (!active || need) break;
+          if (pos >= end) {
+            active = false;
+            break;
           }
-#pragma unroll
-          for (int u = 0; u < kHeavyChunk; ++u) {
-            if (!active || need) break;
-            if (pos >= end) {
-              active = false;
-              break;
+          if ((flags & kInit) && !(flags & kEnable) && !S.is_build) {
+            if (flags & kPlane) {
+              merge_pos = pos - start;
+              calls += end - pos;
+            } else {
+              dropped += end - pos;
             }
-            if ((flags & kInit) && !(flags & kEnable) && !S.is_build) {
-              if (flags & kPlane) {
-                merge_pos = pos - start;
-                calls += end - pos;
-              } else {
-                dropped += end - pos;
-              }
-              active = false;
-              break;
-            }
-            const int i = __float_as_int(qa[u].w);
-            ++pos;
-            const float qv[3] = {qa[u].x, qa[u].y, qa[u].z};
-            const float C[6] = {qb[u].x, qb[u].y, qb[u].z, qb[u].w, qd[u].x, qd[u].y};
-            mom_add(M, qv, C);
-            ++n;
-            if (S.is_build) continue;
-            if (!(flags & kInit)) {
-              // updatePlane after every point until the voxel initialises
-              need = n >= m.upd_pt;
-              continue;
-            }
-            ++newly;
-            if (newly >= m.upd_pt) {
-              need = true;
-              newly = 0;
-            }
-            if (n >= m.max_pt) {
-              flags &= ~kEnable;
-              freeze_t = i;
-            }
+            active = false;
+            break;
+          }
+          const int i = __float_as_int(qa[u].w);
+          ++pos;
+          const float qv[3] = {qa[u].x, qa[u].y, qa[u].z};
+          const float C[6] = {qb[u].x, qb[u].y, qb[u].z, qb[u].w, qd[u].x, qd[u].y};
+          mom_add(M, qv, C);
+          ++n;
+          if (S.is_build) continue;
+          if (!(flags & kInit)) {
+            // updatePlane after every point until the voxel initialises
+            need = n >= m.upd_pt;
+            continue;
+          }
+          ++newly;
+          if (newly >= m.upd_pt) {
+            need = true;
+            newly = 0;
+          }
+          if (n >= m.max_pt) {
+            flags &= ~kEnable;
+            freeze_t = i;
           }
         }
-        if (need) refit(m, M, n, flags, plv, center, refits);
       }
-      if (S.is_build) refit(m, M, n, flags, plv, center, refits);
-      const double invn = 1.0 / double(n);
-      for (int a = 0; a < 3; ++a) plv[a] = center[a] + M.s1[a] * invn;
-      mom_store(mo, M);
-      for (int a = 0; a < 3; ++a) mo[kMomRun + a] = M.s1[a];
-      mo[kMomPend] = 0.0;
-      for (int k = 0; k < kPlaneD; ++k) pl[k] = plv[k];
-      st.flags = flags;
-      st.n = n;
-      st.newly_add = newly;
-      st.freeze_t = freeze_t;
-      st.merge_pos = merge_pos;
-      m.vs[vid] = st;
-      write_hot(m, vid, flags);
+      if (need) refit(m, M, n, flags, plv, center, refits);
     }
+    if (S.is_build) refit(m, M, n, flags, plv, center, refits);
+    const double invn = 1.0 / double(n);
+    for (int a = 0; a < 3; ++a) plv[a] = fma(M.s1[a], invn, center[a]);
+    mom_store(mo, M);
+    for (int a = 0; a < 3; ++a) mo[kMomRun + a] = M.s1[a];
+    mo[kMomPend] = 0.0;
+    for (int k = 0; k < kPlaneD; ++k) pl[k] = plv[k];
+    st.flags = flags;
+    st.n = n;
+    st.newly_add = newly;
+    st.freeze_t = freeze_t;
+    st.merge_pos = merge_pos;
+    m.vs[vid] = st;
+    write_hot(m, vid, flags);
+  }
+}
+
+// The ks refits whose snapshots are in L, lane k evaluating the k-th, then
+// applied in order to the wave-uniform flags and plane record.
+__device__ inline void heavy_flush(const MapDev& m, HeavyLds& L, int ks, const double center[3], uint32_t& flags,
+                                   double plv[kPlaneD]) {
+  wave_lds_sync();
+  const int lane = threadIdx.x & 63;
+  RefitOut o{};
+  // the snapshot rows have the layout of Moments: the refit reads them from LDS
+  if (lane < ks) o = refit_eval(m, *reinterpret_cast<const Moments*>(L.snap[lane]), L.snapn[lane], center);
+  wave_lds_sync();
+  for (int k = 0; k < ks; ++k) {
+    RefitOut r;
+    r.plane = __builtin_amdgcn_readlane(int(o.plane), k) != 0;
+    r.tn = readlane_d(o.tn, k);
+    r.tm = readlane_d(o.tm, k);
+    for (int a = 0; a < 3; ++a) r.nrm[a] = readlane_d(o.nrm[a], k);
+    refit_apply(r, flags, plv);
+  }
+}
+
+// One voxel on one wave; the counts go to lane 0's refits/calls/dropped.
+__device__ inline void heavy_wave(const MapDev& m, const SessInfo& S, const BatchBufs& b, int q, HeavyLds& L,
+                                  int& refits, int& calls, int& dropped) {
+  const int lane = threadIdx.x & 63;
+  const int32_t vid = b.seg_vid[q];
+  const int start = b.seg_start[q];
+  const int len = b.seg_end[q] - start;
+  const VoxState st = m.vs[vid];
+  double* pl = m.plane + size_t(vid) * kPlaneD;
+  double* mo = m.mom + size_t(vid) * kMomD;
+  const int cnt = st.n > 0 ? int(mo[kMomPend]) : 0;
+  if (cnt > m.pend_cap && lane == 0) atomicOr(&m.ctr->error, 32);
+  const int npd = min(cnt, m.pend_cap);
+  // lane c holds moment c and, on lanes 0-4, moment 64 + c
+  const bool has_mom = st.n - cnt > 0;  // some point has reached the moments
+  const int c2 = 64 + min(lane, kMomC - 65);
+  double Ma = has_mom ? mo[lane] : 0.0;
+  double Mb = has_mom ? mo[c2] : 0.0;
+  const int ia1 = mom_i1(lane), ia2 = mom_i2(lane), ib1 = mom_i1(c2), ib2 = mom_i2(c2);
+  double plv[kPlaneD];
+  for (int k = 0; k < kPlaneD; ++k) plv[k] = st.n > 0 ? pl[k] : 0.0;  // a new voxel's record is not cleared
+  int64_t kx, ky, kz;
+  unpack_key(st.key, kx, ky, kz);
+  const double center[3] = {(double(kx) + 0.5) * m.vs64, (double(ky) + 0.5) * m.vs64, (double(kz) + 0.5) * m.vs64};
+  const bool build = S.is_build != 0;
+  const WalkPlan P = walk_plan(st.n, st.newly_add, st.flags, len, m.upd_pt, m.max_pt, build);
+  uint32_t flags = st.flags;
+  const float* pd = m.pend + size_t(vid) * m.pend_cap * kPendF;
+  // the pending points, then the walk's points of this scan
+  const int total = npd + P.walk;
+  int next = P.init_j >= 0 ? P.init_j : P.per_j;  // walk index of the next refit
+  int ks = 0, nref = 0;
+  for (int p0 = 0; p0 < total; p0 += 64) {
+    const int x = p0 + lane;
+    if (x < total) {
+      float qv[3], C[6];
+      if (x < npd) {
+        for (int a = 0; a < 3; ++a) qv[a] = pd[x * kPendF + a];
+        for (int l = 0; l < 6; ++l) C[l] = pd[x * kPendF + 3 + l];
+      } else {
+        const size_t p3 = 3 * size_t(start + x - npd);
+        const float4 qa = b.qc[p3], qb = b.qc[p3 + 1], qd = b.qc[p3 + 2];
+        qv[0] = qa.x, qv[1] = qa.y, qv[2] = qa.z;
+        C[0] = qb.x, C[1] = qb.y, C[2] = qb.z, C[3] = qb.w, C[4] = qd.x, C[5] = qd.y;
+      }
+      mom_factors(qv, C, L.F[lane]);
+    }
+    wave_lds_sync();
+    const int np = min(64, total - p0);
+    int t = 0;
+    while (t < np) {
+      for (; t < np && ks < kSnap; ++t) {
+        const double* f = L.F[t];
+        Ma = fma(f[ia1], f[ia2], Ma);
+        Mb = fma(f[ib1], f[ib2], Mb);
+        const int j = p0 + t - npd;
+        if (j == next) {
+          if (st.n + j + 1 >= m.upd_pt) {  // (the lane walk's refit() is a no-op below the threshold)
+            L.snap[ks][lane] = Ma;
+            if (lane < kMomC - 64) L.snap[ks][64 + lane] = Mb;
+            L.snapn[ks] = st.n + j + 1;
+            ++ks;
+            ++nref;
+          }
+          next = j == P.init_j ? P.per_j : j + m.upd_pt;
+        }
+      }
+      if (ks == kSnap) {
+        heavy_flush(m, L, ks, center, flags, plv);
+        ks = 0;
+      }
+    }
+    wave_lds_sync();
+  }
+  const int n = st.n + P.walk;
+  if (build && n >= m.upd_pt) {  // a build refits once with all the points
+    L.snap[ks][lane] = Ma;
+    if (lane < kMomC - 64) L.snap[ks][64 + lane] = Mb;
+    L.snapn[ks] = n;
+    ++ks;
+    ++nref;
+  }
+  if (ks) heavy_flush(m, L, ks, center, flags, plv);
+  const double invn = 1.0 / double(n);
+  for (int a = 0; a < 3; ++a) plv[a] = fma(readlane_d(Ma, a), invn, center[a]);
+  if (P.freeze_j >= 0) flags &= ~kEnable;
+  int merge_pos = kNever;
+  if (P.stopped) {
+    if (flags & kPlane) {
+      merge_pos = P.walk;
+      if (lane == 0) calls += len - P.walk;
+    } else if (lane == 0) {
+      dropped += len - P.walk;
+    }
+  }
+  const int freeze_t = P.freeze_j >= 0 ? __float_as_int(b.qc[3 * size_t(start + P.freeze_j)].w) : -1;
+  mo[lane] = Ma;
+  if (lane < kMomC - 64) mo[64 + lane] = Mb;
+  if (lane < 3) mo[kMomRun + lane] = Ma;
+  if (lane == 0) mo[kMomPend] = 0.0;
+  double pv = plv[0];
+  for (int k = 1; k < kPlaneD; ++k) pv = lane == k ? plv[k] : pv;
+  if (lane < kPlaneD) pl[lane] = pv;
+  if (lane == 0) {
+    VoxState& vs = m.vs[vid];
+    vs.flags = flags;
+    vs.n = n;
+    vs.newly_add = P.newly;
+    vs.freeze_t = freeze_t;
+    vs.merge_pos = merge_pos;
+    write_hot_pl(m, vid, flags, plv, kx, ky, kz);
+    refits += nref;
+  }
+}
+
+// The first nw voxels of the replay order, one per wave: every refitting
+// voxel when there are at most wave_max of them, else the long ones (cnt[6]).
+__global__ void __launch_bounds__(kMapThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_replay_heavy_wave(const MapDev* maps, const SessInfo* si, BatchBufs b, int wave_max) {
+  __shared__ HeavyLds lds[kMapThreads / 64];
+  constexpr int kWpb = kMapThreads / 64;
+  const int nh = b.cnt[5];
+  const int nw = nh <= wave_max ? nh : b.cnt[6];
+  const int wv = threadIdx.x >> 6;
+  for (int u = blockIdx.x * kWpb + wv; u < nw; u += gridDim.x * kWpb) {
+    int refits = 0, calls = 0, dropped = 0;
+    const int q = b.ord[u];
+    const int s = b.seg_sess[q];
+    heavy_wave(maps[s], si[s], b, q, lds[wv], refits, calls, dropped);
+    const MapDev& m = maps[s];
+    wave_fetch_add(&m.ctr->refits, refits);
+    wave_fetch_add(&m.ctr->merge_calls, calls);
+    wave_fetch_add(&m.ctr->dropped, dropped);
+  }
+}
+
+// The rest of the refitting voxels, one per lane.
+__global__ void __launch_bounds__(kMapThreads) k_replay_heavy(const MapDev* maps, const SessInfo* si, BatchBufs b,
+                                                              int wave_max) {
+  const int nh = b.cnt[5];
+  const int nw = nh <= wave_max ? nh : b.cnt[6];
+  VMP_UNIFORM_LOOP(i, valid, nh - nw) {
+    int refits = 0, calls = 0, dropped = 0;
+    const int q = valid ? b.ord[nw + i] : 0;
+    const int s = valid ? b.seg_sess[q] : 0;
+    heavy_lane(maps[s], si[s], b, q, valid ? b.seg_vid[q] : -1, refits, calls, dropped);
+    const MapDev& m = maps[s];
     wave_fetch_add(&m.ctr->refits, refits);
     wave_fetch_add(&m.ctr->merge_calls, calls);
     wave_fetch_add(&m.ctr->dropped, dropped);
