@@ -1453,7 +1453,8 @@ __global__ void __launch_bounds__(kMapThreads) k_seg_lenkeys(const MapDev* maps,
 //    points walk this way, so the FP64 refits run for 64 voxels at once.
 constexpr int kHeavyChunk = 4;  // points of a heavy voxel's lane walk loaded together
 constexpr int kLightChunk = 8;  // the same for the light walk
-constexpr int kSnap = 8;        // refits evaluated together by heavy_wave
+constexpr int kSnap = 16;       // refits evaluated together by heavy_wave
+constexpr int kAccChunk = 8;    // points whose factors heavy_wave reads together
 struct HeavyLds {
   double F[64][kMomF];        // factors of 64 consecutive points of the walk
   double snap[kSnap][kMomC];  // moments at refit points
@@ -1633,6 +1634,18 @@ __device__ inline void heavy_wave(const MapDev& m, const SessInfo& S, const Batc
   const int32_t vid = b.seg_vid[q];
   const int start = b.seg_start[q];
   const int len = b.seg_end[q] - start;
+  // segment point `lane` and pending point `lane`, loaded with the voxel's
+  // state (before the pending count is known) for the first 64 rows
+  float4 s0a = make_float4(0.f, 0.f, 0.f, 0.f), s0b = s0a, s0d = s0a;
+  if (lane < len) {
+    const size_t p3 = 3 * size_t(start + lane);
+    s0a = b.qc[p3];
+    s0b = b.qc[p3 + 1];
+    s0d = b.qc[p3 + 2];
+  }
+  const float* pd = m.pend + size_t(vid) * m.pend_cap * kPendF;
+  float pv0[kPendF];
+  for (int a = 0; a < kPendF; ++a) pv0[a] = lane < m.pend_cap ? pd[lane * kPendF + a] : 0.f;
   const VoxState st = m.vs[vid];
   double* pl = m.plane + size_t(vid) * kPlaneD;
   double* mo = m.mom + size_t(vid) * kMomD;
@@ -1653,14 +1666,20 @@ __device__ inline void heavy_wave(const MapDev& m, const SessInfo& S, const Batc
   const bool build = S.is_build != 0;
   const WalkPlan P = walk_plan(st.n, st.newly_add, st.flags, len, m.upd_pt, m.max_pt, build);
   uint32_t flags = st.flags;
-  const float* pd = m.pend + size_t(vid) * m.pend_cap * kPendF;
-  // the pending points, then the walk's points of this scan
+  // the pending points, then the walk's points of this scan, as rows of F
+  // 64 at a time
   const int total = npd + P.walk;
   int next = P.init_j >= 0 ? P.init_j : P.per_j;  // walk index of the next refit
   int ks = 0, nref = 0;
+  if (lane < npd) mom_factors(pv0, pv0 + 3, L.F[lane]);
+  if (lane < P.walk && npd + lane < 64) {
+    const float qv[3] = {s0a.x, s0a.y, s0a.z};
+    const float C[6] = {s0b.x, s0b.y, s0b.z, s0b.w, s0d.x, s0d.y};
+    mom_factors(qv, C, L.F[npd + lane]);
+  }
   for (int p0 = 0; p0 < total; p0 += 64) {
     const int x = p0 + lane;
-    if (x < total) {
+    if (p0 > 0 && x < total) {
       float qv[3], C[6];
       if (x < npd) {
         for (int a = 0; a < 3; ++a) qv[a] = pd[x * kPendF + a];
@@ -1675,13 +1694,31 @@ __device__ inline void heavy_wave(const MapDev& m, const SessInfo& S, const Batc
     }
     wave_lds_sync();
     const int np = min(64, total - p0);
-    int t = 0;
-    while (t < np) {
-      for (; t < np && ks < kSnap; ++t) {
-        const double* f = L.F[t];
-        Ma = fma(f[ia1], f[ia2], Ma);
-        Mb = fma(f[ib1], f[ib2], Mb);
-        const int j = p0 + t - npd;
+    for (int t0 = 0; t0 < np; t0 += kAccChunk) {
+      // the refits among the chunk's points must fit beside the snapshots held
+      int nc = 0;
+      for (int nx = next, j0 = p0 + t0 - npd; nx < j0 + kAccChunk && nx < P.walk;
+           nx = nx == P.init_j ? P.per_j : nx + m.upd_pt)
+        ++nc;
+      if (ks + nc > kSnap) {
+        heavy_flush(m, L, ks, center, flags, plv);
+        ks = 0;
+      }
+      double fa[kAccChunk][2], fb[kAccChunk][2];  // the chunk's factors, read together
+#pragma unroll
+      for (int u = 0; u < kAccChunk; ++u) {
+        const double* f = L.F[min(t0 + u, np - 1)];
+        fa[u][0] = f[ia1];
+        fa[u][1] = f[ia2];
+        fb[u][0] = f[ib1];
+        fb[u][1] = f[ib2];
+      }
+#pragma unroll
+      for (int u = 0; u < kAccChunk; ++u) {
+        if (t0 + u >= np) break;
+        Ma = fma(fa[u][0], fa[u][1], Ma);
+        Mb = fma(fb[u][0], fb[u][1], Mb);
+        const int j = p0 + t0 + u - npd;
         if (j == next) {
           if (st.n + j + 1 >= m.upd_pt) {  // (the lane walk's refit() is a no-op below the threshold)
             L.snap[ks][lane] = Ma;
@@ -1693,12 +1730,12 @@ __device__ inline void heavy_wave(const MapDev& m, const SessInfo& S, const Batc
           next = j == P.init_j ? P.per_j : j + m.upd_pt;
         }
       }
-      if (ks == kSnap) {
-        heavy_flush(m, L, ks, center, flags, plv);
-        ks = 0;
-      }
     }
     wave_lds_sync();
+  }
+  if (ks == kSnap) {
+    heavy_flush(m, L, ks, center, flags, plv);
+    ks = 0;
   }
   const int n = st.n + P.walk;
   if (build && n >= m.upd_pt) {  // a build refits once with all the points
