@@ -522,6 +522,9 @@ static bool ws_layout(vmp_ctx* c, BatchWs& w, size_t n, int ns, size_t pre_cap, 
   ok = ok && dev(&b.lv, n * 2) && dev(&b.lv2, n * 2) && dev(&b.lv_cnt, (n * 2 >> 6) + size_t(ns) + 2) &&
        dev(&b.lv_n, size_t(ns));
   ok = ok && dev(&b.prim, prim_ws_entries(int64_t(n))) && dev(&b.qc, n * 3);
+  // the refitting-voxel list of the replay lives in cand_nbr's memory (24 B
+  // per point), which k_spec_eval writes only after the replays
+  if (ok) b.hlist = reinterpret_cast<int4*>(b.cand_nbr);
   // taint-phase prefetch of up to pre_cap acting candidates per batch (the
   // rest take the replay's own lookups)
   b.pre_cap = int32_t(std::min(n, pre_cap));

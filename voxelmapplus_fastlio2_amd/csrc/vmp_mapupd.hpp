@@ -131,6 +131,7 @@ struct BatchBufs {
   int32_t* seg_ct;     // per segment: its candidate times (up to 7, ascending)
   int32_t* seg_nct;
   int32_t* seg_nbr;    // per segment: the six neighbour voxels (current incarnation, -1 none)
+  int4* hlist;         // refitting voxels as (vid, start, length, session), see k_seg_lenkeys
   uint32_t* len_in;    // replay order: segments by descending length
   uint32_t* len_out;
   int32_t* ord_in;
@@ -1396,44 +1397,65 @@ __device__ inline WalkPlan walk_plan(int n0, int newly0, uint32_t flags, int len
 // Replay order: the voxels that refit first (the moment path), then the
 // rest, each part by descending segment length so waves walk segments of
 // similar length; slots past the segment count sort last.  Lengths are
-// clamped below 2^lbits - 1; the class is bit lbits.  Also counts the
-// refitting voxels (cnt[5]) and those of them with at least long_seg points
-// (cnt[6], the first ones of the order when long_seg < 2^lbits - 1; 0 counts
-// none); both cleared by k_seg_heads.
+// clamped below 2^lbits - 1; the class is bit lbits.  Also lists the
+// refitting voxels as (vid, start, length, session) records for the
+// per-wave replay: all of them from the front of hlist (cnt[5] counts them)
+// and, from its back, those with at least long_seg points (cnt[6]; 0 lists
+// none).  The two parts cannot meet: a long segment holds two points or
+// more.  Both counts are cleared by k_seg_heads.
 __global__ void __launch_bounds__(kMapThreads) k_seg_lenkeys(const MapDev* maps, const SessInfo* si, BatchBufs b,
                                                              int lbits, int long_seg) {
-  __shared__ int bsum, lsum;
+  constexpr int kW = kMapThreads / 64;
+  __shared__ int woff[2][kW];
+  __shared__ int bbase[2];
   const int nseg = b.cnt[0];
   const uint32_t top = uint32_t((uint64_t(1) << lbits) - 1);
-  if (threadIdx.x == 0) bsum = lsum = 0;
-  __syncthreads();
-  int heavy_n = 0, long_n = 0;
-  VMP_GRID_LOOP(q, b.ntot) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  VMP_UNIFORM_LOOP(q, valid, b.ntot) {
     uint32_t key = (1u << lbits) | top;
-    if (q < nseg) {
-      const int len = b.seg_end[q] - b.seg_start[q];
-      const int32_t vid = b.seg_vid[q];
-      int heavy = 0;
+    int heavy = 0, lng = 0, start = 0, len = 0, s = 0;
+    int32_t vid = -1;
+    if (valid && q < nseg) {
+      start = b.seg_start[q];
+      len = b.seg_end[q] - start;
+      vid = b.seg_vid[q];
       if (vid >= 0) {
-        const int s = b.seg_sess[q];
+        s = b.seg_sess[q];
         const MapDev m = maps[s];
         const VoxState& st = m.vs[vid];
         heavy = walk_refits(st.n, st.newly_add, st.flags, len, m.upd_pt, m.max_pt, si[s].is_build != 0) ? 1 : 0;
       }
-      heavy_n += heavy;
-      long_n += heavy & int(long_seg > 0 && len >= long_seg);
+      lng = heavy & int(long_seg > 0 && len >= long_seg);
       key = (uint32_t(1 - heavy) << lbits) | (top - min(uint32_t(len), top - 1));
     }
-    b.len_in[q] = key;
-    b.ord_in[q] = q;
+    if (valid) {
+      b.len_in[q] = key;
+      b.ord_in[q] = q;
+    }
+    const uint64_t bh = __ballot(heavy), bl = __ballot(lng);
+    if (lane == 0) {
+      woff[0][wv] = __popcll(bh);
+      woff[1][wv] = __popcll(bl);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {  // the block's share of each list, wave by wave
+      int run = 0;
+      for (int w = 0; w < kW; ++w) {
+        const int c = woff[threadIdx.x][w];
+        woff[threadIdx.x][w] = run;
+        run += c;
+      }
+      bbase[threadIdx.x] = run ? atomicAdd(&b.cnt[5 + threadIdx.x], run) : 0;
+    }
+    __syncthreads();
+    if (heavy) {
+      const uint64_t below = (uint64_t(1) << lane) - 1;
+      const int4 rec = make_int4(vid, start, len, s);
+      b.hlist[bbase[0] + woff[0][wv] + __popcll(bh & below)] = rec;
+      if (lng) b.hlist[b.ntot - 1 - (bbase[1] + woff[1][wv] + __popcll(bl & below))] = rec;
+    }
+    __syncthreads();
   }
-  heavy_n = wave_sum_i(heavy_n);
-  long_n = wave_sum_i(long_n);
-  if ((threadIdx.x & 63) == 0 && heavy_n) atomicAdd(&bsum, heavy_n);
-  if ((threadIdx.x & 63) == 0 && long_n) atomicAdd(&lsum, long_n);
-  __syncthreads();
-  if (threadIdx.x == 0 && bsum) atomicAdd(&b.cnt[5], bsum);
-  if (threadIdx.x == 0 && lsum) atomicAdd(&b.cnt[6], lsum);
 }
 
 // The voxels that refit in this scan (the first cnt[5] of the replay
@@ -1456,7 +1478,8 @@ constexpr int kLightChunk = 8;  // the same for the light walk
 constexpr int kSnap = 16;       // refits evaluated together by heavy_wave
 constexpr int kAccChunk = 8;    // points whose factors heavy_wave reads together
 struct HeavyLds {
-  double F[64][kMomF];        // factors of 64 consecutive points of the walk
+  double F[64][kMomF + 1];    // factors of 64 consecutive points of the walk (rows padded
+                              // to an odd number of doubles: a lane per row, no bank conflicts)
   double snap[kSnap][kMomC];  // moments at refit points
   int snapn[kSnap];           // point count at each of those refits
 };
@@ -1628,12 +1651,9 @@ __device__ inline void heavy_flush(const MapDev& m, HeavyLds& L, int ks, const d
 }
 
 // One voxel on one wave; the counts go to lane 0's refits/calls/dropped.
-__device__ inline void heavy_wave(const MapDev& m, const SessInfo& S, const BatchBufs& b, int q, HeavyLds& L,
-                                  int& refits, int& calls, int& dropped) {
+__device__ inline void heavy_wave(const MapDev& m, bool build, const BatchBufs& b, int32_t vid, int start, int len,
+                                  HeavyLds& L, int& refits, int& calls, int& dropped) {
   const int lane = threadIdx.x & 63;
-  const int32_t vid = b.seg_vid[q];
-  const int start = b.seg_start[q];
-  const int len = b.seg_end[q] - start;
   // segment point `lane` and pending point `lane`, loaded with the voxel's
   // state (before the pending count is known) for the first 64 rows
   float4 s0a = make_float4(0.f, 0.f, 0.f, 0.f), s0b = s0a, s0d = s0a;
@@ -1663,7 +1683,6 @@ __device__ inline void heavy_wave(const MapDev& m, const SessInfo& S, const Batc
   int64_t kx, ky, kz;
   unpack_key(st.key, kx, ky, kz);
   const double center[3] = {(double(kx) + 0.5) * m.vs64, (double(ky) + 0.5) * m.vs64, (double(kz) + 0.5) * m.vs64};
-  const bool build = S.is_build != 0;
   const WalkPlan P = walk_plan(st.n, st.newly_add, st.flags, len, m.upd_pt, m.max_pt, build);
   uint32_t flags = st.flags;
   // the pending points, then the walk's points of this scan, as rows of F
@@ -1778,21 +1797,32 @@ __device__ inline void heavy_wave(const MapDev& m, const SessInfo& S, const Batc
   }
 }
 
-// The first nw voxels of the replay order, one per wave: every refitting
-// voxel when there are at most wave_max of them, else the long ones (cnt[6]).
+// One wave per voxel of hlist: every refitting voxel when there are at most
+// wave_max of them, else the long ones (cnt[6], at hlist's back).  A
+// one-session step reads its map's record without waiting for the list.
 __global__ void __launch_bounds__(kMapThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_replay_heavy_wave(const MapDev* maps, const SessInfo* si, BatchBufs b, int wave_max) {
   __shared__ HeavyLds lds[kMapThreads / 64];
   constexpr int kWpb = kMapThreads / 64;
   const int nh = b.cnt[5];
-  const int nw = nh <= wave_max ? nh : b.cnt[6];
+  const bool all = nh <= wave_max;
+  const int nw = all ? nh : b.cnt[6];
   const int wv = threadIdx.x >> 6;
+  const MapDev m0 = maps[0];
+  const bool build0 = si[0].is_build != 0;
   for (int u = blockIdx.x * kWpb + wv; u < nw; u += gridDim.x * kWpb) {
     int refits = 0, calls = 0, dropped = 0;
-    const int q = b.ord[u];
-    const int s = b.seg_sess[q];
-    heavy_wave(maps[s], si[s], b, q, lds[wv], refits, calls, dropped);
-    const MapDev& m = maps[s];
+    const int4 r = b.hlist[all ? u : b.ntot - 1 - u];
+    MapDev m;
+    bool build;
+    if (b.nsess == 1) {
+      m = m0;
+      build = build0;
+    } else {
+      m = maps[r.w];
+      build = si[r.w].is_build != 0;
+    }
+    heavy_wave(m, build, b, r.x, r.y, r.z, lds[wv], refits, calls, dropped);
     wave_fetch_add(&m.ctr->refits, refits);
     wave_fetch_add(&m.ctr->merge_calls, calls);
     wave_fetch_add(&m.ctr->dropped, dropped);
