@@ -211,7 +211,7 @@ __device__ inline int g_group(int r) {
 }
 
 struct PredShared {
-  double P[2][529];         // row-major, the step's input and output in turn
+  double P[529];            // row-major
   double FP[kFpRows * 23];  // rows fp_row(0..10) of F P
   double E[kPredChunk][9];  // per step of the chunk: exp((gyro - bg) dt), row-major
   double J[kPredChunk][9];  // -Jr((gyro - bg) dt) dt (F and G)
@@ -226,22 +226,20 @@ struct PredShared {
   vmp_state sx;
 };
 
-// phase D for step j of the chunk, from P into Pn (the two buffers swap
-// roles step by step, so no barrier separates a step's reads of P from its
-// writes), on the block's four waves.
+// phase D for step j of the chunk, P in place, on the block's four waves.
 // F P: lane c of every wave holds column c of P (the entries its rows need)
 // and forms column c of some of the 11 non-identity rows of F P: wave 0 rows
 // 0-2, wave 1 rows 3-5, wave 2 rows 12-13, wave 3 rows 14, 21 and 22.
 // (F P) F^T + G Q G^T: lane r of every wave reads row r of F P (a row of P
-// for an identity row of F) into registers, then each wave writes its
-// columns of row r of Pn (wave 0 columns 0-2, wave 1 columns 3-5, wave 2
-// columns 12-13, wave 3 columns 14, 21, 22), wave 0 also the identity
+// for an identity row of F) into registers; after a barrier each wave
+// rewrites its columns of row r of P (wave 0 columns 0-2, wave 1 columns 3-5,
+// wave 2 columns 12-13, wave 3 columns 14, 21, 22), wave 0 also the identity
 // columns of a non-identity row (from F P) and the diagonal of G Q G^T's
-// identity blocks; the waves' other lanes copy the unchanged entries (the
-// identity rows' identity columns).  Every entry is the same expression,
-// summed in the same order, whichever wave forms it.
+// identity blocks.  Every entry is the same expression, summed in the same
+// order, whichever wave forms it.
 constexpr int kPredThreads = 256;
-__device__ inline void predict_cov_step(PredShared& S, int j, int w, int l, const double* P, double* Pn) {
+__device__ inline void predict_cov_step(PredShared& S, int j, int w, int l) {
+  double* P = S.P;
   const double dt = S.dt[j];
   const double* E = S.E[j];
   const double* Jd = S.J[j];
@@ -310,7 +308,7 @@ __device__ inline void predict_cov_step(PredShared& S, int j, int w, int l, cons
     }
   }
   __syncthreads();
-  // (F P) F^T + G Q G^T into Pn: every wave first reads the entries of row r it needs
+  // (F P) F^T + G Q G^T: every wave first reads the entries of row r it needs
   const int r = l;
   const int rs = r < 23 ? fp_slot(r) : -1;
   const double* fr = rs >= 0 ? &S.FP[rs * 23] : &P[(r < 23 ? r : 0) * 23];
@@ -338,15 +336,9 @@ __device__ inline void predict_cov_step(PredShared& S, int j, int w, int l, cons
       for (int t = 0; t < 5; ++t) f[6 + t] = fr[18 + t];
     }
   }
-  if (r >= 23) {  // the rest of the waves' lanes copy the identity rows' identity columns
-    for (int k = w * 41 + (r - 23); k < 144; k += 4 * 41) {
-      const int ri = k / 12, ci = k % 12;
-      const int rr = ri < 6 ? 6 + ri : 9 + ri, cc = ci < 6 ? 6 + ci : 9 + ci;
-      if (rr != cc || rr < 15) Pn[rr * 23 + cc] = P[rr * 23 + cc];
-    }
-  }
+  __syncthreads();  // every read of P's row r comes before the writes below
   if (r < 23) {
-    double* pr = &Pn[r * 23];
+    double* pr = &P[r * 23];
     const int gr = g_group(r);
     if (w == 0) {  // columns 0-2
 #pragma unroll
@@ -361,7 +353,7 @@ __device__ inline void predict_cov_step(PredShared& S, int j, int w, int l, cons
 #pragma unroll
         for (int c = 15; c < 21; ++c) pr[c] = fr[c];
       } else if (gr >= 2) {  // identity row and column: only G Q G^T's diagonal
-        pr[r] = P[r * 23 + r] + (dt * S.q[gr]) * dt;
+        pr[r] = pr[r] + (dt * S.q[gr]) * dt;
       }
     } else if (w == 1) {  // columns 3-5 (+ G Q G^T's block 3-5)
       double o[3];
@@ -433,7 +425,7 @@ k_imu_predict(const PreJob* jobs, int nsess) {
   FilterDev* f = Jb.f;
   const int lane = threadIdx.x;  // phases A-C and the set-up run on the first wave's lanes
   __shared__ PredShared S;
-  for (int i = lane; i < 529; i += kPredThreads) S.P[0][(i % 23) * 23 + i / 23] = f->P[i];  // column-major -> rows
+  for (int i = lane; i < 529; i += kPredThreads) S.P[(i % 23) * 23 + i / 23] = f->P[i];  // column-major -> rows
   load_state(f->x, S.sx, lane);
   if (lane < 4) S.q[lane] = im->q[lane];
   __syncthreads();
@@ -504,7 +496,6 @@ k_imu_predict(const PreJob* jobs, int nsess) {
     }
   };
   int np = 1;  // pose table entries
-  int pc = 0;  // the buffer holding P
   for (int c0 = 0; c0 < nsteps; c0 += kPredChunk) {
     const int cn = min(kPredChunk, nsteps - c0);
     if (lane < cn) {  // A
@@ -526,71 +517,47 @@ k_imu_predict(const PreJob* jobs, int nsess) {
       S.tt[lane] = tt;
     }
     __syncthreads();
-    {  // B: x += delta step after step on the first wave, every lane holding
-       // the whole rotation (row-major) and lanes 0-2 one velocity and
-       // position component; the blocks of a step come from the state before it
+    {  // B: x += delta step after step, the rotation in lanes 0-8 (entry
+       // lane), velocity and position in lanes 0-2; the blocks of a step come
+       // from the state before it
       vmp_state& x = S.sx;
-      double R[9];
-#pragma unroll
-      for (int i = 0; i < 9; ++i) R[i] = x.rot[(i % 3) * 3 + i / 3];  // column-major storage
-      double vel = 0.0, pos = 0.0;
+      double rv = 0.0, vel = 0.0, pos = 0.0;
+      if (lane < 9) rv = x.rot[(lane % 3) * 3 + lane / 3];  // column-major storage
       if (lane < 3) {
         vel = x.vel[lane];
         pos = x.pos[lane];
       }
-      if (lane < 64) {
-        const double g = lane < 3 ? x.g[lane] : 0.0;
+      if (lane < 64) {  // the first wave alone: the rotation's entries cross lanes by shuffles
+        const int r = lane < 9 ? lane / 3 : 0, c = lane % 3, vr = lane < 3 ? lane : 0;
         for (int j = 0; j < cn; ++j) {
-          if (lane == 0) {
-#pragma unroll
-            for (int i = 0; i < 9; ++i) S.R[j][i] = R[i];  // the rotation before step j
-          }
+          if (lane < 9) S.R[j][lane] = rv;  // the rotation before step j (lane k: entry k, row-major)
+          const double Rv0 = __shfl(rv, vr * 3, 64), Rv1 = __shfl(rv, vr * 3 + 1, 64), Rv2 = __shfl(rv, vr * 3 + 2, 64);
+          const double Rr0 = __shfl(rv, r * 3, 64), Rr1 = __shfl(rv, r * 3 + 1, 64), Rr2 = __shfl(rv, r * 3 + 2, 64);
           const double* E = S.E[j];
-          double Ej[9];
-#pragma unroll
-          for (int i = 0; i < 9; ++i) Ej[i] = E[i];
-          const double dt = S.dt[j];
-          const double ab0 = S.acc[j][0] - x.ba[0], ab1 = S.acc[j][1] - x.ba[1], ab2 = S.acc[j][2] - x.ba[2];
-          double ra3[3];
-#pragma unroll
-          for (int r = 0; r < 3; ++r) {
-            double ra = R[r * 3] * ab0;
-            ra += R[r * 3 + 1] * ab1;
-            ra += R[r * 3 + 2] * ab2;
-            ra3[r] = ra;
-          }
-          const double ra = lane == 0 ? ra3[0] : (lane == 1 ? ra3[1] : ra3[2]);
-          const double dvel = dt * (ra + g);
-          const double dpos = dt * vel;
-          pos = pos + dpos;
-          vel = vel + dvel;
           if (lane < 3) {
+            const double dt = S.dt[j];
+            const double ab0 = S.acc[j][0] - x.ba[0], ab1 = S.acc[j][1] - x.ba[1], ab2 = S.acc[j][2] - x.ba[2];
+            double ra = Rv0 * ab0;
+            ra += Rv1 * ab1;
+            ra += Rv2 * ab2;
+            const double dvel = dt * (ra + x.g[lane]);
+            const double dpos = dt * vel;
+            pos = pos + dpos;
+            vel = vel + dvel;
             S.vel[j + 1][lane] = vel;
             S.pos[j + 1][lane] = pos;
           }
-          double Rn[9];
-#pragma unroll
-          for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              double v = R[r * 3] * Ej[c];
-              v += R[r * 3 + 1] * Ej[3 + c];
-              v += R[r * 3 + 2] * Ej[6 + c];
-              Rn[r * 3 + c] = v;
-            }
-#pragma unroll
-          for (int i = 0; i < 9; ++i) R[i] = Rn[i];
+          if (lane < 9) {
+            double v = Rr0 * E[c];
+            v += Rr1 * E[3 + c];
+            v += Rr2 * E[6 + c];
+            rv = v;
+          }
         }
-        if (lane == 0) {
-#pragma unroll
-          for (int i = 0; i < 9; ++i) S.R[cn][i] = R[i];
-        }
+        if (lane < 9) S.R[cn][lane] = rv;
       }
       __syncthreads();
-      if (lane == 0) {
-#pragma unroll
-        for (int i = 0; i < 9; ++i) x.rot[(i % 3) * 3 + i / 3] = R[i];
-      }
+      if (lane < 9) x.rot[(lane % 3) * 3 + lane / 3] = rv;
       if (lane < 3) {
         x.vel[lane] = vel;
         x.pos[lane] = pos;
@@ -637,13 +604,12 @@ k_imu_predict(const PreJob* jobs, int nsess) {
     }
     for (int j = 0; j < cn; ++j) {  // D
       __syncthreads();
-      predict_cov_step(S, j, lane >> 6, lane & 63, S.P[pc], S.P[pc ^ 1]);
-      pc ^= 1;
+      predict_cov_step(S, j, lane >> 6, lane & 63);
     }
     np += (c0 + cn == nsteps) ? cn - 1 : cn;
     __syncthreads();
   }
-  for (int i = lane; i < 529; i += kPredThreads) f->P[i] = S.P[pc][(i % 23) * 23 + i / 23];
+  for (int i = lane; i < 529; i += kPredThreads) f->P[i] = S.P[(i % 23) * 23 + i / 23];
   if (lane == 0) {
     f->x = S.sx;
     if (m > 0) {
