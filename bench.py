@@ -36,13 +36,14 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per unit of work (the per-scan byte model of the path):
 # an observe pass moves 76 B per point (xyz 12, hash probe 16, plane record
 # 32, residual slot 16); an insertion 64 B per point (xyz 12, hash probe 16,
-# retained point + symmetric cov 36); a voxel replayed through a refit
-# 168 B (accumulator read + write 2 x 80, LRU stamp 8); a merge() evaluation
+# retained point + symmetric cov 36); a refit 1,312 B (the voxel's 69 FP64
+# moments read and written 2 x 552, its plane record read and written
+# 2 x 64, its state 80); a merge() evaluation
 # 288 B (6 x (16 B probe + 32 B plane)); an LRU creation or eviction
 # decision 32 B (ring entry 16, state and stamp 16).
 OBS_BYTES_POINT = 76
 INS_BYTES_POINT = 64
-VOXEL_BYTES = 168
+REFIT_BYTES = 2 * 552 + 2 * 64 + 80
 MERGE_BYTES = 288
 LRU_BYTES = 32
 # an IESKF iteration reads the observe pass's partial rows (248 B per block
@@ -554,7 +555,8 @@ def kernel_rooflines(tt, kt, stats, N, scans):
         "k_imu_predict": (stats["pred_bytes"], f"{PRED_BYTES} B + {PRED_SAMPLE_BYTES} B x IMU samples"),
         "k_candidates": (CAND_VOXEL_BYTES * stats["touched"], f"{CAND_VOXEL_BYTES} B x touched voxels"),
         "k_ins_keys": (INS_BYTES_POINT * stats["points"], f"{INS_BYTES_POINT} B x points"),
-        "k_replay_heavy": (VOXEL_BYTES * stats["refits"], f"{VOXEL_BYTES} B x refits"),
+        "k_replay_heavy": (REFIT_BYTES * stats["refits"], f"{REFIT_BYTES} B x refits (k_replay_heavy_wave and "
+                           "k_replay_heavy together)"),
         "k_merge_replay": (MERGE_BYTES * stats["merge_evals"], f"{MERGE_BYTES} B x merge() evaluations"),
         "k_create_evict": (LRU_BYTES * (stats["created"] + stats["evicted"]), f"{LRU_BYTES} B x (creations + evictions)"),
         "k_undistort": (28 * stats["raw_points"], "28 B x raw points (xyz + time in, xyz out)"),
