@@ -108,8 +108,8 @@ def cpu_baseline(pkgs, cfg, n_warm, n_timed):
         "kind": "port",
         "sample": f"street {n} pts/scan, packages {2 + n_warm}-{1 + n_warm + n_timed} of the sequence (after IMU_INIT, "
         f"MAP_INIT and {n_warm} registered scans; map {o.map_size()} voxels at the end; {n_timed} timed, "
-        f"{t_reg:.1f} s), FP64 oracle incl. undistortion, 1 thread; the timed GPU scans run on a full "
-        "2M-voxel map, so the per-point CPU cost here is a lower bound",
+        f"{t_reg:.1f} s), FP64 oracle incl. undistortion, 1 thread; the timed GPU scans run on the map "
+        "after the untimed fill scans (several times larger), so the per-point CPU cost here is a lower bound",
     }
 
 
