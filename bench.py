@@ -249,8 +249,9 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000, groups=1):
     from voxelmapplus_fastlio2_amd.synth import ROOM, SEED0, STREET, ImuSequence
 
     rng = np.random.default_rng(0xBA7C4)
-    total = W + K + 2  # IMU_INIT and MAP_INIT packages first
-    sizes = rng.integers(2000, nmax + 1, size=(total, B))
+    total = W + K + 3  # IMU_INIT and MAP_INIT packages first, one step for the per-launch timing last
+    sizes = rng.integers(2000, nmax + 1, size=(total - 1, B))
+    sizes = np.vstack([sizes, rng.integers(2000, nmax + 1, size=(1, B))])
     # the IMU_INIT package is a full sweep, so it holds the imu_init_num
     # samples (a scan of fewer points ends earlier in its sweep)
     sizes[0, :] = nmax
@@ -345,6 +346,37 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000, groups=1):
         for j, i in enumerate(idx):
             m = last_mst[gi][j]
             ev[i], mg[i], wn[i], rq[i], evi[i] = m.merge_evals, m.merges, m.merge_windows, m.merge_requeued, m.evicted
+    # one more step of group 0, launched kernel by kernel between the
+    # library's HIP events: the batched form's kernels against their byte
+    # models (the timed steps replay no graph here either, but run without
+    # events)
+    lead = regs[members[0][0]]
+    kx = W + K + 2
+    lead.set_timing(True)
+    obs0, map0 = step(kx, members[0])
+    torch.cuda.synchronize()
+    tt0 = lead.timing_totals()
+    kt0 = ctx.kernel_times()
+    lead.set_timing(False)
+    ppt_tab = tuned["group0"]["points_per_thread"]
+    bst = {"points": 0, "raw_points": 0, "refits": 0, "merge_evals": 0, "created": 0, "evicted": 0, "touched": 0,
+           "ieskf_bytes": 0, "pred_bytes": 0}
+    for j, i in enumerate(members[0]):
+        n_i = int(sizes[kx, i])
+        ppt_i = ppt_tab[0 if n_i < 8192 else 1 if n_i < 32768 else 2 if n_i < 131072 else 3]
+        m, o = map0[j], obs0[j]
+        bst["points"] += m.points
+        bst["raw_points"] += n_i
+        bst["touched"] += m.touched
+        bst["ieskf_bytes"] += o.iterations * (PART_ROW_BYTES * -(-m.points // (256 * ppt_i)) + IESKF_ITER_BYTES)
+        bst["pred_bytes"] += PRED_BYTES + PRED_SAMPLE_BYTES * len(steps[kx][2][i][0])
+        for f in ("refits", "merge_evals", "created", "evicted"):
+            bst[f] += getattr(m, f)
+    bkr = kernel_rooflines(tt0, kt0, bst, None, 1)
+    bdom = max(bkr, key=lambda k: bkr[k]["ms_per_scan"])
+    bd = bkr[bdom]
+    log(f"batched step, group 0 ({len(members[0])} sessions, {bst['points']} points), per launch: " +
+        ", ".join(f"{n} {r['avg_launch_ms'] * 1e3:.0f} us" for n, r in sorted(bkr.items(), key=lambda kv: -kv[1]["ms_per_scan"])))
     h = int(ev.argmax())
     log(f"batched last step: merge evals room max {ev[0::2].max()} mean {ev[0::2].mean():.0f}, "
         f"street max {ev[1::2].max()} mean {ev[1::2].mean():.0f}; merges room mean {mg[0::2].mean():.0f} "
@@ -369,6 +401,21 @@ def run_batched(ctx, B, W, K, capacity, nmax=120000, groups=1):
         "submission": "asynchronous (vmp_register_batch_submit): step k+1 is prepared while step k runs",
         "map_capacity_per_session": capacity,
         "evicted_per_street_session_last_step": float(evi[1::2].mean()),
+        "roofline": {
+            "kernel": bdom,
+            "bound": "hbm",
+            "achieved": bd["achieved_gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": bd["achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": bd["bytes_per_launch"],
+            "bytes_model": bd["bytes_model"],
+            "avg_launch_ms": bd["avg_launch_ms"],
+            "timed_over": f"one more step of group 0 ({len(members[0])} sessions, {bst['points']} points) after "
+            "the timed region, HIP events around each launch on the group's stream",
+        },
+        "kernels": bkr,
         "observe_autotune": tuned,
         "workload": f"{B} sequences room/street alternating, raw scans of U[2000,{nmax}] points with their 200 Hz "
         "IMU packages (undistortion on the device)",
