@@ -231,6 +231,31 @@ __device__ inline int wave_fetch_add(int* p, int v) {
   return out;
 }
 
+// LDS written by some lanes of a wave and read by others of the same wave
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Least value over the 64 lanes of a wave (every lane active): row shifts
+// within each 16-lane row, then the row broadcasts, leave it in lane 63.
+__device__ inline uint32_t wave_min_u32(uint32_t v) {
+  constexpr int kId = -1;  // 0xffffffff, the identity of min
+  v = min(v, uint32_t(__builtin_amdgcn_update_dpp(kId, int(v), 0x111, 0xf, 0xf, false)));  // row_shr:1
+  v = min(v, uint32_t(__builtin_amdgcn_update_dpp(kId, int(v), 0x112, 0xf, 0xf, false)));  // row_shr:2
+  v = min(v, uint32_t(__builtin_amdgcn_update_dpp(kId, int(v), 0x114, 0xf, 0xf, false)));  // row_shr:4
+  v = min(v, uint32_t(__builtin_amdgcn_update_dpp(kId, int(v), 0x118, 0xf, 0xf, false)));  // row_shr:8
+  v = min(v, uint32_t(__builtin_amdgcn_update_dpp(kId, int(v), 0x142, 0xa, 0xf, false)));  // row_bcast:15
+  v = min(v, uint32_t(__builtin_amdgcn_update_dpp(kId, int(v), 0x143, 0xc, 0xf, false)));  // row_bcast:31
+  return uint32_t(__builtin_amdgcn_readlane(int(v), 63));
+}
+__device__ inline uint64_t wave_min_u64(uint64_t v) {
+  const uint32_t hi = wave_min_u32(uint32_t(v >> 32));
+  const uint32_t lo = wave_min_u32(uint32_t(v >> 32) == hi ? uint32_t(v) : 0xffffffffu);
+  return (uint64_t(hi) << 32) | lo;
+}
+
 // *inserted = 1 when this call claimed the empty slot (the caller counts
 // used_entries); with inserted == nullptr the count is added here
 __device__ inline int32_t hash_find_or_insert(HashEntry* table, uint32_t mask, uint64_t key, MapCounters* ctr,
@@ -697,6 +722,13 @@ __global__ void __launch_bounds__(kLpThreads) k_ring_pack(const MapDev* maps, co
 // stores drain.  While the next creations precede every pending
 // re-creation, up to 64 of them take their victims in one step.
 constexpr int kStreamSlots = 8;  // chunks of each stream buffered in LDS
+constexpr int kProdBurst = 4;    // chunks of each stream the producer loads per round trip
+// Waits for every vector memory operation of the wave.  The decider's rare
+// paths (direct ring reads, creations below capacity) end with it: their
+// global loads are then complete on every path into the decision loop, so
+// the loop's common path (LDS only) carries no wait on the scattered stores
+// of earlier steps (stores and loads share one counter on this target).
+__device__ __forceinline__ void drain_vm() { __builtin_amdgcn_s_waitcnt(0); }
 
 __global__ void __launch_bounds__(128) k_create_evict(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
   const int s = blockIdx.x;
@@ -730,8 +762,9 @@ __global__ void __launch_bounds__(128) k_create_evict(const MapDev* maps, const 
 
   if (threadIdx.x >= 64) {
     // ---- stream producer: keeps each stream up to kStreamSlots chunks
-    //      ahead of the decider (skipping chunks it no longer needs) and
-    //      stops when the decider is done or every chunk is out
+    //      ahead of the decider (skipping chunks it no longer needs), up to
+    //      kProdBurst chunks of each stream per round trip, and stops when
+    //      the decider is done or every chunk is out
     const int4* lvs = b.lv2 + int64_t(2) * off;
     const __attribute__((address_space(1))) int32_t* fl = as_global(m.freelist);
     int p0 = 0, p1 = 0, p2 = 0;
@@ -742,34 +775,44 @@ __global__ void __launch_bounds__(128) k_create_evict(const MapDev* maps, const 
       p0 = max(p0, u0);
       p1 = max(p1, u1);
       p2 = max(p2, u2);
-      const bool g0 = p0 < nwin && p0 < u0 + kStreamSlots;
-      const bool g1 = p1 < ncch && p1 < u1 + kStreamSlots;
-      const bool g2 = p2 < nfch && p2 < u2 + kStreamSlots;
-      int4 e = make_int4(-1, kNever, -1, 0);
-      int32_t ct = kNever, cs = -1, fv = -1;
-      const int i0 = 64 * p0 + lane, i1 = 64 * p1 + lane, i2 = 64 * p2 + lane;
-      if (g0 && i0 < nlive) e = lvs[i0];
-      if (g1 && i1 < ncre_ev) {
-        ct = b.cre_t[clo + i1];
-        cs = b.cre_seg[clo + i1];
+      const int n0 = max(0, min(kProdBurst, min(nwin, u0 + kStreamSlots) - p0));
+      const int n1 = max(0, min(kProdBurst, min(ncch, u1 + kStreamSlots) - p1));
+      const int n2 = max(0, min(kProdBurst, min(nfch, u2 + kStreamSlots) - p2));
+      int4 e[kProdBurst];
+      int32_t ct[kProdBurst], cs[kProdBurst], fv[kProdBurst];
+#pragma unroll
+      for (int k = 0; k < kProdBurst; ++k) {  // every load of the round first
+        const int i0 = 64 * (p0 + k) + lane, i1 = 64 * (p1 + k) + lane, i2 = 64 * (p2 + k) + lane;
+        e[k] = make_int4(-1, kNever, -1, 0);
+        ct[k] = kNever;
+        cs[k] = -1;
+        fv[k] = -1;
+        if (k < n0 && i0 < nlive) e[k] = lvs[i0];
+        if (k < n1 && i1 < ncre_ev) {
+          ct[k] = b.cre_t[clo + i1];
+          cs[k] = b.cre_seg[clo + i1];
+        }
+        if (k < n2 && i2 < ftop0) fv[k] = fl[ftop0 - 1 - i2];
       }
-      if (g2 && i2 < ftop0) fv = fl[ftop0 - 1 - i2];
-      if (g0) s_live[p0 % kStreamSlots][lane] = e;
-      if (g1) {
-        s_cre[p1 % kStreamSlots][lane] = ct;
-        s_cre[p1 % kStreamSlots][64 + lane] = cs;
+#pragma unroll
+      for (int k = 0; k < kProdBurst; ++k) {
+        if (k < n0) s_live[(p0 + k) % kStreamSlots][lane] = e[k];
+        if (k < n1) {
+          s_cre[(p1 + k) % kStreamSlots][lane] = ct[k];
+          s_cre[(p1 + k) % kStreamSlots][64 + lane] = cs[k];
+        }
+        if (k < n2) s_free[(p2 + k) % kStreamSlots][lane] = fv[k];
       }
-      if (g2) s_free[p2 % kStreamSlots][lane] = fv;
-      p0 += g0;
-      p1 += g1;
-      p2 += g2;
+      p0 += n0;
+      p1 += n1;
+      p2 += n2;
       if (lane == 0) {
         __hip_atomic_store(&s_seq[0], p0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         __hip_atomic_store(&s_seq[1], p1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         __hip_atomic_store(&s_seq[2], p2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
       if (p0 >= nwin && p1 >= ncch && p2 >= nfch) break;
-      if (!(g0 || g1 || g2)) __builtin_amdgcn_s_sleep(2);
+      if (n0 + n1 + n2 == 0) __builtin_amdgcn_s_sleep(2);
     }
     return;
   }
@@ -844,6 +887,7 @@ __global__ void __launch_bounds__(128) k_create_evict(const MapDev* maps, const 
       l_first = (l_cur && st->create_t < 0) ? st->first_t : kNever;
       l_seg = st->seg;
     }
+    drain_vm();
   };
   auto load_cre = [&]() {  // the chunk of creation ci
     cq = ci >> 6;
@@ -882,24 +926,37 @@ __global__ void __launch_bounds__(128) k_create_evict(const MapDev* maps, const 
     }
   };
 
-  int th = kNever;  // time of the earliest pending re-creation (heap top)
+  // pending re-creations, (first touch << 32) | evicted vid, in slots
+  // [0, hn) of heap (segment in heap_seg); slot i belongs to lane i & 63,
+  // which keeps the least key of its slots (lmin, at lpos), so the earliest
+  // one is a wave minimum away
+  int th = kNever;  // time of the earliest pending re-creation
+  uint64_t lmin = ~0ull;
+  int lpos = -1;
+  auto lane_rescan = [&](bool me) {
+    if (me) {
+      lmin = ~0ull;
+      lpos = -1;
+      for (int i = lane; i < hn; i += 64) {
+        const uint64_t k = heap[i];
+        if (k < lmin) {
+          lmin = k;
+          lpos = i;
+        }
+      }
+    }
+  };
   // a re-creation at first touch f of the evicted voxel v (segment sg)
   auto heap_push = [&](int32_t f, int32_t v, int32_t sg) {
-    __builtin_amdgcn_wave_barrier();
-    if (lane == 0) {
-      int i = hn;
-      const uint64_t key = (uint64_t(uint32_t(f)) << 32) | uint32_t(v);
-      while (i > 0) {
-        const int p = (i - 1) >> 1;
-        if (heap[p] <= key) break;
-        heap[i] = heap[p];
-        heap_seg[i] = heap_seg[p];
-        i = p;
+    const uint64_t key = (uint64_t(uint32_t(f)) << 32) | uint32_t(v);
+    if (lane == (hn & 63)) {
+      heap[hn] = key;
+      heap_seg[hn] = sg;
+      if (key < lmin) {
+        lmin = key;
+        lpos = hn;
       }
-      heap[i] = key;
-      heap_seg[i] = sg;
     }
-    __builtin_amdgcn_wave_barrier();
     ++hn;
     th = min(th, f);
   };
@@ -924,6 +981,7 @@ __global__ void __launch_bounds__(128) k_create_evict(const MapDev* maps, const 
             b.co[off + nco + j] = make_int4(vid, b.cre_seg[clo + ci + j], b.cre_t[clo + ci + j], -1);
           }
         }
+        drain_vm();
         if (__ballot(err != 0)) {
           err |= 8;
           break;
@@ -989,9 +1047,9 @@ __global__ void __launch_bounds__(128) k_create_evict(const MapDev* maps, const 
             err |= 8;
             break;
           }
-          uint64_t rem = vmask;  // the ring position after the mm-th victim
-          for (int k = 1; k < mm; ++k) rem &= rem - 1;
-          ptr = head0 + __builtin_amdgcn_readlane(l_rel, __ffsll((unsigned long long)rem) - 1) + 1;
+          // the ring position after the mm-th victim (the lane of vmask's mm-th bit)
+          const uint64_t lastb = __ballot(mine && r == mm - 1);
+          ptr = head0 + __builtin_amdgcn_readlane(l_rel, __ffsll((unsigned long long)lastb) - 1) + 1;
           if (hv >= 0) {
             if (hn >= kHeapCap) {
               err |= 16;
@@ -1022,42 +1080,23 @@ __global__ void __launch_bounds__(128) k_create_evict(const MapDev* maps, const 
       seg = __builtin_amdgcn_readlane(c_seg, d);
       prev = -1;
       ++ci;
-    } else {
-      const uint64_t top = heap[0];
+    } else {  // the earliest pending re-creation leaves its slot to the last one
+      const uint64_t top = wave_min_u64(lmin);
+      const int w = __ffsll((unsigned long long)__ballot(lmin == top)) - 1;  // keys are unique (vid)
+      const int gp = __builtin_amdgcn_readlane(lpos, w);
       t = int(top >> 32);
       prev = int32_t(top & 0xffffffffu);  // evicted incarnation of this key
-      seg = heap_seg[0];
-      __builtin_amdgcn_wave_barrier();
-      if (lane == 0) {
-        const uint64_t lastv = heap[hn - 1];
-        const int32_t lasts = heap_seg[hn - 1];
-        const int nn = hn - 1;
-        int i = 0;
-        while (true) {
-          const int l = 2 * i + 1, r = l + 1;
-          int sm = i;
-          uint64_t sv = lastv;
-          if (l < nn && heap[l] < sv) {
-            sm = l;
-            sv = heap[l];
-          }
-          if (r < nn && heap[r] < sv) {
-            sm = r;
-            sv = heap[r];
-          }
-          if (sm == i) break;
-          heap[i] = heap[sm];
-          heap_seg[i] = heap_seg[sm];
-          i = sm;
-        }
-        if (nn > 0) {
-          heap[i] = lastv;
-          heap_seg[i] = lasts;
-        }
+      seg = heap_seg[gp];
+      const int last = hn - 1, lo = last & 63;
+      wave_lds_sync();
+      if (lane == lo && last != gp) {
+        heap[gp] = heap[last];
+        heap_seg[gp] = heap_seg[last];
       }
-      __builtin_amdgcn_wave_barrier();
+      wave_lds_sync();
       --hn;
-      th = hn > 0 ? int(heap[0] >> 32) : kNever;
+      lane_rescan(lane == w || lane == lo);
+      th = hn > 0 ? int(wave_min_u64(lmin) >> 32) : kNever;
     }
     int32_t vid;
     if (ftop > 0) {
@@ -1484,12 +1523,6 @@ struct HeavyLds {
   int snapn[kSnap];           // point count at each of those refits
 };
 
-// LDS written by some lanes of a wave and read by others of the same wave
-__device__ inline void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 __device__ inline double readlane_d(double v, int k) {
   const uint64_t u = uint64_t(__double_as_longlong(v));
   const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u)), k));
