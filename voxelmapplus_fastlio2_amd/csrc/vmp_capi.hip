@@ -1741,9 +1741,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         });
         timed_launch(c, kTkReplayLight, [&] { hipLaunchKernelGGL(k_replay_light, dim3(gn), dim3(T), 0, st, M, S, b); });
         if (!(mode & kStepBuild)) {
-          hipLaunchKernelGGL(k_fill_cand, dim3(gn), dim3(T), 0, st, b, ntot);
           timed_launch(c, kTkCandidates, [&] { hipLaunchKernelGGL(k_candidates, dim3(gn), dim3(T), 0, st, M, S, b); });
-          hipLaunchKernelGGL(k_dirty_untouched, dim3(gn), dim3(T), 0, st, M, b);
           // the call at point time t is made by the voxel that received point
           // t, so each candidate sits at its session offset plus t: the
           // non-empty slots in order are the list sorted on (session, time)

@@ -1470,6 +1470,13 @@ __global__ void __launch_bounds__(kMapThreads) k_seg_lenkeys(const MapDev* maps,
     if (valid) {
       b.len_in[q] = key;
       b.ord_in[q] = q;
+      // the merge stage's per-point slots start empty (k_candidates fills them)
+      b.cand_in[q] = ~0ull;
+      b.cand_spec[q] = 0;
+      int2* tn = reinterpret_cast<int2*>(b.tnbr + size_t(q) * 6);
+      tn[0] = make_int2(0, 0);
+      tn[1] = make_int2(0, 0);
+      tn[2] = make_int2(0, 0);
     }
     const uint64_t bh = __ballot(heavy), bl = __ballot(lng);
     if (lane == 0) {
@@ -2108,18 +2115,6 @@ struct TailCandStarts {
   __device__ void operator()(int) const { cand_starts_block(maps, nsess, b); }
 };
 
-// merge candidates' key list cleared, their speculative results zeroed and
-// the per-call neighbour cache emptied
-__global__ void k_fill_cand(BatchBufs b, int n) {
-  VMP_GRID_LOOP(i, n) {
-    b.cand_in[i] = ~0ull;
-    b.cand_spec[i] = 0;
-    int2* t = reinterpret_cast<int2*>(b.tnbr + size_t(i) * 6);
-    t[0] = make_int2(0, 0);
-    t[1] = make_int2(0, 0);
-    t[2] = make_int2(0, 0);
-  }
-}
 
 // ---------------------------------------------------- housekeeping -----
 // hash rebuild: every live voxel is re-inserted into a cleared table

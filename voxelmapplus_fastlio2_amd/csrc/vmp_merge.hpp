@@ -241,6 +241,12 @@ __global__ void k_candidates(const MapDev* maps, const SessInfo* si, BatchBufs b
         if (has_calls) emit(nfrz[j]);
       }
       st.dirty_t = dirty;
+      // a frozen plane marks its untouched live neighbours dirty for later
+      // scans (only their kDirty bit changes: this kernel reads kPlane and
+      // kAlive of neighbours, and the own flags of touched voxels only)
+      if (st.freeze_t >= 0 && (st.flags & kPlane))
+        for (int j = 0; j < 6; ++j)
+          if (nvs[j] >= 0 && ntag[j] != tag && (nflg[j] & kAlive)) atomicOr(&m.vs[nvs[j]].flags, kDirty);
     }
     // the session's candidate count, one atomic per wave and session; each
     // candidate goes to its session's offset plus its point time (a call
@@ -264,24 +270,6 @@ __global__ void k_candidates(const MapDev* maps, const SessInfo* si, BatchBufs b
       }
       for (int k = 0; k < ne; ++k) b.seg_ct[q * 7 + k] = emitted[k];
       b.seg_nct[q] = ne;
-    }
-  }
-}
-
-// voxels not touched in this scan whose neighbour froze as a plane
-__global__ void k_dirty_untouched(const MapDev* maps, BatchBufs b) {
-  const int nseg = b.cnt[0];
-  VMP_GRID_LOOP(q, nseg) {
-    if (b.seg_vid[q] < 0) continue;
-    const MapDev m = maps[b.seg_sess[q]];
-    const int tag = m.ctr->tag;
-    const VoxState& st = m.vs[b.seg_vid[q]];
-    if (st.freeze_t < 0 || !(st.flags & kPlane)) continue;
-    for (int j = 0; j < 6; ++j) {
-      const int32_t nv = b.seg_nbr[q * 6 + j];
-      if (nv < 0) continue;
-      VoxState& nbs = m.vs[nv];
-      if (nbs.tag != tag && (nbs.flags & kAlive)) atomicOr(&nbs.flags, kDirty);
     }
   }
 }
