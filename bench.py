@@ -647,13 +647,13 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-warmup", type=int, default=5)
     ap.add_argument("--batch-capacity", type=int, default=200000)
-    ap.add_argument("--batch-groups", type=int, default=3,
+    ap.add_argument("--batch-groups", type=int, default=4,
                     help="session groups, each on its own context (HIP stream) and host thread")
     ap.add_argument("--obs-ppt", type=int, default=0, help="observe points per thread (0 = by scan size)")
     ap.add_argument("--serve-sessions", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
-    ap.add_argument("--serve-executors", type=int, default=3)
+    ap.add_argument("--serve-executors", type=int, default=4)
     ap.add_argument("--serve-max-batch", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
