@@ -1720,6 +1720,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
           hipLaunchKernelGGL(k_cre_split, dim3(gn), dim3(T), 0, st, S, n, b);
         }
         hipLaunchKernelGGL(k_create_fast, dim3(gn), dim3(T), 0, st, M, S, n, b);
+        hipLaunchKernelGGL(k_create_fast_ctr, dim3(n), dim3(1), 0, st, M, n);
         timed_launch(c, kTkRingLive,
                      [&] { hipLaunchKernelGGL(k_ring_live, dim3(kLvBlocks, n), dim3(64 * kLvWaves), 0, st, M, S, n, b); });
         hipLaunchKernelGGL(k_ring_pack, dim3(n), dim3(kLpThreads), 0, st, M, S, n, b);
@@ -1771,6 +1772,7 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         if (!prim) hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.flags, 0, ntot);
         hipLaunchKernelGGL(k_finalize_segs, dim3(gn), dim3(T), 0, st, M, S, b);
         hipLaunchKernelGGL(k_evict_finalize, dim3(gn), dim3(T), 0, st, M, S, n, b);
+        hipLaunchKernelGGL(k_evict_finalize_ctr, dim3(n), dim3(1), 0, st, M, n);
         if (prim) {
           select_stable(st, ntot, PredFlag{b.flags}, OutIndex{b.sel}, b.prim, b.cnt + 2,
                         TailRanges{M, S, n, b.sel, 2, w.d_sfirst, ntot});
@@ -1778,10 +1780,9 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
           if (!select_flags(b.flags, b.sel, b.cnt + 2)) return VMP_EDEVICE;
           hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 64)), dim3(64), 0, st, M, S, n, b.sel, b.cnt + 2, 2);
         }
-        hipLaunchKernelGGL(k_lru_append, dim3(gn), dim3(T), 0, st, M, S, n, b);  // ends the scan too
-      } else {
-        hipLaunchKernelGGL(k_end_scan, dim3(n), dim3(1), 0, st, M, S, n);
+        hipLaunchKernelGGL(k_lru_append, dim3(gn), dim3(T), 0, st, M, S, n, b);
       }
+      hipLaunchKernelGGL(k_end_scan, dim3(n), dim3(1), 0, st, M, S, n);
   
       return VMP_OK;
     };

@@ -256,28 +256,6 @@ __device__ inline uint64_t wave_min_u64(uint64_t v) {
   return (uint64_t(hi) << 32) | lo;
 }
 
-// True in the block of the grid that reaches it last, after every block's
-// writes (thread 0 releases its block's before counting itself, the last
-// one acquires after); that block also rearms the counter for the next
-// launch.  The counters are the spare words of the primitives' control block
-// (zero when the workspace is made).
-__device__ inline bool grid_last_block(int32_t* done) {
-  __shared__ int s_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const int last = atomicAdd(done, 1) == int(gridDim.x) - 1;
-    if (last) {
-      __threadfence();
-      *done = 0;
-    }
-    s_last = last;
-  }
-  __syncthreads();
-  return s_last != 0;
-}
-enum { kDoneCreateFast = 0, kDoneEvict = 1, kDoneLru = 2 };
-
 // *inserted = 1 when this call claimed the empty slot (the caller counts
 // used_entries); with inserted == nullptr the count is added here
 __device__ inline int32_t hash_find_or_insert(HashEntry* table, uint32_t mask, uint64_t key, MapCounters* ctr,
@@ -605,8 +583,23 @@ struct OutCre {
 
 // ------------------------------------------------------------------ K5 ----
 // Parallel creation for sessions whose LRU cannot overflow in this scan.
-// a session's counters after its fast creations (no eviction)
-__device__ inline void create_fast_ctr(const MapDev& m) {
+__global__ void k_create_fast(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
+  const int nc = b.cnt[1];
+  VMP_GRID_LOOP(r, nc) {
+    const int s = sess_of(si, nsess, b.sel[r]);
+    const MapDev m = maps[s];
+    const MapCounters* c = m.ctr;
+    const int ncs = si[s].cre_hi - si[s].cre_lo;
+    if (c->size + ncs > m.capacity) continue;
+    const int rl = r - si[s].cre_lo;
+    const int ftop = c->free_top, bump = c->bump;
+    const int32_t vid = rl < ftop ? m.freelist[ftop - 1 - rl] : bump + (rl - ftop);
+    b.co[si[s].off + rl] = make_int4(vid, b.cre_seg[r], b.cre_t[r], -1);
+  }
+}
+__global__ void k_create_fast_ctr(const MapDev* maps, int nsess) {
+  if (threadIdx.x != 0 || int(blockIdx.x) >= nsess) return;
+  const MapDev m = maps[blockIdx.x];
   MapCounters* c = m.ctr;
   const int nc = c->ncreate_ev;
   if (c->size + nc > m.capacity) return;
@@ -623,23 +616,6 @@ __device__ inline void create_fast_ctr(const MapDev& m) {
   c->ncre = nc;
   c->nevict = 0;
   c->fast = 1;
-}
-__global__ void k_create_fast(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
-  const int nc = b.cnt[1];
-  VMP_GRID_LOOP(r, nc) {
-    const int s = sess_of(si, nsess, b.sel[r]);
-    const MapDev m = maps[s];
-    const MapCounters* c = m.ctr;
-    const int ncs = si[s].cre_hi - si[s].cre_lo;
-    if (c->size + ncs > m.capacity) continue;
-    const int rl = r - si[s].cre_lo;
-    const int ftop = c->free_top, bump = c->bump;
-    const int32_t vid = rl < ftop ? m.freelist[ftop - 1 - rl] : bump + (rl - ftop);
-    b.co[si[s].off + rl] = make_int4(vid, b.cre_seg[r], b.cre_t[r], -1);
-  }
-  // the counters change once every block has read them
-  if (grid_last_block(&prim_ws(b.prim).ctl->pad[kDoneCreateFast]))
-    for (int s = threadIdx.x; s < nsess; s += blockDim.x) create_fast_ctr(maps[s]);
 }
 
 // Chunks of the ring that k_ring_live compacts for a session: enough for the
@@ -2068,21 +2044,15 @@ __global__ void k_evict_finalize(const MapDev* maps, const SessInfo* si, int nse
     m.hot[v].flags = 0;
     m.freelist[c->free_top + r] = v;
   }
-  // the evicted slots join the free list once every block has placed them
-  if (grid_last_block(&prim_ws(b.prim).ctl->pad[kDoneEvict]))
-    for (int s = threadIdx.x; s < nsess; s += blockDim.x) {
-      MapCounters* c = maps[s].ctr;
-      c->free_top += c->nevict;
-      c->tombs += c->nevict;
-    }
 }
 
-// a session's ring tail and point base after the scan
-__device__ inline void end_scan(const MapDev& m, const SessInfo& S) {
-  MapCounters* c = m.ctr;
-  c->ring_tail += c->nlru;
-  c->point_base += S.n;
+__global__ void k_evict_finalize_ctr(const MapDev* maps, int nsess) {
+  if (threadIdx.x != 0 || int(blockIdx.x) >= nsess) return;
+  MapCounters* c = maps[blockIdx.x].ctr;
+  c->free_top += c->nevict;
+  c->tombs += c->nevict;
 }
+
 __global__ void k_lru_append(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
   const int nsel = b.cnt[2];
   VMP_GRID_LOOP(r, nsel) {
@@ -2095,14 +2065,13 @@ __global__ void k_lru_append(const MapDev* maps, const SessInfo* si, int nsess, 
     m.ring_vid[ph] = vid;
     m.ring_stamp[ph] = uint64_t(m.ctr->point_base + t);
   }
-  // the scan ends once every block has read the ring tail and point base
-  if (grid_last_block(&prim_ws(b.prim).ctl->pad[kDoneLru]))
-    for (int s = threadIdx.x; s < nsess; s += blockDim.x) end_scan(maps[s], si[s]);
 }
-// (a step without points)
+
 __global__ void k_end_scan(const MapDev* maps, const SessInfo* si, int nsess) {
   if (threadIdx.x != 0 || int(blockIdx.x) >= nsess) return;
-  end_scan(maps[blockIdx.x], si[blockIdx.x]);
+  MapCounters* c = maps[blockIdx.x].ctr;
+  c->ring_tail += c->nlru;
+  c->point_base += si[blockIdx.x].n;
 }
 
 // start of each session's candidates in the sorted candidate list, and
