@@ -1893,27 +1893,40 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_heavy(const MapDev* maps
 // list that the voxel's next refit folds into the moments.
 __global__ void __launch_bounds__(kMapThreads) k_replay_light(const MapDev* maps, const SessInfo* si, BatchBufs b) {
   const int nseg = b.cnt[0], nh = b.cnt[5];
+  // a one-session step reads its map's record once, before the voxels
+  const MapDev m0 = maps[0];
+  const int build0 = si[0].is_build;
   VMP_UNIFORM_LOOP(qi, valid, nseg - nh) {
     int calls = 0, dropped = 0;
     const int q = valid ? b.ord[nh + qi] : 0;
-    const int s = valid ? b.seg_sess[q] : 0;
-    const MapDev m = maps[s];
     const int32_t vid = valid ? b.seg_vid[q] : -1;
-    const bool frozen0 = vid >= 0 && !si[s].is_build && (m.vs[vid].flags & kInit) && !(m.vs[vid].flags & kEnable);
+    const int start = b.seg_start[q], end = b.seg_end[q];
+    MapDev m;
+    int s, is_build;
+    if (b.nsess == 1) {
+      s = 0;
+      m = m0;
+      is_build = build0;
+    } else {
+      s = valid ? b.seg_sess[q] : 0;
+      m = maps[s];
+      is_build = si[s].is_build;
+    }
+    VoxState st = m.vs[vid < 0 ? 0 : vid];
+    const bool frozen0 = vid >= 0 && !is_build && (st.flags & kInit) && !(st.flags & kEnable);
     if (frozen0) {
       // frozen before this scan: every point is a merge() call (plane) or dropped
-      const int len = b.seg_end[q] - b.seg_start[q];
-      VoxState& st = m.vs[vid];
+      const int len = end - start;
       if (st.flags & kPlane) {
-        st.merge_pos = 0;
+        m.vs[vid].merge_pos = 0;
         calls += len;
       } else {
         dropped += len;
       }
     } else if (vid >= 0) {
-      const SessInfo& S = si[s];
-      const int start = b.seg_start[q], end = b.seg_end[q];
-      VoxState st = m.vs[vid];
+      double* pl = m.plane + size_t(vid) * kPlaneD;
+      double nrm[3];  // the plane's normal, kept (zero for a new voxel)
+      for (int a = 0; a < 3; ++a) nrm[a] = st.n > 0 ? pl[3 + a] : 0.0;
       int64_t kx, ky, kz;
       unpack_key(st.key, kx, ky, kz);
       const double center[3] = {(double(kx) + 0.5) * m.vs64, (double(ky) + 0.5) * m.vs64,
@@ -1944,7 +1957,7 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_light(const MapDev* maps
        for (int u = 0; u < kLightChunk; ++u) {
         const int pos = p0 + u;
         if (pos >= end || !walking) break;
-        if ((flags & kInit) && !(flags & kEnable) && !S.is_build) {
+        if ((flags & kInit) && !(flags & kEnable) && !is_build) {
           if (flags & kPlane) {
             merge_pos = pos - start;
             calls += end - pos;
@@ -1966,7 +1979,7 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_light(const MapDev* maps
         ++cnt;
         for (int a = 0; a < 3; ++a) s1[a] += double(qv[a]);
         ++n;
-        if (S.is_build) continue;
+        if (is_build) continue;
         if (!(flags & kInit)) {
           if (n >= m.upd_pt) atomicOr(&m.ctr->error, 32);  // a refit: the other kernel's voxel
           continue;
@@ -1981,9 +1994,11 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_light(const MapDev* maps
       }
       for (int a = 0; a < 3; ++a) mo[kMomRun + a] = s1[a];
       mo[kMomPend] = double(cnt);
-      double* pl = m.plane + size_t(vid) * kPlaneD;
       const double invn = 1.0 / double(n);
-      for (int a = 0; a < 3; ++a) pl[a] = center[a] + s1[a] * invn;
+      double plv[6];
+      for (int a = 0; a < 3; ++a) plv[a] = center[a] + s1[a] * invn;
+      for (int a = 0; a < 3; ++a) plv[3 + a] = nrm[a];
+      for (int a = 0; a < 3; ++a) pl[a] = plv[a];
       if (st.n == 0)  // a new voxel: its plane record was not cleared at creation
         for (int k = 3; k < kPlaneD; ++k) pl[k] = 0.0;
       st.flags = flags;
@@ -1992,7 +2007,7 @@ __global__ void __launch_bounds__(kMapThreads) k_replay_light(const MapDev* maps
       st.freeze_t = freeze_t;
       st.merge_pos = merge_pos;
       m.vs[vid] = st;
-      write_hot(m, vid, flags);
+      write_hot_pl(m, vid, flags, plv, kx, ky, kz);
     }
     wave_fetch_add(&m.ctr->merge_calls, calls);
     wave_fetch_add(&m.ctr->dropped, dropped);

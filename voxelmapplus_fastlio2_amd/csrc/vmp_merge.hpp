@@ -193,10 +193,13 @@ __device__ inline void merge_fuse(double* pv, double* pn) {
 // ------------------------------------------------------------------ K7 ----
 __global__ void k_candidates(const MapDev* maps, const SessInfo* si, BatchBufs b) {
   const int nseg = b.cnt[0];
+  const MapDev m0 = maps[0];  // a one-session step reads its map's record once
   VMP_UNIFORM_LOOP(q, valid, nseg) {
     const int32_t vid = valid ? b.seg_vid[q] : -1;
-    const int s = valid ? b.seg_sess[q] : 0;
-    const MapDev& m = maps[s];
+    const int s = b.nsess == 1 || !valid ? 0 : b.seg_sess[q];
+    MapDev m;
+    if (b.nsess == 1) m = m0;
+    else m = maps[s];
     MapCounters* c = m.ctr;
     int emitted[7];
     int ne = 0;
