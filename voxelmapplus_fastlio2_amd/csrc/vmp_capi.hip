@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -224,6 +225,8 @@ struct vmp_ctx {
   // observe launches back to back between one event pair (the kernel is
   // idempotent: the repetitions rewrite the same slots and partial rows)
   int obs_time_reps = 1;
+  int32_t* h_sig = nullptr;  // pinned coherent word a signal kernel sets when the stream reaches it
+  int32_t sig_gen = 0;
   int replay_wave_max = 4096;  // refitting voxels per step up to which each replays on its own wave
   int replay_long_seg = 24;    // beyond that, segments of this many points still do (0: none)
   hipEvent_t tev[2 * 64]{};
@@ -382,6 +385,32 @@ static int grid_for(int64_t n, int threads) {
 
 extern "C" const char* vmp_version(void) { return "voxelmapplus_fastlio2_amd 0.2 (gfx950)"; }
 
+// Sets the context's pinned word to v once the stream reaches it (the
+// stream's earlier work, device-to-host copies included, is then complete).
+__global__ void k_signal(int32_t* h, int32_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    __threadfence_system();
+    __hip_atomic_store(h, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+// Waits for the context's stream: a signal kernel and a bounded spin on the
+// pinned word (a round trip about 3 us shorter than hipStreamSynchronize),
+// then hipStreamSynchronize, which also reports any fault of the work.  The
+// spin ends after 2 ms, so longer work waits in the synchronize.
+static hipError_t stream_wait(vmp_ctx* c) {
+  if (c->h_sig) {
+    const int32_t want = ++c->sig_gen;
+    hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, c->stream, c->h_sig, want);
+    if (hipGetLastError() == hipSuccess) {
+      const auto t0 = std::chrono::steady_clock::now();
+      while (__atomic_load_n(c->h_sig, __ATOMIC_ACQUIRE) != want &&
+             std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(2)) {
+      }
+    }
+  }
+  return hipStreamSynchronize(c->stream);
+}
+
 static void noalloc_setup(vmp_ctx* c);
 
 extern "C" int vmp_ctx_create(int device, vmp_ctx** out) {
@@ -397,6 +426,11 @@ extern "C" int vmp_ctx_create(int device, vmp_ctx** out) {
   }
 
   for (hipEvent_t& e : c->ev) hipEventCreate(&e);
+  void* hs = nullptr;
+  if (hipHostMalloc(&hs, 256, hipHostMallocCoherent) == hipSuccess) {
+    c->h_sig = static_cast<int32_t*>(hs);
+    *c->h_sig = 0;
+  }
   noalloc_setup(c);
   *out = c;
   return VMP_OK;
@@ -428,6 +462,7 @@ extern "C" int vmp_ctx_destroy(vmp_ctx* ctx) {
   for (hipEvent_t e : ctx->tev)
     if (e) hipEventDestroy(e);
   if (ctx->h_pkstage) hipHostFree(ctx->h_pkstage);
+  if (ctx->h_sig) hipHostFree(ctx->h_sig);
   if (ctx->d_pkstage) hipFree(ctx->d_pkstage);
   if (ctx->pkstage_done) hipEventDestroy(ctx->pkstage_done);
   if (ctx->h_refs) hipHostFree(ctx->h_refs);
@@ -486,7 +521,7 @@ static int check_device_points(vmp_ctx* c, const PtsRef* refs, int nref) {
   hipLaunchKernelGGL(k_check_points, dim3(gx, std::min(nref, 65535)), dim3(256), 0, st, inl, c->d_refs, nref,
                      c->d_bad, gen);
   if (hipMemcpyAsync(c->h_bad, c->d_bad, sizeof(int32_t) * nref, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipStreamSynchronize(st) != hipSuccess)
+      stream_wait(c) != hipSuccess)
     return -2;
   for (int i = 0; i < nref; ++i)
     if (c->h_bad[i] == gen) return i;
@@ -2119,7 +2154,7 @@ static int step_finish(vmp_ctx* c, vmp_obs_stats* ost, vmp_map_stats* mst) {
   const int mode = c->fl_mode;
   vmp_session* s = ss[0];
   int rc = VMP_OK;
-  VMP_HIP(hipStreamSynchronize(c->stream));
+  VMP_HIP(stream_wait(c));
   int max_iters = 0;
   int64_t pts_iters = 0, hits = 0;
   for (int i = 0; i < n; ++i) {
