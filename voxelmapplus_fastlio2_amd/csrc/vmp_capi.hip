@@ -394,18 +394,19 @@ __global__ void k_signal(int32_t* h, int32_t v) {
   }
 }
 // Waits for the context's stream: a signal kernel and a bounded spin on the
-// pinned word (a round trip about 3 us shorter than hipStreamSynchronize),
-// then hipStreamSynchronize, which also reports any fault of the work.  The
-// spin ends after 2 ms, so longer work waits in the synchronize.
+// pinned word (a round trip about 3 us shorter than hipStreamSynchronize).
+// The word changes only once every earlier operation of the stream has
+// completed (a fault stops the stream before it), so a seen signal ends the
+// wait; otherwise, after 2 ms, hipStreamSynchronize waits for the rest and
+// reports any fault.
 static hipError_t stream_wait(vmp_ctx* c) {
   if (c->h_sig) {
     const int32_t want = ++c->sig_gen;
     hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, c->stream, c->h_sig, want);
     if (hipGetLastError() == hipSuccess) {
       const auto t0 = std::chrono::steady_clock::now();
-      while (__atomic_load_n(c->h_sig, __ATOMIC_ACQUIRE) != want &&
-             std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(2)) {
-      }
+      while (std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(2))
+        if (__atomic_load_n(c->h_sig, __ATOMIC_ACQUIRE) == want) return hipSuccess;
     }
   }
   return hipStreamSynchronize(c->stream);
