@@ -1808,7 +1808,6 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
         if (!prim) hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(T), 0, st, b.flags, 0, ntot);
         hipLaunchKernelGGL(k_finalize_segs, dim3(gn), dim3(T), 0, st, M, S, b);
         hipLaunchKernelGGL(k_evict_finalize, dim3(gn), dim3(T), 0, st, M, S, n, b);
-        hipLaunchKernelGGL(k_evict_finalize_ctr, dim3(n), dim3(1), 0, st, M, n);
         if (prim) {
           select_stable(st, ntot, PredFlag{b.flags}, OutIndex{b.sel}, b.prim, b.cnt + 2,
                         TailRanges{M, S, n, b.sel, 2, w.d_sfirst, ntot});

@@ -2061,13 +2061,6 @@ __global__ void k_evict_finalize(const MapDev* maps, const SessInfo* si, int nse
   }
 }
 
-__global__ void k_evict_finalize_ctr(const MapDev* maps, int nsess) {
-  if (threadIdx.x != 0 || int(blockIdx.x) >= nsess) return;
-  MapCounters* c = maps[blockIdx.x].ctr;
-  c->free_top += c->nevict;
-  c->tombs += c->nevict;
-}
-
 __global__ void k_lru_append(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
   const int nsel = b.cnt[2];
   VMP_GRID_LOOP(r, nsel) {
@@ -2082,11 +2075,17 @@ __global__ void k_lru_append(const MapDev* maps, const SessInfo* si, int nsess, 
   }
 }
 
+// the scan's counter updates: the LRU appends, the points, and the evicted
+// voxels that k_evict_finalize pushed above the free-list top (nothing
+// after k_evict_finalize reads free_top or tombs; nevict stays 0 in a scan
+// without points)
 __global__ void k_end_scan(const MapDev* maps, const SessInfo* si, int nsess) {
   if (threadIdx.x != 0 || int(blockIdx.x) >= nsess) return;
   MapCounters* c = maps[blockIdx.x].ctr;
   c->ring_tail += c->nlru;
   c->point_base += si[blockIdx.x].n;
+  c->free_top += c->nevict;
+  c->tombs += c->nevict;
 }
 
 // start of each session's candidates in the sorted candidate list, and
