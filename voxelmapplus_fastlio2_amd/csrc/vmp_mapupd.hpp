@@ -90,7 +90,7 @@ struct SessInfo {
   int32_t cre_lo, cre_hi;  // its creation events in the batch creation list
   int32_t lru_lo, lru_hi;  // its LRU appends in the batch append list
   int32_t spl_lo, spl_hi;  // its acting merge candidates in the compacted list
-  int32_t pad;
+  int32_t ftop0, bump0;    // free-list top and bump pointer before its fast creations
 };
 
 // batch workspace, all arrays sized for the batch's total points
@@ -378,6 +378,32 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int n) {
   VMP_GRID_LOOP(i, n) p[i] = v;
 }
 
+// A session whose creation events all fit under its capacity (no LRU
+// eviction in this scan) takes its new voxels from the free-list top down,
+// then from the bump pointer.  Its counters advance as soon as its creation
+// count is known; the free-list top and bump pointer they started from stay
+// in S for k_create_fast.  Other sessions keep fast = 0 and go through
+// k_create_evict.
+__device__ inline void create_fast_counters(const MapDev& m, SessInfo& S, int nc) {
+  MapCounters* c = m.ctr;
+  if (c->size + nc > m.capacity) return;
+  const int ftop = c->free_top, bump = c->bump;
+  S.ftop0 = ftop;
+  S.bump0 = bump;
+  if (nc <= ftop) {
+    c->free_top = ftop - nc;
+  } else {
+    c->bump = bump + (nc - ftop);
+    c->free_top = 0;
+  }
+  if (c->bump > m.pool) atomicOr(&c->error, 8);
+  c->size += nc;
+  c->group_counter += uint64_t(nc);
+  c->ncre = nc;
+  c->nevict = 0;
+  c->fast = 1;
+}
+
 // per-session [lo, hi) of an ascending list of n global indices
 // which: 0 segments, 1 creation events, 2 LRU appends
 __device__ inline void session_range(const MapDev* maps, SessInfo* si, int s, const int32_t* list, int n,
@@ -401,6 +427,7 @@ __device__ inline void session_range(const MapDev* maps, SessInfo* si, int s, co
     si[s].cre_lo = lo;
     si[s].cre_hi = hi;
     c->ncreate_ev = hi - lo;
+    create_fast_counters(maps[s], si[s], hi - lo);
   } else {
     si[s].lru_lo = lo;
     si[s].lru_hi = hi;
@@ -455,6 +482,7 @@ struct TailRanges {
       } else if (which == 1) {
         si[s].cre_hi = hi;
         c->ncreate_ev = hi - lo;
+        create_fast_counters(maps[s], si[s], hi - lo);
       } else {
         si[s].lru_hi = hi;
         c->nlru = hi - lo;
@@ -582,40 +610,19 @@ struct OutCre {
 };
 
 // ------------------------------------------------------------------ K5 ----
-// Parallel creation for sessions whose LRU cannot overflow in this scan.
+// Parallel creation for sessions whose LRU cannot overflow in this scan
+// (fast, set with their counters where the creation events were counted).
 __global__ void k_create_fast(const MapDev* maps, const SessInfo* si, int nsess, BatchBufs b) {
   const int nc = b.cnt[1];
   VMP_GRID_LOOP(r, nc) {
     const int s = sess_of(si, nsess, b.sel[r]);
     const MapDev m = maps[s];
-    const MapCounters* c = m.ctr;
-    const int ncs = si[s].cre_hi - si[s].cre_lo;
-    if (c->size + ncs > m.capacity) continue;
+    if (!m.ctr->fast) continue;
     const int rl = r - si[s].cre_lo;
-    const int ftop = c->free_top, bump = c->bump;
+    const int ftop = si[s].ftop0, bump = si[s].bump0;
     const int32_t vid = rl < ftop ? m.freelist[ftop - 1 - rl] : bump + (rl - ftop);
     b.co[si[s].off + rl] = make_int4(vid, b.cre_seg[r], b.cre_t[r], -1);
   }
-}
-__global__ void k_create_fast_ctr(const MapDev* maps, int nsess) {
-  if (threadIdx.x != 0 || int(blockIdx.x) >= nsess) return;
-  const MapDev m = maps[blockIdx.x];
-  MapCounters* c = m.ctr;
-  const int nc = c->ncreate_ev;
-  if (c->size + nc > m.capacity) return;
-  const int ftop = c->free_top;
-  if (nc <= ftop) {
-    c->free_top = ftop - nc;
-  } else {
-    c->bump += nc - ftop;
-    c->free_top = 0;
-  }
-  if (c->bump > m.pool) atomicOr(&c->error, 8);
-  c->size += nc;
-  c->group_counter += uint64_t(nc);
-  c->ncre = nc;
-  c->nevict = 0;
-  c->fast = 1;
 }
 
 // Chunks of the ring that k_ring_live compacts for a session: enough for the
