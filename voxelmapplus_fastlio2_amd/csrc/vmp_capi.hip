@@ -1706,6 +1706,15 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
     // long segments sort first only below the length clamp (2^lbits - 2)
     const int long_seg = c->replay_long_seg < (1 << lbits) - 1 ? c->replay_long_seg : 0;
     const int wave_max = c->replay_wave_max;
+    // every session's counters are the ones its last step returned (a step
+    // on this context starts only after the previous one finished), and a
+    // scan creates at most one voxel per point: when every session's size
+    // plus its points fits its capacity, no session can evict in this scan,
+    // all take the fast creation path and the eviction kernels (which would
+    // return at once) are left out
+    bool all_fast = true;
+    for (int i = 0; i < n; ++i)
+      all_fast &= int64_t(ss[i]->h_ctr.size) + int64_t(ss[i]->n) <= int64_t(ss[i]->map.capacity);
     // batches up to kPrimMaxItems points use this library's compaction and
     // radix sort (kernels only, so the stage can be captured); larger ones
     // rocPRIM's device-wide routines
@@ -1756,10 +1765,14 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
           hipLaunchKernelGGL(k_cre_split, dim3(gn), dim3(T), 0, st, S, n, b);
         }
         hipLaunchKernelGGL(k_create_fast, dim3(gn), dim3(T), 0, st, M, S, n, b);
-        timed_launch(c, kTkRingLive,
-                     [&] { hipLaunchKernelGGL(k_ring_live, dim3(kLvBlocks, n), dim3(64 * kLvWaves), 0, st, M, S, n, b); });
-        hipLaunchKernelGGL(k_ring_pack, dim3(n), dim3(kLpThreads), 0, st, M, S, n, b);
-        timed_launch(c, kTkCreateEvict, [&] { hipLaunchKernelGGL(k_create_evict, dim3(n), dim3(128), 0, st, M, S, n, b); });
+        if (!all_fast) {
+          timed_launch(c, kTkRingLive, [&] {
+            hipLaunchKernelGGL(k_ring_live, dim3(kLvBlocks, n), dim3(64 * kLvWaves), 0, st, M, S, n, b);
+          });
+          hipLaunchKernelGGL(k_ring_pack, dim3(n), dim3(kLpThreads), 0, st, M, S, n, b);
+          timed_launch(c, kTkCreateEvict,
+                       [&] { hipLaunchKernelGGL(k_create_evict, dim3(n), dim3(128), 0, st, M, S, n, b); });
+        }
         hipLaunchKernelGGL(k_create_init, dim3(gn), dim3(T), 0, st, M, S, n, b);
         // replay order by segment length in 6 bits (one radix pass; segments
         // of 62 points and more share the first key): the order only spreads
@@ -1822,7 +1835,8 @@ static int enqueue_step(vmp_ctx* c, vmp_session** ss, int n, int mode) {
     };
     const int64_t key[3] = {int64_t(ntot), int64_t(n),
                             int64_t(hbits) | (int64_t(sbits) << 8) | (int64_t(tbits) << 16) |
-                                (int64_t(lbits) << 24) | (int64_t(mode & kStepBuild) << 32) |
+                                (int64_t(lbits) << 24) | (int64_t(all_fast) << 32) |
+                                (int64_t(mode & kStepBuild) << 32) |
                                 (int64_t(long_seg) << 36) | (int64_t(wave_max) << 42)};
     hipGraphExec_t mexec = (c->use_graph && c->use_map_graph && !c->timing) ? map_graph(c, key, map_stage) : nullptr;
     if (mexec) {
